@@ -1,0 +1,122 @@
+/* vrh.h -- C ABI of libvrh, the MI355X (gfx950) implementation of the vring
+ * RK3 fractional-step substep and Lagrangian particle step.
+ *
+ * The reference has no plugin or FFI layer: its hot path is the sequence of
+ * Fortran calls in loop.F90:63-131, each taking `type(run) :: var` with the
+ * whole state.  These entry points replace that sequence for a Fortran driver
+ * (iso_c_binding) or any C caller.  Conventions:
+ *  - every function returns 0 on success, a negative code on error; the
+ *    message is available from vrh_last_error;
+ *  - host arrays use the reference's Fortran column-major layouts:
+ *      haloed fields  q(ntheta, 0:nr+2, 0:nz+2)      solu_toolsII.F90:88-111
+ *      prg(ntheta, nr, nz, 3)                         solu_toolsII.F90:113
+ *      Poisson rhs/phi(ntheta, nr, nz)                solp_tools.F90:53-55
+ *  - the library never keeps host pointers; all device memory is owned by
+ *    the handle; one HIP stream per handle; calls that return host data are
+ *    synchronous, the others are asynchronous on the handle's stream.
+ */
+#ifndef VRH_H
+#define VRH_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vrh_s vrh_t;
+
+/* Mesh: mesh_tools.F90:86-93, 217-350 (one whole-domain layout). */
+typedef struct {
+  int ntheta, nr, nz;
+  double rmax, zmax;
+} vrh_grid;
+
+/* Parameters: para_tools.F90:68-86 (dt, Re, slip) and the theta eigenvalue
+ * used by the Poisson solve (poisson.F90:386; poisson_solve.F90:286-292). */
+typedef struct {
+  double dt, Re, slip;
+  int theta_eigen; /* VRH_THETA_EXACT_M2 (reference 'wvnex') or VRH_THETA_CONSISTENT */
+} vrh_params;
+
+enum { VRH_THETA_EXACT_M2 = 0, VRH_THETA_CONSISTENT = 1 };
+enum { VRH_FP64 = 0, VRH_MIXED = 1 };
+
+/* Substep phases for vrh_phase (loop.F90:71-127 call order). */
+enum {
+  VRH_PH_HDNBC = 2,    /* momentum_terms_swirl.F90:177 hdnbc                   */
+  VRH_PH_EXPLICIT = 3, /* hdnl1-3 (momentum_terms_swirl.F90:15-174) + step's
+                          explicit right-hand sides (step.F90:32-165)          */
+  VRH_PH_ADI = 6,      /* step's solqi/solqj/solqk sweeps (step.F90:72-187)    */
+  VRH_PH_POISSON = 8,  /* divq + laplace_solve (divergence.F90:16;
+                          poisson_solve.F90:13)                                */
+  VRH_PH_PROJECT = 9   /* upsol + uppres (divergence.F90:69, 239)              */
+};
+
+/* Lifecycle.  Replaces create_mesh_a/alct_solu/alct_solp/prep_poisson_solver
+ * (main.F90:49-68): allocates and zero-initialises every field. */
+int vrh_create(const vrh_grid *grid, const vrh_params *params, int precision, vrh_t **out);
+int vrh_destroy(vrh_t *h);
+int vrh_last_error(vrh_t *h, char *buf, size_t len);
+
+/* Whole-field transfers (restart_load/restart_save, run.F90:390-539, and
+ * the analysis output).  Any pointer may be NULL to skip that field. */
+int vrh_upload_fields(vrh_t *h, const double *q1, const double *q2, const double *q3,
+                      const double *pres);
+int vrh_download_fields(vrh_t *h, double *q1, double *q2, double *q3, double *pres);
+/* Any named state array: q1 q2 q3 dq1 dq2 dq3 ru1 ru2 ru3 pres (haloed),
+ * prg (ntheta,nr,nz,3), dph (haloed), csort dq3no corr3n (ntheta, nr+2). */
+int vrh_upload_state(vrh_t *h, const char *name, const double *host);
+int vrh_download_state(vrh_t *h, const char *name, double *host);
+
+/* Simulation time (par%ntime).  vrh_step advances it as loop.F90:53 does. */
+int vrh_set_time(vrh_t *h, double ntime);
+int vrh_get_time(vrh_t *h, double *ntime);
+
+/* One RK substep irk=1..3 at the current time (loop.F90:71-127). */
+int vrh_substep(vrh_t *h, int irk);
+/* nsteps full steps: ntime += dt, then substeps 1,2,3 (loop.F90:53-131). */
+int vrh_step(vrh_t *h, int nsteps);
+/* One phase of substep irk (parity tests and profiling). */
+int vrh_phase(vrh_t *h, int phase, int irk);
+/* Standalone Poisson solve, host rhs -> host phi, (ntheta,nr,nz) each
+ * (solve_lap_tripar, poisson_solve.F90:140-179). */
+int vrh_poisson_solve(vrh_t *h, const double *rhs, double *phi);
+/* Wait for the handle's stream; returns the deferred "Flow field crash"
+ * error (divergence.F90:396-399) if a NaN was met. */
+int vrh_synchronize(vrh_t *h);
+/* Device address of a named field (for callers that keep data in HBM). */
+int vrh_device_ptr(vrh_t *h, const char *name, void **ptr);
+
+/* Host-side initial condition (no reference counterpart: the reference loads
+ * RESTART, run.F90:469-539).  Gaussian-core vortex rings from a discrete
+ * Stokes streamfunction; rings[4*r..] = (R, z0, sigma, circulation); the ring
+ * radius is R + pert_amp * sum_{m=pert_m0..pert_m1} cos(m theta + phase[m-pert_m0]).
+ * Writes haloed q1,q2,q3 (divergence-free to rounding on the staggered grid). */
+int vrh_ring_init(const vrh_grid *grid, int nrings, const double *rings, double pert_amp,
+                  int pert_m0, int pert_m1, const double *phase, double *q1, double *q2,
+                  double *q3);
+/* Fixed time step from a CFL number on a haloed host field (with an inlet
+ * jet speed floor `ujet`). */
+int vrh_cfl_dt(const vrh_grid *grid, const double *q1, const double *q2, const double *q3,
+               double cfl, double ujet, double *dt);
+
+/* ---- Lagrangian particles (part_tools.F90) ---- */
+enum { VRH_PART_LSQ_EULER = 0, VRH_PART_LSQ_RK3 = 1, VRH_PART_TRILINEAR_RK3 = 2 };
+/* Create nbatch particle batches (init_part, part_tools.F90:228-359): batch b
+ * holds up to cap[b] particles with Stokes number st[b] and nsub[b] sub-steps
+ * per call (part_tools.F90:493-502).  State rows hold 117 values
+ * (part_tools.F90:241-252) for the LSQ schemes, 6 for TRILINEAR_RK3. */
+int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st,
+                         const int *nsub, int scheme);
+/* Host rows (n x nvar, row-major by particle) and alive flags. */
+int vrh_particles_upload(vrh_t *h, int batch, long n, const double *u, const int *alive);
+int vrh_particles_download(vrh_t *h, int batch, long n, double *u, int *alive);
+/* Advance all batches at RK substep irk (iter_part, part_tools.F90:374-1025;
+ * the reference calls it at irk==1 only, loop.F90:67). */
+int vrh_particles_step(vrh_t *h, int irk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,384 @@
+// vrh_api.cpp -- the C ABI of libvrh (declared in include/vrh.h): handle
+// lifecycle, device memory, transfers and the substep driver.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "vrh_internal.h"
+
+namespace {
+
+int fail(vrh_t *h, const char *msg, int code = -1) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                             \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      char b_[256];                                                                 \
+      snprintf(b_, sizeof b_, "%s: %s", #call, hipGetErrorString(e_));              \
+      return fail(h, b_, -3);                                                       \
+    }                                                                               \
+  } while (0)
+
+size_t haloed(const vrh_t *h) { return (size_t)h->nt * (h->nr + 3) * (h->nz + 3); }
+size_t compact(const vrh_t *h) { return (size_t)h->nt * h->nr * h->nz; }
+size_t wsize(const vrh_t *h) { return h->prec == VRH_MIXED ? sizeof(float) : sizeof(double); }
+
+template <typename T> int dalloc(vrh_t *h, T **p, size_t n) {
+  HIPCHK(h, hipMalloc((void **)p, n * sizeof(T) + 64));
+  HIPCHK(h, hipMemsetAsync(*p, 0, n * sizeof(T) + 64, h->stream));
+  return 0;
+}
+int dalloc_bytes(vrh_t *h, void **p, size_t bytes) {
+  HIPCHK(h, hipMalloc(p, bytes + 64));
+  HIPCHK(h, hipMemsetAsync(*p, 0, bytes + 64, h->stream));
+  return 0;
+}
+
+// host double -> device buffer of the work type
+int upload_conv(vrh_t *h, void *dst, const double *src, size_t n, bool work) {
+  if (!work || h->prec == VRH_FP64) {
+    HIPCHK(h, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+  }
+  std::vector<float> tmp(n);
+  for (size_t i = 0; i < n; i++) tmp[i] = (float)src[i];
+  HIPCHK(h, hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+int download_conv(vrh_t *h, double *dst, const void *src, size_t n, bool work) {
+  if (!work || h->prec == VRH_FP64) {
+    HIPCHK(h, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+  }
+  std::vector<float> tmp(n);
+  HIPCHK(h, hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t i = 0; i < n; i++) dst[i] = tmp[i];
+  return 0;
+}
+
+struct Named {
+  void *p;
+  size_t n;
+  bool work;
+};
+bool lookup(vrh_t *h, const char *name, Named &out) {
+  const size_t H = haloed(h), C = compact(h), PL = (size_t)h->nt * (h->nr + 3);
+  struct { const char *n; void *p; size_t c; bool w; } t[] = {
+      {"q1", h->f.q1, H, false}, {"q2", h->f.q2, H, false}, {"q3", h->f.q3, H, false},
+      {"dq1", h->f.dq1, H, false}, {"dq2", h->f.dq2, H, false}, {"dq3", h->f.dq3, H, false},
+      {"pres", h->f.pres, H, false}, {"dph", h->f.dph, H, false},
+      {"ru1", h->f.ru1, H, true}, {"ru2", h->f.ru2, H, true}, {"ru3", h->f.ru3, H, true},
+      {"qrhs1", h->f.r1, H, true}, {"qrhs2", h->f.r2, H, true}, {"qrhs3", h->f.r3, H, true},
+      {"prg1", h->f.prg1, C, true}, {"prg2", h->f.prg2, C, true}, {"prg3", h->f.prg3, C, true},
+      {"spec", h->f.S, C, true},
+      {"csort", h->f.csort, PL, false}, {"dq3no", h->f.dq3no, PL, false}, {"corr3n", h->f.corr3n, PL, false},
+  };
+  for (auto &e : t)
+    if (strcmp(e.n, name) == 0) { out = {e.p, e.c, e.w}; return true; }
+  return false;
+}
+
+template <typename TW> int substep_t(vrh_t *h, int irk, int phase) {
+  const double t0 = h->ntime, t1 = h->ntime + h->dt;   // momentum_terms_swirl.F90:192-193
+  const double unom0 = host_profile(t0), unom1 = host_profile(t1);
+  const double alpha = h->rk[irk].g + h->rk[irk].r;
+  const double c3s = (unom1 - unom0) * alpha;           // :212
+  double qinf = 0.0;                                    // :261-267
+  for (int j = 1; j <= h->nr; j++) {
+    const double cj = (j <= h->jlim) ? c3s : 0.0;
+    for (int i = 1; i <= h->nt; i++) qinf = qinf + cj * h->da[j];
+  }
+  const double cor = (qinf - 0.0) / (h->cor_red + (double)1.e-15f);  // :283
+  if (phase < 0 || phase == VRH_PH_HDNBC) launch_hdnbc<TW>(h, irk, cor, c3s);
+  if (phase < 0 || phase == VRH_PH_EXPLICIT) launch_explicit<TW>(h, irk);
+  if (phase < 0 || phase == VRH_PH_ADI) {
+    launch_theta<TW>(h, irk);
+    launch_sweeps<TW>(h, irk, c3s);
+  }
+  if (phase < 0 || phase == VRH_PH_POISSON) launch_poisson<TW>(h, irk, nullptr, nullptr);
+  if (phase < 0 || phase == VRH_PH_PROJECT) launch_project<TW>(h, irk);
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+int substep(vrh_t *h, int irk, int phase) {
+  if (irk < 1 || irk > 3) return fail(h, "irk must be 1..3");
+  return h->prec == VRH_MIXED ? substep_t<float>(h, irk, phase) : substep_t<double>(h, irk, phase);
+}
+
+int check_flag(vrh_t *h) {
+  int flag = 0;
+  HIPCHK(h, hipMemcpyAsync(&flag, h->f.flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (flag) return fail(h, "Flow field crash (NaN in the pressure gradient)", -2);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vrh_last_error(vrh_t *h, char *buf, size_t len) {
+  if (!buf || len == 0) return -1;
+  const char *m = h ? h->err.c_str() : "null handle";
+  snprintf(buf, len, "%s", m);
+  return 0;
+}
+
+int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t **out) {
+  if (!grid || !p || !out) return -1;
+  *out = nullptr;
+  if (grid->ntheta < 3 || grid->nr < 4 || grid->nz < 4 || grid->nr > 16 * 32 || grid->ntheta > 1024 ||
+      grid->rmax <= 0 || grid->zmax <= 0 || p->dt <= 0 || p->Re <= 0)
+    return -1;
+  if (precision != VRH_FP64 && precision != VRH_MIXED) return -1;
+  vrh_t *h = new (std::nothrow) vrh_t();
+  if (!h) return -1;
+  *out = h;
+  h->nt = grid->ntheta; h->nr = grid->nr; h->nz = grid->nz; h->nrc = h->nr + 1; h->nzc = h->nz + 1;
+  h->rmax = grid->rmax; h->zmax = grid->zmax; h->Re = p->Re; h->dt = p->dt; h->slip = p->slip;
+  h->theta_eigen = p->theta_eigen; h->prec = precision; h->ntime = 0.0;
+  VGrid &g = h->g;
+  g.nt = h->nt; g.nr = h->nr; g.nz = h->nz; g.nrc = h->nrc; g.nzc = h->nzc;
+  g.sj = h->nt; g.sk = (long)h->nt * (h->nr + 3); g.ncomp = (long)compact(h);
+  const double dth = 2.0 * acos(-1.0) / h->nt;                   // mesh_tools.F90:217-219, 315-320
+  g.dx1 = 1.0 / dth; g.dx2 = 1.0 / (h->rmax / h->nr); g.dx3 = 1.0 / (h->zmax / h->nz);
+  g.dx1q = g.dx1 * g.dx1; g.dx2q = g.dx2 * g.dx2; g.dx3q = g.dx3 * g.dx3;
+  g.axsym = (h->nt == 2) ? 0.0 : 1.0;
+  g.dt = h->dt; g.Re = h->Re; g.slip = h->slip;
+  const double gam[4] = {0, 8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};   // para_tools.F90:103-113
+  const double rom[4] = {0, 0.0, -17.0 / 60.0, -5.0 / 12.0};
+  for (int i = 1; i <= 3; i++) h->rk[i] = {gam[i], rom[i], gam[i] + rom[i]};
+  HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  host_mesh_vectors(h);
+  // radial vectors
+  const int nv = h->nr + 3;
+  int rc = dalloc(h, &h->d_vec, (size_t)21 * nv);
+  if (rc) return rc;
+  std::vector<double> vb((size_t)21 * nv);
+  for (int q = 0; q < 21; q++) std::memcpy(&vb[(size_t)q * nv], h->hv[q].data(), nv * sizeof(double));
+  HIPCHK(h, hipMemcpy(h->d_vec, vb.data(), vb.size() * sizeof(double), hipMemcpyHostToDevice));
+  const double **cv[21] = {&h->coef.rm, &h->coef.rc, &h->coef.h1d11, &h->coef.h1d22d, &h->coef.h1d22n,
+                           &h->coef.h1dare, &h->coef.h2d11, &h->coef.h2d22m, &h->coef.h2d22p, &h->coef.h2dare,
+                           &h->coef.s1t11, &h->coef.s1t2n, &h->coef.s1t2s, &h->coef.s1t2c, &h->coef.s2t11,
+                           &h->coef.s2t2p, &h->coef.s2t2m, &h->coef.s2t2c, &h->coef.s3t2n, &h->coef.s3t2s,
+                           &h->coef.qc};
+  for (int q = 0; q < 21; q++) *cv[q] = h->d_vec + (size_t)q * nv;
+  // line coefficients for the three substeps
+  h->Mr = h->nr <= 128 ? 32 : 64;
+  h->Mz = h->nzc <= 160 ? 32 : 64;
+  while ((h->nr + 1 + h->Mr - 1) / h->Mr > 16) h->Mr *= 2;
+  while ((h->nzc + h->Mz - 1) / h->Mz > 16) h->Mz *= 2;
+  if (h->Mr > 64 || h->Mz > 64) return fail(h, "grid too large for the line solver (nr+1, nz+1 <= 1024)");
+  std::vector<double> lcv[4][6][7];
+  int M[6] = {h->Mr, h->Mr, h->Mr, h->Mz, h->Mz, h->Mz};
+  size_t tot = 0;
+  for (int irk = 1; irk <= 3; irk++) {
+    host_line_coefs(h, irk, lcv[irk], M);
+    for (int s = 0; s < 6; s++)
+      for (int q = 0; q < 7; q++) tot += lcv[irk][s][q].size();
+  }
+  rc = dalloc(h, &h->d_line, tot);
+  if (rc) return rc;
+  std::vector<double> lb(tot);
+  size_t off = 0;
+  for (int irk = 1; irk <= 3; irk++)
+    for (int s = 0; s < 6; s++) {
+      const double *ptr[7];
+      for (int q = 0; q < 7; q++) {
+        std::memcpy(&lb[off], lcv[irk][s][q].data(), lcv[irk][s][q].size() * sizeof(double));
+        ptr[q] = h->d_line + off;
+        off += lcv[irk][s][q].size();
+      }
+      LineCoef &L = h->lc[irk][s];
+      L.A = ptr[0]; L.B = ptr[1]; L.C = ptr[2]; L.Pk = ptr[3]; L.Qk = ptr[4]; L.Pseg = ptr[5]; L.Qseg = ptr[6];
+      L.n = (int)lcv[irk][s][0].size();
+      L.P = (int)lcv[irk][s][5].size();
+    }
+  HIPCHK(h, hipMemcpy(h->d_line, lb.data(), tot * sizeof(double), hipMemcpyHostToDevice));
+  // transform tables
+  {
+    std::vector<double> tt(2 * (size_t)h->nt), tz(2 * (size_t)h->nz), ez(2 * (size_t)h->nz);
+    for (int q = 0; q < h->nt; q++) { tt[q] = cos(2.0 * M_PI * q / h->nt); tt[h->nt + q] = sin(2.0 * M_PI * q / h->nt); }
+    for (int q = 0; q < h->nz; q++) {
+      tz[q] = cos(2.0 * M_PI * q / h->nz); tz[h->nz + q] = sin(2.0 * M_PI * q / h->nz);
+      ez[q] = cos(M_PI * q / (2.0 * h->nz)); ez[h->nz + q] = sin(M_PI * q / (2.0 * h->nz));
+    }
+    if ((rc = dalloc(h, &h->d_twt, tt.size())) || (rc = dalloc(h, &h->d_twz, tz.size())) ||
+        (rc = dalloc(h, &h->d_ez, ez.size())))
+      return rc;
+    HIPCHK(h, hipMemcpy(h->d_twt, tt.data(), tt.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_twz, tz.data(), tz.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_ez, ez.data(), ez.size() * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> aab(h->nr + 1), ccb(h->nr + 1);   // poisson.F90:379-381
+    const double dr = h->rmax / h->nr;
+    for (int j = 1; j <= h->nr; j++) {
+      aab[j] = -1.0 / (dr * dr) + 1.0 / (2.0 * h->hv[0][j] * dr);
+      ccb[j] = -1.0 / (dr * dr) - 1.0 / (2.0 * h->hv[0][j] * dr);
+    }
+    if ((rc = dalloc(h, &h->d_aab, aab.size())) || (rc = dalloc(h, &h->d_ccb, ccb.size()))) return rc;
+    HIPCHK(h, hipMemcpy(h->d_aab, aab.data(), aab.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_ccb, ccb.data(), ccb.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  // fields
+  const size_t H = haloed(h), C = compact(h), W = wsize(h), PL = (size_t)h->nt * (h->nr + 3);
+  Fields &f = h->f;
+  double **dfs[] = {&f.q1, &f.q2, &f.q3, &f.dq1, &f.dq2, &f.dq3, &f.pres, &f.dph};
+  for (auto p2 : dfs) if ((rc = dalloc(h, p2, H))) return rc;
+  void **wfs[] = {&f.ru1, &f.ru2, &f.ru3, &f.r1, &f.r2, &f.r3};
+  for (auto p2 : wfs) if ((rc = dalloc_bytes(h, p2, H * W))) return rc;
+  void **cfs[] = {&f.prg1, &f.prg2, &f.prg3, &f.S, &f.ib};
+  for (auto p2 : cfs) if ((rc = dalloc_bytes(h, p2, C * W))) return rc;
+  double **pls[] = {&f.csort, &f.dq3no, &f.corr3n};
+  for (auto p2 : pls) if ((rc = dalloc(h, p2, PL))) return rc;
+  if ((rc = dalloc(h, &f.flag, 1))) return rc;
+  std::vector<double> ib, ib64;
+  host_radial_pivots(h, ib, ib64);
+  if ((rc = upload_conv(h, f.ib, ib.data(), ib.size(), true))) return rc;
+  if (precision == VRH_MIXED) {
+    if ((rc = dalloc(h, &f.S64, (size_t)5 * h->nr * h->nz)) || (rc = dalloc(h, &f.ib64, ib64.size()))) return rc;
+    HIPCHK(h, hipMemcpy(f.ib64, ib64.data(), ib64.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  std::vector<double> mint(9 * 27);
+  host_mint(h, mint.data());
+  if ((rc = dalloc(h, &h->d_mint, mint.size()))) return rc;
+  HIPCHK(h, hipMemcpy(h->d_mint, mint.data(), mint.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int vrh_destroy(vrh_t *h) {
+  if (!h) return -1;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  Fields &f = h->f;
+  void *ptrs[] = {h->d_vec, h->d_line, h->d_twt, h->d_twz, h->d_ez, h->d_aab, h->d_ccb, h->d_mint, f.q1, f.q2,
+                  f.q3, f.dq1, f.dq2, f.dq3, f.pres, f.dph, f.ru1, f.ru2, f.ru3, f.r1, f.r2, f.r3, f.prg1, f.prg2,
+                  f.prg3, f.S, f.ib, f.S64, f.ib64, f.csort, f.dq3no, f.corr3n, f.flag};
+  for (void *p : ptrs) if (p) hipFree(p);
+  for (auto &b : h->pb) {
+    if (b.u) hipFree(b.u);
+    if (b.ddt2) hipFree(b.ddt2);
+    if (b.alive) hipFree(b.alive);
+  }
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+int vrh_upload_fields(vrh_t *h, const double *q1, const double *q2, const double *q3, const double *pres) {
+  if (!h) return -1;
+  const double *src[4] = {q1, q2, q3, pres};
+  double *dst[4] = {h->f.q1, h->f.q2, h->f.q3, h->f.pres};
+  for (int q = 0; q < 4; q++)
+    if (src[q]) { int rc = upload_conv(h, dst[q], src[q], haloed(h), false); if (rc) return rc; }
+  return 0;
+}
+
+int vrh_download_fields(vrh_t *h, double *q1, double *q2, double *q3, double *pres) {
+  if (!h) return -1;
+  double *dst[4] = {q1, q2, q3, pres};
+  const double *src[4] = {h->f.q1, h->f.q2, h->f.q3, h->f.pres};
+  for (int q = 0; q < 4; q++)
+    if (dst[q]) { int rc = download_conv(h, dst[q], src[q], haloed(h), false); if (rc) return rc; }
+  return 0;
+}
+
+int vrh_upload_state(vrh_t *h, const char *name, const double *host) {
+  if (!h || !name || !host) return -1;
+  if (strcmp(name, "prg") == 0) {
+    const size_t C = compact(h);
+    void *d[3] = {h->f.prg1, h->f.prg2, h->f.prg3};
+    for (int c = 0; c < 3; c++) { int rc = upload_conv(h, d[c], host + c * C, C, true); if (rc) return rc; }
+    return 0;
+  }
+  Named n;
+  if (!lookup(h, name, n)) return fail(h, "unknown field name");
+  return upload_conv(h, n.p, host, n.n, n.work);
+}
+
+int vrh_download_state(vrh_t *h, const char *name, double *host) {
+  if (!h || !name || !host) return -1;
+  if (strcmp(name, "prg") == 0) {
+    const size_t C = compact(h);
+    void *d[3] = {h->f.prg1, h->f.prg2, h->f.prg3};
+    for (int c = 0; c < 3; c++) { int rc = download_conv(h, host + c * C, d[c], C, true); if (rc) return rc; }
+    return 0;
+  }
+  Named n;
+  if (!lookup(h, name, n)) return fail(h, "unknown field name");
+  return download_conv(h, host, n.p, n.n, n.work);
+}
+
+int vrh_device_ptr(vrh_t *h, const char *name, void **ptr) {
+  if (!h || !name || !ptr) return -1;
+  Named n;
+  if (!lookup(h, name, n)) return fail(h, "unknown field name");
+  *ptr = n.p;
+  return 0;
+}
+
+int vrh_set_time(vrh_t *h, double t) { if (!h) return -1; h->ntime = t; return 0; }
+int vrh_get_time(vrh_t *h, double *t) { if (!h || !t) return -1; *t = h->ntime; return 0; }
+
+int vrh_substep(vrh_t *h, int irk) {
+  if (!h) return -1;
+  return substep(h, irk, -1);
+}
+
+int vrh_phase(vrh_t *h, int phase, int irk) {
+  if (!h) return -1;
+  if (phase != VRH_PH_HDNBC && phase != VRH_PH_EXPLICIT && phase != VRH_PH_ADI && phase != VRH_PH_POISSON &&
+      phase != VRH_PH_PROJECT)
+    return fail(h, "unknown phase");
+  return substep(h, irk, phase);
+}
+
+int vrh_step(vrh_t *h, int nsteps) {
+  if (!h || nsteps < 0) return -1;
+  for (int s = 0; s < nsteps; s++) {
+    h->ntime = h->dt + h->ntime;  // loop.F90:53
+    for (int irk = 1; irk <= 3; irk++) {
+      if (h->nbatch > 0 && irk == 1) launch_particles(h, irk);  // loop.F90:66-70
+      int rc = substep(h, irk, -1);
+      if (rc) return rc;
+    }
+  }
+  return check_flag(h);
+}
+
+int vrh_synchronize(vrh_t *h) {
+  if (!h) return -1;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return check_flag(h);
+}
+
+int vrh_poisson_solve(vrh_t *h, const double *rhs, double *phi) {
+  if (!h || !rhs || !phi) return -1;
+  const size_t C = compact(h);
+  double *d_rhs = nullptr, *d_phi = nullptr;
+  HIPCHK(h, hipMalloc(&d_rhs, C * sizeof(double)));
+  HIPCHK(h, hipMalloc(&d_phi, C * sizeof(double)));
+  HIPCHK(h, hipMemcpyAsync(d_rhs, rhs, C * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if (h->prec == VRH_MIXED) launch_poisson<float>(h, 1, d_rhs, d_phi);
+  else launch_poisson<double>(h, 1, d_rhs, d_phi);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(phi, d_phi, C * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  hipFree(d_rhs);
+  hipFree(d_phi);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// vrh_internal.h -- shared declarations of libvrh (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "vrh.h"
+
+#define VRH_HD __host__ __device__ __forceinline__
+
+// Grid constants and strides.  Haloed 3-D fields use the reference's
+// Fortran layout q(nt, 0:nr+2, 0:nz+2) (theta fastest); compact fields
+// (prg components, spectral arrays) use (nt, nr, nz).
+struct VGrid {
+  int nt, nr, nz, nrc, nzc;
+  long sj, sk;  // haloed strides: sj = nt, sk = nt*(nr+3)
+  long ncomp;   // nt*nr*nz
+  double dx1, dx2, dx3, dx1q, dx2q, dx3q, axsym, dt, Re, slip;
+};
+VRH_HD long hidx(const VGrid &g, int i, int j, int k) { return (long)(i - 1) + g.sj * j + g.sk * k; }
+VRH_HD long cidx(const VGrid &g, int i, int j, int k) {
+  return (long)(i - 1) + (long)g.nt * ((long)(j - 1) + (long)g.nr * (long)(k - 1));
+}
+
+// Radial metric vectors (momentum_coef.F90:24-69), each nr+3 long, indexed by
+// the Fortran index j.
+struct VCoef {
+  const double *rm, *rc, *h1d11, *h1d22d, *h1d22n, *h1dare, *h2d11, *h2d22m, *h2d22p, *h2dare,
+      *s1t11, *s1t2n, *s1t2s, *s1t2c, *s2t11, *s2t2p, *s2t2m, *s2t2c, *s3t2n, *s3t2s, *qc;
+};
+struct VRK {
+  double g, r, a;  // gamma, rho, alpha of the substep (para_tools.F90:103-113)
+};
+
+// Line-independent coefficients of one tridiagonal sweep for the partitioned
+// solver: forward g_p = A_p r_p + B_p g_{p-1}; backward x_p = g_p - C_p x_{p+1};
+// Pk/Qk prefix products inside each segment of length M, Pseg/Qseg their
+// segment totals.
+struct LineCoef {
+  const double *A, *B, *C, *Pk, *Qk, *Pseg, *Qseg;
+  int n, P;
+};
+
+enum SweepMode { SW_R1 = 0, SW_R3 = 1, SW_R2 = 2, SW_Z1 = 3, SW_Z3 = 4, SW_Z2 = 5 };
+
+// Work type TW (double in FP64 mode, float in mixed mode); state type TS is
+// always double.
+struct Fields {
+  double *q1, *q2, *q3, *dq1, *dq2, *dq3, *pres, *dph;  // haloed, double
+  void *ru1, *ru2, *ru3, *r1, *r2, *r3;                 // haloed, TW
+  void *prg1, *prg2, *prg3;                             // compact, TW
+  void *S, *ib;                                         // compact spectral + radial pivots, TW
+  double *S64, *ib64;                                   // low modes (5, nr, nz) in mixed mode
+  double *csort, *dq3no, *corr3n;                       // planes (nt, nr+2)
+  int *flag;                                            // NaN flag
+};
+
+struct PartBatch {
+  long cap, n;
+  double st;
+  int nsub;
+  double *u;     // cap x nvar (row-major)
+  double *ddt2;  // cap x nvar, previous rate (RK3 schemes)
+  int *alive;
+};
+
+struct vrh_s {
+  int nt, nr, nz, nrc, nzc, prec, theta_eigen;
+  double rmax, zmax, Re, dt, slip, ntime;
+  VGrid g;
+  VRK rk[4];
+  hipStream_t stream;
+  std::string err;
+  // host copies of the radial vectors
+  std::vector<double> hv[21];
+  double *d_vec;  // device block holding the 21 vectors
+  VCoef coef;
+  // line coefficients: [irk 1..3][sweep 0..5]
+  double *d_line;
+  LineCoef lc[4][6];
+  int Mr, Mz;  // segment lengths of the r and z sweeps
+  // transform tables (double): theta twiddles (nt/2 complex), z twiddles
+  // (nz/2 complex), z post-twiddles (nz complex)
+  double *d_twt, *d_twz, *d_ez;
+  double *d_aab, *d_ccb;  // Poisson radial base coefficients (nr+1)
+  Fields f;
+  double cor_red;  // sum qc^2 da (hdnbc mass correction denominator)
+  int jlim;        // rc(j) <= 0.4999 for j <= jlim (inlet rows)
+  std::vector<double> da;
+  // particles
+  int pscheme, nbatch;
+  std::vector<PartBatch> pb;
+  double *d_mint;  // 9 x 27 least-squares pseudo-inverse
+};
+
+// host math (vrh_host.cpp)
+void host_mesh_vectors(vrh_t *h);
+void host_line_coefs(vrh_t *h, int irk, std::vector<double> out[6][7], int M[6]);
+void host_radial_pivots(vrh_t *h, std::vector<double> &ib, std::vector<double> &ib64);
+double host_profile(double t);
+void host_mint(vrh_t *h, double *mint);
+
+// kernel launchers (vrh_kernels.hip); TW = double or float
+template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, double c3s);
+template <typename TW> void launch_explicit(vrh_t *h, int irk);
+template <typename TW> void launch_theta(vrh_t *h, int irk);
+template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s);
+template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_rhs, double *phi_out);
+template <typename TW> void launch_project(vrh_t *h, int irk);
+void launch_particles(vrh_t *h, int irk);

@@ -1,0 +1,907 @@
+// vrh_kernels.hip -- gfx950 kernels of the vring RK3 substep.
+//
+// All kernels read and write HBM-resident fields in the reference's Fortran
+// layout (theta fastest), so 64-lane wavefronts always span contiguous theta
+// rows and every global access is coalesced.  TW is the work type (double in
+// the FP64 path, float in the mixed path); the state q, the predictor dq,
+// phi and p are always double.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "vrh_internal.h"
+
+namespace {
+
+template <typename T> __device__ __forceinline__ T ldg(const T *p) { return __builtin_nontemporal_load(p); }
+
+// ---------------------------------------------------------------------------
+// hdnbc: outlet convective condition and mass correction, one thread per
+// (theta, r) outlet point (momentum_terms_swirl.F90:225-291).
+// ---------------------------------------------------------------------------
+__global__ void k_hdnbc(VGrid g, VCoef c, double gam, double rom, double cor, const double *__restrict__ q3,
+                        double *__restrict__ csort, double *__restrict__ dq3no, double *__restrict__ corr3n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  int j = blockIdx.y + 1;  // 1..nrc
+  if (i > g.nt) return;
+  long p = (long)(i - 1) + (long)g.nt * j;
+  double qn = q3[hidx(g, i, j, g.nzc)], qm = q3[hidx(g, i, j, g.nz)];
+  double cs = csort[p];
+  if (j <= g.nr && qn > 0.0) cs = (double)0.6f * qn;
+  csort[p] = cs;
+  double tconv = -(cs * g.dx3) * (qn - qm);
+  double c3n = (gam * tconv + rom * dq3no[p]) * g.dt;
+  dq3no[p] = tconv;
+  corr3n[p] = c3n + c.qc[j] * cor;
+}
+
+// ---------------------------------------------------------------------------
+// Explicit part: hdnl1-3 (momentum_terms_swirl.F90:15-174) fused with the
+// explicit right-hand sides of step (step.F90:32-165).  One thread per cell;
+// q is read once per block through L1/L2, H goes straight into ru and qrhs.
+// The inlet planes q1s, q2s, q3s are zero (momentum_terms_swirl.F90:216-223).
+// ---------------------------------------------------------------------------
+template <typename TW>
+__global__ __launch_bounds__(256) void k_explicit(VGrid g, VCoef c, VRK rk, const double *__restrict__ q1,
+                                                  const double *__restrict__ q2, const double *__restrict__ q3,
+                                                  const TW *__restrict__ prg1, const TW *__restrict__ prg2,
+                                                  const TW *__restrict__ prg3, TW *__restrict__ ru1,
+                                                  TW *__restrict__ ru2, TW *__restrict__ ru3, TW *__restrict__ r1,
+                                                  TW *__restrict__ r2, TW *__restrict__ r3) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > g.nt || j > g.nr) return;
+  const int ip = (i == g.nt) ? 1 : i + 1, im = (i == 1) ? g.nt : i - 1;
+  const int jm = (j == 1) ? 1 : j - 1;
+  const long o = hidx(g, i, j, k);
+  const long dip = ip - i, dim = im - i, sj = g.sj, sk = g.sk, djm = (long)(jm - j) * sj;
+  const TW ga = (TW)rk.g, ro = (TW)rk.r, al = (TW)rk.a, dt = (TW)g.dt, aRe = (TW)(rk.a / g.Re);
+  const TW dx3 = (TW)g.dx3, dx3q = (TW)g.dx3q;
+  const long pc = cidx(g, i, j, k);
+#define Q1(d) ((TW)q1[o + (d)])
+#define Q2(d) ((TW)q2[o + (d)])
+#define Q3(d) ((TW)q3[o + (d)])
+  // ---- q1 at (theta_c, r_m, z_m) ----
+  {
+    TW qnn = Q1(dip) + Q1(0), qss = Q1(dim) + Q1(0);
+    TW h11 = (qnn * qnn - qss * qss) * (TW)c.h1d11[j];
+    qnn = (Q2(sj) + Q2(dim + sj)) * (Q1(sj) + Q1(0));
+    qss = (Q1(0) + Q1(djm)) * (Q2(0) + Q2(dim));
+    TW h12d = (qnn - qss) * (TW)c.h1d22d[j];
+    qnn = (Q1(dip) + Q1(0)) * (Q2(sj) + Q2(0));
+    qss = (Q1(0) + Q1(dim)) * (Q2(dim + sj) + Q2(dim));
+    TW h12n = (qnn + qss) * (TW)c.h1d22n[j];
+    qnn = (TW)0.25 * (Q3(sk) + Q3(dim + sk)) * (Q1(sk) + Q1(0));
+    qss = (k == 1) ? (TW)0 : (TW)0.25 * (Q3(0) + Q3(dim)) * (Q1(0) + Q1(-sk));
+    TW h13 = (qnn - qss) * dx3;
+    qnn = Q2(sj) + Q2(0);
+    qss = Q2(dim + sj) + Q2(dim);
+    TW d1q2 = (qnn - qss) * (TW)c.h1dare[j];
+    TW H1 = d1q2 - (h11 + (h12d + h12n) + h13);
+    TW d11 = (Q1(dip) - (TW)2 * Q1(0) + Q1(dim)) * (TW)c.s1t11[j];
+    TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (Q1(sk) - (TW)3 * Q1(0)) * dx3q
+                      : (Q1(sk) - (TW)2 * Q1(0) + Q1(-sk)) * dx3q;
+    TW qn2 = (Q1(sj) / (TW)c.rm[j + 1] - Q1(0) / (TW)c.rm[j]) * (TW)c.s1t2n[j];
+    TW qs2 = (Q1(0) / (TW)c.rm[j] - Q1(djm) / (TW)c.rm[jm]) * (TW)c.s1t2s[j];
+    TW d22 = (qn2 - qs2) * (TW)c.s1t2c[j];
+    r1[o] = (ga * H1 + ro * ru1[o] - al * prg1[pc] + aRe * (d11 + d33 + d22)) * dt;
+    ru1[o] = H1;
+  }
+  // ---- q2 at (theta_m, r_c, z_m), j >= 2 ----
+  if (j >= 2) {
+    TW qnn = (Q1(dip) + Q1(dip - sj)) * (Q2(dip) + Q2(0));
+    TW qss = (Q1(0) + Q1(-sj)) * (Q2(0) + Q2(dim));
+    TW h21 = (qnn - qss) * (TW)c.h2d11[j];
+    qnn = Q2(sj) + Q2(0);
+    qss = Q2(0) + Q2(-sj);
+    TW h22 = qnn * qnn * (TW)c.h2d22p[j] - qss * qss * (TW)c.h2d22m[j];
+    qnn = (TW)0.5 * (Q1(0) + Q1(dip));
+    qss = (TW)0.5 * (Q1(-sj) + Q1(dip - sj));
+    TW h11n = (TW)0.25 * (qnn + qss) * (qnn + qss);
+    qnn = (TW)0.25 * (Q3(sk) + Q3(sk - sj)) * (Q2(sk) + Q2(0));
+    qss = (k == 1) ? (TW)0 : (TW)0.25 * (Q3(0) + Q3(-sj)) * (Q2(0) + Q2(-sk));
+    TW h23 = (qnn - qss) * dx3;
+    qnn = Q1(dip) + Q1(dip - sj);
+    qss = Q1(0) + Q1(-sj);
+    TW d1q1 = -((qnn - qss) * (TW)c.h2dare[j]);
+    TW H2 = d1q1 - (h21 + h22 + h23 - h11n);
+    TW d11 = (Q2(dip) - (TW)2 * Q2(0) + Q2(dim)) * (TW)c.s2t11[j];
+    TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (Q2(sk) - (TW)3 * Q2(0)) * dx3q
+                      : (Q2(sk) - (TW)2 * Q2(0) + Q2(-sk)) * dx3q;
+    TW d22 = Q2(sj) * (TW)c.s2t2p[j] - Q2(0) * (TW)c.s2t2c[j] + Q2(-sj) * (TW)c.s2t2m[j];
+    r2[o] = (ga * H2 + ro * ru2[o] - al * prg2[pc] + aRe * (d11 + d33 + d22)) * dt;
+    ru2[o] = H2;
+  }
+  // ---- q3 at (theta_m, r_m, z_c), k >= 2 ----
+  if (k >= 2) {
+    TW qnn = (Q3(sj) + Q3(0)) * (Q2(sj) + Q2(sj - sk));
+    TW qss = (Q3(djm) + Q3(0)) * (Q2(0) + Q2(-sk));
+    TW h32 = (qnn - qss) * (TW)c.h1d22d[j];
+    qnn = (Q3(0) + Q3(dip)) * (Q1(dip) + Q1(dip - sk));
+    qss = (Q3(dim) + Q3(0)) * (Q1(0) + Q1(-sk));
+    TW h31 = (qnn - qss) * (TW)c.h1d11[j];
+    qnn = (TW)0.25 * (Q3(0) + Q3(sk)) * (Q3(0) + Q3(sk));
+    qss = (TW)0.25 * (Q3(0) + Q3(-sk)) * (Q3(0) + Q3(-sk));
+    TW h33 = (qnn - qss) * dx3;
+    TW H3 = -(h31 + h32 + h33);
+    TW d11 = (Q3(dip) - (TW)2 * Q3(0) + Q3(dim)) * (TW)c.s1t11[j];
+    TW d33 = (Q3(sk) - (TW)2 * Q3(0) + Q3(-sk)) * dx3q;
+    TW qn2 = (Q3(sj) - Q3(0)) * (TW)c.s3t2n[j];
+    TW qs2 = (Q3(0) - Q3(djm)) * (TW)c.s3t2s[j];
+    TW d22 = qn2 - qs2;
+    r3[o] = (ga * H3 + ro * ru3[o] - al * prg3[pc] + aRe * (d11 + d33 + d22)) * dt;
+    ru3[o] = H3;
+  }
+#undef Q1
+#undef Q2
+#undef Q3
+}
+
+// ---------------------------------------------------------------------------
+// Periodic tridiagonal along theta (solqi + tripij, step.F90:212-263,
+// 487-534): constant coefficients (-acc, 1+2acc, -acc) on each line, solved by
+// parallel cyclic reduction in LDS, one thread per theta point.  For a
+// power-of-two ntheta the reduction terminates exactly after log2(ntheta)
+// levels; otherwise the off-diagonal decays to below 1e-18 of the diagonal.
+// ---------------------------------------------------------------------------
+template <typename TW>
+__global__ __launch_bounds__(1024) void k_theta(VGrid g, const double *__restrict__ scoef, double betadx, int j0, int k0,
+                                                TW *__restrict__ r) {
+  extern __shared__ unsigned char smem_raw[];
+  TW *f = reinterpret_cast<TW *>(smem_raw);
+  const int N = g.nt, i = threadIdx.x;
+  const int j = blockIdx.x + j0, k = blockIdx.y + k0;
+  const long base = hidx(g, 1, j, k);
+  TW v = (i < N) ? r[base + i] : (TW)0;
+  if (i < N) f[i] = v;
+  const double acc = betadx * scoef[j];
+  double a = -acc, b = 1.0 + 2.0 * acc;
+  const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
+  __syncthreads();
+  for (int s = 1, lvl = 0; lvl < 40; s <<= 1, ++lvl) {
+    if (pow2 && s >= N) break;
+    if (!pow2 && fabs(a) < 1e-18 * fabs(b)) break;
+    const double al = -a / b;
+    TW nv = (TW)0;
+    if (i < N) {
+      int sm = s % N;
+      int imv = i - sm; if (imv < 0) imv += N;
+      int ipv = i + sm; if (ipv >= N) ipv -= N;
+      nv = v + (TW)al * (f[imv] + f[ipv]);
+    }
+    b = b + 2.0 * al * a;
+    a = al * a;
+    __syncthreads();
+    if (i < N) f[i] = nv;
+    v = nv;
+    __syncthreads();
+  }
+  if (i < N) r[base + i] = pow2 ? (TW)(v / (TW)(b + 2.0 * a)) : (TW)(v / (TW)b);
+}
+
+// ---------------------------------------------------------------------------
+// Partitioned Thomas solve of the r and z sweeps (solqj/solqk with
+// solve_lap_par/solve_lap_thom, step.F90:266-485; tripar.F90:11-145).
+// A block owns 64 lines (one per lane, consecutive theta -> coalesced) and P
+// waves; wave s owns positions [s*M, s*M+M) of every line and keeps them in
+// registers.  The two linear recurrences of the Thomas algorithm are closed
+// across segments with precomputed carry products (one LDS exchange each).
+// ---------------------------------------------------------------------------
+struct LineArgs {
+  LineCoef lc;
+  VGrid g;
+  long nlines;
+  int j0;                      // first r index of z-lines
+  double slip;
+  const double *q;             // state component (z sweeps / q2 r sweep)
+  double *dq;                  // predictor output (z sweeps of q1,q3 / r sweep of q2)
+  void *r;                     // qrhs of the component (TW)
+  const double *corr3n;        // outlet row of the q3 z sweep
+  double c3s;                  // inlet row value of the q3 z sweep (uniform in theta)
+  int jlim;                    // inlet rows: j <= jlim
+};
+
+template <typename TW, int M, int MODE>
+__global__ __launch_bounds__(1024) void k_line(LineArgs a) {
+  __shared__ TW sh_a[16][64];
+  const int lane = threadIdx.x & 63;
+  const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long L = (long)blockIdx.x * 64 + lane;
+  const bool act = L < a.nlines;
+  const VGrid &g = a.g;
+  const int n = a.lc.n, P = a.lc.P;
+  const int p0 = s * M;
+  const int len = min(M, n - p0);
+  TW *rr = reinterpret_cast<TW *>(a.r);
+  // line -> (i, second index)
+  const int i = (int)(L % g.nt) + 1;
+  const int l2 = (int)(L / g.nt);
+  long base;  // address of position 0 of the line
+  long step;  // address stride between positions
+  if (MODE <= SW_R2) { base = hidx(g, i, 1, l2 + 1); step = g.sj; }      // r lines at k = l2+1
+  else { base = hidx(g, i, l2 + a.j0, 1); step = g.sk; }                  // z lines at j = l2+j0
+  const int jz = l2 + a.j0;
+  TW x[M];
+  TW gp = (TW)0;
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    if (t < len) {
+      const int p = p0 + t;
+      TW rv = (TW)0;
+      if (act) {
+        if (MODE == SW_R2) {
+          if (p != 0 && p != n - 1) rv = rr[base + step * p];
+        } else if (MODE == SW_Z1 || MODE == SW_Z2) {
+          if (p != 0 && p != n - 1) rv = rr[base + step * p];
+        } else if (MODE == SW_Z3) {
+          if (p == 0) rv = (jz <= a.jlim) ? (TW)a.c3s : (TW)0;
+          else if (p == n - 1) rv = (TW)a.corr3n[(long)(i - 1) + (long)g.nt * jz];
+          else rv = rr[base + step * p];
+        } else {
+          rv = rr[base + step * p];
+        }
+      }
+      TW v = (TW)a.lc.A[p] * rv + (TW)a.lc.B[p] * gp;
+      x[t] = v;
+      gp = v;
+    }
+  }
+  sh_a[s][lane] = gp;
+  __syncthreads();
+  TW G = (TW)0;
+  for (int s2 = 0; s2 < s; ++s2) G = sh_a[s2][lane] + (TW)a.lc.Pseg[s2] * G;
+  __syncthreads();
+  TW xp = (TW)0;
+#pragma unroll
+  for (int t = M - 1; t >= 0; --t) {
+    if (t < len) {
+      const int p = p0 + t;
+      TW v = x[t] + (TW)a.lc.Pk[p] * G - (TW)a.lc.C[p] * xp;
+      x[t] = v;
+      xp = v;
+    }
+  }
+  sh_a[s][lane] = xp;
+  __syncthreads();
+  TW X = (TW)0;
+  for (int s2 = P - 1; s2 > s; --s2) X = sh_a[s2][lane] + (TW)a.lc.Qseg[s2] * X;
+  if (!act) return;
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    if (t < len) {
+      const int p = p0 + t;
+      const TW v = x[t] + (TW)a.lc.Qk[p] * X;
+      const long ad = base + step * p;
+      if (MODE == SW_R1 || MODE == SW_R3 || MODE == SW_Z2) {
+        rr[ad] = v;
+      } else if (MODE == SW_R2) {
+        // dq2 = x + q2 for j = 1..nr; dq2(nrc) = 0 (step.F90:330-333, 183-187)
+        a.dq[ad] = (p == n - 1) ? 0.0 : (double)v + a.q[ad];
+      } else {
+        // dq = q + x for k = 1..nzc; slip row at nrc (step.F90:84-88, 133-138)
+        const double d = a.q[ad] + (double)v;
+        a.dq[ad] = d;
+        if (jz == g.nr) a.dq[ad + g.sj] = a.slip * d;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Poisson solve (poisson_solve.F90:140-179) in five passes: divergence +
+// theta FFT, z DCT-II, radial Thomas, z DCT-III, inverse theta FFT.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int bitrev(int x, int logn) { return (int)(__builtin_bitreverse32((unsigned)x) >> (32 - logn)); }
+
+template <typename T>
+__device__ __forceinline__ void lds_fft(T *re, T *im, int nfft, int n, int logn, const double *twr, const double *twi,
+                                        double sign, int tid, int nth) {
+  // in-place radix-2 decimation in time on bit-reversed input, nfft
+  // sequences of length n stored back to back
+  for (int len = 2; len <= n; len <<= 1) {
+    const int half = len >> 1, tstr = n / len;
+    for (int b = tid; b < nfft * (n >> 1); b += nth) {
+      const int q = b / (n >> 1), bb = b - q * (n >> 1);
+      const int grp = bb / half, pos = bb - grp * half;
+      const int i0 = q * n + grp * len + pos, i1 = i0 + half;
+      const T wr = (T)twr[pos * tstr], wi = (T)(sign * twi[pos * tstr]);
+      const T xr = re[i1] * wr - im[i1] * wi, xi = re[i1] * wi + im[i1] * wr;
+      const T ar = re[i0], ai = im[i0];
+      re[i1] = ar - xr; im[i1] = ai - xi;
+      re[i0] = ar + xr; im[i0] = ai + xi;
+    }
+    __syncthreads();
+  }
+}
+
+// divq (divergence.F90:31-41) + fft_ptf_multi packing
+// (fastfouriertransform.F90:39-43) for LB theta lines per block.  Power-of-two
+// ntheta: two real lines per complex FFT in LDS.  Other ntheta: direct DFT.
+// In mixed mode the packed positions 0..4 (m = 0,1,2) are also produced in
+// double from the double divergence into S64 (layout (5, nr, nz)).
+template <typename TW>
+__global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt, const double *__restrict__ dq1,
+                                                const double *__restrict__ dq2, const double *__restrict__ dq3,
+                                                const double *__restrict__ ext, TW *__restrict__ S,
+                                                double *__restrict__ S64, const double *__restrict__ twr,
+                                                const double *__restrict__ twi, int LB, int logn) {
+  extern __shared__ unsigned char smem_raw[];
+  const int N = g.nt, tid = threadIdx.x, nth = blockDim.x;
+  const long nlines = (long)g.nr * g.nz;
+  const long l0 = (long)blockIdx.x * LB;
+  double *dv = reinterpret_cast<double *>(smem_raw);           // LB*N doubles
+  TW *re = reinterpret_cast<TW *>(dv + (size_t)LB * N);        // (LB/2)*N
+  TW *im = re + (size_t)(LB / 2) * N;
+  const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
+  for (int e = tid; e < LB * N; e += nth) {
+    const int lp = e / N, ii = e - lp * N;
+    const long l = l0 + lp;
+    double val = 0.0;
+    if (l < nlines) {
+      const int j = (int)(l % g.nr) + 1, k = (int)(l / g.nr) + 1, i = ii + 1;
+      if (ext) {
+        val = ext[cidx(g, i, j, k)];
+      } else {
+        const int ip = (i == N) ? 1 : i + 1;
+        const long o = hidx(g, i, j, k);
+        const double t1 = (dq1[o + (ip - i)] - dq1[o]) * g.dx1 * g.axsym / c.rm[j];
+        const double t2 = (dq2[o + g.sj] - dq2[o]) * g.dx2 / (c.rm[j] * 1.0);
+        const double t3 = (dq3[o + g.sk] - dq3[o]) * g.dx3;
+        val = -(t1 + t2 + t3) * inv_adt;
+      }
+    }
+    dv[e] = val;
+    if (pow2) {
+      const int q = lp >> 1;
+      const int pos = q * N + bitrev(ii, logn);
+      if (lp & 1) im[pos] = (TW)val; else re[pos] = (TW)val;
+    }
+  }
+  __syncthreads();
+  if (pow2) lds_fft(re, im, LB / 2, N, logn, twr, twi, -1.0, tid, nth);
+  const int nhalf = N / 2 + 1;
+  for (int e = tid; e < LB * N; e += nth) {
+    const int lp = e / N, pos = e - lp * N;
+    const long l = l0 + lp;
+    if (l >= nlines) continue;
+    const int k = (pos + 1) >> 1;  // wavenumber of this packed position
+    double Fr, Fi;
+    if (pow2) {
+      const int q = lp >> 1, kr = (N - k) & (N - 1);
+      const double zr = re[q * N + k], zi = im[q * N + k], wr = re[q * N + kr], wi = im[q * N + kr];
+      if ((lp & 1) == 0) { Fr = 0.5 * (zr + wr); Fi = 0.5 * (zi - wi); }
+      else { Fr = 0.5 * (zi + wi); Fi = -0.5 * (zr - wr); }
+    } else {
+      Fr = 0.0; Fi = 0.0;
+      for (int jj = 0; jj < N; ++jj) {
+        const int ph = (int)(((long)jj * k) % N);
+        const double x = dv[lp * N + jj];
+        Fr += x * twr[ph];
+        Fi -= x * twi[ph];
+      }
+    }
+    double outv;
+    if (pos == 0) outv = Fr / N;
+    else if (pos & 1) outv = 2.0 * Fr / N;
+    else outv = -2.0 * Fi / N;
+    (void)nhalf;
+    S[(size_t)pos + (size_t)N * l] = (TW)outv;
+    if (S64 && pos < 5) {
+      // exact double DFT of the lowest modes
+      double sr = 0.0, si = 0.0;
+      for (int jj = 0; jj < N; ++jj) {
+        const int ph = (int)(((long)jj * k) % N);
+        const double x = dv[lp * N + jj];
+        sr += x * twr[ph];
+        si -= x * twi[ph];
+      }
+      double o64 = (pos == 0) ? sr / N : ((pos & 1) ? 2.0 * sr / N : -2.0 * si / N);
+      S64[(size_t)pos + 5 * l] = o64;
+    }
+  }
+}
+
+// z DCT-II (REDFT10, then /(2nz): fastfouriertransform.F90:299-332) of TL
+// lines per block; lines are contiguous in memory (line stride 1, position
+// stride nl).  Makhoul's reordering turns each pair of real lines into one
+// complex FFT of length nz.  dir=+1 forward, -1 inverse (REDFT01).
+template <typename TW>
+__global__ __launch_bounds__(256) void k_dctz(TW *__restrict__ S, long nl, int n, int logn, int TL, int dir,
+                                              const double *__restrict__ twr, const double *__restrict__ twi,
+                                              const double *__restrict__ er, const double *__restrict__ ei) {
+  extern __shared__ unsigned char smem_raw[];
+  TW *re = reinterpret_cast<TW *>(smem_raw);
+  TW *im = re + (size_t)(TL / 2) * n;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const long l0 = (long)blockIdx.x * TL;
+  const bool pow2 = n >= 2 && (n & (n - 1)) == 0;
+  if (pow2) {
+    for (int e = tid; e < TL * n; e += nth) {
+      const int lp = e % TL, kz = e / TL;  // consecutive threads -> consecutive lines
+      const long l = l0 + lp;
+      const int q = lp >> 1;
+      if (dir > 0) {
+        TW v = (l < nl) ? S[l + nl * kz] : (TW)0;
+        const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
+        const int slot = q * n + bitrev(p, logn);
+        if (lp & 1) im[slot] = v; else re[slot] = v;
+      } else if ((lp & 1) == 0) {
+        // W[k] = e^{i pi k/2n} ((Ya[k] + Yb[n-k]) + i (Yb[k] - Ya[n-k])), Y[n]=0
+        const long la = l, lb = l + 1;
+        const int k = kz;
+        const TW ya = (la < nl) ? S[la + nl * k] : (TW)0, yb = (lb < nl) ? S[lb + nl * k] : (TW)0;
+        const TW yan = (k > 0 && la < nl) ? S[la + nl * (n - k)] : (TW)0;
+        const TW ybn = (k > 0 && lb < nl) ? S[lb + nl * (n - k)] : (TW)0;
+        const TW u = ya + ybn, w = yb - yan;
+        const TW cr = (TW)er[k], ci = (TW)ei[k];  // e^{+i pi k / 2n}
+        const int slot = q * n + bitrev(k, logn);
+        re[slot] = u * cr - w * ci;
+        im[slot] = u * ci + w * cr;
+      }
+    }
+    __syncthreads();
+    lds_fft(re, im, TL / 2, n, logn, twr, twi, dir > 0 ? -1.0 : 1.0, tid, nth);
+    for (int e = tid; e < TL * n; e += nth) {
+      const int lp = e % TL, kz = e / TL;
+      const long l = l0 + lp;
+      if (l >= nl) continue;
+      const int q = lp >> 1;
+      TW outv;
+      if (dir > 0) {
+        const int k = kz, kr = (n - k) & (n - 1);
+        const TW zr = re[q * n + k], zi = im[q * n + k], wr = re[q * n + kr], wi = im[q * n + kr];
+        TW vr, vi;
+        if ((lp & 1) == 0) { vr = (TW)0.5 * (zr + wr); vi = (TW)0.5 * (zi - wi); }
+        else { vr = (TW)0.5 * (zi + wi); vi = (TW)(-0.5) * (zr - wr); }
+        outv = (vr * (TW)er[k] + vi * (TW)ei[k]) / (TW)n;  // Re(e^{-i pi k/2n} V)/n
+      } else {
+        const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
+        outv = (lp & 1) ? im[q * n + p] : re[q * n + p];
+      }
+      S[l + nl * kz] = outv;
+    }
+  } else {
+    // direct transform for non-power-of-two nz (small test grids)
+    TW *buf = re;  // TL * n
+    for (int e = tid; e < TL * n; e += nth) {
+      const int lp = e % TL, kz = e / TL;
+      const long l = l0 + lp;
+      buf[lp * n + kz] = (l < nl) ? S[l + nl * kz] : (TW)0;
+    }
+    __syncthreads();
+    for (int e = tid; e < TL * n; e += nth) {
+      const int lp = e % TL, kz = e / TL;
+      const long l = l0 + lp;
+      double sacc = 0.0;
+      if (dir > 0) {
+        for (int jj = 0; jj < n; ++jj) sacc += (double)buf[lp * n + jj] * cos(M_PI * (jj + 0.5) * kz / n);
+        sacc = 2.0 * sacc / (2.0 * n);
+      } else {
+        sacc = buf[lp * n];
+        for (int kk = 1; kk < n; ++kk) sacc += 2.0 * (double)buf[lp * n + kk] * cos(M_PI * kk * (kz + 0.5) / n);
+      }
+      if (l < nl) S[l + nl * kz] = (TW)sacc;
+    }
+  }
+}
+
+// Radial tridiagonal solves of every (mode, kz) system (solve_radial,
+// poisson_solve.F90:270-323) with precomputed inverse pivots ib; lanes over
+// the packed mode index (contiguous), P waves of M rows each.
+template <typename T, int M>
+__global__ __launch_bounds__(1024) void k_radial(T *__restrict__ S, const T *__restrict__ ib, long nm, long nlines,
+                                                 int nr, const double *__restrict__ aab, const double *__restrict__ ccb) {
+  __shared__ T sh_v[16][64];
+  __shared__ T sh_p[16][64];
+  const int lane = threadIdx.x & 63;
+  const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int P = blockDim.x >> 6;
+  const long L = (long)blockIdx.x * 64 + lane;
+  const bool act = L < nlines;
+  const long pos = L % nm, kz = L / nm;
+  const long base = pos + nm * (long)nr * kz;
+  const int p0 = s * M, len = min(M, nr - p0);
+  T x[M], w[M];
+  T gp = (T)0, Pacc = (T)1;
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    if (t < len) {
+      const int p = p0 + t;
+      const long ad = base + nm * p;
+      const T ibv = act ? ib[ad] : (T)1, rv = act ? S[ad] : (T)0;
+      const T B = -(T)aab[p + 1] * ibv * (T)(p > 0);
+      const T v = ibv * rv + B * gp;
+      w[t] = ibv;
+      x[t] = v;
+      gp = v;
+      Pacc *= B;
+    }
+  }
+  sh_v[s][lane] = gp;
+  sh_p[s][lane] = Pacc;
+  __syncthreads();
+  T G = (T)0;
+  for (int s2 = 0; s2 < s; ++s2) G = sh_v[s2][lane] + sh_p[s2][lane] * G;
+  __syncthreads();
+  T Pr = (T)1;
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    if (t < len) {
+      const int p = p0 + t;
+      Pr *= -(T)aab[p + 1] * w[t] * (T)(p > 0);
+      x[t] += Pr * G;
+    }
+  }
+  T xp = (T)0, Qacc = (T)1;
+#pragma unroll
+  for (int t = M - 1; t >= 0; --t) {
+    if (t < len) {
+      const int p = p0 + t;
+      const T C = (T)ccb[p + 1] * w[t] * (T)(p < nr - 1);
+      const T v = x[t] - C * xp;
+      x[t] = v;
+      xp = v;
+      Qacc *= -C;
+    }
+  }
+  sh_v[s][lane] = xp;
+  sh_p[s][lane] = Qacc;
+  __syncthreads();
+  T X = (T)0;
+  for (int s2 = P - 1; s2 > s; --s2) X = sh_v[s2][lane] + sh_p[s2][lane] * X;
+  T Qr = (T)1;
+#pragma unroll
+  for (int t = M - 1; t >= 0; --t) {
+    if (t < len) {
+      const int p = p0 + t;
+      Qr *= -(T)ccb[p + 1] * w[t] * (T)(p < nr - 1);
+      const T v = x[t] + Qr * X;
+      if (act) S[base + nm * p] = v;
+    }
+  }
+}
+
+// fft_ftp_multi (fastfouriertransform.F90:74-82) + copy into dph
+// (poisson_solve.F90:29) + dph(nzc)=dph(nz) (divergence.F90:93-99).
+template <typename TW>
+__global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S, const double *__restrict__ S64,
+                                              double *__restrict__ phi_out, double *__restrict__ dph,
+                                              const double *__restrict__ twr, const double *__restrict__ twi, int LB,
+                                              int logn) {
+  extern __shared__ unsigned char smem_raw[];
+  const int N = g.nt, tid = threadIdx.x, nth = blockDim.x;
+  const long nlines = (long)g.nr * g.nz;
+  const long l0 = (long)blockIdx.x * LB;
+  TW *re = reinterpret_cast<TW *>(smem_raw);
+  TW *im = re + (size_t)(LB / 2) * N;
+  TW *tb = im + (size_t)(LB / 2) * N;  // packed input, LB*N
+  const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
+  for (int e = tid; e < LB * N; e += nth) {
+    const int lp = e / N, pos = e - lp * N;
+    const long l = l0 + lp;
+    TW v = (l < nlines) ? S[(size_t)pos + (size_t)N * l] : (TW)0;
+    if (S64 && pos < 5) v = (TW)0;
+    tb[e] = v;
+  }
+  __syncthreads();
+  if (pow2) {
+    // Z[k] = Fa[k] + i Fb[k], Fa[k] = (t(2k-1) - i t(2k))/2, Hermitian extension
+    for (int e = tid; e < (LB / 2) * N; e += nth) {
+      const int q = e / N, k = e - q * N;
+      double ar, ai, br, bi;
+      auto coef = [&](const TW *t, int kk, double &cr, double &ci) {
+        if (kk == 0) { cr = t[0]; ci = 0.0; return; }
+        int kh = kk <= N / 2 ? kk : N - kk;
+        double a = t[2 * kh - 1];
+        double b = (2 * kh < N) ? t[2 * kh] : 0.0;
+        cr = 0.5 * a;
+        ci = (kk <= N / 2) ? -0.5 * b : 0.5 * b;
+        if (2 * kh == N) ci = 0.0;
+      };
+      coef(tb + (2 * q) * N, k, ar, ai);
+      coef(tb + (2 * q + 1) * N, k, br, bi);
+      const int slot = q * N + bitrev(k, logn);
+      re[slot] = (TW)(ar - bi);
+      im[slot] = (TW)(ai + br);
+    }
+    __syncthreads();
+    lds_fft(re, im, LB / 2, N, logn, twr, twi, 1.0, tid, nth);
+  }
+  for (int e = tid; e < LB * N; e += nth) {
+    const int lp = e / N, ii = e - lp * N;
+    const long l = l0 + lp;
+    if (l >= nlines) continue;
+    double v;
+    if (pow2) {
+      const int q = lp >> 1;
+      v = (lp & 1) ? (double)im[q * N + ii] : (double)re[q * N + ii];
+    } else {
+      const TW *t = tb + lp * N;
+      v = t[0];
+      for (int kk = 1; 2 * kk - 1 < N; ++kk) {
+        const int ph = (int)(((long)ii * kk) % N);
+        const double a = t[2 * kk - 1], b = (2 * kk < N) ? t[2 * kk] : 0.0;
+        if (2 * kk == N) v += 0.5 * a * twr[ph];
+        else v += a * twr[ph] + b * twi[ph];
+      }
+    }
+    if (S64) {
+      const double *t = S64 + 5 * l;
+      v += t[0];
+      for (int kk = 1; kk <= 2 && 2 * kk - 1 < N; ++kk) {
+        const int ph = (int)(((long)ii * kk) % N);
+        const double a = t[2 * kk - 1], b = (2 * kk < N) ? t[2 * kk] : 0.0;
+        if (2 * kk == N) v += 0.5 * a * twr[ph];
+        else v += a * twr[ph] + b * twi[ph];
+      }
+    }
+    const int j = (int)(l % g.nr) + 1, k = (int)(l / g.nr) + 1;
+    if (phi_out) phi_out[cidx(g, ii + 1, j, k)] = v;
+    if (dph) {
+      dph[hidx(g, ii + 1, j, k)] = v;
+      if (k == g.nz) dph[hidx(g, ii + 1, j, g.nzc)] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// upsol (divergence.F90:69-236) fused with uppres (divergence.F90:239-480):
+// one thread per (theta, j=0..nrc, k=1..nzc); the gradient qcap of each
+// component is recomputed from dph where the reference stores it.
+// ---------------------------------------------------------------------------
+template <typename TW>
+__global__ __launch_bounds__(256) void k_project(VGrid g, VCoef c, double adt, double be,
+                                                 const double *__restrict__ dph, const double *__restrict__ dq1,
+                                                 const double *__restrict__ dq2, const double *__restrict__ dq3,
+                                                 double *__restrict__ q1, double *__restrict__ q2,
+                                                 double *__restrict__ q3, TW *__restrict__ prg1,
+                                                 TW *__restrict__ prg2, TW *__restrict__ prg3,
+                                                 double *__restrict__ pres, int *__restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..nrc
+  const int k = blockIdx.z + 1;                        // 1..nzc
+  if (i > g.nt || j > g.nrc) return;
+  const int nt = g.nt, nr = g.nr, nz = g.nz, nrc = g.nrc, nzc = g.nzc;
+  const int ip = (i == nt) ? 1 : i + 1, im = (i == 1) ? nt : i - 1;
+  const long o = hidx(g, i, j, k);
+  const long sj = g.sj, sk = g.sk;
+#define PH(di, dj, dk) dph[o + (long)(di) + (long)(dj) * sj + (long)(dk) * sk]
+  // ---- upsol ----
+  if (j >= 1 && j <= nr) {
+    q1[o] = dq1[o] - (adt * g.dx1 / c.rm[j]) * (PH(0, 0, 0) - PH(im - i, 0, 0));
+    if (j == nr) q1[o + sj] = g.slip * q1[o];
+  }
+  if (j >= 2 && j <= nr) q2[o] = dq2[o] - adt * c.rc[j] * g.dx2 * (PH(0, 0, 0) - PH(0, -1, 0)) / 1.0;
+  if (j == 1 || j == nrc) q2[o] = 0.0;
+  if (k == 1) q3[o] = dq3[o];
+  if (k >= 2 && j >= 1 && j <= nr) {
+    const double v = dq3[o] - adt * g.dx3 * (PH(0, 0, 0) - PH(0, 0, -1));
+    q3[o] = v;
+    if (j == nr) q3[o + sj] = g.slip * v;
+  }
+  if (k > nz || j < 1 || j > nr) return;
+  // ---- uppres ----
+  const int jm = (j == 1) ? 1 : j - 1, jp = (j == nr) ? nr : j + 1;
+  const int km = (k == 1) ? 1 : ((k == nz) ? nz : k - 1), kp = k + 1;
+  const long pc = cidx(g, i, j, k);
+  // theta component: qcap_t(i,j,k) = (dph(i)-dph(i-1)) dx1/rm(j), j=1..nr, k=1..nzc
+  auto qt = [&](int ii, int jj, int kk) {
+    const int iim = (ii == 1) ? nt : ii - 1;
+    const long oo = hidx(g, ii, jj, kk);
+    return (dph[oo] - dph[oo + (iim - ii)]) * g.dx1 / c.rm[jj];
+  };
+  if (nt > 1) {
+    const double qc0 = qt(i, j, k);
+    const double d11 = (qt(ip, j, k) - 2.0 * qc0 + qt(im, j, k)) * c.s1t11[j];
+    const double d33 = (qt(i, j, kp) - 2.0 * qc0 + qt(i, j, km)) * g.dx3q;
+    const double qnn = (qt(i, jp, k) / c.rm[j + 1] - qc0 / c.rm[j]);
+    const double qss = (qc0 / c.rm[j] - qt(i, jm, k) / c.rm[jm]);
+    const double d22 = (qnn * c.s1t2n[j] - qss * c.s1t2s[j]) * c.s1t2c[j];
+    prg1[pc] = (TW)((double)prg1[pc] + qc0 - be * (d11 + d33 + d22));
+  }
+  // r component: qcap_r(j) = (dph(j)-dph(j-1)) dx2 rc(j), j=2..nr; 0 at j=1, nrc
+  auto qr = [&](int ii, int jj, int kk) {
+    if (jj <= 1 || jj >= nrc) return 0.0;
+    const long oo = hidx(g, ii, jj, kk);
+    return (dph[oo] - dph[oo - sj]) * g.dx2 * c.rc[jj] / 1.0;
+  };
+  if (j >= 2) {
+    const double qc0 = qr(i, j, k);
+    const double d11 = (qr(ip, j, k) - 2.0 * qc0 + qr(im, j, k)) * c.s2t11[j];
+    const double d33 = (qr(i, j, kp) - 2.0 * qc0 + qr(i, j, km)) * g.dx3q;
+    const double qnn = qr(i, j + 1, k) - qc0, qss = qc0 - qr(i, j - 1, k);
+    const double d22 = qnn * c.s2t2p[j] - qss * c.s2t2m[j];
+    prg2[pc] = (TW)((double)prg2[pc] + qc0 - be * (d11 + d22 + d33));
+  }
+  // z component: qcap_z(k) = (dph(k)-dph(k-1)) dx3, k=2..nzc; 0 at k=1
+  auto qz = [&](int ii, int jj, int kk) {
+    if (kk <= 1) return 0.0;
+    const long oo = hidx(g, ii, jj, kk);
+    return (dph[oo] - dph[oo - sk]) * g.dx3;
+  };
+  {
+    const double qzc = qz(i, j, k + 1);
+    if (qzc != qzc) atomicOr(flag, 1);  // "Flow field crash" (divergence.F90:396-399)
+  }
+  if (k >= 2) {
+    const double qc0 = qz(i, j, k);
+    const double d11 = (qz(ip, j, k) - 2.0 * qc0 + qz(im, j, k)) * c.s1t11[j];
+    const double d33 = (qz(i, j, k + 1) - 2.0 * qc0 + qz(i, j, k - 1)) * g.dx3q;
+    const double qnn = (qz(i, jp, k) - qc0), qss = (qc0 - qz(i, jm, k));
+    const double d22 = qnn * c.s3t2n[j] - qss * c.s3t2s[j];
+    prg3[pc] = (TW)((double)prg3[pc] + qc0 - be * (d11 + d22 + d33));
+  }
+  {
+    const double p0 = PH(0, 0, 0);
+    const double d11 = (PH(ip - i, 0, 0) + PH(im - i, 0, 0) - 2.0 * p0) * c.s1t11[j];
+    const double qnn = PH(0, jp - j, 0) - p0, qss = p0 - PH(0, jm - j, 0);
+    const double d22 = qnn * c.s3t2n[j] - qss * c.s3t2s[j];
+    const double d33 = (PH(0, 0, kp - k) + PH(0, 0, km - k) - 2.0 * p0) * g.dx3q;
+    pres[o] = pres[o] + p0 - be * (d11 + d22 + d33);
+  }
+#undef PH
+}
+
+}  // namespace
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+#define CHECK_LAUNCH() (void)hipGetLastError()
+
+template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, double c3s) {
+  (void)c3s;
+  const VGrid &g = h->g;
+  dim3 blk(64), grd((g.nt + 63) / 64, g.nrc);
+  hipLaunchKernelGGL(k_hdnbc, grd, blk, 0, h->stream, g, h->coef, h->rk[irk].g, h->rk[irk].r, cor, h->f.q3,
+                     h->f.csort, h->f.dq3no, h->f.corr3n);
+}
+
+template <typename TW> void launch_explicit(vrh_t *h, int irk) {
+  const VGrid &g = h->g;
+  int bx = g.nt >= 64 ? 64 : g.nt;
+  int by = 256 / bx;
+  dim3 blk(bx, by), grd((g.nt + bx - 1) / bx, (g.nr + by - 1) / by, g.nz);
+  hipLaunchKernelGGL(k_explicit<TW>, grd, blk, 0, h->stream, g, h->coef, h->rk[irk], h->f.q1, h->f.q2, h->f.q3,
+                     (const TW *)h->f.prg1, (const TW *)h->f.prg2, (const TW *)h->f.prg3, (TW *)h->f.ru1,
+                     (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1, (TW *)h->f.r2, (TW *)h->f.r3);
+}
+
+template <typename TW> void launch_theta(vrh_t *h, int irk) {
+  const VGrid &g = h->g;
+  int nth = 1;
+  while (nth < g.nt) nth <<= 1;
+  if (nth < 64) nth = 64;
+  const double betadx = 0.5 * h->rk[irk].a * h->dt / h->Re;
+  size_t sh = sizeof(TW) * nth;
+  // q1: j=1..nr, k=1..nz; q3: j=1..nr, k=2..nz; q2: j=2..nr, k=1..nz (step.F90:225-236)
+  hipLaunchKernelGGL(k_theta<TW>, dim3(g.nr, g.nz), dim3(nth), sh, h->stream, g, h->coef.s1t11, betadx, 1, 1,
+                     (TW *)h->f.r1);
+  if (g.nz > 1)
+    hipLaunchKernelGGL(k_theta<TW>, dim3(g.nr, g.nz - 1), dim3(nth), sh, h->stream, g, h->coef.s1t11, betadx, 1, 2,
+                       (TW *)h->f.r3);
+  if (g.nr > 1)
+    hipLaunchKernelGGL(k_theta<TW>, dim3(g.nr - 1, g.nz), dim3(nth), sh, h->stream, g, h->coef.s2t11, betadx, 2, 1,
+                       (TW *)h->f.r2);
+}
+
+template <typename TW, int MODE>
+static void line_launch(vrh_t *h, const LineArgs &a, int M) {
+  const int P = a.lc.P;
+  dim3 blk(64 * P), grd((unsigned)((a.nlines + 63) / 64));
+  switch (M) {
+    case 16: hipLaunchKernelGGL((k_line<TW, 16, MODE>), grd, blk, 0, h->stream, a); break;
+    case 32: hipLaunchKernelGGL((k_line<TW, 32, MODE>), grd, blk, 0, h->stream, a); break;
+    case 48: hipLaunchKernelGGL((k_line<TW, 48, MODE>), grd, blk, 0, h->stream, a); break;
+    default: hipLaunchKernelGGL((k_line<TW, 64, MODE>), grd, blk, 0, h->stream, a); break;
+  }
+}
+
+template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s) {
+  const VGrid &g = h->g;
+  LineArgs a{};
+  a.g = g;
+  a.slip = h->slip;
+  a.jlim = h->jlim;
+  a.c3s = c3s;
+  a.corr3n = h->f.corr3n;
+  // q1: r then z (step.F90:75-78)
+  a.lc = h->lc[irk][SW_R1]; a.nlines = (long)g.nt * g.nz; a.r = h->f.r1;
+  line_launch<TW, SW_R1>(h, a, h->Mr);
+  a.lc = h->lc[irk][SW_Z1]; a.nlines = (long)g.nt * g.nr; a.j0 = 1; a.q = h->f.q1; a.dq = h->f.dq1;
+  line_launch<TW, SW_Z1>(h, a, h->Mz);
+  // q3: r then z (step.F90:124-127)
+  a.lc = h->lc[irk][SW_R3]; a.nlines = (long)g.nt * g.nz; a.r = h->f.r3;
+  line_launch<TW, SW_R3>(h, a, h->Mr);
+  a.lc = h->lc[irk][SW_Z3]; a.nlines = (long)g.nt * g.nr; a.j0 = 1; a.q = h->f.q3; a.dq = h->f.dq3;
+  line_launch<TW, SW_Z3>(h, a, h->Mz);
+  // q2: z then r (step.F90:171-177)
+  a.lc = h->lc[irk][SW_Z2]; a.nlines = (long)g.nt * (g.nr - 1); a.j0 = 2; a.r = h->f.r2;
+  line_launch<TW, SW_Z2>(h, a, h->Mz);
+  a.lc = h->lc[irk][SW_R2]; a.nlines = (long)g.nt * g.nzc; a.q = h->f.q2; a.dq = h->f.dq2;
+  line_launch<TW, SW_R2>(h, a, h->Mr);
+}
+
+static int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
+
+template <typename T>
+static void radial_launch(vrh_t *h, T *S, const T *ib, long nm) {
+  const VGrid &g = h->g;
+  const int M = 32;
+  const int P = (g.nr + M - 1) / M;
+  const long nlines = nm * g.nz;
+  dim3 blk(64 * P), grd((unsigned)((nlines + 63) / 64));
+  hipLaunchKernelGGL((k_radial<T, 32>), grd, blk, 0, h->stream, S, ib, nm, nlines, g.nr, h->d_aab, h->d_ccb);
+}
+
+template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_rhs, double *phi_out) {
+  const VGrid &g = h->g;
+  const int N = g.nt, logn = ilog2(N);
+  const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
+  int LB = pow2 ? 1024 / N : 2;
+  if (LB > 16) LB = 16;
+  if (LB < 2) LB = 2;
+  const long nlines = (long)g.nr * g.nz;
+  TW *S = (TW *)h->f.S;
+  double *S64 = (h->prec == VRH_MIXED) ? h->f.S64 : nullptr;
+  const double adt = h->rk[irk].a * h->dt;
+  size_t sh1 = sizeof(double) * LB * N + sizeof(TW) * LB * N;
+  hipLaunchKernelGGL(k_divfft<TW>, dim3((unsigned)((nlines + LB - 1) / LB)), dim3(256), sh1, h->stream, g, h->coef,
+                     1.0 / adt, h->f.dq1, h->f.dq2, h->f.dq3, ext_rhs, S, S64, h->d_twt, h->d_twt + N, LB, logn);
+  // z transforms
+  const int nz = g.nz, lz = ilog2(nz);
+  int TL = (int)(65536 / (sizeof(TW) * nz));  // at most 64 KB of LDS per block
+  if (TL > 16) TL = 16;
+  TL &= ~1;
+  if (TL < 2) TL = 2;
+  int TL64 = (int)(65536 / (sizeof(double) * nz));
+  if (TL64 > 16) TL64 = 16;
+  TL64 &= ~1;
+  if (TL64 < 2) TL64 = 2;
+  const long nl = (long)N * g.nr;
+  size_t sh2 = sizeof(TW) * TL * nz;
+  hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(256), sh2, h->stream, S, nl, nz, lz, TL,
+                     1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
+  if (S64) {
+    const long nl64 = 5L * g.nr;
+    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(256),
+                       sizeof(double) * TL64 * nz, h->stream, S64, nl64, nz, lz, TL64, 1, h->d_twz, h->d_twz + nz,
+                       h->d_ez, h->d_ez + nz);
+  }
+  radial_launch<TW>(h, S, (const TW *)h->f.ib, N);
+  if (S64) radial_launch<double>(h, S64, h->f.ib64, 5);
+  hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(256), sh2, h->stream, S, nl, nz, lz, TL,
+                     -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
+  if (S64) {
+    const long nl64 = 5L * g.nr;
+    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(256),
+                       sizeof(double) * TL64 * nz, h->stream, S64, nl64, nz, lz, TL64, -1, h->d_twz, h->d_twz + nz,
+                       h->d_ez, h->d_ez + nz);
+  }
+  size_t sh3 = sizeof(TW) * 2 * LB * N;
+  hipLaunchKernelGGL(k_ifft<TW>, dim3((unsigned)((nlines + LB - 1) / LB)), dim3(256), sh3, h->stream, g, S, S64,
+                     phi_out, ext_rhs ? nullptr : h->f.dph, h->d_twt, h->d_twt + N, LB, logn);
+}
+
+template <typename TW> void launch_project(vrh_t *h, int irk) {
+  const VGrid &g = h->g;
+  const double adt = h->rk[irk].a * h->dt;
+  const double be = 0.5 * (h->rk[irk].a * h->dt) / h->Re;
+  int bx = g.nt >= 64 ? 64 : g.nt;
+  int by = 256 / bx;
+  dim3 blk(bx, by), grd((g.nt + bx - 1) / bx, (g.nr + 2 + by - 1) / by, g.nzc);
+  hipLaunchKernelGGL(k_project<TW>, grd, blk, 0, h->stream, g, h->coef, adt, be, h->f.dph, h->f.dq1, h->f.dq2,
+                     h->f.dq3, h->f.q1, h->f.q2, h->f.q3, (TW *)h->f.prg1, (TW *)h->f.prg2, (TW *)h->f.prg3,
+                     h->f.pres, h->f.flag);
+}
+
+#define INST(TW)                                                   \
+  template void launch_hdnbc<TW>(vrh_t *, int, double, double);    \
+  template void launch_explicit<TW>(vrh_t *, int);                 \
+  template void launch_theta<TW>(vrh_t *, int);                    \
+  template void launch_sweeps<TW>(vrh_t *, int, double);           \
+  template void launch_poisson<TW>(vrh_t *, int, const double *, double *); \
+  template void launch_project<TW>(vrh_t *, int);
+INST(double)
+INST(float)

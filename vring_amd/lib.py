@@ -1,0 +1,195 @@
+"""ctypes binding of libvrh (include/vrh.h) and a thin Solver class.
+
+Arrays cross the boundary in the reference's Fortran layouts:
+haloed fields (ntheta, nr+3, nz+3) with index (i-1, j, k), prg (ntheta, nr,
+nz, 3), Poisson arrays (ntheta, nr, nz).  There is no CPU fallback: every
+call runs on the GPU through libvrh, and loading fails loudly if the library
+is missing."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libvrh.so")
+
+PRECISION = {"fp64": 0, "mixed": 1}
+THETA = {"exact_m2": 0, "consistent": 1}
+PHASE = {"hdnbc": 2, "explicit": 3, "adi": 6, "poisson": 8, "project": 9}
+PART_SCHEME = {"lsq_euler": 0, "lsq_rk3": 1, "trilinear_rk3": 2}
+
+
+class VrhGrid(ctypes.Structure):
+    _fields_ = [("ntheta", ctypes.c_int), ("nr", ctypes.c_int), ("nz", ctypes.c_int),
+                ("rmax", ctypes.c_double), ("zmax", ctypes.c_double)]
+
+
+class VrhParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("Re", ctypes.c_double), ("slip", ctypes.c_double),
+                ("theta_eigen", ctypes.c_int)]
+
+
+EXPORTS = ["vrh_create", "vrh_destroy", "vrh_last_error", "vrh_upload_fields", "vrh_download_fields",
+           "vrh_upload_state", "vrh_download_state", "vrh_set_time", "vrh_get_time", "vrh_substep",
+           "vrh_step", "vrh_phase", "vrh_poisson_solve", "vrh_synchronize", "vrh_device_ptr",
+           "vrh_ring_init", "vrh_cfl_dt", "vrh_particles_create", "vrh_particles_upload",
+           "vrh_particles_download", "vrh_particles_step"]
+
+_lib = None
+
+
+def load_library(path=LIBPATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libvrh not built: {path} missing (run __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    lp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+    L.vrh_create.argtypes = [ctypes.POINTER(VrhGrid), ctypes.POINTER(VrhParams), I, ctypes.POINTER(P)]
+    L.vrh_destroy.argtypes = [P]
+    L.vrh_last_error.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
+    L.vrh_upload_fields.argtypes = [P, P, P, P, P]
+    L.vrh_download_fields.argtypes = [P, P, P, P, P]
+    L.vrh_upload_state.argtypes = [P, ctypes.c_char_p, dp]
+    L.vrh_download_state.argtypes = [P, ctypes.c_char_p, dp]
+    L.vrh_set_time.argtypes = [P, D]
+    L.vrh_get_time.argtypes = [P, ctypes.POINTER(D)]
+    L.vrh_substep.argtypes = [P, I]
+    L.vrh_step.argtypes = [P, I]
+    L.vrh_phase.argtypes = [P, I, I]
+    L.vrh_poisson_solve.argtypes = [P, dp, dp]
+    L.vrh_synchronize.argtypes = [P]
+    L.vrh_device_ptr.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(P)]
+    L.vrh_ring_init.argtypes = [ctypes.POINTER(VrhGrid), I, dp, D, I, I, P, dp, dp, dp]
+    L.vrh_cfl_dt.argtypes = [ctypes.POINTER(VrhGrid), dp, dp, dp, D, D, ctypes.POINTER(D)]
+    L.vrh_particles_create.argtypes = [P, I, lp, dp, ip, I]
+    L.vrh_particles_upload.argtypes = [P, I, ctypes.c_long, dp, ip]
+    L.vrh_particles_download.argtypes = [P, I, ctypes.c_long, dp, ip]
+    L.vrh_particles_step.argtypes = [P, I]
+    _lib = L
+    return L
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+
+
+def ring_initial_condition(ntheta, nr, nz, rmax, zmax, rings, pert_amp=0.0, pert_modes=(0, -1),
+                           phases=None):
+    """Haloed (q1, q2, q3) of Gaussian-core rings [(R, z0, sigma, Gamma), ...]."""
+    L = load_library()
+    g = VrhGrid(ntheta, nr, nz, rmax, zmax)
+    shape = (ntheta, nr + 3, nz + 3)
+    n = int(np.prod(shape))
+    q1, q2, q3 = np.zeros(n), np.zeros(n), np.zeros(n)
+    rr = np.ascontiguousarray(np.asarray(rings, dtype=np.float64).ravel())
+    ph = None
+    if phases is not None:
+        ph = np.ascontiguousarray(np.asarray(phases, dtype=np.float64))
+    rc = L.vrh_ring_init(ctypes.byref(g), len(rings), rr, pert_amp, int(pert_modes[0]), int(pert_modes[1]),
+                         ph.ctypes.data if ph is not None else None, q1, q2, q3)
+    if rc:
+        raise RuntimeError("vrh_ring_init failed")
+    return tuple(a.reshape(shape, order="F") for a in (q1, q2, q3))
+
+
+def cfl_dt(ntheta, nr, nz, rmax, zmax, q1, q2, q3, cfl=0.3, ujet=1.0):
+    L = load_library()
+    g = VrhGrid(ntheta, nr, nz, rmax, zmax)
+    out = ctypes.c_double(0.0)
+    rc = L.vrh_cfl_dt(ctypes.byref(g), _f64(q1), _f64(q2), _f64(q3), cfl, ujet, ctypes.byref(out))
+    if rc:
+        raise RuntimeError("vrh_cfl_dt failed")
+    return out.value
+
+
+class Solver:
+    """One whole-domain solver state on the GPU (the reference's `type(run)`)."""
+
+    def __init__(self, ntheta, nr, nz, rmax, zmax, Re, dt, slip=0.0, theta_eigen="exact_m2",
+                 precision="fp64"):
+        self.L = load_library()
+        self.nt, self.nr, self.nz = ntheta, nr, nz
+        self.rmax, self.zmax, self.Re, self.dt = rmax, zmax, Re, dt
+        self.precision = precision
+        self.h = ctypes.c_void_p()
+        g = VrhGrid(ntheta, nr, nz, rmax, zmax)
+        p = VrhParams(dt, Re, slip, THETA[theta_eigen])
+        rc = self.L.vrh_create(ctypes.byref(g), ctypes.byref(p), PRECISION[precision], ctypes.byref(self.h))
+        self._check(rc, "vrh_create")
+        self.halo_shape = (ntheta, nr + 3, nz + 3)
+        self.comp_shape = (ntheta, nr, nz)
+        self.plane_shape = (ntheta, nr + 3)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            buf = ctypes.create_string_buffer(512)
+            if self.h:
+                self.L.vrh_last_error(self.h, buf, 512)
+            raise RuntimeError(f"{what} failed ({rc}): {buf.value.decode(errors='replace')}")
+
+    def close(self):
+        if self.h:
+            self.L.vrh_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape_of(self, name):
+        if name in ("prg",):
+            return self.comp_shape + (3,)
+        if name in ("prg1", "prg2", "prg3", "spec"):
+            return self.comp_shape
+        if name in ("csort", "dq3no", "corr3n"):
+            return self.plane_shape
+        return self.halo_shape
+
+    def upload(self, name, arr):
+        a = _f64(arr)
+        assert a.size == int(np.prod(self.shape_of(name))), name
+        self._check(self.L.vrh_upload_state(self.h, name.encode(), a), f"upload {name}")
+
+    def download(self, name):
+        shp = self.shape_of(name)
+        out = np.zeros(int(np.prod(shp)))
+        self._check(self.L.vrh_download_state(self.h, name.encode(), out), f"download {name}")
+        return out.reshape(shp, order="F")
+
+    def set_time(self, t):
+        self._check(self.L.vrh_set_time(self.h, float(t)), "set_time")
+
+    def time(self):
+        t = ctypes.c_double(0.0)
+        self._check(self.L.vrh_get_time(self.h, ctypes.byref(t)), "get_time")
+        return t.value
+
+    def substep(self, irk):
+        self._check(self.L.vrh_substep(self.h, irk), "substep")
+
+    def phase(self, name, irk):
+        self._check(self.L.vrh_phase(self.h, PHASE[name], irk), f"phase {name}")
+
+    def step(self, n=1):
+        self._check(self.L.vrh_step(self.h, int(n)), "step")
+
+    def synchronize(self):
+        self._check(self.L.vrh_synchronize(self.h), "synchronize")
+
+    def poisson(self, rhs):
+        r = _f64(rhs)
+        out = np.zeros_like(r)
+        self._check(self.L.vrh_poisson_solve(self.h, r, out), "poisson")
+        return out.reshape(self.comp_shape, order="F")
+
+    def device_ptr(self, name):
+        p = ctypes.c_void_p()
+        self._check(self.L.vrh_device_ptr(self.h, name.encode(), ctypes.byref(p)), "device_ptr")
+        return p.value
