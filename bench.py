@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark of the vring RK3 substep on one MI355X.
+
+Workload (the north-star shape): nr=256, ntheta=128, nz=512, r in [0,4],
+z in [0,12], Re=3000, one Gaussian-core ring (R=1, z0=3, sigma=0.2, Gamma=1)
+with seeded core-radius perturbations of amplitude 1e-3 on modes m=2..8
+(phases from numpy PCG64(0)), dt at CFL 0.3, inputs resident in HBM.
+A step is one full RK3 step (3 substeps: boundary terms, fused explicit
+stencils, ADI sweeps, FFT/DCT/tridiagonal pressure solve, projection and
+pressure update), plus the Lagrangian particle step when particles are on.
+
+`value` is the mixed-precision path (FP32 stencils, sweeps and transforms;
+FP64 for the m=0..2 pressure modes, the predictor, the projection and the
+state) in Mcell-RK3-steps/s; the FP64 path, the ratio and the mixed path's
+error against the FP64 path are reported beside it.  The roofline object is
+the dominant kernel of the reported path, timed with HIP events on the
+solver's stream over the timed region; the CPU baseline is the FP64 oracle
+(a restatement of the reference algorithm) on one host core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+NT, NR, NZ, RMAX, ZMAX, RE = 128, 256, 512, 4.0, 12.0, 3000.0
+# algorithmic bytes per cell of the reference phases fused in each kernel
+# (phase model: A hdnl 6 passes, B explicit RHS 18, C ADI 21, D divq 4,
+#  E Poisson 10, F upsol 7, G uppres 18 -> 84 per substep, 252 per step)
+PASSES_PER_STEP = 252
+
+
+def kernel_alg_bytes_per_cell(name, prec):
+    """Algorithmic bytes per cell of one launch of kernel `name`."""
+    w = 8 if prec == "fp64" else 4   # work arrays
+    s = 8                            # state q, predictor dq, phi, p (always FP64)
+    if name == "explicit":           # A + B: in q(3), H(3); per comp q,H,ru,prg in, qrhs,ru out
+        return 3 * s + 3 * w + 3 * (s + 5 * w)
+    if name == "project":            # F + G: upsol in dq(3),phi out q(3) ... uppres 3x(phi in,qcap out,
+        # qcap in, prg in, prg out) + phi in, p in, p out
+        return (3 * s + s + 3 * s) + 3 * (s + w + w + w + w) + 3 * s
+    if name == "line_z":             # per launch: rhs in, q in, dq out (one component)
+        return w + s + s
+    if name == "line_r":             # rhs in, rhs out
+        return 2 * w
+    if name == "divfft":             # D: dq(3) in, rhs out; E: fft in/out
+        return 3 * s + w + w + w
+    return None
+
+
+def ring_state(nt=NT, nr=NR, nz=NZ):
+    import vring_amd
+    phases = np.random.Generator(np.random.PCG64(0)).uniform(0.0, 2.0 * np.pi, 7)
+    q = vring_amd.ring_initial_condition(nt, nr, nz, RMAX, ZMAX, [(1.0, 3.0, 0.2, 1.0)], pert_amp=1e-3,
+                                         pert_modes=(2, 8), phases=phases)
+    dt = vring_amd.cfl_dt(nt, nr, nz, RMAX, ZMAX, *q, cfl=0.3, ujet=1.0)
+    return q, dt
+
+
+def run_path(prec, q, dt, steps, warmup, particles=0):
+    import vring_amd
+    s = vring_amd.Solver(NT, NR, NZ, RMAX, ZMAX, RE, dt, precision=prec)
+    for n, a in zip(("q1", "q2", "q3"), q):
+        s.upload(n, a)
+    if particles:
+        # 5 um droplets (batch 4 of part_tools.F90:262-269), nsubdt = 5
+        s.seed_particles_torus(particles, R=1.0, z0=3.0, a=0.6, seed=0, nsub=5)
+    tuned = s.autotune(reps=3)
+    s.step(warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)          # returns after the stream has drained
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    fields = {n: s.download(n) for n in ("q1", "q2", "q3", "pres")}
+    # instrumented pass over the same number of steps: per-kernel HIP events
+    s.enable_ktimer(True)
+    s.step(steps)
+    kt = s.kernel_times()
+    s.enable_ktimer(False)
+    s.close()
+    return wall, fields, kt, tuned
+
+
+def rel_errors(a, b):
+    vs = max(np.abs(b[n]).max() for n in ("q1", "q2", "q3"))
+    out = {n: float(np.abs(a[n] - b[n]).max() / vs) for n in ("q1", "q2", "q3")}
+    out["pres"] = float(np.abs(a["pres"] - b["pres"]).max() / max(np.abs(b["pres"]).max(), 1e-300))
+    return out
+
+
+def roofline(kt, prec, ncell):
+    best = max(kt.items(), key=lambda kv: kv[1][0])
+    name, (ms, cnt) = best
+    per_cell = kernel_alg_bytes_per_cell(name, prec)
+    avg_ms = ms / cnt
+    if per_cell is None:
+        return {"kernel": name, "avg_ms": avg_ms, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": None, "traffic": None}
+    ach = per_cell * ncell / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{name}_{prec}_256x128x512.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    return {"kernel": name, "avg_ms": round(avg_ms, 4), "bound": "hbm", "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 3), "traffic": traffic,
+            "alg_bytes_per_cell": per_cell}
+
+
+def cpu_baseline(q, dt):
+    """FP64 oracle (liboracle.so, one host core) on one RK3 substep of the
+    same workload at full size; Mcell-RK3-steps/s = cells / (3 x substep)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle
+    o = Oracle(NT, NR, NZ, RMAX, ZMAX, RE, dt)
+    for n, a in zip(("q1", "q2", "q3"), q):
+        o.field(n)[...] = a
+    o.set_time(dt)
+    t0 = time.perf_counter()
+    o.substep(1)
+    t = time.perf_counter() - t0
+    o.close()
+    return {"value": round(NT * NR * NZ / (3.0 * t) / 1e6, 4), "unit": "Mcell-RK3-steps/s", "cores": 1,
+            "kind": "port", "sample": f"one RK3 substep at 256x128x512 took {t:.1f} s (x3 per step)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--particles", type=int, default=1000000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    q, dt = ring_state()
+    ncell = NT * NR * NZ
+    wall64, f64, kt64, tune64 = run_path("fp64", q, dt, args.steps, args.warmup, args.particles)
+    wallmx, fmx, ktmx, tunemx = run_path("mixed", q, dt, args.steps, args.warmup, args.particles)
+    v64 = ncell * args.steps / wall64 / 1e6
+    vmx = ncell * args.steps / wallmx / 1e6
+    err = rel_errors(fmx, f64)
+    rf = roofline(ktmx, "mixed", ncell)
+    rf64 = roofline(kt64, "fp64", ncell)
+    step_frac64 = PASSES_PER_STEP * 8 * ncell * (args.steps / wall64) / (HBM_PEAK_GBS * 1e9)
+    out = {
+        "metric": "Mcell-RK3-steps/s at 256x128x512, mixed+learned vs plain FP64 @1e-6 err",
+        "value": round(vmx, 2),
+        "unit": "Mcell-RK3-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wallmx / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64 (mixed)",
+        "data": "synthetic: seeded Gaussian ring with m=2..8 core perturbations",
+        "config": {"workload": "nr=256 ntheta=128 nz=512 Re=3000 ring, 1 GPU", "nr": NR, "ntheta": NT,
+                   "nz": NZ, "dt": dt, "particles": args.particles},
+        "fp64": {"value": round(v64, 2), "ms_per_step": round(wall64 / args.steps * 1e3, 4),
+                 "step_hbm_frac": round(step_frac64, 3), "roofline": rf64},
+        "mixed_over_fp64": round(vmx / v64, 3),
+        "mixed_rel_err_vs_fp64": err,
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ktmx.items()},
+        "fp64_kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt64.items()},
+        "autotune": {"fp64": tune64, "mixed": tunemx},
+        "particles_per_s": (round(args.particles * args.steps / wallmx, 1) if args.particles else None),
+        "roofline": rf,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(q, dt)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

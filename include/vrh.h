@@ -46,8 +46,9 @@ enum { VRH_FP64 = 0, VRH_MIXED = 1 };
 enum {
   VRH_PH_HDNBC = 2,    /* momentum_terms_swirl.F90:177 hdnbc                   */
   VRH_PH_EXPLICIT = 3, /* hdnl1-3 (momentum_terms_swirl.F90:15-174) + step's
-                          explicit right-hand sides (step.F90:32-165)          */
-  VRH_PH_ADI = 6,      /* step's solqi/solqj/solqk sweeps (step.F90:72-187)    */
+                          explicit right-hand sides (step.F90:32-165) + the
+                          periodic theta sweeps solqi (step.F90:212-263)       */
+  VRH_PH_ADI = 6,      /* step's solqj/solqk sweeps (step.F90:266-485)         */
   VRH_PH_POISSON = 8,  /* divq + laplace_solve (divergence.F90:16;
                           poisson_solve.F90:13)                                */
   VRH_PH_PROJECT = 9   /* upsol + uppres (divergence.F90:69, 239)              */
@@ -85,6 +86,18 @@ int vrh_poisson_solve(vrh_t *h, const double *rhs, double *phi);
 /* Wait for the handle's stream; returns the deferred "Flow field crash"
  * error (divergence.F90:396-399) if a NaN was met. */
 int vrh_synchronize(vrh_t *h);
+/* Per-kernel timing with HIP events on the handle's stream: enable (resets
+ * the totals), run work, then read kernel `idx` (0..count-1): its name,
+ * total milliseconds and launch count.  vrh_ktimer_read returns the number
+ * of kernel kinds when idx < 0. */
+int vrh_ktimer_enable(vrh_t *h, int on);
+int vrh_ktimer_read(vrh_t *h, int idx, char *name, size_t len, double *total_ms, long *count);
+/* Autotuner: times every candidate segment length of the six partitioned
+ * line sweeps and the FFT/DCT tile sizes with HIP events on the live
+ * buffers and keeps the fastest.  Call between steps.  choice[8] (optional)
+ * receives M for sweeps r1 r3 r2 z1 z3 z2, theta FFT lines/block, z DCT
+ * lines/block. */
+int vrh_autotune(vrh_t *h, int reps, int *choice);
 /* Device address of a named field (for callers that keep data in HBM). */
 int vrh_device_ptr(vrh_t *h, const char *name, void **ptr);
 

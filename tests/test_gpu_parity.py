@@ -57,3 +57,25 @@ def test_ring_100_steps_fp64_matches_oracle():
     errs = compare(o, s, ["q1", "q2", "q3"], 1e-10, scale=vscale)
     errs.update(compare(o, s, ["pres"], 1e-10))
     print("100-step relative errors", errs, flush=True)
+
+
+def test_mixed_precision_ring_error_vs_fp64():
+    """Mixed path (FP32 stencils/sweeps/transforms, FP64 low modes, predictor,
+    projection and state) against the FP64 path after 50 steps of the ring:
+    relative field error below 1e-6."""
+    nt, nr, nz, rmax, zmax = 32, 64, 128, 4.0, 12.0
+    q1, q2, q3 = vring_amd.ring_initial_condition(nt, nr, nz, rmax, zmax, [(1.0, 3.0, 0.2, 1.0)],
+                                                  pert_amp=1e-3, pert_modes=(2, 8), phases=np.arange(7.0))
+    dt = vring_amd.cfl_dt(nt, nr, nz, rmax, zmax, q1, q2, q3, cfl=0.3, ujet=1.0)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = vring_amd.Solver(nt, nr, nz, rmax, zmax, 1000.0, dt, precision=prec)
+        for n, a in (("q1", q1), ("q2", q2), ("q3", q3)):
+            s.upload(n, a)
+        s.step(50)
+        out[prec] = {n: s.download(n) for n in ("q1", "q2", "q3", "pres")}
+    vs = max(np.abs(out["fp64"][n]).max() for n in ("q1", "q2", "q3"))
+    errs = {n: np.abs(out["mixed"][n] - out["fp64"][n]).max() / vs for n in ("q1", "q2", "q3")}
+    errs["pres"] = np.abs(out["mixed"]["pres"] - out["fp64"]["pres"]).max() / np.abs(out["fp64"]["pres"]).max()
+    print("mixed vs fp64 after 50 steps", errs, flush=True)
+    assert max(errs.values()) < 1e-6, errs

@@ -103,10 +103,7 @@ template <typename TW> int substep_t(vrh_t *h, int irk, int phase) {
   const double cor = (qinf - 0.0) / (h->cor_red + (double)1.e-15f);  // :283
   if (phase < 0 || phase == VRH_PH_HDNBC) launch_hdnbc<TW>(h, irk, cor, c3s);
   if (phase < 0 || phase == VRH_PH_EXPLICIT) launch_explicit<TW>(h, irk);
-  if (phase < 0 || phase == VRH_PH_ADI) {
-    launch_theta<TW>(h, irk);
-    launch_sweeps<TW>(h, irk, c3s);
-  }
+  if (phase < 0 || phase == VRH_PH_ADI) launch_sweeps<TW>(h, irk, c3s);
   if (phase < 0 || phase == VRH_PH_POISSON) launch_poisson<TW>(h, irk, nullptr, nullptr);
   if (phase < 0 || phase == VRH_PH_PROJECT) launch_project<TW>(h, irk);
   HIPCHK(h, hipGetLastError());
@@ -176,38 +173,41 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
                            &h->coef.s2t2p, &h->coef.s2t2m, &h->coef.s2t2c, &h->coef.s3t2n, &h->coef.s3t2s,
                            &h->coef.qc};
   for (int q = 0; q < 21; q++) *cv[q] = h->d_vec + (size_t)q * nv;
-  // line coefficients for the three substeps
-  h->Mr = h->nr <= 128 ? 32 : 64;
-  h->Mz = h->nzc <= 160 ? 32 : 64;
-  while ((h->nr + 1 + h->Mr - 1) / h->Mr > 16) h->Mr *= 2;
-  while ((h->nzc + h->Mz - 1) / h->Mz > 16) h->Mz *= 2;
-  if (h->Mr > 64 || h->Mz > 64) return fail(h, "grid too large for the line solver (nr+1, nz+1 <= 1024)");
-  std::vector<double> lcv[4][6][7];
-  int M[6] = {h->Mr, h->Mr, h->Mr, h->Mz, h->Mz, h->Mz};
-  size_t tot = 0;
+  // parallel-cyclic-reduction depth of the theta solves: log2(nt) for a
+  // power-of-two nt, otherwise until the off-diagonal ratio is below 1e-18
   for (int irk = 1; irk <= 3; irk++) {
-    host_line_coefs(h, irk, lcv[irk], M);
-    for (int s = 0; s < 6; s++)
-      for (int q = 0; q < 7; q++) tot += lcv[irk][s][q].size();
-  }
-  rc = dalloc(h, &h->d_line, tot);
-  if (rc) return rc;
-  std::vector<double> lb(tot);
-  size_t off = 0;
-  for (int irk = 1; irk <= 3; irk++)
-    for (int s = 0; s < 6; s++) {
-      const double *ptr[7];
-      for (int q = 0; q < 7; q++) {
-        std::memcpy(&lb[off], lcv[irk][s][q].data(), lcv[irk][s][q].size() * sizeof(double));
-        ptr[q] = h->d_line + off;
-        off += lcv[irk][s][q].size();
-      }
-      LineCoef &L = h->lc[irk][s];
-      L.A = ptr[0]; L.B = ptr[1]; L.C = ptr[2]; L.Pk = ptr[3]; L.Qk = ptr[4]; L.Pseg = ptr[5]; L.Qseg = ptr[6];
-      L.n = (int)lcv[irk][s][0].size();
-      L.P = (int)lcv[irk][s][5].size();
+    const double betadx = 0.5 * h->rk[irk].a * h->dt / h->Re;
+    int lv = 0;
+    if ((h->nt & (h->nt - 1)) == 0) {
+      while ((1 << lv) < h->nt) lv++;
+    } else {
+      for (int j = 1; j <= h->nr; j++)
+        for (int which = 0; which < 2; which++) {
+          double acc = betadx * h->hv[which ? 14 : 10][j];
+          double a = -acc, b = 1.0 + 2.0 * acc;
+          int l = 0;
+          while (fabs(a) >= 1e-18 * fabs(b) && l < 40) { double al = -a / b; b += 2.0 * al * a; a *= al; l++; }
+          if (l > lv) lv = l;
+        }
     }
-  HIPCHK(h, hipMemcpy(h->d_line, lb.data(), tot * sizeof(double), hipMemcpyHostToDevice));
+    h->theta_levels[irk] = lv;
+  }
+  // line coefficients for the three substeps; default segment lengths
+  // minimise the padded work P*M (P <= 16 waves per block)
+  {
+    const int ns[6] = {h->nr, h->nr, h->nrc, h->nzc, h->nzc, h->nzc};
+    for (int sw = 0; sw < 6; sw++) {
+      int best = -1, bestw = 1 << 30;
+      for (int M : {16, 24, 32, 40, 48, 64}) {
+        int P = (ns[sw] + M - 1) / M;
+        if (P > 16) continue;
+        if (P * M < bestw) { bestw = P * M; best = M; }
+      }
+      if (best < 0) return fail(h, "grid too large for the line solver (nr+1, nz+1 <= 1024)");
+      h->Msw[sw] = best;
+    }
+  }
+  if ((rc = build_line_coefs(h))) return rc;
   // transform tables
   {
     std::vector<double> tt(2 * (size_t)h->nt), tz(2 * (size_t)h->nz), ez(2 * (size_t)h->nz);
@@ -267,6 +267,7 @@ int vrh_destroy(vrh_t *h) {
                   f.q3, f.dq1, f.dq2, f.dq3, f.pres, f.dph, f.ru1, f.ru2, f.ru3, f.r1, f.r2, f.r3, f.prg1, f.prg2,
                   f.prg3, f.S, f.ib, f.S64, f.ib64, f.csort, f.dq3no, f.corr3n, f.flag};
   for (void *p : ptrs) if (p) hipFree(p);
+  for (auto &e : h->kt_pool) hipEventDestroy(e);
   for (auto &b : h->pb) {
     if (b.u) hipFree(b.u);
     if (b.ddt2) hipFree(b.ddt2);
@@ -382,3 +383,174 @@ int vrh_poisson_solve(vrh_t *h, const double *rhs, double *phi) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// partitioned-Thomas coefficients for the current segment lengths
+// ---------------------------------------------------------------------------
+int build_line_coefs(vrh_t *h) {
+  std::vector<double> lcv[4][6][7];
+  size_t tot = 0;
+  for (int irk = 1; irk <= 3; irk++) {
+    host_line_coefs(h, irk, lcv[irk], h->Msw);
+    for (int s = 0; s < 6; s++)
+      for (int q = 0; q < 7; q++) tot += lcv[irk][s][q].size();
+  }
+  if (h->d_line) { hipStreamSynchronize(h->stream); hipFree(h->d_line); h->d_line = nullptr; }
+  int rc = dalloc(h, &h->d_line, tot);
+  if (rc) return rc;
+  std::vector<double> lb(tot);
+  size_t off = 0;
+  for (int irk = 1; irk <= 3; irk++)
+    for (int s = 0; s < 6; s++) {
+      const double *ptr[7];
+      for (int q = 0; q < 7; q++) {
+        std::memcpy(&lb[off], lcv[irk][s][q].data(), lcv[irk][s][q].size() * sizeof(double));
+        ptr[q] = h->d_line + off;
+        off += lcv[irk][s][q].size();
+      }
+      LineCoef &L = h->lc[irk][s];
+      L.A = ptr[0]; L.B = ptr[1]; L.C = ptr[2]; L.Pk = ptr[3]; L.Qk = ptr[4]; L.Pseg = ptr[5]; L.Qseg = ptr[6];
+      L.n = (int)lcv[irk][s][0].size();
+      L.P = (int)lcv[irk][s][5].size();
+    }
+  HIPCHK(h, hipMemcpy(h->d_line, lb.data(), tot * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: segment length of each line sweep and the transform tile sizes,
+// each timed with HIP events on the live buffers (intermediate arrays only;
+// the state q, p, prg is not touched except through idempotent writes of the
+// predictor dq, which the next substep recomputes).
+// ---------------------------------------------------------------------------
+template <typename TW> static double time_sweeps(vrh_t *h, int reps) {
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0); hipEventCreate(&e1);
+  launch_sweeps<TW>(h, 1, 0.0);
+  hipEventRecord(e0, h->stream);
+  for (int r = 0; r < reps; r++) launch_sweeps<TW>(h, 1, 0.0);
+  hipEventRecord(e1, h->stream);
+  hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0); hipEventDestroy(e1);
+  return ms;
+}
+template <typename TW> static double time_poisson(vrh_t *h, int reps) {
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0); hipEventCreate(&e1);
+  launch_poisson<TW>(h, 1, nullptr, nullptr);
+  hipEventRecord(e0, h->stream);
+  for (int r = 0; r < reps; r++) launch_poisson<TW>(h, 1, nullptr, nullptr);
+  hipEventRecord(e1, h->stream);
+  hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0); hipEventDestroy(e1);
+  return ms;
+}
+
+extern "C" int vrh_autotune(vrh_t *h, int reps, int *choice) {
+  if (!h) return -1;
+  if (reps < 1) reps = 3;
+  const bool kt = h->kt_on;
+  h->kt_on = false;
+  const int ns[6] = {h->nr, h->nr, h->nrc, h->nzc, h->nzc, h->nzc};
+  for (int sw = 0; sw < 6; sw++) {
+    int best = h->Msw[sw];
+    double bestt = 1e30;
+    for (int M : {16, 24, 32, 40, 48, 64}) {
+      if ((ns[sw] + M - 1) / M > 16) continue;
+      h->Msw[sw] = M;
+      int rc = build_line_coefs(h);
+      if (rc) return rc;
+      double t = h->prec == VRH_MIXED ? time_sweeps<float>(h, reps) : time_sweeps<double>(h, reps);
+      if (t < bestt) { bestt = t; best = M; }
+    }
+    h->Msw[sw] = best;
+  }
+  int rc = build_line_coefs(h);
+  if (rc) return rc;
+  const bool pow2t = (h->nt & (h->nt - 1)) == 0;
+  if (pow2t) {
+    int best = 0; double bestt = 1e30;
+    for (int lb : {2, 4, 8, 16}) {
+      if (lb * h->nt > 4096) continue;
+      h->fft_lines = lb;
+      double t = h->prec == VRH_MIXED ? time_poisson<float>(h, reps) : time_poisson<double>(h, reps);
+      if (t < bestt) { bestt = t; best = lb; }
+    }
+    h->fft_lines = best;
+  }
+  {
+    int best = 0; double bestt = 1e30;
+    for (int tl : {2, 4, 8, 16}) {
+      h->dct_lines = tl;
+      double t = h->prec == VRH_MIXED ? time_poisson<float>(h, reps) : time_poisson<double>(h, reps);
+      if (t < bestt) { bestt = t; best = tl; }
+    }
+    h->dct_lines = best;
+  }
+  h->kt_on = kt;
+  if (choice) {
+    for (int sw = 0; sw < 6; sw++) choice[sw] = h->Msw[sw];
+    choice[6] = h->fft_lines;
+    choice[7] = h->dct_lines;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// per-kernel event timer
+// ---------------------------------------------------------------------------
+const char *const kKernelNames[KT_COUNT] = {"hdnbc", "explicit", "theta_pcr", "line_r", "line_z", "divfft",
+                                            "dct_fwd", "radial", "dct_inv", "ifft", "project", "particles"};
+
+KTimer::KTimer(vrh_s *h_, int id_) : h(h_), id(id_), slot(-1) {
+  if (!h || !h->kt_on) return;
+  if ((size_t)(2 * h->kt_used + 2) > h->kt_pool.size()) ktimer_harvest(h);
+  slot = h->kt_used++;
+  h->kt_ids[slot] = id;
+  hipEventRecord(h->kt_pool[2 * slot], h->stream);
+}
+KTimer::~KTimer() {
+  if (slot >= 0) hipEventRecord(h->kt_pool[2 * slot + 1], h->stream);
+}
+
+void ktimer_harvest(vrh_t *h) {
+  if (h->kt_used == 0) return;
+  hipStreamSynchronize(h->stream);
+  for (int s = 0; s < h->kt_used; s++) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->kt_pool[2 * s], h->kt_pool[2 * s + 1]) == hipSuccess) {
+      h->kt_ms[h->kt_ids[s]] += ms;
+      h->kt_cnt[h->kt_ids[s]] += 1;
+    }
+  }
+  h->kt_used = 0;
+}
+
+extern "C" int vrh_ktimer_enable(vrh_t *h, int on) {
+  if (!h) return -1;
+  if (h->kt_pool.empty()) {
+    h->kt_pool.resize(4096);
+    h->kt_ids.resize(2048);
+    for (auto &e : h->kt_pool) HIPCHK(h, hipEventCreate(&e));
+  }
+  ktimer_harvest(h);
+  for (int k = 0; k < KT_COUNT; k++) { h->kt_ms[k] = 0.0; h->kt_cnt[k] = 0; }
+  h->kt_on = on != 0;
+  return 0;
+}
+
+extern "C" int vrh_ktimer_read(vrh_t *h, int idx, char *name, size_t len, double *total_ms, long *count) {
+  if (!h) return -1;
+  if (idx < 0) return KT_COUNT;
+  if (idx >= KT_COUNT) return -1;
+  ktimer_harvest(h);
+  if (name && len) snprintf(name, len, "%s", kKernelNames[idx]);
+  if (total_ms) *total_ms = h->kt_ms[idx];
+  if (count) *count = h->kt_cnt[idx];
+  return 0;
+}

@@ -88,7 +88,7 @@ static void line_coef(const std::vector<double> &aa0, const std::vector<double> 
 // coesys (momentum_coef.F90:71-465) for substep irk, as six sweeps:
 // r-q1, r-q3, r-q2 (identity rows at 1 and nrc, step.F90:315-328),
 // z-q1, z-q3, z-q2.
-void host_line_coefs(vrh_t *h, int irk, std::vector<double> out[6][7], int M[6]) {
+void host_line_coefs(vrh_t *h, int irk, std::vector<double> out[6][7], const int M[6]) {
   const int nr = h->nr, nrc = h->nrc, nz = h->nz, nzc = h->nzc;
   const VGrid &g = h->g;
   const std::vector<double> &rm = h->hv[0], &rc = h->hv[1];

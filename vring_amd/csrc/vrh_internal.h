@@ -65,6 +65,24 @@ struct PartBatch {
   int *alive;
 };
 
+// Kernel ids for the per-kernel event timer.
+enum KernelId {
+  KT_HDNBC = 0, KT_EXPLICIT, KT_THETA, KT_LINE_R, KT_LINE_Z, KT_DIVFFT, KT_DCT_FWD, KT_RADIAL, KT_DCT_INV,
+  KT_IFFT, KT_PROJECT, KT_PARTICLES, KT_COUNT
+};
+extern const char *const kKernelNames[KT_COUNT];
+
+struct vrh_s;
+// RAII bracket of one kernel launch with HIP events on the handle's stream
+// (active only while the timer is enabled).
+struct KTimer {
+  vrh_s *h;
+  int id;
+  int slot;
+  KTimer(vrh_s *h_, int id_);
+  ~KTimer();
+};
+
 struct vrh_s {
   int nt, nr, nz, nrc, nzc, prec, theta_eigen;
   double rmax, zmax, Re, dt, slip, ntime;
@@ -79,7 +97,10 @@ struct vrh_s {
   // line coefficients: [irk 1..3][sweep 0..5]
   double *d_line;
   LineCoef lc[4][6];
-  int Mr, Mz;  // segment lengths of the r and z sweeps
+  int Msw[6];        // segment length M of each sweep (partitioned Thomas)
+  int fft_lines = 0;  // theta lines per block of the FFT kernels (0: default)
+  int dct_lines = 0;  // z lines per block of the DCT kernels (0: default)
+  int theta_levels[4];  // PCR levels of the theta solve per substep
   // transform tables (double): theta twiddles (nt/2 complex), z twiddles
   // (nz/2 complex), z post-twiddles (nz complex)
   double *d_twt, *d_twz, *d_ez;
@@ -92,11 +113,20 @@ struct vrh_s {
   int pscheme, nbatch;
   std::vector<PartBatch> pb;
   double *d_mint;  // 9 x 27 least-squares pseudo-inverse
+  // kernel timer
+  bool kt_on = false;
+  std::vector<hipEvent_t> kt_pool;
+  std::vector<int> kt_ids;  // kernel id per used event pair
+  int kt_used = 0;
+  double kt_ms[KT_COUNT] = {0};
+  long kt_cnt[KT_COUNT] = {0};
 };
+void ktimer_harvest(vrh_t *h);
 
 // host math (vrh_host.cpp)
 void host_mesh_vectors(vrh_t *h);
-void host_line_coefs(vrh_t *h, int irk, std::vector<double> out[6][7], int M[6]);
+void host_line_coefs(vrh_t *h, int irk, std::vector<double> out[6][7], const int M[6]);
+int build_line_coefs(vrh_t *h);
 void host_radial_pivots(vrh_t *h, std::vector<double> &ib, std::vector<double> &ib64);
 double host_profile(double t);
 void host_mint(vrh_t *h, double *mint);
@@ -104,7 +134,6 @@ void host_mint(vrh_t *h, double *mint);
 // kernel launchers (vrh_kernels.hip); TW = double or float
 template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, double c3s);
 template <typename TW> void launch_explicit(vrh_t *h, int irk);
-template <typename TW> void launch_theta(vrh_t *h, int irk);
 template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s);
 template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_rhs, double *phi_out);
 template <typename TW> void launch_project(vrh_t *h, int irk);

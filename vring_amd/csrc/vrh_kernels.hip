@@ -42,16 +42,21 @@ __global__ void k_hdnbc(VGrid g, VCoef c, double gam, double rom, double cor, co
 // The inlet planes q1s, q2s, q3s are zero (momentum_terms_swirl.F90:216-223).
 // ---------------------------------------------------------------------------
 template <typename TW>
-__global__ __launch_bounds__(256) void k_explicit(VGrid g, VCoef c, VRK rk, const double *__restrict__ q1,
-                                                  const double *__restrict__ q2, const double *__restrict__ q3,
-                                                  const TW *__restrict__ prg1, const TW *__restrict__ prg2,
-                                                  const TW *__restrict__ prg3, TW *__restrict__ ru1,
-                                                  TW *__restrict__ ru2, TW *__restrict__ ru3, TW *__restrict__ r1,
-                                                  TW *__restrict__ r2, TW *__restrict__ r3) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  if (i > g.nt || j > g.nr) return;
+__global__ __launch_bounds__(1024) void k_explicit(VGrid g, VCoef c, VRK rk, double betadx, int nlev,
+                                                   const double *__restrict__ q1, const double *__restrict__ q2,
+                                                   const double *__restrict__ q3, const TW *__restrict__ prg1,
+                                                   const TW *__restrict__ prg2, const TW *__restrict__ prg3,
+                                                   TW *__restrict__ ru1, TW *__restrict__ ru2, TW *__restrict__ ru3,
+                                                   TW *__restrict__ r1, TW *__restrict__ r2, TW *__restrict__ r3) {
+  extern __shared__ unsigned char smem_raw[];
+  const int NTP = blockDim.x, RJ = blockDim.y, ti = threadIdx.x, tj = threadIdx.y;
+  const int i = ti + 1;
+  const int j = blockIdx.x * RJ + tj + 1;
+  const int k = blockIdx.y + 1;
+  const bool cell = (i <= g.nt) && (j <= g.nr);
+  const int jc = cell ? j : 1;  // coefficient row for idle threads
+  TW v1 = (TW)0, v2 = (TW)0, v3 = (TW)0;
+  if (cell) {
   const int ip = (i == g.nt) ? 1 : i + 1, im = (i == 1) ? g.nt : i - 1;
   const int jm = (j == 1) ? 1 : j - 1;
   const long o = hidx(g, i, j, k);
@@ -85,7 +90,7 @@ __global__ __launch_bounds__(256) void k_explicit(VGrid g, VCoef c, VRK rk, cons
     TW qn2 = (Q1(sj) / (TW)c.rm[j + 1] - Q1(0) / (TW)c.rm[j]) * (TW)c.s1t2n[j];
     TW qs2 = (Q1(0) / (TW)c.rm[j] - Q1(djm) / (TW)c.rm[jm]) * (TW)c.s1t2s[j];
     TW d22 = (qn2 - qs2) * (TW)c.s1t2c[j];
-    r1[o] = (ga * H1 + ro * ru1[o] - al * prg1[pc] + aRe * (d11 + d33 + d22)) * dt;
+    v1 = (ga * H1 + ro * ru1[o] - al * prg1[pc] + aRe * (d11 + d33 + d22)) * dt;
     ru1[o] = H1;
   }
   // ---- q2 at (theta_m, r_c, z_m), j >= 2 ----
@@ -110,7 +115,7 @@ __global__ __launch_bounds__(256) void k_explicit(VGrid g, VCoef c, VRK rk, cons
     TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (Q2(sk) - (TW)3 * Q2(0)) * dx3q
                       : (Q2(sk) - (TW)2 * Q2(0) + Q2(-sk)) * dx3q;
     TW d22 = Q2(sj) * (TW)c.s2t2p[j] - Q2(0) * (TW)c.s2t2c[j] + Q2(-sj) * (TW)c.s2t2m[j];
-    r2[o] = (ga * H2 + ro * ru2[o] - al * prg2[pc] + aRe * (d11 + d33 + d22)) * dt;
+    v2 = (ga * H2 + ro * ru2[o] - al * prg2[pc] + aRe * (d11 + d33 + d22)) * dt;
     ru2[o] = H2;
   }
   // ---- q3 at (theta_m, r_m, z_c), k >= 2 ----
@@ -130,54 +135,50 @@ __global__ __launch_bounds__(256) void k_explicit(VGrid g, VCoef c, VRK rk, cons
     TW qn2 = (Q3(sj) - Q3(0)) * (TW)c.s3t2n[j];
     TW qs2 = (Q3(0) - Q3(djm)) * (TW)c.s3t2s[j];
     TW d22 = qn2 - qs2;
-    r3[o] = (ga * H3 + ro * ru3[o] - al * prg3[pc] + aRe * (d11 + d33 + d22)) * dt;
+    v3 = (ga * H3 + ro * ru3[o] - al * prg3[pc] + aRe * (d11 + d33 + d22)) * dt;
     ru3[o] = H3;
   }
 #undef Q1
 #undef Q2
 #undef Q3
-}
-
-// ---------------------------------------------------------------------------
-// Periodic tridiagonal along theta (solqi + tripij, step.F90:212-263,
-// 487-534): constant coefficients (-acc, 1+2acc, -acc) on each line, solved by
-// parallel cyclic reduction in LDS, one thread per theta point.  For a
-// power-of-two ntheta the reduction terminates exactly after log2(ntheta)
-// levels; otherwise the off-diagonal decays to below 1e-18 of the diagonal.
-// ---------------------------------------------------------------------------
-template <typename TW>
-__global__ __launch_bounds__(1024) void k_theta(VGrid g, const double *__restrict__ scoef, double betadx, int j0, int k0,
-                                                TW *__restrict__ r) {
-  extern __shared__ unsigned char smem_raw[];
-  TW *f = reinterpret_cast<TW *>(smem_raw);
-  const int N = g.nt, i = threadIdx.x;
-  const int j = blockIdx.x + j0, k = blockIdx.y + k0;
-  const long base = hidx(g, 1, j, k);
-  TW v = (i < N) ? r[base + i] : (TW)0;
-  if (i < N) f[i] = v;
-  const double acc = betadx * scoef[j];
-  double a = -acc, b = 1.0 + 2.0 * acc;
+  }
+  // ---- periodic theta solves of the three components (solqi/tripij,
+  // step.F90:212-263, 487-534): constant coefficients (-acc, 1+2acc, -acc)
+  // per line, parallel cyclic reduction on the LDS-resident lines ----
+  TW *f1 = reinterpret_cast<TW *>(smem_raw) + (size_t)tj * NTP;
+  TW *f2 = f1 + (size_t)RJ * NTP;
+  TW *f3 = f2 + (size_t)RJ * NTP;
+  f1[ti] = v1; f2[ti] = v2; f3[ti] = v3;
+  double a1 = -(betadx * c.s1t11[jc]), b1 = 1.0 + 2.0 * (betadx * c.s1t11[jc]);
+  double a2 = -(betadx * c.s2t11[jc]), b2 = 1.0 + 2.0 * (betadx * c.s2t11[jc]);
+  const int N = g.nt;
   const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
   __syncthreads();
-  for (int s = 1, lvl = 0; lvl < 40; s <<= 1, ++lvl) {
-    if (pow2 && s >= N) break;
-    if (!pow2 && fabs(a) < 1e-18 * fabs(b)) break;
-    const double al = -a / b;
-    TW nv = (TW)0;
-    if (i < N) {
-      int sm = s % N;
-      int imv = i - sm; if (imv < 0) imv += N;
-      int ipv = i + sm; if (ipv >= N) ipv -= N;
-      nv = v + (TW)al * (f[imv] + f[ipv]);
+  int sm = 1 % N;
+  for (int lvl = 0; lvl < nlev; ++lvl) {
+    const double al1 = -a1 / b1, al2 = -a2 / b2;
+    TW n1 = v1, n2 = v2, n3 = v3;
+    if (ti < N) {
+      int imv = ti - sm; if (imv < 0) imv += N;
+      int ipv = ti + sm; if (ipv >= N) ipv -= N;
+      n1 = v1 + (TW)al1 * (f1[imv] + f1[ipv]);
+      n2 = v2 + (TW)al2 * (f2[imv] + f2[ipv]);
+      n3 = v3 + (TW)al1 * (f3[imv] + f3[ipv]);
     }
-    b = b + 2.0 * al * a;
-    a = al * a;
+    b1 = b1 + 2.0 * al1 * a1; a1 = al1 * a1;
+    b2 = b2 + 2.0 * al2 * a2; a2 = al2 * a2;
+    sm = (2 * sm) % N;
     __syncthreads();
-    if (i < N) f[i] = nv;
-    v = nv;
+    f1[ti] = n1; f2[ti] = n2; f3[ti] = n3;
+    v1 = n1; v2 = n2; v3 = n3;
     __syncthreads();
   }
-  if (i < N) r[base + i] = pow2 ? (TW)(v / (TW)(b + 2.0 * a)) : (TW)(v / (TW)b);
+  if (!cell) return;
+  const double d1 = pow2 ? b1 + 2.0 * a1 : b1, d2 = pow2 ? b2 + 2.0 * a2 : b2;
+  const long o = hidx(g, i, j, k);
+  r1[o] = (TW)(v1 / (TW)d1);
+  if (j >= 2) r2[o] = (TW)(v2 / (TW)d2);
+  if (k >= 2) r3[o] = (TW)(v3 / (TW)d1);
 }
 
 // ---------------------------------------------------------------------------
@@ -203,7 +204,7 @@ struct LineArgs {
 };
 
 template <typename TW, int M, int MODE>
-__global__ __launch_bounds__(1024) void k_line(LineArgs a) {
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_per_eu(1, 4))) void k_line(LineArgs a) {
   __shared__ TW sh_a[16][64];
   const int lane = threadIdx.x & 63;
   const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -374,11 +375,13 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
       else { Fr = 0.5 * (zi + wi); Fi = -0.5 * (zr - wr); }
     } else {
       Fr = 0.0; Fi = 0.0;
+      int ph = 0;
       for (int jj = 0; jj < N; ++jj) {
-        const int ph = (int)(((long)jj * k) % N);
         const double x = dv[lp * N + jj];
         Fr += x * twr[ph];
         Fi -= x * twi[ph];
+        ph += k;
+        if (ph >= N) ph -= N;
       }
     }
     double outv;
@@ -390,11 +393,13 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
     if (S64 && pos < 5) {
       // exact double DFT of the lowest modes
       double sr = 0.0, si = 0.0;
+      int ph = 0;
       for (int jj = 0; jj < N; ++jj) {
-        const int ph = (int)(((long)jj * k) % N);
         const double x = dv[lp * N + jj];
         sr += x * twr[ph];
         si -= x * twi[ph];
+        ph += k;
+        if (ph >= N) ph -= N;
       }
       double o64 = (pos == 0) ? sr / N : ((pos & 1) ? 2.0 * sr / N : -2.0 * si / N);
       S64[(size_t)pos + 5 * l] = o64;
@@ -490,7 +495,7 @@ __global__ __launch_bounds__(256) void k_dctz(TW *__restrict__ S, long nl, int n
 // poisson_solve.F90:270-323) with precomputed inverse pivots ib; lanes over
 // the packed mode index (contiguous), P waves of M rows each.
 template <typename T, int M>
-__global__ __launch_bounds__(1024) void k_radial(T *__restrict__ S, const T *__restrict__ ib, long nm, long nlines,
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_per_eu(1, 4))) void k_radial(T *__restrict__ S, const T *__restrict__ ib, long nm, long nlines,
                                                  int nr, const double *__restrict__ aab, const double *__restrict__ ccb) {
   __shared__ T sh_v[16][64];
   __shared__ T sh_p[16][64];
@@ -630,7 +635,8 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
       const double *t = S64 + 5 * l;
       v += t[0];
       for (int kk = 1; kk <= 2 && 2 * kk - 1 < N; ++kk) {
-        const int ph = (int)(((long)ii * kk) % N);
+        int ph = ii * kk;
+        if (ph >= N) ph -= N;
         const double a = t[2 * kk - 1], b = (2 * kk < N) ? t[2 * kk] : 0.0;
         if (2 * kk == N) v += 0.5 * a * twr[ph];
         else v += a * twr[ph] + b * twi[ph];
@@ -754,45 +760,38 @@ template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, double c
   (void)c3s;
   const VGrid &g = h->g;
   dim3 blk(64), grd((g.nt + 63) / 64, g.nrc);
+  KTimer kt(h, KT_HDNBC);
   hipLaunchKernelGGL(k_hdnbc, grd, blk, 0, h->stream, g, h->coef, h->rk[irk].g, h->rk[irk].r, cor, h->f.q3,
                      h->f.csort, h->f.dq3no, h->f.corr3n);
 }
 
 template <typename TW> void launch_explicit(vrh_t *h, int irk) {
   const VGrid &g = h->g;
-  int bx = g.nt >= 64 ? 64 : g.nt;
-  int by = 256 / bx;
-  dim3 blk(bx, by), grd((g.nt + bx - 1) / bx, (g.nr + by - 1) / by, g.nz);
-  hipLaunchKernelGGL(k_explicit<TW>, grd, blk, 0, h->stream, g, h->coef, h->rk[irk], h->f.q1, h->f.q2, h->f.q3,
-                     (const TW *)h->f.prg1, (const TW *)h->f.prg2, (const TW *)h->f.prg3, (TW *)h->f.ru1,
-                     (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1, (TW *)h->f.r2, (TW *)h->f.r3);
-}
-
-template <typename TW> void launch_theta(vrh_t *h, int irk) {
-  const VGrid &g = h->g;
-  int nth = 1;
-  while (nth < g.nt) nth <<= 1;
-  if (nth < 64) nth = 64;
+  int ntp = 1;
+  while (ntp < g.nt) ntp <<= 1;
+  if (ntp < 64) ntp = 64;
+  int rj = 256 / ntp;
+  if (rj < 1) rj = 1;
   const double betadx = 0.5 * h->rk[irk].a * h->dt / h->Re;
-  size_t sh = sizeof(TW) * nth;
-  // q1: j=1..nr, k=1..nz; q3: j=1..nr, k=2..nz; q2: j=2..nr, k=1..nz (step.F90:225-236)
-  hipLaunchKernelGGL(k_theta<TW>, dim3(g.nr, g.nz), dim3(nth), sh, h->stream, g, h->coef.s1t11, betadx, 1, 1,
-                     (TW *)h->f.r1);
-  if (g.nz > 1)
-    hipLaunchKernelGGL(k_theta<TW>, dim3(g.nr, g.nz - 1), dim3(nth), sh, h->stream, g, h->coef.s1t11, betadx, 1, 2,
-                       (TW *)h->f.r3);
-  if (g.nr > 1)
-    hipLaunchKernelGGL(k_theta<TW>, dim3(g.nr - 1, g.nz), dim3(nth), sh, h->stream, g, h->coef.s2t11, betadx, 2, 1,
-                       (TW *)h->f.r2);
+  const int nlev = h->theta_levels[irk];
+  dim3 blk(ntp, rj), grd((g.nr + rj - 1) / rj, g.nz);
+  size_t sh = sizeof(TW) * 3 * (size_t)rj * ntp;
+  KTimer kt(h, KT_EXPLICIT);
+  hipLaunchKernelGGL(k_explicit<TW>, grd, blk, sh, h->stream, g, h->coef, h->rk[irk], betadx, nlev, h->f.q1,
+                     h->f.q2, h->f.q3, (const TW *)h->f.prg1, (const TW *)h->f.prg2, (const TW *)h->f.prg3,
+                     (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1, (TW *)h->f.r2, (TW *)h->f.r3);
 }
 
 template <typename TW, int MODE>
 static void line_launch(vrh_t *h, const LineArgs &a, int M) {
   const int P = a.lc.P;
   dim3 blk(64 * P), grd((unsigned)((a.nlines + 63) / 64));
+  KTimer kt(h, MODE <= SW_R2 ? KT_LINE_R : KT_LINE_Z);
   switch (M) {
     case 16: hipLaunchKernelGGL((k_line<TW, 16, MODE>), grd, blk, 0, h->stream, a); break;
+    case 24: hipLaunchKernelGGL((k_line<TW, 24, MODE>), grd, blk, 0, h->stream, a); break;
     case 32: hipLaunchKernelGGL((k_line<TW, 32, MODE>), grd, blk, 0, h->stream, a); break;
+    case 40: hipLaunchKernelGGL((k_line<TW, 40, MODE>), grd, blk, 0, h->stream, a); break;
     case 48: hipLaunchKernelGGL((k_line<TW, 48, MODE>), grd, blk, 0, h->stream, a); break;
     default: hipLaunchKernelGGL((k_line<TW, 64, MODE>), grd, blk, 0, h->stream, a); break;
   }
@@ -808,19 +807,19 @@ template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s) {
   a.corr3n = h->f.corr3n;
   // q1: r then z (step.F90:75-78)
   a.lc = h->lc[irk][SW_R1]; a.nlines = (long)g.nt * g.nz; a.r = h->f.r1;
-  line_launch<TW, SW_R1>(h, a, h->Mr);
+  line_launch<TW, SW_R1>(h, a, h->Msw[SW_R1]);
   a.lc = h->lc[irk][SW_Z1]; a.nlines = (long)g.nt * g.nr; a.j0 = 1; a.q = h->f.q1; a.dq = h->f.dq1;
-  line_launch<TW, SW_Z1>(h, a, h->Mz);
+  line_launch<TW, SW_Z1>(h, a, h->Msw[SW_Z1]);
   // q3: r then z (step.F90:124-127)
   a.lc = h->lc[irk][SW_R3]; a.nlines = (long)g.nt * g.nz; a.r = h->f.r3;
-  line_launch<TW, SW_R3>(h, a, h->Mr);
+  line_launch<TW, SW_R3>(h, a, h->Msw[SW_R3]);
   a.lc = h->lc[irk][SW_Z3]; a.nlines = (long)g.nt * g.nr; a.j0 = 1; a.q = h->f.q3; a.dq = h->f.dq3;
-  line_launch<TW, SW_Z3>(h, a, h->Mz);
+  line_launch<TW, SW_Z3>(h, a, h->Msw[SW_Z3]);
   // q2: z then r (step.F90:171-177)
   a.lc = h->lc[irk][SW_Z2]; a.nlines = (long)g.nt * (g.nr - 1); a.j0 = 2; a.r = h->f.r2;
-  line_launch<TW, SW_Z2>(h, a, h->Mz);
+  line_launch<TW, SW_Z2>(h, a, h->Msw[SW_Z2]);
   a.lc = h->lc[irk][SW_R2]; a.nlines = (long)g.nt * g.nzc; a.q = h->f.q2; a.dq = h->f.dq2;
-  line_launch<TW, SW_R2>(h, a, h->Mr);
+  line_launch<TW, SW_R2>(h, a, h->Msw[SW_R2]);
 }
 
 static int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
@@ -832,6 +831,7 @@ static void radial_launch(vrh_t *h, T *S, const T *ib, long nm) {
   const int P = (g.nr + M - 1) / M;
   const long nlines = nm * g.nz;
   dim3 blk(64 * P), grd((unsigned)((nlines + 63) / 64));
+  KTimer kt(h, KT_RADIAL);
   hipLaunchKernelGGL((k_radial<T, 32>), grd, blk, 0, h->stream, S, ib, nm, nlines, g.nr, h->d_aab, h->d_ccb);
 }
 
@@ -839,7 +839,7 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   const VGrid &g = h->g;
   const int N = g.nt, logn = ilog2(N);
   const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
-  int LB = pow2 ? 1024 / N : 2;
+  int LB = pow2 ? (h->fft_lines > 0 ? h->fft_lines : 1024 / N) : 2;
   if (LB > 16) LB = 16;
   if (LB < 2) LB = 2;
   const long nlines = (long)g.nr * g.nz;
@@ -847,12 +847,14 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   double *S64 = (h->prec == VRH_MIXED) ? h->f.S64 : nullptr;
   const double adt = h->rk[irk].a * h->dt;
   size_t sh1 = sizeof(double) * LB * N + sizeof(TW) * LB * N;
+  { KTimer kt(h, KT_DIVFFT);
   hipLaunchKernelGGL(k_divfft<TW>, dim3((unsigned)((nlines + LB - 1) / LB)), dim3(256), sh1, h->stream, g, h->coef,
-                     1.0 / adt, h->f.dq1, h->f.dq2, h->f.dq3, ext_rhs, S, S64, h->d_twt, h->d_twt + N, LB, logn);
+                     1.0 / adt, h->f.dq1, h->f.dq2, h->f.dq3, ext_rhs, S, S64, h->d_twt, h->d_twt + N, LB, logn); }
   // z transforms
   const int nz = g.nz, lz = ilog2(nz);
   int TL = (int)(65536 / (sizeof(TW) * nz));  // at most 64 KB of LDS per block
   if (TL > 16) TL = 16;
+  if (h->dct_lines > 0 && h->dct_lines < TL) TL = h->dct_lines;
   TL &= ~1;
   if (TL < 2) TL = 2;
   int TL64 = (int)(65536 / (sizeof(double) * nz));
@@ -861,8 +863,9 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   if (TL64 < 2) TL64 = 2;
   const long nl = (long)N * g.nr;
   size_t sh2 = sizeof(TW) * TL * nz;
+  { KTimer kt(h, KT_DCT_FWD);
   hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(256), sh2, h->stream, S, nl, nz, lz, TL,
-                     1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
+                     1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
   if (S64) {
     const long nl64 = 5L * g.nr;
     hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(256),
@@ -871,8 +874,9 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   }
   radial_launch<TW>(h, S, (const TW *)h->f.ib, N);
   if (S64) radial_launch<double>(h, S64, h->f.ib64, 5);
+  { KTimer kt(h, KT_DCT_INV);
   hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(256), sh2, h->stream, S, nl, nz, lz, TL,
-                     -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
+                     -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
   if (S64) {
     const long nl64 = 5L * g.nr;
     hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(256),
@@ -880,6 +884,7 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
                        h->d_ez, h->d_ez + nz);
   }
   size_t sh3 = sizeof(TW) * 2 * LB * N;
+  KTimer kti(h, KT_IFFT);
   hipLaunchKernelGGL(k_ifft<TW>, dim3((unsigned)((nlines + LB - 1) / LB)), dim3(256), sh3, h->stream, g, S, S64,
                      phi_out, ext_rhs ? nullptr : h->f.dph, h->d_twt, h->d_twt + N, LB, logn);
 }
@@ -891,6 +896,7 @@ template <typename TW> void launch_project(vrh_t *h, int irk) {
   int bx = g.nt >= 64 ? 64 : g.nt;
   int by = 256 / bx;
   dim3 blk(bx, by), grd((g.nt + bx - 1) / bx, (g.nr + 2 + by - 1) / by, g.nzc);
+  KTimer kt(h, KT_PROJECT);
   hipLaunchKernelGGL(k_project<TW>, grd, blk, 0, h->stream, g, h->coef, adt, be, h->f.dph, h->f.dq1, h->f.dq2,
                      h->f.dq3, h->f.q1, h->f.q2, h->f.q3, (TW *)h->f.prg1, (TW *)h->f.prg2, (TW *)h->f.prg3,
                      h->f.pres, h->f.flag);
@@ -899,7 +905,6 @@ template <typename TW> void launch_project(vrh_t *h, int irk) {
 #define INST(TW)                                                   \
   template void launch_hdnbc<TW>(vrh_t *, int, double, double);    \
   template void launch_explicit<TW>(vrh_t *, int);                 \
-  template void launch_theta<TW>(vrh_t *, int);                    \
   template void launch_sweeps<TW>(vrh_t *, int, double);           \
   template void launch_poisson<TW>(vrh_t *, int, const double *, double *); \
   template void launch_project<TW>(vrh_t *, int);

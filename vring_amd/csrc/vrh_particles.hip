@@ -1,16 +1,430 @@
-// vrh_particles.hip -- Lagrangian particle step (part_tools.F90:374-1025).
+// vrh_particles.hip -- Lagrangian particle step (iter_part, part_tools.F90:374-1025).
+//
+// One thread per particle.  Particle state lives in HBM as structure of
+// arrays, u[v * cap + p] (v = 0..nvar-1), so every lane of a wavefront reads
+// consecutive addresses for each state variable.  The 27-point gathers hit
+// the L2-resident velocity fields.  T is the arithmetic type of the
+// interpolation and right-hand side (double in FP64 mode, float in mixed
+// mode); the state update is accumulated in double.
 #include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "vrh_internal.h"
 
-void launch_particles(vrh_t *h, int irk) { (void)h; (void)irk; }
+namespace {
+
+struct PartArgs {
+  VGrid g;
+  const double *q1, *q2, *q3;
+  const double *mint;  // 9 x 27
+  double *u, *ddt2;
+  int *alive;
+  long cap, n;
+  int nvar, nsub, scheme, irk;
+  double st, partdt, gam, rom, dtheta, dr, dz, rmax, zmax;
+};
+
+template <typename T> struct V3 { T x[3]; };
+
+// Interpolation and chain rule of one particle, part_tools.F90:519-797.
+// Returns false if the 27-point stencil would leave the haloed arrays.
+template <typename T>
+__device__ bool interp_lsq(const PartArgs &a, double xp, double yp, double zp, T vel[3], T dvel[9], T ddvel[27]) {
+  const VGrid &g = a.g;
+  const double r = sqrt(xp * xp + yp * yp);
+  const double pi = (double)4.0f * atan(1.0);
+  double theta;
+  if (xp >= 0.0 && yp >= 0.0) theta = acos(xp / r);           // part_tools.F90:524-534
+  else if (xp < 0.0 && yp >= 0.0) theta = acos(xp / r);
+  else theta = acos(-xp / r) + pi;
+  int it = (int)floor(theta / a.dtheta) + 1;
+  if (g.nt == 1) it = 1;
+  const int ir = (int)floor(r / a.dr) + 1;
+  const int iz = (int)floor(zp / a.dz) + 1;
+  if (!(r == r) || ir < 1 || ir > g.nr + 1 || iz < 1 || iz > g.nz + 1) return false;
+  T ff[3], dff[3][3], ddff[3][3][3];
+  const T sn = (T)sin(theta), cs = (T)cos(theta);
+  const T sn2 = (T)sin(2.0 * theta), cs2 = (T)cos(2.0 * theta);
+  const T rr = (T)r, r2 = rr * rr;
+  const double dx[3] = {a.dtheta, a.dr, a.dz};
+  for (int ivar = 1; ivar <= 3; ivar++) {
+    T y[27];
+    for (int c = -1; c <= 1; c++)
+      for (int b = -1; b <= 1; b++)
+        for (int aa = -1; aa <= 1; aa++) {
+          const int iv = (aa + 1) + 3 * (b + 1) + 9 * (c + 1);
+          int itof = it + aa;
+          if (itof > g.nt) itof = 1;
+          if (itof < 1) itof = g.nt;
+          const int irof = ir + b, izof = iz + c;
+          const long o = hidx(g, itof, irof, izof);
+          double v;
+          if (ivar == 1) v = a.q1[o];
+          else if (ivar == 2) {
+            const double rc = (irof - 1.0) * a.dr;
+            v = (rc > (double)0.0001f) ? a.q2[o] / rc : 0.0;
+          } else v = a.q3[o];
+          y[iv] = (T)v;
+        }
+    const T y0 = y[13];
+    for (int iv = 0; iv < 27; iv++) y[iv] = y[iv] - y0;
+    // least-squares fit, iterated as part_tools.F90:609-624 (converges in
+    // one pass since mint is the exact pseudo-inverse; the second pass adds
+    // the rounding-level correction)
+    T acoef[9];
+    for (int q = 0; q < 9; q++) acoef[q] = (T)0;
+    for (int iint = 1; iint <= 100; iint++) {
+      T mp[9];
+      T nrm = (T)0;
+      for (int q = 0; q < 9; q++) mp[q] = (T)0;
+      for (int kk = 0; kk < 27; kk++) {
+        const int a1 = kk % 3 - 1, b1 = (kk / 3) % 3 - 1, c1 = kk / 9 - 1;
+        const T x0 = (T)(a1 * dx[0]), x1 = (T)(b1 * dx[1]), x2 = (T)(c1 * dx[2]);
+        const T rint = x0 * acoef[0] + x1 * acoef[1] + x2 * acoef[2] + x0 * x0 * acoef[3] + x0 * x1 * acoef[4] +
+                       x1 * x1 * acoef[5] + x1 * x2 * acoef[6] + x2 * x2 * acoef[7] + x2 * x0 * acoef[8];
+        const T res = rint - y[kk];
+        for (int q = 0; q < 9; q++) mp[q] = mp[q] - (T)a.mint[q * 27 + kk] * res;
+      }
+      for (int q = 0; q < 9; q++) { acoef[q] = acoef[q] + mp[q]; nrm += mp[q] * mp[q]; }
+      if (sqrt((double)nrm) < (double)0.000001f) break;
+    }
+    double dtp, drp, dzp;
+    if (ivar == 1) { dtp = theta - (it - 1.0) * a.dtheta; drp = r - (ir - 0.5) * a.dr; dzp = zp - (iz - 0.5) * a.dz; }
+    else if (ivar == 2) { dtp = theta - (it - 0.5) * a.dtheta; drp = r - (ir - 1.0) * a.dr; dzp = zp - (iz - 0.5) * a.dz; }
+    else { dtp = theta - (it - 0.5) * a.dtheta; drp = r - (ir - 0.5) * a.dr; dzp = zp - (iz - 1.0) * a.dz; }
+    const double twopi = 8.0 * atan(1.0);
+    if (dtp > a.dtheta) dtp -= twopi;
+    if (dtp < -a.dtheta) dtp += twopi;
+    const T t_ = (T)dtp, r_ = (T)drp, z_ = (T)dzp;
+    const T *A = acoef;
+    const T f = t_ * A[0] + r_ * A[1] + z_ * A[2] + t_ * t_ * A[3] + t_ * r_ * A[4] + r_ * r_ * A[5] + r_ * z_ * A[6] +
+                z_ * z_ * A[7] + z_ * t_ * A[8] + y0;
+    T FI[12];
+    FI[0] = A[0] + (T)2 * t_ * A[3] + r_ * A[4] + z_ * A[8];
+    FI[1] = A[1] + t_ * A[4] + (T)2 * r_ * A[5] + z_ * A[6];
+    FI[2] = A[2] + r_ * A[6] + (T)2 * z_ * A[7] + t_ * A[8];
+    FI[3] = (T)2 * A[3];
+    FI[4] = r_ * A[4];   // ftr as part_tools.F90:657
+    FI[5] = z_ * A[8];   // ftz as part_tools.F90:658
+    FI[6] = A[4]; FI[7] = (T)2 * A[5]; FI[8] = A[6];
+    FI[9] = A[8]; FI[10] = A[6]; FI[11] = (T)2 * A[7];
+    ff[ivar - 1] = f;
+    // chain rule to Cartesian derivatives (:671-698)
+    dff[ivar - 1][0] = -sn / rr * FI[0] + cs * FI[1];
+    dff[ivar - 1][1] = cs / rr * FI[0] + sn * FI[1];
+    dff[ivar - 1][2] = FI[2];
+    // second derivatives (id1,id2) = rows 4..12
+    const T xx = sn2 / r2 * FI[0] + sn * sn / rr * FI[1] + sn * sn / r2 * FI[3] - cs * sn / rr * FI[4] -
+                 cs * sn / rr * FI[6] + cs * cs * FI[7];
+    const T yx = -cs2 / r2 * FI[0] - cs * sn / rr * FI[1] - cs * sn / r2 * FI[3] - sn * sn / rr * FI[4] +
+                 cs * cs / rr * FI[6] + cs * sn * FI[7];
+    const T zx = -sn / rr * FI[5] + cs * FI[8];
+    const T xy = -cs2 / r2 * FI[0] - cs * sn / rr * FI[1] - cs * sn / r2 * FI[3] + cs * cs / rr * FI[4] -
+                 sn * sn / rr * FI[6] + cs * sn * FI[7];
+    const T yy = -sn2 / r2 * FI[0] + cs * cs / rr * FI[1] + cs * cs / r2 * FI[3] + cs * sn / rr * FI[4] +
+                 cs * sn / rr * FI[6] + sn * sn * FI[7];
+    const T zy = cs / rr * FI[5] + sn * FI[8];
+    const T xz = -sn / rr * FI[9] + cs * FI[10];
+    const T yz = cs / rr * FI[9] + sn * FI[10];
+    const T zz = FI[11];
+    ddff[ivar - 1][0][0] = xx; ddff[ivar - 1][1][0] = yx; ddff[ivar - 1][2][0] = zx;
+    ddff[ivar - 1][0][1] = xy; ddff[ivar - 1][1][1] = yy; ddff[ivar - 1][2][1] = zy;
+    ddff[ivar - 1][0][2] = xz; ddff[ivar - 1][1][2] = yz; ddff[ivar - 1][2][2] = zz;
+  }
+  // rotation to Cartesian velocity, gradient and Hessian (:700-797)
+  const T rs = (T)fmax((double)0.000001f, r), rs2 = rs * rs;
+  T NM[3][3] = {{-sn, cs, (T)0}, {cs, sn, (T)0}, {(T)0, (T)0, (T)1}};
+  T DN[3][3][3];  // DN[j][i][k] = d NM(i,k) / d x_j
+  for (int q = 0; q < 27; q++) (&DN[0][0][0])[q] = (T)0;
+  DN[0][0][0] = cs * sn / rs; DN[0][0][1] = sn * sn / rs;
+  DN[0][1][0] = sn * sn / rs; DN[0][1][1] = -cs * sn / rs;
+  DN[1][0][0] = -cs * cs / rs; DN[1][0][1] = -sn * cs / rs;
+  DN[1][1][0] = -sn * cs / rs; DN[1][1][1] = cs * cs / rs;
+  const T Aa = sn * (-(T)2 * (cs * cs) + sn * sn) / rs2;
+  const T Bb = cs * ((cs * cs) - (T)2 * (sn * sn)) / rs2;
+  const T Cc = -(T)3 * cs * (sn * sn) / rs2;
+  const T Dd = (T)3 * (cs * cs) * sn / rs2;
+  // DDN[l][j][i][k] = d2 NM(i,k) / dx_l dx_j; only i,k in {0,1} and l,j in {0,1}
+  auto DDN = [&](int l, int j, int i, int k) -> T {
+    if (l > 1 || j > 1 || i > 1 || k > 1) return (T)0;
+    const int key = l * 8 + j * 4 + i * 2 + k;
+    switch (key) {
+      case 0: return Aa; case 1: return Cc; case 2: return Cc; case 3: return -Aa;
+      case 4: return Bb; case 5: return -Aa; case 6: return -Aa; case 7: return -Bb;
+      case 8: return Bb; case 9: return -Aa; case 10: return -Aa; case 11: return -Bb;
+      case 12: return Dd; case 13: return -Bb; case 14: return -Bb; case 15: return -Dd;
+    }
+    return (T)0;
+  };
+  for (int id = 0; id < 3; id++) {
+    T s = (T)0;
+    for (int jd = 0; jd < 3; jd++) s = s + ff[jd] * NM[id][jd];
+    vel[id] = s;
+  }
+  for (int id = 0; id < 3; id++)
+    for (int jd = 0; jd < 3; jd++) {
+      T s = (T)0;
+      for (int kd = 0; kd < 3; kd++) s = s + dff[kd][jd] * NM[id][kd] + ff[kd] * DN[jd][id][kd];
+      dvel[id + 3 * jd] = s;
+    }
+  for (int id = 0; id < 3; id++)
+    for (int jd = 0; jd < 3; jd++)
+      for (int ld = 0; ld < 3; ld++) {
+        T s = (T)0;
+        for (int kd = 0; kd < 3; kd++)
+          s = s + ddff[kd][jd][ld] * NM[id][kd] + dff[kd][jd] * DN[ld][id][kd] + dff[kd][ld] * DN[jd][id][kd] +
+              ff[kd] * DDN(ld, jd, id, kd);
+        ddvel[id + 3 * jd + 9 * ld] = s;
+      }
+  return true;
+}
+
+// DDNMAT(l,j,i,k) of part_tools.F90:725-760 with 0-based l,j,i,k encoded as
+// key = 8l + 4j + 2i + k.
+
+template <typename T>
+__global__ __launch_bounds__(128) void k_part_lsq(PartArgs a) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n || !a.alive[p]) return;
+  const long cap = a.cap;
+  double *u = a.u + p;      // u[v*cap]
+  double *d2 = a.ddt2 ? a.ddt2 + p : nullptr;
+#define U(v) u[(long)(v) * cap]
+  for (int isub = 0; isub < a.nsub; isub++) {
+    T vel[3], dvel[9], ddvel[27];
+    if (!interp_lsq<T>(a, U(0), U(1), U(2), vel, dvel, ddvel)) {
+      a.alive[p] = 0;
+      for (int v = 0; v < a.nvar; v++) U(v) = 0.0;
+      return;
+    }
+    // rates (part_tools.F90:836-870); jac/hes rows are read from u
+    double rate[78];
+    const double inv = (double)1.0f / a.st;
+    for (int i = 0; i < 3; i++) {
+      rate[i] = U(3 + i);
+      rate[3 + i] = inv * ((double)vel[i] - U(3 + i));
+    }
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        rate[6 + i + 3 * j] = U(15 + i + 3 * j);
+        const double sumj = U(6 + 0 + 3 * j) * (double)dvel[i + 0] + U(6 + 1 + 3 * j) * (double)dvel[i + 3] +
+                            U(6 + 2 + 3 * j) * (double)dvel[i + 6];
+        rate[15 + i + 3 * j] = inv * (sumj - U(15 + i + 3 * j));
+      }
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        for (int k = 0; k < 3; k++) {
+          const int ix = i + 3 * j + 9 * k;
+          rate[24 + ix] = U(51 + ix);
+          double sumh = 0.0;
+          for (int l = 0; l < 3; l++) sumh += U(24 + l + 3 * j + 9 * k) * (double)dvel[i + 3 * l];
+          for (int m = 0; m < 3; m++)
+            for (int l = 0; l < 3; l++)
+              sumh += U(6 + l + 3 * j) * U(6 + m + 3 * k) * (double)ddvel[i + 3 * l + 9 * m];
+          rate[51 + ix] = inv * (sumh - U(51 + ix));
+        }
+    if (a.scheme == VRH_PART_LSQ_EULER) {          // part_tools.F90:926-945
+      for (int v = 0; v < 78; v++) U(v) = U(v) + rate[v] * a.partdt;
+      for (int i = 0; i < 3; i++) U(78 + i) = vel[i];
+      for (int i = 0; i < 9; i++) U(81 + i) = dvel[i];
+      for (int i = 0; i < 27; i++) U(90 + i) = ddvel[i];
+    } else {                                          // dertest/part_tools.F90:902-906
+      for (int v = 0; v < 78; v++) {
+        U(v) = U(v) + a.gam * rate[v] * a.partdt + a.rom * d2[(long)v * cap] * a.partdt;
+        d2[(long)v * cap] = rate[v];
+      }
+    }
+    // removal of particles leaving the domain (part_tools.F90:962-1025)
+    const double x = U(0), y = U(1), z = U(2), r = sqrt(x * x + y * y);
+    if (z > a.zmax || z < 0.0 || r > a.rmax || r < 0.0 || !(r == r)) {
+      a.alive[p] = 0;
+      for (int v = 0; v < a.nvar; v++) U(v) = 0.0;
+      if (d2) for (int v = 0; v < a.nvar; v++) d2[(long)v * cap] = 0.0;
+      return;
+    }
+  }
+#undef U
+}
+
+// ---- trilinear option: (u_theta, u_r, u_z) from the staggered grids ----
+template <typename T>
+__device__ T tri_comp(const PartArgs &a, int ivar, double theta, double r, double z) {
+  const VGrid &g = a.g;
+  const double th0 = (ivar == 1) ? 0.0 : 0.5 * a.dtheta;
+  const double r0 = (ivar == 2) ? 0.0 : 0.5 * a.dr;
+  const double z0 = (ivar == 3) ? 0.0 : 0.5 * a.dz;
+  const double st = (theta - th0) / a.dtheta, sr = (r - r0) / a.dr, sz = (z - z0) / a.dz;
+  const double ft = floor(st), fr = floor(sr), fz = floor(sz);
+  double wt = st - ft, wr = sr - fr, wz = sz - fz;
+  const int i0 = (int)ft;
+  int j0 = (int)fr + 1, k0 = (int)fz + 1;
+  if (j0 < 0) { j0 = 0; wr = 0.0; }
+  if (j0 > g.nr) { j0 = g.nr; wr = 1.0; }
+  if (k0 < 0) { k0 = 0; wz = 0.0; }
+  if (k0 > g.nz + 1) { k0 = g.nz + 1; wz = 1.0; }
+  T s = (T)0;
+  for (int c = 0; c <= 1; c++)
+    for (int b = 0; b <= 1; b++)
+      for (int aa = 0; aa <= 1; aa++) {
+        int ii = (i0 + aa) % g.nt;
+        if (ii < 0) ii += g.nt;
+        ii += 1;
+        const int jj = j0 + b;
+        int kk = k0 + c;
+        if (kk > g.nz + 2) kk = g.nz + 2;
+        const double w = (aa ? wt : 1.0 - wt) * (b ? wr : 1.0 - wr) * (c ? wz : 1.0 - wz);
+        const long o = hidx(g, ii, jj, kk);
+        double v;
+        if (ivar == 1) v = a.q1[o];
+        else if (ivar == 2) { const double rc = (jj - 1.0) * a.dr; v = (rc > 1e-12) ? a.q2[o] / rc : 0.0; }
+        else v = a.q3[o];
+        s += (T)(w * v);
+      }
+  return s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_part_tri(PartArgs a) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n || !a.alive[p]) return;
+  const long cap = a.cap;
+  double *s = a.u + p, *r2 = a.ddt2 + p;
+#define S(v) s[(long)(v) * cap]
+#define R(v) r2[(long)(v) * cap]
+  const double x = S(0), y = S(1), z = S(2), r = sqrt(x * x + y * y);
+  double theta = atan2(y, x);
+  if (theta < 0) theta += 2.0 * M_PI;
+  const double ut = (double)tri_comp<T>(a, 1, theta, r, z), ur = (double)tri_comp<T>(a, 2, theta, r, z),
+               uz = (double)tri_comp<T>(a, 3, theta, r, z);
+  const double sn = (r > 0) ? y / r : 0.0, cs = (r > 0) ? x / r : 1.0;
+  const double u[3] = {-sn * ut + cs * ur, cs * ut + sn * ur, uz};
+  double f[6];
+  for (int i = 0; i < 3; i++) {
+    if (a.st > 0.0) { f[i] = S(3 + i); f[3 + i] = (u[i] - S(3 + i)) / a.st; }
+    else { f[i] = u[i]; f[3 + i] = 0.0; }
+  }
+  for (int i = 0; i < 6; i++) {
+    S(i) = S(i) + a.gam * a.partdt * f[i] + a.rom * a.partdt * R(i);
+    R(i) = f[i];
+  }
+  if (a.st <= 0.0) for (int i = 0; i < 3; i++) S(3 + i) = u[i];
+  const double rn = sqrt(S(0) * S(0) + S(1) * S(1));
+  if (S(2) > a.zmax || S(2) < 0.0 || rn > a.rmax) a.alive[p] = 0;
+#undef S
+#undef R
+}
+
+}  // namespace
+
+void launch_particles(vrh_t *h, int irk) {
+  if (h->nbatch <= 0) return;
+  const bool every = h->pscheme != VRH_PART_LSQ_EULER;
+  if (!every && irk != 1) return;  // loop.F90:67 calls iter_part at irk==1 only
+  KTimer kt(h, KT_PARTICLES);
+  for (auto &b : h->pb) {
+    if (b.n <= 0) continue;
+    PartArgs a{};
+    a.g = h->g;
+    a.q1 = h->f.q1; a.q2 = h->f.q2; a.q3 = h->f.q3;
+    a.mint = h->d_mint;
+    a.u = b.u; a.ddt2 = b.ddt2; a.alive = b.alive;
+    a.cap = b.cap; a.n = b.n;
+    a.nvar = h->pscheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
+    a.nsub = b.nsub; a.scheme = h->pscheme; a.irk = irk;
+    a.st = b.st;
+    a.partdt = h->dt / (double)(float)b.nsub;  // partdt = dt/float(nsubdt), part_tools.F90:505
+    a.gam = h->rk[irk].g; a.rom = h->rk[irk].r;
+    a.dtheta = 2.0 * acos(-1.0) / h->nt; a.dr = h->rmax / h->nr; a.dz = h->zmax / h->nz;
+    a.rmax = h->rmax; a.zmax = h->zmax;
+    if (h->pscheme == VRH_PART_TRILINEAR_RK3) {
+      a.partdt = h->dt;
+      a.nsub = 1;
+      dim3 blk(256), grd((unsigned)((b.n + 255) / 256));
+      if (h->prec == VRH_MIXED) hipLaunchKernelGGL(k_part_tri<float>, grd, blk, 0, h->stream, a);
+      else hipLaunchKernelGGL(k_part_tri<double>, grd, blk, 0, h->stream, a);
+    } else {
+      dim3 blk(128), grd((unsigned)((b.n + 127) / 128));
+      if (h->prec == VRH_MIXED) hipLaunchKernelGGL(k_part_lsq<float>, grd, blk, 0, h->stream, a);
+      else hipLaunchKernelGGL(k_part_lsq<double>, grd, blk, 0, h->stream, a);
+    }
+  }
+}
 
 extern "C" {
-int vrh_particles_create(vrh_t *h, int, const long *, const double *, const int *, int) {
-  if (h) h->err = "particles: not implemented yet";
-  return -1;
+
+int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st, const int *nsub, int scheme) {
+  if (!h || nbatch < 1 || !cap || !st || !nsub) return -1;
+  if (scheme < VRH_PART_LSQ_EULER || scheme > VRH_PART_TRILINEAR_RK3) { h->err = "unknown particle scheme"; return -1; }
+  if (!h->pb.empty()) { h->err = "particles already created"; return -1; }
+  const int nvar = scheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
+  h->pscheme = scheme;
+  h->pb.resize(nbatch);
+  for (int b = 0; b < nbatch; b++) {
+    PartBatch &B = h->pb[b];
+    B.cap = cap[b]; B.n = 0; B.st = st[b]; B.nsub = nsub[b] < 1 ? 1 : nsub[b];
+    if (B.cap < 1 || (scheme != VRH_PART_TRILINEAR_RK3 && !(B.st > 0.0))) { h->err = "bad particle batch"; return -1; }
+    if (hipMalloc(&B.u, sizeof(double) * nvar * B.cap) != hipSuccess ||
+        hipMalloc(&B.alive, sizeof(int) * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
+    hipMemsetAsync(B.u, 0, sizeof(double) * nvar * B.cap, h->stream);
+    hipMemsetAsync(B.alive, 0, sizeof(int) * B.cap, h->stream);
+    B.ddt2 = nullptr;
+    if (scheme != VRH_PART_LSQ_EULER) {
+      if (hipMalloc(&B.ddt2, sizeof(double) * nvar * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
+      hipMemsetAsync(B.ddt2, 0, sizeof(double) * nvar * B.cap, h->stream);
+    }
+  }
+  h->nbatch = nbatch;
+  return hipStreamSynchronize(h->stream) == hipSuccess ? 0 : -3;
 }
-int vrh_particles_upload(vrh_t *h, int, long, const double *, const int *) { return h ? -1 : -1; }
-int vrh_particles_download(vrh_t *h, int, long, double *, int *) { return h ? -1 : -1; }
-int vrh_particles_step(vrh_t *h, int) { return h ? -1 : -1; }
+
+int vrh_particles_upload(vrh_t *h, int batch, long n, const double *u, const int *alive) {
+  if (!h || batch < 0 || batch >= h->nbatch || !u || n < 0) return -1;
+  PartBatch &B = h->pb[batch];
+  if (n > B.cap) { h->err = "more particles than the batch capacity"; return -1; }
+  const int nvar = h->pscheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
+  std::vector<double> soa((size_t)nvar * B.cap, 0.0);
+  for (long p = 0; p < n; p++)
+    for (int v = 0; v < nvar; v++) soa[(size_t)v * B.cap + p] = u[(size_t)p * nvar + v];
+  std::vector<int> al(B.cap, 0);
+  for (long p = 0; p < n; p++) al[p] = alive ? alive[p] : 1;
+  if (hipMemcpy(B.u, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(B.alive, al.data(), al.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    h->err = "particle upload failed";
+    return -3;
+  }
+  if (B.ddt2) hipMemset(B.ddt2, 0, sizeof(double) * nvar * B.cap);
+  B.n = n;
+  return 0;
 }
+
+int vrh_particles_download(vrh_t *h, int batch, long n, double *u, int *alive) {
+  if (!h || batch < 0 || batch >= h->nbatch || !u || n < 0) return -1;
+  PartBatch &B = h->pb[batch];
+  if (n > B.cap) n = B.cap;
+  const int nvar = h->pscheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
+  std::vector<double> soa((size_t)nvar * B.cap);
+  std::vector<int> al(B.cap);
+  if (hipStreamSynchronize(h->stream) != hipSuccess ||
+      hipMemcpy(soa.data(), B.u, soa.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(al.data(), B.alive, al.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+    h->err = "particle download failed";
+    return -3;
+  }
+  for (long p = 0; p < n; p++)
+    for (int v = 0; v < nvar; v++) u[(size_t)p * nvar + v] = soa[(size_t)v * B.cap + p];
+  if (alive)
+    for (long p = 0; p < n; p++) alive[p] = al[p];
+  return 0;
+}
+
+int vrh_particles_step(vrh_t *h, int irk) {
+  if (!h || irk < 1 || irk > 3) return -1;
+  if (h->nbatch <= 0) { h->err = "no particles"; return -1; }
+  launch_particles(h, irk);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+}  // extern "C"

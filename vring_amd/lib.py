@@ -33,7 +33,7 @@ EXPORTS = ["vrh_create", "vrh_destroy", "vrh_last_error", "vrh_upload_fields", "
            "vrh_upload_state", "vrh_download_state", "vrh_set_time", "vrh_get_time", "vrh_substep",
            "vrh_step", "vrh_phase", "vrh_poisson_solve", "vrh_synchronize", "vrh_device_ptr",
            "vrh_ring_init", "vrh_cfl_dt", "vrh_particles_create", "vrh_particles_upload",
-           "vrh_particles_download", "vrh_particles_step"]
+           "vrh_particles_download", "vrh_particles_step", "vrh_ktimer_enable", "vrh_ktimer_read", "vrh_autotune"]
 
 _lib = None
 
@@ -70,6 +70,10 @@ def load_library(path=LIBPATH):
     L.vrh_particles_upload.argtypes = [P, I, ctypes.c_long, dp, ip]
     L.vrh_particles_download.argtypes = [P, I, ctypes.c_long, dp, ip]
     L.vrh_particles_step.argtypes = [P, I]
+    L.vrh_autotune.argtypes = [P, I, ip]
+    L.vrh_ktimer_enable.argtypes = [P, I]
+    L.vrh_ktimer_read.argtypes = [P, I, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(D),
+                                  ctypes.POINTER(ctypes.c_long)]
     _lib = L
     return L
 
@@ -105,6 +109,25 @@ def cfl_dt(ntheta, nr, nz, rmax, zmax, q1, q2, q3, cfl=0.3, ujet=1.0):
     if rc:
         raise RuntimeError("vrh_cfl_dt failed")
     return out.value
+
+
+def torus_points(n, R, z0, a, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = np.zeros((0, 3))
+    while out.shape[0] < n:
+        m = 2 * (n - out.shape[0]) + 16
+        p = rng.uniform([-(R + a), -(R + a), z0 - a], [R + a, R + a, z0 + a], size=(m, 3))
+        rr = np.sqrt(p[:, 0] ** 2 + p[:, 1] ** 2)
+        ok = (rr - R) ** 2 + (p[:, 2] - z0) ** 2 <= a * a
+        out = np.concatenate([out, p[ok]])
+    return out[:n]
+
+
+def stokes_number(radius, Re):
+    """st = (1/18)(998/1.2) r^2 Re with the reference's default-real literals
+    (part_tools.F90:273)."""
+    f = np.float32
+    return float(np.float64(f(f(1.0) / f(18.0)) * f(f(998.0) / f(1.2))) * radius ** 2 * Re)
 
 
 class Solver:
@@ -188,6 +211,68 @@ class Solver:
         out = np.zeros_like(r)
         self._check(self.L.vrh_poisson_solve(self.h, r, out), "poisson")
         return out.reshape(self.comp_shape, order="F")
+
+    def kernel_times(self):
+        """{kernel: (total_ms, launches)} accumulated since enable_ktimer(True)."""
+        out = {}
+        n = self.L.vrh_ktimer_read(self.h, -1, None, 0, None, None)
+        for k in range(n):
+            name = ctypes.create_string_buffer(64)
+            ms, cnt = ctypes.c_double(0), ctypes.c_long(0)
+            self._check(self.L.vrh_ktimer_read(self.h, k, name, 64, ctypes.byref(ms), ctypes.byref(cnt)),
+                        "ktimer_read")
+            if cnt.value:
+                out[name.value.decode()] = (ms.value, cnt.value)
+        return out
+
+    def enable_ktimer(self, on=True):
+        self._check(self.L.vrh_ktimer_enable(self.h, 1 if on else 0), "ktimer_enable")
+
+    def autotune(self, reps=3):
+        """Times the candidate line-solver segment lengths and FFT/DCT tile
+        sizes on the GPU and keeps the fastest; returns the choice."""
+        ch = np.zeros(8, np.int32)
+        self._check(self.L.vrh_autotune(self.h, int(reps), ch), "autotune")
+        keys = ["M_r1", "M_r3", "M_r2", "M_z1", "M_z3", "M_z2", "fft_lines", "dct_lines"]
+        return dict(zip(keys, ch.tolist()))
+
+    # ---- particles (part_tools.F90) ----
+    def particles_create(self, caps, st, nsub, scheme="lsq_euler"):
+        self.pscheme = scheme
+        self.nvar = 6 if scheme == "trilinear_rk3" else 117
+        rc = self.L.vrh_particles_create(self.h, len(caps), np.ascontiguousarray(caps, dtype=np.int64),
+                                         np.ascontiguousarray(st, dtype=np.float64),
+                                         np.ascontiguousarray(nsub, dtype=np.int32), PART_SCHEME[scheme])
+        self._check(rc, "particles_create")
+
+    def particles_upload(self, batch, u, alive=None):
+        u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1, self.nvar))
+        n = u.shape[0]
+        al = np.ones(n, np.int32) if alive is None else np.ascontiguousarray(alive, dtype=np.int32)
+        self._check(self.L.vrh_particles_upload(self.h, batch, n, u.ravel(), al), "particles_upload")
+
+    def particles_download(self, batch, n):
+        u = np.zeros(n * self.nvar)
+        al = np.zeros(n, np.int32)
+        self._check(self.L.vrh_particles_download(self.h, batch, n, u, al), "particles_download")
+        return u.reshape(n, self.nvar), al
+
+    def particles_step(self, irk):
+        self._check(self.L.vrh_particles_step(self.h, irk), "particles_step")
+
+    def seed_particles_torus(self, n, R=1.0, z0=3.0, a=0.6, seed=0, st=None, nsub=1, scheme="lsq_euler"):
+        """n particles uniform in the torus (sqrt(x^2+y^2)-R)^2+(z-z0)^2 <= a^2
+        (rejection sampling, numpy PCG64(seed)), at rest, identity Jacobian."""
+        xyz = torus_points(n, R, z0, a, seed)
+        if st is None:
+            st = stokes_number(5.0e-6 / 0.02, self.Re)
+        self.particles_create([n], [st], [nsub], scheme)
+        u = np.zeros((n, self.nvar))
+        u[:, :3] = xyz
+        if self.nvar == 117:
+            u[:, [6, 10, 14]] = 1.0
+        self.particles_upload(0, u)
+        return xyz
 
     def device_ptr(self, name):
         p = ctypes.c_void_p()
