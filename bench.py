@@ -7,7 +7,8 @@ with seeded core-radius perturbations of amplitude 1e-3 on modes m=2..8
 (phases from numpy PCG64(0)), dt at CFL 0.3, inputs resident in HBM.
 A step is one full RK3 step (3 substeps: boundary terms, fused explicit
 stencils, ADI sweeps, FFT/DCT/tridiagonal pressure solve, projection and
-pressure update), plus the Lagrangian particle step when particles are on.
+pressure update).  The Lagrangian particle step is timed separately on the
+same field and reported as particles/s.
 
 `value` is the mixed-precision path (FP32 stencils, sweeps and transforms;
 FP64 for the m=0..2 pressure modes, the predictor, the projection and the
@@ -63,14 +64,12 @@ def ring_state(nt=NT, nr=NR, nz=NZ):
     return q, dt
 
 
-def run_path(prec, q, dt, steps, warmup, particles=0):
+def run_path(prec, q, dt, steps, warmup):
+    """Flow solver only: K timed RK3 steps after W warm-up steps."""
     import vring_amd
     s = vring_amd.Solver(NT, NR, NZ, RMAX, ZMAX, RE, dt, precision=prec)
     for n, a in zip(("q1", "q2", "q3"), q):
         s.upload(n, a)
-    if particles:
-        # 5 um droplets (batch 4 of part_tools.F90:262-269), nsubdt = 5
-        s.seed_particles_torus(particles, R=1.0, z0=3.0, a=0.6, seed=0, nsub=5)
     tuned = s.autotune(reps=3)
     s.step(warmup)
     s.synchronize()
@@ -86,6 +85,28 @@ def run_path(prec, q, dt, steps, warmup, particles=0):
     s.enable_ktimer(False)
     s.close()
     return wall, fields, kt, tuned
+
+
+def run_particles(prec, q, dt, n, steps):
+    """Particle step on the workload's field: n particles (5 um droplets,
+    batch 4 of part_tools.F90:262-269, nsubdt=5) uniform in the ring's torus;
+    timed over `steps` calls (one per RK3 step, as loop.F90:67)."""
+    import vring_amd
+    s = vring_amd.Solver(NT, NR, NZ, RMAX, ZMAX, RE, dt, precision=prec)
+    for name, a in zip(("q1", "q2", "q3"), q):
+        s.upload(name, a)
+    s.seed_particles_torus(n, R=1.0, z0=3.0, a=0.6, seed=0, nsub=5)
+    s.particles_step(1)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.particles_step(1)
+    s.synchronize()
+    t = time.perf_counter() - t0
+    s.close()
+    return {"particles": n, "substeps_per_step": 5, "ms_per_step": round(t / steps * 1e3, 3),
+            "particles_per_s": round(n * steps / t, 1), "particle_substeps_per_s": round(5 * n * steps / t, 1),
+            "hbm_bytes_per_particle_substep": 2520}
 
 
 def rel_errors(a, b):
@@ -144,8 +165,12 @@ def main():
         sys.exit(2)
     q, dt = ring_state()
     ncell = NT * NR * NZ
-    wall64, f64, kt64, tune64 = run_path("fp64", q, dt, args.steps, args.warmup, args.particles)
-    wallmx, fmx, ktmx, tunemx = run_path("mixed", q, dt, args.steps, args.warmup, args.particles)
+    wall64, f64, kt64, tune64 = run_path("fp64", q, dt, args.steps, args.warmup)
+    wallmx, fmx, ktmx, tunemx = run_path("mixed", q, dt, args.steps, args.warmup)
+    parts = None
+    if args.particles:
+        parts = {"fp64": run_particles("fp64", q, dt, args.particles, max(2, args.steps // 4)),
+                 "mixed": run_particles("mixed", q, dt, args.particles, max(2, args.steps // 4))}
     v64 = ncell * args.steps / wall64 / 1e6
     vmx = ncell * args.steps / wallmx / 1e6
     err = rel_errors(fmx, f64)
@@ -174,7 +199,7 @@ def main():
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ktmx.items()},
         "fp64_kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt64.items()},
         "autotune": {"fp64": tune64, "mixed": tunemx},
-        "particles_per_s": (round(args.particles * args.steps / wallmx, 1) if args.particles else None),
+        "particles": parts,
         "roofline": rf,
     }
     if not args.no_cpu_baseline:

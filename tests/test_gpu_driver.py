@@ -1,0 +1,105 @@
+"""The Fortran driver end to end on the GPU: inp.txt with a Gaussian ring,
+10 RK3 steps, restart files in the reference's format (run.F90:390-466),
+compared with the FP64 oracle run from the same initial condition."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+from scipy.io import FortranFile
+
+from oracle_lib import Oracle
+
+import vring_amd
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "driver", "vring_driver")
+
+DECK = """===Comm===
+dim                  3
+npth                 1     1    1
+
+===Mesh===
+nx                  {nt}    {nr}   {nz}
+maxr                 4.0
+maxz                12.0
+
+===Flag===
+poisson              fftp
+
+===Para===
+dt                   0.01
+totstp               {totstp}
+expstp               5
+anlstp               5
+restrt               {restrt}
+verbose              0
+u                    0.0
+v                    0.0
+w                    0.0
+w                    0.0   0.0  0.0 16.0
+rw.r0.rh             1.0   0.5  5.0
+p                    1.0
+Re                1000.0
+Pe                   1.0
+nrkstep              3
+slip                 0.0
+
+===Ring===
+nring 1
+ring 1.0 3.0 0.2 1.0
+pert 0.0 0 -1
+cfl 0.3 1.0
+"""
+
+
+def read_restart(path, nt, nr, nz):
+    f = FortranFile(path, "r")
+    hdr = f.read_record(np.dtype([("n", "<i4"), ("t", "<f8"), ("a", "<i4"), ("b", "<i4"), ("c", "<i4")]))
+    q = np.zeros((3, nt, nr, nz))
+    for iz in range(nz):
+        for ir in range(nr):
+            for ith in range(nt):
+                q[:, ith, ir, iz] = f.read_reals("<f8")
+    f.close()
+    return int(hdr["n"][0]), float(hdr["t"][0]), q
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+def test_driver_ring_run_matches_oracle(tmp_path):
+    nt, nr, nz = 33, 32, 64
+    print("case driver ring 33x32x64", flush=True)
+    (tmp_path / "inp.txt").write_text(DECK.format(nt=nt, nr=nr, nz=nz, totstp=9, restrt=0))
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    for name in ["out.txt", "analysis.dat", "restart00005RK03.dat", "restart00009RK03.dat",
+                 "2Dfld00005RK03.dat", "3Dfld00005RK03PR00.dat"]:
+        assert (tmp_path / name).exists(), name
+    nstep, ntime, qd = read_restart(tmp_path / "restart00009RK03.dat", nt, nr, nz)
+    q1, q2, q3 = vring_amd.ring_initial_condition(nt, nr, nz, 4.0, 12.0, [(1.0, 3.0, 0.2, 1.0)])
+    dt = vring_amd.cfl_dt(nt, nr, nz, 4.0, 12.0, q1, q2, q3, cfl=0.3, ujet=1.0)
+    assert nstep == 9 and abs(ntime - 10 * dt) < 1e-12
+    o = Oracle(nt, nr, nz, 4.0, 12.0, 1000.0, dt)
+    for n, a in (("q1", q1), ("q2", q2), ("q3", q3)):
+        o.field(n)[...] = a
+    for _ in range(10):
+        assert o.rk3_step() == 0
+    ref = np.stack([o.field(n)[:, 1 : nr + 1, 1 : nz + 1] for n in ("q1", "q2", "q3")])
+    assert np.abs(qd - ref).max() <= 1e-10 * np.abs(ref).max()
+    # restart: load the step-5 file as RESTART and take one step
+    shutil.copy(tmp_path / "restart00005RK03.dat", tmp_path / "RESTART")
+    (tmp_path / "inp.txt").write_text(DECK.format(nt=nt, nr=nr, nz=nz, totstp=0, restrt=1))
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    n5, t5, q5 = read_restart(tmp_path / "RESTART", nt, nr, nz)
+    n6, t6, q6 = read_restart(tmp_path / "restart00005RK03.dat", nt, nr, nz)
+    o2 = Oracle(nt, nr, nz, 4.0, 12.0, 1000.0, dt)
+    for c, n in enumerate(("q1", "q2", "q3")):
+        o2.field(n)[:, 1 : nr + 1, 1 : nz + 1] = q5[c]
+    o2.set_time(t5)
+    assert o2.rk3_step() == 0
+    ref2 = np.stack([o2.field(n)[:, 1 : nr + 1, 1 : nz + 1] for n in ("q1", "q2", "q3")])
+    assert abs(t6 - (t5 + dt)) < 1e-12
+    assert np.abs(q6 - ref2).max() <= 1e-10 * np.abs(ref2).max()

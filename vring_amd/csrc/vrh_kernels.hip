@@ -390,18 +390,38 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
     else outv = -2.0 * Fi / N;
     (void)nhalf;
     S[(size_t)pos + (size_t)N * l] = (TW)outv;
-    if (S64 && pos < 5) {
-      // exact double DFT of the lowest modes
-      double sr = 0.0, si = 0.0;
-      int ph = 0;
-      for (int jj = 0; jj < N; ++jj) {
-        const double x = dv[lp * N + jj];
-        sr += x * twr[ph];
-        si -= x * twi[ph];
+  }
+  if (S64) {
+    // FP64 DFT of the lowest modes m=0,1,2 from the FP64 divergence: 8 fixed
+    // chunks per (line, coefficient), partial sums in LDS, summed in a fixed
+    // order (deterministic)
+    double *tw = reinterpret_cast<double *>(im + (size_t)(LB / 2) * N);  // cos (N) then sin (N)
+    double *part = tw + 2 * (size_t)N;             // LB * 5 * 8
+    for (int q = tid; q < N; q += nth) { tw[q] = twr[q]; tw[N + q] = twi[q]; }
+    __syncthreads();
+    const int nch = 8, chl = (N + nch - 1) / nch;
+    for (int e = tid; e < LB * 5 * nch; e += nth) {
+      const int lp = e / (5 * nch), rem = e - lp * 5 * nch, pos = rem / nch, ch = rem - pos * nch;
+      const int k = (pos + 1) >> 1;
+      const bool im = pos > 0 && (pos & 1) == 0;
+      double acc = 0.0;
+      const int j0 = ch * chl, j1 = min(N, j0 + chl);
+      int ph = (int)(((long)j0 * k) % N);
+      for (int jj = j0; jj < j1; ++jj) {
+        acc += dv[lp * N + jj] * (im ? tw[N + ph] : tw[ph]);
         ph += k;
         if (ph >= N) ph -= N;
       }
-      double o64 = (pos == 0) ? sr / N : ((pos & 1) ? 2.0 * sr / N : -2.0 * si / N);
+      part[e] = acc;
+    }
+    __syncthreads();
+    for (int e = tid; e < LB * 5; e += nth) {
+      const int lp = e / 5, pos = e - lp * 5;
+      const long l = l0 + lp;
+      if (l >= nlines || pos >= N) continue;
+      double acc = 0.0;
+      for (int ch = 0; ch < nch; ++ch) acc += part[e * nch + ch];
+      const double o64 = (pos == 0) ? acc / N : 2.0 * acc / N;  // t(1), t(2k), t(2k+1) packing
       S64[(size_t)pos + 5 * l] = o64;
     }
   }
@@ -847,6 +867,7 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   double *S64 = (h->prec == VRH_MIXED) ? h->f.S64 : nullptr;
   const double adt = h->rk[irk].a * h->dt;
   size_t sh1 = sizeof(double) * LB * N + sizeof(TW) * LB * N;
+  if (S64) sh1 += sizeof(double) * (2 * (size_t)N + (size_t)LB * 5 * 8);
   { KTimer kt(h, KT_DIVFFT);
   hipLaunchKernelGGL(k_divfft<TW>, dim3((unsigned)((nlines + LB - 1) / LB)), dim3(256), sh1, h->stream, g, h->coef,
                      1.0 / adt, h->f.dq1, h->f.dq2, h->f.dq3, ext_rhs, S, S64, h->d_twt, h->d_twt + N, LB, logn); }
