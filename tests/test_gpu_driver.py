@@ -1,5 +1,5 @@
 """The Fortran driver end to end on the GPU: inp.txt with a Gaussian ring,
-10 RK3 steps, restart files in the reference's format (run.F90:390-466),
+11 RK3 steps (istep = 0..10), restart files in the reference's format (run.F90:390-466),
 compared with the FP64 oracle run from the same initial condition."""
 import os
 import shutil
@@ -71,20 +71,21 @@ def read_restart(path, nt, nr, nz):
 def test_driver_ring_run_matches_oracle(tmp_path):
     nt, nr, nz = 33, 32, 64
     print("case driver ring 33x32x64", flush=True)
-    (tmp_path / "inp.txt").write_text(DECK.format(nt=nt, nr=nr, nz=nz, totstp=9, restrt=0))
+    (tmp_path / "inp.txt").write_text(DECK.format(nt=nt, nr=nr, nz=nz, totstp=10, restrt=0))
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    for name in ["out.txt", "analysis.dat", "restart00005RK03.dat", "restart00009RK03.dat",
+    # exports after nstep = 0, 5, 10 (mod(nstep, expstp) == 0, loop.F90:136)
+    for name in ["out.txt", "analysis.dat", "restart00005RK03.dat", "restart00010RK03.dat",
                  "2Dfld00005RK03.dat", "3Dfld00005RK03PR00.dat"]:
         assert (tmp_path / name).exists(), name
-    nstep, ntime, qd = read_restart(tmp_path / "restart00009RK03.dat", nt, nr, nz)
+    nstep, ntime, qd = read_restart(tmp_path / "restart00010RK03.dat", nt, nr, nz)
     q1, q2, q3 = vring_amd.ring_initial_condition(nt, nr, nz, 4.0, 12.0, [(1.0, 3.0, 0.2, 1.0)])
     dt = vring_amd.cfl_dt(nt, nr, nz, 4.0, 12.0, q1, q2, q3, cfl=0.3, ujet=1.0)
-    assert nstep == 9 and abs(ntime - 10 * dt) < 1e-12
+    assert nstep == 10 and abs(ntime - 11 * dt) < 1e-12
     o = Oracle(nt, nr, nz, 4.0, 12.0, 1000.0, dt)
     for n, a in (("q1", q1), ("q2", q2), ("q3", q3)):
         o.field(n)[...] = a
-    for _ in range(10):
+    for _ in range(11):
         assert o.rk3_step() == 0
     ref = np.stack([o.field(n)[:, 1 : nr + 1, 1 : nz + 1] for n in ("q1", "q2", "q3")])
     assert np.abs(qd - ref).max() <= 1e-10 * np.abs(ref).max()
