@@ -64,13 +64,13 @@ def ring_state(nt=NT, nr=NR, nz=NZ):
     return q, dt
 
 
-def run_path(prec, q, dt, steps, warmup):
+def run_path(prec, q, dt, steps, warmup, tune_reps=3):
     """Flow solver only: K timed RK3 steps after W warm-up steps."""
     import vring_amd
     s = vring_amd.Solver(NT, NR, NZ, RMAX, ZMAX, RE, dt, precision=prec)
     for n, a in zip(("q1", "q2", "q3"), q):
         s.upload(n, a)
-    tuned = s.autotune(reps=3)
+    tuned = s.autotune(reps=tune_reps)
     s.step(warmup)
     s.synchronize()
     t0 = time.perf_counter()
@@ -126,10 +126,10 @@ def roofline(kt, prec, ncell):
                 "unit": "GB/s", "frac": None, "traffic": None}
     ach = per_cell * ncell / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_{name}_{prec}_256x128x512.json")
+    tf = os.path.join(ROOT, "profiles", "traffic_256x128x512.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            traffic = json.load(f).get(prec, {}).get(name)
     return {"kernel": name, "avg_ms": round(avg_ms, 4), "bound": "hbm", "achieved": round(ach, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 3), "traffic": traffic,
             "alg_bytes_per_cell": per_cell}
@@ -152,6 +152,21 @@ def cpu_baseline(q, dt):
             "kind": "port", "sample": f"one RK3 substep at 256x128x512 took {t:.1f} s (x3 per step)"}
 
 
+def copy_calibration(mib, reps=3):
+    """Plain device-to-device copy of `mib` MiB (torch elementwise copy
+    kernel): reads and writes exactly mib MiB per launch.  Profiling it with
+    the FETCH_SIZE / WRITE_SIZE counters gives the scale factor between the
+    counters and real HBM bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.rand(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,14 +174,29 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--particles", type=int, default=1000000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--autotune-reps", type=int, default=3)
+    ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
+                    help="run one path only and print its timing (profiling runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run only a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     q, dt = ring_state()
     ncell = NT * NR * NZ
-    wall64, f64, kt64, tune64 = run_path("fp64", q, dt, args.steps, args.warmup)
-    wallmx, fmx, ktmx, tunemx = run_path("mixed", q, dt, args.steps, args.warmup)
+    if args.only:
+        wall, _, kt, tune = run_path(args.only, q, dt, args.steps, args.warmup, args.autotune_reps)
+        print(json.dumps({"path": args.only, "ms_per_step": round(wall / args.steps * 1e3, 4),
+                          "Mcell_RK3_steps_per_s": round(ncell * args.steps / wall / 1e6, 2),
+                          "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+                          "autotune": tune}))
+        return
+    wall64, f64, kt64, tune64 = run_path("fp64", q, dt, args.steps, args.warmup, args.autotune_reps)
+    wallmx, fmx, ktmx, tunemx = run_path("mixed", q, dt, args.steps, args.warmup, args.autotune_reps)
     parts = None
     if args.particles:
         parts = {"fp64": run_particles("fp64", q, dt, args.particles, max(2, args.steps // 4)),

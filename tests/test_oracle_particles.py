@@ -107,3 +107,70 @@ def test_particles_leaving_domain_are_removed():
     d1, d2, alive = np.zeros(117), np.zeros(117), np.ones(1, np.int32)
     o.L.orc_part_advance(o.h, 1, 0, 1, 1.0, 1e-3, 117, 1, u, d1, d2, alive)
     assert alive[0] == 0 and np.all(u == 0)
+
+
+def _linear_fields(o, nt, nr, nz, rmax, zmax, cf):
+    """q1 = u_theta, q2 = rc * u_r, q3 = u_z with each velocity component
+    linear in (theta, r, z) at its own staggered nodes (mesh_tools.F90:217-350:
+    q1 at (thc, rm, zm), q2 at (thm, rc, zm), q3 at (thm, rm, zc))."""
+    dth, dr, dz = 2 * np.pi / nt, rmax / nr, zmax / nz
+    i = np.arange(1, nt + 1)[:, None, None]
+    j = np.arange(nr + 3)[None, :, None]
+    k = np.arange(nz + 3)[None, None, :]
+    thc, thm = (i - 1) * dth, (i - 0.5) * dth
+    rc, rm = (j - 1) * dr, (j - 0.5) * dr
+    zc, zm = (k - 1) * dz, (k - 0.5) * dz
+    lin = lambda c, th, r, z: c[0] + c[1] * th + c[2] * r + c[3] * z
+    o.field("q1")[...] = lin(cf[0], thc, rm, zm)
+    o.field("q2")[...] = rc * lin(cf[1], thm, rc, zm)
+    o.field("q3")[...] = lin(cf[2], thm, rm, zc)
+    return lin
+
+
+def test_trilinear_exact_for_linear_fields():
+    """Trilinear interpolation reproduces fields linear in (theta, r, z) at
+    interior points (away from the theta wrap and the axis row where the
+    oracle sets u_r = q2/rc to 0), and rotates (u_theta, u_r) to Cartesian."""
+    nt, nr, nz, rmax, zmax = 16, 20, 24, 2.0, 3.0
+    o = Oracle(nt, nr, nz, rmax, zmax, 1000.0, 1e-3)
+    cf = [(0.3, 0.05, -0.4, 0.7), (-0.2, 0.11, 0.9, -0.3), (1.1, -0.07, 0.25, 0.5)]
+    lin = _linear_fields(o, nt, nr, nz, rmax, zmax, cf)
+    rng = np.random.Generator(np.random.PCG64(3))
+    dth, dr, dz = 2 * np.pi / nt, rmax / nr, zmax / nz
+    for _ in range(200):
+        th = rng.uniform(dth, 2 * np.pi - 2 * dth)
+        r = rng.uniform(1.5 * dr, rmax - 1.5 * dr)
+        z = rng.uniform(1.5 * dz, zmax - 1.5 * dz)
+        x = np.array([r * np.cos(th), r * np.sin(th), z])
+        vel = np.zeros(3)
+        o.L.orc_part_trilinear(o.h, x, vel)
+        ut, ur, uz = (lin(c, th, r, z) for c in cf)
+        want = [-np.sin(th) * ut + np.cos(th) * ur, np.cos(th) * ut + np.sin(th) * ur, uz]
+        assert np.max(np.abs(vel - want)) < 1e-12, (th, r, z, vel, want)
+
+
+def test_trilinear_rk3_is_cubic_taylor_map_of_affine_ode():
+    """Uniform flow u: x' = v, v' = (u - v)/st is affine, so one RK3 step
+    (three substeps with gamma, rho of para_tools.F90:103-113) equals the
+    cubic Taylor polynomial of the exact flow:  w = v - u, lam = -1/st,
+    w1 = w0 (1 + h lam + (h lam)^2/2 + (h lam)^3/6),
+    x1 = x0 + h u + w0 (h + lam h^2/2 + lam^2 h^3/6)."""
+    nt, nr, nz, rmax, zmax = 16, 20, 24, 2.0, 3.0
+    o = Oracle(nt, nr, nz, rmax, zmax, 1000.0, 1e-3)
+    u_z = 0.8
+    _linear_fields(o, nt, nr, nz, rmax, zmax, [(0, 0, 0, 0), (0, 0, 0, 0), (u_z, 0, 0, 0)])
+    st, h = 0.03, 0.01
+    x0 = np.array([0.4, -0.3, 1.2])
+    v0 = np.array([0.2, 0.1, -0.5])
+    s = np.concatenate([x0, v0])
+    r2, alive = np.zeros(6), np.ones(1, np.int32)
+    for irk in (1, 2, 3):
+        o.L.orc_part_rk3_trilinear(o.h, 1, st, irk, h, s, r2, alive)
+    u = np.array([0.0, 0.0, u_z])
+    lam, w0 = -1.0 / st, v0 - u
+    z = h * lam
+    w1 = w0 * (1 + z + z * z / 2 + z ** 3 / 6)
+    x1 = x0 + h * u + w0 * (h + lam * h * h / 2 + lam * lam * h ** 3 / 6)
+    assert alive[0] == 1
+    assert np.max(np.abs(s[3:] - (u + w1))) < 1e-14
+    assert np.max(np.abs(s[:3] - x1)) < 1e-14
