@@ -167,6 +167,58 @@ def copy_calibration(mib, reps=3):
     print(json.dumps({"copy_calibration_bytes": n * 4, "reps": reps}))
 
 
+KERNEL_KEYS = {  # rocprof kernel name prefix -> kernel timer name
+    "k_explicit_cell": "explicit", "k_explicit_march": "explicit", "k_explicit": "explicit",
+    "k_project": "project", "k_divfft": "divfft", "k_dctz": "dct", "k_radial": "radial", "k_ifft": "ifft",
+    "k_hdnbc": "hdnbc", "k_part_lsq": "particles", "k_part_tri": "particles", "k_scalars": "scalars",
+}
+
+
+def _kernel_key(name):
+    """('explicit', 'fp64') from a demangled rocprof kernel name."""
+    base = name.split("(anonymous namespace)::", 1)[-1]
+    fn = base.split("<")[0]
+    targs = base[len(fn) + 1:].split(">")[0] if "<" in base else ""
+    prec = "fp64" if targs.startswith("double") else ("mixed" if targs.startswith("float") else "")
+    if fn == "k_line":
+        mode = int(targs.split(",")[2])
+        key = "line_r" if mode in (0, 1, 2) else "line_z"
+    else:
+        key = KERNEL_KEYS.get(fn, fn)
+    return key, prec
+
+
+def summarize_traffic(pmc_root, out_path):
+    """HBM bytes per launch of each kernel from rocprofv3 --pmc runs
+    (pmc_<prec>_FETCH_SIZE, pmc_<prec>_WRITE_SIZE), each counter scaled by
+    its reading on the 1 GiB plain copy (pmc_copy_*)."""
+    import csv
+
+    def rows(d):
+        with open(os.path.join(pmc_root, d, "p_counter_collection.csv")) as f:
+            return list(csv.DictReader(f))
+
+    copy = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        vals = [float(r["Counter_Value"]) for r in rows(f"pmc_copy_{c}") if "copyBuffer" in r["Kernel_Name"]]
+        copy[c] = (1 << 30) / 1024.0 / (sum(vals) / len(vals))  # true KiB per counted KiB
+    out = {"unit": "bytes per launch", "scale": {"FETCH_SIZE": copy["FETCH_SIZE"], "WRITE_SIZE": copy["WRITE_SIZE"]},
+           "calibration": "1 GiB device copy: true bytes / counter bytes"}
+    for prec in ("fp64", "mixed"):
+        acc = {}
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            for r in rows(f"pmc_{prec}_{c}"):
+                key, kp = _kernel_key(r["Kernel_Name"])
+                if kp != ("fp64" if prec == "fp64" else "mixed"):
+                    continue
+                acc.setdefault(key, {}).setdefault(c, []).append(float(r["Counter_Value"]) * copy[c] * 1024.0)
+        out[prec] = {k: round(sum(v["FETCH_SIZE"]) / len(v["FETCH_SIZE"]) + sum(v["WRITE_SIZE"]) / len(v["WRITE_SIZE"]))
+                     for k, v in acc.items() if "FETCH_SIZE" in v and "WRITE_SIZE" in v}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,7 +231,12 @@ def main():
                     help="run one path only and print its timing (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--summarize-traffic", metavar="PMC_DIR", default=None,
+                    help="write profiles/traffic_256x128x512.json from rocprofv3 --pmc output under PMC_DIR")
     args = ap.parse_args()
+    if args.summarize_traffic:
+        summarize_traffic(args.summarize_traffic, os.path.join(ROOT, "profiles", "traffic_256x128x512.json"))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)

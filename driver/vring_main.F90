@@ -14,7 +14,7 @@ program vring_main
   real(c_double), allocatable, target :: q1(:, :, :), q2(:, :, :), q3(:, :, :), pres(:, :, :)
   real(c_double), allocatable, target :: pu(:, :), phs(:)
   integer(c_int), allocatable, target :: palive(:)
-  integer(c_int), target :: choice(8)
+  integer(c_int), target :: choice(9)
   real(c_double) :: ntime, stat(20), dtcfl, st
   real(c_double) :: t0, t1, t2, t3
   integer :: nt, nr, nz, istep, nstep, nstepi, rc, np, ip
@@ -57,7 +57,9 @@ program vring_main
                        c_loc(q2), c_loc(q3))
     if (rc /= 0) stop 'vrh_ring_init failed'
   end if
-  if (d%cfl > 0.d0) then
+  ! a CFL number fixes dt from the initial ring; a restart keeps the deck's dt
+  ! (the reference reads dt from inp.txt, para_tools.F90:68-86)
+  if (d%cfl > 0.d0 .and. d%restrt /= 1) then
     rc = vrh_cfl_dt(grid, c_loc(q1), c_loc(q2), c_loc(q3), d%cfl, d%ujet, dtcfl)
     d%dt = dtcfl
     write(*, '(a,es16.8)') 'dt from CFL: ', d%dt
@@ -97,7 +99,7 @@ program vring_main
   end if
   if (d%autotune /= 0) then
     call vrh_check(h, vrh_autotune(h, 3, c_loc(choice)), 'autotune')
-    write(*, '(a,8i5)') 'autotune choice: ', choice
+    write(*, '(a,9i5)') 'autotune choice: ', choice
   end if
   stat = 0.d0
   call cpu_time(t0)

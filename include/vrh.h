@@ -78,6 +78,11 @@ int vrh_get_time(vrh_t *h, double *ntime);
 int vrh_substep(vrh_t *h, int irk);
 /* nsteps full steps: ntime += dt, then substeps 1,2,3 (loop.F90:53-131). */
 int vrh_step(vrh_t *h, int nsteps);
+/* vrh_step replays one RK3 step captured as a HIP graph (captured on first
+ * use, recaptured after vrh_autotune or a particle change; not used while the
+ * kernel timer is on).  on = 0 launches the kernels one by one instead.  No
+ * reference counterpart. */
+int vrh_set_graph(vrh_t *h, int on);
 /* One phase of substep irk (parity tests and profiling). */
 int vrh_phase(vrh_t *h, int phase, int irk);
 /* Standalone Poisson solve, host rhs -> host phi, (ntheta,nr,nz) each
@@ -93,11 +98,15 @@ int vrh_synchronize(vrh_t *h);
 int vrh_ktimer_enable(vrh_t *h, int on);
 int vrh_ktimer_read(vrh_t *h, int idx, char *name, size_t len, double *total_ms, long *count);
 /* Autotuner: times every candidate segment length of the six partitioned
- * line sweeps and the FFT/DCT tile sizes with HIP events on the live
- * buffers and keeps the fastest.  Call between steps.  choice[8] (optional)
+ * line sweeps, the FFT/DCT tile sizes and the two forms of the explicit
+ * kernel (one z plane per block, or z marching) with HIP events on the live
+ * buffers and keeps the fastest.  Call between steps.  choice[9] (optional)
  * receives M for sweeps r1 r3 r2 z1 z3 z2, theta FFT lines/block, z DCT
- * lines/block. */
+ * lines/block and the explicit-kernel form (0 plane, 1 marching). */
 int vrh_autotune(vrh_t *h, int reps, int *choice);
+/* Apply a tuning returned by vrh_autotune (9 entries; 0 keeps a segment
+ * length or tile size, -1 selects the default explicit form). */
+int vrh_set_tuning(vrh_t *h, const int *choice);
 /* Device address of a named field (for callers that keep data in HBM). */
 int vrh_device_ptr(vrh_t *h, const char *name, void **ptr);
 
@@ -113,6 +122,19 @@ int vrh_ring_init(const vrh_grid *grid, int nrings, const double *rings, double 
  * jet speed floor `ujet`). */
 int vrh_cfl_dt(const vrh_grid *grid, const double *q1, const double *q2, const double *q3,
                double cfl, double ujet, double *dt);
+
+/* ---- Ensembles of independent rings ----
+ * No reference counterpart (the reference runs one ring per MPI job).  The
+ * members (handles of any grid size, Re, precision and particle load) are
+ * advanced together: one RK3 step of every member is one HIP graph launch,
+ * the members' kernels running concurrently.  Members must not be used
+ * elsewhere during vrh_ensemble_step; their times advance as with vrh_step.
+ * The ensemble keeps the handle pointers; destroy it before its members. */
+typedef struct vrh_ens_s vrh_ens_t;
+int vrh_ensemble_create(vrh_t *const *members, int n, vrh_ens_t **out);
+int vrh_ensemble_step(vrh_ens_t *e, int nsteps);
+int vrh_ensemble_last_error(vrh_ens_t *e, char *buf, size_t len);
+int vrh_ensemble_destroy(vrh_ens_t *e);
 
 /* ---- Lagrangian particles (part_tools.F90) ---- */
 enum { VRH_PART_LSQ_EULER = 0, VRH_PART_LSQ_RK3 = 1, VRH_PART_TRILINEAR_RK3 = 2 };

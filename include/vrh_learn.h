@@ -1,0 +1,91 @@
+/* vrh_learn.h -- C ABI of the per-Fourier-mode (r, z) Helmholtz subsystem of
+ * libvrh: the pressure operator of one azimuthal mode on a uniform r grid and
+ * a tanh-stretched z grid, its direct solve, plain Krylov solvers, and a
+ * learned warm start / preconditioner (a 3x3 convolutional network with
+ * BF16 MFMA inference and training kernels, FP32 accumulation).
+ *
+ * No reference counterpart: the reference solves its pressure equation only
+ * on a uniform z grid by FFT/DCT and radial tridiagonal solves
+ * (poisson_solve.F90:140-179, 270-323).  The operator here is that equation's
+ * (r, z) part for one mode m in symmetric finite-volume form,
+ *     S_m p = V f,
+ * with the radial fluxes of poisson.F90:368-388, the exact theta eigenvalue
+ * m^2/r^2 ('wvnex', poisson_solve.F90:286-289) and zero-flux axis, wall and
+ * z ends; V is the cell volume r dr dz.  For m = 0 the volume-weighted mean of
+ * f is removed and the solution has zero volume-weighted mean.
+ *
+ * A handle holds `nslots` independent systems (slots), each with its own m.
+ * Host arrays are C order (nslots, nz, nr), r fastest.  Every call returns 0
+ * or a negative code; vrh_hz_last_error gives the message.
+ */
+#ifndef VRH_LEARN_H
+#define VRH_LEARN_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vrh_hz_s vrh_hz_t;
+
+typedef struct {
+  int nr, nz, nslots;      /* nr a multiple of 16 */
+  double rmax, zmax;
+  double stretch;          /* z_k = zmax (1 + tanh(s (k/nz - 1)) / tanh s); 0: uniform */
+} vrh_hz_grid;
+
+enum { VRH_HZ_PCG = 0, VRH_HZ_BICGSTAB = 1, VRH_HZ_LEARNED = 2 };
+
+int vrh_hz_create(const vrh_hz_grid *grid, vrh_hz_t **out);
+int vrh_hz_destroy(vrh_hz_t *h);
+int vrh_hz_last_error(vrh_hz_t *h, char *buf, size_t len);
+/* Azimuthal mode of each slot (nslots values, 0..). */
+int vrh_hz_set_modes(vrh_hz_t *h, const int *m);
+/* Mesh: z faces (nz+1), z centres (nz), r centres (nr); any may be NULL. */
+int vrh_hz_mesh(vrh_hz_t *h, double *zf, double *zm, double *rm);
+/* y = S_m x per slot. */
+int vrh_hz_apply(vrh_hz_t *h, const double *x, double *y);
+/* Direct solve per slot: generalised eigenvectors of the z operator (dense
+ * transforms) and one radial tridiagonal solve per (slot, z eigenvalue). */
+int vrh_hz_direct(vrh_hz_t *h, const double *f, double *p);
+/* Iterative solve of every slot to ||V f - S p|| <= tol ||V f||:
+ *   VRH_HZ_PCG       Jacobi-preconditioned CG, zero start;
+ *   VRH_HZ_BICGSTAB  Jacobi-preconditioned BiCGSTAB, zero start;
+ *   VRH_HZ_LEARNED   flexible CG preconditioned by the network, started from
+ *                    the network's estimate; a slot that does not converge
+ *                    within maxit falls back to plain PCG from where it is.
+ * The true residual of the returned p is recomputed and reported per slot in
+ * relres; iters receives the iterations per slot (fallback included), and
+ * ms (optional) the solve time in milliseconds (GPU events). */
+int vrh_hz_solve(vrh_hz_t *h, int method, double tol, int maxit, const double *f, double *p, int *iters,
+                 double *relres, double *ms);
+
+/* ---- network: 4 layers of 3x3 convolutions, dilations 1, 2, 4, 8, widths
+ * 16 (4 input features) -> 32 -> 32 -> 32 -> 16 (channel 0 is the output),
+ * ReLU between layers; parameters as fp32, per layer W (9, cout, cin) then
+ * b (cout). */
+int vrh_hz_net_size(vrh_hz_t *h);
+int vrh_hz_net_init(vrh_hz_t *h, unsigned long long seed);
+int vrh_hz_net_get(vrh_hz_t *h, float *params);
+int vrh_hz_net_set(vrh_hz_t *h, const float *params);
+/* Network output channel (nslots, nz, nr) for the features of f, and the
+ * feature scale rms(f) dr^2 per slot (the estimate of p is scale * out). */
+int vrh_hz_net_forward(vrh_hz_t *h, const double *f, float *out, double *scale);
+/* Gradient of sum(gout * out) with respect to the parameters, for the
+ * features of f (same packing as the parameters). */
+int vrh_hz_net_grad(vrh_hz_t *h, const double *f, const float *gout, float *grads);
+/* Right-hand sides the trainer draws: family 0 Gaussian ring cores
+ * (4 per sample, radius 0.1..0.3 of the r extent scale), family 1 white noise. */
+int vrh_hz_sample(vrh_hz_t *h, unsigned long long seed, int family, double *f);
+/* Train on nsamples (rhs, p) pairs generated on the GPU (rhs from both
+ * families, random m in 0..63 per slot, p from the direct solve), nslots per
+ * batch, Adam with learning rate lr, energy-norm loss
+ * (scale out - p)^T S (scale out - p) / (p^T V f).  loss (optional) receives
+ * the mean loss of each batch. */
+int vrh_hz_train(vrh_hz_t *h, int nsamples, double lr, unsigned long long seed, float *loss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

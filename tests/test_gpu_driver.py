@@ -30,7 +30,7 @@ maxz                12.0
 poisson              fftp
 
 ===Para===
-dt                   0.01
+dt                   {dt}
 totstp               {totstp}
 expstp               5
 anlstp               5
@@ -71,7 +71,7 @@ def read_restart(path, nt, nr, nz):
 def test_driver_ring_run_matches_oracle(tmp_path):
     nt, nr, nz = 33, 32, 64
     print("case driver ring 33x32x64", flush=True)
-    (tmp_path / "inp.txt").write_text(DECK.format(nt=nt, nr=nr, nz=nz, totstp=10, restrt=0))
+    (tmp_path / "inp.txt").write_text(DECK.format(nt=nt, nr=nr, nz=nz, totstp=10, restrt=0, dt=0.01))
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     # exports after nstep = 0, 5, 10 (mod(nstep, expstp) == 0, loop.F90:136)
@@ -89,9 +89,10 @@ def test_driver_ring_run_matches_oracle(tmp_path):
         assert o.rk3_step() == 0
     ref = np.stack([o.field(n)[:, 1 : nr + 1, 1 : nz + 1] for n in ("q1", "q2", "q3")])
     assert np.abs(qd - ref).max() <= 1e-10 * np.abs(ref).max()
-    # restart: load the step-5 file as RESTART and take one step
+    # restart: load the step-5 file as RESTART and take one step with the
+    # first run's dt (a restart keeps the deck's dt; CFL applies to a fresh ring)
     shutil.copy(tmp_path / "restart00005RK03.dat", tmp_path / "RESTART")
-    (tmp_path / "inp.txt").write_text(DECK.format(nt=nt, nr=nr, nz=nz, totstp=0, restrt=1))
+    (tmp_path / "inp.txt").write_text(DECK.format(nt=nt, nr=nr, nz=nz, totstp=0, restrt=1, dt=repr(dt)))
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     n5, t5, q5 = read_restart(tmp_path / "RESTART", nt, nr, nz)

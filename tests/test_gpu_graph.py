@@ -1,0 +1,71 @@
+"""Graph-captured steps and ensembles against eager launches of the same
+kernels (bit-identical), and against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle
+
+import vring_amd
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("q1", "q2", "q3", "pres")
+
+
+def ring_solver(nt, nr, nz, Re, precision="fp64", graph=True, amp=0.0):
+    phases = np.linspace(0.3, 2.0, 3)
+    q = vring_amd.ring_initial_condition(nt, nr, nz, 4.0, 12.0, [(1.0, 3.0, 0.3, 1.0)], pert_amp=amp,
+                                         pert_modes=(2, 4), phases=phases)
+    dt = vring_amd.cfl_dt(nt, nr, nz, 4.0, 12.0, *q, cfl=0.3, ujet=1.0)
+    s = vring_amd.Solver(nt, nr, nz, 4.0, 12.0, Re, dt, precision=precision)
+    for n, a in zip(("q1", "q2", "q3"), q):
+        s.upload(n, a)
+    s.set_graph(graph)
+    return s, q, dt
+
+
+def test_graph_replays_match_eager_and_oracle():
+    nt, nr, nz, Re = 32, 32, 64, 1000.0
+    print(f"case graph nt={nt} nr={nr} nz={nz}", flush=True)
+    sg, q, dt = ring_solver(nt, nr, nz, Re, graph=True, amp=1e-2)
+    se, _, _ = ring_solver(nt, nr, nz, Re, graph=False, amp=1e-2)
+    o = Oracle(nt, nr, nz, 4.0, 12.0, Re, dt)
+    for n, a in zip(("q1", "q2", "q3"), q):
+        o.field(n)[...] = a
+    for rep in range(4):
+        sg.step(1 + rep % 2)
+        se.step(1 + rep % 2)
+        for _ in range(1 + rep % 2):
+            assert o.rk3_step() == 0
+        assert sg.time() == se.time()
+        assert abs(sg.time() - o.time()) < 1e-14
+        scale = max(np.abs(o.field(n)).max() for n in ("q1", "q2", "q3"))
+        for n in FIELDS:
+            g, e = sg.download(n), se.download(n)
+            assert np.array_equal(g, e), (rep, n)
+            sc = scale if n != "pres" else np.abs(o.field("pres")).max()
+            assert np.abs(g - o.field(n)).max() < 1e-11 * sc, (rep, n)
+    sg.close()
+    se.close()
+
+
+def test_ensemble_matches_members_stepped_alone():
+    cases = [(16, 24, 48, 500.0, "fp64"), (33, 32, 64, 2000.0, "fp64"), (32, 48, 96, 1000.0, "mixed"),
+             (64, 32, 128, 4000.0, "mixed")]
+    print("case ensemble", cases, flush=True)
+    ens_members, alone = [], []
+    for nt, nr, nz, Re, prec in cases:
+        ens_members.append(ring_solver(nt, nr, nz, Re, prec, graph=True, amp=1e-2)[0])
+        alone.append(ring_solver(nt, nr, nz, Re, prec, graph=False, amp=1e-2)[0])
+    ens = vring_amd.Ensemble(ens_members)
+    for rep in range(3):
+        ens.step(2)
+        for s in alone:
+            s.step(2)
+        for m, s in zip(ens_members, alone):
+            assert m.time() == s.time()
+            for n in FIELDS:
+                assert np.array_equal(m.download(n), s.download(n)), (rep, m.nt, n)
+    ens.close()
+    for s in ens_members + alone:
+        s.close()

@@ -10,10 +10,13 @@ import vring_amd
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("nt,nr,nz", [(9, 16, 32), (32, 32, 64), (33, 24, 40), (16, 64, 128)])
-def test_substep_phases_match_oracle(nt, nr, nz):
-    print(f"case phases nt={nt} nr={nr} nz={nz}", flush=True)
+@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("nt,nr,nz", [(9, 16, 32), (32, 32, 64), (33, 24, 40), (16, 64, 128), (128, 20, 40)])
+def test_substep_phases_match_oracle(nt, nr, nz, form):
+    """form: explicit kernel with one z plane per block (0) or z marching (1)."""
+    print(f"case phases nt={nt} nr={nr} nz={nz} form={form}", flush=True)
     o, s = make_pair(nt, nr, nz, slip=0.0)
+    s.set_tuning(explicit_form=form)
     seed_random(o, s, seed=nt + nr + nz)
     o.set_time(0.13)
     s.set_time(0.13)

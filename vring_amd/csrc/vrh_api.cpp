@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "vrh_internal.h"
@@ -90,6 +91,17 @@ bool lookup(vrh_t *h, const char *name, Named &out) {
   return false;
 }
 
+// One substep whose time-dependent scalars (inlet increment, mass
+// correction) are read from d_sc, as written by k_scalars at the start of
+// the step: the form that is captured in a graph.
+template <typename TW> void substep_dev(vrh_t *h, int irk) {
+  launch_hdnbc<TW>(h, irk, 0.0, h->d_sc + 3 + irk);
+  launch_explicit<TW>(h, irk);
+  launch_sweeps<TW>(h, irk, 0.0, h->d_sc + irk);
+  launch_poisson<TW>(h, irk, nullptr, nullptr);
+  launch_project<TW>(h, irk);
+}
+
 template <typename TW> int substep_t(vrh_t *h, int irk, int phase) {
   const double t0 = h->ntime, t1 = h->ntime + h->dt;   // momentum_terms_swirl.F90:192-193
   const double unom0 = host_profile(t0), unom1 = host_profile(t1);
@@ -101,9 +113,9 @@ template <typename TW> int substep_t(vrh_t *h, int irk, int phase) {
     for (int i = 1; i <= h->nt; i++) qinf = qinf + cj * h->da[j];
   }
   const double cor = (qinf - 0.0) / (h->cor_red + (double)1.e-15f);  // :283
-  if (phase < 0 || phase == VRH_PH_HDNBC) launch_hdnbc<TW>(h, irk, cor, c3s);
+  if (phase < 0 || phase == VRH_PH_HDNBC) launch_hdnbc<TW>(h, irk, cor, nullptr);
   if (phase < 0 || phase == VRH_PH_EXPLICIT) launch_explicit<TW>(h, irk);
-  if (phase < 0 || phase == VRH_PH_ADI) launch_sweeps<TW>(h, irk, c3s);
+  if (phase < 0 || phase == VRH_PH_ADI) launch_sweeps<TW>(h, irk, c3s, nullptr);
   if (phase < 0 || phase == VRH_PH_POISSON) launch_poisson<TW>(h, irk, nullptr, nullptr);
   if (phase < 0 || phase == VRH_PH_PROJECT) launch_project<TW>(h, irk);
   HIPCHK(h, hipGetLastError());
@@ -125,6 +137,43 @@ int check_flag(vrh_t *h) {
 
 }  // namespace
 
+// A whole RK3 step (loop.F90:53-131): scalars, particles at irk 1, three
+// substeps.  Nothing here synchronises or copies, so it can be captured.
+int step_body(vrh_t *h) {
+  launch_scalars(h);
+  for (int irk = 1; irk <= 3; irk++) {
+    if (h->nbatch > 0 && irk == 1) launch_particles(h, irk);  // loop.F90:66-70
+    if (h->prec == VRH_MIXED) substep_dev<float>(h, irk);
+    else substep_dev<double>(h, irk);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : fail(h, "kernel launch failed", -3);
+}
+
+void graph_invalidate(vrh_t *h) {
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  h->graph_ok = false;
+  h->epoch++;
+}
+
+namespace {
+int build_graph(vrh_t *h) {
+  graph_invalidate(h);
+  HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = step_body(h);
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(h->stream, &gr);
+  if (rc) { if (gr) hipGraphDestroy(gr); return rc; }
+  HIPCHK(h, e);
+  h->graph = gr;
+  HIPCHK(h, hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+  h->graph_ok = true;
+  return 0;
+}
+}  // namespace
+
 extern "C" {
 
 int vrh_last_error(vrh_t *h, char *buf, size_t len) {
@@ -137,7 +186,7 @@ int vrh_last_error(vrh_t *h, char *buf, size_t len) {
 int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t **out) {
   if (!grid || !p || !out) return -1;
   *out = nullptr;
-  if (grid->ntheta < 3 || grid->nr < 4 || grid->nz < 4 || grid->nr > 16 * 32 || grid->ntheta > 1024 ||
+  if (grid->ntheta < 3 || grid->nr < 4 || grid->nz < 4 || grid->nr > 16 * 32 || grid->ntheta > 256 ||
       grid->rmax <= 0 || grid->zmax <= 0 || p->dt <= 0 || p->Re <= 0)
     return -1;
   if (precision != VRH_FP64 && precision != VRH_MIXED) return -1;
@@ -255,6 +304,11 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
   host_mint(h, mint.data());
   if ((rc = dalloc(h, &h->d_mint, mint.size()))) return rc;
   HIPCHK(h, hipMemcpy(h->d_mint, mint.data(), mint.size() * sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = dalloc(h, &h->d_sc, 16))) return rc;
+  HIPCHK(h, hipHostMalloc((void **)&h->h_pin, 16 * sizeof(double), hipHostMallocDefault));
+  h->inlet_area = 0.0;  // momentum_terms_swirl.F90:261-267 with c3s factored out
+  for (int j = 1; j <= h->jlim; j++)
+    for (int i = 1; i <= h->nt; i++) h->inlet_area = h->inlet_area + h->da[j];
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
@@ -268,6 +322,9 @@ int vrh_destroy(vrh_t *h) {
                   f.prg3, f.S, f.ib, f.S64, f.ib64, f.csort, f.dq3no, f.corr3n, f.flag};
   for (void *p : ptrs) if (p) hipFree(p);
   for (auto &e : h->kt_pool) hipEventDestroy(e);
+  graph_invalidate(h);
+  if (h->d_sc) hipFree(h->d_sc);
+  if (h->h_pin) hipHostFree(h->h_pin);
   for (auto &b : h->pb) {
     if (b.u) hipFree(b.u);
     if (b.ddt2) hipFree(b.ddt2);
@@ -348,15 +405,30 @@ int vrh_phase(vrh_t *h, int phase, int irk) {
 
 int vrh_step(vrh_t *h, int nsteps) {
   if (!h || nsteps < 0) return -1;
+  if (nsteps == 0) return 0;
+  // the device copy of ntime starts from the host value; h_pin is free here
+  // because every vrh_step ends with the stream drained
+  h->h_pin[0] = h->ntime;
+  HIPCHK(h, hipMemcpyAsync(h->d_sc, h->h_pin, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  const bool graph = h->use_graph && !h->kt_on;
+  if (graph && !h->graph_ok) { int rc = build_graph(h); if (rc) return rc; }
   for (int s = 0; s < nsteps; s++) {
-    h->ntime = h->dt + h->ntime;  // loop.F90:53
-    for (int irk = 1; irk <= 3; irk++) {
-      if (h->nbatch > 0 && irk == 1) launch_particles(h, irk);  // loop.F90:66-70
-      int rc = substep(h, irk, -1);
+    h->ntime = h->dt + h->ntime;  // loop.F90:53 (k_scalars does the same on the device)
+    if (graph) {
+      HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+    } else {
+      int rc = step_body(h);
       if (rc) return rc;
     }
   }
   return check_flag(h);
+}
+
+int vrh_set_graph(vrh_t *h, int on) {
+  if (!h) return -1;
+  h->use_graph = on != 0;
+  if (!h->use_graph) graph_invalidate(h);
+  return 0;
 }
 
 int vrh_synchronize(vrh_t *h) {
@@ -388,6 +460,7 @@ int vrh_poisson_solve(vrh_t *h, const double *rhs, double *phi) {
 // partitioned-Thomas coefficients for the current segment lengths
 // ---------------------------------------------------------------------------
 int build_line_coefs(vrh_t *h) {
+  graph_invalidate(h);
   std::vector<double> lcv[4][6][7];
   size_t tot = 0;
   for (int irk = 1; irk <= 3; irk++) {
@@ -426,9 +499,9 @@ int build_line_coefs(vrh_t *h) {
 template <typename TW> static double time_sweeps(vrh_t *h, int reps) {
   hipEvent_t e0, e1;
   hipEventCreate(&e0); hipEventCreate(&e1);
-  launch_sweeps<TW>(h, 1, 0.0);
+  launch_sweeps<TW>(h, 1, 0.0, nullptr);
   hipEventRecord(e0, h->stream);
-  for (int r = 0; r < reps; r++) launch_sweeps<TW>(h, 1, 0.0);
+  for (int r = 0; r < reps; r++) launch_sweeps<TW>(h, 1, 0.0, nullptr);
   hipEventRecord(e1, h->stream);
   hipEventSynchronize(e1);
   float ms = 0.f;
@@ -442,6 +515,22 @@ template <typename TW> static double time_poisson(vrh_t *h, int reps) {
   launch_poisson<TW>(h, 1, nullptr, nullptr);
   hipEventRecord(e0, h->stream);
   for (int r = 0; r < reps; r++) launch_poisson<TW>(h, 1, nullptr, nullptr);
+  hipEventRecord(e1, h->stream);
+  hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0); hipEventDestroy(e1);
+  return ms;
+}
+
+// The explicit kernel at irk = 1 (rho = 0): its outputs r1..r3 and ru are
+// rewritten by the next substep before they are read with a nonzero weight.
+template <typename TW> static double time_explicit(vrh_t *h, int reps) {
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0); hipEventCreate(&e1);
+  launch_explicit<TW>(h, 1);
+  hipEventRecord(e0, h->stream);
+  for (int r = 0; r < reps; r++) launch_explicit<TW>(h, 1);
   hipEventRecord(e1, h->stream);
   hipEventSynchronize(e1);
   float ms = 0.f;
@@ -491,11 +580,22 @@ extern "C" int vrh_autotune(vrh_t *h, int reps, int *choice) {
     }
     h->dct_lines = best;
   }
+  {
+    int best = 0; double bestt = 1e30;
+    for (int v : {0, 1}) {
+      h->explicit_variant = v;
+      double t = h->prec == VRH_MIXED ? time_explicit<float>(h, reps) : time_explicit<double>(h, reps);
+      if (t < bestt) { bestt = t; best = v; }
+    }
+    h->explicit_variant = best;
+  }
+  graph_invalidate(h);
   h->kt_on = kt;
   if (choice) {
     for (int sw = 0; sw < 6; sw++) choice[sw] = h->Msw[sw];
     choice[6] = h->fft_lines;
     choice[7] = h->dct_lines;
+    choice[8] = h->explicit_variant;
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
@@ -552,5 +652,160 @@ extern "C" int vrh_ktimer_read(vrh_t *h, int idx, char *name, size_t len, double
   if (name && len) snprintf(name, len, "%s", kKernelNames[idx]);
   if (total_ms) *total_ms = h->kt_ms[idx];
   if (count) *count = h->kt_cnt[idx];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles: independent rings (each its own handle, grid, Re and stream)
+// advanced together.  One RK3 step of every member is captured into a single
+// HIP graph, forked from an origin stream into the members' streams and
+// joined back, so a step of the whole ensemble is one graph launch and the
+// members' kernels run concurrently.  No reference counterpart (the
+// reference runs one ring per MPI job).
+// ---------------------------------------------------------------------------
+struct vrh_ens_s {
+  std::vector<vrh_t *> m;
+  std::vector<long> epochs;
+  hipStream_t origin = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::string err;
+};
+
+namespace {
+int efail(vrh_ens_t *e, const std::string &msg, int code = -1) {
+  if (e) e->err = msg;
+  return code;
+}
+#define ECHK(e, call)                                                               \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) return efail(e, std::string(#call) + ": " + hipGetErrorString(e_), -3); \
+  } while (0)
+
+void ens_release_graph(vrh_ens_t *e) {
+  if (e->gexec) hipGraphExecDestroy(e->gexec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  e->gexec = nullptr;
+  e->graph = nullptr;
+}
+
+int ens_build(vrh_ens_t *e) {
+  ens_release_graph(e);
+  ECHK(e, hipStreamBeginCapture(e->origin, hipStreamCaptureModeThreadLocal));
+  ECHK(e, hipEventRecord(e->fork, e->origin));
+  int rc = 0;
+  for (size_t q = 0; q < e->m.size() && rc == 0; q++) {
+    vrh_t *h = e->m[q];
+    if (hipStreamWaitEvent(h->stream, e->fork, 0) != hipSuccess) { rc = -3; break; }
+    rc = step_body(h);
+    if (rc) { e->err = "member " + std::to_string(q) + ": " + h->err; break; }
+    if (hipEventRecord(e->join[q], h->stream) != hipSuccess ||
+        hipStreamWaitEvent(e->origin, e->join[q], 0) != hipSuccess) rc = -3;
+  }
+  hipGraph_t gr = nullptr;
+  hipError_t ee = hipStreamEndCapture(e->origin, &gr);
+  if (rc) { if (gr) hipGraphDestroy(gr); return rc; }
+  ECHK(e, ee);
+  e->graph = gr;
+  ECHK(e, hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+  for (size_t q = 0; q < e->m.size(); q++) e->epochs[q] = e->m[q]->epoch;
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int vrh_ensemble_create(vrh_t *const *members, int n, vrh_ens_t **out) {
+  if (!members || n < 1 || !out) return -1;
+  *out = nullptr;
+  vrh_ens_t *e = new (std::nothrow) vrh_ens_t();
+  if (!e) return -1;
+  *out = e;
+  for (int q = 0; q < n; q++) {
+    if (!members[q]) return efail(e, "null member handle");
+    for (int p = 0; p < q; p++)
+      if (members[p] == members[q]) return efail(e, "a handle appears twice in the ensemble");
+    e->m.push_back(members[q]);
+  }
+  e->epochs.assign(n, -1);
+  ECHK(e, hipStreamCreateWithFlags(&e->origin, hipStreamNonBlocking));
+  ECHK(e, hipEventCreateWithFlags(&e->fork, hipEventDisableTiming));
+  e->join.assign(n, nullptr);
+  for (int q = 0; q < n; q++) ECHK(e, hipEventCreateWithFlags(&e->join[q], hipEventDisableTiming));
+  return 0;
+}
+
+int vrh_ensemble_step(vrh_ens_t *e, int nsteps) {
+  if (!e || nsteps < 0) return -1;
+  if (nsteps == 0) return 0;
+  bool stale = !e->gexec;
+  for (size_t q = 0; q < e->m.size(); q++) {
+    vrh_t *h = e->m[q];
+    if (h->kt_on) return efail(e, "member " + std::to_string(q) + " has the kernel timer enabled");
+    ECHK(e, hipStreamSynchronize(h->stream));
+    if (h->epoch != e->epochs[q]) stale = true;
+  }
+  if (stale) { int rc = ens_build(e); if (rc) return rc; }
+  for (vrh_t *h : e->m) {
+    h->h_pin[0] = h->ntime;
+    ECHK(e, hipMemcpyAsync(h->d_sc, h->h_pin, sizeof(double), hipMemcpyHostToDevice, e->origin));
+  }
+  for (int s = 0; s < nsteps; s++) {
+    for (vrh_t *h : e->m) h->ntime = h->dt + h->ntime;  // loop.F90:53
+    ECHK(e, hipGraphLaunch(e->gexec, e->origin));
+  }
+  ECHK(e, hipStreamSynchronize(e->origin));
+  for (size_t q = 0; q < e->m.size(); q++) {
+    int rc = check_flag(e->m[q]);
+    if (rc) return efail(e, "member " + std::to_string(q) + ": " + e->m[q]->err, rc);
+  }
+  return 0;
+}
+
+int vrh_ensemble_last_error(vrh_ens_t *e, char *buf, size_t len) {
+  if (!buf || len == 0) return -1;
+  snprintf(buf, len, "%s", e ? e->err.c_str() : "null ensemble");
+  return 0;
+}
+
+int vrh_ensemble_destroy(vrh_ens_t *e) {
+  if (!e) return -1;
+  if (e->origin) hipStreamSynchronize(e->origin);
+  ens_release_graph(e);
+  if (e->fork) hipEventDestroy(e->fork);
+  for (auto ev : e->join) if (ev) hipEventDestroy(ev);
+  if (e->origin) hipStreamDestroy(e->origin);
+  delete e;
+  return 0;
+}
+
+}  // extern "C"
+
+// Apply a tuning previously returned by vrh_autotune (same 9 entries) without
+// timing anything; 0 entries keep the current value.
+extern "C" int vrh_set_tuning(vrh_t *h, const int *choice) {
+  if (!h || !choice) return -1;
+  const int ns[6] = {h->nr, h->nr, h->nrc, h->nzc, h->nzc, h->nzc};
+  for (int sw = 0; sw < 6; sw++) {
+    const int M = choice[sw];
+    if (M == 0) continue;
+    if (M != 16 && M != 24 && M != 32 && M != 40 && M != 48 && M != 64) return fail(h, "segment length must be 16..64");
+    if ((ns[sw] + M - 1) / M > 16) return fail(h, "segment length too short for the line length");
+  }
+  if (choice[6] != 0 && choice[6] != 2 && choice[6] != 4 && choice[6] != 8 && choice[6] != 16)
+    return fail(h, "fft lines per block must be 2, 4, 8 or 16");
+  if (choice[7] != 0 && choice[7] != 2 && choice[7] != 4 && choice[7] != 8 && choice[7] != 16)
+    return fail(h, "dct lines per block must be 2, 4, 8 or 16");
+  if (choice[8] < -1 || choice[8] > 1) return fail(h, "explicit form must be -1, 0 or 1");
+  for (int sw = 0; sw < 6; sw++) if (choice[sw]) h->Msw[sw] = choice[sw];
+  if (choice[6]) h->fft_lines = choice[6];
+  if (choice[7]) h->dct_lines = choice[7];
+  h->explicit_variant = choice[8];
+  int rc = build_line_coefs(h);
+  if (rc) return rc;
+  graph_invalidate(h);
   return 0;
 }

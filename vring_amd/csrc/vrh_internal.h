@@ -100,6 +100,7 @@ struct vrh_s {
   int Msw[6];        // segment length M of each sweep (partitioned Thomas)
   int fft_lines = 0;  // theta lines per block of the FFT kernels (0: default)
   int dct_lines = 0;  // z lines per block of the DCT kernels (0: default)
+  int explicit_variant = -1;  // 0: plane per block, 1: z marching, -1: default per precision
   int theta_levels[4];  // PCR levels of the theta solve per substep
   // transform tables (double): theta twiddles (nt/2 complex), z twiddles
   // (nz/2 complex), z post-twiddles (nz complex)
@@ -120,7 +121,23 @@ struct vrh_s {
   int kt_used = 0;
   double kt_ms[KT_COUNT] = {0};
   long kt_cnt[KT_COUNT] = {0};
+  // step scalars on the device: [0] ntime, [1..3] c3s of substeps 1..3,
+  // [4..6] mass-correction factor cor of substeps 1..3; inlet_area is the
+  // sum of da over the inlet rows (both theta and r)
+  double *d_sc = nullptr;
+  double *h_pin = nullptr;  // pinned host staging (time)
+  double inlet_area = 0.0;
+  // one RK3 step captured as a HIP graph (rebuilt when launch shapes,
+  // particles or the timer change)
+  bool use_graph = true;
+  bool graph_ok = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  long epoch = 0;  // bumped whenever captured launch parameters change
 };
+void graph_invalidate(vrh_t *h);
+// queue one whole RK3 step whose scalars come from d_sc (captured or eager)
+int step_body(vrh_t *h);
 void ktimer_harvest(vrh_t *h);
 
 // host math (vrh_host.cpp)
@@ -132,9 +149,10 @@ double host_profile(double t);
 void host_mint(vrh_t *h, double *mint);
 
 // kernel launchers (vrh_kernels.hip); TW = double or float
-template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, double c3s);
+template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, const double *cor_dev);
+void launch_scalars(vrh_t *h);
 template <typename TW> void launch_explicit(vrh_t *h, int irk);
-template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s);
+template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const double *c3s_dev);
 template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_rhs, double *phi_out);
 template <typename TW> void launch_project(vrh_t *h, int irk);
 void launch_particles(vrh_t *h, int irk);

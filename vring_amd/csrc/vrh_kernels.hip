@@ -19,11 +19,13 @@ template <typename T> __device__ __forceinline__ T ldg(const T *p) { return __bu
 // hdnbc: outlet convective condition and mass correction, one thread per
 // (theta, r) outlet point (momentum_terms_swirl.F90:225-291).
 // ---------------------------------------------------------------------------
-__global__ void k_hdnbc(VGrid g, VCoef c, double gam, double rom, double cor, const double *__restrict__ q3,
-                        double *__restrict__ csort, double *__restrict__ dq3no, double *__restrict__ corr3n) {
+__global__ void k_hdnbc(VGrid g, VCoef c, double gam, double rom, double cor, const double *__restrict__ cor_dev,
+                        const double *__restrict__ q3, double *__restrict__ csort, double *__restrict__ dq3no,
+                        double *__restrict__ corr3n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   int j = blockIdx.y + 1;  // 1..nrc
   if (i > g.nt) return;
+  if (cor_dev) cor = *cor_dev;
   long p = (long)(i - 1) + (long)g.nt * j;
   double qn = q3[hidx(g, i, j, g.nzc)], qm = q3[hidx(g, i, j, g.nz)];
   double cs = csort[p];
@@ -35,14 +37,315 @@ __global__ void k_hdnbc(VGrid g, VCoef c, double gam, double rom, double cor, co
   corr3n[p] = c3n + c.qc[j] * cor;
 }
 
+// Inlet time profile (gen_tools_swirl.F90:159-175): smooth ramp to 1 over
+// 0.1n, hold until n, ramp down (n = 2).
+__device__ __host__ inline double profile_fn(double t) {
+  const double n = 2.0, t1 = (double)0.1f * n;
+  double u = 0.0;
+  if (t > 0.0 && t < t1) { double x = t / t1; u = 3.0 * (x * x) - 2.0 * (x * x * x); }
+  if (t > t1 && t < n) u = 1.0;
+  if (t > n && t < n + t1) { double x = (n + t1 - t) / t1; u = 3.0 * (x * x) - 2.0 * (x * x * x); }
+  if (t > n + t1) u = 0.0;
+  return u;
+}
+
+// Per-step scalars (loop.F90:53; momentum_terms_swirl.F90:192-212, 261-283):
+// advance ntime by dt, then for each substep the inlet increment
+// c3s = (profile(t+dt) - profile(t)) alpha and the mass-correction factor
+// cor = c3s * inlet_area / (sum qc^2 da + 1e-15).
+__global__ void k_scalars(double *sc, double dt, double a1, double a2, double a3, double inlet_area,
+                          double cor_red) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double t = dt + sc[0];
+  sc[0] = t;
+  const double u0 = profile_fn(t), u1 = profile_fn(t + dt);
+  const double al[3] = {a1, a2, a3};
+  for (int q = 0; q < 3; q++) {
+    const double c3s = (u1 - u0) * al[q];
+    sc[1 + q] = c3s;
+    sc[4 + q] = (c3s * inlet_area - 0.0) / (cor_red + (double)1.e-15f);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Explicit part: hdnl1-3 (momentum_terms_swirl.F90:15-174) fused with the
-// explicit right-hand sides of step (step.F90:32-165).  One thread per cell;
-// q is read once per block through L1/L2, H goes straight into ru and qrhs.
-// The inlet planes q1s, q2s, q3s are zero (momentum_terms_swirl.F90:216-223).
+// explicit right-hand sides of step (step.F90:32-165) and the periodic theta
+// solves solqi/tripij (step.F90:212-263, 487-534).
+// A block owns whole theta rows (NTP >= nt lanes) of RJ consecutive r rows
+// and marches up a chunk of z planes.  The planes k-1, k, k+1 of q1, q2, q3
+// (rows j0-1 .. j0+RJ) sit in a 3-slot LDS ring, so every stencil operand
+// comes from LDS; the next plane is prefetched into registers while the
+// current one is computed.  The theta systems (constant coefficients
+// (-acc, 1+2acc, -acc) per r row) are solved by parallel cyclic reduction
+// on LDS-resident lines; the reduction ratios of each row are computed once
+// per block.  The inlet planes q1s, q2s, q3s are zero
+// (momentum_terms_swirl.F90:216-223).
 // ---------------------------------------------------------------------------
+constexpr int kMaxPcrLevels = 40;
+constexpr int kRowCoefs = 24;
+
 template <typename TW>
-__global__ __launch_bounds__(1024) void k_explicit(VGrid g, VCoef c, VRK rk, double betadx, int nlev,
+__global__ __launch_bounds__(512) void k_explicit_march(VGrid g, VCoef c, VRK rk, double betadx, int nlev, int kchunk,
+                                                  const double *__restrict__ q1, const double *__restrict__ q2,
+                                                  const double *__restrict__ q3, const TW *__restrict__ prg1,
+                                                  const TW *__restrict__ prg2, const TW *__restrict__ prg3,
+                                                  TW *__restrict__ ru1, TW *__restrict__ ru2, TW *__restrict__ ru3,
+                                                  TW *__restrict__ r1, TW *__restrict__ r2, TW *__restrict__ r3) {
+  extern __shared__ unsigned char smem_raw[];
+  const int NTP = blockDim.x, RJ = blockDim.y, ti = threadIdx.x, tj = threadIdx.y;
+  const int NTH = NTP * RJ, tid = tj * NTP + ti;
+  const int nt = g.nt, nr = g.nr, nz = g.nz;
+  const long sj = g.sj, sk = g.sk;
+  const int j0 = 1 + blockIdx.x * RJ;
+  const int kb = 1 + blockIdx.y * kchunk, ke = min(kb + kchunk - 1, nz);
+  const int LRW = RJ + 2, PL = LRW * NTP;
+  double *pcr = reinterpret_cast<double *>(smem_raw);             // [RJ][kMaxPcrLevels][2] + [RJ][2]
+  double *fin = pcr + RJ * kMaxPcrLevels * 2;
+  TW *cfs = reinterpret_cast<TW *>(fin + RJ * 2);                 // [RJ][kRowCoefs]
+  TW *ring = cfs + RJ * kRowCoefs;                                // [3 slots][3 arrays][LRW][NTP]
+  TW *fb = ring + 9 * PL;                                         // [2][3][RJ][NTP]
+  const int j = j0 + tj;
+  const bool cell = (ti < nt) && (j <= nr);
+  const int jc = (j <= nr) ? j : nr;
+  // PCR ratios of this row (one lane per row)
+  if (ti == 0) {
+    double a1 = -(betadx * c.s1t11[jc]), b1 = 1.0 + 2.0 * (betadx * c.s1t11[jc]);
+    double a2 = -(betadx * c.s2t11[jc]), b2 = 1.0 + 2.0 * (betadx * c.s2t11[jc]);
+    for (int lvl = 0; lvl < nlev; ++lvl) {
+      const double al1 = -a1 / b1, al2 = -a2 / b2;
+      pcr[(tj * kMaxPcrLevels + lvl) * 2] = al1;
+      pcr[(tj * kMaxPcrLevels + lvl) * 2 + 1] = al2;
+      b1 = b1 + 2.0 * al1 * a1; a1 = al1 * a1;
+      b2 = b2 + 2.0 * al2 * a2; a2 = al2 * a2;
+    }
+    const bool pow2 = nt >= 2 && (nt & (nt - 1)) == 0;
+    fin[tj * 2] = 1.0 / (pow2 ? b1 + 2.0 * a1 : b1);
+    fin[tj * 2 + 1] = 1.0 / (pow2 ? b2 + 2.0 * a2 : b2);
+    const int jm = (jc == 1) ? 1 : jc - 1;
+    TW *cf = cfs + tj * kRowCoefs;
+    cf[0] = (TW)c.h1d11[jc]; cf[1] = (TW)c.h1d22d[jc]; cf[2] = (TW)c.h1d22n[jc]; cf[3] = (TW)c.h1dare[jc];
+    cf[4] = (TW)c.h2d11[jc]; cf[5] = (TW)c.h2d22p[jc]; cf[6] = (TW)c.h2d22m[jc]; cf[7] = (TW)c.h2dare[jc];
+    cf[8] = (TW)c.s1t11[jc]; cf[9] = (TW)c.s1t2n[jc]; cf[10] = (TW)c.s1t2s[jc]; cf[11] = (TW)c.s1t2c[jc];
+    cf[12] = (TW)c.s2t11[jc]; cf[13] = (TW)c.s2t2p[jc]; cf[14] = (TW)c.s2t2c[jc]; cf[15] = (TW)c.s2t2m[jc];
+    cf[16] = (TW)c.s3t2n[jc]; cf[17] = (TW)c.s3t2s[jc];
+    cf[18] = (TW)(1.0 / c.rm[jc + 1]); cf[19] = (TW)(1.0 / c.rm[jc]); cf[20] = (TW)(1.0 / c.rm[jm]);
+  }
+  // plane fetch: element e = tid + l*NTH of the (LRW x NTP) tile, three arrays
+  constexpr int NL = 3;
+  long goff[NL];
+  int lidx[NL];
+  bool lval[NL];
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    const int e = tid + l * NTH;
+    const int r = e / NTP, cc = e - r * NTP;
+    const int jj = j0 - 1 + r;
+    lidx[l] = (e < PL) ? e : -1;
+    lval[l] = (e < PL) && cc < nt && jj <= nr + 2;
+    goff[l] = (long)cc + (long)jj * sj;
+  }
+  TW pre[3][NL];
+  auto fetch = [&](int kk) {
+#pragma unroll
+    for (int l = 0; l < NL; l++) {
+      const long o = goff[l] + (long)kk * sk;
+      pre[0][l] = lval[l] ? (TW)q1[o] : (TW)0;
+      pre[1][l] = lval[l] ? (TW)q2[o] : (TW)0;
+      pre[2][l] = lval[l] ? (TW)q3[o] : (TW)0;
+    }
+  };
+  auto stash = [&](int slot) {
+#pragma unroll
+    for (int l = 0; l < NL; l++)
+      if (lidx[l] >= 0) {
+        ring[(slot * 3 + 0) * PL + lidx[l]] = pre[0][l];
+        ring[(slot * 3 + 1) * PL + lidx[l]] = pre[1][l];
+        ring[(slot * 3 + 2) * PL + lidx[l]] = pre[2][l];
+      }
+  };
+  int sM = (kb - 1) % 3, s0 = kb % 3, sP = (kb + 1) % 3;
+  fetch(kb - 1); stash(sM);
+  fetch(kb); stash(s0);
+  fetch(kb + 1);
+  // per-row coefficients
+  const TW ga = (TW)rk.g, ro = (TW)rk.r, al = (TW)rk.a, dt = (TW)g.dt, aRe = (TW)(rk.a / g.Re);
+  const TW dx3 = (TW)g.dx3, dx3q = (TW)g.dx3q;
+  const TW *cf = cfs + tj * kRowCoefs;  // per-row metric coefficients (LDS, broadcast reads)
+#define h1d11 cf[0]
+#define h1d22d cf[1]
+#define h1d22n cf[2]
+#define h1dare cf[3]
+#define h2d11 cf[4]
+#define h2d22p cf[5]
+#define h2d22m cf[6]
+#define h2dare cf[7]
+#define s1t11 cf[8]
+#define s1t2n cf[9]
+#define s1t2s cf[10]
+#define s1t2c cf[11]
+#define s2t11 cf[12]
+#define s2t2p cf[13]
+#define s2t2c cf[14]
+#define s2t2m cf[15]
+#define s3t2n cf[16]
+#define s3t2s cf[17]
+#define irmn cf[18]
+#define irm cf[19]
+#define irmm cf[20]
+  const int ci = ti, cip = (ti + 1 >= nt) ? 0 : ti + 1, cim = (ti == 0) ? nt - 1 : ti - 1;
+  const int rw = tj + 1, rm1 = (jc == 1) ? rw : rw - 1;   // LDS rows of j and jm
+  for (int k = kb; k <= ke; k++) {
+    stash(sP);
+    __syncthreads();
+    fetch(k + 2);
+    TW v1 = (TW)0, v2 = (TW)0, v3 = (TW)0;
+    const long o = (long)ti + (long)j * sj + (long)k * sk;
+    const long pc = (long)ti + (long)nt * ((long)(j - 1) + (long)nr * (long)(k - 1));
+    if (cell) {
+      const TW *A1m = ring + (sM * 3 + 0) * PL, *A1 = ring + (s0 * 3 + 0) * PL, *A1p = ring + (sP * 3 + 0) * PL;
+      const TW *A2m = ring + (sM * 3 + 1) * PL, *A2 = ring + (s0 * 3 + 1) * PL, *A2p = ring + (sP * 3 + 1) * PL;
+      const TW *A3m = ring + (sM * 3 + 2) * PL, *A3 = ring + (s0 * 3 + 2) * PL, *A3p = ring + (sP * 3 + 2) * PL;
+#define L(A, row, col) A[(row) * NTP + (col)]
+      const TW q1c = L(A1, rw, ci), q1ip = L(A1, rw, cip), q1im = L(A1, rw, cim);
+      const TW q1n = L(A1, rw + 1, ci), q1jm = L(A1, rm1, ci), q1s = L(A1, rw - 1, ci), q1ips = L(A1, rw - 1, cip);
+      const TW q1kp = L(A1p, rw, ci), q1km = L(A1m, rw, ci), q1ipkm = L(A1m, rw, cip);
+      const TW q2c = L(A2, rw, ci), q2n = L(A2, rw + 1, ci), q2imn = L(A2, rw + 1, cim), q2im = L(A2, rw, cim);
+      const TW q2ip = L(A2, rw, cip), q2s = L(A2, rw - 1, ci), q2kp = L(A2p, rw, ci), q2km = L(A2m, rw, ci);
+      const TW q2nkm = L(A2m, rw + 1, ci);
+      const TW q3c = L(A3, rw, ci), q3kp = L(A3p, rw, ci), q3imkp = L(A3p, rw, cim), q3im = L(A3, rw, cim);
+      const TW q3ip = L(A3, rw, cip), q3s = L(A3, rw - 1, ci), q3skp = L(A3p, rw - 1, ci), q3n = L(A3, rw + 1, ci);
+      const TW q3jm = L(A3, rm1, ci), q3km = L(A3m, rw, ci);
+#undef L
+      // ---- q1 at (theta_c, r_m, z_m) ----
+      {
+        TW qnn = q1ip + q1c, qss = q1im + q1c;
+        TW h11 = (qnn * qnn - qss * qss) * h1d11;
+        qnn = (q2n + q2imn) * (q1n + q1c);
+        qss = (q1c + q1jm) * (q2c + q2im);
+        TW h12d = (qnn - qss) * h1d22d;
+        qnn = (q1ip + q1c) * (q2n + q2c);
+        qss = (q1c + q1im) * (q2imn + q2im);
+        TW h12n = (qnn + qss) * h1d22n;
+        qnn = (TW)0.25 * (q3kp + q3imkp) * (q1kp + q1c);
+        qss = (k == 1) ? (TW)0 : (TW)0.25 * (q3c + q3im) * (q1c + q1km);
+        TW h13 = (qnn - qss) * dx3;
+        qnn = q2n + q2c;
+        qss = q2imn + q2im;
+        TW d1q2 = (qnn - qss) * h1dare;
+        TW H1 = d1q2 - (h11 + (h12d + h12n) + h13);
+        TW d11 = (q1ip - (TW)2 * q1c + q1im) * s1t11;
+        TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (q1kp - (TW)3 * q1c) * dx3q : (q1kp - (TW)2 * q1c + q1km) * dx3q;
+        TW qn2 = (q1n * irmn - q1c * irm) * s1t2n;
+        TW qs2 = (q1c * irm - q1jm * irmm) * s1t2s;
+        TW d22 = (qn2 - qs2) * s1t2c;
+        v1 = (ga * H1 + ro * ru1[o] - al * prg1[pc] + aRe * (d11 + d33 + d22)) * dt;
+        ru1[o] = H1;
+      }
+      // ---- q2 at (theta_m, r_c, z_m), j >= 2 ----
+      if (j >= 2) {
+        TW qnn = (q1ip + q1ips) * (q2ip + q2c);
+        TW qss = (q1c + q1s) * (q2c + q2im);
+        TW h21 = (qnn - qss) * h2d11;
+        qnn = q2n + q2c;
+        qss = q2c + q2s;
+        TW h22 = qnn * qnn * h2d22p - qss * qss * h2d22m;
+        qnn = (TW)0.5 * (q1c + q1ip);
+        qss = (TW)0.5 * (q1s + q1ips);
+        TW h11n = (TW)0.25 * (qnn + qss) * (qnn + qss);
+        qnn = (TW)0.25 * (q3kp + q3skp) * (q2kp + q2c);
+        qss = (k == 1) ? (TW)0 : (TW)0.25 * (q3c + q3s) * (q2c + q2km);
+        TW h23 = (qnn - qss) * dx3;
+        qnn = q1ip + q1ips;
+        qss = q1c + q1s;
+        TW d1q1 = -((qnn - qss) * h2dare);
+        TW H2 = d1q1 - (h21 + h22 + h23 - h11n);
+        TW d11 = (q2ip - (TW)2 * q2c + q2im) * s2t11;
+        TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (q2kp - (TW)3 * q2c) * dx3q : (q2kp - (TW)2 * q2c + q2km) * dx3q;
+        TW d22 = q2n * s2t2p - q2c * s2t2c + q2s * s2t2m;
+        v2 = (ga * H2 + ro * ru2[o] - al * prg2[pc] + aRe * (d11 + d33 + d22)) * dt;
+        ru2[o] = H2;
+      }
+      // ---- q3 at (theta_m, r_m, z_c), k >= 2 ----
+      if (k >= 2) {
+        TW qnn = (q3n + q3c) * (q2n + q2nkm);
+        TW qss = (q3jm + q3c) * (q2c + q2km);
+        TW h32 = (qnn - qss) * h1d22d;
+        qnn = (q3c + q3ip) * (q1ip + q1ipkm);
+        qss = (q3im + q3c) * (q1c + q1km);
+        TW h31 = (qnn - qss) * h1d11;
+        qnn = (TW)0.25 * (q3c + q3kp) * (q3c + q3kp);
+        qss = (TW)0.25 * (q3c + q3km) * (q3c + q3km);
+        TW h33 = (qnn - qss) * dx3;
+        TW H3 = -(h31 + h32 + h33);
+        TW d11 = (q3ip - (TW)2 * q3c + q3im) * s1t11;
+        TW d33 = (q3kp - (TW)2 * q3c + q3km) * dx3q;
+        TW qn2 = (q3n - q3c) * s3t2n;
+        TW qs2 = (q3c - q3jm) * s3t2s;
+        TW d22 = qn2 - qs2;
+        v3 = (ga * H3 + ro * ru3[o] - al * prg3[pc] + aRe * (d11 + d33 + d22)) * dt;
+        ru3[o] = H3;
+      }
+    }
+    // ---- periodic theta solves by PCR (ping-pong buffers: one barrier per level) ----
+    const long S3 = 3L * RJ * NTP;
+    TW *f1 = fb + tj * NTP, *f2 = f1 + RJ * NTP, *f3 = f2 + RJ * NTP;
+    f1[ti] = v1; f2[ti] = v2; f3[ti] = v3;
+    __syncthreads();
+    int sm = 1 % nt;
+    for (int lvl = 0; lvl < nlev; ++lvl) {
+      const TW al1 = (TW)pcr[(tj * kMaxPcrLevels + lvl) * 2], al2 = (TW)pcr[(tj * kMaxPcrLevels + lvl) * 2 + 1];
+      TW n1 = v1, n2 = v2, n3 = v3;
+      if (ti < nt) {
+        int imv = ti - sm; if (imv < 0) imv += nt;
+        int ipv = ti + sm; if (ipv >= nt) ipv -= nt;
+        n1 = v1 + al1 * (f1[imv] + f1[ipv]);
+        n2 = v2 + al2 * (f2[imv] + f2[ipv]);
+        n3 = v3 + al1 * (f3[imv] + f3[ipv]);
+      }
+      sm = (2 * sm) % nt;
+      const long flip = (lvl & 1) ? -S3 : S3;
+      f1 += flip; f2 += flip; f3 += flip;
+      f1[ti] = n1; f2[ti] = n2; f3[ti] = n3;
+      v1 = n1; v2 = n2; v3 = n3;
+      __syncthreads();
+    }
+    if (cell) {
+      const TW d1 = (TW)fin[tj * 2], d2 = (TW)fin[tj * 2 + 1];
+      r1[o] = v1 * d1;
+      if (j >= 2) r2[o] = v2 * d2;
+      if (k >= 2) r3[o] = v3 * d1;
+    }
+    sM = s0; s0 = sP; sP = (sP == 2) ? 0 : sP + 1;
+    __syncthreads();
+  }
+#undef h1d11
+#undef h1d22d
+#undef h1d22n
+#undef h1dare
+#undef h2d11
+#undef h2d22p
+#undef h2d22m
+#undef h2dare
+#undef s1t11
+#undef s1t2n
+#undef s1t2s
+#undef s1t2c
+#undef s2t11
+#undef s2t2p
+#undef s2t2c
+#undef s2t2m
+#undef s3t2n
+#undef s3t2s
+#undef irmn
+#undef irm
+#undef irmm
+}
+
+// Variant with one z plane per block (no z marching): every stencil operand
+// is read through L1/L2.  Fewer registers and LDS than the marching form;
+// the autotuner picks whichever is faster for the precision and grid.
+template <typename TW>
+__global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk, double betadx, int nlev,
                                                    const double *__restrict__ q1, const double *__restrict__ q2,
                                                    const double *__restrict__ q3, const TW *__restrict__ prg1,
                                                    const TW *__restrict__ prg2, const TW *__restrict__ prg3,
@@ -154,6 +457,8 @@ __global__ __launch_bounds__(1024) void k_explicit(VGrid g, VCoef c, VRK rk, dou
   const int N = g.nt;
   const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
   __syncthreads();
+  // ping-pong buffers: one barrier per reduction level
+  const long S3 = 3L * RJ * NTP;
   int sm = 1 % N;
   for (int lvl = 0; lvl < nlev; ++lvl) {
     const double al1 = -a1 / b1, al2 = -a2 / b2;
@@ -168,7 +473,8 @@ __global__ __launch_bounds__(1024) void k_explicit(VGrid g, VCoef c, VRK rk, dou
     b1 = b1 + 2.0 * al1 * a1; a1 = al1 * a1;
     b2 = b2 + 2.0 * al2 * a2; a2 = al2 * a2;
     sm = (2 * sm) % N;
-    __syncthreads();
+    const long flip = (lvl & 1) ? -S3 : S3;
+    f1 += flip; f2 += flip; f3 += flip;
     f1[ti] = n1; f2[ti] = n2; f3[ti] = n3;
     v1 = n1; v2 = n2; v3 = n3;
     __syncthreads();
@@ -200,6 +506,7 @@ struct LineArgs {
   void *r;                     // qrhs of the component (TW)
   const double *corr3n;        // outlet row of the q3 z sweep
   double c3s;                  // inlet row value of the q3 z sweep (uniform in theta)
+  const double *c3s_dev;       // same value read from the device (captured steps), or null
   int jlim;                    // inlet rows: j <= jlim
 };
 
@@ -236,7 +543,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_pe
         } else if (MODE == SW_Z1 || MODE == SW_Z2) {
           if (p != 0 && p != n - 1) rv = rr[base + step * p];
         } else if (MODE == SW_Z3) {
-          if (p == 0) rv = (jz <= a.jlim) ? (TW)a.c3s : (TW)0;
+          if (p == 0) rv = (jz <= a.jlim) ? (TW)(a.c3s_dev ? *a.c3s_dev : a.c3s) : (TW)0;
           else if (p == n - 1) rv = (TW)a.corr3n[(long)(i - 1) + (long)g.nt * jz];
           else rv = rr[base + step * p];
         } else {
@@ -672,101 +979,165 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
 }
 
 // ---------------------------------------------------------------------------
-// upsol (divergence.F90:69-236) fused with uppres (divergence.F90:239-480):
-// one thread per (theta, j=0..nrc, k=1..nzc); the gradient qcap of each
-// component is recomputed from dph where the reference stores it.
+// upsol (divergence.F90:69-236) fused with uppres (divergence.F90:239-480).
+// A block owns a 64 (theta) x R (r) column tile and marches up a chunk of z
+// planes.  The planes k-2..k+1 of dph live in a 4-slot LDS ring (with two
+// theta columns and two r rows of halo on the low side, one on the high
+// side), so each dph value is fetched from HBM about once; the gradient
+// components qcap are recomputed from the ring where the reference stores
+// them.  Everything that depends on the thread's r row only (metric ratios,
+// reciprocals of rm) is hoisted out of the z loop.
 // ---------------------------------------------------------------------------
-template <typename TW>
-__global__ __launch_bounds__(256) void k_project(VGrid g, VCoef c, double adt, double be,
-                                                 const double *__restrict__ dph, const double *__restrict__ dq1,
-                                                 const double *__restrict__ dq2, const double *__restrict__ dq3,
-                                                 double *__restrict__ q1, double *__restrict__ q2,
-                                                 double *__restrict__ q3, TW *__restrict__ prg1,
-                                                 TW *__restrict__ prg2, TW *__restrict__ prg3,
-                                                 double *__restrict__ pres, int *__restrict__ flag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;  // 0..nrc
-  const int k = blockIdx.z + 1;                        // 1..nzc
-  if (i > g.nt || j > g.nrc) return;
+template <typename TW, int R>
+__global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt, double be, int kchunk,
+                                                    const double *__restrict__ dph, const double *__restrict__ dq1,
+                                                    const double *__restrict__ dq2, const double *__restrict__ dq3,
+                                                    double *__restrict__ q1, double *__restrict__ q2,
+                                                    double *__restrict__ q3, TW *__restrict__ prg1,
+                                                    TW *__restrict__ prg2, TW *__restrict__ prg3,
+                                                    double *__restrict__ pres, int *__restrict__ flag) {
+  constexpr int W = 64, LC = W + 3, LR = R + 3, NE = LR * LC, NT = 64 * R;
+  constexpr int NL = (NE + NT - 1) / NT;
+  __shared__ double ring[4][LR][LC + 1];
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * W + tx;
   const int nt = g.nt, nr = g.nr, nz = g.nz, nrc = g.nrc, nzc = g.nzc;
-  const int ip = (i == nt) ? 1 : i + 1, im = (i == 1) ? nt : i - 1;
-  const long o = hidx(g, i, j, k);
   const long sj = g.sj, sk = g.sk;
-#define PH(di, dj, dk) dph[o + (long)(di) + (long)(dj) * sj + (long)(dk) * sk]
-  // ---- upsol ----
-  if (j >= 1 && j <= nr) {
-    q1[o] = dq1[o] - (adt * g.dx1 / c.rm[j]) * (PH(0, 0, 0) - PH(im - i, 0, 0));
-    if (j == nr) q1[o + sj] = g.slip * q1[o];
+  const int i0 = blockIdx.x * W;  // 0-based theta of the tile's first column
+  const int j0 = blockIdx.y * R;  // rows 0..nrc (row 0 only takes the inlet copy of q3)
+  const int kb = 1 + blockIdx.z * kchunk;
+  const int ke = min(kb + kchunk - 1, nzc);
+  // ring fill: element e = tid + l*NT of the (LR x LC) plane tile
+  long goff[NL];
+  int lrow[NL], lcol[NL];
+  bool lval[NL];
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    const int e = tid + l * NT;
+    const int r = e / LC, cc = e - r * LC;
+    const int jj = j0 - 2 + r;
+    int ii = i0 - 2 + cc;
+    ii %= nt;
+    if (ii < 0) ii += nt;
+    lrow[l] = r; lcol[l] = cc;
+    lval[l] = (e < NE) && jj >= 0 && jj <= nr + 2;
+    goff[l] = (long)ii + (long)jj * sj;
   }
-  if (j >= 2 && j <= nr) q2[o] = dq2[o] - adt * c.rc[j] * g.dx2 * (PH(0, 0, 0) - PH(0, -1, 0)) / 1.0;
-  if (j == 1 || j == nrc) q2[o] = 0.0;
-  if (k == 1) q3[o] = dq3[o];
-  if (k >= 2 && j >= 1 && j <= nr) {
-    const double v = dq3[o] - adt * g.dx3 * (PH(0, 0, 0) - PH(0, 0, -1));
-    q3[o] = v;
-    if (j == nr) q3[o + sj] = g.slip * v;
-  }
-  if (k > nz || j < 1 || j > nr) return;
-  // ---- uppres ----
-  const int jm = (j == 1) ? 1 : j - 1, jp = (j == nr) ? nr : j + 1;
-  const int km = (k == 1) ? 1 : ((k == nz) ? nz : k - 1), kp = k + 1;
-  const long pc = cidx(g, i, j, k);
-  // theta component: qcap_t(i,j,k) = (dph(i)-dph(i-1)) dx1/rm(j), j=1..nr, k=1..nzc
-  auto qt = [&](int ii, int jj, int kk) {
-    const int iim = (ii == 1) ? nt : ii - 1;
-    const long oo = hidx(g, ii, jj, kk);
-    return (dph[oo] - dph[oo + (iim - ii)]) * g.dx1 / c.rm[jj];
+  double pre[NL];
+  auto fetch = [&](int kk) {
+#pragma unroll
+    for (int l = 0; l < NL; l++)
+      pre[l] = (lval[l] && kk >= 0 && kk <= nz + 2) ? dph[goff[l] + (long)kk * sk] : 0.0;
   };
-  if (nt > 1) {
-    const double qc0 = qt(i, j, k);
-    const double d11 = (qt(ip, j, k) - 2.0 * qc0 + qt(im, j, k)) * c.s1t11[j];
-    const double d33 = (qt(i, j, kp) - 2.0 * qc0 + qt(i, j, km)) * g.dx3q;
-    const double qnn = (qt(i, jp, k) / c.rm[j + 1] - qc0 / c.rm[j]);
-    const double qss = (qc0 / c.rm[j] - qt(i, jm, k) / c.rm[jm]);
-    const double d22 = (qnn * c.s1t2n[j] - qss * c.s1t2s[j]) * c.s1t2c[j];
-    prg1[pc] = (TW)((double)prg1[pc] + qc0 - be * (d11 + d33 + d22));
-  }
-  // r component: qcap_r(j) = (dph(j)-dph(j-1)) dx2 rc(j), j=2..nr; 0 at j=1, nrc
-  auto qr = [&](int ii, int jj, int kk) {
-    if (jj <= 1 || jj >= nrc) return 0.0;
-    const long oo = hidx(g, ii, jj, kk);
-    return (dph[oo] - dph[oo - sj]) * g.dx2 * c.rc[jj] / 1.0;
+  auto stash = [&](int kk) {
+#pragma unroll
+    for (int l = 0; l < NL; l++)
+      if (tid + l * NT < NE) ring[kk & 3][lrow[l]][lcol[l]] = pre[l];
   };
-  if (j >= 2) {
-    const double qc0 = qr(i, j, k);
-    const double d11 = (qr(ip, j, k) - 2.0 * qc0 + qr(im, j, k)) * c.s2t11[j];
-    const double d33 = (qr(i, j, kp) - 2.0 * qc0 + qr(i, j, km)) * g.dx3q;
-    const double qnn = qr(i, j + 1, k) - qc0, qss = qc0 - qr(i, j - 1, k);
-    const double d22 = qnn * c.s2t2p[j] - qss * c.s2t2m[j];
-    prg2[pc] = (TW)((double)prg2[pc] + qc0 - be * (d11 + d22 + d33));
+  fetch(kb - 2); stash(kb - 2);
+  fetch(kb - 1); stash(kb - 1);
+  fetch(kb); stash(kb);
+  fetch(kb + 1);
+  // the thread's cell column
+  const int i = i0 + tx + 1, j = j0 + ty;
+  const bool act = (i <= nt) && (j <= nrc);
+  const int jr = (j <= nrc) ? j : nrc;           // clamped row for coefficient loads
+  const int jq = (jr >= 1 && jr <= nr) ? jr : 1;  // rows of the uppres coefficients
+  const int djp = (jq == nr) ? 0 : 1, djm = (jq == 1) ? 0 : -1;
+  const double a1 = adt * g.dx1 / c.rm[jq], a2 = adt * c.rc[jr] * g.dx2, a3 = adt * g.dx3;
+  const double irm = 1.0 / c.rm[jq], irmp = 1.0 / c.rm[jq + djp], irmm = 1.0 / c.rm[jq + djm];
+  const double irmn = 1.0 / c.rm[jq + 1];
+  const double s1t11 = c.s1t11[jq], s1t2n = c.s1t2n[jq], s1t2s = c.s1t2s[jq], s1t2c = c.s1t2c[jq];
+  const double s2t11 = c.s2t11[jq], s2t2p = c.s2t2p[jq], s2t2m = c.s2t2m[jq];
+  const double s3t2n = c.s3t2n[jq], s3t2s = c.s3t2s[jq];
+  const double rcj = c.rc[jq], rcp = c.rc[min(jq + 1, nrc)], rcm = c.rc[max(jq - 1, 1)];
+  const double dx1 = g.dx1, dx2 = g.dx2, dx3 = g.dx3, dx3q = g.dx3q, slip = g.slip;
+  const int cx = tx + 2, cy = ty + 2;
+  for (int k = kb; k <= ke; k++) {
+    stash(k + 1);
+    __syncthreads();
+    fetch(k + 2);
+    if (act) {
+      const double(*P0)[LC + 1] = ring[k & 3];
+      const double(*Pm)[LC + 1] = ring[(k - 1) & 3];
+      const double(*Pp)[LC + 1] = ring[(k + 1) & 3];
+      const double(*Pm2)[LC + 1] = ring[(k - 2) & 3];
+      // kmv quirk of uppres: km = 1 at k = 1 and km = nz at k = nz
+      const double(*Pkm)[LC + 1] = (k == 1 || k == nz) ? P0 : Pm;
+      const long o = (long)(i - 1) + (long)j * sj + (long)k * sk;
+      const double p0 = P0[cy][cx];
+      // ---- upsol ----
+      if (j >= 1 && j <= nr) {
+        const double v = dq1[o] - a1 * (p0 - P0[cy][cx - 1]);
+        q1[o] = v;
+        if (j == nr) q1[o + sj] = slip * v;
+      }
+      if (j >= 2 && j <= nr) q2[o] = dq2[o] - a2 * (p0 - P0[cy - 1][cx]) / 1.0;
+      if (j == 1 || j == nrc) q2[o] = 0.0;
+      if (k == 1) q3[o] = dq3[o];
+      if (k >= 2 && j >= 1 && j <= nr) {
+        const double v = dq3[o] - a3 * (p0 - Pm[cy][cx]);
+        q3[o] = v;
+        if (j == nr) q3[o + sj] = slip * v;
+      }
+      if (k <= nz && j >= 1 && j <= nr) {
+        const long pc = (long)(i - 1) + (long)nt * ((long)(j - 1) + (long)nr * (long)(k - 1));
+        // theta component: qcap = (dph(i) - dph(i-1)) dx1 / rm
+        {
+          const double qc0 = (p0 - P0[cy][cx - 1]) * dx1 * irm;
+          const double qip = (P0[cy][cx + 1] - p0) * dx1 * irm;
+          const double qim = (P0[cy][cx - 1] - P0[cy][cx - 2]) * dx1 * irm;
+          const double qkp = (Pp[cy][cx] - Pp[cy][cx - 1]) * dx1 * irm;
+          const double qkm = (Pkm[cy][cx] - Pkm[cy][cx - 1]) * dx1 * irm;
+          const double qjp = (P0[cy + djp][cx] - P0[cy + djp][cx - 1]) * dx1 * irmp;
+          const double qjm = (P0[cy + djm][cx] - P0[cy + djm][cx - 1]) * dx1 * irmm;
+          const double d11 = (qip - 2.0 * qc0 + qim) * s1t11;
+          const double d33 = (qkp - 2.0 * qc0 + qkm) * dx3q;
+          const double qnn = qjp * irmn - qc0 * irm;
+          const double qss = qc0 * irm - qjm * irmm;
+          const double d22 = (qnn * s1t2n - qss * s1t2s) * s1t2c;
+          prg1[pc] = (TW)((double)prg1[pc] + qc0 - be * (d11 + d33 + d22));
+        }
+        // r component: qcap = (dph(j) - dph(j-1)) dx2 rc, zero at j = 1 and nrc
+        if (j >= 2) {
+          const double qc0 = (p0 - P0[cy - 1][cx]) * dx2 * rcj;
+          const double qip = (P0[cy][cx + 1] - P0[cy - 1][cx + 1]) * dx2 * rcj;
+          const double qim = (P0[cy][cx - 1] - P0[cy - 1][cx - 1]) * dx2 * rcj;
+          const double qkp = (Pp[cy][cx] - Pp[cy - 1][cx]) * dx2 * rcj;
+          const double qkm = (Pkm[cy][cx] - Pkm[cy - 1][cx]) * dx2 * rcj;
+          const double qjp = (j + 1 < nrc) ? (P0[cy + 1][cx] - p0) * dx2 * rcp : 0.0;
+          const double qjm = (j - 1 > 1) ? (P0[cy - 1][cx] - P0[cy - 2][cx]) * dx2 * rcm : 0.0;
+          const double d11 = (qip - 2.0 * qc0 + qim) * s2t11;
+          const double d33 = (qkp - 2.0 * qc0 + qkm) * dx3q;
+          const double d22 = (qjp - qc0) * s2t2p - (qc0 - qjm) * s2t2m;
+          prg2[pc] = (TW)((double)prg2[pc] + qc0 - be * (d11 + d22 + d33));
+        }
+        // z component: qcap = (dph(k) - dph(k-1)) dx3, zero at k = 1
+        {
+          const double qkp = (Pp[cy][cx] - p0) * dx3;
+          if (qkp != qkp) atomicOr(flag, 1);  // "Flow field crash" (divergence.F90:396-399)
+          if (k >= 2) {
+            const double qc0 = (p0 - Pm[cy][cx]) * dx3;
+            const double qip = (P0[cy][cx + 1] - Pm[cy][cx + 1]) * dx3;
+            const double qim = (P0[cy][cx - 1] - Pm[cy][cx - 1]) * dx3;
+            const double qkm = (k - 1 > 1) ? (Pm[cy][cx] - Pm2[cy][cx]) * dx3 : 0.0;
+            const double qjp = (P0[cy + djp][cx] - Pm[cy + djp][cx]) * dx3;
+            const double qjm = (P0[cy + djm][cx] - Pm[cy + djm][cx]) * dx3;
+            const double d11 = (qip - 2.0 * qc0 + qim) * s1t11;
+            const double d33 = (qkp - 2.0 * qc0 + qkm) * dx3q;
+            const double d22 = (qjp - qc0) * s3t2n - (qc0 - qjm) * s3t2s;
+            prg3[pc] = (TW)((double)prg3[pc] + qc0 - be * (d11 + d22 + d33));
+          }
+        }
+        {
+          const double d11 = (P0[cy][cx + 1] + P0[cy][cx - 1] - 2.0 * p0) * s1t11;
+          const double d22 = (P0[cy + djp][cx] - p0) * s3t2n - (p0 - P0[cy + djm][cx]) * s3t2s;
+          const double d33 = (Pp[cy][cx] + Pkm[cy][cx] - 2.0 * p0) * dx3q;
+          pres[o] = pres[o] + p0 - be * (d11 + d22 + d33);
+        }
+      }
+    }
+    __syncthreads();
   }
-  // z component: qcap_z(k) = (dph(k)-dph(k-1)) dx3, k=2..nzc; 0 at k=1
-  auto qz = [&](int ii, int jj, int kk) {
-    if (kk <= 1) return 0.0;
-    const long oo = hidx(g, ii, jj, kk);
-    return (dph[oo] - dph[oo - sk]) * g.dx3;
-  };
-  {
-    const double qzc = qz(i, j, k + 1);
-    if (qzc != qzc) atomicOr(flag, 1);  // "Flow field crash" (divergence.F90:396-399)
-  }
-  if (k >= 2) {
-    const double qc0 = qz(i, j, k);
-    const double d11 = (qz(ip, j, k) - 2.0 * qc0 + qz(im, j, k)) * c.s1t11[j];
-    const double d33 = (qz(i, j, k + 1) - 2.0 * qc0 + qz(i, j, k - 1)) * g.dx3q;
-    const double qnn = (qz(i, jp, k) - qc0), qss = (qc0 - qz(i, jm, k));
-    const double d22 = qnn * c.s3t2n[j] - qss * c.s3t2s[j];
-    prg3[pc] = (TW)((double)prg3[pc] + qc0 - be * (d11 + d22 + d33));
-  }
-  {
-    const double p0 = PH(0, 0, 0);
-    const double d11 = (PH(ip - i, 0, 0) + PH(im - i, 0, 0) - 2.0 * p0) * c.s1t11[j];
-    const double qnn = PH(0, jp - j, 0) - p0, qss = p0 - PH(0, jm - j, 0);
-    const double d22 = qnn * c.s3t2n[j] - qss * c.s3t2s[j];
-    const double d33 = (PH(0, 0, kp - k) + PH(0, 0, km - k) - 2.0 * p0) * g.dx3q;
-    pres[o] = pres[o] + p0 - be * (d11 + d22 + d33);
-  }
-#undef PH
 }
 
 }  // namespace
@@ -776,13 +1147,17 @@ __global__ __launch_bounds__(256) void k_project(VGrid g, VCoef c, double adt, d
 // ===========================================================================
 #define CHECK_LAUNCH() (void)hipGetLastError()
 
-template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, double c3s) {
-  (void)c3s;
+template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, const double *cor_dev) {
   const VGrid &g = h->g;
   dim3 blk(64), grd((g.nt + 63) / 64, g.nrc);
   KTimer kt(h, KT_HDNBC);
-  hipLaunchKernelGGL(k_hdnbc, grd, blk, 0, h->stream, g, h->coef, h->rk[irk].g, h->rk[irk].r, cor, h->f.q3,
-                     h->f.csort, h->f.dq3no, h->f.corr3n);
+  hipLaunchKernelGGL(k_hdnbc, grd, blk, 0, h->stream, g, h->coef, h->rk[irk].g, h->rk[irk].r, cor, cor_dev,
+                     h->f.q3, h->f.csort, h->f.dq3no, h->f.corr3n);
+}
+
+void launch_scalars(vrh_t *h) {
+  hipLaunchKernelGGL(k_scalars, dim3(1), dim3(64), 0, h->stream, h->d_sc, h->dt, h->rk[1].a, h->rk[2].a,
+                     h->rk[3].a, h->inlet_area, h->cor_red);
 }
 
 template <typename TW> void launch_explicit(vrh_t *h, int irk) {
@@ -790,16 +1165,36 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
   int ntp = 1;
   while (ntp < g.nt) ntp <<= 1;
   if (ntp < 64) ntp = 64;
-  int rj = 256 / ntp;
-  if (rj < 1) rj = 1;
   const double betadx = 0.5 * h->rk[irk].a * h->dt / h->Re;
   const int nlev = h->theta_levels[irk];
-  dim3 blk(ntp, rj), grd((g.nr + rj - 1) / rj, g.nz);
-  size_t sh = sizeof(TW) * 3 * (size_t)rj * ntp;
+  const int var = h->explicit_variant >= 0 ? h->explicit_variant : (sizeof(TW) == 8 ? 0 : 1);
   KTimer kt(h, KT_EXPLICIT);
-  hipLaunchKernelGGL(k_explicit<TW>, grd, blk, sh, h->stream, g, h->coef, h->rk[irk], betadx, nlev, h->f.q1,
-                     h->f.q2, h->f.q3, (const TW *)h->f.prg1, (const TW *)h->f.prg2, (const TW *)h->f.prg3,
-                     (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1, (TW *)h->f.r2, (TW *)h->f.r3);
+  if (var == 0) {
+    // one z plane per block
+    int rj = 256 / ntp;
+    if (rj < 1) rj = 1;
+    dim3 blk(ntp, rj), grd((g.nr + rj - 1) / rj, g.nz);
+    const size_t sh = sizeof(TW) * 6 * (size_t)rj * ntp;
+    hipLaunchKernelGGL(k_explicit_cell<TW>, grd, blk, sh, h->stream, g, h->coef, h->rk[irk], betadx, nlev,
+                       h->f.q1, h->f.q2, h->f.q3, (const TW *)h->f.prg1, (const TW *)h->f.prg2,
+                       (const TW *)h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,
+                       (TW *)h->f.r2, (TW *)h->f.r3);
+    return;
+  }
+  // z marching: rows per block for 512 threads within 64 KB of LDS
+  auto lds = [&](int rj) {
+    return sizeof(double) * (size_t)rj * (kMaxPcrLevels * 2 + 2) +
+           sizeof(TW) * ((size_t)rj * kRowCoefs + (size_t)9 * (rj + 2) * ntp + (size_t)6 * rj * ntp);
+  };
+  int rj = 512 / ntp;
+  if (rj < 1) rj = 1;
+  while (rj > 1 && lds(rj) > 65536) rj--;
+  const int kchunk = 16;
+  dim3 blk(ntp, rj), grd((g.nr + rj - 1) / rj, (g.nz + kchunk - 1) / kchunk);
+  hipLaunchKernelGGL(k_explicit_march<TW>, grd, blk, lds(rj), h->stream, g, h->coef, h->rk[irk], betadx, nlev,
+                     kchunk, h->f.q1, h->f.q2, h->f.q3, (const TW *)h->f.prg1, (const TW *)h->f.prg2,
+                     (const TW *)h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,
+                     (TW *)h->f.r2, (TW *)h->f.r3);
 }
 
 template <typename TW, int MODE>
@@ -817,13 +1212,14 @@ static void line_launch(vrh_t *h, const LineArgs &a, int M) {
   }
 }
 
-template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s) {
+template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const double *c3s_dev) {
   const VGrid &g = h->g;
   LineArgs a{};
   a.g = g;
   a.slip = h->slip;
   a.jlim = h->jlim;
   a.c3s = c3s;
+  a.c3s_dev = c3s_dev;
   a.corr3n = h->f.corr3n;
   // q1: r then z (step.F90:75-78)
   a.lc = h->lc[irk][SW_R1]; a.nlines = (long)g.nt * g.nz; a.r = h->f.r1;
@@ -914,19 +1310,19 @@ template <typename TW> void launch_project(vrh_t *h, int irk) {
   const VGrid &g = h->g;
   const double adt = h->rk[irk].a * h->dt;
   const double be = 0.5 * (h->rk[irk].a * h->dt) / h->Re;
-  int bx = g.nt >= 64 ? 64 : g.nt;
-  int by = 256 / bx;
-  dim3 blk(bx, by), grd((g.nt + bx - 1) / bx, (g.nr + 2 + by - 1) / by, g.nzc);
+  constexpr int R = 4;
+  const int kchunk = 32;
+  dim3 blk(64, R), grd((g.nt + 63) / 64, (g.nrc + 1 + R - 1) / R, (g.nzc + kchunk - 1) / kchunk);
   KTimer kt(h, KT_PROJECT);
-  hipLaunchKernelGGL(k_project<TW>, grd, blk, 0, h->stream, g, h->coef, adt, be, h->f.dph, h->f.dq1, h->f.dq2,
-                     h->f.dq3, h->f.q1, h->f.q2, h->f.q3, (TW *)h->f.prg1, (TW *)h->f.prg2, (TW *)h->f.prg3,
-                     h->f.pres, h->f.flag);
+  hipLaunchKernelGGL((k_project<TW, R>), grd, blk, 0, h->stream, g, h->coef, adt, be, kchunk, h->f.dph, h->f.dq1,
+                     h->f.dq2, h->f.dq3, h->f.q1, h->f.q2, h->f.q3, (TW *)h->f.prg1, (TW *)h->f.prg2,
+                     (TW *)h->f.prg3, h->f.pres, h->f.flag);
 }
 
 #define INST(TW)                                                   \
-  template void launch_hdnbc<TW>(vrh_t *, int, double, double);    \
+  template void launch_hdnbc<TW>(vrh_t *, int, double, const double *); \
   template void launch_explicit<TW>(vrh_t *, int);                 \
-  template void launch_sweeps<TW>(vrh_t *, int, double);           \
+  template void launch_sweeps<TW>(vrh_t *, int, double, const double *); \
   template void launch_poisson<TW>(vrh_t *, int, const double *, double *); \
   template void launch_project<TW>(vrh_t *, int);
 INST(double)

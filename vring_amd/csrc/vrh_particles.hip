@@ -360,6 +360,7 @@ void launch_particles(vrh_t *h, int irk) {
 extern "C" {
 
 int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st, const int *nsub, int scheme) {
+  if (h) graph_invalidate(h);
   if (!h || nbatch < 1 || !cap || !st || !nsub) return -1;
   if (scheme < VRH_PART_LSQ_EULER || scheme > VRH_PART_TRILINEAR_RK3) { h->err = "unknown particle scheme"; return -1; }
   if (!h->pb.empty()) { h->err = "particles already created"; return -1; }
@@ -385,6 +386,7 @@ int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st
 }
 
 int vrh_particles_upload(vrh_t *h, int batch, long n, const double *u, const int *alive) {
+  if (h) graph_invalidate(h);
   if (!h || batch < 0 || batch >= h->nbatch || !u || n < 0) return -1;
   PartBatch &B = h->pb[batch];
   if (n > B.cap) { h->err = "more particles than the batch capacity"; return -1; }

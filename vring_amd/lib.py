@@ -33,7 +33,9 @@ EXPORTS = ["vrh_create", "vrh_destroy", "vrh_last_error", "vrh_upload_fields", "
            "vrh_upload_state", "vrh_download_state", "vrh_set_time", "vrh_get_time", "vrh_substep",
            "vrh_step", "vrh_phase", "vrh_poisson_solve", "vrh_synchronize", "vrh_device_ptr",
            "vrh_ring_init", "vrh_cfl_dt", "vrh_particles_create", "vrh_particles_upload",
-           "vrh_particles_download", "vrh_particles_step", "vrh_ktimer_enable", "vrh_ktimer_read", "vrh_autotune"]
+           "vrh_particles_download", "vrh_particles_step", "vrh_ktimer_enable", "vrh_ktimer_read", "vrh_autotune",
+           "vrh_set_graph", "vrh_set_tuning", "vrh_ensemble_create", "vrh_ensemble_step", "vrh_ensemble_last_error",
+           "vrh_ensemble_destroy"]
 
 _lib = None
 
@@ -74,6 +76,12 @@ def load_library(path=LIBPATH):
     L.vrh_ktimer_enable.argtypes = [P, I]
     L.vrh_ktimer_read.argtypes = [P, I, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(D),
                                   ctypes.POINTER(ctypes.c_long)]
+    L.vrh_set_graph.argtypes = [P, I]
+    L.vrh_set_tuning.argtypes = [P, ip]
+    L.vrh_ensemble_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
+    L.vrh_ensemble_step.argtypes = [P, I]
+    L.vrh_ensemble_last_error.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
+    L.vrh_ensemble_destroy.argtypes = [P]
     _lib = L
     return L
 
@@ -206,6 +214,11 @@ class Solver:
     def synchronize(self):
         self._check(self.L.vrh_synchronize(self.h), "synchronize")
 
+    def set_graph(self, on=True):
+        """vrh_step replays a captured HIP graph of one RK3 step (default) or
+        launches the kernels one by one."""
+        self._check(self.L.vrh_set_graph(self.h, 1 if on else 0), "set_graph")
+
     def poisson(self, rhs):
         r = _f64(rhs)
         out = np.zeros_like(r)
@@ -231,10 +244,15 @@ class Solver:
     def autotune(self, reps=3):
         """Times the candidate line-solver segment lengths and FFT/DCT tile
         sizes on the GPU and keeps the fastest; returns the choice."""
-        ch = np.zeros(8, np.int32)
+        ch = np.zeros(9, np.int32)
         self._check(self.L.vrh_autotune(self.h, int(reps), ch), "autotune")
-        keys = ["M_r1", "M_r3", "M_r2", "M_z1", "M_z3", "M_z2", "fft_lines", "dct_lines"]
+        keys = ["M_r1", "M_r3", "M_r2", "M_z1", "M_z3", "M_z2", "fft_lines", "dct_lines", "explicit_form"]
         return dict(zip(keys, ch.tolist()))
+
+    def set_tuning(self, M=(0, 0, 0, 0, 0, 0), fft_lines=0, dct_lines=0, explicit_form=-1):
+        """Apply launch shapes without timing (the values vrh_autotune returns)."""
+        ch = np.array(list(M) + [fft_lines, dct_lines, explicit_form], np.int32)
+        self._check(self.L.vrh_set_tuning(self.h, ch), "set_tuning")
 
     # ---- particles (part_tools.F90) ----
     def particles_create(self, caps, st, nsub, scheme="lsq_euler"):
@@ -278,3 +296,37 @@ class Solver:
         p = ctypes.c_void_p()
         self._check(self.L.vrh_device_ptr(self.h, name.encode(), ctypes.byref(p)), "device_ptr")
         return p.value
+
+
+class Ensemble:
+    """Independent rings (Solver objects of any size, Re and precision)
+    advanced together: one HIP graph launch per RK3 step for all members."""
+
+    def __init__(self, solvers):
+        self.L = load_library()
+        self.members = list(solvers)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        self.e = ctypes.c_void_p()
+        rc = self.L.vrh_ensemble_create(arr, len(self.members), ctypes.byref(self.e))
+        self._check(rc, "vrh_ensemble_create")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            buf = ctypes.create_string_buffer(512)
+            if self.e:
+                self.L.vrh_ensemble_last_error(self.e, buf, 512)
+            raise RuntimeError(f"{what} failed ({rc}): {buf.value.decode(errors='replace')}")
+
+    def step(self, n=1):
+        self._check(self.L.vrh_ensemble_step(self.e, int(n)), "ensemble_step")
+
+    def close(self):
+        if self.e:
+            self.L.vrh_ensemble_destroy(self.e)
+            self.e = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
