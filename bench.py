@@ -109,6 +109,79 @@ def run_particles(prec, q, dt, n, steps):
             "hbm_bytes_per_particle_substep": 2520}
 
 
+def run_ensemble(nrings, steps, warmup, seed=0, particles=50000):
+    """Batched ensemble: nrings independent rings with (nr, ntheta, nz) drawn
+    from {64, 96, 128, 192} x {32, 64, 128} x {128, 256, 512}, Re ~ U[500, 5000],
+    core sigma ~ U[0.1, 0.3] (numpy PCG64(seed)), mixed precision, `particles`
+    trilinear-RK3 particles each, advanced by one graph launch per RK3 step.
+    Returns aggregate Mcell-RK3-steps/s and the same rings stepped one after
+    another with their own graphs."""
+    import vring_amd
+    rng = np.random.Generator(np.random.PCG64(seed))
+    members, shapes, cells = [], [], 0
+    for _ in range(nrings):
+        nr, nt, nz = int(rng.choice([64, 96, 128, 192])), int(rng.choice([32, 64, 128])), int(rng.choice([128, 256, 512]))
+        Re, sigma = float(rng.uniform(500, 5000)), float(rng.uniform(0.1, 0.3))
+        q = vring_amd.ring_initial_condition(nt, nr, nz, RMAX, ZMAX, [(1.0, 3.0, sigma, 1.0)])
+        dt = vring_amd.cfl_dt(nt, nr, nz, RMAX, ZMAX, *q, cfl=0.3, ujet=1.0)
+        s = vring_amd.Solver(nt, nr, nz, RMAX, ZMAX, Re, dt, precision="mixed")
+        for n, a in zip(("q1", "q2", "q3"), q):
+            s.upload(n, a)
+        if particles:
+            s.seed_particles_torus(particles, R=1.0, z0=3.0, a=3 * sigma, seed=len(members), nsub=1,
+                                   scheme="trilinear_rk3")
+        members.append(s)
+        shapes.append((nr, nt, nz))
+        cells += nr * nt * nz
+    ens = vring_amd.Ensemble(members)
+    ens.step(warmup)
+    t0 = time.perf_counter()
+    ens.step(steps)
+    t_ens = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for s in members:
+        s.step(steps)
+    t_seq = time.perf_counter() - t0
+    ens.close()
+    for s in members:
+        s.close()
+    return {"rings": nrings, "cells": cells, "steps": steps, "particles_per_ring": particles,
+            "ms_per_step": round(t_ens / steps * 1e3, 3),
+            "Mcell_RK3_steps_per_s": round(cells * steps / t_ens / 1e6, 2),
+            "one_by_one_Mcell_RK3_steps_per_s": round(cells * steps / t_seq / 1e6, 2),
+            "shapes_nr_nt_nz": shapes}
+
+
+def run_learned(nr, nz, nslots, train_samples, seed=0):
+    """Per-mode (r, z) Helmholtz on the tanh-stretched z grid (stretch 2),
+    modes m = 0..nslots-1: train the network on `train_samples` (rhs, p)
+    pairs the direct solver generates on the GPU, then solve ring-core right-hand
+    sides to 1e-8 relative residual with plain PCG, BiCGSTAB, the learned
+    flexible CG, and the direct solve."""
+    import vring_amd
+    hz = vring_amd.Helmholtz(nr, nz, nslots, RMAX, ZMAX, 2.0)
+    hz.set_modes(np.arange(nslots, dtype=np.int32))
+    hz.net_init(seed)
+    t0 = time.perf_counter()
+    loss = hz.train(train_samples, lr=1e-3, seed=seed)
+    t_train = time.perf_counter() - t0
+    hz.set_modes(np.arange(nslots, dtype=np.int32))
+    f = hz.sample(seed + 99, 0)
+    out = {"nr": nr, "nz": nz, "modes": f"0..{nslots - 1}", "train_samples": train_samples,
+           "train_s": round(t_train, 2), "loss_first": float(loss[:4].mean()), "loss_last": float(loss[-4:].mean())}
+    pd = hz.direct(f)
+    t0 = time.perf_counter()
+    pd = hz.direct(f)
+    out["direct_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+    for m in ("pcg", "bicgstab", "learned"):
+        p, it, rel, ms = hz.solve(f, m, tol=1e-8, maxit=50000)
+        out[m] = {"iterations_total": int(it.sum()), "iterations_max": int(it.max()), "ms": round(ms, 3),
+                  "max_true_relres": float(rel.max()),
+                  "max_err_vs_direct": float(np.abs(p - pd).max() / np.abs(pd).max())}
+    hz.close()
+    return out
+
+
 def rel_errors(a, b):
     vs = max(np.abs(b[n]).max() for n in ("q1", "q2", "q3"))
     out = {n: float(np.abs(a[n] - b[n]).max() / vs) for n in ("q1", "q2", "q3")}
@@ -231,6 +304,10 @@ def main():
                     help="run one path only and print its timing (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="RINGS",
+                    help="run only the batched-ensemble benchmark with RINGS rings")
+    ap.add_argument("--learned", type=int, default=0, metavar="SAMPLES",
+                    help="run only the learned Helmholtz benchmark, training on SAMPLES pairs")
     ap.add_argument("--summarize-traffic", metavar="PMC_DIR", default=None,
                     help="write profiles/traffic_256x128x512.json from rocprofv3 --pmc output under PMC_DIR")
     args = ap.parse_args()
@@ -242,6 +319,12 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.ensemble:
+        print(json.dumps({"ensemble": run_ensemble(args.ensemble, args.steps, args.warmup)}))
+        return
+    if args.learned:
+        print(json.dumps({"learned": run_learned(256, 512, 64, args.learned)}))
         return
     q, dt = ring_state()
     ncell = NT * NR * NZ

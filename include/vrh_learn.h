@@ -52,9 +52,12 @@ int vrh_hz_direct(vrh_hz_t *h, const double *f, double *p);
 /* Iterative solve of every slot to ||V f - S p|| <= tol ||V f||:
  *   VRH_HZ_PCG       Jacobi-preconditioned CG, zero start;
  *   VRH_HZ_BICGSTAB  Jacobi-preconditioned BiCGSTAB, zero start;
- *   VRH_HZ_LEARNED   flexible CG preconditioned by the network, started from
- *                    the network's estimate; a slot that does not converge
- *                    within maxit falls back to plain PCG from where it is.
+ *   VRH_HZ_LEARNED   flexible CG with the preconditioner
+ *                    M(r) = D^-1 r + scale(r) N(r) (Jacobi plus the network's
+ *                    learned correction; an untrained, zero-output network
+ *                    gives plain PCG), started from M(V f); a slot that does
+ *                    not converge within maxit falls back to plain PCG from
+ *                    where it is.
  * The true residual of the returned p is recomputed and reported per slot in
  * relres; iters receives the iterations per slot (fallback included), and
  * ms (optional) the solve time in milliseconds (GPU events). */
@@ -70,7 +73,8 @@ int vrh_hz_net_init(vrh_hz_t *h, unsigned long long seed);
 int vrh_hz_net_get(vrh_hz_t *h, float *params);
 int vrh_hz_net_set(vrh_hz_t *h, const float *params);
 /* Network output channel (nslots, nz, nr) for the features of f, and the
- * feature scale rms(f) dr^2 per slot (the estimate of p is scale * out). */
+ * feature scale rms(f) dr^2 per slot (the learned correction to the Jacobi
+ * estimate D^-1 V f is scale * out). */
 int vrh_hz_net_forward(vrh_hz_t *h, const double *f, float *out, double *scale);
 /* Gradient of sum(gout * out) with respect to the parameters, for the
  * features of f (same packing as the parameters). */
@@ -80,8 +84,9 @@ int vrh_hz_net_grad(vrh_hz_t *h, const double *f, const float *gout, float *grad
 int vrh_hz_sample(vrh_hz_t *h, unsigned long long seed, int family, double *f);
 /* Train on nsamples (rhs, p) pairs generated on the GPU (rhs from both
  * families, random m in 0..63 per slot, p from the direct solve), nslots per
- * batch, Adam with learning rate lr, energy-norm loss
- * (scale out - p)^T S (scale out - p) / (p^T V f).  loss (optional) receives
+ * batch, Adam with learning rate lr, energy-norm loss e^T S e / (p^T V f)
+ * of e = D^-1 V f + scale out - p (features taken from the rhs divided by
+ * the cell volume, as the solver sees residuals).  loss (optional) receives
  * the mean loss of each batch. */
 int vrh_hz_train(vrh_hz_t *h, int nsamples, double lr, unsigned long long seed, float *loss);
 

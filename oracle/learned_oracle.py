@@ -17,7 +17,8 @@ libvrh's Helmholtz subsystem computes, so that tests can pin it:
   path);
 * the 4-layer 3x3 convolutional network (NHWC, bf16 storage of activations,
   weights and back-propagated gradients, fp32 accumulation) with its
-  backward pass and the energy-norm loss.
+  backward pass and the energy-norm loss of the preconditioned estimate
+  D^-1 V f + scale N(f) (Jacobi plus the learned correction).
 
 Only tests/ may import this module.
 """
@@ -313,9 +314,11 @@ def backward(params, acts, gout, rounding=True):
 
 
 def energy_loss_grad(g, m, out, p, f, scale):
-    """Energy-norm loss of one sample, e = scale * out - p:
-    L = e^T S e / (p^T V f);  dL/d out = 2 scale S e / (p^T V f)."""
-    e = scale * out.astype(np.float64) - p
+    """Energy-norm loss of one sample's preconditioned estimate, with
+    b = V f and e = D^-1 b + scale * out - p (Jacobi plus the learned
+    correction):  L = e^T S e / (p^T b);  dL/d out = 2 scale S e / (p^T b)."""
+    b = rhs(g, m, f)
+    e = b / diag(g, m) + scale * out.astype(np.float64) - p
     Se = apply(g, m, e)
-    den = (p * rhs(g, m, f)).sum()
+    den = (p * b).sum()
     return (e * Se).sum() / den, (2.0 * scale / den) * Se

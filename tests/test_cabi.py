@@ -7,13 +7,13 @@ import re
 import numpy as np
 
 import vring_amd
-from vring_amd.lib import EXPORTS, LIBPATH
+from vring_amd.lib import EXPORTS, HZ_EXPORTS, LIBPATH
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
-    with open(os.path.join(ROOT, "include", "vrh.h")) as f:
+def declared_functions(header="vrh.h"):
+    with open(os.path.join(ROOT, "include", header)) as f:
         src = f.read()
     return sorted(set(re.findall(r"^\s*int\s+(vrh_\w+)\s*\(", src, re.M)))
 
@@ -25,6 +25,12 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(EXPORTS), set(names) ^ set(EXPORTS)
+    hz = declared_functions("vrh_learn.h")
+    for n in hz:
+        assert hasattr(lib, n), n
+    assert set(hz) == set(HZ_EXPORTS), set(hz) ^ set(HZ_EXPORTS)
+    headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    assert headers == ["vrh.h", "vrh_learn.h"], headers
 
 
 def test_ring_initial_condition_is_divergence_free_on_host():

@@ -1,0 +1,141 @@
+"""The per-mode (r, z) Helmholtz subsystem on the GPU (include/vrh_learn.h)
+against the CPU oracle (oracle/learned_oracle.py): operator, direct solve,
+Krylov solvers judged by the true residual, network forward pass and
+parameter gradients, training, and what the trained network gains."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import vring_amd
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import learned_oracle as lo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+RMAX, ZMAX, STRETCH = 4.0, 12.0, 2.0
+
+
+def make(nr, nz, modes):
+    hz = vring_amd.Helmholtz(nr, nz, len(modes), RMAX, ZMAX, STRETCH)
+    hz.set_modes(np.array(modes, np.int32))
+    return hz, lo.mesh(nr, nz, RMAX, ZMAX, STRETCH)
+
+
+def test_mesh_and_operator_match_oracle():
+    print("case hz operator 32x48 modes 0 1 7", flush=True)
+    modes = [0, 1, 7]
+    hz, g = make(32, 48, modes)
+    zf, zm, rm = hz.mesh()
+    assert np.abs(zf - g["zf"]).max() < 1e-13 and np.abs(rm - g["rm"]).max() < 1e-15
+    x = np.random.default_rng(0).standard_normal(hz.shape)
+    y = hz.apply(x)
+    for b, m in enumerate(modes):
+        ref = lo.apply(g, m, x[b])
+        assert np.abs(y[b] - ref).max() < 1e-12 * np.abs(ref).max(), m
+    hz.close()
+
+
+def test_direct_solve_matches_oracle():
+    print("case hz direct 32x64 modes 0 2 5 30", flush=True)
+    modes = [0, 2, 5, 30]
+    hz, g = make(32, 64, modes)
+    f = np.random.default_rng(1).standard_normal(hz.shape)
+    p = hz.direct(f)
+    for b, m in enumerate(modes):
+        ref = lo.direct(g, m, f[b])
+        assert np.abs(p[b] - ref).max() < 1e-9 * np.abs(ref).max(), m
+    hz.close()
+
+
+@pytest.mark.parametrize("method", ["pcg", "bicgstab"])
+def test_krylov_solvers_reach_tolerance_by_true_residual(method):
+    print(f"case hz {method} 32x64", flush=True)
+    modes = [0, 1, 4, 16]
+    hz, g = make(32, 64, modes)
+    f = hz.sample(3, -1)
+    p, it, rel, ms = hz.solve(f, method, tol=1e-8, maxit=20000)
+    pd = hz.direct(f)
+    assert np.all(it > 0), it
+    assert np.all(rel <= 1.05e-8), rel
+    for b in range(len(modes)):
+        assert np.abs(p[b] - pd[b]).max() < 1e-5 * np.abs(pd[b]).max()
+    # oracle CG on one slot takes a comparable number of iterations
+    _, it_ref = lo.cg(g, modes[1], f[1], 1e-8, 20000)
+    if method == "pcg":
+        assert abs(int(it[1]) - it_ref) <= max(3, it_ref // 20), (it[1], it_ref)
+    hz.close()
+
+
+def _oracle_params(w):
+    """Unpack the flat fp32 parameter vector (per layer W (9, cout, cin), b (cout))."""
+    Ws, bs, off = [], [], 0
+    for ci, co in lo.LAYERS:
+        n = 9 * co * ci
+        Ws.append(w[off:off + n].reshape(9, co, ci)); off += n
+        bs.append(w[off:off + co]); off += co
+    assert off == w.size
+    return Ws, bs
+
+
+def test_network_forward_and_gradients_match_oracle():
+    print("case hz net grad 16x24", flush=True)
+    modes = [0, 9]
+    hz, g = make(16, 24, modes)
+    rng = np.random.default_rng(4)
+    hz.net_init(11)
+    w = hz.net_get()
+    w = (w + 0.02 * rng.standard_normal(w.size) * (w != 0)).astype(np.float32)
+    Ws, bs = _oracle_params(w)
+    for l in range(3):
+        bs[l][:] = 0.05 * rng.standard_normal(bs[l].size)
+    Ws[3][:, 0, :] += 0.05 * rng.standard_normal(Ws[3][:, 0, :].shape).astype(np.float32)
+    hz.net_set(w)
+    f = rng.standard_normal(hz.shape)
+    out, scale = hz.net_forward(f)
+    gout = rng.standard_normal(hz.shape).astype(np.float32)
+    grads = hz.net_grad(f, gout)
+    gW = np.zeros_like(w)
+    for b, m in enumerate(modes):
+        x, s = lo.features(g, m, f[b])
+        assert abs(scale[b] - s) < 1e-12 * s
+        ref, acts = lo.forward((Ws, bs), x)
+        err = np.linalg.norm(out[b] - ref) / np.linalg.norm(ref)
+        assert err < 1e-2, (m, err)
+        gw, gb = lo.backward((Ws, bs), acts, gout[b])
+        off = 0
+        for l in range(4):
+            n = gw[l].size
+            gW[off:off + n] += gw[l].ravel(); off += n
+            gW[off:off + gb[l].size] += gb[l]; off += gb[l].size
+    off = 0
+    for l, (ci, co) in enumerate(lo.LAYERS):
+        n = 9 * co * ci + co
+        a, r = grads[off:off + n], gW[off:off + n]
+        off += n
+        err = np.linalg.norm(a - r) / np.linalg.norm(r)
+        assert err < 3e-2, (l, err)
+    hz.close()
+
+
+def test_training_lowers_loss_and_learned_solver_gains_over_plain_pcg():
+    print("case hz train 32x64", flush=True)
+    S = 16
+    hz, g = make(32, 64, list(range(S)))
+    hz.net_init(5)
+    loss = hz.train(S * 300, lr=1e-3, seed=7)
+    print("loss first/last", loss[:10].mean(), loss[-30:].mean(), flush=True)
+    assert np.all(np.isfinite(loss))
+    assert loss[-30:].mean() < 0.75 * loss[:10].mean(), (loss[:10], loss[-30:])
+    modes = np.arange(0, 64, 4, dtype=np.int32)
+    hz.set_modes(modes)
+    f = hz.sample(12345, 0)
+    p0, it0, rel0, ms0 = hz.solve(f, "pcg", tol=1e-8, maxit=20000)
+    p1, it1, rel1, ms1 = hz.solve(f, "learned", tol=1e-8, maxit=20000)
+    print("pcg iterations", it0.tolist(), "learned", it1.tolist(), flush=True)
+    assert np.all(rel0 <= 1.05e-8) and np.all(rel1 <= 1.05e-8), (rel0, rel1)
+    assert np.abs(p1 - p0).max() < 1e-5 * np.abs(p0).max()
+    assert it1.sum() < it0.sum(), (it0, it1)
+    hz.close()

@@ -70,3 +70,24 @@ def test_trilinear_rk3_particles_match_oracle():
     g, gal = s.particles_download(0, n)
     np.testing.assert_array_equal(gal, alive)
     assert np.abs(g - xv).max() / np.abs(xv).max() < 1e-11
+
+
+def test_particles_advance_inside_whole_steps():
+    """vrh_step (graph-captured) with trilinear RK3 particles: one particle
+    stage before each fluid substep, against the oracle doing the same."""
+    print("case particles in vrh_step trilinear", flush=True)
+    o, s, u0, st, dt = setup(scheme="trilinear_rk3", nsub=1, n=256)
+    n = u0.shape[0]
+    xv = u0.copy()
+    r2, alive = np.zeros((n, 6)), np.ones(n, np.int32)
+    for _ in range(2):
+        s.step(1)
+        o.set_time(dt + o.time())
+        for irk in (1, 2, 3):
+            o.L.orc_part_rk3_trilinear(o.h, n, st, irk, dt, xv.ravel(), r2.ravel(), alive)
+            assert o.substep(irk) == 0
+    g, gal = s.particles_download(0, n)
+    np.testing.assert_array_equal(gal, alive)
+    assert np.abs(g - xv).max() / np.abs(xv).max() < 1e-10
+    scale = np.abs(o.field("q3")).max()
+    assert np.abs(s.download("q3") - o.field("q3")).max() < 1e-10 * scale

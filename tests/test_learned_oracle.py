@@ -129,8 +129,9 @@ def test_energy_loss_gradient_matches_finite_differences():
         op[idx] += h; om[idx] -= h
         fd = (lo.energy_loss_grad(g, 1, op, p, f, scale)[0] - lo.energy_loss_grad(g, 1, om, p, f, scale)[0]) / (2 * h)
         assert abs(fd - gr[idx]) < 1e-6 * max(1.0, abs(fd))
-    # zero at the exact solution
-    L1, g1 = lo.energy_loss_grad(g, 1, p / scale, p, f, scale)
+    # zero when the correction completes the Jacobi estimate to the exact solution
+    corr = (p - lo.rhs(g, 1, f) / lo.diag(g, 1)) / scale
+    L1, g1 = lo.energy_loss_grad(g, 1, corr, p, f, scale)
     assert L1 < 1e-24 and np.abs(g1).max() < 1e-9
 
 

@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libvrh.so")
-SOURCES = ["vrh_api.cpp", "vrh_host.cpp", "vrh_kernels.hip", "vrh_particles.hip"]
+SOURCES = ["vrh_api.cpp", "vrh_host.cpp", "vrh_kernels.hip", "vrh_particles.hip", "vrh_learn.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          # packed FP32 instructions are disabled for all device code
@@ -35,7 +35,9 @@ def build(verbose=False):
     if not ok:
         raise RuntimeError("hipcc failed")
     tmp = OUT + ".tmp"
-    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, check=True)
+    # rocBLAS: the dense FP64 z transforms of the stretched-grid direct solve
+    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
+                   ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"], check=True)
     os.replace(tmp, OUT)
     return OUT
 

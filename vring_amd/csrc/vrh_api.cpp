@@ -142,7 +142,9 @@ int check_flag(vrh_t *h) {
 int step_body(vrh_t *h) {
   launch_scalars(h);
   for (int irk = 1; irk <= 3; irk++) {
-    if (h->nbatch > 0 && irk == 1) launch_particles(h, irk);  // loop.F90:66-70
+    // loop.F90:66-70 calls iter_part at irk == 1 (the Euler scheme); the RK3
+    // particle schemes take one stage per substep (launch_particles decides)
+    if (h->nbatch > 0) launch_particles(h, irk);
     if (h->prec == VRH_MIXED) substep_dev<float>(h, irk);
     else substep_dev<double>(h, irk);
   }

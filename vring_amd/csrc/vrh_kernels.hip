@@ -603,17 +603,20 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_pe
 __device__ __forceinline__ int bitrev(int x, int logn) { return (int)(__builtin_bitreverse32((unsigned)x) >> (32 - logn)); }
 
 template <typename T>
-__device__ __forceinline__ void lds_fft(T *re, T *im, int nfft, int n, int logn, const double *twr, const double *twi,
-                                        double sign, int tid, int nth) {
+__device__ __forceinline__ void lds_fft(T *re, T *im, int nfft, int logn, const T *twr, const T *twi, T sign, int tid,
+                                        int nth) {
   // in-place radix-2 decimation in time on bit-reversed input, nfft
-  // sequences of length n stored back to back
-  for (int len = 2; len <= n; len <<= 1) {
-    const int half = len >> 1, tstr = n / len;
-    for (int b = tid; b < nfft * (n >> 1); b += nth) {
-      const int q = b / (n >> 1), bb = b - q * (n >> 1);
-      const int grp = bb / half, pos = bb - grp * half;
-      const int i0 = q * n + grp * len + pos, i1 = i0 + half;
-      const T wr = (T)twr[pos * tstr], wi = (T)(sign * twi[pos * tstr]);
+  // sequences of length n = 2^logn stored back to back; twiddles
+  // w_p = (twr[p], sign * twi[p]), p < n/2, from LDS; all index arithmetic
+  // by shifts
+  const int nh = 1 << (logn - 1);
+  for (int st = 1; st <= logn; st++) {
+    const int half = 1 << (st - 1), tsh = logn - st;
+    for (int b = tid; b < nfft * nh; b += nth) {
+      const int q = b >> (logn - 1), bb = b & (nh - 1);
+      const int grp = bb >> (st - 1), pos = bb & (half - 1);
+      const int i0 = (q << logn) + (grp << st) + pos, i1 = i0 + half;
+      const T wr = twr[pos << tsh], wi = sign * twi[pos << tsh];
       const T xr = re[i1] * wr - im[i1] * wi, xi = re[i1] * wi + im[i1] * wr;
       const T ar = re[i0], ai = im[i0];
       re[i1] = ar - xr; im[i1] = ai - xi;
@@ -622,6 +625,9 @@ __device__ __forceinline__ void lds_fft(T *re, T *im, int nfft, int n, int logn,
     __syncthreads();
   }
 }
+
+// byte offset rounded up to 16
+__host__ __device__ constexpr size_t al16(size_t b) { return (b + 15) & ~(size_t)15; }
 
 // divq (divergence.F90:31-41) + fft_ptf_multi packing
 // (fastfouriertransform.F90:39-43) for LB theta lines per block.  Power-of-two
@@ -638,16 +644,33 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
   const int N = g.nt, tid = threadIdx.x, nth = blockDim.x;
   const long nlines = (long)g.nr * g.nz;
   const long l0 = (long)blockIdx.x * LB;
-  double *dv = reinterpret_cast<double *>(smem_raw);           // LB*N doubles
-  TW *re = reinterpret_cast<TW *>(dv + (size_t)LB * N);        // (LB/2)*N
+  // LDS: dv (LB*N double) | re, im ((LB/2)*N TW each) | [tw (2N double), part (LB*40 double)] | twl (N TW) | jk (2 LB int)
+  size_t off = al16(sizeof(double) * LB * N);
+  double *dv = reinterpret_cast<double *>(smem_raw);
+  TW *re = reinterpret_cast<TW *>(smem_raw + off);
   TW *im = re + (size_t)(LB / 2) * N;
+  off = al16(off + sizeof(TW) * LB * N);
+  double *tw = reinterpret_cast<double *>(smem_raw + off);     // cos (N) then sin (N), S64 only
+  double *part = tw + 2 * (size_t)N;                           // LB * 5 * 8, S64 only
+  if (S64) off = al16(off + sizeof(double) * (2 * (size_t)N + (size_t)LB * 40));
+  TW *twl = reinterpret_cast<TW *>(smem_raw + off);            // FFT twiddles: N/2 cos, N/2 sin
+  off = al16(off + sizeof(TW) * N);
+  int *jk = reinterpret_cast<int *>(smem_raw + off);           // (j, k) of each line
   const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
+  if (tid < LB) {
+    const long l = l0 + tid;
+    jk[2 * tid] = (int)(l % g.nr) + 1;
+    jk[2 * tid + 1] = (int)(l / g.nr) + 1;
+  }
+  if (pow2)
+    for (int q = tid; q < N / 2; q += nth) { twl[q] = (TW)twr[q]; twl[N / 2 + q] = (TW)twi[q]; }
+  __syncthreads();
   for (int e = tid; e < LB * N; e += nth) {
-    const int lp = e / N, ii = e - lp * N;
+    const int lp = pow2 ? e >> logn : e / N, ii = e - lp * N;
     const long l = l0 + lp;
     double val = 0.0;
     if (l < nlines) {
-      const int j = (int)(l % g.nr) + 1, k = (int)(l / g.nr) + 1, i = ii + 1;
+      const int j = jk[2 * lp], k = jk[2 * lp + 1], i = ii + 1;
       if (ext) {
         val = ext[cidx(g, i, j, k)];
       } else {
@@ -667,10 +690,10 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
     }
   }
   __syncthreads();
-  if (pow2) lds_fft(re, im, LB / 2, N, logn, twr, twi, -1.0, tid, nth);
+  if (pow2) lds_fft(re, im, LB / 2, logn, twl, twl + N / 2, (TW)-1, tid, nth);
   const int nhalf = N / 2 + 1;
   for (int e = tid; e < LB * N; e += nth) {
-    const int lp = e / N, pos = e - lp * N;
+    const int lp = pow2 ? e >> logn : e / N, pos = e - lp * N;
     const long l = l0 + lp;
     if (l >= nlines) continue;
     const int k = (pos + 1) >> 1;  // wavenumber of this packed position
@@ -702,8 +725,6 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
     // FP64 DFT of the lowest modes m=0,1,2 from the FP64 divergence: 8 fixed
     // chunks per (line, coefficient), partial sums in LDS, summed in a fixed
     // order (deterministic)
-    double *tw = reinterpret_cast<double *>(im + (size_t)(LB / 2) * N);  // cos (N) then sin (N)
-    double *part = tw + 2 * (size_t)N;             // LB * 5 * 8
     for (int q = tid; q < N; q += nth) { tw[q] = twr[q]; tw[N + q] = twi[q]; }
     __syncthreads();
     const int nch = 8, chl = (N + nch - 1) / nch;
@@ -739,18 +760,20 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
 // stride nl).  Makhoul's reordering turns each pair of real lines into one
 // complex FFT of length nz.  dir=+1 forward, -1 inverse (REDFT01).
 template <typename TW>
-__global__ __launch_bounds__(256) void k_dctz(TW *__restrict__ S, long nl, int n, int logn, int TL, int dir,
+__global__ __launch_bounds__(256) void k_dctz(TW *__restrict__ S, long nl, int n, int logn, int TL, int logtl, int dir,
                                               const double *__restrict__ twr, const double *__restrict__ twi,
                                               const double *__restrict__ er, const double *__restrict__ ei) {
   extern __shared__ unsigned char smem_raw[];
   TW *re = reinterpret_cast<TW *>(smem_raw);
   TW *im = re + (size_t)(TL / 2) * n;
+  TW *twl = im + (size_t)(TL / 2) * n;  // n/2 cos, n/2 sin
   const int tid = threadIdx.x, nth = blockDim.x;
   const long l0 = (long)blockIdx.x * TL;
   const bool pow2 = n >= 2 && (n & (n - 1)) == 0;
   if (pow2) {
+    for (int q = tid; q < n / 2; q += nth) { twl[q] = (TW)twr[q]; twl[n / 2 + q] = (TW)twi[q]; }
     for (int e = tid; e < TL * n; e += nth) {
-      const int lp = e % TL, kz = e / TL;  // consecutive threads -> consecutive lines
+      const int lp = e & (TL - 1), kz = e >> logtl;  // consecutive threads -> consecutive lines
       const long l = l0 + lp;
       const int q = lp >> 1;
       if (dir > 0) {
@@ -773,9 +796,9 @@ __global__ __launch_bounds__(256) void k_dctz(TW *__restrict__ S, long nl, int n
       }
     }
     __syncthreads();
-    lds_fft(re, im, TL / 2, n, logn, twr, twi, dir > 0 ? -1.0 : 1.0, tid, nth);
+    lds_fft(re, im, TL / 2, logn, twl, twl + n / 2, (TW)(dir > 0 ? -1 : 1), tid, nth);
     for (int e = tid; e < TL * n; e += nth) {
-      const int lp = e % TL, kz = e / TL;
+      const int lp = e & (TL - 1), kz = e >> logtl;
       const long l = l0 + lp;
       if (l >= nl) continue;
       const int q = lp >> 1;
@@ -908,9 +931,18 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
   TW *re = reinterpret_cast<TW *>(smem_raw);
   TW *im = re + (size_t)(LB / 2) * N;
   TW *tb = im + (size_t)(LB / 2) * N;  // packed input, LB*N
+  TW *twl = tb + (size_t)LB * N;       // N/2 cos, N/2 sin
+  int *jk = reinterpret_cast<int *>(smem_raw + al16(sizeof(TW) * ((size_t)2 * LB * N + N)));
   const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
+  if (tid < LB) {
+    const long l = l0 + tid;
+    jk[2 * tid] = (int)(l % g.nr) + 1;
+    jk[2 * tid + 1] = (int)(l / g.nr) + 1;
+  }
+  if (pow2)
+    for (int q = tid; q < N / 2; q += nth) { twl[q] = (TW)twr[q]; twl[N / 2 + q] = (TW)twi[q]; }
   for (int e = tid; e < LB * N; e += nth) {
-    const int lp = e / N, pos = e - lp * N;
+    const int lp = pow2 ? e >> logn : e / N, pos = e - lp * N;
     const long l = l0 + lp;
     TW v = (l < nlines) ? S[(size_t)pos + (size_t)N * l] : (TW)0;
     if (S64 && pos < 5) v = (TW)0;
@@ -920,7 +952,7 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
   if (pow2) {
     // Z[k] = Fa[k] + i Fb[k], Fa[k] = (t(2k-1) - i t(2k))/2, Hermitian extension
     for (int e = tid; e < (LB / 2) * N; e += nth) {
-      const int q = e / N, k = e - q * N;
+      const int q = e >> logn, k = e - q * N;
       double ar, ai, br, bi;
       auto coef = [&](const TW *t, int kk, double &cr, double &ci) {
         if (kk == 0) { cr = t[0]; ci = 0.0; return; }
@@ -938,10 +970,10 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
       im[slot] = (TW)(ai + br);
     }
     __syncthreads();
-    lds_fft(re, im, LB / 2, N, logn, twr, twi, 1.0, tid, nth);
+    lds_fft(re, im, LB / 2, logn, twl, twl + N / 2, (TW)1, tid, nth);
   }
   for (int e = tid; e < LB * N; e += nth) {
-    const int lp = e / N, ii = e - lp * N;
+    const int lp = pow2 ? e >> logn : e / N, ii = e - lp * N;
     const long l = l0 + lp;
     if (l >= nlines) continue;
     double v;
@@ -969,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
         else v += a * twr[ph] + b * twi[ph];
       }
     }
-    const int j = (int)(l % g.nr) + 1, k = (int)(l / g.nr) + 1;
+    const int j = jk[2 * lp], k = jk[2 * lp + 1];
     if (phi_out) phi_out[cidx(g, ii + 1, j, k)] = v;
     if (dph) {
       dph[hidx(g, ii + 1, j, k)] = v;
@@ -1052,10 +1084,34 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
   const double rcj = c.rc[jq], rcp = c.rc[min(jq + 1, nrc)], rcm = c.rc[max(jq - 1, 1)];
   const double dx1 = g.dx1, dx2 = g.dx2, dx3 = g.dx3, dx3q = g.dx3q, slip = g.slip;
   const int cx = tx + 2, cy = ty + 2;
+  // the streamed operands of the next plane are loaded while this one is computed
+  const bool rowu = act && j >= 1 && j <= nr;
+  double n_dq1 = 0.0, n_dq2 = 0.0, n_dq3 = 0.0, n_pres = 0.0;
+  TW n_p1 = (TW)0, n_p2 = (TW)0, n_p3 = (TW)0;
+  auto load_cell = [&](int kk) {
+    if (!act || kk > ke) return;
+    const long oo = (long)(i - 1) + (long)j * sj + (long)kk * sk;
+    n_dq3 = dq3[oo];
+    if (rowu) {
+      n_dq1 = dq1[oo];
+      if (j >= 2) n_dq2 = dq2[oo];
+      if (kk <= nz) {
+        const long pp = (long)(i - 1) + (long)nt * ((long)(j - 1) + (long)nr * (long)(kk - 1));
+        n_p1 = prg1[pp];
+        if (j >= 2) n_p2 = prg2[pp];
+        n_p3 = prg3[pp];
+        n_pres = pres[oo];
+      }
+    }
+  };
+  load_cell(kb);
   for (int k = kb; k <= ke; k++) {
     stash(k + 1);
+    const double c_dq1 = n_dq1, c_dq2 = n_dq2, c_dq3 = n_dq3, c_pres = n_pres;
+    const TW c_p1 = n_p1, c_p2 = n_p2, c_p3 = n_p3;
     __syncthreads();
     fetch(k + 2);
+    load_cell(k + 1);
     if (act) {
       const double(*P0)[LC + 1] = ring[k & 3];
       const double(*Pm)[LC + 1] = ring[(k - 1) & 3];
@@ -1067,15 +1123,15 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
       const double p0 = P0[cy][cx];
       // ---- upsol ----
       if (j >= 1 && j <= nr) {
-        const double v = dq1[o] - a1 * (p0 - P0[cy][cx - 1]);
+        const double v = c_dq1 - a1 * (p0 - P0[cy][cx - 1]);
         q1[o] = v;
         if (j == nr) q1[o + sj] = slip * v;
       }
-      if (j >= 2 && j <= nr) q2[o] = dq2[o] - a2 * (p0 - P0[cy - 1][cx]) / 1.0;
+      if (j >= 2 && j <= nr) q2[o] = c_dq2 - a2 * (p0 - P0[cy - 1][cx]) / 1.0;
       if (j == 1 || j == nrc) q2[o] = 0.0;
-      if (k == 1) q3[o] = dq3[o];
+      if (k == 1) q3[o] = c_dq3;
       if (k >= 2 && j >= 1 && j <= nr) {
-        const double v = dq3[o] - a3 * (p0 - Pm[cy][cx]);
+        const double v = c_dq3 - a3 * (p0 - Pm[cy][cx]);
         q3[o] = v;
         if (j == nr) q3[o + sj] = slip * v;
       }
@@ -1095,7 +1151,7 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
           const double qnn = qjp * irmn - qc0 * irm;
           const double qss = qc0 * irm - qjm * irmm;
           const double d22 = (qnn * s1t2n - qss * s1t2s) * s1t2c;
-          prg1[pc] = (TW)((double)prg1[pc] + qc0 - be * (d11 + d33 + d22));
+          prg1[pc] = (TW)((double)c_p1 + qc0 - be * (d11 + d33 + d22));
         }
         // r component: qcap = (dph(j) - dph(j-1)) dx2 rc, zero at j = 1 and nrc
         if (j >= 2) {
@@ -1109,7 +1165,7 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
           const double d11 = (qip - 2.0 * qc0 + qim) * s2t11;
           const double d33 = (qkp - 2.0 * qc0 + qkm) * dx3q;
           const double d22 = (qjp - qc0) * s2t2p - (qc0 - qjm) * s2t2m;
-          prg2[pc] = (TW)((double)prg2[pc] + qc0 - be * (d11 + d22 + d33));
+          prg2[pc] = (TW)((double)c_p2 + qc0 - be * (d11 + d22 + d33));
         }
         // z component: qcap = (dph(k) - dph(k-1)) dx3, zero at k = 1
         {
@@ -1125,14 +1181,14 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
             const double d11 = (qip - 2.0 * qc0 + qim) * s1t11;
             const double d33 = (qkp - 2.0 * qc0 + qkm) * dx3q;
             const double d22 = (qjp - qc0) * s3t2n - (qc0 - qjm) * s3t2s;
-            prg3[pc] = (TW)((double)prg3[pc] + qc0 - be * (d11 + d22 + d33));
+            prg3[pc] = (TW)((double)c_p3 + qc0 - be * (d11 + d22 + d33));
           }
         }
         {
           const double d11 = (P0[cy][cx + 1] + P0[cy][cx - 1] - 2.0 * p0) * s1t11;
           const double d22 = (P0[cy + djp][cx] - p0) * s3t2n - (p0 - P0[cy + djm][cx]) * s3t2s;
           const double d33 = (Pp[cy][cx] + Pkm[cy][cx] - 2.0 * p0) * dx3q;
-          pres[o] = pres[o] + p0 - be * (d11 + d22 + d33);
+          pres[o] = c_pres + p0 - be * (d11 + d22 + d33);
         }
       }
     }
@@ -1262,45 +1318,44 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   TW *S = (TW *)h->f.S;
   double *S64 = (h->prec == VRH_MIXED) ? h->f.S64 : nullptr;
   const double adt = h->rk[irk].a * h->dt;
-  size_t sh1 = sizeof(double) * LB * N + sizeof(TW) * LB * N;
-  if (S64) sh1 += sizeof(double) * (2 * (size_t)N + (size_t)LB * 5 * 8);
+  size_t sh1 = al16(sizeof(double) * LB * N) + al16(sizeof(TW) * LB * N);
+  if (S64) sh1 = al16(sh1 + sizeof(double) * (2 * (size_t)N + (size_t)LB * 40));
+  sh1 = al16(sh1 + sizeof(TW) * N) + sizeof(int) * 2 * LB;
   { KTimer kt(h, KT_DIVFFT);
   hipLaunchKernelGGL(k_divfft<TW>, dim3((unsigned)((nlines + LB - 1) / LB)), dim3(256), sh1, h->stream, g, h->coef,
                      1.0 / adt, h->f.dq1, h->f.dq2, h->f.dq3, ext_rhs, S, S64, h->d_twt, h->d_twt + N, LB, logn); }
   // z transforms
   const int nz = g.nz, lz = ilog2(nz);
-  int TL = (int)(65536 / (sizeof(TW) * nz));  // at most 64 KB of LDS per block
-  if (TL > 16) TL = 16;
-  if (h->dct_lines > 0 && h->dct_lines < TL) TL = h->dct_lines;
-  TL &= ~1;
-  if (TL < 2) TL = 2;
-  int TL64 = (int)(65536 / (sizeof(double) * nz));
-  if (TL64 > 16) TL64 = 16;
-  TL64 &= ~1;
-  if (TL64 < 2) TL64 = 2;
+  // lines per block: a power of two, data + twiddles within 64 KB of LDS
+  auto lines = [&](size_t es, int want) {
+    int tl = 2;
+    while (tl * 2 <= 16 && (size_t)(tl * 2) * nz * es + (size_t)nz * es <= 65536) tl *= 2;
+    if (want > 0) while (tl > 2 && tl > want) tl /= 2;
+    return tl;
+  };
+  const int TL = lines(sizeof(TW), h->dct_lines), TL64 = lines(sizeof(double), 0);
+  const int ltl = ilog2(TL), ltl64 = ilog2(TL64);
   const long nl = (long)N * g.nr;
-  size_t sh2 = sizeof(TW) * TL * nz;
+  const size_t sh2 = sizeof(TW) * ((size_t)TL * nz + nz), sh64 = sizeof(double) * ((size_t)TL64 * nz + nz);
   { KTimer kt(h, KT_DCT_FWD);
   hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(256), sh2, h->stream, S, nl, nz, lz, TL,
-                     1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
+                     ltl, 1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
   if (S64) {
     const long nl64 = 5L * g.nr;
-    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(256),
-                       sizeof(double) * TL64 * nz, h->stream, S64, nl64, nz, lz, TL64, 1, h->d_twz, h->d_twz + nz,
-                       h->d_ez, h->d_ez + nz);
+    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(256), sh64, h->stream, S64,
+                       nl64, nz, lz, TL64, ltl64, 1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
   }
   radial_launch<TW>(h, S, (const TW *)h->f.ib, N);
   if (S64) radial_launch<double>(h, S64, h->f.ib64, 5);
   { KTimer kt(h, KT_DCT_INV);
   hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(256), sh2, h->stream, S, nl, nz, lz, TL,
-                     -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
+                     ltl, -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
   if (S64) {
     const long nl64 = 5L * g.nr;
-    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(256),
-                       sizeof(double) * TL64 * nz, h->stream, S64, nl64, nz, lz, TL64, -1, h->d_twz, h->d_twz + nz,
-                       h->d_ez, h->d_ez + nz);
+    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(256), sh64, h->stream, S64,
+                       nl64, nz, lz, TL64, ltl64, -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
   }
-  size_t sh3 = sizeof(TW) * 2 * LB * N;
+  const size_t sh3 = al16(sizeof(TW) * ((size_t)2 * LB * N + N)) + sizeof(int) * 2 * LB;
   KTimer kti(h, KT_IFFT);
   hipLaunchKernelGGL(k_ifft<TW>, dim3((unsigned)((nlines + LB - 1) / LB)), dim3(256), sh3, h->stream, g, S, S64,
                      phi_out, ext_rhs ? nullptr : h->f.dph, h->d_twt, h->d_twt + N, LB, logn);

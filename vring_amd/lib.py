@@ -330,3 +330,145 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# per-mode (r, z) Helmholtz subsystem (include/vrh_learn.h)
+# ---------------------------------------------------------------------------
+class VrhHzGrid(ctypes.Structure):
+    _fields_ = [("nr", ctypes.c_int), ("nz", ctypes.c_int), ("nslots", ctypes.c_int),
+                ("rmax", ctypes.c_double), ("zmax", ctypes.c_double), ("stretch", ctypes.c_double)]
+
+
+HZ_EXPORTS = ["vrh_hz_create", "vrh_hz_destroy", "vrh_hz_last_error", "vrh_hz_set_modes", "vrh_hz_mesh",
+              "vrh_hz_apply", "vrh_hz_direct", "vrh_hz_solve", "vrh_hz_net_size", "vrh_hz_net_init",
+              "vrh_hz_net_get", "vrh_hz_net_set", "vrh_hz_net_forward", "vrh_hz_net_grad", "vrh_hz_sample",
+              "vrh_hz_train"]
+HZ_METHOD = {"pcg": 0, "bicgstab": 1, "learned": 2}
+
+
+def _hz_bind(L):
+    if getattr(L, "_hz_bound", False):
+        return L
+    P, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+    U = ctypes.c_ulonglong
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    L.vrh_hz_create.argtypes = [ctypes.POINTER(VrhHzGrid), ctypes.POINTER(P)]
+    L.vrh_hz_destroy.argtypes = [P]
+    L.vrh_hz_last_error.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
+    L.vrh_hz_set_modes.argtypes = [P, ip]
+    L.vrh_hz_mesh.argtypes = [P, dp, dp, dp]
+    L.vrh_hz_apply.argtypes = [P, dp, dp]
+    L.vrh_hz_direct.argtypes = [P, dp, dp]
+    L.vrh_hz_solve.argtypes = [P, I, D, I, dp, dp, ip, dp, ctypes.POINTER(D)]
+    L.vrh_hz_net_size.argtypes = [P]
+    L.vrh_hz_net_init.argtypes = [P, U]
+    L.vrh_hz_net_get.argtypes = [P, fp]
+    L.vrh_hz_net_set.argtypes = [P, fp]
+    L.vrh_hz_net_forward.argtypes = [P, dp, fp, dp]
+    L.vrh_hz_net_grad.argtypes = [P, dp, fp, fp]
+    L.vrh_hz_sample.argtypes = [P, U, I, dp]
+    L.vrh_hz_train.argtypes = [P, I, D, U, fp]
+    L._hz_bound = True
+    return L
+
+
+class Helmholtz:
+    """Per-mode (r, z) pressure systems S_m p = V f on a tanh-stretched z grid
+    (nslots independent slots), with direct, Krylov and learned solvers."""
+
+    def __init__(self, nr, nz, nslots, rmax=4.0, zmax=12.0, stretch=2.0):
+        self.L = _hz_bind(load_library())
+        self.nr, self.nz, self.S = nr, nz, nslots
+        self.h = ctypes.c_void_p()
+        g = VrhHzGrid(nr, nz, nslots, rmax, zmax, stretch)
+        self._check(self.L.vrh_hz_create(ctypes.byref(g), ctypes.byref(self.h)), "vrh_hz_create")
+        self.shape = (nslots, nz, nr)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            buf = ctypes.create_string_buffer(512)
+            if self.h:
+                self.L.vrh_hz_last_error(self.h, buf, 512)
+            raise RuntimeError(f"{what} failed ({rc}): {buf.value.decode(errors='replace')}")
+
+    def close(self):
+        if self.h:
+            self.L.vrh_hz_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _arr(self, a):
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), self.shape))
+        return a
+
+    def set_modes(self, m):
+        self._check(self.L.vrh_hz_set_modes(self.h, np.ascontiguousarray(m, np.int32)), "set_modes")
+
+    def mesh(self):
+        zf, zm, rm = np.zeros(self.nz + 1), np.zeros(self.nz), np.zeros(self.nr)
+        self._check(self.L.vrh_hz_mesh(self.h, zf, zm, rm), "mesh")
+        return zf, zm, rm
+
+    def apply(self, x):
+        y = np.zeros(self.shape)
+        self._check(self.L.vrh_hz_apply(self.h, self._arr(x), y), "apply")
+        return y
+
+    def direct(self, f):
+        p = np.zeros(self.shape)
+        self._check(self.L.vrh_hz_direct(self.h, self._arr(f), p), "direct")
+        return p
+
+    def solve(self, f, method="pcg", tol=1e-8, maxit=5000):
+        p = np.zeros(self.shape)
+        it = np.zeros(self.S, np.int32)
+        rel = np.zeros(self.S)
+        ms = ctypes.c_double(0.0)
+        self._check(self.L.vrh_hz_solve(self.h, HZ_METHOD[method], float(tol), int(maxit), self._arr(f), p, it, rel,
+                                        ctypes.byref(ms)), f"solve {method}")
+        return p, it, rel, ms.value
+
+    def net_size(self):
+        return self.L.vrh_hz_net_size(self.h)
+
+    def net_init(self, seed):
+        self._check(self.L.vrh_hz_net_init(self.h, int(seed)), "net_init")
+
+    def net_get(self):
+        w = np.zeros(self.net_size(), np.float32)
+        self._check(self.L.vrh_hz_net_get(self.h, w), "net_get")
+        return w
+
+    def net_set(self, w):
+        self._check(self.L.vrh_hz_net_set(self.h, np.ascontiguousarray(w, np.float32)), "net_set")
+
+    def net_forward(self, f):
+        out = np.zeros(self.shape, np.float32)
+        sc = np.zeros(self.S)
+        self._check(self.L.vrh_hz_net_forward(self.h, self._arr(f), out, sc), "net_forward")
+        return out, sc
+
+    def net_grad(self, f, gout):
+        g = np.zeros(self.net_size(), np.float32)
+        self._check(self.L.vrh_hz_net_grad(self.h, self._arr(f), np.ascontiguousarray(gout, np.float32), g),
+                    "net_grad")
+        return g
+
+    def sample(self, seed, family=-1):
+        f = np.zeros(self.shape)
+        self._check(self.L.vrh_hz_sample(self.h, int(seed), int(family), f), "sample")
+        return f
+
+    def train(self, nsamples, lr=1e-3, seed=0):
+        nb = (nsamples + self.S - 1) // self.S
+        loss = np.zeros(nb, np.float32)
+        self._check(self.L.vrh_hz_train(self.h, int(nsamples), float(lr), int(seed), loss), "train")
+        return loss
