@@ -173,7 +173,7 @@ def run_learned(nr, nz, nslots, train_samples, seed=0):
     t0 = time.perf_counter()
     pd = hz.direct(f)
     out["direct_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
-    for m in ("pcg", "bicgstab", "learned"):
+    for m in ("pcg", "bicgstab", "learned", "learned_fcg"):
         p, it, rel, ms = hz.solve(f, m, tol=1e-8, maxit=50000)
         out[m] = {"iterations_total": int(it.sum()), "iterations_max": int(it.max()), "ms": round(ms, 3),
                   "max_true_relres": float(rel.max()),

@@ -150,6 +150,20 @@ int vrh_particles_download(vrh_t *h, int batch, long n, double *u, int *alive);
 /* Advance all batches at RK substep irk (iter_part, part_tools.F90:374-1025;
  * the reference calls it at irk==1 only, loop.F90:67). */
 int vrh_particles_step(vrh_t *h, int irk);
+/* Pack the live particles of a batch to the front, order preserved (the
+ * reference's free list, remv_part/injc_part, part_tools.F90:141-225). */
+int vrh_particles_compact(vrh_t *h, int batch);
+/* Live particles of a batch (synchronous). */
+int vrh_particles_count(vrh_t *h, int batch, long *n);
+/* Inlet injection (inlt_part, part_tools.F90:95-138), after a compaction:
+ * for every inlet cell (rc(ir) < r0, first z plane, theta order per row)
+ * inflow(ir) += c a q3 dt inj / nprcl with a = dtheta (rc(ir+1)^2 - rc(ir)^2),
+ * c = 1e6 0.02^3, nprcl = 0.001; floor(inflow) particles are appended,
+ * uniform in the cell, velocity (0, 0, q3), identity Jacobian.  Positions
+ * come from a counter-based stream (splitmix64 of `seed`, the call number
+ * and the cell), not Fortran's random_number.  Particles beyond the capacity
+ * are dropped.  ninj (optional, synchronous) receives the number injected. */
+int vrh_particles_inject(vrh_t *h, int batch, double r0, int inj, unsigned long long seed, long *ninj);
 
 #ifdef __cplusplus
 }

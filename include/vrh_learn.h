@@ -35,7 +35,7 @@ typedef struct {
   double stretch;          /* z_k = zmax (1 + tanh(s (k/nz - 1)) / tanh s); 0: uniform */
 } vrh_hz_grid;
 
-enum { VRH_HZ_PCG = 0, VRH_HZ_BICGSTAB = 1, VRH_HZ_LEARNED = 2 };
+enum { VRH_HZ_PCG = 0, VRH_HZ_BICGSTAB = 1, VRH_HZ_LEARNED = 2, VRH_HZ_LEARNED_FCG = 3 };
 
 int vrh_hz_create(const vrh_hz_grid *grid, vrh_hz_t **out);
 int vrh_hz_destroy(vrh_hz_t *h);
@@ -52,12 +52,15 @@ int vrh_hz_direct(vrh_hz_t *h, const double *f, double *p);
 /* Iterative solve of every slot to ||V f - S p|| <= tol ||V f||:
  *   VRH_HZ_PCG       Jacobi-preconditioned CG, zero start;
  *   VRH_HZ_BICGSTAB  Jacobi-preconditioned BiCGSTAB, zero start;
- *   VRH_HZ_LEARNED   flexible CG with the preconditioner
- *                    M(r) = D^-1 r + scale(r) N(r) (Jacobi plus the network's
- *                    learned correction; an untrained, zero-output network
- *                    gives plain PCG), started from M(V f); a slot that does
- *                    not converge within maxit falls back to plain PCG from
- *                    where it is.
+ *   VRH_HZ_LEARNED   plain PCG started from the network's warm start: the
+ *                    Jacobi estimate corrected by the network,
+ *                    M(V f) = D^-1 V f + scale N(V f), scaled to its
+ *                    energy-optimal multiple (so the start is never worse
+ *                    than zero in the energy norm);
+ *   VRH_HZ_LEARNED_FCG  the same start, then flexible CG (Polak-Ribiere)
+ *                    preconditioned by M; a slot that breaks down or does not
+ *                    converge within maxit falls back to plain PCG from where
+ *                    it is.
  * The true residual of the returned p is recomputed and reported per slot in
  * relres; iters receives the iterations per slot (fallback included), and
  * ms (optional) the solve time in milliseconds (GPU events). */

@@ -11,7 +11,7 @@ from vring_amd.lib import stokes_number, torus_points
 pytestmark = pytest.mark.gpu
 
 
-def setup(nt=16, nr=64, nz=96, scheme="lsq_euler", n=512, nsub=5, seed=1):
+def setup(nt=16, nr=64, nz=96, scheme="lsq_euler", n=512, nsub=5, seed=1, cap=None):
     rmax, zmax = 4.0, 12.0
     q1, q2, q3 = vring_amd.ring_initial_condition(nt, nr, nz, rmax, zmax, [(1.0, 3.0, 0.3, 1.0)],
                                                   pert_amp=0.05, pert_modes=(2, 3), phases=np.array([0.3, 1.1]))
@@ -31,7 +31,7 @@ def setup(nt=16, nr=64, nz=96, scheme="lsq_euler", n=512, nsub=5, seed=1):
     u[:, 3:6] = 0.1 * rng.standard_normal((n, 3))
     if nvar == 117:
         u[:, [6, 10, 14]] = 1.0
-    s.particles_create([n], [st], [nsub], scheme)
+    s.particles_create([cap or n], [st], [nsub], scheme)
     s.particles_upload(0, u)
     return o, s, u, st, dt
 
@@ -91,3 +91,35 @@ def test_particles_advance_inside_whole_steps():
     assert np.abs(g - xv).max() / np.abs(xv).max() < 1e-10
     scale = np.abs(o.field("q3")).max()
     assert np.abs(s.download("q3") - o.field("q3")).max() < 1e-10 * scale
+
+
+def test_inlet_injection_and_compaction_match_oracle():
+    """vrh_particles_inject after some particles died: live particles packed
+    to the front in order, then the inlet particles of inlt_part
+    (part_tools.F90:95-138) from the same counter-based stream."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import inject_oracle as io
+    print("case particles inject", flush=True)
+    nt, nr, nz, rmax, zmax = 16, 32, 48, 4.0, 12.0
+    o, s, u0, st, dt = setup(nt=nt, nr=nr, nz=nz, scheme="lsq_euler", n=100, cap=20000)
+    q3 = s.download("q3")
+    th = (np.arange(nt) + 0.5) * 2 * np.pi / nt
+    q3[:, :, 1] = 1.0 + 0.3 * np.sin(th)[:, None] + 0.1 * np.arange(nr + 3)[None, :] / nr
+    s.upload("q3", q3)
+    alive = np.ones(100, np.int32)
+    alive[::3] = 0
+    s.particles_upload(0, u0, alive)
+    inflow = np.zeros(nr + 2)
+    expect = u0[alive == 1]
+    for call in range(3):
+        nnew = s.particles_inject(0, r0=0.9, inj=10, seed=5)
+        rows, counts = io.inject(q3[:, :, 1], inflow, nt, nr, rmax, zmax, nz, dt, 0.9, 10, 5, call)
+        assert nnew == rows.shape[0] == counts.sum()
+        expect = np.concatenate([expect, rows])
+    n = s.particles_count(0)
+    assert n == expect.shape[0]
+    g, gal = s.particles_download(0, n)
+    assert np.all(gal == 1)
+    assert np.abs(g - expect).max() < 1e-12 * max(1.0, np.abs(expect).max())

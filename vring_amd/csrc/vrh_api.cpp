@@ -331,6 +331,8 @@ int vrh_destroy(vrh_t *h) {
     if (b.u) hipFree(b.u);
     if (b.ddt2) hipFree(b.ddt2);
     if (b.alive) hipFree(b.alive);
+    void *extra[] = {b.d_n, b.inflow, b.u_tmp, b.ddt2_tmp, b.alive_tmp, b.work};
+    for (void *q : extra) if (q) hipFree(q);
   }
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;

@@ -60,9 +60,16 @@ struct PartBatch {
   long cap, n;
   double st;
   int nsub;
-  double *u;     // cap x nvar (row-major)
-  double *ddt2;  // cap x nvar, previous rate (RK3 schemes)
+  double *u;     // nvar rows of cap (structure of arrays)
+  double *ddt2;  // nvar rows of cap, previous rate (RK3 schemes)
   int *alive;
+  long *d_n;     // used slots [0, n) on the device (kernels read it)
+  // inlet injection and compaction (allocated on first use)
+  double *inflow = nullptr;  // accumulated inlet flux per r row (nr + 1)
+  double *u_tmp = nullptr, *ddt2_tmp = nullptr;
+  int *alive_tmp = nullptr;
+  long *work = nullptr;      // scan scratch
+  long ncall = 0;            // injection calls so far (random stream index)
 };
 
 // Kernel ids for the per-kernel event timer.

@@ -1081,6 +1081,26 @@ __global__ void k_fcg_scalars(int S, const double *part_z, const double *part_rr
   if (rr <= tol2 * s[SC_BB]) s[SC_DONE] = 1.0;
   else if (!(rz > 0.0) || !isfinite(rr)) s[SC_DONE] = 2.0;
 }
+// energy-optimal scaling of a warm start x: alpha = (x.b) / (x.Sx) per slot,
+// so that the start's energy error never exceeds that of x = 0
+__global__ void k_wsdots(Geo g, const double *__restrict__ x, const double *__restrict__ rhs, double *part) {
+  const int b = blockIdx.y;
+  double v[2] = {0.0, 0.0};
+  CELL_LOOP(g, b, c, k, j) {
+    const double *xb = x + (long)b * ncs;
+    const double sx = sapply(g, b, k, j, [&](int kk, int jj) { return xb[(long)kk * g.nr + jj]; });
+    v[0] += x[c] * rhs[c];
+    v[1] += x[c] * sx;
+  }
+  block_partials<2>(v, part, 2, b);
+}
+__global__ void k_wsscale(Geo g, const double *part, double *__restrict__ x) {
+  const int b = blockIdx.y;
+  const double xsx = psum(part, 2, b, 1);
+  const double a = xsx > 0.0 ? psum(part, 2, b, 0) / xsx : 0.0;
+  CELL_LOOP(g, b, c, k, j) { x[c] *= a; }
+}
+
 // restart a broken slot from zero
 __global__ void k_reset_broken(Geo g, const double *sc, double *__restrict__ x) {
   const int b = blockIdx.y;
@@ -1307,7 +1327,8 @@ int vrh_hz_direct(vrh_hz_t *h, const double *f, double *p) {
 int vrh_hz_solve(vrh_hz_t *h, int method, double tol, int maxit, const double *f, double *p, int *iters,
                  double *relres, double *ms) {
   if (!h || !f || !p || tol <= 0 || maxit < 1) return -1;
-  if (method != VRH_HZ_PCG && method != VRH_HZ_BICGSTAB && method != VRH_HZ_LEARNED) return hfail(h, "unknown method");
+  if (method != VRH_HZ_PCG && method != VRH_HZ_BICGSTAB && method != VRH_HZ_LEARNED && method != VRH_HZ_LEARNED_FCG)
+    return hfail(h, "unknown method");
   int rc = upload_host(h, h->vec[0], f);
   if (rc) return rc;
   hipEvent_t e0, e1;
@@ -1317,11 +1338,18 @@ int vrh_hz_solve(vrh_hz_t *h, int method, double tol, int maxit, const double *f
   double *rhs = h->vec[1], *x = h->vec[10];
   launch_rhs(h, h->vec[0], rhs);
   std::vector<int> its(h->S, 0), its2;
-  if (method == VRH_HZ_LEARNED) {
-    // warm start from the preconditioned estimate of the solution
+  if (method == VRH_HZ_LEARNED || method == VRH_HZ_LEARNED_FCG) {
+    // warm start: the network-corrected Jacobi estimate M(V f), scaled to
+    // its energy-optimal multiple
     net_features(h, rhs, 1);
     net_forward(h);
     hipLaunchKernelGGL(k_scale_out, red_grid(h), dim3(256), 0, h->stream, h->geo(), h->d_out, h->d_sc, rhs, x);
+    hipLaunchKernelGGL(k_wsdots, red_grid(h), dim3(256), 0, h->stream, h->geo(), x, rhs, h->d_part);
+    hipLaunchKernelGGL(k_wsscale, red_grid(h), dim3(256), 0, h->stream, h->geo(), h->d_part, x);
+  }
+  if (method == VRH_HZ_LEARNED) {
+    if ((rc = pcg_dev(h, rhs, x, tol, maxit, its))) return rc;
+  } else if (method == VRH_HZ_LEARNED_FCG) {
     if ((rc = fcg_dev(h, rhs, x, tol, maxit, its))) return rc;
     bool fallback = false;
     for (int b = 0; b < h->S; b++) fallback = fallback || its[b] < 0;

@@ -22,6 +22,7 @@ struct PartArgs {
   const double *mint;  // 9 x 27
   double *u, *ddt2;
   int *alive;
+  const long *n_dev;  // used slots, read on the device
   long cap, n;
   int nvar, nsub, scheme, irk;
   double st, partdt, gam, rom, dtheta, dr, dz, rmax, zmax;
@@ -192,7 +193,7 @@ __device__ bool interp_lsq(const PartArgs &a, double xp, double yp, double zp, T
 template <typename T>
 __global__ __launch_bounds__(128) void k_part_lsq(PartArgs a) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n || !a.alive[p]) return;
+  if (p >= a.cap || p >= *a.n_dev || !a.alive[p]) return;
   const long cap = a.cap;
   double *u = a.u + p;      // u[v*cap]
   double *d2 = a.ddt2 ? a.ddt2 + p : nullptr;
@@ -293,7 +294,7 @@ __device__ T tri_comp(const PartArgs &a, int ivar, double theta, double r, doubl
 template <typename T>
 __global__ __launch_bounds__(256) void k_part_tri(PartArgs a) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n || !a.alive[p]) return;
+  if (p >= a.cap || p >= *a.n_dev || !a.alive[p]) return;
   const long cap = a.cap;
   double *s = a.u + p, *r2 = a.ddt2 + p;
 #define S(v) s[(long)(v) * cap]
@@ -321,6 +322,136 @@ __global__ __launch_bounds__(256) void k_part_tri(PartArgs a) {
 #undef R
 }
 
+// ---------------------------------------------------------------------------
+// Compaction of the live particles and inlet injection (inlt_part, injc_part
+// and remv_part, part_tools.F90:95-225).  The reference keeps a free list of
+// slots; here the live particles are packed to the front (prefix sums over
+// the alive flags, order preserved) and new particles are appended, so the
+// kernels only see slots [0, n).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long psmix(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+// uniform in (0, 1) from a counter-based stream (splitmix64)
+__device__ __forceinline__ double pu01(unsigned long long seed, unsigned long long i) {
+  return ((psmix(seed ^ psmix(i)) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// live slots per block of 256
+__global__ void k_alive_count(const int *__restrict__ alive, const long *n_dev, long cap, long *__restrict__ bsum) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  const long n = *n_dev;
+  int x = (p < cap && p < n && alive[p]) ? 1 : 0;
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  __shared__ int red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) bsum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// in-place exclusive scan of m values by one block of 1024 threads; the
+// total goes to *total
+__global__ void k_scan1(long *v, long m, long *total) {
+  __shared__ long part[1024];
+  const int t = threadIdx.x;
+  const long chunk = (m + 1023) / 1024, b0 = t * chunk, b1 = min(m, b0 + chunk);
+  long s = 0;
+  for (long q = b0; q < b1; q++) s += v[q];
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    long acc = 0;
+    for (int q = 0; q < 1024; q++) { const long x = part[q]; part[q] = acc; acc += x; }
+    *total = acc;
+  }
+  __syncthreads();
+  long acc = part[t];
+  for (long q = b0; q < b1; q++) { const long x = v[q]; v[q] = acc; acc += x; }
+}
+
+// scatter the live particles of each block to boff[block] + rank
+__global__ void k_compact(const double *__restrict__ u, const double *__restrict__ d2, const int *__restrict__ alive,
+                          const long *n_dev, long cap, int nvar, const long *__restrict__ boff,
+                          double *__restrict__ ut, double *__restrict__ d2t, int *__restrict__ at) {
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  const long n = *n_dev;
+  const int a = (p < cap && p < n && alive[p]) ? 1 : 0;
+  __shared__ int cnt[256];
+  cnt[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {  // inclusive scan by doubling
+    const int x = ((int)threadIdx.x >= o) ? cnt[threadIdx.x - o] : 0;
+    __syncthreads();
+    cnt[threadIdx.x] += x;
+    __syncthreads();
+  }
+  if (!a) return;
+  const long dst = boff[blockIdx.x] + cnt[threadIdx.x] - 1;
+  for (int v = 0; v < nvar; v++) ut[(long)v * cap + dst] = u[(long)v * cap + p];
+  if (d2)
+    for (int v = 0; v < nvar; v++) d2t[(long)v * cap + dst] = d2[(long)v * cap + p];
+  at[dst] = 1;
+}
+
+__global__ void k_set_count(long *n_dev, const long *total) { *n_dev = *total; }
+
+// inlet flux per (r row, theta cell), in the reference's theta order per row:
+// inflow(ir) += c a q3 dt inj / nprcl, n = floor(inflow), inflow -= n
+__global__ void k_inflow(VGrid g, VCoef c, const double *__restrict__ q3, double *__restrict__ inflow, int jr0,
+                         double dth, double dt, int inj, double cc, double nprcl, long *__restrict__ counts) {
+  const int ir = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (ir > jr0) return;
+  const double a = dth * (c.rc[ir + 1] * c.rc[ir + 1] - c.rc[ir] * c.rc[ir]);
+  double acc = inflow[ir];
+  for (int it = 1; it <= g.nt; it++) {
+    acc = acc + (cc * a * q3[hidx(g, it, ir, 1)] * dt * (double)(float)inj / nprcl);
+    const long nn = (long)floor(acc);
+    acc = acc - (double)(float)nn;
+    counts[(long)(ir - 1) * g.nt + (it - 1)] = nn > 0 ? nn : 0;
+  }
+  inflow[ir] = acc;
+}
+
+// write the new particles of each inlet cell: uniform in the cell, velocity
+// (0, 0, q3), identity Jacobian (part_tools.F90:120-134, 164-181)
+__global__ void k_inject(VGrid g, VCoef c, const double *__restrict__ q3, const long *__restrict__ counts,
+                         const long *__restrict__ offs, const long *n_dev, long cap, int nvar, double *__restrict__ u,
+                         double *__restrict__ d2, int *__restrict__ alive, int jr0, double dth, double dr, double dz,
+                         unsigned long long seed, long call) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)jr0 * g.nt) return;
+  const int ir = (int)(t / g.nt) + 1, it = (int)(t - (long)(ir - 1) * g.nt) + 1;
+  const long cnt = counts[t], base = *n_dev + offs[t];
+  const double uz = q3[hidx(g, it, ir, 1)];
+  for (long q = 0; q < cnt; q++) {
+    const long idx = base + q;
+    if (idx >= cap) break;
+    const unsigned long long key =
+        ((((unsigned long long)call * (unsigned long long)((long)jr0 * g.nt) + (unsigned long long)t) << 20) +
+         (unsigned long long)q) * 3ull;
+    const double th = (it - 1) * dth + dth * pu01(seed, key);
+    const double r = c.rc[ir] + dr * pu01(seed, key + 1);
+    const double z = 0.0 + dz * pu01(seed, key + 2);
+    for (int v = 0; v < nvar; v++) u[(long)v * cap + idx] = 0.0;
+    if (d2)
+      for (int v = 0; v < nvar; v++) d2[(long)v * cap + idx] = 0.0;
+    u[idx] = r * cos(th);
+    u[cap + idx] = r * sin(th);
+    u[2 * cap + idx] = z;
+    u[5 * cap + idx] = uz;
+    if (nvar == 117) { u[6 * cap + idx] = 1.0; u[10 * cap + idx] = 1.0; u[14 * cap + idx] = 1.0; }
+    alive[idx] = 1;
+  }
+}
+
+__global__ void k_add_count(long *n_dev, const long *total, long cap) {
+  const long n = *n_dev + *total;
+  *n_dev = n < cap ? n : cap;
+}
+
 }  // namespace
 
 void launch_particles(vrh_t *h, int irk) {
@@ -329,13 +460,12 @@ void launch_particles(vrh_t *h, int irk) {
   if (!every && irk != 1) return;  // loop.F90:67 calls iter_part at irk==1 only
   KTimer kt(h, KT_PARTICLES);
   for (auto &b : h->pb) {
-    if (b.n <= 0) continue;
     PartArgs a{};
     a.g = h->g;
     a.q1 = h->f.q1; a.q2 = h->f.q2; a.q3 = h->f.q3;
     a.mint = h->d_mint;
     a.u = b.u; a.ddt2 = b.ddt2; a.alive = b.alive;
-    a.cap = b.cap; a.n = b.n;
+    a.cap = b.cap; a.n = b.cap; a.n_dev = b.d_n;
     a.nvar = h->pscheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
     a.nsub = b.nsub; a.scheme = h->pscheme; a.irk = irk;
     a.st = b.st;
@@ -346,11 +476,11 @@ void launch_particles(vrh_t *h, int irk) {
     if (h->pscheme == VRH_PART_TRILINEAR_RK3) {
       a.partdt = h->dt;
       a.nsub = 1;
-      dim3 blk(256), grd((unsigned)((b.n + 255) / 256));
+      dim3 blk(256), grd((unsigned)((b.cap + 255) / 256));
       if (h->prec == VRH_MIXED) hipLaunchKernelGGL(k_part_tri<float>, grd, blk, 0, h->stream, a);
       else hipLaunchKernelGGL(k_part_tri<double>, grd, blk, 0, h->stream, a);
     } else {
-      dim3 blk(128), grd((unsigned)((b.n + 127) / 128));
+      dim3 blk(128), grd((unsigned)((b.cap + 127) / 128));
       if (h->prec == VRH_MIXED) hipLaunchKernelGGL(k_part_lsq<float>, grd, blk, 0, h->stream, a);
       else hipLaunchKernelGGL(k_part_lsq<double>, grd, blk, 0, h->stream, a);
     }
@@ -375,6 +505,8 @@ int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st
         hipMalloc(&B.alive, sizeof(int) * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
     hipMemsetAsync(B.u, 0, sizeof(double) * nvar * B.cap, h->stream);
     hipMemsetAsync(B.alive, 0, sizeof(int) * B.cap, h->stream);
+    if (hipMalloc(&B.d_n, sizeof(long)) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
+    hipMemsetAsync(B.d_n, 0, sizeof(long), h->stream);
     B.ddt2 = nullptr;
     if (scheme != VRH_PART_LSQ_EULER) {
       if (hipMalloc(&B.ddt2, sizeof(double) * nvar * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
@@ -403,6 +535,10 @@ int vrh_particles_upload(vrh_t *h, int batch, long n, const double *u, const int
   }
   if (B.ddt2) hipMemset(B.ddt2, 0, sizeof(double) * nvar * B.cap);
   B.n = n;
+  if (hipMemcpy(B.d_n, &n, sizeof(long), hipMemcpyHostToDevice) != hipSuccess) {
+    h->err = "particle upload failed";
+    return -3;
+  }
   return 0;
 }
 
@@ -423,6 +559,91 @@ int vrh_particles_download(vrh_t *h, int batch, long n, double *u, int *alive) {
     for (int v = 0; v < nvar; v++) u[(size_t)p * nvar + v] = soa[(size_t)v * B.cap + p];
   if (alive)
     for (long p = 0; p < n; p++) alive[p] = al[p];
+  return 0;
+}
+
+// pack the live particles of batch b to the front (device only)
+static int compact_batch(vrh_t *h, PartBatch &B) {
+  const int nvar = h->pscheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
+  const long nblk = (B.cap + 255) / 256;
+  if (!B.u_tmp) {
+    if (hipMalloc(&B.u_tmp, sizeof(double) * nvar * B.cap) != hipSuccess ||
+        hipMalloc(&B.alive_tmp, sizeof(int) * B.cap) != hipSuccess ||
+        hipMalloc(&B.work, sizeof(long) * (nblk + 8 + (long)(h->nr + 1) * h->nt * 2)) != hipSuccess ||
+        hipMalloc(&B.inflow, sizeof(double) * (h->nr + 2)) != hipSuccess) {
+      h->err = "particle allocation failed";
+      return -3;
+    }
+    if (B.ddt2 && hipMalloc(&B.ddt2_tmp, sizeof(double) * nvar * B.cap) != hipSuccess) {
+      h->err = "particle allocation failed";
+      return -3;
+    }
+    hipMemsetAsync(B.inflow, 0, sizeof(double) * (h->nr + 2), h->stream);
+  }
+  long *bsum = B.work, *total = B.work + nblk;
+  hipLaunchKernelGGL(k_alive_count, dim3((unsigned)nblk), dim3(256), 0, h->stream, B.alive, B.d_n, B.cap, bsum);
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, h->stream, bsum, nblk, total);
+  hipMemsetAsync(B.alive_tmp, 0, sizeof(int) * B.cap, h->stream);
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)nblk), dim3(256), 0, h->stream, B.u, B.ddt2, B.alive, B.d_n, B.cap,
+                     nvar, bsum, B.u_tmp, B.ddt2_tmp, B.alive_tmp);
+  hipMemcpyAsync(B.u, B.u_tmp, sizeof(double) * nvar * B.cap, hipMemcpyDeviceToDevice, h->stream);
+  if (B.ddt2) hipMemcpyAsync(B.ddt2, B.ddt2_tmp, sizeof(double) * nvar * B.cap, hipMemcpyDeviceToDevice, h->stream);
+  hipMemcpyAsync(B.alive, B.alive_tmp, sizeof(int) * B.cap, hipMemcpyDeviceToDevice, h->stream);
+  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, B.d_n, total);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int vrh_particles_compact(vrh_t *h, int batch) {
+  if (!h || batch < 0 || batch >= h->nbatch) return -1;
+  return compact_batch(h, h->pb[batch]);
+}
+
+int vrh_particles_count(vrh_t *h, int batch, long *n) {
+  if (!h || batch < 0 || batch >= h->nbatch || !n) return -1;
+  if (hipMemcpyAsync(n, h->pb[batch].d_n, sizeof(long), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess) {
+    h->err = "particle count failed";
+    return -3;
+  }
+  return 0;
+}
+
+int vrh_particles_inject(vrh_t *h, int batch, double r0, int inj, unsigned long long seed, long *ninj) {
+  if (!h || batch < 0 || batch >= h->nbatch || inj < 1) return -1;
+  PartBatch &B = h->pb[batch];
+  int rc = compact_batch(h, B);
+  if (rc) return rc;
+  const int nvar = h->pscheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
+  // inlet rows: rc(ir) < r0 (part_tools.F90:107)
+  int jr0 = 0;
+  const double dr = h->rmax / h->nr;
+  for (int ir = 1; ir <= h->nr; ir++)
+    if ((ir - 1) * dr < r0) jr0 = ir;
+  if (jr0 == 0) { if (ninj) *ninj = 0; return 0; }
+  const long nblk = (B.cap + 255) / 256;
+  long *total = B.work + nblk + 1, *counts = B.work + nblk + 8, *offs = counts + (long)jr0 * h->nt;
+  const double dth = 2.0 * acos(-1.0) / h->nt, dz = h->zmax / h->nz;
+  // c = 10^6 * 0.02^3 and nprcl = 0.001 in single precision (part_tools.F90:275-276)
+  const float c02 = 0.02f;
+  const double cc = (double)(1000000.0f * ((c02 * c02) * c02)), nprcl = (double)0.001f;
+  hipLaunchKernelGGL(k_inflow, dim3((unsigned)((jr0 + 63) / 64)), dim3(64), 0, h->stream, h->g, h->coef, h->f.q3,
+                     B.inflow, jr0, dth, h->dt, inj, cc, nprcl, counts);
+  hipMemcpyAsync(offs, counts, sizeof(long) * jr0 * h->nt, hipMemcpyDeviceToDevice, h->stream);
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, h->stream, offs, (long)jr0 * h->nt, total);
+  const long ncell = (long)jr0 * h->nt;
+  hipLaunchKernelGGL(k_inject, dim3((unsigned)((ncell + 127) / 128)), dim3(128), 0, h->stream, h->g, h->coef,
+                     h->f.q3, counts, offs, B.d_n, B.cap, nvar, B.u, B.ddt2, B.alive, jr0, dth, dr, dz, seed,
+                     B.ncall);
+  hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, h->stream, B.d_n, total, B.cap);
+  B.ncall++;
+  if (hipGetLastError() != hipSuccess) { h->err = "particle injection failed"; return -3; }
+  if (ninj) {
+    if (hipMemcpyAsync(ninj, total, sizeof(long), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) {
+      h->err = "particle injection failed";
+      return -3;
+    }
+  }
   return 0;
 }
 

@@ -34,7 +34,8 @@ EXPORTS = ["vrh_create", "vrh_destroy", "vrh_last_error", "vrh_upload_fields", "
            "vrh_step", "vrh_phase", "vrh_poisson_solve", "vrh_synchronize", "vrh_device_ptr",
            "vrh_ring_init", "vrh_cfl_dt", "vrh_particles_create", "vrh_particles_upload",
            "vrh_particles_download", "vrh_particles_step", "vrh_ktimer_enable", "vrh_ktimer_read", "vrh_autotune",
-           "vrh_set_graph", "vrh_set_tuning", "vrh_ensemble_create", "vrh_ensemble_step", "vrh_ensemble_last_error",
+           "vrh_set_graph", "vrh_set_tuning", "vrh_particles_compact", "vrh_particles_count",
+           "vrh_particles_inject", "vrh_ensemble_create", "vrh_ensemble_step", "vrh_ensemble_last_error",
            "vrh_ensemble_destroy"]
 
 _lib = None
@@ -77,6 +78,9 @@ def load_library(path=LIBPATH):
     L.vrh_ktimer_read.argtypes = [P, I, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(D),
                                   ctypes.POINTER(ctypes.c_long)]
     L.vrh_set_graph.argtypes = [P, I]
+    L.vrh_particles_compact.argtypes = [P, I]
+    L.vrh_particles_count.argtypes = [P, I, ctypes.POINTER(ctypes.c_long)]
+    L.vrh_particles_inject.argtypes = [P, I, D, I, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_long)]
     L.vrh_set_tuning.argtypes = [P, ip]
     L.vrh_ensemble_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
     L.vrh_ensemble_step.argtypes = [P, I]
@@ -292,6 +296,21 @@ class Solver:
         self.particles_upload(0, u)
         return xyz
 
+    def particles_compact(self, batch=0):
+        self._check(self.L.vrh_particles_compact(self.h, batch), "particles_compact")
+
+    def particles_count(self, batch=0):
+        n = ctypes.c_long(0)
+        self._check(self.L.vrh_particles_count(self.h, batch, ctypes.byref(n)), "particles_count")
+        return n.value
+
+    def particles_inject(self, batch=0, r0=0.5, inj=10, seed=0):
+        """Inlet injection (inlt_part); returns the number of new particles."""
+        n = ctypes.c_long(0)
+        self._check(self.L.vrh_particles_inject(self.h, batch, float(r0), int(inj), int(seed), ctypes.byref(n)),
+                    "particles_inject")
+        return n.value
+
     def device_ptr(self, name):
         p = ctypes.c_void_p()
         self._check(self.L.vrh_device_ptr(self.h, name.encode(), ctypes.byref(p)), "device_ptr")
@@ -344,7 +363,7 @@ HZ_EXPORTS = ["vrh_hz_create", "vrh_hz_destroy", "vrh_hz_last_error", "vrh_hz_se
               "vrh_hz_apply", "vrh_hz_direct", "vrh_hz_solve", "vrh_hz_net_size", "vrh_hz_net_init",
               "vrh_hz_net_get", "vrh_hz_net_set", "vrh_hz_net_forward", "vrh_hz_net_grad", "vrh_hz_sample",
               "vrh_hz_train"]
-HZ_METHOD = {"pcg": 0, "bicgstab": 1, "learned": 2}
+HZ_METHOD = {"pcg": 0, "bicgstab": 1, "learned": 2, "learned_fcg": 3}
 
 
 def _hz_bind(L):
