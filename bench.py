@@ -87,26 +87,36 @@ def run_path(prec, q, dt, steps, warmup, tune_reps=3):
     return wall, fields, kt, tuned
 
 
-def run_particles(prec, q, dt, n, steps):
-    """Particle step on the workload's field: n particles (5 um droplets,
-    batch 4 of part_tools.F90:262-269, nsubdt=5) uniform in the ring's torus;
-    timed over `steps` calls (one per RK3 step, as loop.F90:67)."""
+def run_particles(prec, q, dt, n, steps, scheme="lsq_euler"):
+    """Particle step on the workload's field: n particles uniform in the
+    ring's torus, timed over `steps` calls of one RK3 step's particle work.
+    lsq_euler: the reference's scheme (27-point least squares, Euler, 5 um
+    droplets of batch 4 of part_tools.F90:262-269, nsubdt = 5, called at
+    irk = 1 as loop.F90:67); trilinear_rk3: trilinear interpolation and the
+    RK3 stages (one per substep)."""
     import vring_amd
     s = vring_amd.Solver(NT, NR, NZ, RMAX, ZMAX, RE, dt, precision=prec)
     for name, a in zip(("q1", "q2", "q3"), q):
         s.upload(name, a)
-    s.seed_particles_torus(n, R=1.0, z0=3.0, a=0.6, seed=0, nsub=5)
-    s.particles_step(1)
+    nsub = 5 if scheme == "lsq_euler" else 1
+    s.seed_particles_torus(n, R=1.0, z0=3.0, a=0.6, seed=0, nsub=nsub, scheme=scheme)
+    stages = (1,) if scheme == "lsq_euler" else (1, 2, 3)
+    for irk in stages:
+        s.particles_step(irk)
     s.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        s.particles_step(1)
+        for irk in stages:
+            s.particles_step(irk)
     s.synchronize()
     t = time.perf_counter() - t0
     s.close()
-    return {"particles": n, "substeps_per_step": 5, "ms_per_step": round(t / steps * 1e3, 3),
-            "particles_per_s": round(n * steps / t, 1), "particle_substeps_per_s": round(5 * n * steps / t, 1),
-            "hbm_bytes_per_particle_substep": 2520}
+    nvar = 117 if scheme == "lsq_euler" else 6
+    per = (2 * nvar + (81 if scheme == "lsq_euler" else 2 * nvar)) * 8
+    return {"scheme": scheme, "particles": n, "substeps_per_step": nsub * len(stages),
+            "ms_per_step": round(t / steps * 1e3, 3), "particles_per_s": round(n * steps / t, 1),
+            "hbm_bytes_per_particle_substep": per,
+            "hbm_GBps": round(per * n * nsub * len(stages) * steps / t / 1e9, 1)}
 
 
 def run_ensemble(nrings, steps, warmup, seed=0, particles=50000):
@@ -152,14 +162,14 @@ def run_ensemble(nrings, steps, warmup, seed=0, particles=50000):
             "shapes_nr_nt_nz": shapes}
 
 
-def run_learned(nr, nz, nslots, train_samples, seed=0):
+def run_learned(nr, nz, nslots, train_samples, seed=0, pool=8):
     """Per-mode (r, z) Helmholtz on the tanh-stretched z grid (stretch 2),
     modes m = 0..nslots-1: train the network on `train_samples` (rhs, p)
     pairs the direct solver generates on the GPU, then solve ring-core right-hand
     sides to 1e-8 relative residual with plain PCG, BiCGSTAB, the learned
     flexible CG, and the direct solve."""
     import vring_amd
-    hz = vring_amd.Helmholtz(nr, nz, nslots, RMAX, ZMAX, 2.0)
+    hz = vring_amd.Helmholtz(nr, nz, nslots, RMAX, ZMAX, 2.0, pool=pool)
     hz.set_modes(np.arange(nslots, dtype=np.int32))
     hz.net_init(seed)
     t0 = time.perf_counter()
@@ -167,14 +177,21 @@ def run_learned(nr, nz, nslots, train_samples, seed=0):
     t_train = time.perf_counter() - t0
     hz.set_modes(np.arange(nslots, dtype=np.int32))
     f = hz.sample(seed + 99, 0)
-    out = {"nr": nr, "nz": nz, "modes": f"0..{nslots - 1}", "train_samples": train_samples,
+    out = {"nr": nr, "nz": nz, "modes": f"0..{nslots - 1}", "network_grid": [nz // pool, nr // pool],
+           "train_samples": train_samples,
            "train_s": round(t_train, 2), "loss_first": float(loss[:4].mean()), "loss_last": float(loss[-4:].mean())}
     pd = hz.direct(f)
     t0 = time.perf_counter()
     pd = hz.direct(f)
     out["direct_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
-    for m in ("pcg", "bicgstab", "learned", "learned_fcg"):
-        p, it, rel, ms = hz.solve(f, m, tol=1e-8, maxit=50000)
+    for m in ("pcg", "bicgstab", "learned", "learned_fcg", "jacobi_warm_start"):
+        w = None
+        if m == "jacobi_warm_start":  # the learned method with the network switched off
+            w = hz.net_get()
+            hz.net_set(np.zeros_like(w))
+        p, it, rel, ms = hz.solve(f, "learned" if w is not None else m, tol=1e-8, maxit=50000)
+        if w is not None:
+            hz.net_set(w)
         out[m] = {"iterations_total": int(it.sum()), "iterations_max": int(it.max()), "ms": round(ms, 3),
                   "max_true_relres": float(rel.max()),
                   "max_err_vs_direct": float(np.abs(p - pd).max() / np.abs(pd).max())}
@@ -250,7 +267,7 @@ KERNEL_KEYS = {  # rocprof kernel name prefix -> kernel timer name
 def _kernel_key(name):
     """('explicit', 'fp64') from a demangled rocprof kernel name."""
     base = name.split("(anonymous namespace)::", 1)[-1]
-    fn = base.split("<")[0]
+    fn = base.split("<")[0].split("(")[0]
     targs = base[len(fn) + 1:].split(">")[0] if "<" in base else ""
     prec = "fp64" if targs.startswith("double") else ("mixed" if targs.startswith("float") else "")
     if fn == "k_line":
@@ -292,6 +309,33 @@ def summarize_traffic(pmc_root, out_path):
     print(json.dumps(out))
 
 
+def summarize_stats(csv_path, out_path, command):
+    """Text summary of a rocprofv3 --kernel-trace --stats kernel_stats.csv:
+    per kernel (timer name, precision) calls and average duration."""
+    import csv
+    with open(csv_path) as f:
+        rows = list(csv.DictReader(f))
+    agg = {}
+    for r in rows:
+        key, prec = _kernel_key(r["Name"])
+        a = agg.setdefault((key, prec or "-"), [0, 0.0])
+        a[0] += int(r["Calls"])
+        a[1] += float(r["TotalDurationNs"])
+    tot = sum(v[1] for v in agg.values())
+    lines = [f"# rocprofv3 kernel statistics, workload 256x128x512 ring (bench.py)", "",
+             f"command: `{command}`", "",
+             "Kernel names are mapped to the solver's kernel-timer names; `type` is the",
+             "work type of the instantiation (in the mixed path the FP64 low-mode",
+             "transforms and radial solves appear as fp64).  Calls include the",
+             "autotuner's timing launches.", "",
+             "| kernel | type | calls | avg us | total ms | % |", "|---|---|---|---|---|---|"]
+    for (key, prec), (calls, ns) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"| {key} | {prec} | {calls} | {ns / calls / 1e3:.1f} | {ns / 1e6:.2f} | {100 * ns / tot:.1f} |")
+    with open(out_path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,7 +354,12 @@ def main():
                     help="run only the learned Helmholtz benchmark, training on SAMPLES pairs")
     ap.add_argument("--summarize-traffic", metavar="PMC_DIR", default=None,
                     help="write profiles/traffic_256x128x512.json from rocprofv3 --pmc output under PMC_DIR")
+    ap.add_argument("--summarize-stats", nargs=3, metavar=("CSV", "OUT", "COMMAND"), default=None,
+                    help="write a text summary of a rocprofv3 kernel_stats.csv")
     args = ap.parse_args()
+    if args.summarize_stats:
+        summarize_stats(*args.summarize_stats)
+        return
     if args.summarize_traffic:
         summarize_traffic(args.summarize_traffic, os.path.join(ROOT, "profiles", "traffic_256x128x512.json"))
         return
@@ -339,8 +388,9 @@ def main():
     wallmx, fmx, ktmx, tunemx = run_path("mixed", q, dt, args.steps, args.warmup, args.autotune_reps)
     parts = None
     if args.particles:
-        parts = {"fp64": run_particles("fp64", q, dt, args.particles, max(2, args.steps // 4)),
-                 "mixed": run_particles("mixed", q, dt, args.particles, max(2, args.steps // 4))}
+        ps = max(2, args.steps // 4)
+        parts = {f"{prec}_{sch}": run_particles(prec, q, dt, args.particles, ps, sch)
+                 for prec in ("fp64", "mixed") for sch in ("lsq_euler", "trilinear_rk3")}
     v64 = ncell * args.steps / wall64 / 1e6
     vmx = ncell * args.steps / wallmx / 1e6
     err = rel_errors(fmx, f64)
