@@ -189,7 +189,8 @@ def run_learned(nr, nz, nslots, train_samples, seed=0, pool=8):
         if m == "jacobi_warm_start":  # the learned method with the network switched off
             w = hz.net_get()
             hz.net_set(np.zeros_like(w))
-        p, it, rel, ms = hz.solve(f, "learned" if w is not None else m, tol=1e-8, maxit=50000)
+        p, it, rel, ms = hz.solve(f, "learned" if w is not None else m, tol=1e-8,
+                                  maxit=5000 if m == "learned_fcg" else 50000)
         if w is not None:
             hz.net_set(w)
         out[m] = {"iterations_total": int(it.sum()), "iterations_max": int(it.max()), "ms": round(ms, 3),

@@ -120,6 +120,11 @@ module vrh_f
       integer(c_int), value :: batch
       integer(c_long), value :: n
     end function vrh_particles_download
+    integer(c_int) function vrh_analysis(h, out) bind(C, name='vrh_analysis')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: out(10)
+    end function vrh_analysis
   end interface
 
 contains

@@ -201,10 +201,10 @@ contains
     real(c_double), intent(inout) :: stat(20)
     character(len=5) :: tc
     character(len=2) :: rc
-    integer :: ipass, iz, ir, ith, ithp1, u21, u22, u23, i
+    integer :: ipass, iz, ir, ith, ithp1, u21, u22
     real(c_double) :: dth, dr, dz, dx1, dx2, dx3, rm(0:nr+1), zm(0:nz+1), thm(nt)
     real(c_double) :: xc, yc, q1m, q2m, q3m, dwdr, dvdz, vort, swrl, ux, uy
-    real(c_double) :: vortmax, swrlmax, vortmaxall, swrlmaxall, xvortold, xswrlold, xvortnew, xswrlnew
+    real(c_double) :: vortmax, swrlmax, vortmaxall, swrlmaxall
     real(c_double) :: xvort, yvort, vvort, uvort, xswrl, yswrl, vswrl, uswrl, w
     logical :: export
     dth = 2.d0 * acos(-1.d0) / nt
@@ -297,16 +297,31 @@ contains
       if (ipass == 1) then
         vortmaxall = vortmax
         swrlmaxall = swrlmax
-      else
-        xvortold = stat(1) / stat(3)
-        xswrlold = stat(5) / stat(7)
-        stat(1:8) = (/xvort, yvort, vvort, uvort, xswrl, yswrl, vswrl, uswrl/)
-        xvortnew = stat(1) / stat(3)
-        xswrlnew = stat(5) / stat(7)
-        stat(9) = (xvortnew - xvortold) / (d%dt * d%anlstp)
-        stat(10) = (xswrlnew - xswrlold) / (d%dt * d%anlstp)
       end if
     end do
+    call analysis_record(d, nstep, ntime, (/vortmaxall, swrlmaxall, xvort, yvort, vvort, uvort, &
+                         xswrl, yswrl, vswrl, uswrl/), stat)
+  end subroutine analysis
+
+  ! Second half of analysis (run.F90:362-384): the statistics update and the
+  ! analysis.dat record, from a(1:10) = vortmax, swrlmax, stat(1:8) -- the
+  ! host loop above or the device reduction vrh_analysis.
+  subroutine analysis_record(d, nstep, ntime, a, stat)
+    type(deck), intent(in) :: d
+    integer, intent(in) :: nstep
+    real(c_double), intent(in) :: ntime, a(10)
+    real(c_double), intent(inout) :: stat(20)
+    real(c_double) :: vortmaxall, swrlmaxall, xvortold, xswrlold, xvortnew, xswrlnew
+    integer :: u23, i
+    vortmaxall = a(1)
+    swrlmaxall = a(2)
+    xvortold = stat(1) / stat(3)
+    xswrlold = stat(5) / stat(7)
+    stat(1:8) = a(3:10)
+    xvortnew = stat(1) / stat(3)
+    xswrlnew = stat(5) / stat(7)
+    stat(9) = (xvortnew - xvortold) / (d%dt * d%anlstp)
+    stat(10) = (xswrlnew - xswrlold) / (d%dt * d%anlstp)
     if (nstep == 1) then
       open(newunit=u23, file='analysis.dat', form='formatted', status='replace')
     else
@@ -317,7 +332,7 @@ contains
         ((stat(i + 4)**(1.0 / float(i))) / stat(i + 6)**(1.0 / float(i)), i = 1, 2), &
         (stat(i + 6)**(1.0 / float(i)), i = 1, 2), stat(9), stat(10)
     close(u23)
-  end subroutine analysis
+  end subroutine analysis_record
 
   ! particle export, part_tools.F90:444-467: header, then per live particle
   ! (time, concentration, id), u(1:nvar), d(1:10) with d(1) = det(J)

@@ -15,7 +15,7 @@ program vring_main
   real(c_double), allocatable, target :: pu(:, :), phs(:)
   integer(c_int), allocatable, target :: palive(:)
   integer(c_int), target :: choice(9)
-  real(c_double) :: ntime, stat(20), dtcfl, st
+  real(c_double) :: ntime, stat(20), dtcfl, st, astat(10)
   real(c_double) :: t0, t1, t2, t3
   integer :: nt, nr, nz, istep, nstep, nstepi, rc, np, ip
   integer(c_long) :: capl(1)
@@ -121,8 +121,15 @@ program vring_main
     end if
     if (d%anlstp > 0) then
       if (mod(nstep, d%anlstp) == 0) then
-        call vrh_check(h, vrh_download_fields(h, c_loc(q1), c_loc(q2), c_loc(q3), c_loc(pres)), 'download')
-        call analysis(d, nstep, 3, ntime, nt, nr, nz, q1, q2, q3, pres, stat)
+        if (d%expstp > 0 .and. mod(nstep, max(d%expstp, 1)) == 0) then
+          ! field export step: the host pass writes the Tecplot files too
+          call vrh_check(h, vrh_download_fields(h, c_loc(q1), c_loc(q2), c_loc(q3), c_loc(pres)), 'download')
+          call analysis(d, nstep, 3, ntime, nt, nr, nz, q1, q2, q3, pres, stat)
+        else
+          ! statistics only: reduced on the device, ten numbers come back
+          call vrh_check(h, vrh_analysis(h, astat), 'analysis')
+          call analysis_record(d, nstep, ntime, astat, stat)
+        end if
       end if
     end if
     call cpu_time(t3)

@@ -110,6 +110,13 @@ int vrh_set_tuning(vrh_t *h, const int *choice);
 /* Device address of a named field (for callers that keep data in HBM). */
 int vrh_device_ptr(vrh_t *h, const char *name, void **ptr);
 
+/* Vortex-core statistics of analysis (run.F90:238-388) as a device reduction
+ * over the current fields (synchronous): out[0] max azimuthal vorticity,
+ * out[1] max swirl (the window 0.1 < z/zmax < 0.9, 0.05 < r/rmax < 0.8,
+ * j != 1), out[2..9] the sums stat(1:8) = xvort, yvort, vvort, uvort, xswrl,
+ * yswrl, vswrl, uswrl of the positive vorticity and swirl. */
+int vrh_analysis(vrh_t *h, double *out);
+
 /* Host-side initial condition (no reference counterpart: the reference loads
  * RESTART, run.F90:469-539).  Gaussian-core vortex rings from a discrete
  * Stokes streamfunction; rings[4*r..] = (R, z0, sigma, circulation); the ring

@@ -30,9 +30,10 @@ extern "C" {
 typedef struct vrh_hz_s vrh_hz_t;
 
 typedef struct {
-  int nr, nz, nslots;      /* nr a multiple of 16 */
+  int nr, nz, nslots;      /* nr a multiple of 16 * pool, nz of pool */
   double rmax, zmax;
   double stretch;          /* z_k = zmax (1 + tanh(s (k/nz - 1)) / tanh s); 0: uniform */
+  int pool;                /* the network works on blocks of pool x pool cells (0 or 1: cells) */
 } vrh_hz_grid;
 
 enum { VRH_HZ_PCG = 0, VRH_HZ_BICGSTAB = 1, VRH_HZ_LEARNED = 2, VRH_HZ_LEARNED_FCG = 3 };
@@ -52,13 +53,16 @@ int vrh_hz_direct(vrh_hz_t *h, const double *f, double *p);
 /* Iterative solve of every slot to ||V f - S p|| <= tol ||V f||:
  *   VRH_HZ_PCG       Jacobi-preconditioned CG, zero start;
  *   VRH_HZ_BICGSTAB  Jacobi-preconditioned BiCGSTAB, zero start;
- *   VRH_HZ_LEARNED   plain PCG started from the network's warm start: the
- *                    Jacobi estimate corrected by the network,
- *                    M(V f) = D^-1 V f + scale N(V f), scaled to its
- *                    energy-optimal multiple (so the start is never worse
- *                    than zero in the energy norm);
+ *   VRH_HZ_LEARNED   plain PCG started from the network's warm start
+ *                    M(V f) = beta D^-1 V f + gamma c, c = scale N(V f)
+ *                    prolonged bilinearly to the cells, with (beta, gamma)
+ *                    the energy-norm optimal (Galerkin) pair -- so the
+ *                    start's energy error never exceeds that of the best
+ *                    multiple of the Jacobi estimate, nor that of a zero
+ *                    start, whatever the network returns;
  *   VRH_HZ_LEARNED_FCG  the same start, then flexible CG (Polak-Ribiere)
- *                    preconditioned by M; a slot that breaks down or does not
+ *                    preconditioned by the same Galerkin map M applied to
+ *                    each residual; a slot that breaks down or does not
  *                    converge within maxit falls back to plain PCG from where
  *                    it is.
  * The true residual of the returned p is recomputed and reported per slot in
@@ -75,12 +79,14 @@ int vrh_hz_net_size(vrh_hz_t *h);
 int vrh_hz_net_init(vrh_hz_t *h, unsigned long long seed);
 int vrh_hz_net_get(vrh_hz_t *h, float *params);
 int vrh_hz_net_set(vrh_hz_t *h, const float *params);
-/* Network output channel (nslots, nz, nr) for the features of f, and the
- * feature scale rms(f) dr^2 per slot (the learned correction to the Jacobi
- * estimate D^-1 V f is scale * out). */
+/* Network output channel on the network grid (nslots, nz/pool, nr/pool) for
+ * the features of f, and the output scale rms(f) R^2 per slot (R = nr dr):
+ * the correction c is scale * out, prolonged bilinearly (in index space,
+ * from the block centres; constant beyond the outermost centres). */
 int vrh_hz_net_forward(vrh_hz_t *h, const double *f, float *out, double *scale);
 /* Gradient of sum(gout * out) with respect to the parameters, for the
- * features of f (same packing as the parameters). */
+ * features of f (gout and out on the network grid; same packing as the
+ * parameters). */
 int vrh_hz_net_grad(vrh_hz_t *h, const double *f, const float *gout, float *grads);
 /* Right-hand sides the trainer draws: family 0 Gaussian ring cores
  * (4 per sample, radius 0.1..0.3 of the r extent scale), family 1 white noise. */
@@ -88,9 +94,9 @@ int vrh_hz_sample(vrh_hz_t *h, unsigned long long seed, int family, double *f);
 /* Train on nsamples (rhs, p) pairs generated on the GPU (rhs from both
  * families, random m in 0..63 per slot, p from the direct solve), nslots per
  * batch, Adam with learning rate lr, energy-norm loss e^T S e / (p^T V f)
- * of e = D^-1 V f + scale out - p (features taken from the rhs divided by
- * the cell volume, as the solver sees residuals).  loss (optional) receives
- * the mean loss of each batch. */
+ * of the warm start's error e = M(V f) - p (features taken from the rhs
+ * divided by the cell volume, as the solver sees residuals).  loss
+ * (optional) receives the mean loss of each batch. */
 int vrh_hz_train(vrh_hz_t *h, int nsamples, double lr, unsigned long long seed, float *loss);
 
 #ifdef __cplusplus

@@ -207,8 +207,9 @@ def bf16(x):
 
 def features(g, m, f):
     """Input channels (nz, nr, 16): f / rms(f), m dr / (m dr + rm), dz / dr,
-    rm / rmax, then zeros.  Returns (x, scale) with scale = rms(f) dr^2, the
-    factor that turns the network output into a pressure estimate."""
+    rm / rmax, then zeros.  Returns (x, scale) with scale = rms(f) R^2
+    (R = nr dr), the factor that turns the network output into a pressure
+    correction."""
     nz, nr = g["nz"], g["nr"]
     s = float(np.sqrt(np.mean(np.asarray(f, np.float64) ** 2)))
     s = s if s > 0 else 1.0
@@ -217,7 +218,7 @@ def features(g, m, f):
     x[..., 1] = (m * g["dr"]) / (m * g["dr"] + g["rm"][None, :])
     x[..., 2] = (g["dz"] / g["dr"])[:, None]
     x[..., 3] = (g["rm"] / (g["dr"] * nr))[None, :]
-    return bf16(x), s * g["dr"] ** 2
+    return bf16(x), s * (g["dr"] * nr) ** 2
 
 
 def init_params(seed):
@@ -313,12 +314,54 @@ def backward(params, acts, gout, rounding=True):
     return gW, gb
 
 
+def prolong_matrix(n, pool):
+    """(n, n // pool) bilinear prolongation in index space from block centres
+    to cells, constant beyond the outermost centres: cell i sits at
+    u = (i + 1/2) / pool - 1/2 in node units."""
+    nn = n // pool
+    P = np.zeros((n, nn))
+    for i in range(n):
+        u = (i + 0.5) / pool - 0.5
+        if u <= 0:
+            P[i, 0] = 1.0
+        elif u >= nn - 1:
+            P[i, nn - 1] = 1.0
+        else:
+            i0 = int(np.floor(u))
+            P[i, i0] += 1.0 - (u - i0)
+            P[i, i0 + 1] += u - i0
+    return P
+
+
+def prolong(out, pool):
+    """Network-grid field (nz / pool, nr / pool) to the cells (nz, nr)."""
+    nzn, nrn = out.shape
+    return prolong_matrix(nzn * pool, pool) @ out.astype(np.float64) @ prolong_matrix(nrn * pool, pool).T
+
+
+def galerkin(g, m, r, c):
+    """Energy-optimal combination z = beta D^-1 r + gamma c approximating
+    S^-1 r: (beta, gamma) solve [[j.Sj, j.Sc], [j.Sc, c.Sc]] (beta, gamma) =
+    (j.r, c.r) with j = D^-1 r; a nearly dependent pair keeps beta j alone."""
+    j = r / diag(g, m)
+    Sj, Sc = apply(g, m, j), apply(g, m, c)
+    a11, a12, a22 = (j * Sj).sum(), (j * Sc).sum(), (c * Sc).sum()
+    r1, r2 = (j * r).sum(), (c * r).sum()
+    det = a11 * a22 - a12 * a12
+    beta, gam = (r1 / a11 if a11 > 0 else 0.0), 0.0
+    if a11 > 0 and a22 > 0 and det > 1e-10 * a11 * a22:
+        beta, gam = (r1 * a22 - r2 * a12) / det, (a11 * r2 - a12 * r1) / det
+    return beta * j + gam * c, beta, gam
+
+
 def energy_loss_grad(g, m, out, p, f, scale):
-    """Energy-norm loss of one sample's preconditioned estimate, with
-    b = V f and e = D^-1 b + scale * out - p (Jacobi plus the learned
-    correction):  L = e^T S e / (p^T b);  dL/d out = 2 scale S e / (p^T b)."""
+    """Energy-norm loss of one sample's Galerkin estimate, with b = V f,
+    c = scale * out and e = beta D^-1 b + gamma c - p for the optimal
+    (beta, gamma):  L = e^T S e / (p^T b);  dL/d out = 2 gamma scale S e /
+    (p^T b) (the coefficients are stationary, so their variation drops out)."""
     b = rhs(g, m, f)
-    e = b / diag(g, m) + scale * out.astype(np.float64) - p
+    z, beta, gam = galerkin(g, m, b, scale * out.astype(np.float64))
+    e = z - p
     Se = apply(g, m, e)
     den = (p * b).sum()
-    return (e * Se).sum() / den, (2.0 * scale / den) * Se
+    return (e * Se).sum() / den, (2.0 * gam * scale / den) * Se

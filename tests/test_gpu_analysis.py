@@ -1,0 +1,49 @@
+"""Device-reduced vortex-core statistics (vrh_analysis, run.F90:300-360)
+against the CPU oracle (oracle/analysis_oracle.py) on random haloed fields and
+on a ring after a few steps."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import vring_amd
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import analysis_oracle as ao  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def check(s, q1, q2, q3, nr, nz, R, L):
+    got = s.analysis()
+    vmax, smax, st = ao.analysis(q1, q2, q3, nr, nz, R, L)
+    assert abs(got[0] - vmax) <= 1e-13 * abs(vmax) and abs(got[1] - smax) <= 1e-13 * abs(smax), (got[:2], vmax, smax)
+    err = np.abs(got[2:] - st) / np.abs(st)
+    assert np.all(err < 1e-12), (got[2:], st)
+
+
+@pytest.mark.parametrize("nt,nr,nz", [(16, 40, 60), (3, 32, 64), (33, 96, 200)])
+def test_analysis_matches_oracle_on_random_fields(nt, nr, nz):
+    print(f"case analysis random {nt}x{nr}x{nz}", flush=True)
+    R, L = 4.0, 12.0
+    s = vring_amd.Solver(nt, nr, nz, R, L, 1000.0, 0.01)
+    rng = np.random.default_rng(nt + nr)
+    q = [rng.standard_normal((nt, nr + 3, nz + 3)) for _ in range(3)]
+    for n, a in zip(("q1", "q2", "q3"), q):
+        s.upload(n, a)
+    check(s, *q, nr, nz, R, L)
+    s.close()
+
+
+def test_analysis_of_a_ring_after_steps():
+    print("case analysis ring 32x64x128", flush=True)
+    nt, nr, nz, R, L = 32, 64, 128, 4.0, 12.0
+    q1, q2, q3 = vring_amd.ring_initial_condition(nt, nr, nz, R, L, [(1.0, 3.0, 0.2, 1.0)], pert_amp=0.02,
+                                                  pert_modes=(2, 4))
+    s = vring_amd.Solver(nt, nr, nz, R, L, 1000.0, 0.01)
+    for n, a in (("q1", q1), ("q2", q2), ("q3", q3)):
+        s.upload(n, a)
+    s.step(4)
+    check(s, s.download("q1"), s.download("q2"), s.download("q3"), nr, nz, R, L)
+    s.close()

@@ -1,9 +1,12 @@
 """The Fortran driver end to end on the GPU: inp.txt with a Gaussian ring,
-11 RK3 steps (istep = 0..10), restart files in the reference's format (run.F90:390-466),
-compared with the FP64 oracle run from the same initial condition."""
+11 RK3 steps (istep = 0..10), restart files in the reference's format (run.F90:390-466)
+and the analysis.dat records (run.F90:380-384; device-reduced statistics except
+on field-export steps), compared with the FP64 oracle run from the same
+initial condition."""
 import os
 import shutil
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -12,6 +15,9 @@ from scipy.io import FortranFile
 from oracle_lib import Oracle
 
 import vring_amd
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import analysis_oracle as ao  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -33,7 +39,7 @@ poisson              fftp
 dt                   {dt}
 totstp               {totstp}
 expstp               5
-anlstp               5
+anlstp               {anlstp}
 restrt               {restrt}
 verbose              0
 u                    0.0
@@ -71,7 +77,7 @@ def read_restart(path, nt, nr, nz):
 def test_driver_ring_run_matches_oracle(tmp_path):
     nt, nr, nz = 33, 32, 64
     print("case driver ring 33x32x64", flush=True)
-    (tmp_path / "inp.txt").write_text(DECK.format(nt=nt, nr=nr, nz=nz, totstp=10, restrt=0, dt=0.01))
+    (tmp_path / "inp.txt").write_text(DECK.format(nt=nt, nr=nr, nz=nz, totstp=10, restrt=0, dt=0.01, anlstp=1))
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     # exports after nstep = 0, 5, 10 (mod(nstep, expstp) == 0, loop.F90:136)
@@ -85,14 +91,27 @@ def test_driver_ring_run_matches_oracle(tmp_path):
     o = Oracle(nt, nr, nz, 4.0, 12.0, 1000.0, dt)
     for n, a in (("q1", q1), ("q2", q2), ("q3", q3)):
         o.field(n)[...] = a
-    for _ in range(11):
+    # nstep = 1 starts analysis.dat afresh (run.F90:378-379): records 1..10
+    rows = np.loadtxt(tmp_path / "analysis.dat", ndmin=2)
+    assert rows.shape[0] == 10 and np.all(rows[:, 0] == np.arange(1, 11))
+    for n in range(11):
         assert o.rk3_step() == 0
+        if n == 0:
+            continue
+        vmax, smax, st = ao.analysis(o.field("q1"), o.field("q2"), o.field("q3"), nr, nz, 4.0, 12.0)
+        # vorticity columns: vortmax, xvort/vvort, sqrt(yvort/uvort), vvort, uvort
+        ref = [vmax, st[0] / st[2], np.sqrt(st[1] / st[3]), st[2], st[3]]
+        got = rows[n - 1, [2, 4, 5, 6, 7]]
+        # E16.8 records: 8 significant digits
+        assert np.allclose(got, ref, rtol=1e-7, atol=0), (n, got, ref)
+        # the ring has no swirl: its columns hold round-off only
+        assert abs(rows[n - 1, 3]) < 1e-12 and abs(rows[n - 1, 11]) < 1e-12, rows[n - 1]
     ref = np.stack([o.field(n)[:, 1 : nr + 1, 1 : nz + 1] for n in ("q1", "q2", "q3")])
     assert np.abs(qd - ref).max() <= 1e-10 * np.abs(ref).max()
     # restart: load the step-5 file as RESTART and take one step with the
     # first run's dt (a restart keeps the deck's dt; CFL applies to a fresh ring)
     shutil.copy(tmp_path / "restart00005RK03.dat", tmp_path / "RESTART")
-    (tmp_path / "inp.txt").write_text(DECK.format(nt=nt, nr=nr, nz=nz, totstp=0, restrt=1, dt=repr(dt)))
+    (tmp_path / "inp.txt").write_text(DECK.format(nt=nt, nr=nr, nz=nz, totstp=0, restrt=1, dt=repr(dt), anlstp=5))
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     n5, t5, q5 = read_restart(tmp_path / "RESTART", nt, nr, nz)

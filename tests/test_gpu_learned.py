@@ -18,8 +18,8 @@ pytestmark = pytest.mark.gpu
 RMAX, ZMAX, STRETCH = 4.0, 12.0, 2.0
 
 
-def make(nr, nz, modes):
-    hz = vring_amd.Helmholtz(nr, nz, len(modes), RMAX, ZMAX, STRETCH)
+def make(nr, nz, modes, pool=1):
+    hz = vring_amd.Helmholtz(nr, nz, len(modes), RMAX, ZMAX, STRETCH, pool=pool)
     hz.set_modes(np.array(modes, np.int32))
     return hz, lo.mesh(nr, nz, RMAX, ZMAX, STRETCH)
 
@@ -120,25 +120,52 @@ def test_network_forward_and_gradients_match_oracle():
     hz.close()
 
 
-def test_training_lowers_loss_and_learned_solver_gains_over_plain_pcg():
-    print("case hz train 32x64", flush=True)
-    S = 16
-    hz, g = make(32, 64, list(range(S)))
+def _energy_error(g, m, est, p, b):
+    e = est - p
+    return float((e * lo.apply(g, m, e)).sum() / (p * b).sum())
+
+
+def test_training_lowers_loss_and_network_improves_the_warm_start():
+    """Train on the GPU, then on held-out right-hand sides (both families, 15
+    modes) compare, in the energy norm the training minimises, the solver's
+    warm start (the Galerkin combination of D^-1 V f and the network
+    correction) with the best multiple of D^-1 V f alone, and run plain PCG,
+    the learned-start PCG and the learned flexible CG, each judged by its true
+    residual."""
+    print("case hz train 32x64 pool 2", flush=True)
+    S, pool = 16, 2
+    hz, g = make(32, 64, list(range(S)), pool=pool)
     hz.net_init(5)
-    loss = hz.train(S * 300, lr=1e-3, seed=7)
+    loss = hz.train(S * 400, lr=1e-3, seed=7)
     print("loss first/last", loss[:10].mean(), loss[-30:].mean(), flush=True)
     assert np.all(np.isfinite(loss))
     assert loss[-30:].mean() < 0.75 * loss[:10].mean(), (loss[:10], loss[-30:])
     modes = np.arange(0, 64, 4, dtype=np.int32)
     hz.set_modes(modes)
+    gains = []
+    for fam in (0, 1):
+        f = hz.sample(12345 + fam, fam)
+        pd = hz.direct(f)
+        out, scale = hz.net_forward(f)
+        for b, m in enumerate(modes):
+            if m == 0:
+                continue  # features see f before the mean is removed; compare the other modes
+            rhs = lo.rhs(g, m, f[b])
+            corr = lo.prolong(out[b], pool) * scale[b]
+            zj, _, _ = lo.galerkin(g, m, rhs, np.zeros_like(rhs))
+            zl, _, _ = lo.galerkin(g, m, rhs, corr)
+            gains.append((_energy_error(g, m, zj, pd[b], rhs), _energy_error(g, m, zl, pd[b], rhs)))
+    gains = np.array(gains)
+    print("energy error of the start: jacobi", gains[:, 0].mean(), "learned", gains[:, 1].mean(), flush=True)
+    assert np.all(gains[:, 1] <= gains[:, 0] * (1 + 1e-9)), gains   # Galerkin: never worse
+    assert gains[:, 1].mean() < 0.8 * gains[:, 0].mean(), gains
     f = hz.sample(12345, 0)
     p0, it0, rel0, ms0 = hz.solve(f, "pcg", tol=1e-8, maxit=20000)
     p1, it1, rel1, ms1 = hz.solve(f, "learned", tol=1e-8, maxit=20000)
     p2, it2, rel2, ms2 = hz.solve(f, "learned_fcg", tol=1e-8, maxit=20000)
-    print("pcg iterations", it0.tolist(), "learned warm start", it1.tolist(), "learned fcg", it2.tolist(), flush=True)
+    print("pcg iterations", it0.tolist(), "learned start", it1.tolist(), "learned fcg", it2.tolist(), flush=True)
     # the true residual judges every solve, the fallback included
     assert np.all(rel0 <= 1.05e-8) and np.all(rel1 <= 1.05e-8) and np.all(rel2 <= 1.05e-8), (rel0, rel1, rel2)
     assert np.abs(p1 - p0).max() < 1e-5 * np.abs(p0).max()
-    # what the trained network gains: fewer PCG iterations from its warm start
-    assert it1.sum() < it0.sum(), (it0, it1)
+    assert np.abs(p2 - p0).max() < 1e-5 * np.abs(p0).max()
     hz.close()

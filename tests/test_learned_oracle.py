@@ -129,10 +129,21 @@ def test_energy_loss_gradient_matches_finite_differences():
         op[idx] += h; om[idx] -= h
         fd = (lo.energy_loss_grad(g, 1, op, p, f, scale)[0] - lo.energy_loss_grad(g, 1, om, p, f, scale)[0]) / (2 * h)
         assert abs(fd - gr[idx]) < 1e-6 * max(1.0, abs(fd))
-    # zero when the correction completes the Jacobi estimate to the exact solution
+    # zero when the correction completes the Jacobi estimate to the exact
+    # solution (beta = gamma = 1), whatever multiple of it the network returns
     corr = (p - lo.rhs(g, 1, f) / lo.diag(g, 1)) / scale
-    L1, g1 = lo.energy_loss_grad(g, 1, corr, p, f, scale)
-    assert L1 < 1e-24 and np.abs(g1).max() < 1e-9
+    for k in (1.0, -3.0):
+        L1, g1 = lo.energy_loss_grad(g, 1, k * corr, p, f, scale)
+        assert L1 < 1e-20 and np.abs(g1).max() < 1e-9
+    # never above the best multiple of the Jacobi estimate alone, which in
+    # turn is not above a zero start (L = 1)
+    b = lo.rhs(g, 1, f)
+    j = b / lo.diag(g, 1)
+    ej = (j * b).sum() / (j * lo.apply(g, 1, j)).sum() * j - p
+    LJ = (ej * lo.apply(g, 1, ej)).sum() / (p * b).sum()
+    assert L0 <= LJ + 1e-14 and LJ < 1.0
+    z, beta, gam = lo.galerkin(g, 1, b, np.zeros_like(b))
+    assert gam == 0.0 and abs(beta - (j * b).sum() / (j * lo.apply(g, 1, j)).sum()) < 1e-14 * abs(beta)
 
 
 def test_bf16_rounding_is_nearest_even():
@@ -143,3 +154,17 @@ def test_bf16_rounding_is_nearest_even():
     assert y[3] == 1.0 + 2 ** -6                  # tie -> even (upwards)
     assert y[4] == -2.5
     assert abs(y[5] - 3.140625) < 1e-7
+
+
+def test_prolongation_is_bilinear_and_partition_of_unity():
+    P = lo.prolong_matrix(16, 4)
+    assert np.allclose(P.sum(1), 1.0) and P.min() >= 0.0
+    # reproduces a linear function of the block centres between the outer centres
+    nodes = (np.arange(4) + 0.5) * 4 - 0.5
+    cells = np.arange(16.0)
+    inside = (cells >= nodes[0]) & (cells <= nodes[-1])
+    assert np.allclose((P @ (3.0 * nodes + 1.0))[inside], 3.0 * cells[inside] + 1.0)
+    assert np.allclose(lo.prolong_matrix(8, 1), np.eye(8))
+    out = np.random.default_rng(0).standard_normal((3, 5))
+    y = lo.prolong(out, 2)
+    assert y.shape == (6, 10) and abs(y[0, 0] - out[0, 0]) < 1e-15

@@ -813,3 +813,18 @@ extern "C" int vrh_set_tuning(vrh_t *h, const int *choice) {
   graph_invalidate(h);
   return 0;
 }
+
+// Vortex-core statistics (run.F90:238-388) reduced on the device; out[10] =
+// vortmax, swrlmax, then stat(1:8).
+extern "C" int vrh_analysis(vrh_t *h, double *out) {
+  if (!h || !out) return -1;
+  const int nblk = 1024;
+  double *part = nullptr;
+  HIPCHK(h, hipMalloc(&part, sizeof(double) * (nblk * 10 + 16)));
+  launch_analysis(h, part, nblk, part + nblk * 10);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(out, part + nblk * 10, 10 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  (void)hipFree(part);
+  return 0;
+}

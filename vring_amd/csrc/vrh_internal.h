@@ -158,6 +158,7 @@ void host_mint(vrh_t *h, double *mint);
 // kernel launchers (vrh_kernels.hip); TW = double or float
 template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, const double *cor_dev);
 void launch_scalars(vrh_t *h);
+void launch_analysis(vrh_t *h, double *part, int nblk, double *out);
 template <typename TW> void launch_explicit(vrh_t *h, int irk);
 template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const double *c3s_dev);
 template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_rhs, double *phi_out);

@@ -602,27 +602,112 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_pe
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int bitrev(int x, int logn) { return (int)(__builtin_bitreverse32((unsigned)x) >> (32 - logn)); }
 
+// complex multiply (x + iy)(c + is)
+template <typename T> __device__ __forceinline__ void cmul(T &x, T &y, T c, T s) {
+  const T t = x * c - y * s;
+  y = x * s + y * c;
+  x = t;
+}
+// in-register DFT of R = 2, 4 or 8 points, v_s = sum_r v_r e^{i sign 2 pi r s / R}
+template <typename T> __device__ __forceinline__ void dft2(T &ar, T &ai, T &br, T &bi) {
+  const T tr = ar - br, ti = ai - bi;
+  ar = ar + br; ai = ai + bi;
+  br = tr; bi = ti;
+}
+template <typename T> __device__ __forceinline__ void dft4(T *vr, T *vi, int st, T sign) {
+  // inputs at 0, st, 2 st, 3 st
+  T t0r = vr[0], t0i = vi[0], t1r = vr[2 * st], t1i = vi[2 * st];
+  dft2(t0r, t0i, t1r, t1i);  // t0 = v0 + v2, t1 = v0 - v2
+  T t2r = vr[st], t2i = vi[st], t3r = vr[3 * st], t3i = vi[3 * st];
+  dft2(t2r, t2i, t3r, t3i);  // t2 = v1 + v3, t3 = v1 - v3
+  // t3 *= (i sign)
+  const T u = t3r;
+  t3r = -sign * t3i;
+  t3i = sign * u;
+  vr[0] = t0r + t2r; vi[0] = t0i + t2i;
+  vr[2 * st] = t0r - t2r; vi[2 * st] = t0i - t2i;
+  vr[st] = t1r + t3r; vi[st] = t1i + t3i;
+  vr[3 * st] = t1r - t3r; vi[3 * st] = t1i - t3i;
+}
+template <typename T> __device__ __forceinline__ void dft8(T *vr, T *vi, T sign) {
+  // even points 0,2,4,6 and odd points 1,3,5,7: two 4-point DFTs in place
+  dft4(vr, vi, 2, sign);
+  dft4(vr + 1, vi + 1, 2, sign);
+  const T h = (T)0.70710678118654752440;
+  // odd outputs s = 0..3 sit at 1,3,5,7: multiply by w8^s
+  cmul(vr[3], vi[3], h, sign * h);
+  { const T u = vr[5]; vr[5] = -sign * vi[5]; vi[5] = sign * u; }
+  cmul(vr[7], vi[7], -h, sign * h);
+  T er[4] = {vr[0], vr[2], vr[4], vr[6]}, ei[4] = {vi[0], vi[2], vi[4], vi[6]};
+  T orr[4] = {vr[1], vr[3], vr[5], vr[7]}, oi[4] = {vi[1], vi[3], vi[5], vi[7]};
+  for (int q = 0; q < 4; q++) {
+    vr[q] = er[q] + orr[q]; vi[q] = ei[q] + oi[q];
+    vr[q + 4] = er[q] - orr[q]; vi[q + 4] = ei[q] - oi[q];
+  }
+}
+
+// radix of pass with `bits` bits still to do: 8 while the rest stays a
+// multiple of 3, else 4 (2 for a lone bit)
+__host__ __device__ inline int fft_pass_bits(int bits) {
+  if (bits % 3 == 0) return 3;
+  return bits == 1 ? 1 : 2;
+}
+__host__ inline int fft_min_radix(int logn) {
+  int bits = logn, m = 8;
+  while (bits > 0) { const int lr = fft_pass_bits(bits); m = m < (1 << lr) ? m : (1 << lr); bits -= lr; }
+  return m;
+}
+
+// In-place Stockham autosort FFT of nfft sequences of length n = 2^logn in
+// LDS (re, im), natural order in and out, radix-8/4/2 passes with the DFT in
+// registers.  Twiddles w_p = (twr[p], sign twi[p]), p < n/2, from LDS (the
+// upper half by symmetry).  Each thread holds at most two butterflies of a
+// pass across the barrier that separates the reads from the writes; the
+// caller keeps nfft n / (min radix) <= 2 nth.
 template <typename T>
 __device__ __forceinline__ void lds_fft(T *re, T *im, int nfft, int logn, const T *twr, const T *twi, T sign, int tid,
                                         int nth) {
-  // in-place radix-2 decimation in time on bit-reversed input, nfft
-  // sequences of length n = 2^logn stored back to back; twiddles
-  // w_p = (twr[p], sign * twi[p]), p < n/2, from LDS; all index arithmetic
-  // by shifts
   const int nh = 1 << (logn - 1);
-  for (int st = 1; st <= logn; st++) {
-    const int half = 1 << (st - 1), tsh = logn - st;
-    for (int b = tid; b < nfft * nh; b += nth) {
-      const int q = b >> (logn - 1), bb = b & (nh - 1);
-      const int grp = bb >> (st - 1), pos = bb & (half - 1);
-      const int i0 = (q << logn) + (grp << st) + pos, i1 = i0 + half;
-      const T wr = twr[pos << tsh], wi = sign * twi[pos << tsh];
-      const T xr = re[i1] * wr - im[i1] * wi, xi = re[i1] * wi + im[i1] * wr;
-      const T ar = re[i0], ai = im[i0];
-      re[i1] = ar - xr; im[i1] = ai - xi;
-      re[i0] = ar + xr; im[i0] = ai + xi;
+  int ls = 0;
+  while (ls < logn) {
+    const int lr = fft_pass_bits(logn - ls), R = 1 << lr, lg = logn - lr;
+    const int tasks = nfft << lg;
+    T vr[2][8], vi[2][8];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int t = tid + q * nth;
+      if (t < tasks) {
+        const int f = t >> lg, j = t & ((1 << lg) - 1), k = j & ((1 << ls) - 1);
+        const T *br = re + (f << logn), *bi = im + (f << logn);
+        for (int r = 0; r < R; r++) {
+          T x = br[j + (r << lg)], y = bi[j + (r << lg)];
+          if (r > 0 && k > 0) {
+            const int idx = (r * k) << (logn - ls - lr);
+            T c, sn;
+            if (idx < nh) { c = twr[idx]; sn = twi[idx]; }
+            else { c = -twr[idx - nh]; sn = -twi[idx - nh]; }
+            cmul(x, y, c, sign * sn);
+          }
+          vr[q][r] = x; vi[q][r] = y;
+        }
+      }
     }
     __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int t = tid + q * nth;
+      if (t < tasks) {
+        const int f = t >> lg, j = t & ((1 << lg) - 1), k = j & ((1 << ls) - 1);
+        if (lr == 3) dft8(vr[q], vi[q], sign);
+        else if (lr == 2) dft4(vr[q], vi[q], 1, sign);
+        else dft2(vr[q][0], vi[q][0], vr[q][1], vi[q][1]);
+        const int d = ((j >> ls) << (ls + lr)) + k;
+        T *br = re + (f << logn), *bi = im + (f << logn);
+        for (int r = 0; r < R; r++) { br[d + (r << ls)] = vr[q][r]; bi[d + (r << ls)] = vi[q][r]; }
+      }
+    }
+    __syncthreads();
+    ls += lr;
   }
 }
 
@@ -685,7 +770,7 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
     dv[e] = val;
     if (pow2) {
       const int q = lp >> 1;
-      const int pos = q * N + bitrev(ii, logn);
+      const int pos = q * N + ii;
       if (lp & 1) im[pos] = (TW)val; else re[pos] = (TW)val;
     }
   }
@@ -779,7 +864,7 @@ __global__ __launch_bounds__(256) void k_dctz(TW *__restrict__ S, long nl, int n
       if (dir > 0) {
         TW v = (l < nl) ? S[l + nl * kz] : (TW)0;
         const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
-        const int slot = q * n + bitrev(p, logn);
+        const int slot = q * n + p;
         if (lp & 1) im[slot] = v; else re[slot] = v;
       } else if ((lp & 1) == 0) {
         // W[k] = e^{i pi k/2n} ((Ya[k] + Yb[n-k]) + i (Yb[k] - Ya[n-k])), Y[n]=0
@@ -790,7 +875,7 @@ __global__ __launch_bounds__(256) void k_dctz(TW *__restrict__ S, long nl, int n
         const TW ybn = (k > 0 && lb < nl) ? S[lb + nl * (n - k)] : (TW)0;
         const TW u = ya + ybn, w = yb - yan;
         const TW cr = (TW)er[k], ci = (TW)ei[k];  // e^{+i pi k / 2n}
-        const int slot = q * n + bitrev(k, logn);
+        const int slot = q * n + k;
         re[slot] = u * cr - w * ci;
         im[slot] = u * ci + w * cr;
       }
@@ -965,7 +1050,7 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
       };
       coef(tb + (2 * q) * N, k, ar, ai);
       coef(tb + (2 * q + 1) * N, k, br, bi);
-      const int slot = q * N + bitrev(k, logn);
+      const int slot = q * N + k;
       re[slot] = (TW)(ar - bi);
       im[slot] = (TW)(ai + br);
     }
@@ -1196,6 +1281,74 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
   }
 }
 
+// ---------------------------------------------------------------------------
+// Vortex-core statistics of analysis (run.F90:238-388) as a device
+// reduction: per cell the azimuthal vorticity (dvdz - dwdr) and the swirl
+// q1m of the cell centre; inside the window 0.1 < z/L < 0.9,
+// 0.05 < r/R < 0.8, j != 1: their maxima and the sums of z w^2 r, w^2 r,
+// w^2 r, w^2 (w = vorticity where > 0, then swirl where > 0) times dx1 dx2
+// dx3.  Per-block partials [block][10], summed in a fixed order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_analysis(VGrid g, double rmax, double zmax, const double *__restrict__ q1,
+                                                  const double *__restrict__ q2, const double *__restrict__ q3,
+                                                  double *__restrict__ part) {
+  const long ncell = (long)g.nt * g.nr * g.nz;
+  const double dr = rmax / g.nr, dz = zmax / g.nz, w = g.dx1 * g.dx2 * g.dx3;
+  double v[10] = {-10000.0, -10000.0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (long c = (long)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (long)gridDim.x * 256) {
+    const int ith = (int)(c % g.nt) + 1;
+    const long rest = c / g.nt;
+    const int ir = (int)(rest % g.nr) + 1, iz = (int)(rest / g.nr) + 1;
+    const double rm = (ir - 0.5) * dr, zm = (iz - 0.5) * dz;
+    if (!(zm < 0.90 * zmax && zm > 0.10 * zmax && rm < 0.80 * rmax && rm > rmax * 0.05 && ir != 1)) continue;
+    const int ithp1 = ith == g.nt ? 1 : ith + 1;
+    const double q1m = g.nt > 1 ? 0.5 * (q1[hidx(g, ith, ir, iz)] + q1[hidx(g, ithp1, ir, iz)]) : q1[hidx(g, ith, ir, iz)];
+    const double rp = (ir + 0.5) * dr, rmm = (ir - 1.5) * dr;
+    const double dwdr = 0.5 * ((q3[hidx(g, ith, ir + 1, iz)] + q3[hidx(g, ith, ir + 1, iz + 1)]) -
+                               (q3[hidx(g, ith, ir - 1, iz)] + q3[hidx(g, ith, ir - 1, iz + 1)])) / (rp - rmm);
+    const double zp = (iz + 0.5) * dz, zmm = (iz - 1.5) * dz;
+    const double dvdz = 0.5 * ((q2[hidx(g, ith, ir, iz + 1)] / rm + q2[hidx(g, ith, ir + 1, iz + 1)] / rp) -
+                               (q2[hidx(g, ith, ir + 1, iz - 1)] / rp + q2[hidx(g, ith, ir, iz - 1)] / rm)) / (zp - zmm);
+    const double vort = +(dvdz - dwdr), swrl = q1m;
+    v[0] = fmax(vort, v[0]);
+    v[1] = fmax(swrl, v[1]);
+    if (vort > 0.0) {
+      v[2] += zm * vort * vort * rm * w;
+      v[3] += vort * vort * rm * w;
+      v[4] += vort * vort * rm * w;
+      v[5] += vort * vort * w;
+    }
+    if (swrl > 0.0) {
+      v[6] += zm * swrl * swrl * rm * w;
+      v[7] += swrl * swrl * rm * w;
+      v[8] += swrl * swrl * rm * w;
+      v[9] += swrl * swrl * w;
+    }
+  }
+  __shared__ double red[10][4];
+  for (int q = 0; q < 10; q++) {
+    double x = v[q];
+    for (int o = 32; o > 0; o >>= 1) {
+      const double y = __shfl_xor(x, o);
+      x = (q < 2) ? fmax(x, y) : x + y;
+    }
+    if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    const int q = threadIdx.x;
+    const double a = red[q][0], b = red[q][1], cc = red[q][2], d = red[q][3];
+    part[(long)blockIdx.x * 10 + q] = (q < 2) ? fmax(fmax(a, b), fmax(cc, d)) : ((a + b) + (cc + d));
+  }
+}
+__global__ void k_analysis_sum(const double *__restrict__ part, int nblk, double *__restrict__ out) {
+  const int q = threadIdx.x;
+  if (q >= 10) return;
+  double x = (q < 2) ? -10000.0 : 0.0;
+  for (int b = 0; b < nblk; b++) x = (q < 2) ? fmax(x, part[(long)b * 10 + q]) : x + part[(long)b * 10 + q];
+  out[q] = x;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1313,6 +1466,9 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
   int LB = pow2 ? (h->fft_lines > 0 ? h->fft_lines : 1024 / N) : 2;
   if (LB > 16) LB = 16;
+  if (pow2) {  // the LDS FFT holds at most two butterflies per thread: (LB/2) N / rmin <= 512
+    while (LB > 2 && (LB / 2) * N / fft_min_radix(logn) > 512) LB /= 2;
+  }
   if (LB < 2) LB = 2;
   const long nlines = (long)g.nr * g.nz;
   TW *S = (TW *)h->f.S;
@@ -1329,7 +1485,9 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   // lines per block: a power of two, data + twiddles within 64 KB of LDS
   auto lines = [&](size_t es, int want) {
     int tl = 2;
-    while (tl * 2 <= 16 && (size_t)(tl * 2) * nz * es + (size_t)nz * es <= 65536) tl *= 2;
+    while (tl * 2 <= 16 && (size_t)(tl * 2) * nz * es + (size_t)nz * es <= 65536 &&
+           (tl / 2 * 2) * nz / fft_min_radix(lz) <= 512)
+      tl *= 2;
     if (want > 0) while (tl > 2 && tl > want) tl /= 2;
     return tl;
   };
@@ -1372,6 +1530,12 @@ template <typename TW> void launch_project(vrh_t *h, int irk) {
   hipLaunchKernelGGL((k_project<TW, R>), grd, blk, 0, h->stream, g, h->coef, adt, be, kchunk, h->f.dph, h->f.dq1,
                      h->f.dq2, h->f.dq3, h->f.q1, h->f.q2, h->f.q3, (TW *)h->f.prg1, (TW *)h->f.prg2,
                      (TW *)h->f.prg3, h->f.pres, h->f.flag);
+}
+
+void launch_analysis(vrh_t *h, double *part, int nblk, double *out) {
+  hipLaunchKernelGGL(k_analysis, dim3(nblk), dim3(256), 0, h->stream, h->g, h->rmax, h->zmax, h->f.q1, h->f.q2,
+                     h->f.q3, part);
+  hipLaunchKernelGGL(k_analysis_sum, dim3(1), dim3(64), 0, h->stream, part, nblk, out);
 }
 
 #define INST(TW)                                                   \

@@ -35,7 +35,7 @@ constexpr int kLayerCout[4] = {kWidth, kWidth, kWidth, kCout4};
 constexpr int kDwRows = 16;      // z rows per block of the weight-gradient kernel
 
 // scalar slots
-enum { SC_RZ = 0, SC_DONE, SC_IT, SC_BB, SC_RHO, SC_ALPHA, SC_OMEGA, SC_RZP, SC_SCALE, SC_DEN };
+enum { SC_RZ = 0, SC_DONE, SC_IT, SC_BB, SC_RHO, SC_ALPHA, SC_OMEGA, SC_RZP, SC_SCALE, SC_DEN, SC_GB, SC_GA };
 
 __device__ __forceinline__ bf16_t f2bf(float f) {
   unsigned u = __float_as_uint(f);
@@ -79,7 +79,7 @@ __device__ __forceinline__ double psum(const double *part, int nq, int b, int q)
   return s;
 }
 
-// Block reduction of up to 4 values; thread 0 writes partials.
+// Block reduction of NQ values; thread 0 writes partials.
 template <int NQ>
 __device__ __forceinline__ void block_partials(double (&v)[NQ], double *part, int nq, int b) {
   __shared__ double red[NQ][256 / 64];
@@ -405,26 +405,77 @@ __global__ void k_rsq(Geo g, const double *__restrict__ r, double *part) {
   CELL_LOOP(g, b, c, k, j) { const double f = r[c] / vol(g, k, j); v[0] += f * f; }
   block_partials<1>(v, part, 1, b);
 }
-__global__ void k_features(Geo g, const double *__restrict__ f, int fv, const double *part, bf16_t *__restrict__ X,
-                           double *sc) {
+// Features on the network grid, whose cells are blocks of pool x pool
+// cells: block mean of f / rms(f), m drn / (m drn + rn), dzn / drn, rn / R
+// (drn = pool dr, rn the block's centre radius, dzn the block height).  The
+// network output times scale = rms(f) R^2 is the correction, prolonged
+// piecewise constant to the cells of each block.
+__global__ void k_features(Geo g, int pool, const double *__restrict__ f, int fv, const double *part,
+                           bf16_t *__restrict__ X, double *sc) {
   const int b = blockIdx.y;
+  const int nrn = g.nr / pool, nzn = g.nz / pool;
   double s = sqrt(psum(part, 1, b, 0) / ((double)g.nz * g.nr));
   if (!(s > 0.0)) s = 1.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc[b * kNsc + SC_SCALE] = s * g.dr * g.dr;
-  const double mdr = (double)g.m[b] * g.dr;
-  CELL_LOOP(g, b, c, k, j) {
-    const double fc = fv ? f[c] / vol(g, k, j) : f[c];
+  const double drn = g.dr * pool, R = g.dr * g.nr;
+  // output scale rms(f) R^2: the size of smooth pressure solutions (a scale of
+  // drn^2 would leave the last layer ~(R/drn)^2 too small to start)
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[b * kNsc + SC_SCALE] = s * R * R;
+  const double mdr = (double)g.m[b] * drn;
+  const long ncs = (long)g.nz * g.nr, ncn = (long)nzn * nrn;
+  for (long cn = (long)blockIdx.x * 256 + threadIdx.x; cn < ncn; cn += 256L * kRedBlocks) {
+    const int kn = (int)(cn / nrn), jn = (int)(cn - (long)kn * nrn);
+    double acc = 0.0, dzs = 0.0;
+    for (int dk = 0; dk < pool; dk++) {
+      const int k = kn * pool + dk;
+      dzs += g.dz[k];
+      for (int dj = 0; dj < pool; dj++) {
+        const int j = jn * pool + dj;
+        const long c = (long)b * ncs + (long)k * g.nr + j;
+        acc += fv ? f[c] / vol(g, k, j) : f[c];
+      }
+    }
+    const double fc = acc / (double)(pool * pool), rn = (jn + 0.5) * drn;
     bf16_t v[16];
     v[0] = f2bf((float)(fc / s));
-    v[1] = f2bf((float)(mdr / (mdr + g.rm[j])));
-    v[2] = f2bf((float)(g.dz[k] / g.dr));
-    v[3] = f2bf((float)(g.rm[j] / (g.dr * g.nr)));
+    v[1] = f2bf((float)(mdr / (mdr + rn)));
+    v[2] = f2bf((float)(dzs / drn));
+    v[3] = f2bf((float)(rn / (g.dr * g.nr)));
     for (int q = 4; q < 16; q++) v[q] = 0;
-    uint4 *dst = reinterpret_cast<uint4 *>(X + c * 16);
+    uint4 *dst = reinterpret_cast<uint4 *>(X + ((long)b * ncn + cn) * 16);
     const unsigned *w = reinterpret_cast<const unsigned *>(v);
     dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
     dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
   }
+}
+// Bilinear prolongation from the network grid (block centres) to the cells,
+// in index space, constant beyond the outermost block centres.  1-D weights
+// of cell i: u = (i + 1/2) / pool - 1/2, node floor(u) gets 1 - t and the
+// next t (t = u - floor(u)).
+__device__ __forceinline__ void pweights(int i, int pool, int nn, int &i0, double &t) {
+  const double u = (i + 0.5) / pool - 0.5;
+  if (u <= 0.0) { i0 = 0; t = 0.0; return; }
+  if (u >= nn - 1) { i0 = nn - 1; t = 0.0; return; }
+  i0 = (int)u;
+  t = u - i0;
+}
+__device__ __forceinline__ double pweight(int i, int pool, int nn, int node) {
+  int i0;
+  double t;
+  pweights(i, pool, nn, i0, t);
+  return (i0 == node ? 1.0 - t : 0.0) + (i0 + 1 == node ? t : 0.0);
+}
+__device__ __forceinline__ double prolong(const Geo &g, int pool, const float *__restrict__ out, int b, int k,
+                                          int j) {
+  const int nrn = g.nr / pool, nzn = g.nz / pool;
+  int k0, j0;
+  double tk, tj;
+  pweights(k, pool, nzn, k0, tk);
+  pweights(j, pool, nrn, j0, tj);
+  const float *o = out + (long)b * nzn * nrn;
+  const int k1 = min(k0 + 1, nzn - 1), j1 = min(j0 + 1, nrn - 1);
+  const double a = (1.0 - tj) * (double)o[(long)k0 * nrn + j0] + tj * (double)o[(long)k0 * nrn + j1];
+  const double c = (1.0 - tj) * (double)o[(long)k1 * nrn + j0] + tj * (double)o[(long)k1 * nrn + j1];
+  return (1.0 - tk) * a + tk * c;
 }
 
 // 3x3 dilated convolution as an implicit GEMM on 16x16x16 BF16 MFMA tiles.
@@ -595,37 +646,106 @@ __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float
   w[i] -= lr * (a * c1) / (sqrtf(v * c2) + 1e-8f);
 }
 
-// network estimate of S^-1 r: x = D^-1 r + scale * out (Jacobi plus the learned correction)
-__global__ void k_scale_out(Geo g, const float *__restrict__ out, const double *sc, const double *__restrict__ r,
-                            double *__restrict__ x) {
+// Galerkin combination of the Jacobi estimate j = D^-1 r and the network
+// correction c = scale * out (prolonged) as an approximation of S^-1 r:
+// z = beta j + gamma c with (beta, gamma) minimising the energy error
+// ||z - S^-1 r||_S, i.e. solving
+//   [j.Sj  j.Sc] [beta ]   [j.r]
+//   [j.Sc  c.Sc] [gamma] = [c.r].
+// z is never worse in that norm than the best multiple of j alone (gamma = 0
+// is admissible), whatever the network returns, and z.r = z.Sz > 0.
+// Partials per slot: j.Sj, j.Sc, c.Sc, j.r, c.r.
+__global__ void k_gal_dots(Geo g, int pool, const float *__restrict__ out, const double *sc,
+                           const double *__restrict__ r, double *part) {
   const int b = blockIdx.y;
   const double s = sc[b * kNsc + SC_SCALE];
-  CELL_LOOP(g, b, c, k, j) { x[c] = r[c] / sdiag(g, b, k, j) + s * (double)out[c]; }
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  CELL_LOOP(g, b, c, k, j) {
+    const long base = (long)b * ncs;
+    auto jf = [&](int kk, int jj) { return r[base + (long)kk * g.nr + jj] / sdiag(g, b, kk, jj); };
+    auto cf = [&](int kk, int jj) { return s * prolong(g, pool, out, b, kk, jj); };
+    const double jc = jf(k, j), cc = cf(k, j);
+    const double sj = sapply(g, b, k, j, jf), scv = sapply(g, b, k, j, cf);
+    v[0] += jc * sj;
+    v[1] += jc * scv;
+    v[2] += cc * scv;
+    v[3] += jc * r[c];
+    v[4] += cc * r[c];
+  }
+  block_partials<5>(v, part, 5, b);
+}
+__global__ void k_gal_solve(int S, const double *part, double *sc) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= S) return;
+  const double a11 = psum(part, 5, b, 0), a12 = psum(part, 5, b, 1), a22 = psum(part, 5, b, 2);
+  const double r1 = psum(part, 5, b, 3), r2 = psum(part, 5, b, 4);
+  const double det = a11 * a22 - a12 * a12;
+  double beta = a11 > 0.0 ? r1 / a11 : 0.0, gam = 0.0;
+  // a nearly dependent pair (or a zero / non-finite correction) keeps the Jacobi term alone
+  if (a11 > 0.0 && a22 > 0.0 && det > 1e-10 * a11 * a22 && isfinite(det)) {
+    beta = (r1 * a22 - r2 * a12) / det;
+    gam = (a11 * r2 - a12 * r1) / det;
+  }
+  sc[b * kNsc + SC_GB] = beta;
+  sc[b * kNsc + SC_GA] = gam;
+}
+// z = beta D^-1 r + gamma scale out
+__global__ void k_gal_combine(Geo g, int pool, const float *__restrict__ out, const double *sc,
+                              const double *__restrict__ r, double *__restrict__ z) {
+  const int b = blockIdx.y;
+  const double s = sc[b * kNsc + SC_SCALE], be = sc[b * kNsc + SC_GB], ga = sc[b * kNsc + SC_GA];
+  CELL_LOOP(g, b, c, k, j) { z[c] = be * (r[c] / sdiag(g, b, k, j)) + ga * (s * prolong(g, pool, out, b, k, j)); }
 }
 
-// energy-norm loss of the preconditioned estimate D^-1 b + scale out:
-// e = D^-1 b + scale out - p; per slot L = e.Se / den, gradient
-// 2 scale Se / (den S) into channel 0 of the 16-channel bf16 tensor G;
-// partial e.Se
-__global__ void k_loss(Geo g, const float *__restrict__ out, const double *__restrict__ p,
-                       const double *__restrict__ rhs, const double *sc, bf16_t *__restrict__ G, double *part) {
+// energy-norm loss of the Galerkin estimate of p = S^-1 b:
+// e = beta D^-1 b + gamma scale out - p; per slot L = e.Se / den.  The
+// coefficients are optimal, so dL/d(out) = 2 gamma scale Se / den (the
+// terms through beta and gamma vanish); it goes, divided by the batch size,
+// into gf; partial e.Se
+__global__ void k_loss(Geo g, int pool, const float *__restrict__ out, const double *__restrict__ p,
+                       const double *__restrict__ rhs, const double *sc, double *__restrict__ gf, double *part) {
   const int b = blockIdx.y;
   const double s = sc[b * kNsc + SC_SCALE], den = sc[b * kNsc + SC_DEN];
+  const double be = sc[b * kNsc + SC_GB], ga = sc[b * kNsc + SC_GA];
   double v[1] = {0.0};
   CELL_LOOP(g, b, c, k, j) {
     const long base = (long)b * ncs;
     auto e = [&](int kk, int jj) {
       const long cc = base + (long)kk * g.nr + jj;
-      return rhs[cc] / sdiag(g, b, kk, jj) + s * (double)out[cc] - p[cc];
+      return be * (rhs[cc] / sdiag(g, b, kk, jj)) + ga * (s * prolong(g, pool, out, b, kk, jj)) - p[cc];
     };
     const double se = sapply(g, b, k, j, e);
     v[0] += e(k, j) * se;
-    const float gr = (float)(2.0 * s * se / (den * g.S));
-    uint4 *dst = reinterpret_cast<uint4 *>(G + c * 16);
-    dst[0] = make_uint4((unsigned)f2bf(gr), 0u, 0u, 0u);
-    dst[1] = make_uint4(0u, 0u, 0u, 0u);
+    gf[c] = 2.0 * ga * s * se / (den * g.S);
   }
   block_partials<1>(v, part, 1, b);
+}
+// gradient on the network grid: the adjoint of the bilinear prolongation
+// applied to the cell gradients (each node gathers the cells within two
+// blocks of it with their weights), into channel 0 of G
+__global__ void k_pool_grad(Geo g, int pool, const double *__restrict__ gf, bf16_t *__restrict__ G) {
+  const int b = blockIdx.y;
+  const int nrn = g.nr / pool, nzn = g.nz / pool;
+  const long ncs = (long)g.nz * g.nr, ncn = (long)nzn * nrn;
+  for (long cn = (long)blockIdx.x * 256 + threadIdx.x; cn < ncn; cn += 256L * kRedBlocks) {
+    const int kn = (int)(cn / nrn), jn = (int)(cn - (long)kn * nrn);
+    double acc = 0.0;
+    const int ka = max(0, (kn - 1) * pool), kb = min(g.nz - 1, (kn + 2) * pool - 1);
+    const int ja = max(0, (jn - 1) * pool), jb = min(g.nr - 1, (jn + 2) * pool - 1);
+    for (int k = ka; k <= kb; k++) {
+      const double wk = pweight(k, pool, nzn, kn);
+      if (wk == 0.0) continue;
+      double row = 0.0;
+      for (int j = ja; j <= jb; j++) {
+        const double wj = pweight(j, pool, nrn, jn);
+        if (wj != 0.0) row += wj * gf[(long)b * ncs + (long)k * g.nr + j];
+      }
+      acc += wk * row;
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(G + ((long)b * ncn + cn) * 16);
+    dst[0] = make_uint4((unsigned)f2bf((float)acc), 0u, 0u, 0u);
+    dst[1] = make_uint4(0u, 0u, 0u, 0u);
+  }
 }
 // den = p . (V f) per slot (data of one batch)
 __global__ void k_den(Geo g, const double *__restrict__ p, const double *__restrict__ rhs, double *part) {
@@ -777,7 +897,7 @@ void tridiag_eig(const std::vector<double> &a, const std::vector<double> &e, std
 // handle
 // ---------------------------------------------------------------------------
 struct vrh_hz_s {
-  int nr = 0, nz = 0, S = 0;
+  int nr = 0, nz = 0, S = 0, pool = 1;
   double rmax = 0, zmax = 0, stretch = 0, dr = 0;
   hipStream_t stream = nullptr;
   rocblas_handle blas = nullptr;
@@ -807,6 +927,7 @@ struct vrh_hz_s {
     return g;
   }
   size_t ncell() const { return (size_t)S * nz * nr; }
+  size_t nnet() const { return (size_t)S * (nz / pool) * (nr / pool); }
 };
 
 namespace {
@@ -874,9 +995,9 @@ int direct_dev(vrh_hz_t *h, const double *rhs, double *p) {
   return 0;
 }
 
-// network forward on features already in d_x0; output fp32 in d_out
+// network forward on features already in d_x0 (network grid); output fp32 in d_out
 void net_forward(vrh_hz_t *h) {
-  const int nr = h->nr, nz = h->nz, S = h->S;
+  const int nr = h->nr / h->pool, nz = h->nz / h->pool, S = h->S;
   dim3 grd((unsigned)((nr + 63) / 64), nz, S), blk(256);
   const bf16_t *wf = h->d_wf;
   hipLaunchKernelGGL((k_conv<16, 32, 0>), grd, blk, 0, h->stream, h->d_x0, wf + h->wfo[0], h->d_w + h->boff[0],
@@ -894,12 +1015,13 @@ void net_features(vrh_hz_t *h, const double *f, int fv) {
   Geo g = h->geo();
   if (fv) hipLaunchKernelGGL(k_rsq, red_grid(h), dim3(256), 0, h->stream, g, f, h->d_part);
   else hipLaunchKernelGGL(k_fsq, red_grid(h), dim3(256), 0, h->stream, g, f, h->d_part);
-  hipLaunchKernelGGL(k_features, red_grid(h), dim3(256), 0, h->stream, g, f, fv, h->d_part, h->d_x0, h->d_sc);
+  hipLaunchKernelGGL(k_features, red_grid(h), dim3(256), 0, h->stream, g, h->pool, f, fv, h->d_part, h->d_x0,
+                     h->d_sc);
 }
 
 // backward from the 16-channel gradient in d_g4: parameter gradients into d_g
 void net_backward(vrh_hz_t *h) {
-  const int nr = h->nr, nz = h->nz, S = h->S;
+  const int nr = h->nr / h->pool, nz = h->nz / h->pool, S = h->S;
   dim3 grd((unsigned)((nr + 63) / 64), nz, S), blk(256);
   dim3 gdw((unsigned)((nz + kDwRows - 1) / kDwRows), S);
   const long nblk = (long)gdw.x * S;
@@ -933,6 +1055,19 @@ void pack_weights(vrh_hz_t *h) {
     hipLaunchKernelGGL(k_pack_layer, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_w + h->woff[l],
                        kLayerCin[l], kLayerCout[l], h->d_wf + h->wfo[l], h->d_wb + h->wfo[l]);
   }
+}
+
+// the network's Galerkin estimate of S^-1 r into z (features, forward, the
+// 2x2 energy-optimal coefficients, the combination); z may be null to stop
+// after the coefficients
+double *gal_part(vrh_hz_t *h) { return h->d_part + (size_t)h->S * kRedBlocks * 16; }
+void net_galerkin(vrh_hz_t *h, const double *r, double *z) {
+  Geo g = h->geo();
+  net_features(h, r, 1);
+  net_forward(h);
+  hipLaunchKernelGGL(k_gal_dots, red_grid(h), dim3(256), 0, h->stream, g, h->pool, h->d_out, h->d_sc, r, gal_part(h));
+  hipLaunchKernelGGL(k_gal_solve, dim3((h->S + 63) / 64), dim3(64), 0, h->stream, h->S, gal_part(h), h->d_sc);
+  if (z) hipLaunchKernelGGL(k_gal_combine, red_grid(h), dim3(256), 0, h->stream, g, h->pool, h->d_out, h->d_sc, r, z);
 }
 
 int sync_scalars(vrh_hz_t *h, std::vector<double> &sc) {
@@ -1013,14 +1148,16 @@ int bicgstab_dev(vrh_hz_t *h, const double *rhs, double *x, double tol, int maxi
   return 0;
 }
 
-// flexible CG (Polak-Ribiere beta) preconditioned by the network
-__global__ void k_fcg_z(Geo g, const float *__restrict__ out, const double *sc, const double *__restrict__ r,
+// flexible CG (Polak-Ribiere beta) preconditioned by the network: z is the
+// Galerkin combination of D^-1 r and the network correction for r; partials
+// r.z, rold.z
+__global__ void k_fcg_z(Geo g, int pool, const float *__restrict__ out, const double *sc, const double *__restrict__ r,
                         const double *__restrict__ rold, double *__restrict__ z, double *part) {
   const int b = blockIdx.y;
-  const double s = sc[b * kNsc + SC_SCALE];
+  const double s = sc[b * kNsc + SC_SCALE], be = sc[b * kNsc + SC_GB], ga = sc[b * kNsc + SC_GA];
   double v[2] = {0.0, 0.0};
   CELL_LOOP(g, b, c, k, j) {
-    const double zz = r[c] / sdiag(g, b, k, j) + s * (double)out[c];
+    const double zz = be * (r[c] / sdiag(g, b, k, j)) + ga * (s * prolong(g, pool, out, b, k, j));
     z[c] = zz;
     v[0] += r[c] * zz;
     v[1] += rold[c] * zz;
@@ -1081,26 +1218,6 @@ __global__ void k_fcg_scalars(int S, const double *part_z, const double *part_rr
   if (rr <= tol2 * s[SC_BB]) s[SC_DONE] = 1.0;
   else if (!(rz > 0.0) || !isfinite(rr)) s[SC_DONE] = 2.0;
 }
-// energy-optimal scaling of a warm start x: alpha = (x.b) / (x.Sx) per slot,
-// so that the start's energy error never exceeds that of x = 0
-__global__ void k_wsdots(Geo g, const double *__restrict__ x, const double *__restrict__ rhs, double *part) {
-  const int b = blockIdx.y;
-  double v[2] = {0.0, 0.0};
-  CELL_LOOP(g, b, c, k, j) {
-    const double *xb = x + (long)b * ncs;
-    const double sx = sapply(g, b, k, j, [&](int kk, int jj) { return xb[(long)kk * g.nr + jj]; });
-    v[0] += x[c] * rhs[c];
-    v[1] += x[c] * sx;
-  }
-  block_partials<2>(v, part, 2, b);
-}
-__global__ void k_wsscale(Geo g, const double *part, double *__restrict__ x) {
-  const int b = blockIdx.y;
-  const double xsx = psum(part, 2, b, 1);
-  const double a = xsx > 0.0 ? psum(part, 2, b, 0) / xsx : 0.0;
-  CELL_LOOP(g, b, c, k, j) { x[c] *= a; }
-}
-
 // restart a broken slot from zero
 __global__ void k_reset_broken(Geo g, const double *sc, double *__restrict__ x) {
   const int b = blockIdx.y;
@@ -1138,9 +1255,9 @@ int fcg_dev(vrh_hz_t *h, const double *rhs, double *x, double tol, int maxit, st
   hipLaunchKernelGGL(k_fcg_start, dim3((h->S + 63) / 64), dim3(64), 0, h->stream, h->S, pa, h->d_sc, tol2);
   std::vector<double> sc;
   for (int it = 0; it < maxit; it++) {
-    net_features(h, r, 1);  // uses d_part[0..] as scratch before part_z
-    net_forward(h);
-    hipLaunchKernelGGL(k_fcg_z, red_grid(h), dim3(256), 0, h->stream, g, h->d_out, h->d_sc, r, rold, z, part_z);
+    net_galerkin(h, r, nullptr);  // d_part[0..] and the Galerkin partials as scratch
+    hipLaunchKernelGGL(k_fcg_z, red_grid(h), dim3(256), 0, h->stream, g, h->pool, h->d_out, h->d_sc, r, rold, z,
+                       part_z);
     hipLaunchKernelGGL(k_fcg_p, red_grid(h), dim3(256), 0, h->stream, g, z, p, part_z, h->d_sc, it == 0 ? 1 : 0);
     hipLaunchKernelGGL(k_fcg_q, red_grid(h), dim3(256), 0, h->stream, g, p, q, part_pq, part_z, h->d_sc);
     hipLaunchKernelGGL(k_fcg_update, red_grid(h), dim3(256), 0, h->stream, g, x, r, rold, p, q, part_pq, part_z,
@@ -1195,13 +1312,14 @@ int vrh_hz_last_error(vrh_hz_t *h, char *buf, size_t len) {
 int vrh_hz_create(const vrh_hz_grid *gr, vrh_hz_t **out) {
   if (!gr || !out) return -1;
   *out = nullptr;
-  if (gr->nr < 16 || gr->nr % 16 != 0 || gr->nz < 4 || gr->nslots < 1 || gr->rmax <= 0 || gr->zmax <= 0 ||
-      gr->stretch < 0)
+  const int pool = gr->pool > 1 ? gr->pool : 1;
+  if (gr->nr < 16 || gr->nr % (16 * pool) != 0 || gr->nz < 4 || gr->nz % pool != 0 || gr->nslots < 1 ||
+      gr->rmax <= 0 || gr->zmax <= 0 || gr->stretch < 0)
     return -1;
   vrh_hz_t *h = new (std::nothrow) vrh_hz_t();
   if (!h) return -1;
   *out = h;
-  h->nr = gr->nr; h->nz = gr->nz; h->S = gr->nslots;
+  h->nr = gr->nr; h->nz = gr->nz; h->S = gr->nslots; h->pool = pool;
   h->rmax = gr->rmax; h->zmax = gr->zmax; h->stretch = gr->stretch;
   const int nr = h->nr, nz = h->nz;
   h->dr = h->rmax / nr;
@@ -1246,7 +1364,7 @@ int vrh_hz_create(const vrh_hz_grid *gr, vrh_hz_t **out) {
   h->vec.assign(12, nullptr);
   for (auto &v : h->vec)
     if ((rc = halloc(h, &v, h->ncell()))) return rc;
-  if ((rc = halloc(h, &h->d_part, (size_t)h->S * kRedBlocks * 16)) || (rc = halloc(h, &h->d_sc, (size_t)h->S * kNsc)))
+  if ((rc = halloc(h, &h->d_part, (size_t)h->S * kRedBlocks * 24)) || (rc = halloc(h, &h->d_sc, (size_t)h->S * kNsc)))
     return rc;
   // network
   int off = 0, wo = 0;
@@ -1339,13 +1457,9 @@ int vrh_hz_solve(vrh_hz_t *h, int method, double tol, int maxit, const double *f
   launch_rhs(h, h->vec[0], rhs);
   std::vector<int> its(h->S, 0), its2;
   if (method == VRH_HZ_LEARNED || method == VRH_HZ_LEARNED_FCG) {
-    // warm start: the network-corrected Jacobi estimate M(V f), scaled to
-    // its energy-optimal multiple
-    net_features(h, rhs, 1);
-    net_forward(h);
-    hipLaunchKernelGGL(k_scale_out, red_grid(h), dim3(256), 0, h->stream, h->geo(), h->d_out, h->d_sc, rhs, x);
-    hipLaunchKernelGGL(k_wsdots, red_grid(h), dim3(256), 0, h->stream, h->geo(), x, rhs, h->d_part);
-    hipLaunchKernelGGL(k_wsscale, red_grid(h), dim3(256), 0, h->stream, h->geo(), h->d_part, x);
+    // warm start: the Galerkin combination of the Jacobi estimate and the
+    // network correction for V f
+    net_galerkin(h, rhs, x);
   }
   if (method == VRH_HZ_LEARNED) {
     if ((rc = pcg_dev(h, rhs, x, tol, maxit, its))) return rc;
@@ -1430,7 +1544,7 @@ int vrh_hz_net_forward(vrh_hz_t *h, const double *f, float *out, double *scale) 
   net_features(h, h->vec[0], 0);
   net_forward(h);
   HZCHK(h, hipGetLastError());
-  HZCHK(h, hipMemcpyAsync(out, h->d_out, h->ncell() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HZCHK(h, hipMemcpyAsync(out, h->d_out, h->nnet() * sizeof(float), hipMemcpyDeviceToHost, h->stream));
   std::vector<double> sc;
   if ((rc = sync_scalars(h, sc))) return rc;
   if (scale)
@@ -1444,8 +1558,8 @@ int vrh_hz_net_grad(vrh_hz_t *h, const double *f, const float *gout, float *grad
   if (rc) return rc;
   net_features(h, h->vec[0], 0);
   net_forward(h);
-  HZCHK(h, hipMemcpyAsync(h->d_out, gout, h->ncell() * sizeof(float), hipMemcpyHostToDevice, h->stream));
-  const long n = (long)h->ncell();
+  HZCHK(h, hipMemcpyAsync(h->d_out, gout, h->nnet() * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  const long n = (long)h->nnet();
   hipLaunchKernelGGL(k_gout, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, n, h->d_out, h->d_g4);
   net_backward(h);
   HZCHK(h, hipGetLastError());
@@ -1478,10 +1592,10 @@ int vrh_hz_train(vrh_hz_t *h, int nsamples, double lr, unsigned long long seed, 
     if (rc) return rc;
     hipLaunchKernelGGL(k_den, red_grid(h), dim3(256), 0, h->stream, g, pt, rhs, h->d_part);
     hipLaunchKernelGGL(k_store_den, dim3((h->S + 63) / 64), dim3(64), 0, h->stream, h->S, h->d_part, h->d_sc);
-    net_features(h, rhs, 1);
-    net_forward(h);
-    hipLaunchKernelGGL(k_loss, red_grid(h), dim3(256), 0, h->stream, g, h->d_out, pt, rhs, h->d_sc, h->d_g4,
-                       h->d_part);
+    net_galerkin(h, rhs, nullptr);
+    hipLaunchKernelGGL(k_loss, red_grid(h), dim3(256), 0, h->stream, g, h->pool, h->d_out, pt, rhs, h->d_sc,
+                       h->vec[9], h->d_part);
+    hipLaunchKernelGGL(k_pool_grad, red_grid(h), dim3(256), 0, h->stream, g, h->pool, h->vec[9], h->d_g4);
     hipLaunchKernelGGL(k_batch_loss, dim3(1), dim3(64), 0, h->stream, h->S, h->d_part, h->d_sc, h->d_loss + it);
     net_backward(h);
     h->adam_t++;

@@ -205,42 +205,65 @@ __global__ __launch_bounds__(128) void k_part_lsq(PartArgs a) {
       for (int v = 0; v < a.nvar; v++) U(v) = 0.0;
       return;
     }
-    // rates (part_tools.F90:836-870); jac/hes rows are read from u
-    double rate[78];
+    // rates (part_tools.F90:836-870) and the update of states 1..78
+    // (part_tools.F90:926-930 Euler; dertest RK3 form), state by state: every
+    // rate reads only old states, so the Hessian rows go first (they read the
+    // old Jacobian, kept in J), then the Jacobian rows, then position and
+    // velocity; no array of 78 rates is kept
     const double inv = (double)1.0f / a.st;
-    for (int i = 0; i < 3; i++) {
-      rate[i] = U(3 + i);
-      rate[3 + i] = inv * ((double)vel[i] - U(3 + i));
-    }
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) {
-        rate[6 + i + 3 * j] = U(15 + i + 3 * j);
-        const double sumj = U(6 + 0 + 3 * j) * (double)dvel[i + 0] + U(6 + 1 + 3 * j) * (double)dvel[i + 3] +
-                            U(6 + 2 + 3 * j) * (double)dvel[i + 6];
-        rate[15 + i + 3 * j] = inv * (sumj - U(15 + i + 3 * j));
+    const bool euler = a.scheme == VRH_PART_LSQ_EULER;
+    auto upd = [&](int v, double rate) {
+      if (euler) {
+        U(v) = U(v) + rate * a.partdt;
+      } else {
+        U(v) = U(v) + a.gam * rate * a.partdt + a.rom * d2[(long)v * cap] * a.partdt;
+        d2[(long)v * cap] = rate;
       }
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++)
-        for (int k = 0; k < 3; k++) {
+    };
+    double J[9];
+    for (int q = 0; q < 9; q++) J[q] = U(6 + q);
+    for (int k = 0; k < 3; k++)
+      for (int j = 0; j < 3; j++) {
+        double H[3], ra[3], rb[3];
+        for (int l = 0; l < 3; l++) H[l] = U(24 + l + 3 * j + 9 * k);
+        for (int i = 0; i < 3; i++) {
           const int ix = i + 3 * j + 9 * k;
-          rate[24 + ix] = U(51 + ix);
+          ra[i] = U(51 + ix);
           double sumh = 0.0;
-          for (int l = 0; l < 3; l++) sumh += U(24 + l + 3 * j + 9 * k) * (double)dvel[i + 3 * l];
+          for (int l = 0; l < 3; l++) sumh += H[l] * (double)dvel[i + 3 * l];
           for (int m = 0; m < 3; m++)
-            for (int l = 0; l < 3; l++)
-              sumh += U(6 + l + 3 * j) * U(6 + m + 3 * k) * (double)ddvel[i + 3 * l + 9 * m];
-          rate[51 + ix] = inv * (sumh - U(51 + ix));
+            for (int l = 0; l < 3; l++) sumh += J[l + 3 * j] * J[m + 3 * k] * (double)ddvel[i + 3 * l + 9 * m];
+          rb[i] = inv * (sumh - ra[i]);
         }
-    if (a.scheme == VRH_PART_LSQ_EULER) {          // part_tools.F90:926-945
-      for (int v = 0; v < 78; v++) U(v) = U(v) + rate[v] * a.partdt;
+        for (int i = 0; i < 3; i++) {
+          const int ix = i + 3 * j + 9 * k;
+          upd(24 + ix, ra[i]);
+          upd(51 + ix, rb[i]);
+        }
+      }
+    for (int j = 0; j < 3; j++) {
+      double ra[3], rb[3];
+      for (int i = 0; i < 3; i++) {
+        ra[i] = U(15 + i + 3 * j);
+        const double sumj =
+            J[0 + 3 * j] * (double)dvel[i + 0] + J[1 + 3 * j] * (double)dvel[i + 3] + J[2 + 3 * j] * (double)dvel[i + 6];
+        rb[i] = inv * (sumj - ra[i]);
+      }
+      for (int i = 0; i < 3; i++) {
+        upd(6 + i + 3 * j, ra[i]);
+        upd(15 + i + 3 * j, rb[i]);
+      }
+    }
+    {
+      double v0[3];
+      for (int i = 0; i < 3; i++) v0[i] = U(3 + i);
+      for (int i = 0; i < 3; i++) upd(i, v0[i]);
+      for (int i = 0; i < 3; i++) upd(3 + i, inv * ((double)vel[i] - v0[i]));
+    }
+    if (euler) {
       for (int i = 0; i < 3; i++) U(78 + i) = vel[i];
       for (int i = 0; i < 9; i++) U(81 + i) = dvel[i];
       for (int i = 0; i < 27; i++) U(90 + i) = ddvel[i];
-    } else {                                          // dertest/part_tools.F90:902-906
-      for (int v = 0; v < 78; v++) {
-        U(v) = U(v) + a.gam * rate[v] * a.partdt + a.rom * d2[(long)v * cap] * a.partdt;
-        d2[(long)v * cap] = rate[v];
-      }
     }
     // removal of particles leaving the domain (part_tools.F90:962-1025)
     const double x = U(0), y = U(1), z = U(2), r = sqrt(x * x + y * y);

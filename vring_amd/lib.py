@@ -34,7 +34,7 @@ EXPORTS = ["vrh_create", "vrh_destroy", "vrh_last_error", "vrh_upload_fields", "
            "vrh_step", "vrh_phase", "vrh_poisson_solve", "vrh_synchronize", "vrh_device_ptr",
            "vrh_ring_init", "vrh_cfl_dt", "vrh_particles_create", "vrh_particles_upload",
            "vrh_particles_download", "vrh_particles_step", "vrh_ktimer_enable", "vrh_ktimer_read", "vrh_autotune",
-           "vrh_set_graph", "vrh_set_tuning", "vrh_particles_compact", "vrh_particles_count",
+           "vrh_set_graph", "vrh_set_tuning", "vrh_analysis", "vrh_particles_compact", "vrh_particles_count",
            "vrh_particles_inject", "vrh_ensemble_create", "vrh_ensemble_step", "vrh_ensemble_last_error",
            "vrh_ensemble_destroy"]
 
@@ -78,6 +78,7 @@ def load_library(path=LIBPATH):
     L.vrh_ktimer_read.argtypes = [P, I, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(D),
                                   ctypes.POINTER(ctypes.c_long)]
     L.vrh_set_graph.argtypes = [P, I]
+    L.vrh_analysis.argtypes = [P, dp]
     L.vrh_particles_compact.argtypes = [P, I]
     L.vrh_particles_count.argtypes = [P, I, ctypes.POINTER(ctypes.c_long)]
     L.vrh_particles_inject.argtypes = [P, I, D, I, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_long)]
@@ -218,6 +219,12 @@ class Solver:
     def synchronize(self):
         self._check(self.L.vrh_synchronize(self.h), "synchronize")
 
+    def analysis(self):
+        """Device-reduced vortex-core statistics: (vortmax, swrlmax, stat[0:8])."""
+        out = np.zeros(10)
+        self._check(self.L.vrh_analysis(self.h, out), "analysis")
+        return out
+
     def set_graph(self, on=True):
         """vrh_step replays a captured HIP graph of one RK3 step (default) or
         launches the kernels one by one."""
@@ -356,7 +363,8 @@ class Ensemble:
 # ---------------------------------------------------------------------------
 class VrhHzGrid(ctypes.Structure):
     _fields_ = [("nr", ctypes.c_int), ("nz", ctypes.c_int), ("nslots", ctypes.c_int),
-                ("rmax", ctypes.c_double), ("zmax", ctypes.c_double), ("stretch", ctypes.c_double)]
+                ("rmax", ctypes.c_double), ("zmax", ctypes.c_double), ("stretch", ctypes.c_double),
+                ("pool", ctypes.c_int)]
 
 
 HZ_EXPORTS = ["vrh_hz_create", "vrh_hz_destroy", "vrh_hz_last_error", "vrh_hz_set_modes", "vrh_hz_mesh",
@@ -398,13 +406,14 @@ class Helmholtz:
     """Per-mode (r, z) pressure systems S_m p = V f on a tanh-stretched z grid
     (nslots independent slots), with direct, Krylov and learned solvers."""
 
-    def __init__(self, nr, nz, nslots, rmax=4.0, zmax=12.0, stretch=2.0):
+    def __init__(self, nr, nz, nslots, rmax=4.0, zmax=12.0, stretch=2.0, pool=1):
         self.L = _hz_bind(load_library())
-        self.nr, self.nz, self.S = nr, nz, nslots
+        self.nr, self.nz, self.S, self.pool = nr, nz, nslots, pool
         self.h = ctypes.c_void_p()
-        g = VrhHzGrid(nr, nz, nslots, rmax, zmax, stretch)
+        g = VrhHzGrid(nr, nz, nslots, rmax, zmax, stretch, pool)
         self._check(self.L.vrh_hz_create(ctypes.byref(g), ctypes.byref(self.h)), "vrh_hz_create")
         self.shape = (nslots, nz, nr)
+        self.net_shape = (nslots, nz // pool, nr // pool)
 
     def _check(self, rc, what):
         if rc != 0:
@@ -470,7 +479,7 @@ class Helmholtz:
         self._check(self.L.vrh_hz_net_set(self.h, np.ascontiguousarray(w, np.float32)), "net_set")
 
     def net_forward(self, f):
-        out = np.zeros(self.shape, np.float32)
+        out = np.zeros(self.net_shape, np.float32)
         sc = np.zeros(self.S)
         self._check(self.L.vrh_hz_net_forward(self.h, self._arr(f), out, sc), "net_forward")
         return out, sc
