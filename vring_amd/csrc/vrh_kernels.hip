@@ -658,55 +658,74 @@ __host__ inline int fft_min_radix(int logn) {
   return m;
 }
 
+// butterflies each thread holds across the barrier of an FFT pass (one: the
+// register budget keeps five or more waves per SIMD in both precisions)
+template <typename T> constexpr int fft_qt() { return 1; }
+
+// One radix-2^LR pass of the Stockham autosort FFT (ls bits already done):
+// every butterfly reads its R inputs at stride n/R, applies the twiddles,
+// and after the barrier writes the in-register DFT at stride 2^ls.
+template <typename T, int LR, int QT>
+__device__ __forceinline__ void fft_pass(T *re, T *im, int nfft, int logn, int ls, const T *twr, const T *twi, T sign,
+                                         int tid, int nth) {
+  constexpr int R = 1 << LR;
+  const int nh = 1 << (logn - 1), lg = logn - LR;
+  const int tasks = nfft << lg;
+  T vr[QT][R], vi[QT][R];
+#pragma unroll
+  for (int q = 0; q < QT; q++) {
+    const int t = tid + q * nth;
+    if (t < tasks) {
+      const int f = t >> lg, j = t & ((1 << lg) - 1), k = j & ((1 << ls) - 1);
+      const T *br = re + (f << logn), *bi = im + (f << logn);
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        T x = br[j + (r << lg)], y = bi[j + (r << lg)];
+        if (r > 0 && k > 0) {
+          const int idx = (r * k) << (logn - ls - LR);
+          T c, sn;
+          if (idx < nh) { c = twr[idx]; sn = twi[idx]; }
+          else { c = -twr[idx - nh]; sn = -twi[idx - nh]; }
+          cmul(x, y, c, sign * sn);
+        }
+        vr[q][r] = x; vi[q][r] = y;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < QT; q++) {
+    const int t = tid + q * nth;
+    if (t < tasks) {
+      const int f = t >> lg, j = t & ((1 << lg) - 1), k = j & ((1 << ls) - 1);
+      if (LR == 3) dft8(vr[q], vi[q], sign);
+      else if (LR == 2) dft4(vr[q], vi[q], 1, sign);
+      else dft2(vr[q][0], vi[q][0], vr[q][1], vi[q][1]);
+      const int d = ((j >> ls) << (ls + LR)) + k;
+      T *br = re + (f << logn), *bi = im + (f << logn);
+#pragma unroll
+      for (int r = 0; r < R; r++) { br[d + (r << ls)] = vr[q][r]; bi[d + (r << ls)] = vi[q][r]; }
+    }
+  }
+  __syncthreads();
+}
+
 // In-place Stockham autosort FFT of nfft sequences of length n = 2^logn in
 // LDS (re, im), natural order in and out, radix-8/4/2 passes with the DFT in
 // registers.  Twiddles w_p = (twr[p], sign twi[p]), p < n/2, from LDS (the
-// upper half by symmetry).  Each thread holds at most two butterflies of a
-// pass across the barrier that separates the reads from the writes; the
-// caller keeps nfft n / (min radix) <= 2 nth.
+// upper half by symmetry).  Each thread holds at most fft_qt<T>() butterflies
+// of a pass across the barrier that separates the reads from the writes; the
+// caller keeps nfft n / (min radix) <= fft_qt<T>() nth.
 template <typename T>
 __device__ __forceinline__ void lds_fft(T *re, T *im, int nfft, int logn, const T *twr, const T *twi, T sign, int tid,
                                         int nth) {
-  const int nh = 1 << (logn - 1);
+  constexpr int QT = fft_qt<T>();
   int ls = 0;
   while (ls < logn) {
-    const int lr = fft_pass_bits(logn - ls), R = 1 << lr, lg = logn - lr;
-    const int tasks = nfft << lg;
-    T vr[2][8], vi[2][8];
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-      const int t = tid + q * nth;
-      if (t < tasks) {
-        const int f = t >> lg, j = t & ((1 << lg) - 1), k = j & ((1 << ls) - 1);
-        const T *br = re + (f << logn), *bi = im + (f << logn);
-        for (int r = 0; r < R; r++) {
-          T x = br[j + (r << lg)], y = bi[j + (r << lg)];
-          if (r > 0 && k > 0) {
-            const int idx = (r * k) << (logn - ls - lr);
-            T c, sn;
-            if (idx < nh) { c = twr[idx]; sn = twi[idx]; }
-            else { c = -twr[idx - nh]; sn = -twi[idx - nh]; }
-            cmul(x, y, c, sign * sn);
-          }
-          vr[q][r] = x; vi[q][r] = y;
-        }
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 2; q++) {
-      const int t = tid + q * nth;
-      if (t < tasks) {
-        const int f = t >> lg, j = t & ((1 << lg) - 1), k = j & ((1 << ls) - 1);
-        if (lr == 3) dft8(vr[q], vi[q], sign);
-        else if (lr == 2) dft4(vr[q], vi[q], 1, sign);
-        else dft2(vr[q][0], vi[q][0], vr[q][1], vi[q][1]);
-        const int d = ((j >> ls) << (ls + lr)) + k;
-        T *br = re + (f << logn), *bi = im + (f << logn);
-        for (int r = 0; r < R; r++) { br[d + (r << ls)] = vr[q][r]; bi[d + (r << ls)] = vi[q][r]; }
-      }
-    }
-    __syncthreads();
+    const int lr = fft_pass_bits(logn - ls);
+    if (lr == 3) fft_pass<T, 3, QT>(re, im, nfft, logn, ls, twr, twi, sign, tid, nth);
+    else if (lr == 2) fft_pass<T, 2, QT>(re, im, nfft, logn, ls, twr, twi, sign, tid, nth);
+    else fft_pass<T, 1, QT>(re, im, nfft, logn, ls, twr, twi, sign, tid, nth);
     ls += lr;
   }
 }
@@ -1466,8 +1485,8 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
   int LB = pow2 ? (h->fft_lines > 0 ? h->fft_lines : 1024 / N) : 2;
   if (LB > 16) LB = 16;
-  if (pow2) {  // the LDS FFT holds at most two butterflies per thread: (LB/2) N / rmin <= 512
-    while (LB > 2 && (LB / 2) * N / fft_min_radix(logn) > 512) LB /= 2;
+  if (pow2) {  // butterflies per thread of the LDS FFT: (LB/2) N / rmin <= qt * 256
+    while (LB > 2 && (LB / 2) * N / fft_min_radix(logn) > 256 * fft_qt<TW>()) LB /= 2;
   }
   if (LB < 2) LB = 2;
   const long nlines = (long)g.nr * g.nz;
@@ -1486,7 +1505,7 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   auto lines = [&](size_t es, int want) {
     int tl = 2;
     while (tl * 2 <= 16 && (size_t)(tl * 2) * nz * es + (size_t)nz * es <= 65536 &&
-           (tl / 2 * 2) * nz / fft_min_radix(lz) <= 512)
+           (tl * 2 / 2) * nz / fft_min_radix(lz) <= 256)
       tl *= 2;
     if (want > 0) while (tl > 2 && tl > want) tl /= 2;
     return tl;
