@@ -470,12 +470,12 @@ int build_line_coefs(vrh_t *h) {
   for (int irk = 1; irk <= 3; irk++) {
     host_line_coefs(h, irk, lcv[irk], h->Msw);
     for (int s = 0; s < 6; s++)
-      for (int q = 0; q < 7; q++) tot += lcv[irk][s][q].size();
+      for (int q = 0; q < 7; q++) tot += lcv[irk][s][q].size() + (q < 5 ? kLinePad : 0);
   }
   if (h->d_line) { hipStreamSynchronize(h->stream); hipFree(h->d_line); h->d_line = nullptr; }
   int rc = dalloc(h, &h->d_line, tot);
   if (rc) return rc;
-  std::vector<double> lb(tot);
+  std::vector<double> lb(tot, 0.0);
   size_t off = 0;
   for (int irk = 1; irk <= 3; irk++)
     for (int s = 0; s < 6; s++) {
@@ -483,7 +483,9 @@ int build_line_coefs(vrh_t *h) {
       for (int q = 0; q < 7; q++) {
         std::memcpy(&lb[off], lcv[irk][s][q].data(), lcv[irk][s][q].size() * sizeof(double));
         ptr[q] = h->d_line + off;
-        off += lcv[irk][s][q].size();
+        // per-position arrays carry kLinePad zeros: a segment reads its M
+        // positions without clamping (rows past the line end are inert)
+        off += lcv[irk][s][q].size() + (q < 5 ? kLinePad : 0);
       }
       LineCoef &L = h->lc[irk][s];
       L.A = ptr[0]; L.B = ptr[1]; L.C = ptr[2]; L.Pk = ptr[3]; L.Qk = ptr[4]; L.Pseg = ptr[5]; L.Qseg = ptr[6];

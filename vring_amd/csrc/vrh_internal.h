@@ -37,6 +37,9 @@ struct VRK {
 // solver: forward g_p = A_p r_p + B_p g_{p-1}; backward x_p = g_p - C_p x_{p+1};
 // Pk/Qk prefix products inside each segment of length M, Pseg/Qseg their
 // segment totals.
+// A, B, C, Pk, Qk: n positions followed by kLinePad zeros (>= the longest
+// segment), so that a segment may read all its positions unclamped.
+constexpr int kLinePad = 64;
 struct LineCoef {
   const double *A, *B, *C, *Pk, *Qk, *Pseg, *Qseg;
   int n, P;

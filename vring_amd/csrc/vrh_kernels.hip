@@ -517,43 +517,43 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_pe
   const int s = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const long L = (long)blockIdx.x * 64 + lane;
   const bool act = L < a.nlines;
+  const long Lc = act ? L : a.nlines - 1;  // addresses stay in bounds; results of !act lanes are dropped
   const VGrid &g = a.g;
   const int n = a.lc.n, P = a.lc.P;
   const int p0 = s * M;
   const int len = min(M, n - p0);
   TW *rr = reinterpret_cast<TW *>(a.r);
   // line -> (i, second index)
-  const int i = (int)(L % g.nt) + 1;
-  const int l2 = (int)(L / g.nt);
+  const int i = (int)(Lc % g.nt) + 1;
+  const int l2 = (int)(Lc / g.nt);
   long base;  // address of position 0 of the line
   long step;  // address stride between positions
   if (MODE <= SW_R2) { base = hidx(g, i, 1, l2 + 1); step = g.sj; }      // r lines at k = l2+1
   else { base = hidx(g, i, l2 + a.j0, 1); step = g.sk; }                  // z lines at j = l2+j0
   const int jz = l2 + a.j0;
+  // every load of the segment is issued before the first use (positions
+  // beyond the line clamp to its last one and are masked afterwards)
   TW x[M];
+#pragma unroll
+  for (int t = 0; t < M; ++t) x[t] = rr[base + step * min(p0 + t, n - 1)];
+  TW rlast = (TW)0;
+  if (MODE == SW_Z3) rlast = (TW)a.corr3n[(long)(i - 1) + (long)g.nt * jz];
+  __builtin_amdgcn_sched_barrier(0);
   TW gp = (TW)0;
 #pragma unroll
   for (int t = 0; t < M; ++t) {
-    if (t < len) {
-      const int p = p0 + t;
-      TW rv = (TW)0;
-      if (act) {
-        if (MODE == SW_R2) {
-          if (p != 0 && p != n - 1) rv = rr[base + step * p];
-        } else if (MODE == SW_Z1 || MODE == SW_Z2) {
-          if (p != 0 && p != n - 1) rv = rr[base + step * p];
-        } else if (MODE == SW_Z3) {
-          if (p == 0) rv = (jz <= a.jlim) ? (TW)(a.c3s_dev ? *a.c3s_dev : a.c3s) : (TW)0;
-          else if (p == n - 1) rv = (TW)a.corr3n[(long)(i - 1) + (long)g.nt * jz];
-          else rv = rr[base + step * p];
-        } else {
-          rv = rr[base + step * p];
-        }
-      }
-      TW v = (TW)a.lc.A[p] * rv + (TW)a.lc.B[p] * gp;
-      x[t] = v;
-      gp = v;
+    const int p = p0 + t;
+    TW rv = x[t];
+    if (MODE == SW_R2 || MODE == SW_Z1 || MODE == SW_Z2) {
+      if (p == 0 || p == n - 1) rv = (TW)0;
+    } else if (MODE == SW_Z3) {
+      if (p == 0) rv = (jz <= a.jlim) ? (TW)(a.c3s_dev ? *a.c3s_dev : a.c3s) : (TW)0;
+      else if (p == n - 1) rv = rlast;
     }
+    const TW v = (TW)a.lc.A[p] * rv + (TW)a.lc.B[p] * gp;
+    const bool in = t < len;
+    x[t] = in ? v : (TW)0;
+    gp = in ? v : gp;
   }
   sh_a[s][lane] = gp;
   __syncthreads();
@@ -563,34 +563,52 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_pe
   TW xp = (TW)0;
 #pragma unroll
   for (int t = M - 1; t >= 0; --t) {
-    if (t < len) {
-      const int p = p0 + t;
-      TW v = x[t] + (TW)a.lc.Pk[p] * G - (TW)a.lc.C[p] * xp;
-      x[t] = v;
-      xp = v;
-    }
+    const int p = p0 + t;
+    const TW v = x[t] + (TW)a.lc.Pk[p] * G - (TW)a.lc.C[p] * xp;
+    const bool in = t < len;
+    x[t] = in ? v : x[t];
+    xp = in ? v : xp;
   }
   sh_a[s][lane] = xp;
   __syncthreads();
   TW X = (TW)0;
   for (int s2 = P - 1; s2 > s; --s2) X = sh_a[s2][lane] + (TW)a.lc.Qseg[s2] * X;
-  if (!act) return;
 #pragma unroll
-  for (int t = 0; t < M; ++t) {
-    if (t < len) {
-      const int p = p0 + t;
-      const TW v = x[t] + (TW)a.lc.Qk[p] * X;
-      const long ad = base + step * p;
-      if (MODE == SW_R1 || MODE == SW_R3 || MODE == SW_Z2) {
-        rr[ad] = v;
-      } else if (MODE == SW_R2) {
-        // dq2 = x + q2 for j = 1..nr; dq2(nrc) = 0 (step.F90:330-333, 183-187)
-        a.dq[ad] = (p == n - 1) ? 0.0 : (double)v + a.q[ad];
-      } else {
-        // dq = q + x for k = 1..nzc; slip row at nrc (step.F90:84-88, 133-138)
-        const double d = a.q[ad] + (double)v;
-        a.dq[ad] = d;
-        if (jz == g.nr) a.dq[ad + g.sj] = a.slip * d;
+  for (int t = 0; t < M; ++t) x[t] = x[t] + (TW)a.lc.Qk[p0 + t] * X;
+  if (MODE == SW_R1 || MODE == SW_R3 || MODE == SW_Z2) {
+    if (!act) return;
+#pragma unroll
+    for (int t = 0; t < M; ++t)
+      if (t < len) rr[base + step * (p0 + t)] = x[t];
+  } else {
+    // dq = q + x, in chunks of 8 positions whose q loads go out together
+    // before their stores
+    constexpr int CH = 8;
+#pragma unroll
+    for (int c0 = 0; c0 < M; c0 += CH) {
+      double qv[CH];
+#pragma unroll
+      for (int u = 0; u < CH; ++u)
+        qv[u] = (c0 + u < M) ? a.q[base + step * min(p0 + c0 + u, n - 1)] : 0.0;
+      __builtin_amdgcn_sched_barrier(0);
+      if (act) {
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          const int t = c0 + u;
+          if (t < M && t < len) {
+            const int p = p0 + t;
+            const long ad = base + step * p;
+            if (MODE == SW_R2) {
+              // dq2 = x + q2 for j = 1..nr; dq2(nrc) = 0 (step.F90:330-333, 183-187)
+              a.dq[ad] = (p == n - 1) ? 0.0 : (double)x[t] + qv[u];
+            } else {
+              // dq = q + x for k = 1..nzc; slip row at nrc (step.F90:84-88, 133-138)
+              const double d = qv[u] + (double)x[t];
+              a.dq[ad] = d;
+              if (jz == g.nr) a.dq[ad + g.sj] = a.slip * d;
+            }
+          }
+        }
       }
     }
   }
@@ -730,6 +748,18 @@ __device__ __forceinline__ void lds_fft(T *re, T *im, int nfft, int logn, const 
   }
 }
 
+// XCD-aware tile index: the dispatcher deals consecutive workgroups round
+// robin to the 8 XCDs, each with its own L2.  Renumbering so that tiles t
+// and t + 1 run on the same XCD, one dispatch round apart, lets tiles that
+// share 128-byte lines (narrow tiles of a strided layout) meet in one L2
+// instead of each fetching the whole line from HBM.  A bijection on
+// [0, nblk) for any nblk.
+__device__ __forceinline__ long xcd_tile(long b, long nblk) {
+  constexpr long kX = 8;
+  const long q = nblk / kX, rem = nblk % kX, x = b % kX, r = b / kX;
+  return x < rem ? x * (q + 1) + r : rem * (q + 1) + (x - rem) * q + r;
+}
+
 // byte offset rounded up to 16
 __host__ __device__ constexpr size_t al16(size_t b) { return (b + 15) & ~(size_t)15; }
 
@@ -769,6 +799,7 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
   if (pow2)
     for (int q = tid; q < N / 2; q += nth) { twl[q] = (TW)twr[q]; twl[N / 2 + q] = (TW)twi[q]; }
   __syncthreads();
+  #pragma unroll 4
   for (int e = tid; e < LB * N; e += nth) {
     const int lp = pow2 ? e >> logn : e / N, ii = e - lp * N;
     const long l = l0 + lp;
@@ -872,26 +903,38 @@ __global__ __launch_bounds__(256) void k_dctz(TW *__restrict__ S, long nl, int n
   TW *im = re + (size_t)(TL / 2) * n;
   TW *twl = im + (size_t)(TL / 2) * n;  // n/2 cos, n/2 sin
   const int tid = threadIdx.x, nth = blockDim.x;
-  const long l0 = (long)blockIdx.x * TL;
+  const long l0 = xcd_tile(blockIdx.x, gridDim.x) * TL;
   const bool pow2 = n >= 2 && (n & (n - 1)) == 0;
   if (pow2) {
     for (int q = tid; q < n / 2; q += nth) { twl[q] = (TW)twr[q]; twl[n / 2 + q] = (TW)twi[q]; }
-    for (int e = tid; e < TL * n; e += nth) {
-      const int lp = e & (TL - 1), kz = e >> logtl;  // consecutive threads -> consecutive lines
-      const long l = l0 + lp;
-      const int q = lp >> 1;
-      if (dir > 0) {
-        TW v = (l < nl) ? S[l + nl * kz] : (TW)0;
+    // loads are unconditional (line index clamped, then masked) so that the
+    // unrolled iterations issue them together
+    if (dir > 0) {
+#pragma unroll 4
+      for (int e = tid; e < TL * n; e += nth) {
+        const int lp = e & (TL - 1), kz = e >> logtl;  // consecutive threads -> consecutive lines
+        const long l = l0 + lp;
+        const int q = lp >> 1;
+        TW v = S[min(l, nl - 1) + nl * kz];
+        v = (l < nl) ? v : (TW)0;
         const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
         const int slot = q * n + p;
         if (lp & 1) im[slot] = v; else re[slot] = v;
-      } else if ((lp & 1) == 0) {
+      }
+    } else {
+      const int ltq = logtl - 1, TQ = TL >> 1;
+#pragma unroll 2
+      for (int e = tid; e < TQ * n; e += nth) {
         // W[k] = e^{i pi k/2n} ((Ya[k] + Yb[n-k]) + i (Yb[k] - Ya[n-k])), Y[n]=0
-        const long la = l, lb = l + 1;
-        const int k = kz;
-        const TW ya = (la < nl) ? S[la + nl * k] : (TW)0, yb = (lb < nl) ? S[lb + nl * k] : (TW)0;
-        const TW yan = (k > 0 && la < nl) ? S[la + nl * (n - k)] : (TW)0;
-        const TW ybn = (k > 0 && lb < nl) ? S[lb + nl * (n - k)] : (TW)0;
+        const int q = e & (TQ - 1), k = e >> ltq;
+        const long la = l0 + 2 * q, lb = la + 1;
+        const long ca = min(la, nl - 1), cb = min(lb, nl - 1);
+        const int kn = (n - k) & (n - 1);
+        TW ya = S[ca + nl * k], yb = S[cb + nl * k], yan = S[ca + nl * kn], ybn = S[cb + nl * kn];
+        ya = (la < nl) ? ya : (TW)0;
+        yb = (lb < nl) ? yb : (TW)0;
+        yan = (k > 0 && la < nl) ? yan : (TW)0;
+        ybn = (k > 0 && lb < nl) ? ybn : (TW)0;
         const TW u = ya + ybn, w = yb - yan;
         const TW cr = (TW)er[k], ci = (TW)ei[k];  // e^{+i pi k / 2n}
         const int slot = q * n + k;
@@ -958,24 +1001,31 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_pe
   const int P = blockDim.x >> 6;
   const long L = (long)blockIdx.x * 64 + lane;
   const bool act = L < nlines;
-  const long pos = L % nm, kz = L / nm;
+  const long Lc = act ? L : nlines - 1;  // addresses stay in bounds; results of !act lanes are dropped
+  const long pos = Lc % nm, kz = Lc / nm;
   const long base = pos + nm * (long)nr * kz;
   const int p0 = s * M, len = min(M, nr - p0);
+  // all loads of the segment go out before the first use; positions past
+  // the line end clamp to its last row and are masked (uniform selects)
   T x[M], w[M];
+#pragma unroll
+  for (int t = 0; t < M; ++t) {
+    const long ad = base + nm * min(p0 + t, nr - 1);
+    w[t] = ib[ad];
+    x[t] = S[ad];
+  }
+  __builtin_amdgcn_sched_barrier(0);
   T gp = (T)0, Pacc = (T)1;
 #pragma unroll
   for (int t = 0; t < M; ++t) {
-    if (t < len) {
-      const int p = p0 + t;
-      const long ad = base + nm * p;
-      const T ibv = act ? ib[ad] : (T)1, rv = act ? S[ad] : (T)0;
-      const T B = -(T)aab[p + 1] * ibv * (T)(p > 0);
-      const T v = ibv * rv + B * gp;
-      w[t] = ibv;
-      x[t] = v;
-      gp = v;
-      Pacc *= B;
-    }
+    const int p = p0 + t, pc = min(p, nr - 1);
+    const bool in = t < len;
+    const T ibv = w[t];
+    const T B = -(T)aab[pc + 1] * ibv * (T)(p > 0);
+    const T v = ibv * x[t] + B * gp;
+    x[t] = in ? v : (T)0;
+    gp = in ? v : gp;
+    Pacc = in ? Pacc * B : Pacc;
   }
   sh_v[s][lane] = gp;
   sh_p[s][lane] = Pacc;
@@ -986,37 +1036,35 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_pe
   T Pr = (T)1;
 #pragma unroll
   for (int t = 0; t < M; ++t) {
-    if (t < len) {
-      const int p = p0 + t;
-      Pr *= -(T)aab[p + 1] * w[t] * (T)(p > 0);
-      x[t] += Pr * G;
-    }
+    const int p = p0 + t, pc = min(p, nr - 1);
+    const bool in = t < len;
+    Pr = in ? Pr * (-(T)aab[pc + 1] * w[t] * (T)(p > 0)) : Pr;
+    x[t] = in ? x[t] + Pr * G : x[t];
   }
   T xp = (T)0, Qacc = (T)1;
 #pragma unroll
   for (int t = M - 1; t >= 0; --t) {
-    if (t < len) {
-      const int p = p0 + t;
-      const T C = (T)ccb[p + 1] * w[t] * (T)(p < nr - 1);
-      const T v = x[t] - C * xp;
-      x[t] = v;
-      xp = v;
-      Qacc *= -C;
-    }
+    const int p = p0 + t, pc = min(p, nr - 1);
+    const bool in = t < len;
+    const T C = (T)ccb[pc + 1] * w[t] * (T)(p < nr - 1);
+    const T v = x[t] - C * xp;
+    x[t] = in ? v : x[t];
+    xp = in ? v : xp;
+    Qacc = in ? Qacc * -C : Qacc;
   }
   sh_v[s][lane] = xp;
   sh_p[s][lane] = Qacc;
   __syncthreads();
   T X = (T)0;
   for (int s2 = P - 1; s2 > s; --s2) X = sh_v[s2][lane] + sh_p[s2][lane] * X;
+  if (!act) return;
   T Qr = (T)1;
 #pragma unroll
   for (int t = M - 1; t >= 0; --t) {
     if (t < len) {
       const int p = p0 + t;
       Qr *= -(T)ccb[p + 1] * w[t] * (T)(p < nr - 1);
-      const T v = x[t] + Qr * X;
-      if (act) S[base + nm * p] = v;
+      S[base + nm * p] = x[t] + Qr * X;
     }
   }
 }
@@ -1045,10 +1093,12 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
   }
   if (pow2)
     for (int q = tid; q < N / 2; q += nth) { twl[q] = (TW)twr[q]; twl[N / 2 + q] = (TW)twi[q]; }
+  #pragma unroll 4
   for (int e = tid; e < LB * N; e += nth) {
     const int lp = pow2 ? e >> logn : e / N, pos = e - lp * N;
     const long l = l0 + lp;
-    TW v = (l < nlines) ? S[(size_t)pos + (size_t)N * l] : (TW)0;
+    TW v = S[(size_t)pos + (size_t)N * min(l, nlines - 1)];
+    v = (l < nlines) ? v : (TW)0;
     if (S64 && pos < 5) v = (TW)0;
     tb[e] = v;
   }
@@ -1471,12 +1521,17 @@ static int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
 template <typename T>
 static void radial_launch(vrh_t *h, T *S, const T *ib, long nm) {
   const VGrid &g = h->g;
-  const int M = 32;
+  // rows per wave: 16 keeps the segment's rhs and pivots in registers (a
+  // 1024-lane block allows 128 VGPRs) up to nr = 256; 32 beyond (P <= 16)
+  const int M = g.nr <= 256 ? 16 : 32;
   const int P = (g.nr + M - 1) / M;
   const long nlines = nm * g.nz;
   dim3 blk(64 * P), grd((unsigned)((nlines + 63) / 64));
   KTimer kt(h, KT_RADIAL);
-  hipLaunchKernelGGL((k_radial<T, 32>), grd, blk, 0, h->stream, S, ib, nm, nlines, g.nr, h->d_aab, h->d_ccb);
+  if (M == 16)
+    hipLaunchKernelGGL((k_radial<T, 16>), grd, blk, 0, h->stream, S, ib, nm, nlines, g.nr, h->d_aab, h->d_ccb);
+  else
+    hipLaunchKernelGGL((k_radial<T, 32>), grd, blk, 0, h->stream, S, ib, nm, nlines, g.nr, h->d_aab, h->d_ccb);
 }
 
 template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_rhs, double *phi_out) {
