@@ -85,7 +85,7 @@ constexpr int kMaxPcrLevels = 40;
 constexpr int kRowCoefs = 24;
 
 template <typename TW>
-__global__ __launch_bounds__(512) void k_explicit_march(VGrid g, VCoef c, VRK rk, double betadx, int nlev, int kchunk,
+__global__ __launch_bounds__(512, sizeof(TW) == 4 ? 4 : 3) void k_explicit_march(VGrid g, VCoef c, VRK rk, double betadx, int nlev, int kchunk,
                                                   const double *__restrict__ q1, const double *__restrict__ q2,
                                                   const double *__restrict__ q3, const TW *__restrict__ prg1,
                                                   const TW *__restrict__ prg2, const TW *__restrict__ prg3,
@@ -142,16 +142,18 @@ __global__ __launch_bounds__(512) void k_explicit_march(VGrid g, VCoef c, VRK rk
     const int jj = j0 - 1 + r;
     lidx[l] = (e < PL) ? e : -1;
     lval[l] = (e < PL) && cc < nt && jj <= nr + 2;
-    goff[l] = (long)cc + (long)jj * sj;
+    goff[l] = lval[l] ? (long)cc + (long)jj * sj : 0;
   }
   TW pre[3][NL];
+  // unconditional loads (masked after) so that a plane's loads issue together
   auto fetch = [&](int kk) {
 #pragma unroll
     for (int l = 0; l < NL; l++) {
       const long o = goff[l] + (long)kk * sk;
-      pre[0][l] = lval[l] ? (TW)q1[o] : (TW)0;
-      pre[1][l] = lval[l] ? (TW)q2[o] : (TW)0;
-      pre[2][l] = lval[l] ? (TW)q3[o] : (TW)0;
+      const double a1 = q1[o], a2 = q2[o], a3 = q3[o];
+      pre[0][l] = lval[l] ? (TW)a1 : (TW)0;
+      pre[1][l] = lval[l] ? (TW)a2 : (TW)0;
+      pre[2][l] = lval[l] ? (TW)a3 : (TW)0;
     }
   };
   auto stash = [&](int slot) {
@@ -194,13 +196,36 @@ __global__ __launch_bounds__(512) void k_explicit_march(VGrid g, VCoef c, VRK rk
 #define irmm cf[20]
   const int ci = ti, cip = (ti + 1 >= nt) ? 0 : ti + 1, cim = (ti == 0) ? nt - 1 : ti - 1;
   const int rw = tj + 1, rm1 = (jc == 1) ? rw : rw - 1;   // LDS rows of j and jm
+  // ru and prg of the next plane stream in while this one is computed
+  // (clamped in-bounds addresses for lanes and planes that do not use them)
+  const int tic = min(ti, nt - 1);
+  TW n_ru[3], n_pg[3];
+  auto load_rp = [&](int kk) {
+    const int kc = min(kk, ke);
+    const long oo = (long)tic + (long)jc * sj + (long)kc * sk;
+    const long pp = (long)tic + (long)nt * ((long)(jc - 1) + (long)nr * (long)(kc - 1));
+    n_ru[0] = ru1[oo]; n_ru[1] = ru2[oo]; n_ru[2] = ru3[oo];
+    n_pg[0] = prg1[pp]; n_pg[1] = prg2[pp]; n_pg[2] = prg3[pp];
+  };
+  // FP32: one plane ahead; FP64 (register budget): this plane, issued with
+  // the ring fetch
+  constexpr bool kAhead = sizeof(TW) == 4;
+  if (kAhead) load_rp(kb);
   for (int k = kb; k <= ke; k++) {
     stash(sP);
+    TW c_ru1 = n_ru[0], c_ru2 = n_ru[1], c_ru3 = n_ru[2];
+    TW c_pg1 = n_pg[0], c_pg2 = n_pg[1], c_pg3 = n_pg[2];
     __syncthreads();
     fetch(k + 2);
+    if (kAhead) {
+      load_rp(k + 1);
+    } else {
+      load_rp(k);
+      c_ru1 = n_ru[0]; c_ru2 = n_ru[1]; c_ru3 = n_ru[2];
+      c_pg1 = n_pg[0]; c_pg2 = n_pg[1]; c_pg3 = n_pg[2];
+    }
     TW v1 = (TW)0, v2 = (TW)0, v3 = (TW)0;
     const long o = (long)ti + (long)j * sj + (long)k * sk;
-    const long pc = (long)ti + (long)nt * ((long)(j - 1) + (long)nr * (long)(k - 1));
     if (cell) {
       const TW *A1m = ring + (sM * 3 + 0) * PL, *A1 = ring + (s0 * 3 + 0) * PL, *A1p = ring + (sP * 3 + 0) * PL;
       const TW *A2m = ring + (sM * 3 + 1) * PL, *A2 = ring + (s0 * 3 + 1) * PL, *A2p = ring + (sP * 3 + 1) * PL;
@@ -238,7 +263,7 @@ __global__ __launch_bounds__(512) void k_explicit_march(VGrid g, VCoef c, VRK rk
         TW qn2 = (q1n * irmn - q1c * irm) * s1t2n;
         TW qs2 = (q1c * irm - q1jm * irmm) * s1t2s;
         TW d22 = (qn2 - qs2) * s1t2c;
-        v1 = (ga * H1 + ro * ru1[o] - al * prg1[pc] + aRe * (d11 + d33 + d22)) * dt;
+        v1 = (ga * H1 + ro * c_ru1 - al * c_pg1 + aRe * (d11 + d33 + d22)) * dt;
         ru1[o] = H1;
       }
       // ---- q2 at (theta_m, r_c, z_m), j >= 2 ----
@@ -262,7 +287,7 @@ __global__ __launch_bounds__(512) void k_explicit_march(VGrid g, VCoef c, VRK rk
         TW d11 = (q2ip - (TW)2 * q2c + q2im) * s2t11;
         TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (q2kp - (TW)3 * q2c) * dx3q : (q2kp - (TW)2 * q2c + q2km) * dx3q;
         TW d22 = q2n * s2t2p - q2c * s2t2c + q2s * s2t2m;
-        v2 = (ga * H2 + ro * ru2[o] - al * prg2[pc] + aRe * (d11 + d33 + d22)) * dt;
+        v2 = (ga * H2 + ro * c_ru2 - al * c_pg2 + aRe * (d11 + d33 + d22)) * dt;
         ru2[o] = H2;
       }
       // ---- q3 at (theta_m, r_m, z_c), k >= 2 ----
@@ -282,7 +307,7 @@ __global__ __launch_bounds__(512) void k_explicit_march(VGrid g, VCoef c, VRK rk
         TW qn2 = (q3n - q3c) * s3t2n;
         TW qs2 = (q3c - q3jm) * s3t2s;
         TW d22 = qn2 - qs2;
-        v3 = (ga * H3 + ro * ru3[o] - al * prg3[pc] + aRe * (d11 + d33 + d22)) * dt;
+        v3 = (ga * H3 + ro * c_ru3 - al * c_pg3 + aRe * (d11 + d33 + d22)) * dt;
         ru3[o] = H3;
       }
     }
@@ -1206,13 +1231,19 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
     if (ii < 0) ii += nt;
     lrow[l] = r; lcol[l] = cc;
     lval[l] = (e < NE) && jj >= 0 && jj <= nr + 2;
-    goff[l] = (long)ii + (long)jj * sj;
+    goff[l] = lval[l] ? (long)ii + (long)jj * sj : 0;
   }
   double pre[NL];
+  // unconditional loads from in-bounds (clamped) addresses, masked after:
+  // the loads of one plane issue together
   auto fetch = [&](int kk) {
+    const long kc = (long)min(max(kk, 0), nz + 2) * sk;
+    const bool kin = kk >= 0 && kk <= nz + 2;
 #pragma unroll
-    for (int l = 0; l < NL; l++)
-      pre[l] = (lval[l] && kk >= 0 && kk <= nz + 2) ? dph[goff[l] + (long)kk * sk] : 0.0;
+    for (int l = 0; l < NL; l++) {
+      const double v = dph[goff[l] + kc];
+      pre[l] = (lval[l] && kin) ? v : 0.0;
+    }
   };
   auto stash = [&](int kk) {
 #pragma unroll
@@ -1242,21 +1273,21 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
   const bool rowu = act && j >= 1 && j <= nr;
   double n_dq1 = 0.0, n_dq2 = 0.0, n_dq3 = 0.0, n_pres = 0.0;
   TW n_p1 = (TW)0, n_p2 = (TW)0, n_p3 = (TW)0;
+  // operands of cell (i, j, kk); lanes, rows and planes where a value is not
+  // used read a clamped in-bounds address instead of branching, so that the
+  // seven loads issue together
+  const int ic = min(i, nt), jc1 = min(max(j, 1), nr);
   auto load_cell = [&](int kk) {
-    if (!act || kk > ke) return;
-    const long oo = (long)(i - 1) + (long)j * sj + (long)kk * sk;
+    const int kc = min(kk, ke), kp = min(max(kc, 1), nz);
+    const long oo = (long)(ic - 1) + (long)min(j, nrc) * sj + (long)kc * sk;
+    const long pp = (long)(ic - 1) + (long)nt * ((long)(jc1 - 1) + (long)nr * (long)(kp - 1));
     n_dq3 = dq3[oo];
-    if (rowu) {
-      n_dq1 = dq1[oo];
-      if (j >= 2) n_dq2 = dq2[oo];
-      if (kk <= nz) {
-        const long pp = (long)(i - 1) + (long)nt * ((long)(j - 1) + (long)nr * (long)(kk - 1));
-        n_p1 = prg1[pp];
-        if (j >= 2) n_p2 = prg2[pp];
-        n_p3 = prg3[pp];
-        n_pres = pres[oo];
-      }
-    }
+    n_dq1 = dq1[oo];
+    n_dq2 = dq2[oo];
+    n_p1 = prg1[pp];
+    n_p2 = prg2[pp];
+    n_p3 = prg3[pp];
+    n_pres = pres[oo];
   };
   load_cell(kb);
   for (int k = kb; k <= ke; k++) {
