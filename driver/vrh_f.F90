@@ -120,6 +120,23 @@ module vrh_f
       integer(c_int), value :: batch
       integer(c_long), value :: n
     end function vrh_particles_download
+    integer(c_int) function vrh_step_async(h, nsteps) bind(C, name='vrh_step_async')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: nsteps
+    end function vrh_step_async
+
+    integer(c_int) function vrh_snapshot_begin(h) bind(C, name='vrh_snapshot_begin')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function vrh_snapshot_begin
+
+    integer(c_int) function vrh_snapshot_wait(h, q1, q2, q3, pres, t) bind(C, name='vrh_snapshot_wait')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h, q1, q2, q3, pres
+      real(c_double), intent(out) :: t
+    end function vrh_snapshot_wait
+
     integer(c_int) function vrh_analysis(h, out) bind(C, name='vrh_analysis')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h

@@ -15,7 +15,9 @@ program vring_main
   real(c_double), allocatable, target :: pu(:, :), phs(:)
   integer(c_int), allocatable, target :: palive(:)
   integer(c_int), target :: choice(9)
-  real(c_double) :: ntime, stat(20), dtcfl, st, astat(10)
+  real(c_double) :: ntime, stat(20), dtcfl, st, astat(10), tsnap
+  logical :: pending, panl, export, anl
+  integer :: pstep
   real(c_double) :: t0, t1, t2, t3
   integer :: nt, nr, nz, istep, nstep, nstepi, rc, np, ip
   integer(c_long) :: capl(1)
@@ -102,35 +104,37 @@ program vring_main
     write(*, '(a,9i5)') 'autotune choice: ', choice
   end if
   stat = 0.d0
+  pending = .false.
   call cpu_time(t0)
   do istep = 0, d%totstp
     nstep = istep + nstepi
     call cpu_time(t1)
-    call vrh_check(h, vrh_step(h, 1), 'step')
+    ! the step runs on the GPU while the host writes the files of the last
+    ! field export (taken as a device snapshot at that step)
+    call vrh_check(h, vrh_step_async(h, 1), 'step')
+    if (pending) call flush_export()
+    call vrh_check(h, vrh_synchronize(h), 'step')
     call cpu_time(t2)
     call vrh_check(h, vrh_get_time(h, ntime), 'get_time')
-    if (d%expstp > 0) then
-      if (mod(nstep, d%expstp) == 0) then
-        call vrh_check(h, vrh_download_fields(h, c_loc(q1), c_loc(q2), c_loc(q3), c_loc(pres)), 'download')
-        call restart_save(nstep, 3, ntime, nt, nr, nz, q1, q2, q3)
-        if (d%npart > 0) then
-          call vrh_check(h, vrh_particles_download(h, 0, int(np, c_long), c_loc(pu), c_loc(palive)), 'particles')
-          call part_export(nstep, 1, d%pradius / 0.02d0, st, np, size(pu, 1), pu, palive)
-        end if
+    export = .false.
+    if (d%expstp > 0) export = mod(nstep, d%expstp) == 0
+    anl = .false.
+    if (d%anlstp > 0) anl = mod(nstep, d%anlstp) == 0
+    if (export) then
+      ! restart file, and (on analysis steps) the host analysis pass with the
+      ! Tecplot files, from the snapshot during the next step
+      call vrh_check(h, vrh_snapshot_begin(h), 'snapshot')
+      pending = .true.
+      pstep = nstep
+      panl = anl
+      if (d%npart > 0) then
+        call vrh_check(h, vrh_particles_download(h, 0, int(np, c_long), c_loc(pu), c_loc(palive)), 'particles')
+        call part_export(nstep, 1, d%pradius / 0.02d0, st, np, size(pu, 1), pu, palive)
       end if
-    end if
-    if (d%anlstp > 0) then
-      if (mod(nstep, d%anlstp) == 0) then
-        if (d%expstp > 0 .and. mod(nstep, max(d%expstp, 1)) == 0) then
-          ! field export step: the host pass writes the Tecplot files too
-          call vrh_check(h, vrh_download_fields(h, c_loc(q1), c_loc(q2), c_loc(q3), c_loc(pres)), 'download')
-          call analysis(d, nstep, 3, ntime, nt, nr, nz, q1, q2, q3, pres, stat)
-        else
-          ! statistics only: reduced on the device, ten numbers come back
-          call vrh_check(h, vrh_analysis(h, astat), 'analysis')
-          call analysis_record(d, nstep, ntime, astat, stat)
-        end if
-      end if
+    else if (anl) then
+      ! statistics only: reduced on the device, ten numbers come back
+      call vrh_check(h, vrh_analysis(h, astat), 'analysis')
+      call analysis_record(d, nstep, ntime, astat, stat)
     end if
     call cpu_time(t3)
     ! loop.F90:144-151 (times in minutes)
@@ -143,6 +147,19 @@ program vring_main
     write(*, "(a,I10,3(a,F10.6),a,I10)") " E: ", d%expstp, " Dt:", d%dt, " P2:", 0.d0, " TT:", &
         (t3 - t0) / 60.d0 / real(max(istep, 1), c_double), " TZ:", 1
   end do
+  if (pending) call flush_export()
   call vrh_check(h, vrh_download_fields(h, c_loc(q1), c_loc(q2), c_loc(q3), c_loc(pres)), 'download')
   rc = vrh_destroy(h)
+
+contains
+
+  ! files of the pending field export: restart_save (run.F90:390-466), then
+  ! the analysis pass that writes the Tecplot files (run.F90:238-388)
+  subroutine flush_export()
+    call vrh_check(h, vrh_snapshot_wait(h, c_loc(q1), c_loc(q2), c_loc(q3), c_loc(pres), tsnap), 'snapshot')
+    call restart_save(pstep, 3, tsnap, nt, nr, nz, q1, q2, q3)
+    if (panl) call analysis(d, pstep, 3, tsnap, nt, nr, nz, q1, q2, q3, pres, stat)
+    pending = .false.
+  end subroutine flush_export
+
 end program vring_main

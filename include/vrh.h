@@ -78,6 +78,21 @@ int vrh_get_time(vrh_t *h, double *ntime);
 int vrh_substep(vrh_t *h, int irk);
 /* nsteps full steps: ntime += dt, then substeps 1,2,3 (loop.F90:53-131). */
 int vrh_step(vrh_t *h, int nsteps);
+/* vrh_step without the final wait: the steps are queued on the handle's
+ * stream and the call returns; vrh_synchronize waits and reports the crash
+ * flag.  The host may work meanwhile (e.g. write the files of a snapshot);
+ * calls that read or replace device state wait on the stream themselves.
+ * No reference counterpart (the reference steps synchronously). */
+int vrh_step_async(vrh_t *h, int nsteps);
+/* Asynchronous field export (restart_save and the
+ * Tecplot writers, run.F90:238-466, read these fields): _begin copies q1,
+ * q2, q3, pres on the device (stream-ordered after queued steps) and drains
+ * the copy to pinned host memory on a second stream, so stepping may go on
+ * at once; _wait blocks until the copy has landed and hands the haloed
+ * fields (any pointer may be NULL) and their time out.  One snapshot may be
+ * pending per handle. */
+int vrh_snapshot_begin(vrh_t *h);
+int vrh_snapshot_wait(vrh_t *h, double *q1, double *q2, double *q3, double *pres, double *ntime);
 /* vrh_step replays one RK3 step captured as a HIP graph (captured on first
  * use, recaptured after vrh_autotune or a particle change; not used while the
  * kernel timer is on).  on = 0 launches the kernels one by one instead.  No

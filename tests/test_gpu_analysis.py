@@ -47,3 +47,38 @@ def test_analysis_of_a_ring_after_steps():
     s.step(4)
     check(s, s.download("q1"), s.download("q2"), s.download("q3"), nr, nz, R, L)
     s.close()
+
+
+def test_snapshot_is_the_state_at_its_time_while_stepping_goes_on():
+    """vrh_snapshot_begin/_wait hand out exactly the fields of the moment the
+    snapshot was taken, although steps queued after it (vrh_step_async)
+    change them; asynchronous and synchronous stepping give identical
+    states."""
+    print("case snapshot ring 32x48x96", flush=True)
+    nt, nr, nz, R, L = 32, 48, 96, 4.0, 12.0
+    q = vring_amd.ring_initial_condition(nt, nr, nz, R, L, [(1.0, 3.0, 0.2, 1.0)], pert_amp=0.02,
+                                         pert_modes=(2, 4))
+    a = vring_amd.Solver(nt, nr, nz, R, L, 1000.0, 0.01)
+    b = vring_amd.Solver(nt, nr, nz, R, L, 1000.0, 0.01)
+    for s in (a, b):
+        for n, arr in zip(("q1", "q2", "q3"), q):
+            s.upload(n, arr)
+    a.step(2)
+    ref = [a.download(n) for n in ("q1", "q2", "q3", "pres")]
+    t_ref = a.time()
+    a.snapshot_begin()
+    a.step_async(3)
+    snap = a.snapshot_wait()
+    a.synchronize()
+    for x, y in zip(snap[:4], ref):
+        assert np.array_equal(x, y)
+    assert snap[4] == t_ref
+    assert not np.array_equal(a.download("q3"), ref[2])
+    b.step_async(2)
+    b.step_async(3)
+    b.synchronize()
+    assert a.time() == b.time()
+    for n in ("q1", "q2", "q3", "pres"):
+        assert np.array_equal(a.download(n), b.download(n)), n
+    a.close()
+    b.close()

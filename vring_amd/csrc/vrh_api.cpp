@@ -327,6 +327,11 @@ int vrh_destroy(vrh_t *h) {
   graph_invalidate(h);
   if (h->d_sc) hipFree(h->d_sc);
   if (h->h_pin) hipHostFree(h->h_pin);
+  if (h->copy_stream) hipStreamSynchronize(h->copy_stream);
+  if (h->d_snap) hipFree(h->d_snap);
+  if (h->h_snap) hipHostFree(h->h_snap);
+  if (h->snap_ev) hipEventDestroy(h->snap_ev);
+  if (h->copy_stream) hipStreamDestroy(h->copy_stream);
   for (auto &b : h->pb) {
     if (b.u) hipFree(b.u);
     if (b.ddt2) hipFree(b.ddt2);
@@ -391,7 +396,12 @@ int vrh_device_ptr(vrh_t *h, const char *name, void **ptr) {
   return 0;
 }
 
-int vrh_set_time(vrh_t *h, double t) { if (!h) return -1; h->ntime = t; return 0; }
+int vrh_set_time(vrh_t *h, double t) {
+  if (!h) return -1;
+  h->ntime = t;
+  h->time_dirty = true;
+  return 0;
+}
 int vrh_get_time(vrh_t *h, double *t) { if (!h || !t) return -1; *t = h->ntime; return 0; }
 
 int vrh_substep(vrh_t *h, int irk) {
@@ -407,13 +417,17 @@ int vrh_phase(vrh_t *h, int phase, int irk) {
   return substep(h, irk, phase);
 }
 
-int vrh_step(vrh_t *h, int nsteps) {
-  if (!h || nsteps < 0) return -1;
-  if (nsteps == 0) return 0;
-  // the device copy of ntime starts from the host value; h_pin is free here
-  // because every vrh_step ends with the stream drained
-  h->h_pin[0] = h->ntime;
-  HIPCHK(h, hipMemcpyAsync(h->d_sc, h->h_pin, sizeof(double), hipMemcpyHostToDevice, h->stream));
+namespace {
+// queue nsteps whole steps; the device copy of ntime advances with them and
+// is refreshed from the host value only after vrh_set_time (or creation),
+// when the handle is idle, so h_pin is never overwritten under a pending copy
+int step_enqueue(vrh_t *h, int nsteps) {
+  if (h->time_dirty) {
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->h_pin[0] = h->ntime;
+    HIPCHK(h, hipMemcpyAsync(h->d_sc, h->h_pin, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    h->time_dirty = false;
+  }
   const bool graph = h->use_graph && !h->kt_on;
   if (graph && !h->graph_ok) { int rc = build_graph(h); if (rc) return rc; }
   for (int s = 0; s < nsteps; s++) {
@@ -425,7 +439,55 @@ int vrh_step(vrh_t *h, int nsteps) {
       if (rc) return rc;
     }
   }
+  return 0;
+}
+}  // namespace
+
+int vrh_step(vrh_t *h, int nsteps) {
+  if (!h || nsteps < 0) return -1;
+  if (nsteps == 0) return 0;
+  int rc = step_enqueue(h, nsteps);
+  if (rc) return rc;
   return check_flag(h);
+}
+
+int vrh_step_async(vrh_t *h, int nsteps) {
+  if (!h || nsteps < 0) return -1;
+  return nsteps == 0 ? 0 : step_enqueue(h, nsteps);
+}
+
+int vrh_snapshot_begin(vrh_t *h) {
+  if (!h) return -1;
+  if (h->snap_pending) return fail(h, "a snapshot is pending: vrh_snapshot_wait first");
+  const size_t n = haloed(h);
+  if (!h->d_snap) {
+    HIPCHK(h, hipMalloc(&h->d_snap, 4 * n * sizeof(double)));
+    HIPCHK(h, hipHostMalloc(&h->h_snap, 4 * n * sizeof(double), hipHostMallocDefault));
+    HIPCHK(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+    HIPCHK(h, hipEventCreateWithFlags(&h->snap_ev, hipEventDisableTiming));
+  }
+  const double *src[4] = {h->f.q1, h->f.q2, h->f.q3, h->f.pres};
+  for (int q = 0; q < 4; q++)
+    HIPCHK(h, hipMemcpyAsync(h->d_snap + q * n, src[q], n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  HIPCHK(h, hipEventRecord(h->snap_ev, h->stream));
+  HIPCHK(h, hipStreamWaitEvent(h->copy_stream, h->snap_ev, 0));
+  HIPCHK(h, hipMemcpyAsync(h->h_snap, h->d_snap, 4 * n * sizeof(double), hipMemcpyDeviceToHost, h->copy_stream));
+  h->snap_time = h->ntime;
+  h->snap_pending = true;
+  return 0;
+}
+
+int vrh_snapshot_wait(vrh_t *h, double *q1, double *q2, double *q3, double *pres, double *t) {
+  if (!h) return -1;
+  if (!h->snap_pending) return fail(h, "no snapshot pending");
+  HIPCHK(h, hipStreamSynchronize(h->copy_stream));
+  h->snap_pending = false;
+  const size_t n = haloed(h);
+  double *dst[4] = {q1, q2, q3, pres};
+  for (int q = 0; q < 4; q++)
+    if (dst[q]) std::memcpy(dst[q], h->h_snap + q * n, n * sizeof(double));
+  if (t) *t = h->snap_time;
+  return 0;
 }
 
 int vrh_set_graph(vrh_t *h, int on) {
@@ -758,6 +820,7 @@ int vrh_ensemble_step(vrh_ens_t *e, int nsteps) {
   for (vrh_t *h : e->m) {
     h->h_pin[0] = h->ntime;
     ECHK(e, hipMemcpyAsync(h->d_sc, h->h_pin, sizeof(double), hipMemcpyHostToDevice, e->origin));
+    h->time_dirty = false;
   }
   for (int s = 0; s < nsteps; s++) {
     for (vrh_t *h : e->m) h->ntime = h->dt + h->ntime;  // loop.F90:53

@@ -136,7 +136,16 @@ struct vrh_s {
   // sum of da over the inlet rows (both theta and r)
   double *d_sc = nullptr;
   double *h_pin = nullptr;  // pinned host staging (time)
+  bool time_dirty = true;   // host ntime set outside stepping: upload before the next step
   double inlet_area = 0.0;
+  // field snapshot for asynchronous export: device copy of q1, q2, q3, pres
+  // taken on the compute stream, drained to pinned host memory on a copy
+  // stream while stepping continues
+  double *d_snap = nullptr, *h_snap = nullptr;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t snap_ev = nullptr;
+  bool snap_pending = false;
+  double snap_time = 0.0;
   // one RK3 step captured as a HIP graph (rebuilt when launch shapes,
   // particles or the timer change)
   bool use_graph = true;

@@ -34,7 +34,8 @@ EXPORTS = ["vrh_create", "vrh_destroy", "vrh_last_error", "vrh_upload_fields", "
            "vrh_step", "vrh_phase", "vrh_poisson_solve", "vrh_synchronize", "vrh_device_ptr",
            "vrh_ring_init", "vrh_cfl_dt", "vrh_particles_create", "vrh_particles_upload",
            "vrh_particles_download", "vrh_particles_step", "vrh_ktimer_enable", "vrh_ktimer_read", "vrh_autotune",
-           "vrh_set_graph", "vrh_set_tuning", "vrh_analysis", "vrh_particles_compact", "vrh_particles_count",
+           "vrh_set_graph", "vrh_set_tuning", "vrh_analysis", "vrh_step_async", "vrh_snapshot_begin",
+           "vrh_snapshot_wait", "vrh_particles_compact", "vrh_particles_count",
            "vrh_particles_inject", "vrh_ensemble_create", "vrh_ensemble_step", "vrh_ensemble_last_error",
            "vrh_ensemble_destroy"]
 
@@ -79,6 +80,9 @@ def load_library(path=LIBPATH):
                                   ctypes.POINTER(ctypes.c_long)]
     L.vrh_set_graph.argtypes = [P, I]
     L.vrh_analysis.argtypes = [P, dp]
+    L.vrh_step_async.argtypes = [P, I]
+    L.vrh_snapshot_begin.argtypes = [P]
+    L.vrh_snapshot_wait.argtypes = [P, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_double)]
     L.vrh_particles_compact.argtypes = [P, I]
     L.vrh_particles_count.argtypes = [P, I, ctypes.POINTER(ctypes.c_long)]
     L.vrh_particles_inject.argtypes = [P, I, D, I, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_long)]
@@ -218,6 +222,24 @@ class Solver:
 
     def synchronize(self):
         self._check(self.L.vrh_synchronize(self.h), "synchronize")
+
+    def step_async(self, n=1):
+        """Queue n steps and return; synchronize() waits and checks."""
+        self._check(self.L.vrh_step_async(self.h, int(n)), "step_async")
+
+    def synchronize(self):
+        self._check(self.L.vrh_synchronize(self.h), "synchronize")
+
+    def snapshot_begin(self):
+        self._check(self.L.vrh_snapshot_begin(self.h), "snapshot_begin")
+
+    def snapshot_wait(self):
+        """(q1, q2, q3, pres, time) of the pending snapshot, haloed."""
+        out = [np.zeros(int(np.prod(self.halo_shape))) for _ in range(4)]
+        t = ctypes.c_double()
+        self._check(self.L.vrh_snapshot_wait(self.h, *out, ctypes.byref(t)), "snapshot_wait")
+        q = [a.reshape(self.halo_shape, order="F") for a in out]
+        return q[0], q[1], q[2], q[3], t.value
 
     def analysis(self):
         """Device-reduced vortex-core statistics: (vortmax, swrlmax, stat[0:8])."""
