@@ -1109,7 +1109,12 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
   TW *im = re + (size_t)(LB / 2) * N;
   TW *tb = im + (size_t)(LB / 2) * N;  // packed input, LB*N
   TW *twl = tb + (size_t)LB * N;       // N/2 cos, N/2 sin
-  int *jk = reinterpret_cast<int *>(smem_raw + al16(sizeof(TW) * ((size_t)2 * LB * N + N)));
+  const size_t ojk = al16(sizeof(TW) * ((size_t)2 * LB * N + N));
+  int *jk = reinterpret_cast<int *>(smem_raw + ojk);
+  // mixed path: the FP64 low modes of the block's lines and an FP64 cos/sin
+  // table (N each) staged in LDS for the merge
+  double *s64l = reinterpret_cast<double *>(smem_raw + al16(ojk + sizeof(int) * 2 * LB));
+  double *tw64 = s64l + 5 * (size_t)LB;
   const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
   if (tid < LB) {
     const long l = l0 + tid;
@@ -1118,6 +1123,10 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
   }
   if (pow2)
     for (int q = tid; q < N / 2; q += nth) { twl[q] = (TW)twr[q]; twl[N / 2 + q] = (TW)twi[q]; }
+  if (S64) {
+    for (int q = tid; q < N; q += nth) { tw64[q] = twr[q]; tw64[N + q] = twi[q]; }
+    for (int e = tid; e < 5 * LB; e += nth) s64l[e] = S64[5 * min(l0 + e / 5, nlines - 1) + e % 5];
+  }
   #pragma unroll 4
   for (int e = tid; e < LB * N; e += nth) {
     const int lp = pow2 ? e >> logn : e / N, pos = e - lp * N;
@@ -1170,14 +1179,14 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
       }
     }
     if (S64) {
-      const double *t = S64 + 5 * l;
+      const double *t = s64l + 5 * lp;
       v += t[0];
       for (int kk = 1; kk <= 2 && 2 * kk - 1 < N; ++kk) {
         int ph = ii * kk;
         if (ph >= N) ph -= N;
         const double a = t[2 * kk - 1], b = (2 * kk < N) ? t[2 * kk] : 0.0;
-        if (2 * kk == N) v += 0.5 * a * twr[ph];
-        else v += a * twr[ph] + b * twi[ph];
+        if (2 * kk == N) v += 0.5 * a * tw64[ph];
+        else v += a * tw64[ph] + b * tw64[N + ph];
       }
     }
     const int j = jk[2 * lp], k = jk[2 * lp + 1];
@@ -1618,7 +1627,8 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
     hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(256), sh64, h->stream, S64,
                        nl64, nz, lz, TL64, ltl64, -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
   }
-  const size_t sh3 = al16(sizeof(TW) * ((size_t)2 * LB * N + N)) + sizeof(int) * 2 * LB;
+  size_t sh3 = al16(sizeof(TW) * ((size_t)2 * LB * N + N)) + sizeof(int) * 2 * LB;
+  if (S64) sh3 = al16(sh3) + sizeof(double) * (5 * (size_t)LB + 2 * (size_t)N);
   KTimer kti(h, KT_IFFT);
   hipLaunchKernelGGL(k_ifft<TW>, dim3((unsigned)((nlines + LB - 1) / LB)), dim3(256), sh3, h->stream, g, S, S64,
                      phi_out, ext_rhs ? nullptr : h->f.dph, h->d_twt, h->d_twt + N, LB, logn);
