@@ -650,7 +650,7 @@ extern "C" int vrh_autotune(vrh_t *h, int reps, int *choice) {
   }
   {
     int best = 0; double bestt = 1e30;
-    for (int v : {0, 1}) {
+    for (int v : {0, 1, 2}) {
       h->explicit_variant = v;
       double t = h->prec == VRH_MIXED ? time_explicit<float>(h, reps) : time_explicit<double>(h, reps);
       if (t < bestt) { bestt = t; best = v; }
@@ -868,7 +868,7 @@ extern "C" int vrh_set_tuning(vrh_t *h, const int *choice) {
     return fail(h, "fft lines per block must be 2, 4, 8 or 16");
   if (choice[7] != 0 && choice[7] != 2 && choice[7] != 4 && choice[7] != 8 && choice[7] != 16)
     return fail(h, "dct lines per block must be 2, 4, 8 or 16");
-  if (choice[8] < -1 || choice[8] > 1) return fail(h, "explicit form must be -1, 0 or 1");
+  if (choice[8] < -1 || choice[8] > 2) return fail(h, "explicit form must be -1, 0, 1 or 2");
   for (int sw = 0; sw < 6; sw++) if (choice[sw]) h->Msw[sw] = choice[sw];
   if (choice[6]) h->fft_lines = choice[6];
   if (choice[7]) h->dct_lines = choice[7];

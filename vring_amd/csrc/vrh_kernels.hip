@@ -369,6 +369,322 @@ __global__ __launch_bounds__(512, sizeof(TW) == 4 ? 4 : 3) void k_explicit_march
 // Variant with one z plane per block (no z marching): every stencil operand
 // is read through L1/L2.  Fewer registers and LDS than the marching form;
 // the autotuner picks whichever is faster for the precision and grid.
+// The explicit kernel with one wavefront per r row (nθ <= 64 CPL): each
+// lane computes the cells θ = lane + 64 c, and the periodic θ solves run by
+// PCR inside the wavefront (LDS exchange ordered by wave-local fences, no
+// block barrier per level), which leaves two block barriers per z plane for
+// the plane ring.  Same arithmetic as k_explicit_march.
+template <typename TW, int CPL, int RJT, int WPE>
+__global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef c, VRK rk, double betadx, int nlev, int kchunk,
+                                                  const double *__restrict__ q1, const double *__restrict__ q2,
+                                                  const double *__restrict__ q3, const TW *__restrict__ prg1,
+                                                  const TW *__restrict__ prg2, const TW *__restrict__ prg3,
+                                                  TW *__restrict__ ru1, TW *__restrict__ ru2, TW *__restrict__ ru3,
+                                                  TW *__restrict__ r1, TW *__restrict__ r2, TW *__restrict__ r3) {
+  extern __shared__ unsigned char smem_raw[];
+  // one wave per r row; lane holds theta cells lane + 64 c, c < CPL
+  constexpr int NTP = 64 * CPL, RJ = RJT;
+  const int lane = threadIdx.x, tj = threadIdx.y;
+  const int NTH = 64 * RJ, tid = tj * 64 + lane;
+  const int nt = g.nt, nr = g.nr, nz = g.nz;
+  const long sj = g.sj, sk = g.sk;
+  const int j0 = 1 + blockIdx.x * RJ;
+  const int kb = 1 + blockIdx.y * kchunk, ke = min(kb + kchunk - 1, nz);
+  const int LRW = RJ + 2, PL = LRW * NTP;
+  double *pcr = reinterpret_cast<double *>(smem_raw);             // [RJ][kMaxPcrLevels][2] + [RJ][2]
+  double *fin = pcr + RJ * kMaxPcrLevels * 2;
+  TW *cfs = reinterpret_cast<TW *>(fin + RJ * 2);                 // [RJ][kRowCoefs]
+  TW *ring = cfs + RJ * kRowCoefs;                                // [3 slots][3 arrays][LRW][NTP]
+  TW *fb = ring + 9 * PL;                                         // [2][3][RJ][NTP]
+  const int j = j0 + tj;
+  const int jc = (j <= nr) ? j : nr;
+  // PCR ratios of this row (one lane per row)
+  if (lane == 0) {
+    double a1 = -(betadx * c.s1t11[jc]), b1 = 1.0 + 2.0 * (betadx * c.s1t11[jc]);
+    double a2 = -(betadx * c.s2t11[jc]), b2 = 1.0 + 2.0 * (betadx * c.s2t11[jc]);
+    for (int lvl = 0; lvl < nlev; ++lvl) {
+      const double al1 = -a1 / b1, al2 = -a2 / b2;
+      pcr[(tj * kMaxPcrLevels + lvl) * 2] = al1;
+      pcr[(tj * kMaxPcrLevels + lvl) * 2 + 1] = al2;
+      b1 = b1 + 2.0 * al1 * a1; a1 = al1 * a1;
+      b2 = b2 + 2.0 * al2 * a2; a2 = al2 * a2;
+    }
+    const bool pow2 = nt >= 2 && (nt & (nt - 1)) == 0;
+    fin[tj * 2] = 1.0 / (pow2 ? b1 + 2.0 * a1 : b1);
+    fin[tj * 2 + 1] = 1.0 / (pow2 ? b2 + 2.0 * a2 : b2);
+    const int jm = (jc == 1) ? 1 : jc - 1;
+    TW *cf = cfs + tj * kRowCoefs;
+    cf[0] = (TW)c.h1d11[jc]; cf[1] = (TW)c.h1d22d[jc]; cf[2] = (TW)c.h1d22n[jc]; cf[3] = (TW)c.h1dare[jc];
+    cf[4] = (TW)c.h2d11[jc]; cf[5] = (TW)c.h2d22p[jc]; cf[6] = (TW)c.h2d22m[jc]; cf[7] = (TW)c.h2dare[jc];
+    cf[8] = (TW)c.s1t11[jc]; cf[9] = (TW)c.s1t2n[jc]; cf[10] = (TW)c.s1t2s[jc]; cf[11] = (TW)c.s1t2c[jc];
+    cf[12] = (TW)c.s2t11[jc]; cf[13] = (TW)c.s2t2p[jc]; cf[14] = (TW)c.s2t2c[jc]; cf[15] = (TW)c.s2t2m[jc];
+    cf[16] = (TW)c.s3t2n[jc]; cf[17] = (TW)c.s3t2s[jc];
+    cf[18] = (TW)(1.0 / c.rm[jc + 1]); cf[19] = (TW)(1.0 / c.rm[jc]); cf[20] = (TW)(1.0 / c.rm[jm]);
+  }
+  // plane fetch: element e = tid + l*NTH of the (LRW x NTP) tile, three arrays
+  constexpr int NL = (CPL * (RJT + 2) + RJT - 1) / RJT;  // ceil((RJ + 2) NTP / (64 RJ))
+  long goff[NL];
+  int lidx[NL];
+  bool lval[NL];
+#pragma unroll
+  for (int l = 0; l < NL; l++) {
+    const int e = tid + l * NTH;
+    const int r = e / NTP, cc = e - r * NTP;
+    const int jj = j0 - 1 + r;
+    lidx[l] = (e < PL) ? e : -1;
+    lval[l] = (e < PL) && cc < nt && jj <= nr + 2;
+    goff[l] = lval[l] ? (long)cc + (long)jj * sj : 0;
+  }
+  TW pre[3][NL];
+  // unconditional loads (masked after) so that a plane's loads issue together
+  auto fetch = [&](int kk) {
+#pragma unroll
+    for (int l = 0; l < NL; l++) {
+      const long o = goff[l] + (long)kk * sk;
+      const double a1 = q1[o], a2 = q2[o], a3 = q3[o];
+      pre[0][l] = lval[l] ? (TW)a1 : (TW)0;
+      pre[1][l] = lval[l] ? (TW)a2 : (TW)0;
+      pre[2][l] = lval[l] ? (TW)a3 : (TW)0;
+    }
+  };
+  auto stash = [&](int slot) {
+#pragma unroll
+    for (int l = 0; l < NL; l++)
+      if (lidx[l] >= 0) {
+        ring[(slot * 3 + 0) * PL + lidx[l]] = pre[0][l];
+        ring[(slot * 3 + 1) * PL + lidx[l]] = pre[1][l];
+        ring[(slot * 3 + 2) * PL + lidx[l]] = pre[2][l];
+      }
+  };
+  int sM = (kb - 1) % 3, s0 = kb % 3, sP = (kb + 1) % 3;
+  fetch(kb - 1); stash(sM);
+  fetch(kb); stash(s0);
+  fetch(kb + 1);
+  // per-row coefficients
+  const TW ga = (TW)rk.g, ro = (TW)rk.r, al = (TW)rk.a, dt = (TW)g.dt, aRe = (TW)(rk.a / g.Re);
+  const TW dx3 = (TW)g.dx3, dx3q = (TW)g.dx3q;
+  const TW *cf = cfs + tj * kRowCoefs;  // per-row metric coefficients (LDS, broadcast reads)
+#define h1d11 cf[0]
+#define h1d22d cf[1]
+#define h1d22n cf[2]
+#define h1dare cf[3]
+#define h2d11 cf[4]
+#define h2d22p cf[5]
+#define h2d22m cf[6]
+#define h2dare cf[7]
+#define s1t11 cf[8]
+#define s1t2n cf[9]
+#define s1t2s cf[10]
+#define s1t2c cf[11]
+#define s2t11 cf[12]
+#define s2t2p cf[13]
+#define s2t2c cf[14]
+#define s2t2m cf[15]
+#define s3t2n cf[16]
+#define s3t2s cf[17]
+#define irmn cf[18]
+#define irm cf[19]
+#define irmm cf[20]
+  const int rw = tj + 1, rm1 = (jc == 1) ? rw : rw - 1;   // LDS rows of j and jm
+  // ru and prg of the next plane stream in while this one is computed
+  // (clamped in-bounds addresses for lanes and planes that do not use them)
+  TW n_ru[CPL][3], n_pg[CPL][3];
+  auto load_rp = [&](int kk) {
+    const int kc = min(kk, ke);
+#pragma unroll
+    for (int cc = 0; cc < CPL; cc++) {
+      const int tic = min(lane + 64 * cc, nt - 1);
+      const long oo = (long)tic + (long)jc * sj + (long)kc * sk;
+      const long pp = (long)tic + (long)nt * ((long)(jc - 1) + (long)nr * (long)(kc - 1));
+      n_ru[cc][0] = ru1[oo]; n_ru[cc][1] = ru2[oo]; n_ru[cc][2] = ru3[oo];
+      n_pg[cc][0] = prg1[pp]; n_pg[cc][1] = prg2[pp]; n_pg[cc][2] = prg3[pp];
+    }
+  };
+  auto wave_sync = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  };
+  load_rp(kb);
+  for (int k = kb; k <= ke; k++) {
+    stash(sP);
+    TW cru[CPL][3], cpg[CPL][3];
+#pragma unroll
+    for (int cc = 0; cc < CPL; cc++)
+#pragma unroll
+      for (int q = 0; q < 3; q++) { cru[cc][q] = n_ru[cc][q]; cpg[cc][q] = n_pg[cc][q]; }
+    __syncthreads();
+    fetch(k + 2);
+    load_rp(k + 1);
+    TW w1[CPL], w2[CPL], w3[CPL];
+#pragma unroll
+    for (int cc = 0; cc < CPL; cc++) {
+      const int ti = lane + 64 * cc;
+      const int ci = ti, cip = (ti + 1 >= nt) ? 0 : ti + 1, cim = (ti == 0) ? nt - 1 : ti - 1;
+      const bool cell = (ti < nt) && (j <= nr);
+      const TW c_ru1 = cru[cc][0], c_ru2 = cru[cc][1], c_ru3 = cru[cc][2];
+      const TW c_pg1 = cpg[cc][0], c_pg2 = cpg[cc][1], c_pg3 = cpg[cc][2];
+      TW v1 = (TW)0, v2 = (TW)0, v3 = (TW)0;
+      const long o = (long)ti + (long)j * sj + (long)k * sk;
+      if (cell) {
+      const TW *A1m = ring + (sM * 3 + 0) * PL, *A1 = ring + (s0 * 3 + 0) * PL, *A1p = ring + (sP * 3 + 0) * PL;
+      const TW *A2m = ring + (sM * 3 + 1) * PL, *A2 = ring + (s0 * 3 + 1) * PL, *A2p = ring + (sP * 3 + 1) * PL;
+      const TW *A3m = ring + (sM * 3 + 2) * PL, *A3 = ring + (s0 * 3 + 2) * PL, *A3p = ring + (sP * 3 + 2) * PL;
+#define L(A, row, col) A[(row) * NTP + (col)]
+      const TW q1c = L(A1, rw, ci), q1ip = L(A1, rw, cip), q1im = L(A1, rw, cim);
+      const TW q1n = L(A1, rw + 1, ci), q1jm = L(A1, rm1, ci), q1s = L(A1, rw - 1, ci), q1ips = L(A1, rw - 1, cip);
+      const TW q1kp = L(A1p, rw, ci), q1km = L(A1m, rw, ci), q1ipkm = L(A1m, rw, cip);
+      const TW q2c = L(A2, rw, ci), q2n = L(A2, rw + 1, ci), q2imn = L(A2, rw + 1, cim), q2im = L(A2, rw, cim);
+      const TW q2ip = L(A2, rw, cip), q2s = L(A2, rw - 1, ci), q2kp = L(A2p, rw, ci), q2km = L(A2m, rw, ci);
+      const TW q2nkm = L(A2m, rw + 1, ci);
+      const TW q3c = L(A3, rw, ci), q3kp = L(A3p, rw, ci), q3imkp = L(A3p, rw, cim), q3im = L(A3, rw, cim);
+      const TW q3ip = L(A3, rw, cip), q3s = L(A3, rw - 1, ci), q3skp = L(A3p, rw - 1, ci), q3n = L(A3, rw + 1, ci);
+      const TW q3jm = L(A3, rm1, ci), q3km = L(A3m, rw, ci);
+#undef L
+      // ---- q1 at (theta_c, r_m, z_m) ----
+      {
+        TW qnn = q1ip + q1c, qss = q1im + q1c;
+        TW h11 = (qnn * qnn - qss * qss) * h1d11;
+        qnn = (q2n + q2imn) * (q1n + q1c);
+        qss = (q1c + q1jm) * (q2c + q2im);
+        TW h12d = (qnn - qss) * h1d22d;
+        qnn = (q1ip + q1c) * (q2n + q2c);
+        qss = (q1c + q1im) * (q2imn + q2im);
+        TW h12n = (qnn + qss) * h1d22n;
+        qnn = (TW)0.25 * (q3kp + q3imkp) * (q1kp + q1c);
+        qss = (k == 1) ? (TW)0 : (TW)0.25 * (q3c + q3im) * (q1c + q1km);
+        TW h13 = (qnn - qss) * dx3;
+        qnn = q2n + q2c;
+        qss = q2imn + q2im;
+        TW d1q2 = (qnn - qss) * h1dare;
+        TW H1 = d1q2 - (h11 + (h12d + h12n) + h13);
+        TW d11 = (q1ip - (TW)2 * q1c + q1im) * s1t11;
+        TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (q1kp - (TW)3 * q1c) * dx3q : (q1kp - (TW)2 * q1c + q1km) * dx3q;
+        TW qn2 = (q1n * irmn - q1c * irm) * s1t2n;
+        TW qs2 = (q1c * irm - q1jm * irmm) * s1t2s;
+        TW d22 = (qn2 - qs2) * s1t2c;
+        v1 = (ga * H1 + ro * c_ru1 - al * c_pg1 + aRe * (d11 + d33 + d22)) * dt;
+        ru1[o] = H1;
+      }
+      // ---- q2 at (theta_m, r_c, z_m), j >= 2 ----
+      if (j >= 2) {
+        TW qnn = (q1ip + q1ips) * (q2ip + q2c);
+        TW qss = (q1c + q1s) * (q2c + q2im);
+        TW h21 = (qnn - qss) * h2d11;
+        qnn = q2n + q2c;
+        qss = q2c + q2s;
+        TW h22 = qnn * qnn * h2d22p - qss * qss * h2d22m;
+        qnn = (TW)0.5 * (q1c + q1ip);
+        qss = (TW)0.5 * (q1s + q1ips);
+        TW h11n = (TW)0.25 * (qnn + qss) * (qnn + qss);
+        qnn = (TW)0.25 * (q3kp + q3skp) * (q2kp + q2c);
+        qss = (k == 1) ? (TW)0 : (TW)0.25 * (q3c + q3s) * (q2c + q2km);
+        TW h23 = (qnn - qss) * dx3;
+        qnn = q1ip + q1ips;
+        qss = q1c + q1s;
+        TW d1q1 = -((qnn - qss) * h2dare);
+        TW H2 = d1q1 - (h21 + h22 + h23 - h11n);
+        TW d11 = (q2ip - (TW)2 * q2c + q2im) * s2t11;
+        TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (q2kp - (TW)3 * q2c) * dx3q : (q2kp - (TW)2 * q2c + q2km) * dx3q;
+        TW d22 = q2n * s2t2p - q2c * s2t2c + q2s * s2t2m;
+        v2 = (ga * H2 + ro * c_ru2 - al * c_pg2 + aRe * (d11 + d33 + d22)) * dt;
+        ru2[o] = H2;
+      }
+      // ---- q3 at (theta_m, r_m, z_c), k >= 2 ----
+      if (k >= 2) {
+        TW qnn = (q3n + q3c) * (q2n + q2nkm);
+        TW qss = (q3jm + q3c) * (q2c + q2km);
+        TW h32 = (qnn - qss) * h1d22d;
+        qnn = (q3c + q3ip) * (q1ip + q1ipkm);
+        qss = (q3im + q3c) * (q1c + q1km);
+        TW h31 = (qnn - qss) * h1d11;
+        qnn = (TW)0.25 * (q3c + q3kp) * (q3c + q3kp);
+        qss = (TW)0.25 * (q3c + q3km) * (q3c + q3km);
+        TW h33 = (qnn - qss) * dx3;
+        TW H3 = -(h31 + h32 + h33);
+        TW d11 = (q3ip - (TW)2 * q3c + q3im) * s1t11;
+        TW d33 = (q3kp - (TW)2 * q3c + q3km) * dx3q;
+        TW qn2 = (q3n - q3c) * s3t2n;
+        TW qs2 = (q3c - q3jm) * s3t2s;
+        TW d22 = qn2 - qs2;
+        v3 = (ga * H3 + ro * c_ru3 - al * c_pg3 + aRe * (d11 + d33 + d22)) * dt;
+        ru3[o] = H3;
+      }
+          }
+      w1[cc] = v1; w2[cc] = v2; w3[cc] = v3;
+    }
+    // ---- periodic theta solves by PCR within the wavefront ----
+    const long S3 = 3L * RJ * NTP;
+    TW *f1 = fb + tj * NTP, *f2 = f1 + RJ * NTP, *f3 = f2 + RJ * NTP;
+#pragma unroll
+    for (int cc = 0; cc < CPL; cc++) {
+      const int ti = lane + 64 * cc;
+      f1[ti] = w1[cc]; f2[ti] = w2[cc]; f3[ti] = w3[cc];
+    }
+    wave_sync();
+    int sm = 1 % nt;
+    for (int lvl = 0; lvl < nlev; ++lvl) {
+      const TW al1 = (TW)pcr[(tj * kMaxPcrLevels + lvl) * 2], al2 = (TW)pcr[(tj * kMaxPcrLevels + lvl) * 2 + 1];
+      TW n1[CPL], n2[CPL], n3[CPL];
+#pragma unroll
+      for (int cc = 0; cc < CPL; cc++) {
+        const int ti = lane + 64 * cc;
+        n1[cc] = w1[cc]; n2[cc] = w2[cc]; n3[cc] = w3[cc];
+        if (ti < nt) {
+          int imv = ti - sm; if (imv < 0) imv += nt;
+          int ipv = ti + sm; if (ipv >= nt) ipv -= nt;
+          n1[cc] = w1[cc] + al1 * (f1[imv] + f1[ipv]);
+          n2[cc] = w2[cc] + al2 * (f2[imv] + f2[ipv]);
+          n3[cc] = w3[cc] + al1 * (f3[imv] + f3[ipv]);
+        }
+      }
+      sm = (2 * sm) % nt;
+      const long flip = (lvl & 1) ? -S3 : S3;
+      f1 += flip; f2 += flip; f3 += flip;
+#pragma unroll
+      for (int cc = 0; cc < CPL; cc++) {
+        const int ti = lane + 64 * cc;
+        f1[ti] = n1[cc]; f2[ti] = n2[cc]; f3[ti] = n3[cc];
+        w1[cc] = n1[cc]; w2[cc] = n2[cc]; w3[cc] = n3[cc];
+      }
+      wave_sync();
+    }
+    const TW d1 = (TW)fin[tj * 2], d2 = (TW)fin[tj * 2 + 1];
+#pragma unroll
+    for (int cc = 0; cc < CPL; cc++) {
+      const int ti = lane + 64 * cc;
+      if (ti < nt && j <= nr) {
+        const long o = (long)ti + (long)j * sj + (long)k * sk;
+        r1[o] = w1[cc] * d1;
+        if (j >= 2) r2[o] = w2[cc] * d2;
+        if (k >= 2) r3[o] = w3[cc] * d1;
+      }
+    }
+    sM = s0; s0 = sP; sP = (sP == 2) ? 0 : sP + 1;
+    __syncthreads();
+  }
+#undef h1d11
+#undef h1d22d
+#undef h1d22n
+#undef h1dare
+#undef h2d11
+#undef h2d22p
+#undef h2d22m
+#undef h2dare
+#undef s1t11
+#undef s1t2n
+#undef s1t2s
+#undef s1t2c
+#undef s2t11
+#undef s2t2p
+#undef s2t2c
+#undef s2t2m
+#undef s3t2n
+#undef s3t2s
+#undef irmn
+#undef irm
+#undef irmm
+}
+
 template <typename TW>
 __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk, double betadx, int nlev,
                                                    const double *__restrict__ q1, const double *__restrict__ q2,
@@ -1506,8 +1822,33 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
   };
   int rj = 512 / ntp;
   if (rj < 1) rj = 1;
-  while (rj > 1 && lds(rj) > 65536) rj--;
   const int kchunk = 16;
+  if (var == 2) {
+    // one wavefront per r row: the most rows (<= 8) whose LDS fits 64 KB;
+    // the instantiations cover the shapes that fit, others fall back to the
+    // marching form.  WPE: waves per SIMD the LDS allows (register budget).
+    int rw = 8;
+    while (rw > 1 && lds(rw) > 65536) rw--;
+    const int cpl = ntp / 64;
+    dim3 grd((g.nr + rw - 1) / rw, (g.nz + kchunk - 1) / kchunk);
+    bool done = lds(rw) <= 65536;
+#define EXPL_WAVE(C, R, W)                                                                                          \
+  hipLaunchKernelGGL((k_explicit_wave<TW, C, R, W>), grd, dim3(64, R), lds(R), h->stream, g, h->coef, h->rk[irk],   \
+                     betadx, nlev, kchunk, h->f.q1, h->f.q2, h->f.q3, (const TW *)h->f.prg1, (const TW *)h->f.prg2, \
+                     (const TW *)h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,          \
+                     (TW *)h->f.r2, (TW *)h->f.r3)
+    if (done) {
+      if (cpl == 1 && rw == 8) EXPL_WAVE(1, 8, 4);
+      else if (cpl == 1 && rw == 6) EXPL_WAVE(1, 6, 3);
+      else if (cpl == 2 && rw == 6) EXPL_WAVE(2, 6, 3);
+      else if (cpl == 2 && rw == 2) EXPL_WAVE(2, 2, 2);
+      else if (cpl == 4 && rw == 2) EXPL_WAVE(4, 2, 2);
+      else done = false;
+    }
+#undef EXPL_WAVE
+    if (done) return;
+  }
+  while (rj > 1 && lds(rj) > 65536) rj--;
   dim3 blk(ntp, rj), grd((g.nr + rj - 1) / rj, (g.nz + kchunk - 1) / kchunk);
   hipLaunchKernelGGL(k_explicit_march<TW>, grd, blk, lds(rj), h->stream, g, h->coef, h->rk[irk], betadx, nlev,
                      kchunk, h->f.q1, h->f.q2, h->f.q3, (const TW *)h->f.prg1, (const TW *)h->f.prg2,
