@@ -345,7 +345,7 @@ def main():
     ap.add_argument("--particles", type=int, default=1000000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--autotune-reps", type=int, default=3)
-    ap.add_argument("--only", choices=("fp64", "mixed"), default=None,
+    ap.add_argument("--only", choices=("fp64", "mixed", "particles"), default=None,
                     help="run one path only and print its timing (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (counter calibration)")
@@ -378,6 +378,10 @@ def main():
         return
     q, dt = ring_state()
     ncell = NT * NR * NZ
+    if args.only == "particles":
+        print(json.dumps({f"{prec}_{sch}": run_particles(prec, q, dt, args.particles, 5, sch)
+                          for prec in ("fp64", "mixed") for sch in ("lsq_euler", "trilinear_rk3")}))
+        return
     if args.only:
         wall, _, kt, tune = run_path(args.only, q, dt, args.steps, args.warmup, args.autotune_reps)
         print(json.dumps({"path": args.only, "ms_per_step": round(wall / args.steps * 1e3, 4),

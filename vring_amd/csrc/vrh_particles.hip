@@ -46,49 +46,96 @@ __device__ bool interp_lsq(const PartArgs &a, double xp, double yp, double zp, T
   const int ir = (int)floor(r / a.dr) + 1;
   const int iz = (int)floor(zp / a.dz) + 1;
   if (!(r == r) || ir < 1 || ir > g.nr + 1 || iz < 1 || iz > g.nz + 1) return false;
-  T ff[3], dff[3][3], ddff[3][3][3];
   const T sn = (T)sin(theta), cs = (T)cos(theta);
   const T sn2 = (T)sin(2.0 * theta), cs2 = (T)cos(2.0 * theta);
   const T rr = (T)r, r2 = rr * rr;
   const double dx[3] = {a.dtheta, a.dr, a.dz};
-  for (int ivar = 1; ivar <= 3; ivar++) {
+  // rotation to Cartesian velocity, gradient and Hessian (:700-797): the
+  // outputs are linear in each component's (f, grad f, Hessian f), so every
+  // component adds its share as soon as it is fitted, and only the 39
+  // outputs stay live across the three components
+  const T rs = (T)fmax((double)0.000001f, r), rs2 = rs * rs;
+  const T NM[3][3] = {{-sn, cs, (T)0}, {cs, sn, (T)0}, {(T)0, (T)0, (T)1}};
+  T DN[3][3][3];  // DN[j][i][k] = d NM(i,k) / d x_j
+#pragma unroll
+  for (int q = 0; q < 27; q++) (&DN[0][0][0])[q] = (T)0;
+  DN[0][0][0] = cs * sn / rs; DN[0][0][1] = sn * sn / rs;
+  DN[0][1][0] = sn * sn / rs; DN[0][1][1] = -cs * sn / rs;
+  DN[1][0][0] = -cs * cs / rs; DN[1][0][1] = -sn * cs / rs;
+  DN[1][1][0] = -sn * cs / rs; DN[1][1][1] = cs * cs / rs;
+  const T Aa = sn * (-(T)2 * (cs * cs) + sn * sn) / rs2;
+  const T Bb = cs * ((cs * cs) - (T)2 * (sn * sn)) / rs2;
+  const T Cc = -(T)3 * cs * (sn * sn) / rs2;
+  const T Dd = (T)3 * (cs * cs) * sn / rs2;
+  // DDN[l][j][i][k] = d2 NM(i,k) / dx_l dx_j; only i,k in {0,1} and l,j in {0,1}
+  auto DDN = [&](int l, int j, int i, int k) -> T {
+    if (l > 1 || j > 1 || i > 1 || k > 1) return (T)0;
+    const int key = l * 8 + j * 4 + i * 2 + k;
+    switch (key) {
+      case 0: return Aa; case 1: return Cc; case 2: return Cc; case 3: return -Aa;
+      case 4: return Bb; case 5: return -Aa; case 6: return -Aa; case 7: return -Bb;
+      case 8: return Bb; case 9: return -Aa; case 10: return -Aa; case 11: return -Bb;
+      case 12: return Dd; case 13: return -Bb; case 14: return -Bb; case 15: return -Dd;
+    }
+    return (T)0;
+  };
+#pragma unroll
+  for (int q = 0; q < 3; q++) vel[q] = (T)0;
+#pragma unroll
+  for (int q = 0; q < 9; q++) dvel[q] = (T)0;
+#pragma unroll
+  for (int q = 0; q < 27; q++) ddvel[q] = (T)0;
+#pragma unroll 1
+  for (int kd = 0; kd < 3; kd++) {
+    const int ivar = kd + 1;
     T y[27];
-    for (int c = -1; c <= 1; c++)
-      for (int b = -1; b <= 1; b++)
-        for (int aa = -1; aa <= 1; aa++) {
-          const int iv = (aa + 1) + 3 * (b + 1) + 9 * (c + 1);
-          int itof = it + aa;
-          if (itof > g.nt) itof = 1;
-          if (itof < 1) itof = g.nt;
-          const int irof = ir + b, izof = iz + c;
-          const long o = hidx(g, itof, irof, izof);
-          double v;
-          if (ivar == 1) v = a.q1[o];
-          else if (ivar == 2) {
-            const double rc = (irof - 1.0) * a.dr;
-            v = (rc > (double)0.0001f) ? a.q2[o] / rc : 0.0;
-          } else v = a.q3[o];
-          y[iv] = (T)v;
-        }
+#pragma unroll
+    for (int iv = 0; iv < 27; iv++) {
+      const int aa = iv % 3 - 1, b = (iv / 3) % 3 - 1, c = iv / 9 - 1;
+      int itof = it + aa;
+      if (itof > g.nt) itof = 1;
+      if (itof < 1) itof = g.nt;
+      const int irof = ir + b, izof = iz + c;
+      const long o = hidx(g, itof, irof, izof);
+      double v;
+      if (ivar == 1) v = a.q1[o];
+      else if (ivar == 2) {
+        const double rc = (irof - 1.0) * a.dr;
+        v = (rc > (double)0.0001f) ? a.q2[o] / rc : 0.0;
+      } else v = a.q3[o];
+      y[iv] = (T)v;
+    }
     const T y0 = y[13];
+#pragma unroll
     for (int iv = 0; iv < 27; iv++) y[iv] = y[iv] - y0;
     // least-squares fit, iterated as part_tools.F90:609-624 (converges in
     // one pass since mint is the exact pseudo-inverse; the second pass adds
     // the rounding-level correction)
     T acoef[9];
+#pragma unroll
     for (int q = 0; q < 9; q++) acoef[q] = (T)0;
+#pragma unroll 1
     for (int iint = 1; iint <= 100; iint++) {
+      // the 243 entries of mint are read where they are used: the empty asm
+      // hides the pointer from loop-invariant hoisting, which would keep
+      // them all in registers across the loops
+      const double *mint = a.mint;
+      asm volatile("" : "+s"(mint));
       T mp[9];
       T nrm = (T)0;
+#pragma unroll
       for (int q = 0; q < 9; q++) mp[q] = (T)0;
+#pragma unroll
       for (int kk = 0; kk < 27; kk++) {
         const int a1 = kk % 3 - 1, b1 = (kk / 3) % 3 - 1, c1 = kk / 9 - 1;
         const T x0 = (T)(a1 * dx[0]), x1 = (T)(b1 * dx[1]), x2 = (T)(c1 * dx[2]);
         const T rint = x0 * acoef[0] + x1 * acoef[1] + x2 * acoef[2] + x0 * x0 * acoef[3] + x0 * x1 * acoef[4] +
                        x1 * x1 * acoef[5] + x1 * x2 * acoef[6] + x2 * x2 * acoef[7] + x2 * x0 * acoef[8];
         const T res = rint - y[kk];
-        for (int q = 0; q < 9; q++) mp[q] = mp[q] - (T)a.mint[q * 27 + kk] * res;
+#pragma unroll
+        for (int q = 0; q < 9; q++) mp[q] = mp[q] - (T)mint[q * 27 + kk] * res;
       }
+#pragma unroll
       for (int q = 0; q < 9; q++) { acoef[q] = acoef[q] + mp[q]; nrm += mp[q] * mp[q]; }
       if (sqrt((double)nrm) < (double)0.000001f) break;
       // mint is the exact pseudo-inverse, so the second pass only adds the
@@ -116,74 +163,50 @@ __device__ bool interp_lsq(const PartArgs &a, double xp, double yp, double zp, T
     FI[5] = z_ * A[8];   // ftz as part_tools.F90:658
     FI[6] = A[4]; FI[7] = (T)2 * A[5]; FI[8] = A[6];
     FI[9] = A[8]; FI[10] = A[6]; FI[11] = (T)2 * A[7];
-    ff[ivar - 1] = f;
     // chain rule to Cartesian derivatives (:671-698)
-    dff[ivar - 1][0] = -sn / rr * FI[0] + cs * FI[1];
-    dff[ivar - 1][1] = cs / rr * FI[0] + sn * FI[1];
-    dff[ivar - 1][2] = FI[2];
+    T df[3], ddf[3][3];
+    df[0] = -sn / rr * FI[0] + cs * FI[1];
+    df[1] = cs / rr * FI[0] + sn * FI[1];
+    df[2] = FI[2];
     // second derivatives (id1,id2) = rows 4..12
-    const T xx = sn2 / r2 * FI[0] + sn * sn / rr * FI[1] + sn * sn / r2 * FI[3] - cs * sn / rr * FI[4] -
-                 cs * sn / rr * FI[6] + cs * cs * FI[7];
-    const T yx = -cs2 / r2 * FI[0] - cs * sn / rr * FI[1] - cs * sn / r2 * FI[3] - sn * sn / rr * FI[4] +
-                 cs * cs / rr * FI[6] + cs * sn * FI[7];
-    const T zx = -sn / rr * FI[5] + cs * FI[8];
-    const T xy = -cs2 / r2 * FI[0] - cs * sn / rr * FI[1] - cs * sn / r2 * FI[3] + cs * cs / rr * FI[4] -
-                 sn * sn / rr * FI[6] + cs * sn * FI[7];
-    const T yy = -sn2 / r2 * FI[0] + cs * cs / rr * FI[1] + cs * cs / r2 * FI[3] + cs * sn / rr * FI[4] +
-                 cs * sn / rr * FI[6] + sn * sn * FI[7];
-    const T zy = cs / rr * FI[5] + sn * FI[8];
-    const T xz = -sn / rr * FI[9] + cs * FI[10];
-    const T yz = cs / rr * FI[9] + sn * FI[10];
-    const T zz = FI[11];
-    ddff[ivar - 1][0][0] = xx; ddff[ivar - 1][1][0] = yx; ddff[ivar - 1][2][0] = zx;
-    ddff[ivar - 1][0][1] = xy; ddff[ivar - 1][1][1] = yy; ddff[ivar - 1][2][1] = zy;
-    ddff[ivar - 1][0][2] = xz; ddff[ivar - 1][1][2] = yz; ddff[ivar - 1][2][2] = zz;
-  }
-  // rotation to Cartesian velocity, gradient and Hessian (:700-797)
-  const T rs = (T)fmax((double)0.000001f, r), rs2 = rs * rs;
-  T NM[3][3] = {{-sn, cs, (T)0}, {cs, sn, (T)0}, {(T)0, (T)0, (T)1}};
-  T DN[3][3][3];  // DN[j][i][k] = d NM(i,k) / d x_j
-  for (int q = 0; q < 27; q++) (&DN[0][0][0])[q] = (T)0;
-  DN[0][0][0] = cs * sn / rs; DN[0][0][1] = sn * sn / rs;
-  DN[0][1][0] = sn * sn / rs; DN[0][1][1] = -cs * sn / rs;
-  DN[1][0][0] = -cs * cs / rs; DN[1][0][1] = -sn * cs / rs;
-  DN[1][1][0] = -sn * cs / rs; DN[1][1][1] = cs * cs / rs;
-  const T Aa = sn * (-(T)2 * (cs * cs) + sn * sn) / rs2;
-  const T Bb = cs * ((cs * cs) - (T)2 * (sn * sn)) / rs2;
-  const T Cc = -(T)3 * cs * (sn * sn) / rs2;
-  const T Dd = (T)3 * (cs * cs) * sn / rs2;
-  // DDN[l][j][i][k] = d2 NM(i,k) / dx_l dx_j; only i,k in {0,1} and l,j in {0,1}
-  auto DDN = [&](int l, int j, int i, int k) -> T {
-    if (l > 1 || j > 1 || i > 1 || k > 1) return (T)0;
-    const int key = l * 8 + j * 4 + i * 2 + k;
-    switch (key) {
-      case 0: return Aa; case 1: return Cc; case 2: return Cc; case 3: return -Aa;
-      case 4: return Bb; case 5: return -Aa; case 6: return -Aa; case 7: return -Bb;
-      case 8: return Bb; case 9: return -Aa; case 10: return -Aa; case 11: return -Bb;
-      case 12: return Dd; case 13: return -Bb; case 14: return -Bb; case 15: return -Dd;
-    }
-    return (T)0;
-  };
-  for (int id = 0; id < 3; id++) {
-    T s = (T)0;
-    for (int jd = 0; jd < 3; jd++) s = s + ff[jd] * NM[id][jd];
-    vel[id] = s;
-  }
-  for (int id = 0; id < 3; id++)
-    for (int jd = 0; jd < 3; jd++) {
-      T s = (T)0;
-      for (int kd = 0; kd < 3; kd++) s = s + dff[kd][jd] * NM[id][kd] + ff[kd] * DN[jd][id][kd];
-      dvel[id + 3 * jd] = s;
-    }
-  for (int id = 0; id < 3; id++)
+    ddf[0][0] = sn2 / r2 * FI[0] + sn * sn / rr * FI[1] + sn * sn / r2 * FI[3] - cs * sn / rr * FI[4] -
+                cs * sn / rr * FI[6] + cs * cs * FI[7];
+    ddf[1][0] = -cs2 / r2 * FI[0] - cs * sn / rr * FI[1] - cs * sn / r2 * FI[3] - sn * sn / rr * FI[4] +
+                cs * cs / rr * FI[6] + cs * sn * FI[7];
+    ddf[2][0] = -sn / rr * FI[5] + cs * FI[8];
+    ddf[0][1] = -cs2 / r2 * FI[0] - cs * sn / rr * FI[1] - cs * sn / r2 * FI[3] + cs * cs / rr * FI[4] -
+                sn * sn / rr * FI[6] + cs * sn * FI[7];
+    ddf[1][1] = -sn2 / r2 * FI[0] + cs * cs / rr * FI[1] + cs * cs / r2 * FI[3] + cs * sn / rr * FI[4] +
+                cs * sn / rr * FI[6] + sn * sn * FI[7];
+    ddf[2][1] = cs / rr * FI[5] + sn * FI[8];
+    ddf[0][2] = -sn / rr * FI[9] + cs * FI[10];
+    ddf[1][2] = cs / rr * FI[9] + sn * FI[10];
+    ddf[2][2] = FI[11];
+    // this component's share of the rotated outputs, added in the order of
+    // the sums over components (kd) of the reference formulas
+    T nm[3], dn[3][3];
+#pragma unroll
+    for (int id = 0; id < 3; id++) nm[id] = kd == 0 ? NM[id][0] : (kd == 1 ? NM[id][1] : NM[id][2]);
+#pragma unroll
     for (int jd = 0; jd < 3; jd++)
-      for (int ld = 0; ld < 3; ld++) {
-        T s = (T)0;
-        for (int kd = 0; kd < 3; kd++)
-          s = s + ddff[kd][jd][ld] * NM[id][kd] + dff[kd][jd] * DN[ld][id][kd] + dff[kd][ld] * DN[jd][id][kd] +
-              ff[kd] * DDN(ld, jd, id, kd);
-        ddvel[id + 3 * jd + 9 * ld] = s;
-      }
+#pragma unroll
+      for (int id = 0; id < 3; id++)
+        dn[jd][id] = kd == 0 ? DN[jd][id][0] : (kd == 1 ? DN[jd][id][1] : DN[jd][id][2]);
+#pragma unroll
+    for (int id = 0; id < 3; id++) vel[id] = vel[id] + f * nm[id];
+#pragma unroll
+    for (int id = 0; id < 3; id++)
+#pragma unroll
+      for (int jd = 0; jd < 3; jd++) dvel[id + 3 * jd] = dvel[id + 3 * jd] + df[jd] * nm[id] + f * dn[jd][id];
+#pragma unroll
+    for (int id = 0; id < 3; id++)
+#pragma unroll
+      for (int jd = 0; jd < 3; jd++)
+#pragma unroll
+        for (int ld = 0; ld < 3; ld++)
+          ddvel[id + 3 * jd + 9 * ld] = ddvel[id + 3 * jd + 9 * ld] + ddf[jd][ld] * nm[id] + df[jd] * dn[ld][id] +
+                                        df[ld] * dn[jd][id] + f * DDN(ld, jd, id, kd);
+  }
   return true;
 }
 
