@@ -196,6 +196,19 @@ def run_learned(nr, nz, nslots, train_samples, seed=0, pool=8):
         out[m] = {"iterations_total": int(it.sum()), "iterations_max": int(it.max()), "ms": round(ms, 3),
                   "max_true_relres": float(rel.max()),
                   "max_err_vs_direct": float(np.abs(p - pd).max() / np.abs(pd).max())}
+    # loose tolerances (inner solves that need only a few digits)
+    for tol in (1e-2, 1e-4):
+        row = {}
+        for m in ("pcg", "learned", "jacobi_warm_start"):
+            w = None
+            if m == "jacobi_warm_start":
+                w = hz.net_get()
+                hz.net_set(np.zeros_like(w))
+            p, it, rel, ms = hz.solve(f, "learned" if w is not None else m, tol=tol, maxit=50000)
+            if w is not None:
+                hz.net_set(w)
+            row[m] = {"iterations_total": int(it.sum()), "ms": round(ms, 3), "max_true_relres": float(rel.max())}
+        out[f"tol_{tol:g}"] = row
     hz.close()
     return out
 

@@ -1236,7 +1236,7 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
 // stride nl).  Makhoul's reordering turns each pair of real lines into one
 // complex FFT of length nz.  dir=+1 forward, -1 inverse (REDFT01).
 template <typename TW>
-__global__ __launch_bounds__(256) void k_dctz(TW *__restrict__ S, long nl, int n, int logn, int TL, int logtl, int dir,
+__global__ __launch_bounds__(512) void k_dctz(TW *__restrict__ S, long nl, int n, int logn, int TL, int logtl, int dir,
                                               const double *__restrict__ twr, const double *__restrict__ twi,
                                               const double *__restrict__ er, const double *__restrict__ ei) {
   extern __shared__ unsigned char smem_raw[];
@@ -1938,34 +1938,38 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   // z transforms
   const int nz = g.nz, lz = ilog2(nz);
   // lines per block: a power of two, data + twiddles within 64 KB of LDS
+  // lines per block: a power of two, data + twiddles within 64 KB of LDS,
+  // one FFT butterfly per thread with up to 512 threads
   auto lines = [&](size_t es, int want) {
     int tl = 2;
     while (tl * 2 <= 16 && (size_t)(tl * 2) * nz * es + (size_t)nz * es <= 65536 &&
-           (tl * 2 / 2) * nz / fft_min_radix(lz) <= 256)
+           (tl * 2 / 2) * nz / fft_min_radix(lz) <= 512)
       tl *= 2;
     if (want > 0) while (tl > 2 && tl > want) tl /= 2;
     return tl;
   };
+  auto threads = [&](int tl) { return (tl / 2) * nz / fft_min_radix(lz) > 256 ? 512 : 256; };
   const int TL = lines(sizeof(TW), h->dct_lines), TL64 = lines(sizeof(double), 0);
   const int ltl = ilog2(TL), ltl64 = ilog2(TL64);
+  const int NTD = threads(TL), NTD64 = threads(TL64);
   const long nl = (long)N * g.nr;
   const size_t sh2 = sizeof(TW) * ((size_t)TL * nz + nz), sh64 = sizeof(double) * ((size_t)TL64 * nz + nz);
   { KTimer kt(h, KT_DCT_FWD);
-  hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(256), sh2, h->stream, S, nl, nz, lz, TL,
+  hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(NTD), sh2, h->stream, S, nl, nz, lz, TL,
                      ltl, 1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
   if (S64) {
     const long nl64 = 5L * g.nr;
-    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(256), sh64, h->stream, S64,
+    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(NTD64), sh64, h->stream, S64,
                        nl64, nz, lz, TL64, ltl64, 1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
   }
   radial_launch<TW>(h, S, (const TW *)h->f.ib, N);
   if (S64) radial_launch<double>(h, S64, h->f.ib64, 5);
   { KTimer kt(h, KT_DCT_INV);
-  hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(256), sh2, h->stream, S, nl, nz, lz, TL,
+  hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(NTD), sh2, h->stream, S, nl, nz, lz, TL,
                      ltl, -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
   if (S64) {
     const long nl64 = 5L * g.nr;
-    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(256), sh64, h->stream, S64,
+    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(NTD64), sh64, h->stream, S64,
                        nl64, nz, lz, TL64, ltl64, -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
   }
   size_t sh3 = al16(sizeof(TW) * ((size_t)2 * LB * N + N)) + sizeof(int) * 2 * LB;
