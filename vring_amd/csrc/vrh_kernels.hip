@@ -1025,8 +1025,8 @@ template <typename T> constexpr int fft_qt() { return 1; }
 // every butterfly reads its R inputs at stride n/R, applies the twiddles,
 // and after the barrier writes the in-register DFT at stride 2^ls.
 template <typename T, int LR, int QT>
-__device__ __forceinline__ void fft_pass(T *re, T *im, int nfft, int logn, int ls, const T *twr, const T *twi, T sign,
-                                         int tid, int nth) {
+__device__ __forceinline__ void fft_pass(T *re, T *im, int nfft, int logn, int sstr, int ls, const T *twr,
+                                         const T *twi, T sign, int tid, int nth) {
   constexpr int R = 1 << LR;
   const int nh = 1 << (logn - 1), lg = logn - LR;
   const int tasks = nfft << lg;
@@ -1036,7 +1036,7 @@ __device__ __forceinline__ void fft_pass(T *re, T *im, int nfft, int logn, int l
     const int t = tid + q * nth;
     if (t < tasks) {
       const int f = t >> lg, j = t & ((1 << lg) - 1), k = j & ((1 << ls) - 1);
-      const T *br = re + (f << logn), *bi = im + (f << logn);
+      const T *br = re + f * sstr, *bi = im + f * sstr;
 #pragma unroll
       for (int r = 0; r < R; r++) {
         T x = br[j + (r << lg)], y = bi[j + (r << lg)];
@@ -1061,7 +1061,7 @@ __device__ __forceinline__ void fft_pass(T *re, T *im, int nfft, int logn, int l
       else if (LR == 2) dft4(vr[q], vi[q], 1, sign);
       else dft2(vr[q][0], vi[q][0], vr[q][1], vi[q][1]);
       const int d = ((j >> ls) << (ls + LR)) + k;
-      T *br = re + (f << logn), *bi = im + (f << logn);
+      T *br = re + f * sstr, *bi = im + f * sstr;
 #pragma unroll
       for (int r = 0; r < R; r++) { br[d + (r << ls)] = vr[q][r]; bi[d + (r << ls)] = vi[q][r]; }
     }
@@ -1070,21 +1070,22 @@ __device__ __forceinline__ void fft_pass(T *re, T *im, int nfft, int logn, int l
 }
 
 // In-place Stockham autosort FFT of nfft sequences of length n = 2^logn in
-// LDS (re, im), natural order in and out, radix-8/4/2 passes with the DFT in
+// LDS (re, im; sequence f starts at f * sstr), natural order in and out,
+// radix-8/4/2 passes with the DFT in
 // registers.  Twiddles w_p = (twr[p], sign twi[p]), p < n/2, from LDS (the
 // upper half by symmetry).  Each thread holds at most fft_qt<T>() butterflies
 // of a pass across the barrier that separates the reads from the writes; the
 // caller keeps nfft n / (min radix) <= fft_qt<T>() nth.
 template <typename T>
-__device__ __forceinline__ void lds_fft(T *re, T *im, int nfft, int logn, const T *twr, const T *twi, T sign, int tid,
-                                        int nth) {
+__device__ __forceinline__ void lds_fft(T *re, T *im, int nfft, int logn, int sstr, const T *twr, const T *twi, T sign,
+                                        int tid, int nth) {
   constexpr int QT = fft_qt<T>();
   int ls = 0;
   while (ls < logn) {
     const int lr = fft_pass_bits(logn - ls);
-    if (lr == 3) fft_pass<T, 3, QT>(re, im, nfft, logn, ls, twr, twi, sign, tid, nth);
-    else if (lr == 2) fft_pass<T, 2, QT>(re, im, nfft, logn, ls, twr, twi, sign, tid, nth);
-    else fft_pass<T, 1, QT>(re, im, nfft, logn, ls, twr, twi, sign, tid, nth);
+    if (lr == 3) fft_pass<T, 3, QT>(re, im, nfft, logn, sstr, ls, twr, twi, sign, tid, nth);
+    else if (lr == 2) fft_pass<T, 2, QT>(re, im, nfft, logn, sstr, ls, twr, twi, sign, tid, nth);
+    else fft_pass<T, 1, QT>(re, im, nfft, logn, sstr, ls, twr, twi, sign, tid, nth);
     ls += lr;
   }
 }
@@ -1166,7 +1167,7 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
     }
   }
   __syncthreads();
-  if (pow2) lds_fft(re, im, LB / 2, logn, twl, twl + N / 2, (TW)-1, tid, nth);
+  if (pow2) lds_fft(re, im, LB / 2, logn, N, twl, twl + N / 2, (TW)-1, tid, nth);
   const int nhalf = N / 2 + 1;
   for (int e = tid; e < LB * N; e += nth) {
     const int lp = pow2 ? e >> logn : e / N, pos = e - lp * N;
@@ -1241,8 +1242,12 @@ __global__ __launch_bounds__(512) void k_dctz(TW *__restrict__ S, long nl, int n
                                               const double *__restrict__ er, const double *__restrict__ ei) {
   extern __shared__ unsigned char smem_raw[];
   TW *re = reinterpret_cast<TW *>(smem_raw);
-  TW *im = re + (size_t)(TL / 2) * n;
-  TW *twl = im + (size_t)(TL / 2) * n;  // n/2 cos, n/2 sin
+  // sequences at stride n + 1 in LDS: the line-to-sequence transposes
+  // below touch consecutive sequences from consecutive lanes, which would
+  // all hit one bank at stride n
+  const int sn = n + 1;
+  TW *im = re + (size_t)(TL / 2) * sn;
+  TW *twl = im + (size_t)(TL / 2) * sn;  // n/2 cos, n/2 sin
   const int tid = threadIdx.x, nth = blockDim.x;
   const long l0 = xcd_tile(blockIdx.x, gridDim.x) * TL;
   const bool pow2 = n >= 2 && (n & (n - 1)) == 0;
@@ -1259,7 +1264,7 @@ __global__ __launch_bounds__(512) void k_dctz(TW *__restrict__ S, long nl, int n
         TW v = S[min(l, nl - 1) + nl * kz];
         v = (l < nl) ? v : (TW)0;
         const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
-        const int slot = q * n + p;
+        const int slot = q * sn + p;
         if (lp & 1) im[slot] = v; else re[slot] = v;
       }
     } else {
@@ -1278,13 +1283,13 @@ __global__ __launch_bounds__(512) void k_dctz(TW *__restrict__ S, long nl, int n
         ybn = (k > 0 && lb < nl) ? ybn : (TW)0;
         const TW u = ya + ybn, w = yb - yan;
         const TW cr = (TW)er[k], ci = (TW)ei[k];  // e^{+i pi k / 2n}
-        const int slot = q * n + k;
+        const int slot = q * sn + k;
         re[slot] = u * cr - w * ci;
         im[slot] = u * ci + w * cr;
       }
     }
     __syncthreads();
-    lds_fft(re, im, TL / 2, logn, twl, twl + n / 2, (TW)(dir > 0 ? -1 : 1), tid, nth);
+    lds_fft(re, im, TL / 2, logn, sn, twl, twl + n / 2, (TW)(dir > 0 ? -1 : 1), tid, nth);
     for (int e = tid; e < TL * n; e += nth) {
       const int lp = e & (TL - 1), kz = e >> logtl;
       const long l = l0 + lp;
@@ -1293,14 +1298,14 @@ __global__ __launch_bounds__(512) void k_dctz(TW *__restrict__ S, long nl, int n
       TW outv;
       if (dir > 0) {
         const int k = kz, kr = (n - k) & (n - 1);
-        const TW zr = re[q * n + k], zi = im[q * n + k], wr = re[q * n + kr], wi = im[q * n + kr];
+        const TW zr = re[q * sn + k], zi = im[q * sn + k], wr = re[q * sn + kr], wi = im[q * sn + kr];
         TW vr, vi;
         if ((lp & 1) == 0) { vr = (TW)0.5 * (zr + wr); vi = (TW)0.5 * (zi - wi); }
         else { vr = (TW)0.5 * (zi + wi); vi = (TW)(-0.5) * (zr - wr); }
         outv = (vr * (TW)er[k] + vi * (TW)ei[k]) / (TW)n;  // Re(e^{-i pi k/2n} V)/n
       } else {
         const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
-        outv = (lp & 1) ? im[q * n + p] : re[q * n + p];
+        outv = (lp & 1) ? im[q * sn + p] : re[q * sn + p];
       }
       S[l + nl * kz] = outv;
     }
@@ -1474,7 +1479,7 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
       im[slot] = (TW)(ai + br);
     }
     __syncthreads();
-    lds_fft(re, im, LB / 2, logn, twl, twl + N / 2, (TW)1, tid, nth);
+    lds_fft(re, im, LB / 2, logn, N, twl, twl + N / 2, (TW)1, tid, nth);
   }
   for (int e = tid; e < LB * N; e += nth) {
     const int lp = pow2 ? e >> logn : e / N, ii = e - lp * N;
@@ -1942,7 +1947,7 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   // one FFT butterfly per thread with up to 512 threads
   auto lines = [&](size_t es, int want) {
     int tl = 2;
-    while (tl * 2 <= 16 && (size_t)(tl * 2) * nz * es + (size_t)nz * es <= 65536 &&
+    while (tl * 2 <= 16 && (size_t)(tl * 2) * (nz + 1) * es + (size_t)nz * es <= 65536 &&
            (tl * 2 / 2) * nz / fft_min_radix(lz) <= 512)
       tl *= 2;
     if (want > 0) while (tl > 2 && tl > want) tl /= 2;
@@ -1953,7 +1958,7 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   const int ltl = ilog2(TL), ltl64 = ilog2(TL64);
   const int NTD = threads(TL), NTD64 = threads(TL64);
   const long nl = (long)N * g.nr;
-  const size_t sh2 = sizeof(TW) * ((size_t)TL * nz + nz), sh64 = sizeof(double) * ((size_t)TL64 * nz + nz);
+  const size_t sh2 = sizeof(TW) * ((size_t)TL * (nz + 1) + nz), sh64 = sizeof(double) * ((size_t)TL64 * (nz + 1) + nz);
   { KTimer kt(h, KT_DCT_FWD);
   hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(NTD), sh2, h->stream, S, nl, nz, lz, TL,
                      ltl, 1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
