@@ -306,6 +306,7 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
   host_mint(h, mint.data());
   if ((rc = dalloc(h, &h->d_mint, mint.size()))) return rc;
   HIPCHK(h, hipMemcpy(h->d_mint, mint.data(), mint.size() * sizeof(double), hipMemcpyHostToDevice));
+  for (size_t q = 0; q < mint.size() && q < 243; q++) h->h_mint[q] = mint[q];
   if ((rc = dalloc(h, &h->d_sc, 16))) return rc;
   HIPCHK(h, hipHostMalloc((void **)&h->h_pin, 16 * sizeof(double), hipHostMallocDefault));
   h->inlet_area = 0.0;  // momentum_terms_swirl.F90:261-267 with c3s factored out
@@ -334,6 +335,7 @@ int vrh_destroy(vrh_t *h) {
   if (h->copy_stream) hipStreamDestroy(h->copy_stream);
   for (auto &b : h->pb) {
     if (b.u) hipFree(b.u);
+    if (b.fit) hipFree(b.fit);
     if (b.ddt2) hipFree(b.ddt2);
     if (b.alive) hipFree(b.alive);
     void *extra[] = {b.d_n, b.inflow, b.u_tmp, b.ddt2_tmp, b.alive_tmp, b.work};

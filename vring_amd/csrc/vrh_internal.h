@@ -65,6 +65,7 @@ struct PartBatch {
   int nsub;
   double *u;     // nvar rows of cap (structure of arrays)
   double *ddt2;  // nvar rows of cap, previous rate (RK3 schemes)
+  double *fit = nullptr;  // LSQ schemes: 39 rows of cap, per-component fits of a substep
   int *alive;
   long *d_n;     // used slots [0, n) on the device (kernels read it)
   // inlet injection and compaction (allocated on first use)
@@ -124,6 +125,7 @@ struct vrh_s {
   int pscheme, nbatch;
   std::vector<PartBatch> pb;
   double *d_mint;  // 9 x 27 least-squares pseudo-inverse
+  double h_mint[243];  // host copy (passed by value to the fit kernels)
   // kernel timer
   bool kt_on = false;
   std::vector<hipEvent_t> kt_pool;

@@ -21,6 +21,7 @@ struct PartArgs {
   const double *q1, *q2, *q3;
   const double *mint;  // 9 x 27
   double *u, *ddt2;
+  void *fit;          // LSQ fit rows (39 x cap of the work type)
   int *alive;
   const long *n_dev;  // used slots, read on the device
   long cap, n;
@@ -30,30 +31,142 @@ struct PartArgs {
 
 template <typename T> struct V3 { T x[3]; };
 
-// Interpolation and chain rule of one particle, part_tools.F90:519-797.
-// Returns false if the 27-point stencil would leave the haloed arrays.
-template <typename T>
-__device__ bool interp_lsq(const PartArgs &a, double xp, double yp, double zp, T vel[3], T dvel[9], T ddvel[27]) {
+// Interpolation and chain rule of one particle, part_tools.F90:519-797, in
+// two kernels per substep: the fit kernel gathers the 27-point stencil of
+// each velocity component, fits the quadratic and stores the component's
+// value, Cartesian gradient and Hessian (13 numbers, 39 per particle); the
+// step kernel rotates them to the Cartesian velocity, gradient and Hessian
+// (:700-797) and updates the particle state.  Splitting keeps each kernel's
+// live registers small enough for several waves per SIMD.
+constexpr int kFitRows = 39;
+
+// the pseudo-inverse by value: kernel arguments sit in the constant segment,
+// so its entries are uniform scalar loads the compiler re-reads at will
+// instead of 243 vector registers
+template <typename T> struct MintArg { T m[243]; };
+
+// cell of a particle and its azimuth (part_tools.F90:519-538); false if
+// the 27-point stencil would leave the haloed arrays
+__device__ inline bool lsq_locate(const PartArgs &a, double xp, double yp, double zp, double &theta, double &r,
+                                  int &it, int &ir, int &iz) {
   const VGrid &g = a.g;
-  const double r = sqrt(xp * xp + yp * yp);
+  r = sqrt(xp * xp + yp * yp);
   const double pi = (double)4.0f * atan(1.0);
-  double theta;
   if (xp >= 0.0 && yp >= 0.0) theta = acos(xp / r);           // part_tools.F90:524-534
   else if (xp < 0.0 && yp >= 0.0) theta = acos(xp / r);
   else theta = acos(-xp / r) + pi;
-  int it = (int)floor(theta / a.dtheta) + 1;
+  it = (int)floor(theta / a.dtheta) + 1;
   if (g.nt == 1) it = 1;
-  const int ir = (int)floor(r / a.dr) + 1;
-  const int iz = (int)floor(zp / a.dz) + 1;
-  if (!(r == r) || ir < 1 || ir > g.nr + 1 || iz < 1 || iz > g.nz + 1) return false;
+  ir = (int)floor(r / a.dr) + 1;
+  iz = (int)floor(zp / a.dz) + 1;
+  return (r == r) && ir >= 1 && ir <= g.nr + 1 && iz >= 1 && iz <= g.nz + 1;
+}
+
+// fit of component ivar (1..3) and its value, Cartesian gradient and
+// Hessian at the particle (part_tools.F90:540-698)
+template <typename T>
+__device__ inline void lsq_component(const PartArgs &a, const MintArg<T> &mt, int ivar, double theta, double r,
+                                     double zp, int it, int ir, int iz, T &fv, T df[3], T ddf[3][3]) {
+  const VGrid &g = a.g;
   const T sn = (T)sin(theta), cs = (T)cos(theta);
   const T sn2 = (T)sin(2.0 * theta), cs2 = (T)cos(2.0 * theta);
   const T rr = (T)r, r2 = rr * rr;
   const double dx[3] = {a.dtheta, a.dr, a.dz};
-  // rotation to Cartesian velocity, gradient and Hessian (:700-797): the
-  // outputs are linear in each component's (f, grad f, Hessian f), so every
-  // component adds its share as soon as it is fitted, and only the 39
-  // outputs stay live across the three components
+  T y[27];
+#pragma unroll
+  for (int iv = 0; iv < 27; iv++) {
+    const int aa = iv % 3 - 1, b = (iv / 3) % 3 - 1, c = iv / 9 - 1;
+    int itof = it + aa;
+    if (itof > g.nt) itof = 1;
+    if (itof < 1) itof = g.nt;
+    const int irof = ir + b, izof = iz + c;
+    const long o = hidx(g, itof, irof, izof);
+    double v;
+    if (ivar == 1) v = a.q1[o];
+    else if (ivar == 2) {
+      const double rc = (irof - 1.0) * a.dr;
+      v = (rc > (double)0.0001f) ? a.q2[o] / rc : 0.0;
+    } else v = a.q3[o];
+    y[iv] = (T)v;
+  }
+  const T y0 = y[13];
+#pragma unroll
+  for (int iv = 0; iv < 27; iv++) y[iv] = y[iv] - y0;
+  // least-squares fit, iterated as part_tools.F90:609-624 (converges in
+  // one pass since mint is the exact pseudo-inverse; the second pass adds
+  // the rounding-level correction)
+  T acoef[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) acoef[q] = (T)0;
+#pragma unroll 1
+  for (int iint = 1; iint <= 100; iint++) {
+    T mp[9];
+    T nrm = (T)0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) mp[q] = (T)0;
+#pragma unroll
+    for (int kk = 0; kk < 27; kk++) {
+      const int a1 = kk % 3 - 1, b1 = (kk / 3) % 3 - 1, c1 = kk / 9 - 1;
+      const T x0 = (T)(a1 * dx[0]), x1 = (T)(b1 * dx[1]), x2 = (T)(c1 * dx[2]);
+      const T rint = x0 * acoef[0] + x1 * acoef[1] + x2 * acoef[2] + x0 * x0 * acoef[3] + x0 * x1 * acoef[4] +
+                     x1 * x1 * acoef[5] + x1 * x2 * acoef[6] + x2 * x2 * acoef[7] + x2 * x0 * acoef[8];
+      const T res = rint - y[kk];
+#pragma unroll
+      for (int q = 0; q < 9; q++) mp[q] = mp[q] - mt.m[q * 27 + kk] * res;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) { acoef[q] = acoef[q] + mp[q]; nrm += mp[q] * mp[q]; }
+    if (sqrt((double)nrm) < (double)0.000001f) break;
+    // in FP32 the rounding-level correction of the second pass sits near
+    // 1e-7 of the coefficients and can stay above the absolute 1e-6 tolerance
+    if (sizeof(T) == sizeof(float) && iint >= 2) break;
+  }
+  double dtp, drp, dzp;
+  if (ivar == 1) { dtp = theta - (it - 1.0) * a.dtheta; drp = r - (ir - 0.5) * a.dr; dzp = zp - (iz - 0.5) * a.dz; }
+  else if (ivar == 2) { dtp = theta - (it - 0.5) * a.dtheta; drp = r - (ir - 1.0) * a.dr; dzp = zp - (iz - 0.5) * a.dz; }
+  else { dtp = theta - (it - 0.5) * a.dtheta; drp = r - (ir - 0.5) * a.dr; dzp = zp - (iz - 1.0) * a.dz; }
+  const double twopi = 8.0 * atan(1.0);
+  if (dtp > a.dtheta) dtp -= twopi;
+  if (dtp < -a.dtheta) dtp += twopi;
+  const T t_ = (T)dtp, r_ = (T)drp, z_ = (T)dzp;
+  const T *A = acoef;
+  fv = t_ * A[0] + r_ * A[1] + z_ * A[2] + t_ * t_ * A[3] + t_ * r_ * A[4] + r_ * r_ * A[5] + r_ * z_ * A[6] +
+       z_ * z_ * A[7] + z_ * t_ * A[8] + y0;
+  T FI[12];
+  FI[0] = A[0] + (T)2 * t_ * A[3] + r_ * A[4] + z_ * A[8];
+  FI[1] = A[1] + t_ * A[4] + (T)2 * r_ * A[5] + z_ * A[6];
+  FI[2] = A[2] + r_ * A[6] + (T)2 * z_ * A[7] + t_ * A[8];
+  FI[3] = (T)2 * A[3];
+  FI[4] = r_ * A[4];   // ftr as part_tools.F90:657
+  FI[5] = z_ * A[8];   // ftz as part_tools.F90:658
+  FI[6] = A[4]; FI[7] = (T)2 * A[5]; FI[8] = A[6];
+  FI[9] = A[8]; FI[10] = A[6]; FI[11] = (T)2 * A[7];
+  // chain rule to Cartesian derivatives (:671-698)
+  df[0] = -sn / rr * FI[0] + cs * FI[1];
+  df[1] = cs / rr * FI[0] + sn * FI[1];
+  df[2] = FI[2];
+  // second derivatives (id1,id2) = rows 4..12
+  ddf[0][0] = sn2 / r2 * FI[0] + sn * sn / rr * FI[1] + sn * sn / r2 * FI[3] - cs * sn / rr * FI[4] -
+              cs * sn / rr * FI[6] + cs * cs * FI[7];
+  ddf[1][0] = -cs2 / r2 * FI[0] - cs * sn / rr * FI[1] - cs * sn / r2 * FI[3] - sn * sn / rr * FI[4] +
+              cs * cs / rr * FI[6] + cs * sn * FI[7];
+  ddf[2][0] = -sn / rr * FI[5] + cs * FI[8];
+  ddf[0][1] = -cs2 / r2 * FI[0] - cs * sn / rr * FI[1] - cs * sn / r2 * FI[3] + cs * cs / rr * FI[4] -
+              sn * sn / rr * FI[6] + cs * sn * FI[7];
+  ddf[1][1] = -sn2 / r2 * FI[0] + cs * cs / rr * FI[1] + cs * cs / r2 * FI[3] + cs * sn / rr * FI[4] +
+              cs * sn / rr * FI[6] + sn * sn * FI[7];
+  ddf[2][1] = cs / rr * FI[5] + sn * FI[8];
+  ddf[0][2] = -sn / rr * FI[9] + cs * FI[10];
+  ddf[1][2] = cs / rr * FI[9] + sn * FI[10];
+  ddf[2][2] = FI[11];
+}
+
+// rotation to Cartesian velocity, gradient and Hessian (:700-797) from the
+// three components' (f, grad f, Hessian f) in fit rows kd*13 + (0 | 1..3 |
+// 4..12), summed over components in order
+template <typename T>
+__device__ inline void lsq_rotate(double theta, double r, const T *fit, long cap, T vel[3], T dvel[9], T ddvel[27]) {
+  const T sn = (T)sin(theta), cs = (T)cos(theta);
   const T rs = (T)fmax((double)0.000001f, r), rs2 = rs * rs;
   const T NM[3][3] = {{-sn, cs, (T)0}, {cs, sn, (T)0}, {(T)0, (T)0, (T)1}};
   T DN[3][3][3];  // DN[j][i][k] = d NM(i,k) / d x_j
@@ -85,148 +198,86 @@ __device__ bool interp_lsq(const PartArgs &a, double xp, double yp, double zp, T
   for (int q = 0; q < 9; q++) dvel[q] = (T)0;
 #pragma unroll
   for (int q = 0; q < 27; q++) ddvel[q] = (T)0;
-#pragma unroll 1
+#pragma unroll
   for (int kd = 0; kd < 3; kd++) {
-    const int ivar = kd + 1;
-    T y[27];
-#pragma unroll
-    for (int iv = 0; iv < 27; iv++) {
-      const int aa = iv % 3 - 1, b = (iv / 3) % 3 - 1, c = iv / 9 - 1;
-      int itof = it + aa;
-      if (itof > g.nt) itof = 1;
-      if (itof < 1) itof = g.nt;
-      const int irof = ir + b, izof = iz + c;
-      const long o = hidx(g, itof, irof, izof);
-      double v;
-      if (ivar == 1) v = a.q1[o];
-      else if (ivar == 2) {
-        const double rc = (irof - 1.0) * a.dr;
-        v = (rc > (double)0.0001f) ? a.q2[o] / rc : 0.0;
-      } else v = a.q3[o];
-      y[iv] = (T)v;
-    }
-    const T y0 = y[13];
-#pragma unroll
-    for (int iv = 0; iv < 27; iv++) y[iv] = y[iv] - y0;
-    // least-squares fit, iterated as part_tools.F90:609-624 (converges in
-    // one pass since mint is the exact pseudo-inverse; the second pass adds
-    // the rounding-level correction)
-    T acoef[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) acoef[q] = (T)0;
-#pragma unroll 1
-    for (int iint = 1; iint <= 100; iint++) {
-      // the 243 entries of mint are read where they are used: the empty asm
-      // hides the pointer from loop-invariant hoisting, which would keep
-      // them all in registers across the loops
-      const double *mint = a.mint;
-      asm volatile("" : "+s"(mint));
-      T mp[9];
-      T nrm = (T)0;
-#pragma unroll
-      for (int q = 0; q < 9; q++) mp[q] = (T)0;
-#pragma unroll
-      for (int kk = 0; kk < 27; kk++) {
-        const int a1 = kk % 3 - 1, b1 = (kk / 3) % 3 - 1, c1 = kk / 9 - 1;
-        const T x0 = (T)(a1 * dx[0]), x1 = (T)(b1 * dx[1]), x2 = (T)(c1 * dx[2]);
-        const T rint = x0 * acoef[0] + x1 * acoef[1] + x2 * acoef[2] + x0 * x0 * acoef[3] + x0 * x1 * acoef[4] +
-                       x1 * x1 * acoef[5] + x1 * x2 * acoef[6] + x2 * x2 * acoef[7] + x2 * x0 * acoef[8];
-        const T res = rint - y[kk];
-#pragma unroll
-        for (int q = 0; q < 9; q++) mp[q] = mp[q] - (T)mint[q * 27 + kk] * res;
-      }
-#pragma unroll
-      for (int q = 0; q < 9; q++) { acoef[q] = acoef[q] + mp[q]; nrm += mp[q] * mp[q]; }
-      if (sqrt((double)nrm) < (double)0.000001f) break;
-      // mint is the exact pseudo-inverse, so the second pass only adds the
-      // rounding-level correction; in FP32 that correction sits near 1e-7 of
-      // the coefficients and can stay above the absolute 1e-6 tolerance
-      if (sizeof(T) == sizeof(float) && iint >= 2) break;
-    }
-    double dtp, drp, dzp;
-    if (ivar == 1) { dtp = theta - (it - 1.0) * a.dtheta; drp = r - (ir - 0.5) * a.dr; dzp = zp - (iz - 0.5) * a.dz; }
-    else if (ivar == 2) { dtp = theta - (it - 0.5) * a.dtheta; drp = r - (ir - 1.0) * a.dr; dzp = zp - (iz - 0.5) * a.dz; }
-    else { dtp = theta - (it - 0.5) * a.dtheta; drp = r - (ir - 0.5) * a.dr; dzp = zp - (iz - 1.0) * a.dz; }
-    const double twopi = 8.0 * atan(1.0);
-    if (dtp > a.dtheta) dtp -= twopi;
-    if (dtp < -a.dtheta) dtp += twopi;
-    const T t_ = (T)dtp, r_ = (T)drp, z_ = (T)dzp;
-    const T *A = acoef;
-    const T f = t_ * A[0] + r_ * A[1] + z_ * A[2] + t_ * t_ * A[3] + t_ * r_ * A[4] + r_ * r_ * A[5] + r_ * z_ * A[6] +
-                z_ * z_ * A[7] + z_ * t_ * A[8] + y0;
-    T FI[12];
-    FI[0] = A[0] + (T)2 * t_ * A[3] + r_ * A[4] + z_ * A[8];
-    FI[1] = A[1] + t_ * A[4] + (T)2 * r_ * A[5] + z_ * A[6];
-    FI[2] = A[2] + r_ * A[6] + (T)2 * z_ * A[7] + t_ * A[8];
-    FI[3] = (T)2 * A[3];
-    FI[4] = r_ * A[4];   // ftr as part_tools.F90:657
-    FI[5] = z_ * A[8];   // ftz as part_tools.F90:658
-    FI[6] = A[4]; FI[7] = (T)2 * A[5]; FI[8] = A[6];
-    FI[9] = A[8]; FI[10] = A[6]; FI[11] = (T)2 * A[7];
-    // chain rule to Cartesian derivatives (:671-698)
+    const T *fr = fit + (long)(kd * 13) * cap;
+    const T f = fr[0];
     T df[3], ddf[3][3];
-    df[0] = -sn / rr * FI[0] + cs * FI[1];
-    df[1] = cs / rr * FI[0] + sn * FI[1];
-    df[2] = FI[2];
-    // second derivatives (id1,id2) = rows 4..12
-    ddf[0][0] = sn2 / r2 * FI[0] + sn * sn / rr * FI[1] + sn * sn / r2 * FI[3] - cs * sn / rr * FI[4] -
-                cs * sn / rr * FI[6] + cs * cs * FI[7];
-    ddf[1][0] = -cs2 / r2 * FI[0] - cs * sn / rr * FI[1] - cs * sn / r2 * FI[3] - sn * sn / rr * FI[4] +
-                cs * cs / rr * FI[6] + cs * sn * FI[7];
-    ddf[2][0] = -sn / rr * FI[5] + cs * FI[8];
-    ddf[0][1] = -cs2 / r2 * FI[0] - cs * sn / rr * FI[1] - cs * sn / r2 * FI[3] + cs * cs / rr * FI[4] -
-                sn * sn / rr * FI[6] + cs * sn * FI[7];
-    ddf[1][1] = -sn2 / r2 * FI[0] + cs * cs / rr * FI[1] + cs * cs / r2 * FI[3] + cs * sn / rr * FI[4] +
-                cs * sn / rr * FI[6] + sn * sn * FI[7];
-    ddf[2][1] = cs / rr * FI[5] + sn * FI[8];
-    ddf[0][2] = -sn / rr * FI[9] + cs * FI[10];
-    ddf[1][2] = cs / rr * FI[9] + sn * FI[10];
-    ddf[2][2] = FI[11];
-    // this component's share of the rotated outputs, added in the order of
-    // the sums over components (kd) of the reference formulas
-    T nm[3], dn[3][3];
 #pragma unroll
-    for (int id = 0; id < 3; id++) nm[id] = kd == 0 ? NM[id][0] : (kd == 1 ? NM[id][1] : NM[id][2]);
+    for (int q = 0; q < 3; q++) df[q] = fr[(long)(1 + q) * cap];
 #pragma unroll
-    for (int jd = 0; jd < 3; jd++)
+    for (int q = 0; q < 9; q++) ddf[q % 3][q / 3] = fr[(long)(4 + q) * cap];
 #pragma unroll
-      for (int id = 0; id < 3; id++)
-        dn[jd][id] = kd == 0 ? DN[jd][id][0] : (kd == 1 ? DN[jd][id][1] : DN[jd][id][2]);
-#pragma unroll
-    for (int id = 0; id < 3; id++) vel[id] = vel[id] + f * nm[id];
+    for (int id = 0; id < 3; id++) vel[id] = vel[id] + f * NM[id][kd];
 #pragma unroll
     for (int id = 0; id < 3; id++)
 #pragma unroll
-      for (int jd = 0; jd < 3; jd++) dvel[id + 3 * jd] = dvel[id + 3 * jd] + df[jd] * nm[id] + f * dn[jd][id];
+      for (int jd = 0; jd < 3; jd++)
+        dvel[id + 3 * jd] = dvel[id + 3 * jd] + df[jd] * NM[id][kd] + f * DN[jd][id][kd];
 #pragma unroll
     for (int id = 0; id < 3; id++)
 #pragma unroll
       for (int jd = 0; jd < 3; jd++)
 #pragma unroll
         for (int ld = 0; ld < 3; ld++)
-          ddvel[id + 3 * jd + 9 * ld] = ddvel[id + 3 * jd + 9 * ld] + ddf[jd][ld] * nm[id] + df[jd] * dn[ld][id] +
-                                        df[ld] * dn[jd][id] + f * DDN(ld, jd, id, kd);
+          ddvel[id + 3 * jd + 9 * ld] = ddvel[id + 3 * jd + 9 * ld] + ddf[jd][ld] * NM[id][kd] +
+                                        df[jd] * DN[ld][id][kd] + df[ld] * DN[jd][id][kd] + f * DDN(ld, jd, id, kd);
   }
-  return true;
 }
 
-// DDNMAT(l,j,i,k) of part_tools.F90:725-760 with 0-based l,j,i,k encoded as
-// key = 8l + 4j + 2i + k.
-
+// one substep, part 1: the fits (a particle whose stencil leaves the arrays
+// is removed, part_tools.F90:519-538)
 template <typename T>
-__global__ __launch_bounds__(128) void k_part_lsq(PartArgs a) {
+__global__ __launch_bounds__(128, 2) void k_lsq_fit(PartArgs a, MintArg<T> mt) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.cap || p >= *a.n_dev || !a.alive[p]) return;
+  const long cap = a.cap;
+  double *u = a.u + p;
+#define U(v) u[(long)(v) * cap]
+  double theta, r;
+  int it, ir, iz;
+  const double zp = U(2);
+  if (!lsq_locate(a, U(0), U(1), zp, theta, r, it, ir, iz)) {
+    a.alive[p] = 0;
+    for (int v = 0; v < a.nvar; v++) U(v) = 0.0;
+    return;
+  }
+  T *fit = reinterpret_cast<T *>(a.fit) + p;
+#pragma unroll 1
+  for (int kd = 0; kd < 3; kd++) {
+    T f, df[3], ddf[3][3];
+    lsq_component<T>(a, mt, kd + 1, theta, r, zp, it, ir, iz, f, df, ddf);
+    T *fr = fit + (long)(kd * 13) * cap;
+    fr[0] = f;
+#pragma unroll
+    for (int q = 0; q < 3; q++) fr[(long)(1 + q) * cap] = df[q];
+#pragma unroll
+    for (int q = 0; q < 9; q++) fr[(long)(4 + q) * cap] = ddf[q % 3][q / 3];
+  }
+#undef U
+}
+
+// one substep, part 2: rotation, rates and state update (part_tools.F90:
+// 700-930), then removal of particles that left the domain (:962-1025)
+template <typename T>
+__global__ __launch_bounds__(128, 2) void k_lsq_step(PartArgs a) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.cap || p >= *a.n_dev || !a.alive[p]) return;
   const long cap = a.cap;
   double *u = a.u + p;      // u[v*cap]
   double *d2 = a.ddt2 ? a.ddt2 + p : nullptr;
 #define U(v) u[(long)(v) * cap]
-  for (int isub = 0; isub < a.nsub; isub++) {
+  {
     T vel[3], dvel[9], ddvel[27];
-    if (!interp_lsq<T>(a, U(0), U(1), U(2), vel, dvel, ddvel)) {
-      a.alive[p] = 0;
-      for (int v = 0; v < a.nvar; v++) U(v) = 0.0;
-      return;
+    {
+      const double xp = U(0), yp = U(1);
+      const double r = sqrt(xp * xp + yp * yp);
+      const double pi = (double)4.0f * atan(1.0);
+      double theta;
+      if (xp >= 0.0 && yp >= 0.0) theta = acos(xp / r);
+      else if (xp < 0.0 && yp >= 0.0) theta = acos(xp / r);
+      else theta = acos(-xp / r) + pi;
+      lsq_rotate<T>(theta, r, reinterpret_cast<const T *>(a.fit) + p, cap, vel, dvel, ddvel);
     }
     // rates (part_tools.F90:836-870) and the update of states 1..78
     // (part_tools.F90:926-930 Euler; dertest RK3 form), state by state: every
@@ -296,7 +347,7 @@ __global__ __launch_bounds__(128) void k_part_lsq(PartArgs a) {
       if (d2) for (int v = 0; v < a.nvar; v++) d2[(long)v * cap] = 0.0;
       return;
     }
-  }
+    }
 #undef U
 }
 
@@ -526,9 +577,21 @@ void launch_particles(vrh_t *h, int irk) {
       if (h->prec == VRH_MIXED) hipLaunchKernelGGL(k_part_tri<float>, grd, blk, 0, h->stream, a);
       else hipLaunchKernelGGL(k_part_tri<double>, grd, blk, 0, h->stream, a);
     } else {
+      a.fit = b.fit;
+      a.nsub = 1;
       dim3 blk(128), grd((unsigned)((b.cap + 127) / 128));
-      if (h->prec == VRH_MIXED) hipLaunchKernelGGL(k_part_lsq<float>, grd, blk, 0, h->stream, a);
-      else hipLaunchKernelGGL(k_part_lsq<double>, grd, blk, 0, h->stream, a);
+      MintArg<float> mf;
+      MintArg<double> md;
+      for (int q = 0; q < 243; q++) { mf.m[q] = (float)h->h_mint[q]; md.m[q] = h->h_mint[q]; }
+      for (int isub = 0; isub < b.nsub; isub++) {
+        if (h->prec == VRH_MIXED) {
+          hipLaunchKernelGGL(k_lsq_fit<float>, grd, blk, 0, h->stream, a, mf);
+          hipLaunchKernelGGL(k_lsq_step<float>, grd, blk, 0, h->stream, a);
+        } else {
+          hipLaunchKernelGGL(k_lsq_fit<double>, grd, blk, 0, h->stream, a, md);
+          hipLaunchKernelGGL(k_lsq_step<double>, grd, blk, 0, h->stream, a);
+        }
+      }
     }
   }
 }
@@ -554,6 +617,9 @@ int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st
     if (hipMalloc(&B.d_n, sizeof(long)) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
     hipMemsetAsync(B.d_n, 0, sizeof(long), h->stream);
     B.ddt2 = nullptr;
+    B.fit = nullptr;
+    if (scheme != VRH_PART_TRILINEAR_RK3 &&
+        hipMalloc(&B.fit, sizeof(double) * kFitRows * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
     if (scheme != VRH_PART_LSQ_EULER) {
       if (hipMalloc(&B.ddt2, sizeof(double) * nvar * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
       hipMemsetAsync(B.ddt2, 0, sizeof(double) * nvar * B.cap, h->stream);
