@@ -69,3 +69,25 @@ def test_ensemble_matches_members_stepped_alone():
     ens.close()
     for s in ens_members + alone:
         s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_captured_step_after_set_time_equals_eager(precision):
+    """A handle's first step after vrh_set_time, captured into a graph, equals
+    the eager step bit for bit (the eager path forks the independent sweep
+    chains onto side streams; the captured step keeps them in order)."""
+    nt, nr, nz = 33, 32, 64
+    print(f"case captured step after set_time {precision}", flush=True)
+    q, dt = ring_solver(nt, nr, nz, 1000.0, precision, graph=False, amp=1e-2)[1:]
+    out = []
+    for graph in (True, False):
+        s = vring_amd.Solver(nt, nr, nz, 4.0, 12.0, 1000.0, dt, precision=precision)
+        for n, a in zip(("q1", "q2", "q3"), q):
+            s.upload(n, a)
+        s.set_time(0.37)
+        s.set_graph(graph)
+        s.step(1)
+        out.append([s.download(n) for n in FIELDS])
+        s.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)

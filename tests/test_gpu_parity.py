@@ -45,6 +45,22 @@ def test_poisson_matches_oracle(nt, nr, nz, theta):
     assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("precision,lines", [("fp64", 8), ("mixed", 16), ("mixed", 2)])
+def test_poisson_long_z_lines_match_oracle(precision, lines):
+    """nz = 512 z transforms at the bench's tile widths (16 mixed lines per
+    block run with 512 threads) against the oracle; FP64 to 1e-11, mixed to
+    the FP32 transforms' rounding."""
+    nt, nr, nz = 8, 12, 512
+    print(f"case poisson long z {precision} lines={lines}", flush=True)
+    o, s = make_pair(nt, nr, nz, precision=precision)
+    s.set_tuning(dct_lines=lines)
+    rhs = np.random.default_rng(9).standard_normal((nt, nr, nz))
+    ref = o.poisson(rhs)
+    got = s.poisson(rhs)
+    tol = 1e-11 if precision == "fp64" else 2e-5
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+
+
 def test_ring_100_steps_fp64_matches_oracle():
     """Config-1 shape at odd ntheta (the reference's Poisson packing needs odd
     ntheta): nr=64, ntheta=33, nz=128, r in [0,4], z in [0,12], Re=1000, one

@@ -164,7 +164,9 @@ namespace {
 int build_graph(vrh_t *h) {
   graph_invalidate(h);
   HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  h->capturing = true;
   int rc = step_body(h);
+  h->capturing = false;
   hipGraph_t gr = nullptr;
   hipError_t e = hipStreamEndCapture(h->stream, &gr);
   if (rc) { if (gr) hipGraphDestroy(gr); return rc; }
@@ -210,6 +212,11 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
   const double rom[4] = {0, 0.0, -17.0 / 60.0, -5.0 / 12.0};
   for (int i = 1; i <= 3; i++) h->rk[i] = {gam[i], rom[i], gam[i] + rom[i]};
   HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  for (int q = 0; q < 2; q++) HIPCHK(h, hipStreamCreateWithFlags(&h->side[q], hipStreamNonBlocking));
+  for (int irk = 0; irk < 4; irk++) {
+    HIPCHK(h, hipEventCreateWithFlags(&h->fork_ev[irk], hipEventDisableTiming));
+    for (int q = 0; q < 2; q++) HIPCHK(h, hipEventCreateWithFlags(&h->join_ev[irk][q], hipEventDisableTiming));
+  }
   host_mesh_vectors(h);
   // radial vectors
   const int nv = h->nr + 3;
@@ -340,6 +347,13 @@ int vrh_destroy(vrh_t *h) {
     if (b.alive) hipFree(b.alive);
     void *extra[] = {b.d_n, b.inflow, b.u_tmp, b.ddt2_tmp, b.alive_tmp, b.work};
     for (void *q : extra) if (q) hipFree(q);
+  }
+  for (int q = 0; q < 2; q++)
+    if (h->side[q]) hipStreamDestroy(h->side[q]);
+  for (int irk = 0; irk < 4; irk++) {
+    if (h->fork_ev[irk]) hipEventDestroy(h->fork_ev[irk]);
+    for (int q = 0; q < 2; q++)
+      if (h->join_ev[irk][q]) hipEventDestroy(h->join_ev[irk][q]);
   }
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -770,7 +784,9 @@ int ens_build(vrh_ens_t *e) {
   for (size_t q = 0; q < e->m.size() && rc == 0; q++) {
     vrh_t *h = e->m[q];
     if (hipStreamWaitEvent(h->stream, e->fork, 0) != hipSuccess) { rc = -3; break; }
+    h->capturing = true;
     rc = step_body(h);
+    h->capturing = false;
     if (rc) { e->err = "member " + std::to_string(q) + ": " + h->err; break; }
     if (hipEventRecord(e->join[q], h->stream) != hipSuccess ||
         hipStreamWaitEvent(e->origin, e->join[q], 0) != hipSuccess) rc = -3;

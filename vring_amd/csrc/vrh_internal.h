@@ -100,6 +100,15 @@ struct vrh_s {
   VGrid g;
   VRK rk[4];
   hipStream_t stream;
+  // side streams for the independent component chains of the line sweeps
+  // (forked from and joined back into `stream`; captured with the step)
+  hipStream_t side[2] = {nullptr, nullptr};
+  bool capturing = false;  // set while a step is being captured into a graph
+  // one fork and two join events per substep: a captured step records each
+  // event once (an event recorded twice in one capture keeps only one of
+  // the dependency sets)
+  hipEvent_t fork_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t join_ev[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
   std::string err;
   // host copies of the radial vectors
   std::vector<double> hv[21];

@@ -1862,17 +1862,17 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
 }
 
 template <typename TW, int MODE>
-static void line_launch(vrh_t *h, const LineArgs &a, int M) {
+static void line_launch(vrh_t *h, const LineArgs &a, int M, hipStream_t st) {
   const int P = a.lc.P;
   dim3 blk(64 * P), grd((unsigned)((a.nlines + 63) / 64));
   KTimer kt(h, MODE <= SW_R2 ? KT_LINE_R : KT_LINE_Z);
   switch (M) {
-    case 16: hipLaunchKernelGGL((k_line<TW, 16, MODE>), grd, blk, 0, h->stream, a); break;
-    case 24: hipLaunchKernelGGL((k_line<TW, 24, MODE>), grd, blk, 0, h->stream, a); break;
-    case 32: hipLaunchKernelGGL((k_line<TW, 32, MODE>), grd, blk, 0, h->stream, a); break;
-    case 40: hipLaunchKernelGGL((k_line<TW, 40, MODE>), grd, blk, 0, h->stream, a); break;
-    case 48: hipLaunchKernelGGL((k_line<TW, 48, MODE>), grd, blk, 0, h->stream, a); break;
-    default: hipLaunchKernelGGL((k_line<TW, 64, MODE>), grd, blk, 0, h->stream, a); break;
+    case 16: hipLaunchKernelGGL((k_line<TW, 16, MODE>), grd, blk, 0, st, a); break;
+    case 24: hipLaunchKernelGGL((k_line<TW, 24, MODE>), grd, blk, 0, st, a); break;
+    case 32: hipLaunchKernelGGL((k_line<TW, 32, MODE>), grd, blk, 0, st, a); break;
+    case 40: hipLaunchKernelGGL((k_line<TW, 40, MODE>), grd, blk, 0, st, a); break;
+    case 48: hipLaunchKernelGGL((k_line<TW, 48, MODE>), grd, blk, 0, st, a); break;
+    default: hipLaunchKernelGGL((k_line<TW, 64, MODE>), grd, blk, 0, st, a); break;
   }
 }
 
@@ -1885,21 +1885,41 @@ template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const d
   a.c3s = c3s;
   a.c3s_dev = c3s_dev;
   a.corr3n = h->f.corr3n;
+  // The three components' chains are independent: launched eagerly, q3 and
+  // q2 run on the side streams, forked from and joined back into the
+  // handle's stream.  Under graph capture (and with the kernel timer on) all
+  // run in order on the handle's stream: captured steps with this fork gave
+  // results that differed from run to run, so the graph keeps the chains
+  // sequential.
+  const bool par = !h->kt_on && !h->capturing;
+  hipStream_t s1 = par ? h->side[0] : h->stream, s2 = par ? h->side[1] : h->stream;
+  const int ie = (irk >= 1 && irk <= 3) ? irk : 0;
+  if (par) {
+    (void)hipEventRecord(h->fork_ev[ie], h->stream);
+    (void)hipStreamWaitEvent(s1, h->fork_ev[ie], 0);
+    (void)hipStreamWaitEvent(s2, h->fork_ev[ie], 0);
+  }
   // q1: r then z (step.F90:75-78)
   a.lc = h->lc[irk][SW_R1]; a.nlines = (long)g.nt * g.nz; a.r = h->f.r1;
-  line_launch<TW, SW_R1>(h, a, h->Msw[SW_R1]);
+  line_launch<TW, SW_R1>(h, a, h->Msw[SW_R1], h->stream);
   a.lc = h->lc[irk][SW_Z1]; a.nlines = (long)g.nt * g.nr; a.j0 = 1; a.q = h->f.q1; a.dq = h->f.dq1;
-  line_launch<TW, SW_Z1>(h, a, h->Msw[SW_Z1]);
+  line_launch<TW, SW_Z1>(h, a, h->Msw[SW_Z1], h->stream);
   // q3: r then z (step.F90:124-127)
   a.lc = h->lc[irk][SW_R3]; a.nlines = (long)g.nt * g.nz; a.r = h->f.r3;
-  line_launch<TW, SW_R3>(h, a, h->Msw[SW_R3]);
+  line_launch<TW, SW_R3>(h, a, h->Msw[SW_R3], s1);
   a.lc = h->lc[irk][SW_Z3]; a.nlines = (long)g.nt * g.nr; a.j0 = 1; a.q = h->f.q3; a.dq = h->f.dq3;
-  line_launch<TW, SW_Z3>(h, a, h->Msw[SW_Z3]);
+  line_launch<TW, SW_Z3>(h, a, h->Msw[SW_Z3], s1);
   // q2: z then r (step.F90:171-177)
   a.lc = h->lc[irk][SW_Z2]; a.nlines = (long)g.nt * (g.nr - 1); a.j0 = 2; a.r = h->f.r2;
-  line_launch<TW, SW_Z2>(h, a, h->Msw[SW_Z2]);
+  line_launch<TW, SW_Z2>(h, a, h->Msw[SW_Z2], s2);
   a.lc = h->lc[irk][SW_R2]; a.nlines = (long)g.nt * g.nzc; a.q = h->f.q2; a.dq = h->f.dq2;
-  line_launch<TW, SW_R2>(h, a, h->Msw[SW_R2]);
+  line_launch<TW, SW_R2>(h, a, h->Msw[SW_R2], s2);
+  if (par) {
+    (void)hipEventRecord(h->join_ev[ie][0], s1);
+    (void)hipEventRecord(h->join_ev[ie][1], s2);
+    (void)hipStreamWaitEvent(h->stream, h->join_ev[ie][0], 0);
+    (void)hipStreamWaitEvent(h->stream, h->join_ev[ie][1], 0);
+  }
 }
 
 static int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
