@@ -272,7 +272,8 @@ def copy_calibration(mib, reps=3):
 
 
 KERNEL_KEYS = {  # rocprof kernel name prefix -> kernel timer name
-    "k_explicit_cell": "explicit", "k_explicit_march": "explicit", "k_explicit": "explicit",
+    "k_explicit_cell": "explicit", "k_explicit_march": "explicit", "k_explicit_wave": "explicit",
+    "k_explicit": "explicit", "k_lsq_fit": "particles", "k_lsq_step": "particles",
     "k_project": "project", "k_divfft": "divfft", "k_dctz": "dct", "k_radial": "radial", "k_ifft": "ifft",
     "k_hdnbc": "hdnbc", "k_part_lsq": "particles", "k_part_tri": "particles", "k_scalars": "scalars",
 }

@@ -628,6 +628,9 @@ extern "C" int vrh_autotune(vrh_t *h, int reps, int *choice) {
   if (reps < 1) reps = 3;
   const bool kt = h->kt_on;
   h->kt_on = false;
+  // time the sweeps as the steps will run them: in order when steps are
+  // captured into graphs, forked onto the side streams when eager
+  h->sweeps_serial = h->use_graph;
   const int ns[6] = {h->nr, h->nr, h->nrc, h->nzc, h->nzc, h->nzc};
   for (int sw = 0; sw < 6; sw++) {
     int best = h->Msw[sw];
@@ -636,14 +639,14 @@ extern "C" int vrh_autotune(vrh_t *h, int reps, int *choice) {
       if ((ns[sw] + M - 1) / M > 16) continue;
       h->Msw[sw] = M;
       int rc = build_line_coefs(h);
-      if (rc) return rc;
+      if (rc) { h->kt_on = kt; h->sweeps_serial = false; return rc; }
       double t = h->prec == VRH_MIXED ? time_sweeps<float>(h, reps) : time_sweeps<double>(h, reps);
       if (t < bestt) { bestt = t; best = M; }
     }
     h->Msw[sw] = best;
   }
   int rc = build_line_coefs(h);
-  if (rc) return rc;
+  if (rc) { h->kt_on = kt; h->sweeps_serial = false; return rc; }
   const bool pow2t = (h->nt & (h->nt - 1)) == 0;
   if (pow2t) {
     int best = 0; double bestt = 1e30;
@@ -675,6 +678,7 @@ extern "C" int vrh_autotune(vrh_t *h, int reps, int *choice) {
   }
   graph_invalidate(h);
   h->kt_on = kt;
+  h->sweeps_serial = false;
   if (choice) {
     for (int sw = 0; sw < 6; sw++) choice[sw] = h->Msw[sw];
     choice[6] = h->fft_lines;

@@ -1891,7 +1891,7 @@ template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const d
   // run in order on the handle's stream: captured steps with this fork gave
   // results that differed from run to run, so the graph keeps the chains
   // sequential.
-  const bool par = !h->kt_on && !h->capturing;
+  const bool par = !h->kt_on && !h->capturing && !h->sweeps_serial;
   hipStream_t s1 = par ? h->side[0] : h->stream, s2 = par ? h->side[1] : h->stream;
   const int ie = (irk >= 1 && irk <= 3) ? irk : 0;
   if (par) {
