@@ -959,7 +959,6 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_pe
 // Poisson solve (poisson_solve.F90:140-179) in five passes: divergence +
 // theta FFT, z DCT-II, radial Thomas, z DCT-III, inverse theta FFT.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int bitrev(int x, int logn) { return (int)(__builtin_bitreverse32((unsigned)x) >> (32 - logn)); }
 
 // complex multiply (x + iy)(c + is)
 template <typename T> __device__ __forceinline__ void cmul(T &x, T &y, T c, T s) {

@@ -19,7 +19,7 @@ namespace {
 struct PartArgs {
   VGrid g;
   const double *q1, *q2, *q3;
-  const double *mint;  // 9 x 27
+  const double *mint;  // 9 x 27 (device copy; the fit kernels take it by value)
   double *u, *ddt2;
   void *fit;          // LSQ fit rows (39 x cap of the work type)
   int *alive;
@@ -28,8 +28,6 @@ struct PartArgs {
   int nvar, nsub, scheme, irk;
   double st, partdt, gam, rom, dtheta, dr, dz, rmax, zmax;
 };
-
-template <typename T> struct V3 { T x[3]; };
 
 // Interpolation and chain rule of one particle, part_tools.F90:519-797, in
 // two kernels per substep: the fit kernel gathers the 27-point stencil of
