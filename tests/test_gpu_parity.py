@@ -102,7 +102,7 @@ def test_mixed_precision_ring_error_vs_fp64():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("nt,nr,nz", [(128, 40, 64), (48, 36, 64)])
+@pytest.mark.parametrize("nt,nr,nz", [(128, 40, 64), (48, 36, 64), (256, 16, 32)])
 def test_explicit_forms_agree_bitwise(precision, nt, nr, nz):
     """The marching forms of the explicit kernel (one block row per theta
     line, or one wavefront per r row) do the same arithmetic per cell in the
