@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the vring RK3 substep on one MI355X.
 
-Workload (the north-star shape): nr=256, ntheta=128, nz=512, r in [0,4],
+Workload: nr=256, ntheta=128, nz=512, r in [0,4],
 z in [0,12], Re=3000, one Gaussian-core ring (R=1, z0=3, sigma=0.2, Gamma=1)
 with seeded core-radius perturbations of amplitude 1e-3 on modes m=2..8
 (phases from numpy PCG64(0)), dt at CFL 0.3, inputs resident in HBM.

@@ -1314,7 +1314,7 @@ int orc_part_advance(orc_t *o, int field_mode, int scheme, int irk, double st, d
   return bad ? -1 : 0;
 }
 
-/* ---- trilinear option (north-star form; not in the reference's built code) ---- */
+/* ---- trilinear option (an alternative interpolation; not in the reference's built code) ---- */
 static double tri_comp(orc_t *o, int ivar, double theta, double r, double z) {
   double th0 = (ivar == 1) ? 0.0 : 0.5 * o->dtheta;
   double r0 = (ivar == 2) ? 0.0 : 0.5 * o->dr;
