@@ -34,6 +34,29 @@ def test_substep_phases_match_oracle(nt, nr, nz, form):
         compare(o, s, ["dph", "q1", "q2", "q3", "pres", "prg"], 1e-11)
 
 
+@pytest.mark.parametrize("slip", [1.0, 0.5])
+def test_substep_phases_with_wall_slip_match_oracle(slip):
+    """Wall slip (the q1, q3 copies into the ghost row at nrc and the slip
+    rows of the sweeps) against the oracle, every phase of three substeps."""
+    nt, nr, nz = 16, 24, 40
+    print(f"case phases slip={slip}", flush=True)
+    o, s = make_pair(nt, nr, nz, slip=slip)
+    seed_random(o, s, seed=77)
+    o.set_time(0.21)
+    s.set_time(0.21)
+    for irk in (1, 2, 3):
+        for p in ["ucoeff", "coesys", "hdnbc", "hdnl1", "hdnl2", "hdnl3", "step"]:
+            assert o.phase(p, irk) == 0
+        for p in ["hdnbc", "explicit", "adi"]:
+            s.phase(p, irk)
+        compare(o, s, ["dq1", "dq2", "dq3", "ru1", "ru2", "ru3"], 1e-11)
+        for p in ["divq", "laplace", "upsol", "uppres"]:
+            assert o.phase(p, irk) == 0
+        for p in ["poisson", "project"]:
+            s.phase(p, irk)
+        compare(o, s, ["dph", "q1", "q2", "q3", "pres", "prg"], 1e-11)
+
+
 @pytest.mark.parametrize("nt,nr,nz,theta", [(9, 16, 32, "exact_m2"), (16, 32, 64, "consistent"),
                                             (33, 20, 24, "exact_m2"), (128, 64, 128, "exact_m2")])
 def test_poisson_matches_oracle(nt, nr, nz, theta):
