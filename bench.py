@@ -64,7 +64,7 @@ def ring_state(nt=NT, nr=NR, nz=NZ):
     return q, dt
 
 
-def run_path(prec, q, dt, steps, warmup, tune_reps=3):
+def run_path(prec, q, dt, steps, warmup, tune_reps=8):
     """Flow solver only: K timed RK3 steps after W warm-up steps."""
     import vring_amd
     s = vring_amd.Solver(NT, NR, NZ, RMAX, ZMAX, RE, dt, precision=prec)
@@ -358,7 +358,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--particles", type=int, default=1000000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--autotune-reps", type=int, default=3)
+    ap.add_argument("--autotune-reps", type=int, default=8)
     ap.add_argument("--only", choices=("fp64", "mixed", "particles"), default=None,
                     help="run one path only and print its timing (profiling runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
