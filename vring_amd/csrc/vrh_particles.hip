@@ -71,21 +71,35 @@ __device__ inline void lsq_component(const PartArgs &a, const MintArg<T> &mt, in
   const T rr = (T)r, r2 = rr * rr;
   const double dx[3] = {a.dtheta, a.dr, a.dz};
   T y[27];
+  // the three theta columns of the stencil (periodic) and the component's
+  // array; gathers go out one z plane (9 loads) at a time, so only nine
+  // addresses are live at once
+  int itc[3] = {it - 1, it, it + 1};
 #pragma unroll
-  for (int iv = 0; iv < 27; iv++) {
-    const int aa = iv % 3 - 1, b = (iv / 3) % 3 - 1, c = iv / 9 - 1;
-    int itof = it + aa;
-    if (itof > g.nt) itof = 1;
-    if (itof < 1) itof = g.nt;
-    const int irof = ir + b, izof = iz + c;
-    const long o = hidx(g, itof, irof, izof);
-    double v;
-    if (ivar == 1) v = a.q1[o];
-    else if (ivar == 2) {
-      const double rc = (irof - 1.0) * a.dr;
-      v = (rc > (double)0.0001f) ? a.q2[o] / rc : 0.0;
-    } else v = a.q3[o];
-    y[iv] = (T)v;
+  for (int q = 0; q < 3; q++) {
+    if (itc[q] > g.nt) itc[q] = 1;
+    if (itc[q] < 1) itc[q] = g.nt;
+  }
+  const double *qa = ivar == 1 ? a.q1 : (ivar == 2 ? a.q2 : a.q3);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double vv[9];
+#pragma unroll
+    for (int u = 0; u < 9; u++) {
+      const int aa = u % 3, b = u / 3;
+      vv[u] = qa[hidx(g, itc[aa], ir + b - 1, iz + c - 1)];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < 9; u++) {
+      const int b = u / 3, irof = ir + b - 1;
+      double v = vv[u];
+      if (ivar == 2) {
+        const double rc = (irof - 1.0) * a.dr;
+        v = (rc > (double)0.0001f) ? v / rc : 0.0;
+      }
+      y[u + 9 * c] = (T)v;
+    }
   }
   const T y0 = y[13];
 #pragma unroll
@@ -226,7 +240,7 @@ __device__ inline void lsq_rotate(double theta, double r, const T *fit, long cap
 // one substep, part 1: the fits (a particle whose stencil leaves the arrays
 // is removed, part_tools.F90:519-538)
 template <typename T>
-__global__ __launch_bounds__(128, 2) void k_lsq_fit(PartArgs a, MintArg<T> mt) {
+__global__ __launch_bounds__(128, sizeof(T) == 4 ? 3 : 2) void k_lsq_fit(PartArgs a, MintArg<T> mt) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.cap || p >= *a.n_dev || !a.alive[p]) return;
   const long cap = a.cap;
