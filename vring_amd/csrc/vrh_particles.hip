@@ -298,11 +298,13 @@ __global__ __launch_bounds__(128, 2) void k_lsq_step(PartArgs a) {
     // velocity; no array of 78 rates is kept
     const double inv = (double)1.0f / a.st;
     const bool euler = a.scheme == VRH_PART_LSQ_EULER;
-    auto upd = [&](int v, double rate) {
+    // every state is read once (its old value is passed in), so the stores
+    // never have to wait for re-reads of the same rows
+    auto upd = [&](int v, double old, double rate) {
       if (euler) {
-        U(v) = U(v) + rate * a.partdt;
+        U(v) = old + rate * a.partdt;
       } else {
-        U(v) = U(v) + a.gam * rate * a.partdt + a.rom * d2[(long)v * cap] * a.partdt;
+        U(v) = old + a.gam * rate * a.partdt + a.rom * d2[(long)v * cap] * a.partdt;
         d2[(long)v * cap] = rate;
       }
     };
@@ -312,9 +314,8 @@ __global__ __launch_bounds__(128, 2) void k_lsq_step(PartArgs a) {
       for (int j = 0; j < 3; j++) {
         double H[3], ra[3], rb[3];
         for (int l = 0; l < 3; l++) H[l] = U(24 + l + 3 * j + 9 * k);
+        for (int i = 0; i < 3; i++) ra[i] = U(51 + i + 3 * j + 9 * k);
         for (int i = 0; i < 3; i++) {
-          const int ix = i + 3 * j + 9 * k;
-          ra[i] = U(51 + ix);
           double sumh = 0.0;
           for (int l = 0; l < 3; l++) sumh += H[l] * (double)dvel[i + 3 * l];
           for (int m = 0; m < 3; m++)
@@ -323,29 +324,27 @@ __global__ __launch_bounds__(128, 2) void k_lsq_step(PartArgs a) {
         }
         for (int i = 0; i < 3; i++) {
           const int ix = i + 3 * j + 9 * k;
-          upd(24 + ix, ra[i]);
-          upd(51 + ix, rb[i]);
+          upd(24 + ix, H[i], ra[i]);
+          upd(51 + ix, ra[i], rb[i]);
         }
       }
+    double x0[3], v0[3];
+    for (int i = 0; i < 3; i++) { x0[i] = U(i); v0[i] = U(3 + i); }
     for (int j = 0; j < 3; j++) {
       double ra[3], rb[3];
+      for (int i = 0; i < 3; i++) ra[i] = U(15 + i + 3 * j);
       for (int i = 0; i < 3; i++) {
-        ra[i] = U(15 + i + 3 * j);
         const double sumj =
             J[0 + 3 * j] * (double)dvel[i + 0] + J[1 + 3 * j] * (double)dvel[i + 3] + J[2 + 3 * j] * (double)dvel[i + 6];
         rb[i] = inv * (sumj - ra[i]);
       }
       for (int i = 0; i < 3; i++) {
-        upd(6 + i + 3 * j, ra[i]);
-        upd(15 + i + 3 * j, rb[i]);
+        upd(6 + i + 3 * j, J[i + 3 * j], ra[i]);
+        upd(15 + i + 3 * j, ra[i], rb[i]);
       }
     }
-    {
-      double v0[3];
-      for (int i = 0; i < 3; i++) v0[i] = U(3 + i);
-      for (int i = 0; i < 3; i++) upd(i, v0[i]);
-      for (int i = 0; i < 3; i++) upd(3 + i, inv * ((double)vel[i] - v0[i]));
-    }
+    for (int i = 0; i < 3; i++) upd(i, x0[i], v0[i]);
+    for (int i = 0; i < 3; i++) upd(3 + i, v0[i], inv * ((double)vel[i] - v0[i]));
     if (euler) {
       for (int i = 0; i < 3; i++) U(78 + i) = vel[i];
       for (int i = 0; i < 9; i++) U(81 + i) = dvel[i];
