@@ -42,6 +42,15 @@ int dalloc_bytes(vrh_t *h, void **p, size_t bytes) {
   HIPCHK(h, hipMemsetAsync(*p, 0, bytes + 64, h->stream));
   return 0;
 }
+// Host-to-device copy on the handle's stream, complete on return.  The
+// stream is non-blocking, so a plain hipMemcpy (null stream) would not be
+// ordered after the zero fills dalloc queues there, nor after kernels still
+// reading the destination.
+int h2d(vrh_t *h, void *dst, const void *src, size_t bytes) {
+  HIPCHK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
 
 // host double -> device buffer of the work type
 int upload_conv(vrh_t *h, void *dst, const double *src, size_t n, bool work) {
@@ -224,7 +233,7 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
   if (rc) return rc;
   std::vector<double> vb((size_t)21 * nv);
   for (int q = 0; q < 21; q++) std::memcpy(&vb[(size_t)q * nv], h->hv[q].data(), nv * sizeof(double));
-  HIPCHK(h, hipMemcpy(h->d_vec, vb.data(), vb.size() * sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = h2d(h, h->d_vec, vb.data(), vb.size() * sizeof(double)))) return rc;
   const double **cv[21] = {&h->coef.rm, &h->coef.rc, &h->coef.h1d11, &h->coef.h1d22d, &h->coef.h1d22n,
                            &h->coef.h1dare, &h->coef.h2d11, &h->coef.h2d22m, &h->coef.h2d22p, &h->coef.h2dare,
                            &h->coef.s1t11, &h->coef.s1t2n, &h->coef.s1t2s, &h->coef.s1t2c, &h->coef.s2t11,
@@ -277,9 +286,9 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
     if ((rc = dalloc(h, &h->d_twt, tt.size())) || (rc = dalloc(h, &h->d_twz, tz.size())) ||
         (rc = dalloc(h, &h->d_ez, ez.size())))
       return rc;
-    HIPCHK(h, hipMemcpy(h->d_twt, tt.data(), tt.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->d_twz, tz.data(), tz.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->d_ez, ez.data(), ez.size() * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = h2d(h, h->d_twt, tt.data(), tt.size() * sizeof(double)))) return rc;
+    if ((rc = h2d(h, h->d_twz, tz.data(), tz.size() * sizeof(double)))) return rc;
+    if ((rc = h2d(h, h->d_ez, ez.data(), ez.size() * sizeof(double)))) return rc;
     std::vector<double> aab(h->nr + 1), ccb(h->nr + 1);   // poisson.F90:379-381
     const double dr = h->rmax / h->nr;
     for (int j = 1; j <= h->nr; j++) {
@@ -287,8 +296,8 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
       ccb[j] = -1.0 / (dr * dr) - 1.0 / (2.0 * h->hv[0][j] * dr);
     }
     if ((rc = dalloc(h, &h->d_aab, aab.size())) || (rc = dalloc(h, &h->d_ccb, ccb.size()))) return rc;
-    HIPCHK(h, hipMemcpy(h->d_aab, aab.data(), aab.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->d_ccb, ccb.data(), ccb.size() * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = h2d(h, h->d_aab, aab.data(), aab.size() * sizeof(double)))) return rc;
+    if ((rc = h2d(h, h->d_ccb, ccb.data(), ccb.size() * sizeof(double)))) return rc;
   }
   // fields
   const size_t H = haloed(h), C = compact(h), W = wsize(h), PL = (size_t)h->nt * (h->nr + 3);
@@ -307,12 +316,12 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
   if ((rc = upload_conv(h, f.ib, ib.data(), ib.size(), true))) return rc;
   if (precision == VRH_MIXED) {
     if ((rc = dalloc(h, &f.S64, (size_t)5 * h->nr * h->nz)) || (rc = dalloc(h, &f.ib64, ib64.size()))) return rc;
-    HIPCHK(h, hipMemcpy(f.ib64, ib64.data(), ib64.size() * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = h2d(h, f.ib64, ib64.data(), ib64.size() * sizeof(double)))) return rc;
   }
   std::vector<double> mint(9 * 27);
   host_mint(h, mint.data());
   if ((rc = dalloc(h, &h->d_mint, mint.size()))) return rc;
-  HIPCHK(h, hipMemcpy(h->d_mint, mint.data(), mint.size() * sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = h2d(h, h->d_mint, mint.data(), mint.size() * sizeof(double)))) return rc;
   for (size_t q = 0; q < mint.size() && q < 243; q++) h->h_mint[q] = mint[q];
   if ((rc = dalloc(h, &h->d_sc, 16))) return rc;
   HIPCHK(h, hipHostMalloc((void **)&h->h_pin, 16 * sizeof(double), hipHostMallocDefault));
@@ -570,7 +579,7 @@ int build_line_coefs(vrh_t *h) {
       L.n = (int)lcv[irk][s][0].size();
       L.P = (int)lcv[irk][s][5].size();
     }
-  HIPCHK(h, hipMemcpy(h->d_line, lb.data(), tot * sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = h2d(h, h->d_line, lb.data(), tot * sizeof(double)))) return rc;
   return 0;
 }
 

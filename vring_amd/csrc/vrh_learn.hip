@@ -952,6 +952,14 @@ template <typename T> int halloc(vrh_hz_t *h, T **p, size_t n) {
   return 0;
 }
 
+// Host-to-device copy on the handle's stream, complete on return: ordered
+// after the zero fills halloc queues there (a null-stream hipMemcpy is not).
+int hz_h2d(vrh_hz_t *h, void *dst, const void *src, size_t bytes) {
+  HZCHK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+  HZCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
 dim3 red_grid(const vrh_hz_t *h) { return dim3(kRedBlocks, h->S); }
 
 int upload_host(vrh_hz_t *h, double *d, const double *src) {
@@ -1340,11 +1348,11 @@ int vrh_hz_create(const vrh_hz_grid *gr, vrh_hz_t **out) {
   if ((rc = halloc(h, &h->d_rm, nr)) || (rc = halloc(h, &h->d_rf, nr + 1)) || (rc = halloc(h, &h->d_dz, nz)) ||
       (rc = halloc(h, &h->d_izm, nz + 1)) || (rc = halloc(h, &h->d_zm, nz)) || (rc = halloc(h, &h->d_m, h->S)))
     return rc;
-  HZCHK(h, hipMemcpy(h->d_rm, h->rm.data(), nr * sizeof(double), hipMemcpyHostToDevice));
-  HZCHK(h, hipMemcpy(h->d_rf, rf.data(), (nr + 1) * sizeof(double), hipMemcpyHostToDevice));
-  HZCHK(h, hipMemcpy(h->d_dz, h->dz.data(), nz * sizeof(double), hipMemcpyHostToDevice));
-  HZCHK(h, hipMemcpy(h->d_izm, izm.data(), (nz + 1) * sizeof(double), hipMemcpyHostToDevice));
-  HZCHK(h, hipMemcpy(h->d_zm, h->zm.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = hz_h2d(h, h->d_rm, h->rm.data(), nr * sizeof(double)))) return rc;
+  if ((rc = hz_h2d(h, h->d_rf, rf.data(), (nr + 1) * sizeof(double)))) return rc;
+  if ((rc = hz_h2d(h, h->d_dz, h->dz.data(), nz * sizeof(double)))) return rc;
+  if ((rc = hz_h2d(h, h->d_izm, izm.data(), (nz + 1) * sizeof(double)))) return rc;
+  if ((rc = hz_h2d(h, h->d_zm, h->zm.data(), nz * sizeof(double)))) return rc;
   h->hm.assign(h->S, 0);
   // z eigenproblem Z v = lambda W v  ->  T = W^-1/2 Z W^-1/2 (symmetric tridiagonal)
   {
@@ -1357,8 +1365,8 @@ int vrh_hz_create(const vrh_hz_grid *gr, vrh_hz_t **out) {
     for (int q = 0; q < nz; q++)
       for (int k = 0; k < nz; k++) V[(size_t)q * nz + k] = U[(size_t)q * nz + k] / std::sqrt(h->dz[k]);
     if ((rc = halloc(h, &h->d_V, (size_t)nz * nz)) || (rc = halloc(h, &h->d_lam, nz))) return rc;
-    HZCHK(h, hipMemcpy(h->d_V, V.data(), V.size() * sizeof(double), hipMemcpyHostToDevice));
-    HZCHK(h, hipMemcpy(h->d_lam, lam.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+    if ((rc = hz_h2d(h, h->d_V, V.data(), V.size() * sizeof(double)))) return rc;
+    if ((rc = hz_h2d(h, h->d_lam, lam.data(), nz * sizeof(double)))) return rc;
   }
   // work vectors
   h->vec.assign(12, nullptr);

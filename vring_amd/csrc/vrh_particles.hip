@@ -651,17 +651,19 @@ int vrh_particles_upload(vrh_t *h, int batch, long n, const double *u, const int
     for (int v = 0; v < nvar; v++) soa[(size_t)v * B.cap + p] = u[(size_t)p * nvar + v];
   std::vector<int> al(B.cap, 0);
   for (long p = 0; p < n; p++) al[p] = alive ? alive[p] : 1;
-  if (hipMemcpy(B.u, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(B.alive, al.data(), al.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+  // in order on the handle's (non-blocking) stream, behind any step still
+  // running there; the host vectors live until the stream has drained
+  bool ok = hipMemcpyAsync(B.u, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, h->stream) ==
+                hipSuccess &&
+            hipMemcpyAsync(B.alive, al.data(), al.size() * sizeof(int), hipMemcpyHostToDevice, h->stream) ==
+                hipSuccess &&
+            hipMemcpyAsync(B.d_n, &n, sizeof(long), hipMemcpyHostToDevice, h->stream) == hipSuccess;
+  if (ok && B.ddt2) ok = hipMemsetAsync(B.ddt2, 0, sizeof(double) * nvar * B.cap, h->stream) == hipSuccess;
+  if (hipStreamSynchronize(h->stream) != hipSuccess || !ok) {
     h->err = "particle upload failed";
     return -3;
   }
-  if (B.ddt2) hipMemset(B.ddt2, 0, sizeof(double) * nvar * B.cap);
   B.n = n;
-  if (hipMemcpy(B.d_n, &n, sizeof(long), hipMemcpyHostToDevice) != hipSuccess) {
-    h->err = "particle upload failed";
-    return -3;
-  }
   return 0;
 }
 
@@ -672,9 +674,9 @@ int vrh_particles_download(vrh_t *h, int batch, long n, double *u, int *alive) {
   const int nvar = h->pscheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
   std::vector<double> soa((size_t)nvar * B.cap);
   std::vector<int> al(B.cap);
-  if (hipStreamSynchronize(h->stream) != hipSuccess ||
-      hipMemcpy(soa.data(), B.u, soa.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(al.data(), B.alive, al.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+  if (hipMemcpyAsync(soa.data(), B.u, soa.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipMemcpyAsync(al.data(), B.alive, al.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess) {
     h->err = "particle download failed";
     return -3;
   }
