@@ -1132,42 +1132,66 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
   off = al16(off + sizeof(TW) * N);
   int *jk = reinterpret_cast<int *>(smem_raw + off);           // (j, k) of each line
   const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
-  if (tid < LB) {
-    const long l = l0 + tid;
+  if (tid < LB) {  // lines past the end are clamped to the last line (never stored)
+    const long l = min(l0 + tid, nlines - 1);
     jk[2 * tid] = (int)(l % g.nr) + 1;
     jk[2 * tid + 1] = (int)(l / g.nr) + 1;
   }
   if (pow2)
     for (int q = tid; q < N / 2; q += nth) { twl[q] = (TW)twr[q]; twl[N / 2 + q] = (TW)twi[q]; }
   __syncthreads();
-  #pragma unroll 4
-  for (int e = tid; e < LB * N; e += nth) {
-    const int lp = pow2 ? e >> logn : e / N, ii = e - lp * N;
-    const long l = l0 + lp;
-    double val = 0.0;
-    if (l < nlines) {
-      const int j = jk[2 * lp], k = jk[2 * lp + 1], i = ii + 1;
-      if (ext) {
-        val = ext[cidx(g, i, j, k)];
-      } else {
-        const int ip = (i == N) ? 1 : i + 1;
-        const long o = hidx(g, i, j, k);
-        const double t1 = (dq1[o + (ip - i)] - dq1[o]) * g.dx1 * g.axsym / c.rm[j];
-        const double t2 = (dq2[o + g.sj] - dq2[o]) * g.dx2 / (c.rm[j] * 1.0);
-        const double t3 = (dq3[o + g.sk] - dq3[o]) * g.dx3;
-        val = -(t1 + t2 + t3) * inv_adt;
-      }
-    }
-    dv[e] = val;
+  const int tot = LB * N;
+  // the double copy feeds the direct DFT (non power of two) and the FP64 modes
+  const bool keep_dv = !pow2 || S64 != nullptr;
+  auto put = [&](int e, int lp, int ii, double val) {
+    if (keep_dv) dv[e] = val;
     if (pow2) {
-      const int q = lp >> 1;
-      const int pos = q * N + ii;
+      const int pos = (lp >> 1) * N + ii;
       if (lp & 1) im[pos] = (TW)val; else re[pos] = (TW)val;
+    }
+  };
+  if (ext) {
+    for (int e = tid; e < tot; e += nth) {
+      const int lp = pow2 ? e >> logn : e / N, ii = e - lp * N;
+      const double val = (l0 + lp < nlines) ? ext[cidx(g, ii + 1, jk[2 * lp], jk[2 * lp + 1])] : 0.0;
+      put(e, lp, ii, val);
+    }
+  } else {
+    // the loads of U elements are all issued before the first is used;
+    // indices are clamped to valid cells, so the loads are unconditional
+    constexpr int U = 4;
+    for (int eb = tid; eb < tot; eb += U * nth) {
+      double a1[U], b1[U], a2[U], b2[U], a3[U], b3[U], rmj[U];
+      #pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = min(eb + u * nth, tot - 1);
+        const int lp = pow2 ? e >> logn : e / N, i = e - lp * N + 1;
+        const int j = jk[2 * lp], k = jk[2 * lp + 1];
+        const long o = hidx(g, i, j, k);
+        a1[u] = dq1[o];
+        b1[u] = dq1[o + ((i == N) ? 1 - N : 1)];
+        a2[u] = dq2[o];
+        b2[u] = dq2[o + g.sj];
+        a3[u] = dq3[o];
+        b3[u] = dq3[o + g.sk];
+        rmj[u] = c.rm[j];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      #pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = eb + u * nth;
+        if (e < tot) {
+          const int lp = pow2 ? e >> logn : e / N, ii = e - lp * N;
+          const double t1 = (b1[u] - a1[u]) * g.dx1 * g.axsym / rmj[u];
+          const double t2 = (b2[u] - a2[u]) * g.dx2 / (rmj[u] * 1.0);
+          const double t3 = (b3[u] - a3[u]) * g.dx3;
+          put(e, lp, ii, (l0 + lp < nlines) ? -(t1 + t2 + t3) * inv_adt : 0.0);
+        }
+      }
     }
   }
   __syncthreads();
   if (pow2) lds_fft(re, im, LB / 2, logn, N, twl, twl + N / 2, (TW)-1, tid, nth);
-  const int nhalf = N / 2 + 1;
   for (int e = tid; e < LB * N; e += nth) {
     const int lp = pow2 ? e >> logn : e / N, pos = e - lp * N;
     const long l = l0 + lp;
@@ -1194,7 +1218,6 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
     if (pos == 0) outv = Fr / N;
     else if (pos & 1) outv = 2.0 * Fr / N;
     else outv = -2.0 * Fi / N;
-    (void)nhalf;
     S[(size_t)pos + (size_t)N * l] = (TW)outv;
   }
   if (S64) {
@@ -1203,17 +1226,21 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
     // order (deterministic)
     for (int q = tid; q < N; q += nth) { tw[q] = twr[q]; tw[N + q] = twi[q]; }
     __syncthreads();
-    const int nch = 8, chl = (N + nch - 1) / nch;
+    // chunk ch takes the angles ch, ch + 8, ch + 16, ...: the 8 chunks of a
+    // (line, coefficient) read 8 consecutive doubles (no LDS bank conflicts)
+    const int nch = 8;
     for (int e = tid; e < LB * 5 * nch; e += nth) {
       const int lp = e / (5 * nch), rem = e - lp * 5 * nch, pos = rem / nch, ch = rem - pos * nch;
       const int k = (pos + 1) >> 1;
-      const bool im = pos > 0 && (pos & 1) == 0;
+      const double *twp = (pos > 0 && (pos & 1) == 0) ? tw + N : tw;
+      const double *x = dv + (size_t)lp * N;
+      const int dph = (nch * k) % N;
       double acc = 0.0;
-      const int j0 = ch * chl, j1 = min(N, j0 + chl);
-      int ph = (int)(((long)j0 * k) % N);
-      for (int jj = j0; jj < j1; ++jj) {
-        acc += dv[lp * N + jj] * (im ? tw[N + ph] : tw[ph]);
-        ph += k;
+      int ph = (ch * k) % N;
+      #pragma unroll 4
+      for (int jj = ch; jj < N; jj += nch) {
+        acc += x[jj] * twp[ph];
+        ph += dph;
         if (ph >= N) ph -= N;
       }
       part[e] = acc;
