@@ -173,9 +173,7 @@ namespace {
 int build_graph(vrh_t *h) {
   graph_invalidate(h);
   HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-  h->capturing = true;
   int rc = step_body(h);
-  h->capturing = false;
   hipGraph_t gr = nullptr;
   hipError_t e = hipStreamEndCapture(h->stream, &gr);
   if (rc) { if (gr) hipGraphDestroy(gr); return rc; }
@@ -637,9 +635,6 @@ extern "C" int vrh_autotune(vrh_t *h, int reps, int *choice) {
   if (reps < 1) reps = 3;
   const bool kt = h->kt_on;
   h->kt_on = false;
-  // time the sweeps as the steps will run them: in order when steps are
-  // captured into graphs, forked onto the side streams when eager
-  h->sweeps_serial = h->use_graph;
   const int ns[6] = {h->nr, h->nr, h->nrc, h->nzc, h->nzc, h->nzc};
   for (int sw = 0; sw < 6; sw++) {
     int best = h->Msw[sw];
@@ -648,14 +643,14 @@ extern "C" int vrh_autotune(vrh_t *h, int reps, int *choice) {
       if ((ns[sw] + M - 1) / M > 16) continue;
       h->Msw[sw] = M;
       int rc = build_line_coefs(h);
-      if (rc) { h->kt_on = kt; h->sweeps_serial = false; return rc; }
+      if (rc) { h->kt_on = kt; return rc; }
       double t = h->prec == VRH_MIXED ? time_sweeps<float>(h, reps) : time_sweeps<double>(h, reps);
       if (t < bestt) { bestt = t; best = M; }
     }
     h->Msw[sw] = best;
   }
   int rc = build_line_coefs(h);
-  if (rc) { h->kt_on = kt; h->sweeps_serial = false; return rc; }
+  if (rc) { h->kt_on = kt; return rc; }
   const bool pow2t = (h->nt & (h->nt - 1)) == 0;
   if (pow2t) {
     int best = 0; double bestt = 1e30;
@@ -687,7 +682,6 @@ extern "C" int vrh_autotune(vrh_t *h, int reps, int *choice) {
   }
   graph_invalidate(h);
   h->kt_on = kt;
-  h->sweeps_serial = false;
   if (choice) {
     for (int sw = 0; sw < 6; sw++) choice[sw] = h->Msw[sw];
     choice[6] = h->fft_lines;
@@ -797,9 +791,7 @@ int ens_build(vrh_ens_t *e) {
   for (size_t q = 0; q < e->m.size() && rc == 0; q++) {
     vrh_t *h = e->m[q];
     if (hipStreamWaitEvent(h->stream, e->fork, 0) != hipSuccess) { rc = -3; break; }
-    h->capturing = true;
     rc = step_body(h);
-    h->capturing = false;
     if (rc) { e->err = "member " + std::to_string(q) + ": " + h->err; break; }
     if (hipEventRecord(e->join[q], h->stream) != hipSuccess ||
         hipStreamWaitEvent(e->origin, e->join[q], 0) != hipSuccess) rc = -3;

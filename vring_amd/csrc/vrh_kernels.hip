@@ -1119,15 +1119,14 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
   const int N = g.nt, tid = threadIdx.x, nth = blockDim.x;
   const long nlines = (long)g.nr * g.nz;
   const long l0 = (long)blockIdx.x * LB;
-  // LDS: dv (LB*N double) | re, im ((LB/2)*N TW each) | [tw (2N double), part (LB*40 double)] | twl (N TW) | jk (2 LB int)
+  // LDS: dv (LB*N double) | re, im ((LB/2)*N TW each) | [tw (2N double)] | twl (N TW) | jk (2 LB int)
   size_t off = al16(sizeof(double) * LB * N);
   double *dv = reinterpret_cast<double *>(smem_raw);
   TW *re = reinterpret_cast<TW *>(smem_raw + off);
   TW *im = re + (size_t)(LB / 2) * N;
   off = al16(off + sizeof(TW) * LB * N);
   double *tw = reinterpret_cast<double *>(smem_raw + off);     // cos (N) then sin (N), S64 only
-  double *part = tw + 2 * (size_t)N;                           // LB * 5 * 8, S64 only
-  if (S64) off = al16(off + sizeof(double) * (2 * (size_t)N + (size_t)LB * 40));
+  if (S64) off = al16(off + sizeof(double) * 2 * (size_t)N);
   TW *twl = reinterpret_cast<TW *>(smem_raw + off);            // FFT twiddles: N/2 cos, N/2 sin
   off = al16(off + sizeof(TW) * N);
   int *jk = reinterpret_cast<int *>(smem_raw + off);           // (j, k) of each line
@@ -1221,39 +1220,40 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
     S[(size_t)pos + (size_t)N * l] = (TW)outv;
   }
   if (S64) {
-    // FP64 DFT of the lowest modes m=0,1,2 from the FP64 divergence: 8 fixed
-    // chunks per (line, coefficient), partial sums in LDS, summed in a fixed
-    // order (deterministic)
+    // FP64 DFT of the lowest modes m = 0, 1, 2 from the FP64 divergence.
+    // 32 lanes per line: lane c accumulates the angles c, c + 32, ... (the
+    // lanes of a line read consecutive LDS words) for all five packed
+    // coefficients, then a fixed butterfly over the 32 lanes sums them (a
+    // deterministic order); lane p < 5 stores packed position p.
     for (int q = tid; q < N; q += nth) { tw[q] = twr[q]; tw[N + q] = twi[q]; }
     __syncthreads();
-    // chunk ch takes the angles ch, ch + 8, ch + 16, ...: the 8 chunks of a
-    // (line, coefficient) read 8 consecutive doubles (no LDS bank conflicts)
-    const int nch = 8;
-    for (int e = tid; e < LB * 5 * nch; e += nth) {
-      const int lp = e / (5 * nch), rem = e - lp * 5 * nch, pos = rem / nch, ch = rem - pos * nch;
-      const int k = (pos + 1) >> 1;
-      const double *twp = (pos > 0 && (pos & 1) == 0) ? tw + N : tw;
+    const int c = tid & 31;
+    for (int lp = tid >> 5; lp < LB; lp += nth >> 5) {
       const double *x = dv + (size_t)lp * N;
-      const int dph = (nch * k) % N;
-      double acc = 0.0;
-      int ph = (ch * k) % N;
-      #pragma unroll 4
-      for (int jj = ch; jj < N; jj += nch) {
-        acc += x[jj] * twp[ph];
-        ph += dph;
-        if (ph >= N) ph -= N;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+      for (int q = c; q < N; q += 32) {
+        const double v = x[q];
+        int q2 = 2 * q;
+        if (q2 >= N) q2 -= N;
+        s0 += v;
+        s1 += v * tw[q];
+        s2 += v * tw[N + q];
+        s3 += v * tw[q2];
+        s4 += v * tw[N + q2];
       }
-      part[e] = acc;
-    }
-    __syncthreads();
-    for (int e = tid; e < LB * 5; e += nth) {
-      const int lp = e / 5, pos = e - lp * 5;
+      for (int o = 16; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o, 32);
+        s1 += __shfl_xor(s1, o, 32);
+        s2 += __shfl_xor(s2, o, 32);
+        s3 += __shfl_xor(s3, o, 32);
+        s4 += __shfl_xor(s4, o, 32);
+      }
       const long l = l0 + lp;
-      if (l >= nlines || pos >= N) continue;
-      double acc = 0.0;
-      for (int ch = 0; ch < nch; ++ch) acc += part[e * nch + ch];
-      const double o64 = (pos == 0) ? acc / N : 2.0 * acc / N;  // t(1), t(2k), t(2k+1) packing
-      S64[(size_t)pos + 5 * l] = o64;
+      if (c < 5 && c < N && l < nlines) {
+        // t(1), t(2k), t(2k+1) packing: cos and sin sums of wavenumber k
+        const double sc = c == 0 ? s0 : c == 1 ? s1 : c == 2 ? s2 : c == 3 ? s3 : s4;
+        S64[(size_t)c + 5 * l] = (c == 0) ? sc / N : 2.0 * sc / N;
+      }
     }
   }
 }
@@ -1911,13 +1911,11 @@ template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const d
   a.c3s = c3s;
   a.c3s_dev = c3s_dev;
   a.corr3n = h->f.corr3n;
-  // The three components' chains are independent: launched eagerly, q3 and
-  // q2 run on the side streams, forked from and joined back into the
-  // handle's stream.  Under graph capture (and with the kernel timer on) all
-  // run in order on the handle's stream: captured steps with this fork gave
-  // results that differed from run to run, so the graph keeps the chains
-  // sequential.
-  const bool par = !h->kt_on && !h->capturing && !h->sweeps_serial;
+  // The three components' chains are independent: q3 and q2 run on the
+  // side streams, forked from and joined back into the handle's stream
+  // (captured steps record the fork as graph branches).  With the kernel
+  // timer on all run in order on the handle's stream.
+  const bool par = !h->kt_on;
   hipStream_t s1 = par ? h->side[0] : h->stream, s2 = par ? h->side[1] : h->stream;
   const int ie = (irk >= 1 && irk <= 3) ? irk : 0;
   if (par) {
@@ -1981,7 +1979,7 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   double *S64 = (h->prec == VRH_MIXED) ? h->f.S64 : nullptr;
   const double adt = h->rk[irk].a * h->dt;
   size_t sh1 = al16(sizeof(double) * LB * N) + al16(sizeof(TW) * LB * N);
-  if (S64) sh1 = al16(sh1 + sizeof(double) * (2 * (size_t)N + (size_t)LB * 40));
+  if (S64) sh1 = al16(sh1 + sizeof(double) * 2 * (size_t)N);
   sh1 = al16(sh1 + sizeof(TW) * N) + sizeof(int) * 2 * LB;
   { KTimer kt(h, KT_DIVFFT);
   hipLaunchKernelGGL(k_divfft<TW>, dim3((unsigned)((nlines + LB - 1) / LB)), dim3(256), sh1, h->stream, g, h->coef,
