@@ -74,6 +74,19 @@ int vrh_download_state(vrh_t *h, const char *name, double *host);
 int vrh_set_time(vrh_t *h, double ntime);
 int vrh_get_time(vrh_t *h, double *ntime);
 
+/* Non-uniform radial mesh through the reference's radial metric factors.
+ * mesh_tools.F90:325-346 sets rm(j) = (j - 1/2) dr, rc(j) = (j - 1) dr and
+ * dmr = dcr = 1; momentum_coef.F90:24-69, divergence.F90:39, 132, 327 and
+ * momentum_terms_swirl.F90:248 read them.  Four arrays over j = 0..nr+1
+ * (nr + 2 values): cell centres rm, faces rc (rc(1) = 0 on the axis), and
+ * dr/dxi of the mapping r(xi) at the centres (dmr) and faces (dcr), xi
+ * uniform with spacing rmax/nr (the derivatives keep dx2 = nr/rmax).  The
+ * radial Poisson rows become minus divq's radial difference of uppres'
+ * radial gradient, which is poisson.F90:379-381 on the uniform mesh.
+ * Particles and vrh_analysis need the uniform mesh; call before creating
+ * particles. */
+int vrh_set_radial_mesh(vrh_t *h, const double *rm, const double *rc, const double *dmr, const double *dcr);
+
 /* One RK substep irk=1..3 at the current time (loop.F90:71-127). */
 int vrh_substep(vrh_t *h, int irk);
 /* nsteps full steps: ntime += dt, then substeps 1,2,3 (loop.F90:53-131). */

@@ -19,18 +19,23 @@ program ref_harness
   use divergence
   implicit none
   type(run) :: var
-  integer :: ierr, nt, nr, nz, nph, i, ph, irk
+  integer :: ierr, nt, nr, nz, nph, i, ph, irk, hasmesh
   integer, allocatable :: phases(:)
   real(kind=8) :: rmax, zmax, Re, dt, slip, ntime
+  real(kind=8), allocatable :: mrm(:), mrc(:), mdmr(:), mdcr(:)
   character(len=512) :: fin, fout
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
   open(unit=10, file=trim(fin), access='stream', form='unformatted', status='old')
-  read(10) nt, nr, nz, nph
+  read(10) nt, nr, nz, nph, hasmesh
   read(10) rmax, zmax, Re, dt, slip, ntime
   allocate(phases(2*nph))
   read(10) phases
+  ! optional radial mesh (rm, rc, dmr, dcr over j = 0..nr+1) written into
+  ! the mesh module's arrays after create_mesh_a
+  allocate(mrm(0:nr+1), mrc(0:nr+1), mdmr(0:nr+1), mdcr(0:nr+1))
+  if (hasmesh /= 0) read(10) mrm, mrc, mdmr, mdcr
 
   open(unit=11, file='inp.txt', form='formatted', status='replace')
   write(11,'(a)') '===Comm==='
@@ -73,6 +78,12 @@ program ref_harness
   call create_comm(var%com, var%con)
   call create_para(var%par, var%com, var%con)
   call create_mesh_a(var%msh, var%par, var%com, var%con)
+  if (hasmesh /= 0) then
+    var%msh%rm(0:nr+1) = mrm
+    var%msh%rc(0:nr+1) = mrc
+    var%msh%dmr(0:nr+1) = mdmr
+    var%msh%dcr(0:nr+1) = mdcr
+  end if
   call alct_solp(var%sp, var%msh)
   call alct_solu(var%su, var%msh, var%com)
   call zero_all()

@@ -180,7 +180,8 @@ double *orc_field(orc_t *o, const char *name, long *count) {
       {"csort", o->csort, npl}, {"tconv", o->tconv, npl},
       {"rhs", o->rhs, (long)o->nt * o->nr * o->nz}, {"sol", o->sol, (long)o->nt * o->nr * o->nz},
       {"dph", o->dph, (long)o->nt * (o->nr + 2) * (o->nz + 2)},
-      {"rm", o->rm, o->nr + 2}, {"rc", o->rc, o->nr + 2}, {"zm", o->zm, o->nz + 2},
+      {"rm", o->rm, o->nr + 2}, {"rc", o->rc, o->nr + 2}, {"dmr", o->dmr, o->nr + 2},
+      {"dcr", o->dcr, o->nr + 2}, {"zm", o->zm, o->nz + 2},
       {"zc", o->zc, o->nz + 2}, {"amj1", o->amj1, o->nr + 3}, {"acj1", o->acj1, o->nr + 3},
       {"apj1", o->apj1, o->nr + 3}, {"amj2", o->amj2, o->nr + 3}, {"acj2", o->acj2, o->nr + 3},
       {"apj2", o->apj2, o->nr + 3}, {"amj3", o->amj3, o->nr + 3}, {"acj3", o->acj3, o->nr + 3},
@@ -196,6 +197,28 @@ double *orc_field(orc_t *o, const char *name, long *count) {
 }
 
 void orc_set_time(orc_t *o, double ntime) { o->ntime = ntime; }
+
+/* Radial mesh through the reference's metric hooks: rm, rc, dmr, dcr over
+ * j = 0..nr+1 replace the uniform mesh of mesh_tools.F90:325-346; every
+ * routine here reads them where the reference does (ucoeff, coesys, hdnbc's
+ * da, divq, upsol, uppres).  The Poisson radial rows, which
+ * poisson.F90:379-381 writes for the uniform mesh only, become minus divq's
+ * radial difference (divergence.F90:39) of uppres' radial gradient
+ * (divergence.F90:132): the operator that leaves the projected field
+ * solenoidal on this mesh, equal to the reference's rows on the uniform
+ * mesh.  Pinned by the reference's own modules with the same hooks (every
+ * phase but the Poisson solve) and by that solenoidal property. */
+void orc_set_radial_mesh(orc_t *o, const double *rm, const double *rc, const double *dmr, const double *dcr) {
+  for (int j = 0; j <= o->nr + 1; j++) {
+    o->rm[j] = rm[j]; o->rc[j] = rc[j]; o->dmr[j] = dmr[j]; o->dcr[j] = dcr[j];
+  }
+  for (int j = 1; j <= o->nr; j++) {
+    double w = o->dx2q / (o->rm[j] * o->dmr[j]);
+    o->aa_base[j] = -w * o->rc[j] / o->dcr[j];
+    o->cc_base[j] = -w * o->rc[j + 1] / o->dcr[j + 1];
+    o->bb_base[j] = -(o->aa_base[j] + o->cc_base[j]);
+  }
+}
 double orc_get_time(const orc_t *o) { return o->ntime; }
 
 /* ---------------------------------------------------------------------------

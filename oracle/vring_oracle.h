@@ -35,6 +35,8 @@ void orc_destroy(orc_t *o);
 /* Pointer to an internal array by name; *count receives its element count. */
 double *orc_field(orc_t *o, const char *name, long *count);
 void orc_set_time(orc_t *o, double ntime);
+/* non-uniform radial mesh (rm, rc, dmr, dcr over j = 0..nr+1) */
+void orc_set_radial_mesh(orc_t *o, const double *rm, const double *rc, const double *dmr, const double *dcr);
 double orc_get_time(const orc_t *o);
 int orc_phase(orc_t *o, int phase, int irk);
 int orc_substep(orc_t *o, int irk);

@@ -46,6 +46,7 @@ def lib():
         ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
         L.orc_poisson.argtypes = [P, dp, dp]
         L.orc_divergence.argtypes = [P, dp, dp, dp, dp]
+        L.orc_set_radial_mesh.argtypes = [P, dp, dp, dp, dp]
         L.orc_part_interp.argtypes = [P, I, dp, dp, dp, dp]
         L.orc_part_advance.argtypes = [P, I, I, I, D, D, I, ctypes.c_long, dp, dp, dp, ip]
         L.orc_part_trilinear.argtypes = [P, dp, dp]
@@ -99,6 +100,11 @@ class Oracle:
     def set_time(self, t):
         self.L.orc_set_time(self.h, t)
 
+    def set_radial_mesh(self, rm, rc, dmr, dcr):
+        a = [np.ascontiguousarray(np.asarray(x, dtype=np.float64)) for x in (rm, rc, dmr, dcr)]
+        assert all(x.size == self.nr + 2 for x in a)
+        self.L.orc_set_radial_mesh(self.h, *a)
+
     def time(self):
         return self.L.orc_get_time(self.h)
 
@@ -118,16 +124,35 @@ class Oracle:
         return sol.reshape((self.nt, self.nr, self.nz), order="F")
 
 
-def run_ref_harness(nt, nr, nz, rmax, zmax, Re, dt, slip, ntime, phase_list, state):
-    """Run the reference modules on `state` (dict of Fortran-ordered arrays)."""
+def sinh_radial_mesh(nr, rmax, s):
+    """Radial mesh r(xi) = rmax sinh(s xi / rmax) / sinh(s), xi uniform on
+    [0, rmax] with spacing rmax / nr (cells cluster at the axis): rm, rc at
+    xi = (j - 1/2) dxi and (j - 1) dxi, dmr, dcr = dr/dxi there, j = 0..nr+1."""
+    dxi = rmax / nr
+    j = np.arange(nr + 2)
+    xm, xc = (j - 0.5) * dxi, (j - 1.0) * dxi
+    f = lambda x: rmax * np.sinh(s * x / rmax) / np.sinh(s)  # noqa: E731
+    fp = lambda x: s * np.cosh(s * x / rmax) / np.sinh(s)  # noqa: E731
+    rc = f(xc)
+    rc[1] = 0.0
+    return f(xm), rc, fp(xm), fp(xc)
+
+
+def run_ref_harness(nt, nr, nz, rmax, zmax, Re, dt, slip, ntime, phase_list, state, mesh=None):
+    """Run the reference modules on `state` (dict of Fortran-ordered arrays);
+    `mesh` = (rm, rc, dmr, dcr) over j = 0..nr+1 fills the mesh module's
+    radial arrays after they are built."""
     assert os.path.exists(REF_HARNESS), "oracle/_ref/ref_harness not built"
     sh = shapes(nt, nr, nz)
     with tempfile.TemporaryDirectory() as d:
         fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
         with open(fin, "wb") as f:
-            np.array([nt, nr, nz, len(phase_list)], np.int32).tofile(f)
+            np.array([nt, nr, nz, len(phase_list), 0 if mesh is None else 1], np.int32).tofile(f)
             np.array([rmax, zmax, Re, dt, slip, ntime], np.float64).tofile(f)
             np.array([x for p in phase_list for x in (PHASES[p[0]], p[1])], np.int32).tofile(f)
+            if mesh is not None:
+                for a in mesh:
+                    np.asarray(a, np.float64).ravel().tofile(f)
             for name, s in sh:
                 np.asarray(state[name], np.float64).ravel(order="F").tofile(f)
         subprocess.run([REF_HARNESS, fin, fout], cwd=d, check=True, stdout=subprocess.DEVNULL)

@@ -10,12 +10,14 @@ import os
 import numpy as np
 import pytest
 
-from oracle_lib import REF_HARNESS, Oracle, run_ref_harness, shapes
+from oracle_lib import REF_HARNESS, Oracle, run_ref_harness, shapes, sinh_radial_mesh
 
 CASE = dict(nt=7, nr=10, nz=14, rmax=4.0, zmax=12.0, Re=1000.0, dt=0.01, slip=0.0, ntime=0.15)
 PHASE_SEQ = [(p, irk) for irk in (1, 2, 3) for p in
              ["ucoeff", "coesys", "hdnbc", "hdnl1", "hdnl2", "hdnl3", "step", "divq", "upsol", "uppres"]]
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "ref_substep.npz")
+GOLDEN_MESH = os.path.join(os.path.dirname(__file__), "golden", "ref_substep_mesh.npz")
+MESH_STRETCH = 1.5   # sinh-stretched radial mesh for the metric-hook cases
 
 
 def make_state(seed=0):
@@ -27,9 +29,11 @@ def make_state(seed=0):
     return st
 
 
-def run_oracle(st, seq, slip=0.0):
+def run_oracle(st, seq, slip=0.0, mesh=None):
     c = CASE
     o = Oracle(c["nt"], c["nr"], c["nz"], c["rmax"], c["zmax"], c["Re"], c["dt"], slip)
+    if mesh is not None:
+        o.set_radial_mesh(*mesh)
     for n, _ in shapes(c["nt"], c["nr"], c["nz"]):
         o.field(n)[...] = st[n]
     o.set_time(c["ntime"])
@@ -80,3 +84,26 @@ def test_planted_mistake_is_caught():
     o = run_oracle(st2, PHASE_SEQ[10:17])
     with pytest.raises(AssertionError):
         np.testing.assert_allclose(o.field("dq1"), ref["dq1"], rtol=1e-12, atol=0)
+
+
+def test_oracle_on_stretched_mesh_matches_committed_reference_outputs():
+    """The metric hooks (rm, rc, dmr, dcr) filled with a sinh-stretched
+    radial mesh: every phase but the Poisson solve against the reference's
+    own modules run with the same mesh."""
+    c = CASE
+    mesh = sinh_radial_mesh(c["nr"], c["rmax"], MESH_STRETCH)
+    ref = dict(np.load(GOLDEN_MESH))
+    o = run_oracle(make_state(5), PHASE_SEQ, 0.0, mesh)
+    _compare(o, ref)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_HARNESS), reason="oracle/_ref not built")
+@pytest.mark.parametrize("seed,slip", [(6, 0.0), (7, 1.0)])
+def test_oracle_on_stretched_mesh_matches_reference_modules_live(seed, slip):
+    c = CASE
+    mesh = sinh_radial_mesh(c["nr"], c["rmax"], MESH_STRETCH)
+    st = make_state(seed)
+    ref = run_ref_harness(c["nt"], c["nr"], c["nz"], c["rmax"], c["zmax"], c["Re"], c["dt"], slip,
+                          c["ntime"], PHASE_SEQ, st, mesh=mesh)
+    o = run_oracle(st, PHASE_SEQ, slip, mesh)
+    _compare(o, ref)

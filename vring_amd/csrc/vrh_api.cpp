@@ -170,6 +170,53 @@ void graph_invalidate(vrh_t *h) {
 }
 
 namespace {
+// Everything that depends on the radial mesh, recomputed on the host and
+// uploaded into the handle's buffers: the metric vectors, the theta PCR
+// depths, the line-sweep coefficients, the radial Poisson rows and pivots,
+// the inlet area.
+int refresh_mesh(vrh_t *h) {
+  host_mesh_vectors(h);
+  const int nv = h->nr + 3;
+  int rc;
+  std::vector<double> vb((size_t)kNumVec * nv);
+  for (int q = 0; q < kNumVec; q++) std::memcpy(&vb[(size_t)q * nv], h->hv[q].data(), nv * sizeof(double));
+  if ((rc = h2d(h, h->d_vec, vb.data(), vb.size() * sizeof(double)))) return rc;
+  // parallel-cyclic-reduction depth of the theta solves: log2(nt) for a
+  // power-of-two nt, otherwise until the off-diagonal ratio is below 1e-18
+  for (int irk = 1; irk <= 3; irk++) {
+    const double betadx = 0.5 * h->rk[irk].a * h->dt / h->Re;
+    int lv = 0;
+    if ((h->nt & (h->nt - 1)) == 0) {
+      while ((1 << lv) < h->nt) lv++;
+    } else {
+      for (int j = 1; j <= h->nr; j++)
+        for (int which = 0; which < 2; which++) {
+          double acc = betadx * h->hv[which ? 14 : 10][j];
+          double a = -acc, b = 1.0 + 2.0 * acc;
+          int l = 0;
+          while (fabs(a) >= 1e-18 * fabs(b) && l < 40) { double al = -a / b; b += 2.0 * al * a; a *= al; l++; }
+          if (l > lv) lv = l;
+        }
+    }
+    h->theta_levels[irk] = lv;
+  }
+  if ((rc = build_line_coefs(h))) return rc;
+  std::vector<double> aab(h->nr + 1, 0.0), ccb(h->nr + 1, 0.0);
+  for (int j = 1; j <= h->nr; j++) { double dd; host_radial_row(h, j, aab[j], ccb[j], dd); }
+  if ((rc = h2d(h, h->d_aab, aab.data(), aab.size() * sizeof(double)))) return rc;
+  if ((rc = h2d(h, h->d_ccb, ccb.data(), ccb.size() * sizeof(double)))) return rc;
+  std::vector<double> ib, ib64;
+  host_radial_pivots(h, ib, ib64);
+  if ((rc = upload_conv(h, h->f.ib, ib.data(), ib.size(), true))) return rc;
+  if (h->f.ib64 && (rc = h2d(h, h->f.ib64, ib64.data(), ib64.size() * sizeof(double)))) return rc;
+  h->inlet_area = 0.0;  // momentum_terms_swirl.F90:261-267 with c3s factored out
+  for (int j = 1; j <= h->jlim; j++)
+    for (int i = 1; i <= h->nt; i++) h->inlet_area = h->inlet_area + h->da[j];
+  return 0;
+}
+}  // namespace
+
+namespace {
 int build_graph(vrh_t *h) {
   graph_invalidate(h);
   HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -224,39 +271,16 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
     HIPCHK(h, hipEventCreateWithFlags(&h->fork_ev[irk], hipEventDisableTiming));
     for (int q = 0; q < 2; q++) HIPCHK(h, hipEventCreateWithFlags(&h->join_ev[irk][q], hipEventDisableTiming));
   }
-  host_mesh_vectors(h);
-  // radial vectors
+  // radial vectors (values from refresh_mesh below)
   const int nv = h->nr + 3;
-  int rc = dalloc(h, &h->d_vec, (size_t)21 * nv);
+  int rc = dalloc(h, &h->d_vec, (size_t)kNumVec * nv);
   if (rc) return rc;
-  std::vector<double> vb((size_t)21 * nv);
-  for (int q = 0; q < 21; q++) std::memcpy(&vb[(size_t)q * nv], h->hv[q].data(), nv * sizeof(double));
-  if ((rc = h2d(h, h->d_vec, vb.data(), vb.size() * sizeof(double)))) return rc;
-  const double **cv[21] = {&h->coef.rm, &h->coef.rc, &h->coef.h1d11, &h->coef.h1d22d, &h->coef.h1d22n,
-                           &h->coef.h1dare, &h->coef.h2d11, &h->coef.h2d22m, &h->coef.h2d22p, &h->coef.h2dare,
-                           &h->coef.s1t11, &h->coef.s1t2n, &h->coef.s1t2s, &h->coef.s1t2c, &h->coef.s2t11,
-                           &h->coef.s2t2p, &h->coef.s2t2m, &h->coef.s2t2c, &h->coef.s3t2n, &h->coef.s3t2s,
-                           &h->coef.qc};
-  for (int q = 0; q < 21; q++) *cv[q] = h->d_vec + (size_t)q * nv;
-  // parallel-cyclic-reduction depth of the theta solves: log2(nt) for a
-  // power-of-two nt, otherwise until the off-diagonal ratio is below 1e-18
-  for (int irk = 1; irk <= 3; irk++) {
-    const double betadx = 0.5 * h->rk[irk].a * h->dt / h->Re;
-    int lv = 0;
-    if ((h->nt & (h->nt - 1)) == 0) {
-      while ((1 << lv) < h->nt) lv++;
-    } else {
-      for (int j = 1; j <= h->nr; j++)
-        for (int which = 0; which < 2; which++) {
-          double acc = betadx * h->hv[which ? 14 : 10][j];
-          double a = -acc, b = 1.0 + 2.0 * acc;
-          int l = 0;
-          while (fabs(a) >= 1e-18 * fabs(b) && l < 40) { double al = -a / b; b += 2.0 * al * a; a *= al; l++; }
-          if (l > lv) lv = l;
-        }
-    }
-    h->theta_levels[irk] = lv;
-  }
+  const double **cv[kNumVec] = {&h->coef.rm, &h->coef.rc, &h->coef.h1d11, &h->coef.h1d22d, &h->coef.h1d22n,
+                                &h->coef.h1dare, &h->coef.h2d11, &h->coef.h2d22m, &h->coef.h2d22p, &h->coef.h2dare,
+                                &h->coef.s1t11, &h->coef.s1t2n, &h->coef.s1t2s, &h->coef.s1t2c, &h->coef.s2t11,
+                                &h->coef.s2t2p, &h->coef.s2t2m, &h->coef.s2t2c, &h->coef.s3t2n, &h->coef.s3t2s,
+                                &h->coef.qc, &h->coef.dmr, &h->coef.dcr};
+  for (int q = 0; q < kNumVec; q++) *cv[q] = h->d_vec + (size_t)q * nv;
   // line coefficients for the three substeps; default segment lengths
   // minimise the padded work P*M (P <= 16 waves per block)
   {
@@ -272,7 +296,6 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
       h->Msw[sw] = best;
     }
   }
-  if ((rc = build_line_coefs(h))) return rc;
   // transform tables
   {
     std::vector<double> tt(2 * (size_t)h->nt), tz(2 * (size_t)h->nz), ez(2 * (size_t)h->nz);
@@ -287,15 +310,7 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
     if ((rc = h2d(h, h->d_twt, tt.data(), tt.size() * sizeof(double)))) return rc;
     if ((rc = h2d(h, h->d_twz, tz.data(), tz.size() * sizeof(double)))) return rc;
     if ((rc = h2d(h, h->d_ez, ez.data(), ez.size() * sizeof(double)))) return rc;
-    std::vector<double> aab(h->nr + 1), ccb(h->nr + 1);   // poisson.F90:379-381
-    const double dr = h->rmax / h->nr;
-    for (int j = 1; j <= h->nr; j++) {
-      aab[j] = -1.0 / (dr * dr) + 1.0 / (2.0 * h->hv[0][j] * dr);
-      ccb[j] = -1.0 / (dr * dr) - 1.0 / (2.0 * h->hv[0][j] * dr);
-    }
-    if ((rc = dalloc(h, &h->d_aab, aab.size())) || (rc = dalloc(h, &h->d_ccb, ccb.size()))) return rc;
-    if ((rc = h2d(h, h->d_aab, aab.data(), aab.size() * sizeof(double)))) return rc;
-    if ((rc = h2d(h, h->d_ccb, ccb.data(), ccb.size() * sizeof(double)))) return rc;
+    if ((rc = dalloc(h, &h->d_aab, (size_t)h->nr + 1)) || (rc = dalloc(h, &h->d_ccb, (size_t)h->nr + 1))) return rc;
   }
   // fields
   const size_t H = haloed(h), C = compact(h), W = wsize(h), PL = (size_t)h->nt * (h->nr + 3);
@@ -309,13 +324,9 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
   double **pls[] = {&f.csort, &f.dq3no, &f.corr3n};
   for (auto p2 : pls) if ((rc = dalloc(h, p2, PL))) return rc;
   if ((rc = dalloc(h, &f.flag, 1))) return rc;
-  std::vector<double> ib, ib64;
-  host_radial_pivots(h, ib, ib64);
-  if ((rc = upload_conv(h, f.ib, ib.data(), ib.size(), true))) return rc;
-  if (precision == VRH_MIXED) {
-    if ((rc = dalloc(h, &f.S64, (size_t)5 * h->nr * h->nz)) || (rc = dalloc(h, &f.ib64, ib64.size()))) return rc;
-    if ((rc = h2d(h, f.ib64, ib64.data(), ib64.size() * sizeof(double)))) return rc;
-  }
+  if (precision == VRH_MIXED)
+    if ((rc = dalloc(h, &f.S64, (size_t)5 * h->nr * h->nz)) || (rc = dalloc(h, &f.ib64, (size_t)5 * h->nr * h->nz)))
+      return rc;
   std::vector<double> mint(9 * 27);
   host_mint(h, mint.data());
   if ((rc = dalloc(h, &h->d_mint, mint.size()))) return rc;
@@ -323,9 +334,7 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
   for (size_t q = 0; q < mint.size() && q < 243; q++) h->h_mint[q] = mint[q];
   if ((rc = dalloc(h, &h->d_sc, 16))) return rc;
   HIPCHK(h, hipHostMalloc((void **)&h->h_pin, 16 * sizeof(double), hipHostMallocDefault));
-  h->inlet_area = 0.0;  // momentum_terms_swirl.F90:261-267 with c3s factored out
-  for (int j = 1; j <= h->jlim; j++)
-    for (int i = 1; i <= h->nt; i++) h->inlet_area = h->inlet_area + h->da[j];
+  if ((rc = refresh_mesh(h))) return rc;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
@@ -426,6 +435,24 @@ int vrh_set_time(vrh_t *h, double t) {
   return 0;
 }
 int vrh_get_time(vrh_t *h, double *t) { if (!h || !t) return -1; *t = h->ntime; return 0; }
+
+int vrh_set_radial_mesh(vrh_t *h, const double *rm, const double *rc, const double *dmr, const double *dcr) {
+  if (!h) return -1;
+  if (!rm || !rc || !dmr || !dcr) return fail(h, "radial mesh: null array");
+  if (!h->pb.empty()) return fail(h, "radial mesh: particles need the uniform mesh (set the mesh first)");
+  const int nr = h->nr;
+  bool ok = rc[1] == 0.0;
+  for (int j = 1; j <= nr; j++)
+    ok = ok && rc[j] < rm[j] && rm[j] < rc[j + 1] && dmr[j] > 0.0 && dcr[j] > 0.0;
+  ok = ok && dcr[nr + 1] > 0.0 && rm[nr + 1] > rm[nr];
+  if (!ok)
+    return fail(h, "radial mesh: need rc(1) = 0, rc(j) < rm(j) < rc(j+1), rm(nr+1) > rm(nr), dmr, dcr > 0");
+  const double *src[4] = {rm, rc, dmr, dcr};
+  for (int q = 0; q < 4; q++) h->mesh[q].assign(src[q], src[q] + nr + 2);
+  int rc2 = refresh_mesh(h);
+  if (rc2) return rc2;
+  return 0;
+}
 
 int vrh_substep(vrh_t *h, int irk) {
   if (!h) return -1;
@@ -906,6 +933,7 @@ extern "C" int vrh_set_tuning(vrh_t *h, const int *choice) {
 // vortmax, swrlmax, then stat(1:8).
 extern "C" int vrh_analysis(vrh_t *h, double *out) {
   if (!h || !out) return -1;
+  if (!h->mesh[0].empty()) return fail(h, "vrh_analysis: statistics window needs the uniform radial mesh");
   const int nblk = 1024;
   double *part = nullptr;
   HIPCHK(h, hipMalloc(&part, sizeof(double) * (nblk * 10 + 16)));

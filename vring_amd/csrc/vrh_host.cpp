@@ -9,41 +9,49 @@
 #include "vrh_internal.h"
 
 // Radial metric vectors (momentum_coef.F90:24-69) and inlet/outlet weights
-// (momentum_terms_swirl.F90:247-251).  dmr = dcr = 1 (mesh_tools.F90:325-330).
+// (momentum_terms_swirl.F90:247-251) on the radial mesh (rm, rc, dmr, dcr):
+// the uniform one of mesh_tools.F90:325-346 (dmr = dcr = 1), or the one given
+// to vrh_set_radial_mesh, entering through the reference's metric factors.
 void host_mesh_vectors(vrh_t *h) {
   const int nr = h->nr;
   const VGrid &g = h->g;
   for (auto &v : h->hv) v.assign(nr + 3, 0.0);
-  std::vector<double> &rm = h->hv[0], &rc = h->hv[1];
+  std::vector<double> &rm = h->hv[0], &rc = h->hv[1], &dmr = h->hv[21], &dcr = h->hv[22];
   for (int j = 0; j <= nr + 1; j++) {
-    rm[j] = 0.0 + (j - 0.5) * (h->rmax / nr);
-    rc[j] = 0.0 + (j - 1.0) * (h->rmax / nr);
+    if (h->mesh[0].empty()) {
+      rm[j] = 0.0 + (j - 0.5) * (h->rmax / nr);
+      rc[j] = 0.0 + (j - 1.0) * (h->rmax / nr);
+      dmr[j] = 1.0;
+      dcr[j] = 1.0;
+    } else {
+      rm[j] = h->mesh[0][j]; rc[j] = h->mesh[1][j]; dmr[j] = h->mesh[2][j]; dcr[j] = h->mesh[3][j];
+    }
   }
   for (int j = 1; j <= nr; j++) {
     h->hv[2][j] = 0.25 * g.dx1 / (rm[j] * g.axsym);
-    h->hv[3][j] = 0.25 * g.dx2 / (rm[j] * 1.0);
+    h->hv[3][j] = 0.25 * g.dx2 / (rm[j] * dmr[j]);
     h->hv[4][j] = 0.125 / (rm[j] * rm[j]);
     h->hv[5][j] = g.dx1 / (h->Re * (rm[j] * rm[j] * rm[j]));
     h->hv[10][j] = g.dx1q / ((rm[j] * rm[j]) * g.axsym);
-    h->hv[11][j] = (rc[j + 1] * rc[j + 1] * rc[j + 1]) / 1.0;
-    h->hv[12][j] = (rc[j] * rc[j] * rc[j]) / 1.0;
-    h->hv[13][j] = g.dx2q / (1.0 * (rm[j] * rm[j]));
-    h->hv[18][j] = g.dx2q * rc[j + 1] / (rm[j] * 1.0 * 1.0);
-    h->hv[19][j] = g.dx2q * rc[j] / (rm[j] * 1.0 * 1.0);
+    h->hv[11][j] = (rc[j + 1] * rc[j + 1] * rc[j + 1]) / dcr[j + 1];
+    h->hv[12][j] = (rc[j] * rc[j] * rc[j]) / dcr[j];
+    h->hv[13][j] = g.dx2q / (dmr[j] * (rm[j] * rm[j]));
+    h->hv[18][j] = g.dx2q * rc[j + 1] / (rm[j] * dcr[j + 1] * dmr[j]);
+    h->hv[19][j] = g.dx2q * rc[j] / (rm[j] * dcr[j] * dmr[j]);
   }
   for (int j = 2; j <= nr; j++) {
     h->hv[6][j] = 0.25 * g.dx1 / rc[j] * g.axsym;
-    h->hv[7][j] = 0.25 * g.dx2 / (rm[j - 1] * 1.0);
-    h->hv[8][j] = 0.25 * g.dx2 / (rm[j] * 1.0);
+    h->hv[7][j] = 0.25 * g.dx2 / (rm[j - 1] * dcr[j]);
+    h->hv[8][j] = 0.25 * g.dx2 / (rm[j] * dcr[j]);
     h->hv[9][j] = g.dx1 / (h->Re * rc[j] * g.axsym);
     h->hv[14][j] = g.axsym * g.dx1q / (rc[j] * rc[j]);
-    h->hv[15][j] = g.dx2q * rc[j] / rm[j] / (1.0 * 1.0);
-    h->hv[16][j] = g.dx2q * rc[j] / rm[j - 1] / (1.0 * 1.0);
+    h->hv[15][j] = g.dx2q * rc[j] / rm[j] / (dcr[j] * dmr[j]);
+    h->hv[16][j] = g.dx2q * rc[j] / rm[j - 1] / (dcr[j] * dmr[j - 1]);
     h->hv[17][j] = h->hv[15][j] + h->hv[16][j];
   }
   h->da.assign(nr + 3, 0.0);
   for (int j = 1; j <= nr; j++) {
-    h->da[j] = rm[j] * 1.0 / (g.dx1 * g.dx2);
+    h->da[j] = rm[j] * dmr[j] / (g.dx1 * g.dx2);
     double x = rm[j] / h->rmax;
     h->hv[20][j] = 1.0 - x * x;  // qc(nr+1) stays 0
   }
@@ -104,7 +112,7 @@ void host_line_coefs(vrh_t *h, int irk, std::vector<double> out[6][7], const int
   a22m = betadx * s1t2c[nr] * s1t2s[nr];
   ap[nr] = 0.0;
   ac[nr] = 1.0 + a22m / rm[nr] -
-           betadx * s1t2c[nr] * rc[nrc] * rc[nrc] * (rc[nrc] / rm[nr] * slip - rc[nrc] / rm[nrc]) / 1.0;
+           betadx * s1t2c[nr] * rc[nrc] * rc[nrc] * (rc[nrc] / rm[nr] * slip - rc[nrc] / rm[nrc]) / h->hv[22][nrc];
   am[nr] = -a22m / rm[nr - 1];
   for (int j = 2; j <= nr - 1; j++) {
     a22p = betadx * s1t2c[j] * s1t2n[j];
@@ -152,12 +160,33 @@ void host_line_coefs(vrh_t *h, int irk, std::vector<double> out[6][7], const int
   }
 }
 
+// Row j (1..nr) of the radial Poisson operator: sub- and super-diagonal and
+// the radial part of the diagonal.  Uniform mesh: poisson.F90:379-381.  A
+// mesh set by vrh_set_radial_mesh: minus divq's radial difference
+// (divergence.F90:39, with dmr) of uppres' radial gradient (divergence.F90:132,
+// with dcr), so that the projection leaves a discretely solenoidal field on
+// that mesh; on the uniform mesh this is the same operator (rc = rm -+ dr/2).
+void host_radial_row(const vrh_t *h, int j, double &aa, double &cc, double &dd) {
+  const std::vector<double> &rm = h->hv[0], &rc = h->hv[1], &dmr = h->hv[21], &dcr = h->hv[22];
+  if (h->mesh[0].empty()) {
+    const double dr = h->rmax / h->nr;
+    aa = -1.0 / (dr * dr) + 1.0 / (2.0 * rm[j] * dr);
+    cc = -1.0 / (dr * dr) - 1.0 / (2.0 * rm[j] * dr);
+    dd = 2.0 / (dr * dr);
+  } else {
+    const double w = h->g.dx2q / (rm[j] * dmr[j]);
+    aa = -w * rc[j] / dcr[j];
+    cc = -w * rc[j + 1] / dcr[j + 1];
+    dd = -(aa + cc);
+  }
+}
+
 // Inverse Thomas pivots of every (mode, kz) radial system of the Poisson
 // solve (poisson.F90:379-381; poisson_solve.F90:284-319), in the spectral
 // layout (position, j, kz).  The operator does not change during a run.
 void host_radial_pivots(vrh_t *h, std::vector<double> &ib, std::vector<double> &ib64) {
   const int nt = h->nt, nr = h->nr, nz = h->nz;
-  const double dr = h->rmax / nr, dz = h->zmax / nz, dth = 2.0 * acos(-1.0) / nt, pi = acos(-1.0);
+  const double dz = h->zmax / nz, dth = 2.0 * acos(-1.0) / nt, pi = acos(-1.0);
   const std::vector<double> &rm = h->hv[0];
   std::vector<double> aa(nr + 1), bb(nr + 1), cc(nr + 1), beta(nr + 1);
   ib.assign((size_t)nt * nr * nz, 0.0);
@@ -171,9 +200,9 @@ void host_radial_pivots(vrh_t *h, std::vector<double> &ib, std::vector<double> &
                        ? (double)((float)m * (float)m)
                        : 2.0 / (dth * dth) * (1.0 - cos(2.0 * pi * m / nt));
       for (int j = 1; j <= nr; j++) {
-        aa[j] = -1.0 / (dr * dr) + 1.0 / (2.0 * rm[j] * dr);
-        cc[j] = -1.0 / (dr * dr) - 1.0 / (2.0 * rm[j] * dr);
-        bb[j] = 2.0 / (dr * dr) + wth / (rm[j] * rm[j]) + wz;
+        double dd;
+        host_radial_row(h, j, aa[j], cc[j], dd);
+        bb[j] = dd + wth / (rm[j] * rm[j]) + wz;
       }
       int bcr = (m + mz != 0) ? 1 : -1;
       bb[1] = bb[1] + 1 * aa[1];

@@ -27,8 +27,10 @@ VRH_HD long cidx(const VGrid &g, int i, int j, int k) {
 // the Fortran index j.
 struct VCoef {
   const double *rm, *rc, *h1d11, *h1d22d, *h1d22n, *h1dare, *h2d11, *h2d22m, *h2d22p, *h2dare,
-      *s1t11, *s1t2n, *s1t2s, *s1t2c, *s2t11, *s2t2p, *s2t2m, *s2t2c, *s3t2n, *s3t2s, *qc;
+      *s1t11, *s1t2n, *s1t2s, *s1t2c, *s2t11, *s2t2p, *s2t2m, *s2t2c, *s3t2n, *s3t2s, *qc,
+      *dmr, *dcr;  // radial metric factors at centres and faces (mesh_tools.F90:325-330)
 };
+constexpr int kNumVec = 23;  // vectors of VCoef, in that order
 struct VRK {
   double g, r, a;  // gamma, rho, alpha of the substep (para_tools.F90:103-113)
 };
@@ -110,8 +112,11 @@ struct vrh_s {
   hipEvent_t join_ev[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
   std::string err;
   // host copies of the radial vectors
-  std::vector<double> hv[21];
-  double *d_vec;  // device block holding the 21 vectors
+  std::vector<double> hv[kNumVec];
+  double *d_vec;  // device block holding the kNumVec vectors
+  // radial mesh set by vrh_set_radial_mesh (rm, rc, dmr, dcr over j = 0..nr+1);
+  // empty: the uniform mesh with dmr = dcr = 1
+  std::vector<double> mesh[4];
   VCoef coef;
   // line coefficients: [irk 1..3][sweep 0..5]
   double *d_line;
@@ -173,6 +178,7 @@ void ktimer_harvest(vrh_t *h);
 void host_mesh_vectors(vrh_t *h);
 void host_line_coefs(vrh_t *h, int irk, std::vector<double> out[6][7], const int M[6]);
 int build_line_coefs(vrh_t *h);
+void host_radial_row(const vrh_t *h, int j, double &aa, double &cc, double &dd);
 void host_radial_pivots(vrh_t *h, std::vector<double> &ib, std::vector<double> &ib64);
 double host_profile(double t);
 void host_mint(vrh_t *h, double *mint);

@@ -1119,8 +1119,13 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
   const int N = g.nt, tid = threadIdx.x, nth = blockDim.x;
   const long nlines = (long)g.nr * g.nz;
   const long l0 = (long)blockIdx.x * LB;
-  // LDS: dv (LB*N double) | re, im ((LB/2)*N TW each) | [tw (2N double)] | twl (N TW) | jk (2 LB int)
-  size_t off = al16(sizeof(double) * LB * N);
+  // LDS: [dv (LB*N double)] | re, im ((LB/2)*N TW each) | [tw (2N double)] | twl (N TW) | jk (2 LB int) |
+  // 1/rm, dx2/dmr (LB double each).  The double copy dv feeds the direct DFT
+  // (ntheta not a power of two) and the FP64 low modes; the FP64 FFT path
+  // does without it (half the LDS, twice the blocks per CU).
+  const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
+  const bool keep_dv = !pow2 || S64 != nullptr;
+  size_t off = keep_dv ? al16(sizeof(double) * LB * N) : 0;
   double *dv = reinterpret_cast<double *>(smem_raw);
   TW *re = reinterpret_cast<TW *>(smem_raw + off);
   TW *im = re + (size_t)(LB / 2) * N;
@@ -1130,18 +1135,22 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
   TW *twl = reinterpret_cast<TW *>(smem_raw + off);            // FFT twiddles: N/2 cos, N/2 sin
   off = al16(off + sizeof(TW) * N);
   int *jk = reinterpret_cast<int *>(smem_raw + off);           // (j, k) of each line
-  const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
+  // 1/rm(j) and dx2/dmr(j) of each line: divq's theta and r differences
+  // (divergence.F90:38-39) share the 1/rm factor, so a cell needs no division
+  double *lirm = reinterpret_cast<double *>(smem_raw + al16(off + sizeof(int) * 2 * LB));
+  double *lcx2 = lirm + LB;
   if (tid < LB) {  // lines past the end are clamped to the last line (never stored)
     const long l = min(l0 + tid, nlines - 1);
-    jk[2 * tid] = (int)(l % g.nr) + 1;
+    const int j = (int)(l % g.nr) + 1;
+    jk[2 * tid] = j;
     jk[2 * tid + 1] = (int)(l / g.nr) + 1;
+    lirm[tid] = 1.0 / c.rm[j];
+    lcx2[tid] = g.dx2 / c.dmr[j];
   }
   if (pow2)
     for (int q = tid; q < N / 2; q += nth) { twl[q] = (TW)twr[q]; twl[N / 2 + q] = (TW)twi[q]; }
   __syncthreads();
   const int tot = LB * N;
-  // the double copy feeds the direct DFT (non power of two) and the FP64 modes
-  const bool keep_dv = !pow2 || S64 != nullptr;
   auto put = [&](int e, int lp, int ii, double val) {
     if (keep_dv) dv[e] = val;
     if (pow2) {
@@ -1160,7 +1169,7 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
     // indices are clamped to valid cells, so the loads are unconditional
     constexpr int U = 4;
     for (int eb = tid; eb < tot; eb += U * nth) {
-      double a1[U], b1[U], a2[U], b2[U], a3[U], b3[U], rmj[U];
+      double a1[U], b1[U], a2[U], b2[U], a3[U], b3[U], rmv[U], rdv[U];
       #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int e = min(eb + u * nth, tot - 1);
@@ -1173,7 +1182,8 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
         b2[u] = dq2[o + g.sj];
         a3[u] = dq3[o];
         b3[u] = dq3[o + g.sk];
-        rmj[u] = c.rm[j];
+        rmv[u] = lirm[lp];
+        rdv[u] = lcx2[lp];
       }
       __builtin_amdgcn_sched_barrier(0);
       #pragma unroll
@@ -1181,10 +1191,9 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
         const int e = eb + u * nth;
         if (e < tot) {
           const int lp = pow2 ? e >> logn : e / N, ii = e - lp * N;
-          const double t1 = (b1[u] - a1[u]) * g.dx1 * g.axsym / rmj[u];
-          const double t2 = (b2[u] - a2[u]) * g.dx2 / (rmj[u] * 1.0);
+          const double t12 = ((b1[u] - a1[u]) * (g.dx1 * g.axsym) + (b2[u] - a2[u]) * rdv[u]) * rmv[u];
           const double t3 = (b3[u] - a3[u]) * g.dx3;
-          put(e, lp, ii, (l0 + lp < nlines) ? -(t1 + t2 + t3) * inv_adt : 0.0);
+          put(e, lp, ii, (l0 + lp < nlines) ? -(t12 + t3) * inv_adt : 0.0);
         }
       }
     }
@@ -1616,13 +1625,15 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
   const int jr = (j <= nrc) ? j : nrc;           // clamped row for coefficient loads
   const int jq = (jr >= 1 && jr <= nr) ? jr : 1;  // rows of the uppres coefficients
   const int djp = (jq == nr) ? 0 : 1, djm = (jq == 1) ? 0 : -1;
-  const double a1 = adt * g.dx1 / c.rm[jq], a2 = adt * c.rc[jr] * g.dx2, a3 = adt * g.dx3;
+  // radial gradient rc dx2 / dcr (divergence.F90:132, 327)
+  const double a1 = adt * g.dx1 / c.rm[jq], a2 = adt * c.rc[jr] * g.dx2 / c.dcr[jr], a3 = adt * g.dx3;
   const double irm = 1.0 / c.rm[jq], irmp = 1.0 / c.rm[jq + djp], irmm = 1.0 / c.rm[jq + djm];
   const double irmn = 1.0 / c.rm[jq + 1];
   const double s1t11 = c.s1t11[jq], s1t2n = c.s1t2n[jq], s1t2s = c.s1t2s[jq], s1t2c = c.s1t2c[jq];
   const double s2t11 = c.s2t11[jq], s2t2p = c.s2t2p[jq], s2t2m = c.s2t2m[jq];
   const double s3t2n = c.s3t2n[jq], s3t2s = c.s3t2s[jq];
-  const double rcj = c.rc[jq], rcp = c.rc[min(jq + 1, nrc)], rcm = c.rc[max(jq - 1, 1)];
+  const int jqp = min(jq + 1, nrc), jqm = max(jq - 1, 1);
+  const double rcj = c.rc[jq] / c.dcr[jq], rcp = c.rc[jqp] / c.dcr[jqp], rcm = c.rc[jqm] / c.dcr[jqm];
   const double dx1 = g.dx1, dx2 = g.dx2, dx3 = g.dx3, dx3q = g.dx3q, slip = g.slip;
   const int cx = tx + 2, cy = ty + 2;
   // the streamed operands of the next plane are loaded while this one is computed
@@ -1668,7 +1679,7 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
         q1[o] = v;
         if (j == nr) q1[o + sj] = slip * v;
       }
-      if (j >= 2 && j <= nr) q2[o] = c_dq2 - a2 * (p0 - P0[cy - 1][cx]) / 1.0;
+      if (j >= 2 && j <= nr) q2[o] = c_dq2 - a2 * (p0 - P0[cy - 1][cx]);
       if (j == 1 || j == nrc) q2[o] = 0.0;
       if (k == 1) q3[o] = c_dq3;
       if (k >= 2 && j >= 1 && j <= nr) {
@@ -1978,9 +1989,10 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   TW *S = (TW *)h->f.S;
   double *S64 = (h->prec == VRH_MIXED) ? h->f.S64 : nullptr;
   const double adt = h->rk[irk].a * h->dt;
-  size_t sh1 = al16(sizeof(double) * LB * N) + al16(sizeof(TW) * LB * N);
+  const bool keep_dv = !pow2 || S64 != nullptr;   // the kernel's FP64 copy of the divergence
+  size_t sh1 = (keep_dv ? al16(sizeof(double) * LB * N) : 0) + al16(sizeof(TW) * LB * N);
   if (S64) sh1 = al16(sh1 + sizeof(double) * 2 * (size_t)N);
-  sh1 = al16(sh1 + sizeof(TW) * N) + sizeof(int) * 2 * LB;
+  sh1 = al16(al16(sh1 + sizeof(TW) * N) + sizeof(int) * 2 * LB) + sizeof(double) * 2 * LB;
   { KTimer kt(h, KT_DIVFFT);
   hipLaunchKernelGGL(k_divfft<TW>, dim3((unsigned)((nlines + LB - 1) / LB)), dim3(256), sh1, h->stream, g, h->coef,
                      1.0 / adt, h->f.dq1, h->f.dq2, h->f.dq3, ext_rhs, S, S64, h->d_twt, h->d_twt + N, LB, logn); }

@@ -30,7 +30,8 @@ class VrhParams(ctypes.Structure):
 
 
 EXPORTS = ["vrh_create", "vrh_destroy", "vrh_last_error", "vrh_upload_fields", "vrh_download_fields",
-           "vrh_upload_state", "vrh_download_state", "vrh_set_time", "vrh_get_time", "vrh_substep",
+           "vrh_upload_state", "vrh_download_state", "vrh_set_time", "vrh_get_time", "vrh_set_radial_mesh",
+           "vrh_substep",
            "vrh_step", "vrh_phase", "vrh_poisson_solve", "vrh_synchronize", "vrh_device_ptr",
            "vrh_ring_init", "vrh_cfl_dt", "vrh_particles_create", "vrh_particles_upload",
            "vrh_particles_download", "vrh_particles_step", "vrh_ktimer_enable", "vrh_ktimer_read", "vrh_autotune",
@@ -62,6 +63,7 @@ def load_library(path=LIBPATH):
     L.vrh_download_state.argtypes = [P, ctypes.c_char_p, dp]
     L.vrh_set_time.argtypes = [P, D]
     L.vrh_get_time.argtypes = [P, ctypes.POINTER(D)]
+    L.vrh_set_radial_mesh.argtypes = [P, dp, dp, dp, dp]
     L.vrh_substep.argtypes = [P, I]
     L.vrh_step.argtypes = [P, I]
     L.vrh_phase.argtypes = [P, I, I]
@@ -205,6 +207,13 @@ class Solver:
 
     def set_time(self, t):
         self._check(self.L.vrh_set_time(self.h, float(t)), "set_time")
+
+    def set_radial_mesh(self, rm, rc, dmr, dcr):
+        """Non-uniform radial mesh: rm, rc, dmr, dcr over j = 0..nr+1 (cell
+        centres, faces with rc[1] = 0, and dr/dxi at centres and faces)."""
+        arrs = [_f64(a) for a in (rm, rc, dmr, dcr)]
+        assert all(a.size == self.nr + 2 for a in arrs)
+        self._check(self.L.vrh_set_radial_mesh(self.h, *arrs), "set_radial_mesh")
 
     def time(self):
         t = ctypes.c_double(0.0)
