@@ -54,6 +54,13 @@ module vrh_f
       real(c_double), value :: t
     end function vrh_set_time
 
+    ! rm, rc, dmr, dcr over j = 0..nr+1 (the mesh module's radial arrays)
+    integer(c_int) function vrh_set_radial_mesh(h, rm, rc, dmr, dcr) bind(C, name='vrh_set_radial_mesh')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: rm(*), rc(*), dmr(*), dcr(*)
+    end function vrh_set_radial_mesh
+
     integer(c_int) function vrh_get_time(h, t) bind(C, name='vrh_get_time')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
