@@ -3,7 +3,7 @@ kernels (bit-identical), and against the FP64 oracle."""
 import numpy as np
 import pytest
 
-from oracle_lib import Oracle
+from oracle_lib import Oracle, sinh_radial_mesh
 
 import vring_amd
 
@@ -12,12 +12,14 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("q1", "q2", "q3", "pres")
 
 
-def ring_solver(nt, nr, nz, Re, precision="fp64", graph=True, amp=0.0):
+def ring_solver(nt, nr, nz, Re, precision="fp64", graph=True, amp=0.0, stretch=0.0):
     phases = np.linspace(0.3, 2.0, 3)
     q = vring_amd.ring_initial_condition(nt, nr, nz, 4.0, 12.0, [(1.0, 3.0, 0.3, 1.0)], pert_amp=amp,
                                          pert_modes=(2, 4), phases=phases)
     dt = vring_amd.cfl_dt(nt, nr, nz, 4.0, 12.0, *q, cfl=0.3, ujet=1.0)
     s = vring_amd.Solver(nt, nr, nz, 4.0, 12.0, Re, dt, precision=precision)
+    if stretch > 0.0:
+        s.set_radial_mesh(*sinh_radial_mesh(nr, 4.0, stretch))
     for n, a in zip(("q1", "q2", "q3"), q):
         s.upload(n, a)
     s.set_graph(graph)
@@ -49,14 +51,17 @@ def test_graph_replays_match_eager_and_oracle():
     se.close()
 
 
-def test_ensemble_matches_members_stepped_alone():
+@pytest.mark.parametrize("stretch", [0.0, 1.5])
+def test_ensemble_matches_members_stepped_alone(stretch):
+    """stretch > 0: every other member on a sinh-stretched radial mesh."""
     cases = [(16, 24, 48, 500.0, "fp64"), (33, 32, 64, 2000.0, "fp64"), (32, 48, 96, 1000.0, "mixed"),
              (64, 32, 128, 4000.0, "mixed")]
-    print("case ensemble", cases, flush=True)
+    print("case ensemble", cases, "stretch", stretch, flush=True)
     ens_members, alone = [], []
-    for nt, nr, nz, Re, prec in cases:
-        ens_members.append(ring_solver(nt, nr, nz, Re, prec, graph=True, amp=1e-2)[0])
-        alone.append(ring_solver(nt, nr, nz, Re, prec, graph=False, amp=1e-2)[0])
+    for q, (nt, nr, nz, Re, prec) in enumerate(cases):
+        st = stretch if q % 2 else 0.0
+        ens_members.append(ring_solver(nt, nr, nz, Re, prec, graph=True, amp=1e-2, stretch=st)[0])
+        alone.append(ring_solver(nt, nr, nz, Re, prec, graph=False, amp=1e-2, stretch=st)[0])
     ens = vring_amd.Ensemble(ens_members)
     for rep in range(3):
         ens.step(2)
@@ -74,8 +79,8 @@ def test_ensemble_matches_members_stepped_alone():
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_captured_step_after_set_time_equals_eager(precision):
     """A handle's first step after vrh_set_time, captured into a graph, equals
-    the eager step bit for bit (the eager path forks the independent sweep
-    chains onto side streams; the captured step keeps them in order)."""
+    the eager step bit for bit (both fork the independent sweep chains onto
+    the side streams; the captured step records the fork as graph branches)."""
     nt, nr, nz = 33, 32, 64
     print(f"case captured step after set_time {precision}", flush=True)
     q, dt = ring_solver(nt, nr, nz, 1000.0, precision, graph=False, amp=1e-2)[1:]
