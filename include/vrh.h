@@ -107,8 +107,8 @@ int vrh_step_async(vrh_t *h, int nsteps);
 int vrh_snapshot_begin(vrh_t *h);
 int vrh_snapshot_wait(vrh_t *h, double *q1, double *q2, double *q3, double *pres, double *ntime);
 /* vrh_step replays one RK3 step captured as a HIP graph (captured on first
- * use, recaptured after vrh_autotune or a particle change; not used while the
- * kernel timer is on).  on = 0 launches the kernels one by one instead.  No
+ * use, recaptured after vrh_autotune, vrh_set_tuning, vrh_set_radial_mesh or a
+ * particle change; not used while the kernel timer is on).  on = 0 launches the kernels one by one instead.  No
  * reference counterpart. */
 int vrh_set_graph(vrh_t *h, int on);
 /* One phase of substep irk (parity tests and profiling). */
