@@ -15,12 +15,14 @@ pytestmark = pytest.mark.gpu
 R, L = 4.0, 12.0
 
 
-@pytest.mark.parametrize("form", [0, 1, 2])
-def test_substep_phases_on_stretched_mesh_match_oracle(form):
+@pytest.mark.parametrize("form,slip", [(0, 0.0), (1, 0.0), (2, 0.0), (-1, 0.5)])
+def test_substep_phases_on_stretched_mesh_match_oracle(form, slip):
+    """form -1: the default explicit form; slip 0.5: the wall-slip rows,
+    whose q1 r-sweep coefficient carries dcr(nrc)."""
     nt, nr, nz = 16, 24, 40
-    print(f"case mesh phases form={form}", flush=True)
+    print(f"case mesh phases form={form} slip={slip}", flush=True)
     mesh = sinh_radial_mesh(nr, R, 1.5)
-    o, s = make_pair(nt, nr, nz, slip=0.0)
+    o, s = make_pair(nt, nr, nz, slip=slip)
     o.set_radial_mesh(*mesh)
     s.set_radial_mesh(*mesh)
     s.set_tuning(explicit_form=form)
