@@ -1288,61 +1288,106 @@ __global__ __launch_bounds__(512) void k_dctz(TW *__restrict__ S, long nl, int n
   const bool pow2 = n >= 2 && (n & (n - 1)) == 0;
   if (pow2) {
     for (int q = tid; q < n / 2; q += nth) { twl[q] = (TW)twr[q]; twl[n / 2 + q] = (TW)twi[q]; }
-    // loads are unconditional (line index clamped, then masked) so that the
-    // unrolled iterations issue them together
+    // batches of elements: every load of a batch (line index clamped, then
+    // masked; twiddles included) is issued before the first is used
     if (dir > 0) {
-#pragma unroll 4
-      for (int e = tid; e < TL * n; e += nth) {
-        const int lp = e & (TL - 1), kz = e >> logtl;  // consecutive threads -> consecutive lines
-        const long l = l0 + lp;
-        const int q = lp >> 1;
-        TW v = S[min(l, nl - 1) + nl * kz];
-        v = (l < nl) ? v : (TW)0;
-        const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
-        const int slot = q * sn + p;
-        if (lp & 1) im[slot] = v; else re[slot] = v;
+      constexpr int U = 8;
+      const int tot = TL * n;
+      for (int eb = tid; eb < tot; eb += U * nth) {
+        TW v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = min(eb + u * nth, tot - 1);
+          const int lp = e & (TL - 1), kz = e >> logtl;  // consecutive threads -> consecutive lines
+          v[u] = S[min(l0 + lp, nl - 1) + nl * kz];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = eb + u * nth;
+          if (e < tot) {
+            const int lp = e & (TL - 1), kz = e >> logtl;
+            const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
+            TW *dst = (lp & 1) ? im : re;
+            dst[(lp >> 1) * sn + p] = (l0 + lp < nl) ? v[u] : (TW)0;
+          }
+        }
       }
     } else {
-      const int ltq = logtl - 1, TQ = TL >> 1;
-#pragma unroll 2
-      for (int e = tid; e < TQ * n; e += nth) {
-        // W[k] = e^{i pi k/2n} ((Ya[k] + Yb[n-k]) + i (Yb[k] - Ya[n-k])), Y[n]=0
-        const int q = e & (TQ - 1), k = e >> ltq;
-        const long la = l0 + 2 * q, lb = la + 1;
-        const long ca = min(la, nl - 1), cb = min(lb, nl - 1);
-        const int kn = (n - k) & (n - 1);
-        TW ya = S[ca + nl * k], yb = S[cb + nl * k], yan = S[ca + nl * kn], ybn = S[cb + nl * kn];
-        ya = (la < nl) ? ya : (TW)0;
-        yb = (lb < nl) ? yb : (TW)0;
-        yan = (k > 0 && la < nl) ? yan : (TW)0;
-        ybn = (k > 0 && lb < nl) ? ybn : (TW)0;
-        const TW u = ya + ybn, w = yb - yan;
-        const TW cr = (TW)er[k], ci = (TW)ei[k];  // e^{+i pi k / 2n}
-        const int slot = q * sn + k;
-        re[slot] = u * cr - w * ci;
-        im[slot] = u * ci + w * cr;
+      // W[k] = e^{i pi k/2n} ((Ya[k] + Yb[n-k]) + i (Yb[k] - Ya[n-k])), Y[n]=0
+      constexpr int U = 4;
+      const int ltq = logtl - 1, TQ = TL >> 1, tot = TQ * n;
+      for (int eb = tid; eb < tot; eb += U * nth) {
+        TW ya[U], yb[U], yan[U], ybn[U];
+        double cr[U], ci[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = min(eb + u * nth, tot - 1);
+          const int q = e & (TQ - 1), k = e >> ltq;
+          const long la = l0 + 2 * q;
+          const long ca = min(la, nl - 1), cb = min(la + 1, nl - 1);
+          const int kn = (n - k) & (n - 1);
+          ya[u] = S[ca + nl * k];
+          yb[u] = S[cb + nl * k];
+          yan[u] = S[ca + nl * kn];
+          ybn[u] = S[cb + nl * kn];
+          cr[u] = er[k];  // e^{+i pi k / 2n}
+          ci[u] = ei[k];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int e = eb + u * nth;
+          if (e < tot) {
+            const int q = e & (TQ - 1), k = e >> ltq;
+            const long la = l0 + 2 * q, lb = la + 1;
+            const TW a0 = (la < nl) ? ya[u] : (TW)0, b0 = (lb < nl) ? yb[u] : (TW)0;
+            const TW an = (k > 0 && la < nl) ? yan[u] : (TW)0, bn = (k > 0 && lb < nl) ? ybn[u] : (TW)0;
+            const TW uu = a0 + bn, w = b0 - an;
+            const TW c = (TW)cr[u], sv = (TW)ci[u];
+            const int slot = q * sn + k;
+            re[slot] = uu * c - w * sv;
+            im[slot] = uu * sv + w * c;
+          }
+        }
       }
     }
     __syncthreads();
     lds_fft(re, im, TL / 2, logn, sn, twl, twl + n / 2, (TW)(dir > 0 ? -1 : 1), tid, nth);
-    for (int e = tid; e < TL * n; e += nth) {
-      const int lp = e & (TL - 1), kz = e >> logtl;
-      const long l = l0 + lp;
-      if (l >= nl) continue;
-      const int q = lp >> 1;
-      TW outv;
+    const TW inv_n = (TW)1 / (TW)n;  // exact: n is a power of two
+    constexpr int US = 4;
+    for (int eb = tid; eb < TL * n; eb += US * nth) {
+      double cr[US], ci[US];
       if (dir > 0) {
-        const int k = kz, kr = (n - k) & (n - 1);
-        const TW zr = re[q * sn + k], zi = im[q * sn + k], wr = re[q * sn + kr], wi = im[q * sn + kr];
-        TW vr, vi;
-        if ((lp & 1) == 0) { vr = (TW)0.5 * (zr + wr); vi = (TW)0.5 * (zi - wi); }
-        else { vr = (TW)0.5 * (zi + wi); vi = (TW)(-0.5) * (zr - wr); }
-        outv = (vr * (TW)er[k] + vi * (TW)ei[k]) / (TW)n;  // Re(e^{-i pi k/2n} V)/n
-      } else {
-        const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
-        outv = (lp & 1) ? im[q * sn + p] : re[q * sn + p];
+#pragma unroll
+        for (int u = 0; u < US; ++u) {
+          const int kz = min(eb + u * nth, TL * n - 1) >> logtl;
+          cr[u] = er[kz];
+          ci[u] = ei[kz];
+        }
       }
-      S[l + nl * kz] = outv;
+#pragma unroll
+      for (int u = 0; u < US; ++u) {
+        const int e = eb + u * nth;
+        if (e >= TL * n) break;
+        const int lp = e & (TL - 1), kz = e >> logtl;
+        const long l = l0 + lp;
+        if (l >= nl) continue;
+        const int q = lp >> 1;
+        TW outv;
+        if (dir > 0) {
+          const int k = kz, kr = (n - k) & (n - 1);
+          const TW zr = re[q * sn + k], zi = im[q * sn + k], wr = re[q * sn + kr], wi = im[q * sn + kr];
+          TW vr, vi;
+          if ((lp & 1) == 0) { vr = (TW)0.5 * (zr + wr); vi = (TW)0.5 * (zi - wi); }
+          else { vr = (TW)0.5 * (zi + wi); vi = (TW)(-0.5) * (zr - wr); }
+          outv = (vr * (TW)cr[u] + vi * (TW)ci[u]) * inv_n;  // Re(e^{-i pi k/2n} V)/n
+        } else {
+          const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
+          outv = (lp & 1) ? im[q * sn + p] : re[q * sn + p];
+        }
+        S[l + nl * kz] = outv;
+      }
     }
   } else {
     // direct transform for non-power-of-two nz (small test grids)
