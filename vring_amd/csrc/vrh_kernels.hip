@@ -1528,14 +1528,29 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
     for (int q = tid; q < N; q += nth) { tw64[q] = twr[q]; tw64[N + q] = twi[q]; }
     for (int e = tid; e < 5 * LB; e += nth) s64l[e] = S64[5 * min(l0 + e / 5, nlines - 1) + e % 5];
   }
-  #pragma unroll 4
-  for (int e = tid; e < LB * N; e += nth) {
-    const int lp = pow2 ? e >> logn : e / N, pos = e - lp * N;
-    const long l = l0 + lp;
-    TW v = S[(size_t)pos + (size_t)N * min(l, nlines - 1)];
-    v = (l < nlines) ? v : (TW)0;
-    if (S64 && pos < 5) v = (TW)0;
-    tb[e] = v;
+  {
+    // batches of U elements: all loads (line clamped, then masked) go out
+    // before the first is used
+    constexpr int U = 8;
+    const int tot = LB * N;
+    for (int eb = tid; eb < tot; eb += U * nth) {
+      TW v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = min(eb + u * nth, tot - 1);
+        const int lp = pow2 ? e >> logn : e / N, pos = e - lp * N;
+        v[u] = S[(size_t)pos + (size_t)N * min(l0 + lp, nlines - 1)];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = eb + u * nth;
+        if (e < tot) {
+          const int lp = pow2 ? e >> logn : e / N, pos = e - lp * N;
+          tb[e] = (l0 + lp < nlines && !(S64 && pos < 5)) ? v[u] : (TW)0;
+        }
+      }
+    }
   }
   __syncthreads();
   if (pow2) {
