@@ -2058,7 +2058,6 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
                      1.0 / adt, h->f.dq1, h->f.dq2, h->f.dq3, ext_rhs, S, S64, h->d_twt, h->d_twt + N, LB, logn); }
   // z transforms
   const int nz = g.nz, lz = ilog2(nz);
-  // lines per block: a power of two, data + twiddles within 64 KB of LDS
   // lines per block: a power of two, data + twiddles within 64 KB of LDS,
   // one FFT butterfly per thread with up to 512 threads
   auto lines = [&](size_t es, int want) {
