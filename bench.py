@@ -324,26 +324,30 @@ def summarize_traffic(pmc_root, out_path):
     print(json.dumps(out))
 
 
-def summarize_stats(csv_path, out_path, command):
+def summarize_stats(csv_path, out_path, command, title="workload 256x128x512 ring (bench.py)", raw=False):
     """Text summary of a rocprofv3 --kernel-trace --stats kernel_stats.csv:
-    per kernel (timer name, precision) calls and average duration."""
+    per kernel (timer name, or the kernel's own name with raw=True, and
+    precision) calls and average duration."""
     import csv
     with open(csv_path) as f:
         rows = list(csv.DictReader(f))
     agg = {}
     for r in rows:
         key, prec = _kernel_key(r["Name"])
+        if raw:
+            base = r["Name"].split("(anonymous namespace)::", 1)[-1]
+            key = base.split("<")[0].split("(")[0].replace("void ", "")
         a = agg.setdefault((key, prec or "-"), [0, 0.0])
         a[0] += int(r["Calls"])
         a[1] += float(r["TotalDurationNs"])
     tot = sum(v[1] for v in agg.values())
-    lines = [f"# rocprofv3 kernel statistics, workload 256x128x512 ring (bench.py)", "",
-             f"command: `{command}`", "",
-             "Kernel names are mapped to the solver's kernel-timer names; `type` is the",
+    note = (["Kernels by their own names; `type` is the work type of the instantiation."] if raw else
+            ["Kernel names are mapped to the solver's kernel-timer names; `type` is the",
              "work type of the instantiation (in the mixed path the FP64 low-mode",
              "transforms and radial solves appear as fp64).  Calls include the",
-             "autotuner's timing launches.", "",
-             "| kernel | type | calls | avg us | total ms | % |", "|---|---|---|---|---|---|"]
+             "autotuner's timing launches."])
+    lines = [f"# rocprofv3 kernel statistics, {title}", "", f"command: `{command}`", ""] + note + [
+             "", "| kernel | type | calls | avg us | total ms | % |", "|---|---|---|---|---|---|"]
     for (key, prec), (calls, ns) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
         lines.append(f"| {key} | {prec} | {calls} | {ns / calls / 1e3:.1f} | {ns / 1e6:.2f} | {100 * ns / tot:.1f} |")
     with open(out_path, "w") as f:
@@ -369,11 +373,15 @@ def main():
                     help="run only the learned Helmholtz benchmark, training on SAMPLES pairs")
     ap.add_argument("--summarize-traffic", metavar="PMC_DIR", default=None,
                     help="write profiles/traffic_256x128x512.json from rocprofv3 --pmc output under PMC_DIR")
+    ap.add_argument("--stats-title", default="workload 256x128x512 ring (bench.py)",
+                    help="title of the --summarize-stats summary")
+    ap.add_argument("--stats-raw-names", action="store_true",
+                    help="--summarize-stats lists kernels by their own names")
     ap.add_argument("--summarize-stats", nargs=3, metavar=("CSV", "OUT", "COMMAND"), default=None,
                     help="write a text summary of a rocprofv3 kernel_stats.csv")
     args = ap.parse_args()
     if args.summarize_stats:
-        summarize_stats(*args.summarize_stats)
+        summarize_stats(*args.summarize_stats, title=args.stats_title, raw=args.stats_raw_names)
         return
     if args.summarize_traffic:
         summarize_traffic(args.summarize_traffic, os.path.join(ROOT, "profiles", "traffic_256x128x512.json"))
