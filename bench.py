@@ -337,6 +337,8 @@ def summarize_stats(csv_path, out_path, command, title="workload 256x128x512 rin
         if raw:
             base = r["Name"].split("(anonymous namespace)::", 1)[-1]
             key = base.split("<")[0].split("(")[0].replace("void ", "")
+            if key.startswith("Cijk_"):  # rocBLAS/Tensile GEMM: keep the operand layout part
+                key = key.split("_MT")[0] + " (rocBLAS GEMM)"
         a = agg.setdefault((key, prec or "-"), [0, 0.0])
         a[0] += int(r["Calls"])
         a[1] += float(r["TotalDurationNs"])
