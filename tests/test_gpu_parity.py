@@ -34,6 +34,30 @@ def test_substep_phases_match_oracle(nt, nr, nz, form):
         compare(o, s, ["dph", "q1", "q2", "q3", "pres", "prg"], 1e-11)
 
 
+@pytest.mark.parametrize("nt,nr,nz", [(3, 4, 4), (8, 512, 8), (4, 8, 1023)])
+def test_substep_phases_at_size_limits_match_oracle(nt, nr, nz):
+    """The smallest grid vrh_create accepts (ntheta = 3: direct theta DFT),
+    the largest nr (512: radial solves of 16 segments of 32 rows, r sweeps
+    of 513 rows) and the longest z lines (nz = 1023: 1024-row z sweeps of 16
+    segments, direct z DCT for a length that is not a power of two)."""
+    print(f"case phases at size limits nt={nt} nr={nr} nz={nz}", flush=True)
+    o, s = make_pair(nt, nr, nz, slip=0.0)
+    seed_random(o, s, seed=nt + nr + nz)
+    o.set_time(0.13)
+    s.set_time(0.13)
+    for irk in (1, 2, 3):
+        for p in ["ucoeff", "coesys", "hdnbc", "hdnl1", "hdnl2", "hdnl3", "step"]:
+            assert o.phase(p, irk) == 0
+        for p in ["hdnbc", "explicit", "adi"]:
+            s.phase(p, irk)
+        compare(o, s, ["dq1", "dq2", "dq3", "ru1", "ru2", "ru3"], 1e-11)
+        for p in ["divq", "laplace", "upsol", "uppres"]:
+            assert o.phase(p, irk) == 0
+        for p in ["poisson", "project"]:
+            s.phase(p, irk)
+        compare(o, s, ["dph", "q1", "q2", "q3", "pres", "prg"], 1e-11)
+
+
 @pytest.mark.parametrize("slip", [1.0, 0.5])
 def test_substep_phases_with_wall_slip_match_oracle(slip):
     """Wall slip (the q1, q3 copies into the ghost row at nrc and the slip
