@@ -124,3 +124,62 @@ def test_driver_ring_run_matches_oracle(tmp_path):
     ref2 = np.stack([o2.field(n)[:, 1 : nr + 1, 1 : nz + 1] for n in ("q1", "q2", "q3")])
     assert abs(t6 - (t5 + dt)) < 1e-12
     assert np.abs(q6 - ref2).max() <= 1e-10 * np.abs(ref2).max()
+
+
+PART_DECK = """
+===GPU===
+precision            0
+theta                0
+autotune             0
+
+===Part===
+npart              300
+ptorus             1.0 3.0 0.3
+pradius            0.0001
+nsub                 5
+pscheme              0
+pseed                7
+"""
+
+
+def read_part(path):
+    """part*.dat (part_export): two header lines, then id, time, c, u(1:nvar),
+    d(1:10) per live particle."""
+    with open(path) as f:
+        head = [f.readline(), f.readline()]
+        rows = np.loadtxt(f, ndmin=2)
+    return head, rows
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+def test_driver_particle_run_exports_particles(tmp_path):
+    """Driver with LSQ particles seeded in the ring's torus (part_tools.F90's
+    117-value state): export files at nstep = 0, 2, 4 with one row per live
+    particle, positions inside the domain, the particles carried by the
+    flow, and the deformation gradient's determinant (export column d1)
+    near 1 early on.  Radius 1e-4 gives a Stokes number near 1
+    (part_tools.F90:273), so the particles pick up the flow within the run."""
+    nt, nr, nz = 32, 32, 64
+    print("case driver particles 32x32x64", flush=True)
+    deck = DECK.format(nt=nt, nr=nr, nz=nz, totstp=4, restrt=0, dt=0.01, anlstp=0).replace(
+        "expstp               5", "expstp               2") + PART_DECK
+    (tmp_path / "inp.txt").write_text(deck)
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    out = {}
+    for n in (0, 2, 4):
+        head, rows = read_part(tmp_path / f"part{n:05d}size01.dat")
+        assert head[0].strip().startswith("title") and head[1].strip().startswith("variables")
+        assert rows.shape[1] == 3 + 117 + 10
+        out[n] = rows
+    p0, p4 = out[0], out[4]
+    assert p0.shape[0] == 300 and p4.shape[0] >= 290
+    for rows in out.values():
+        x, y, z = rows[:, 3], rows[:, 4], rows[:, 5]
+        assert np.all(np.hypot(x, y) <= 4.0) and np.all((z >= 0.0) & (z <= 12.0))
+    ids = np.intersect1d(p0[:, 2], p4[:, 2])
+    a = p0[np.isin(p0[:, 2], ids)][:, 3:6]
+    b = p4[np.isin(p4[:, 2], ids)][:, 3:6]
+    assert np.abs(b - a).max() > 1e-4          # carried by the flow
+    det0 = p0[:, 3 + 117]
+    assert np.all(np.abs(det0 - 1.0) < 0.05), (det0.min(), det0.max())
