@@ -83,8 +83,8 @@ int vrh_get_time(vrh_t *h, double *ntime);
  * uniform with spacing rmax/nr (the derivatives keep dx2 = nr/rmax).  The
  * radial Poisson rows become minus divq's radial difference of uppres'
  * radial gradient, which is poisson.F90:379-381 on the uniform mesh.
- * Particles and vrh_analysis need the uniform mesh; call before creating
- * particles. */
+ * vrh_analysis reads rm from this mesh as run.F90:300-345 reads msh%rm.
+ * Particles need the uniform mesh; call before creating particles. */
 int vrh_set_radial_mesh(vrh_t *h, const double *rm, const double *rc, const double *dmr, const double *dcr);
 
 /* One RK substep irk=1..3 at the current time (loop.F90:71-127). */

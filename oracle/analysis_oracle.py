@@ -16,10 +16,12 @@ window mean of z and r).
 import numpy as np
 
 
-def window(nr, nz, rmax, zmax):
-    """Cell centres and the mask of cells inside the statistics window."""
+def window(nr, nz, rmax, zmax, rm=None):
+    """Cell centres and the mask of cells inside the statistics window; rm
+    (j = 0..nr+1) is the radial mesh, uniform when not given."""
     dr, dz = rmax / nr, zmax / nz
-    rm = (np.arange(0, nr + 2) - 0.5) * dr   # index j = 0..nr+1
+    if rm is None:
+        rm = (np.arange(0, nr + 2) - 0.5) * dr   # index j = 0..nr+1
     zm = (np.arange(0, nz + 2) - 0.5) * dz   # index k = 0..nz+1
     j = np.arange(1, nr + 1)
     k = np.arange(1, nz + 1)
@@ -28,13 +30,14 @@ def window(nr, nz, rmax, zmax):
     return rm, zm, inr, inz
 
 
-def analysis(q1, q2, q3, nr, nz, rmax, zmax):
+def analysis(q1, q2, q3, nr, nz, rmax, zmax, rm=None):
     """Returns (vortmax, swrlmax, stat[0:8]) with stat = xvort, yvort, vvort,
-    uvort, xswrl, yswrl, vswrl, uswrl."""
+    uvort, xswrl, yswrl, vswrl, uswrl.  rm (j = 0..nr+1): a non-uniform
+    radial mesh, read where run.F90 reads msh%rm (the weights keep dx2)."""
     nt = q1.shape[0]
     dth, dr, dz = 2.0 * np.pi / nt, rmax / nr, zmax / nz
     w = (1.0 / dth) * (1.0 / dr) * (1.0 / dz)
-    rm, zm, inr, inz = window(nr, nz, rmax, zmax)
+    rm, zm, inr, inz = window(nr, nz, rmax, zmax, None if rm is None else np.asarray(rm, dtype=np.float64))
     J = np.arange(1, nr + 1)[:, None]
     K = np.arange(1, nz + 1)[None, :]
     if nt > 1:

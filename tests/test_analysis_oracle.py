@@ -75,3 +75,30 @@ def test_negative_vorticity_contributes_nothing():
     q1[...] = -1.0
     vmax, smax, st = ao.analysis(q1, q2, q3, NR, NZ, R, L)
     assert vmax < 0 and smax == -1.0 and np.all(st == 0.0)
+
+
+def test_vorticity_on_a_stretched_radial_mesh():
+    """With the radial mesh rm(j) of a sinh stretching: q3 = b rm^2 gives
+    dwdr = b (rm(j+1) + rm(j-1)) exactly, and a z-linear q2 gives
+    dvdz = a (1/rm(j) + 1/rm(j+1)) / 2; the window follows rm."""
+    a, b, s = 0.3, 0.02, 1.5
+    dxi = R / NR
+    j = np.arange(NR + 2)
+    rmesh = R * np.sinh(s * (j - 0.5) * dxi / R) / np.sinh(s)
+    q1, q2, q3 = fields()
+    dz = L / NZ
+    zc = (np.arange(NZ + 3) - 0.5) * dz
+    q2[...] = a * zc[None, None, :]
+    rext = np.append(rmesh, 2 * rmesh[-1] - rmesh[-2])   # j = nr+2 (outside the window)
+    q3[...] = b * rext[None, :, None] ** 2
+    vmax, smax, st = ao.analysis(q1, q2, q3, NR, NZ, R, L, rm=rmesh)
+    js = [jj for jj in range(2, NR + 1) if 0.05 * R < rmesh[jj] < 0.8 * R]
+    v = np.array([0.5 * a * (1 / rmesh[jj] + 1 / rmesh[jj + 1]) - b * (rmesh[jj + 1] + rmesh[jj - 1]) for jj in js])
+    assert abs(vmax - v.max()) < 1e-12 * abs(v.max())
+    ks = [k for k in range(1, NZ + 1) if 0.1 * L < (k - 0.5) * dz < 0.9 * L]
+    w = NT / (2 * np.pi) / dxi / dz
+    pos = v > 0
+    u = NT * len(ks) * w * (v[pos] ** 2).sum()
+    assert abs(st[3] - u) < 1e-12 * u
+    # the stretched window holds other rows than the uniform one
+    assert len(js) != len([jj for jj in range(2, NR + 1) if 0.05 * R < (jj - 0.5) * dxi < 0.8 * R])

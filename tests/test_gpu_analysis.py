@@ -15,9 +15,9 @@ import analysis_oracle as ao  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-def check(s, q1, q2, q3, nr, nz, R, L):
+def check(s, q1, q2, q3, nr, nz, R, L, rm=None):
     got = s.analysis()
-    vmax, smax, st = ao.analysis(q1, q2, q3, nr, nz, R, L)
+    vmax, smax, st = ao.analysis(q1, q2, q3, nr, nz, R, L, rm=rm)
     assert abs(got[0] - vmax) <= 1e-13 * abs(vmax) and abs(got[1] - smax) <= 1e-13 * abs(smax), (got[:2], vmax, smax)
     err = np.abs(got[2:] - st) / np.abs(st)
     assert np.all(err < 1e-12), (got[2:], st)
@@ -33,6 +33,22 @@ def test_analysis_matches_oracle_on_random_fields(nt, nr, nz):
     for n, a in zip(("q1", "q2", "q3"), q):
         s.upload(n, a)
     check(s, *q, nr, nz, R, L)
+    s.close()
+
+
+def test_analysis_on_a_stretched_radial_mesh():
+    """The statistics read rm from the radial mesh set on the handle."""
+    from oracle_lib import sinh_radial_mesh
+    print("case analysis stretched mesh 16x48x60", flush=True)
+    nt, nr, nz, R, L = 16, 48, 60, 4.0, 12.0
+    mesh = sinh_radial_mesh(nr, R, 1.5)
+    s = vring_amd.Solver(nt, nr, nz, R, L, 1000.0, 0.01)
+    s.set_radial_mesh(*mesh)
+    rng = np.random.default_rng(5)
+    q = [rng.standard_normal((nt, nr + 3, nz + 3)) for _ in range(3)]
+    for n, a in zip(("q1", "q2", "q3"), q):
+        s.upload(n, a)
+    check(s, *q, nr, nz, R, L, rm=mesh[0])
     s.close()
 
 

@@ -933,7 +933,6 @@ extern "C" int vrh_set_tuning(vrh_t *h, const int *choice) {
 // vortmax, swrlmax, then stat(1:8).
 extern "C" int vrh_analysis(vrh_t *h, double *out) {
   if (!h || !out) return -1;
-  if (!h->mesh[0].empty()) return fail(h, "vrh_analysis: statistics window needs the uniform radial mesh");
   const int nblk = 1024;
   double *part = nullptr;
   HIPCHK(h, hipMalloc(&part, sizeof(double) * (nblk * 10 + 16)));

@@ -1816,21 +1816,21 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
 // w^2 r, w^2 (w = vorticity where > 0, then swirl where > 0) times dx1 dx2
 // dx3.  Per-block partials [block][10], summed in a fixed order.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_analysis(VGrid g, double rmax, double zmax, const double *__restrict__ q1,
-                                                  const double *__restrict__ q2, const double *__restrict__ q3,
-                                                  double *__restrict__ part) {
+__global__ __launch_bounds__(256) void k_analysis(VGrid g, double rmax, double zmax, const double *__restrict__ rmv,
+                                                  const double *__restrict__ q1, const double *__restrict__ q2,
+                                                  const double *__restrict__ q3, double *__restrict__ part) {
   const long ncell = (long)g.nt * g.nr * g.nz;
-  const double dr = rmax / g.nr, dz = zmax / g.nz, w = g.dx1 * g.dx2 * g.dx3;
+  const double dz = zmax / g.nz, w = g.dx1 * g.dx2 * g.dx3;
   double v[10] = {-10000.0, -10000.0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (long c = (long)blockIdx.x * 256 + threadIdx.x; c < ncell; c += (long)gridDim.x * 256) {
     const int ith = (int)(c % g.nt) + 1;
     const long rest = c / g.nt;
     const int ir = (int)(rest % g.nr) + 1, iz = (int)(rest / g.nr) + 1;
-    const double rm = (ir - 0.5) * dr, zm = (iz - 0.5) * dz;
+    const double rm = rmv[ir], zm = (iz - 0.5) * dz;  // msh%rm: the radial mesh (uniform or set)
     if (!(zm < 0.90 * zmax && zm > 0.10 * zmax && rm < 0.80 * rmax && rm > rmax * 0.05 && ir != 1)) continue;
     const int ithp1 = ith == g.nt ? 1 : ith + 1;
     const double q1m = g.nt > 1 ? 0.5 * (q1[hidx(g, ith, ir, iz)] + q1[hidx(g, ithp1, ir, iz)]) : q1[hidx(g, ith, ir, iz)];
-    const double rp = (ir + 0.5) * dr, rmm = (ir - 1.5) * dr;
+    const double rp = rmv[ir + 1], rmm = rmv[ir - 1];
     const double dwdr = 0.5 * ((q3[hidx(g, ith, ir + 1, iz)] + q3[hidx(g, ith, ir + 1, iz + 1)]) -
                                (q3[hidx(g, ith, ir - 1, iz)] + q3[hidx(g, ith, ir - 1, iz + 1)])) / (rp - rmm);
     const double zp = (iz + 0.5) * dz, zmm = (iz - 1.5) * dz;
@@ -2113,8 +2113,8 @@ template <typename TW> void launch_project(vrh_t *h, int irk) {
 }
 
 void launch_analysis(vrh_t *h, double *part, int nblk, double *out) {
-  hipLaunchKernelGGL(k_analysis, dim3(nblk), dim3(256), 0, h->stream, h->g, h->rmax, h->zmax, h->f.q1, h->f.q2,
-                     h->f.q3, part);
+  hipLaunchKernelGGL(k_analysis, dim3(nblk), dim3(256), 0, h->stream, h->g, h->rmax, h->zmax, h->coef.rm, h->f.q1,
+                     h->f.q2, h->f.q3, part);
   hipLaunchKernelGGL(k_analysis_sum, dim3(1), dim3(64), 0, h->stream, part, nblk, out);
 }
 
