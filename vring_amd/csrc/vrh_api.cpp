@@ -818,7 +818,9 @@ int ens_build(vrh_ens_t *e) {
   for (size_t q = 0; q < e->m.size() && rc == 0; q++) {
     vrh_t *h = e->m[q];
     if (hipStreamWaitEvent(h->stream, e->fork, 0) != hipSuccess) { rc = -3; break; }
+    h->serial_sweeps = true;
     rc = step_body(h);
+    h->serial_sweeps = false;
     if (rc) { e->err = "member " + std::to_string(q) + ": " + h->err; break; }
     if (hipEventRecord(e->join[q], h->stream) != hipSuccess ||
         hipStreamWaitEvent(e->origin, e->join[q], 0) != hipSuccess) rc = -3;

@@ -105,6 +105,7 @@ struct vrh_s {
   // side streams for the independent component chains of the line sweeps
   // (forked from and joined back into `stream`; captured with the step)
   hipStream_t side[2] = {nullptr, nullptr};
+  bool serial_sweeps = false;  // set while captured into an ensemble graph
   // one fork and two join events per substep: a captured step records each
   // event once (an event recorded twice in one capture keeps only one of
   // the dependency sets)

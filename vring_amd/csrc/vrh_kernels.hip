@@ -1985,8 +1985,9 @@ template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const d
   // The three components' chains are independent: q3 and q2 run on the
   // side streams, forked from and joined back into the handle's stream
   // (captured steps record the fork as graph branches).  With the kernel
-  // timer on all run in order on the handle's stream.
-  const bool par = !h->kt_on;
+  // timer on, and inside an ensemble graph (whose members already run side
+  // by side), all run in order on the handle's stream.
+  const bool par = !h->kt_on && !h->serial_sweeps;
   hipStream_t s1 = par ? h->side[0] : h->stream, s2 = par ? h->side[1] : h->stream;
   const int ie = (irk >= 1 && irk <= 3) ? irk : 0;
   if (par) {
