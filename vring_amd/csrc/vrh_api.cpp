@@ -244,7 +244,8 @@ int vrh_last_error(vrh_t *h, char *buf, size_t len) {
 int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t **out) {
   if (!grid || !p || !out) return -1;
   *out = nullptr;
-  if (grid->ntheta < 3 || grid->nr < 4 || grid->nz < 4 || grid->nr > 16 * 32 || grid->ntheta > 256 ||
+  if (grid->ntheta < 3 || grid->nr < 4 || grid->nz < 4 || grid->nr > 16 * 32 || grid->nz > 16 * 64 - 1 ||
+      grid->ntheta > 256 ||
       grid->rmax <= 0 || grid->zmax <= 0 || p->dt <= 0 || p->Re <= 0)
     return -1;
   if (precision != VRH_FP64 && precision != VRH_MIXED) return -1;
