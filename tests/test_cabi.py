@@ -46,3 +46,24 @@ def test_ring_initial_condition_is_divergence_free_on_host():
     div[:, :, nz - 1] = 0.0  # outlet ghost q3(nz+1) is not part of the field
     assert np.abs(div).max() < 1e-10 * np.abs(q3).max() / dz
     assert np.all(q1 == 0.0)
+
+
+def test_create_rejects_grids_outside_the_limits_before_any_device_call():
+    """vrh_create checks the grid limits (3 <= ntheta <= 256, 4 <= nr <= 512,
+    4 <= nz <= 1023) and the parameters before it touches a device, so the
+    rejections run here without a GPU."""
+    from vring_amd.lib import VrhGrid, VrhParams
+    lib = ctypes.CDLL(LIBPATH)
+    lib.vrh_create.argtypes = [ctypes.POINTER(VrhGrid), ctypes.POINTER(VrhParams), ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_void_p)]
+    ok_p = VrhParams(0.01, 1000.0, 0.0, 0)
+    for nt, nr, nz in [(2, 32, 64), (257, 32, 64), (16, 3, 64), (16, 513, 64), (16, 32, 3), (16, 32, 1024)]:
+        h = ctypes.c_void_p(1)
+        assert lib.vrh_create(ctypes.byref(VrhGrid(nt, nr, nz, 4.0, 12.0)), ctypes.byref(ok_p), 0,
+                              ctypes.byref(h)) == -1, (nt, nr, nz)
+        assert not h.value
+    for p in [VrhParams(0.0, 1000.0, 0.0, 0), VrhParams(0.01, -1.0, 0.0, 0)]:
+        h = ctypes.c_void_p(1)
+        assert lib.vrh_create(ctypes.byref(VrhGrid(16, 32, 64, 4.0, 12.0)), ctypes.byref(p), 0, ctypes.byref(h)) == -1
+    h = ctypes.c_void_p(1)
+    assert lib.vrh_create(ctypes.byref(VrhGrid(16, 32, 64, 4.0, 12.0)), ctypes.byref(ok_p), 7, ctypes.byref(h)) == -1
