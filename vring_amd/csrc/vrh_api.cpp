@@ -354,6 +354,7 @@ int vrh_destroy(vrh_t *h) {
   if (h->h_pin) hipHostFree(h->h_pin);
   if (h->copy_stream) hipStreamSynchronize(h->copy_stream);
   if (h->d_snap) hipFree(h->d_snap);
+  if (h->d_anl) hipFree(h->d_anl);
   if (h->h_snap) hipHostFree(h->h_snap);
   if (h->snap_ev) hipEventDestroy(h->snap_ev);
   if (h->copy_stream) hipStreamDestroy(h->copy_stream);
@@ -937,12 +938,13 @@ extern "C" int vrh_set_tuning(vrh_t *h, const int *choice) {
 extern "C" int vrh_analysis(vrh_t *h, double *out) {
   if (!h || !out) return -1;
   const int nblk = 1024;
-  double *part = nullptr;
-  HIPCHK(h, hipMalloc(&part, sizeof(double) * (nblk * 10 + 16)));
+  // every block writes its ten partials and the sum kernel writes the ten
+  // results, so the buffer needs no clearing between calls
+  if (!h->d_anl) HIPCHK(h, hipMalloc(&h->d_anl, sizeof(double) * (nblk * 10 + 16)));
+  double *part = h->d_anl;
   launch_analysis(h, part, nblk, part + nblk * 10);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(out, part + nblk * 10, 10 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  (void)hipFree(part);
   return 0;
 }

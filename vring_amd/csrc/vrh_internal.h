@@ -158,6 +158,7 @@ struct vrh_s {
   // taken on the compute stream, drained to pinned host memory on a copy
   // stream while stepping continues
   double *d_snap = nullptr, *h_snap = nullptr;
+  double *d_anl = nullptr;  // vrh_analysis block partials and result (first use)
   hipStream_t copy_stream = nullptr;
   hipEvent_t snap_ev = nullptr;
   bool snap_pending = false;
