@@ -1119,19 +1119,20 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
   const int N = g.nt, tid = threadIdx.x, nth = blockDim.x;
   const long nlines = (long)g.nr * g.nz;
   const long l0 = (long)blockIdx.x * LB;
-  // LDS: [dv (LB*N double)] | re, im ((LB/2)*N TW each) | [tw (2N double)] | twl (N TW) | jk (2 LB int) |
-  // 1/rm, dx2/dmr (LB double each).  The double copy dv feeds the direct DFT
-  // (ntheta not a power of two) and the FP64 low modes; the FP64 FFT path
-  // does without it (half the LDS, twice the blocks per CU).
+  // LDS: [dv (LB*N double)] | re, im ((LB/2)*N TW each) | [tw (2N double), part (10 LB double)] |
+  // twl (N TW) | jk (2 LB int) | 1/rm, dx2/dmr (LB double each).  The double
+  // copy dv feeds only the direct DFT (ntheta not a power of two); the FP64
+  // low modes of the mixed path are summed in registers.
   const bool pow2 = N >= 2 && (N & (N - 1)) == 0;
-  const bool keep_dv = !pow2 || S64 != nullptr;
+  const bool keep_dv = !pow2;
   size_t off = keep_dv ? al16(sizeof(double) * LB * N) : 0;
   double *dv = reinterpret_cast<double *>(smem_raw);
   TW *re = reinterpret_cast<TW *>(smem_raw + off);
   TW *im = re + (size_t)(LB / 2) * N;
   off = al16(off + sizeof(TW) * LB * N);
   double *tw = reinterpret_cast<double *>(smem_raw + off);     // cos (N) then sin (N), S64 only
-  if (S64) off = al16(off + sizeof(double) * 2 * (size_t)N);
+  double *part = tw + 2 * (size_t)N;                           // (line, coefficient, 2 waves), S64 only
+  if (S64) off = al16(off + sizeof(double) * (2 * (size_t)N + 10 * (size_t)LB));
   TW *twl = reinterpret_cast<TW *>(smem_raw + off);            // FFT twiddles: N/2 cos, N/2 sin
   off = al16(off + sizeof(TW) * N);
   int *jk = reinterpret_cast<int *>(smem_raw + off);           // (j, k) of each line
@@ -1149,53 +1150,83 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
   }
   if (pow2)
     for (int q = tid; q < N / 2; q += nth) { twl[q] = (TW)twr[q]; twl[N / 2 + q] = (TW)twi[q]; }
+  if (S64)
+    for (int q = tid; q < N; q += nth) { tw[q] = twr[q]; tw[N + q] = twi[q]; }
   __syncthreads();
-  const int tot = LB * N;
-  auto put = [&](int e, int lp, int ii, double val) {
-    if (keep_dv) dv[e] = val;
+  // TPL threads per line: thread tl of line lpt takes the angles tl,
+  // tl + TPL, ... (consecutive lanes read consecutive angles), so its FP64
+  // low-mode sums (mixed path) stay within one line
+  const int TPL = nth / LB, lpt = tid / TPL, tl = tid - lpt * TPL;
+  const int NU = (N + TPL - 1) / TPL;
+  const bool lvalid = l0 + lpt < nlines;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+  auto put = [&](int ii, double val) {
+    if (keep_dv) dv[lpt * N + ii] = val;
     if (pow2) {
-      const int pos = (lp >> 1) * N + ii;
-      if (lp & 1) im[pos] = (TW)val; else re[pos] = (TW)val;
+      const int pos = (lpt >> 1) * N + ii;
+      if (lpt & 1) im[pos] = (TW)val; else re[pos] = (TW)val;
+    }
+    if (S64) {  // cos and sin sums of wavenumbers 0, 1, 2
+      int q2 = 2 * ii;
+      if (q2 >= N) q2 -= N;
+      s0 += val;
+      s1 += val * tw[ii];
+      s2 += val * tw[N + ii];
+      s3 += val * tw[q2];
+      s4 += val * tw[N + q2];
     }
   };
+  const int jl = jk[2 * lpt], kl = jk[2 * lpt + 1];
   if (ext) {
-    for (int e = tid; e < tot; e += nth) {
-      const int lp = pow2 ? e >> logn : e / N, ii = e - lp * N;
-      const double val = (l0 + lp < nlines) ? ext[cidx(g, ii + 1, jk[2 * lp], jk[2 * lp + 1])] : 0.0;
-      put(e, lp, ii, val);
+    for (int u = 0; u < NU; ++u) {
+      const int ii = tl + TPL * u;
+      if (ii < N) put(ii, lvalid ? ext[cidx(g, ii + 1, jl, kl)] : 0.0);
     }
   } else {
-    // the loads of U elements are all issued before the first is used;
+    // the loads of U angles are all issued before the first is used;
     // indices are clamped to valid cells, so the loads are unconditional
     constexpr int U = 4;
-    for (int eb = tid; eb < tot; eb += U * nth) {
-      double a1[U], b1[U], a2[U], b2[U], a3[U], b3[U], rmv[U], rdv[U];
+    const double rmv = lirm[lpt], rdv = lcx2[lpt];
+    const long ob = hidx(g, 1, jl, kl);
+    for (int ub = 0; ub < NU; ub += U) {
+      double a1[U], b1[U], a2[U], b2[U], a3[U], b3[U];
       #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int e = min(eb + u * nth, tot - 1);
-        const int lp = pow2 ? e >> logn : e / N, i = e - lp * N + 1;
-        const int j = jk[2 * lp], k = jk[2 * lp + 1];
-        const long o = hidx(g, i, j, k);
+        const int i = min(tl + TPL * (ub + u), N - 1) + 1;
+        const long o = ob + (i - 1);
         a1[u] = dq1[o];
         b1[u] = dq1[o + ((i == N) ? 1 - N : 1)];
         a2[u] = dq2[o];
         b2[u] = dq2[o + g.sj];
         a3[u] = dq3[o];
         b3[u] = dq3[o + g.sk];
-        rmv[u] = lirm[lp];
-        rdv[u] = lcx2[lp];
       }
       __builtin_amdgcn_sched_barrier(0);
       #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int e = eb + u * nth;
-        if (e < tot) {
-          const int lp = pow2 ? e >> logn : e / N, ii = e - lp * N;
-          const double t12 = ((b1[u] - a1[u]) * (g.dx1 * g.axsym) + (b2[u] - a2[u]) * rdv[u]) * rmv[u];
+        const int ii = tl + TPL * (ub + u);
+        if (ii < N) {
+          const double t12 = ((b1[u] - a1[u]) * (g.dx1 * g.axsym) + (b2[u] - a2[u]) * rdv) * rmv;
           const double t3 = (b3[u] - a3[u]) * g.dx3;
-          put(e, lp, ii, (l0 + lp < nlines) ? -(t12 + t3) * inv_adt : 0.0);
+          put(ii, lvalid ? -(t12 + t3) * inv_adt : 0.0);
         }
       }
+    }
+  }
+  if (S64) {
+    // the line's TPL lanes: a fixed butterfly within each wave (deterministic
+    // order), then one partial per wave into LDS
+    const int grp = TPL < 64 ? TPL : 64;
+    for (int o = grp >> 1; o > 0; o >>= 1) {
+      s0 += __shfl_xor(s0, o, grp);
+      s1 += __shfl_xor(s1, o, grp);
+      s2 += __shfl_xor(s2, o, grp);
+      s3 += __shfl_xor(s3, o, grp);
+      s4 += __shfl_xor(s4, o, grp);
+    }
+    if ((tl & (grp - 1)) == 0) {
+      double *pp = part + (size_t)lpt * 10 + tl / grp;
+      pp[0] = s0; pp[2] = s1; pp[4] = s2; pp[6] = s3; pp[8] = s4;
     }
   }
   __syncthreads();
@@ -1229,40 +1260,16 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
     S[(size_t)pos + (size_t)N * l] = (TW)outv;
   }
   if (S64) {
-    // FP64 DFT of the lowest modes m = 0, 1, 2 from the FP64 divergence.
-    // 32 lanes per line: lane c accumulates the angles c, c + 32, ... (the
-    // lanes of a line read consecutive LDS words) for all five packed
-    // coefficients, then a fixed butterfly over the 32 lanes sums them (a
-    // deterministic order); lane p < 5 stores packed position p.
-    for (int q = tid; q < N; q += nth) { tw[q] = twr[q]; tw[N + q] = twi[q]; }
-    __syncthreads();
-    const int c = tid & 31;
-    for (int lp = tid >> 5; lp < LB; lp += nth >> 5) {
-      const double *x = dv + (size_t)lp * N;
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
-      for (int q = c; q < N; q += 32) {
-        const double v = x[q];
-        int q2 = 2 * q;
-        if (q2 >= N) q2 -= N;
-        s0 += v;
-        s1 += v * tw[q];
-        s2 += v * tw[N + q];
-        s3 += v * tw[q2];
-        s4 += v * tw[N + q2];
-      }
-      for (int o = 16; o > 0; o >>= 1) {
-        s0 += __shfl_xor(s0, o, 32);
-        s1 += __shfl_xor(s1, o, 32);
-        s2 += __shfl_xor(s2, o, 32);
-        s3 += __shfl_xor(s3, o, 32);
-        s4 += __shfl_xor(s4, o, 32);
-      }
+    // FP64 modes m = 0, 1, 2 (t(1), t(2k), t(2k+1) packing) from the wave
+    // partials, summed in wave order
+    const int nw = TPL > 64 ? TPL / 64 : 1;
+    for (int e = tid; e < LB * 5; e += nth) {
+      const int lp = e / 5, c5 = e - lp * 5;
       const long l = l0 + lp;
-      if (c < 5 && c < N && l < nlines) {
-        // t(1), t(2k), t(2k+1) packing: cos and sin sums of wavenumber k
-        const double sc = c == 0 ? s0 : c == 1 ? s1 : c == 2 ? s2 : c == 3 ? s3 : s4;
-        S64[(size_t)c + 5 * l] = (c == 0) ? sc / N : 2.0 * sc / N;
-      }
+      if (l >= nlines || c5 >= N) continue;
+      double acc = part[(size_t)e * 2];
+      for (int w = 1; w < nw; ++w) acc += part[(size_t)e * 2 + w];
+      S64[(size_t)c5 + 5 * l] = (c5 == 0) ? acc / N : 2.0 * acc / N;
     }
   }
 }
@@ -2050,9 +2057,9 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   TW *S = (TW *)h->f.S;
   double *S64 = (h->prec == VRH_MIXED) ? h->f.S64 : nullptr;
   const double adt = h->rk[irk].a * h->dt;
-  const bool keep_dv = !pow2 || S64 != nullptr;   // the kernel's FP64 copy of the divergence
+  const bool keep_dv = !pow2;   // the kernel's FP64 copy of the lines (direct DFT)
   size_t sh1 = (keep_dv ? al16(sizeof(double) * LB * N) : 0) + al16(sizeof(TW) * LB * N);
-  if (S64) sh1 = al16(sh1 + sizeof(double) * 2 * (size_t)N);
+  if (S64) sh1 = al16(sh1 + sizeof(double) * (2 * (size_t)N + 10 * (size_t)LB));
   sh1 = al16(al16(sh1 + sizeof(TW) * N) + sizeof(int) * 2 * LB) + sizeof(double) * 2 * LB;
   { KTimer kt(h, KT_DIVFFT);
   hipLaunchKernelGGL(k_divfft<TW>, dim3((unsigned)((nlines + LB - 1) / LB)), dim3(256), sh1, h->stream, g, h->coef,
