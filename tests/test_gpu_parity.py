@@ -148,19 +148,20 @@ def test_mixed_precision_ring_error_vs_fp64():
     assert max(errs.values()) < 1e-6, errs
 
 
-@pytest.mark.parametrize("nt,nr,nz", [(33, 32, 64), (9, 16, 32), (48, 24, 40)])
-def test_mixed_precision_at_other_ntheta_vs_fp64(nt, nr, nz):
+@pytest.mark.parametrize("nt,nr,nz,slip", [(33, 32, 64, 0.0), (9, 16, 32, 0.0), (48, 24, 40, 0.0), (32, 24, 40, 0.5)])
+def test_mixed_precision_at_other_ntheta_vs_fp64(nt, nr, nz, slip):
     """The mixed path where theta or z is not a power of two (direct
-    transforms; the FP64 low modes of a line from two waves' partials):
-    relative field error against FP64 after 20 ring steps below 1e-6."""
-    print(f"case mixed vs fp64 nt={nt} nr={nr} nz={nz}", flush=True)
+    transforms; the FP64 low modes of a line from two waves' partials), and
+    with wall slip: relative field error against FP64 after 20 ring steps
+    below 1e-6."""
+    print(f"case mixed vs fp64 nt={nt} nr={nr} nz={nz} slip={slip}", flush=True)
     rmax, zmax = 4.0, 12.0
     q1, q2, q3 = vring_amd.ring_initial_condition(nt, nr, nz, rmax, zmax, [(1.0, 3.0, 0.3, 1.0)],
                                                   pert_amp=1e-2, pert_modes=(1, 3), phases=np.arange(3.0))
     dt = vring_amd.cfl_dt(nt, nr, nz, rmax, zmax, q1, q2, q3, cfl=0.3, ujet=1.0)
     out = {}
     for prec in ("fp64", "mixed"):
-        s = vring_amd.Solver(nt, nr, nz, rmax, zmax, 1000.0, dt, precision=prec)
+        s = vring_amd.Solver(nt, nr, nz, rmax, zmax, 1000.0, dt, slip, precision=prec)
         for n, a in (("q1", q1), ("q2", q2), ("q3", q3)):
             s.upload(n, a)
         s.step(20)
