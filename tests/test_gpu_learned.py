@@ -1,6 +1,6 @@
 """The per-mode (r, z) Helmholtz subsystem on the GPU (include/vrh_learn.h)
 against the CPU oracle (oracle/learned_oracle.py): operator, direct solve,
-Krylov solvers judged by the true residual, network forward pass and
+Krylov solvers checked by the true residual, network forward pass and
 parameter gradients, training, and what the trained network gains."""
 import os
 import sys
@@ -130,7 +130,7 @@ def test_training_lowers_loss_and_network_improves_the_warm_start():
     modes) compare, in the energy norm the training minimises, the solver's
     warm start (the Galerkin combination of D^-1 V f and the network
     correction) with the best multiple of D^-1 V f alone, and run plain PCG,
-    the learned-start PCG and the learned flexible CG, each judged by its true
+    the learned-start PCG and the learned flexible CG, each checked by its true
     residual."""
     print("case hz train 32x64 pool 2", flush=True)
     S, pool = 16, 2
@@ -164,7 +164,7 @@ def test_training_lowers_loss_and_network_improves_the_warm_start():
     p1, it1, rel1, ms1 = hz.solve(f, "learned", tol=1e-8, maxit=20000)
     p2, it2, rel2, ms2 = hz.solve(f, "learned_fcg", tol=1e-8, maxit=20000)
     print("pcg iterations", it0.tolist(), "learned start", it1.tolist(), "learned fcg", it2.tolist(), flush=True)
-    # the true residual judges every solve, the fallback included
+    # the true residual is checked for every solve, the fallback included
     assert np.all(rel0 <= 1.05e-8) and np.all(rel1 <= 1.05e-8) and np.all(rel2 <= 1.05e-8), (rel0, rel1, rel2)
     assert np.abs(p1 - p0).max() < 1e-5 * np.abs(p0).max()
     assert np.abs(p2 - p0).max() < 1e-5 * np.abs(p0).max()
