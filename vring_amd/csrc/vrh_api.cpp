@@ -104,9 +104,8 @@ bool lookup(vrh_t *h, const char *name, Named &out) {
 // correction) are read from d_sc, as written by k_scalars at the start of
 // the step: the form that is captured in a graph.
 template <typename TW> void substep_dev(vrh_t *h, int irk) {
-  launch_hdnbc<TW>(h, irk, 0.0, h->d_sc + 3 + irk);
   launch_explicit<TW>(h, irk);
-  launch_sweeps<TW>(h, irk, 0.0, h->d_sc + irk);
+  launch_sweeps<TW>(h, irk, 0.0, h->d_sc + irk, true, 0.0, h->d_sc + 3 + irk);
   launch_poisson<TW>(h, irk, nullptr, nullptr);
   launch_project<TW>(h, irk);
 }
@@ -122,9 +121,9 @@ template <typename TW> int substep_t(vrh_t *h, int irk, int phase) {
     for (int i = 1; i <= h->nt; i++) qinf = qinf + cj * h->da[j];
   }
   const double cor = (qinf - 0.0) / (h->cor_red + (double)1.e-15f);  // :283
-  if (phase < 0 || phase == VRH_PH_HDNBC) launch_hdnbc<TW>(h, irk, cor, nullptr);
+  if (phase == VRH_PH_HDNBC) launch_hdnbc<TW>(h, irk, cor, nullptr);
   if (phase < 0 || phase == VRH_PH_EXPLICIT) launch_explicit<TW>(h, irk);
-  if (phase < 0 || phase == VRH_PH_ADI) launch_sweeps<TW>(h, irk, c3s, nullptr);
+  if (phase < 0 || phase == VRH_PH_ADI) launch_sweeps<TW>(h, irk, c3s, nullptr, phase < 0, cor, nullptr);
   if (phase < 0 || phase == VRH_PH_POISSON) launch_poisson<TW>(h, irk, nullptr, nullptr);
   if (phase < 0 || phase == VRH_PH_PROJECT) launch_project<TW>(h, irk);
   HIPCHK(h, hipGetLastError());

@@ -190,7 +190,10 @@ template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, const do
 void launch_scalars(vrh_t *h);
 void launch_analysis(vrh_t *h, double *part, int nblk, double *out);
 template <typename TW> void launch_explicit(vrh_t *h, int irk);
-template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const double *c3s_dev);
+// hdnbc: also launch the outlet-row kernel of hdnbc at the head of the q3
+// sweep chain (whole steps), with the mass correction cor or *cor_dev
+template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const double *c3s_dev, bool hdnbc = false,
+                                          double cor = 0.0, const double *cor_dev = nullptr);
 template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_rhs, double *phi_out);
 template <typename TW> void launch_project(vrh_t *h, int irk);
 void launch_particles(vrh_t *h, int irk);

@@ -1890,12 +1890,15 @@ __global__ void k_analysis_sum(const double *__restrict__ part, int nblk, double
 // ===========================================================================
 #define CHECK_LAUNCH() (void)hipGetLastError()
 
-template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, const double *cor_dev) {
+static void hdnbc_on(vrh_t *h, int irk, double cor, const double *cor_dev, hipStream_t st) {
   const VGrid &g = h->g;
   dim3 blk(64), grd((g.nt + 63) / 64, g.nrc);
   KTimer kt(h, KT_HDNBC);
-  hipLaunchKernelGGL(k_hdnbc, grd, blk, 0, h->stream, g, h->coef, h->rk[irk].g, h->rk[irk].r, cor, cor_dev,
-                     h->f.q3, h->f.csort, h->f.dq3no, h->f.corr3n);
+  hipLaunchKernelGGL(k_hdnbc, grd, blk, 0, st, g, h->coef, h->rk[irk].g, h->rk[irk].r, cor, cor_dev, h->f.q3,
+                     h->f.csort, h->f.dq3no, h->f.corr3n);
+}
+template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, const double *cor_dev) {
+  hdnbc_on(h, irk, cor, cor_dev, h->stream);
 }
 
 void launch_scalars(vrh_t *h) {
@@ -1980,7 +1983,8 @@ static void line_launch(vrh_t *h, const LineArgs &a, int M, hipStream_t st) {
   }
 }
 
-template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const double *c3s_dev) {
+template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const double *c3s_dev, bool hdnbc,
+                                          double cor, const double *cor_dev) {
   const VGrid &g = h->g;
   LineArgs a{};
   a.g = g;
@@ -2007,7 +2011,11 @@ template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const d
   line_launch<TW, SW_R1>(h, a, h->Msw[SW_R1], h->stream);
   a.lc = h->lc[irk][SW_Z1]; a.nlines = (long)g.nt * g.nr; a.j0 = 1; a.q = h->f.q1; a.dq = h->f.dq1;
   line_launch<TW, SW_Z1>(h, a, h->Msw[SW_Z1], h->stream);
-  // q3: r then z (step.F90:124-127)
+  // q3: r then z (step.F90:124-127).  In a whole step the outlet rows of
+  // hdnbc (csort, dq3no, corr3n: read only by the q3 z sweep; computed from
+  // q3, which the explicit kernel leaves alone) head this chain, off the
+  // critical path.
+  if (hdnbc) hdnbc_on(h, irk, cor, cor_dev, s1);
   a.lc = h->lc[irk][SW_R3]; a.nlines = (long)g.nt * g.nz; a.r = h->f.r3;
   line_launch<TW, SW_R3>(h, a, h->Msw[SW_R3], s1);
   a.lc = h->lc[irk][SW_Z3]; a.nlines = (long)g.nt * g.nr; a.j0 = 1; a.q = h->f.q3; a.dq = h->f.dq3;
@@ -2129,7 +2137,7 @@ void launch_analysis(vrh_t *h, double *part, int nblk, double *out) {
 #define INST(TW)                                                   \
   template void launch_hdnbc<TW>(vrh_t *, int, double, const double *); \
   template void launch_explicit<TW>(vrh_t *, int);                 \
-  template void launch_sweeps<TW>(vrh_t *, int, double, const double *); \
+  template void launch_sweeps<TW>(vrh_t *, int, double, const double *, bool, double, const double *); \
   template void launch_poisson<TW>(vrh_t *, int, const double *, double *); \
   template void launch_project<TW>(vrh_t *, int);
 INST(double)
