@@ -270,6 +270,7 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
   for (int irk = 0; irk < 4; irk++) {
     HIPCHK(h, hipEventCreateWithFlags(&h->fork_ev[irk], hipEventDisableTiming));
     for (int q = 0; q < 2; q++) HIPCHK(h, hipEventCreateWithFlags(&h->join_ev[irk][q], hipEventDisableTiming));
+    for (int q = 0; q < 2; q++) HIPCHK(h, hipEventCreateWithFlags(&h->poi_ev[irk][q], hipEventDisableTiming));
   }
   // radial vectors (values from refresh_mesh below)
   const int nv = h->nr + 3;
@@ -371,6 +372,8 @@ int vrh_destroy(vrh_t *h) {
     if (h->fork_ev[irk]) hipEventDestroy(h->fork_ev[irk]);
     for (int q = 0; q < 2; q++)
       if (h->join_ev[irk][q]) hipEventDestroy(h->join_ev[irk][q]);
+    for (int q = 0; q < 2; q++)
+      if (h->poi_ev[irk][q]) hipEventDestroy(h->poi_ev[irk][q]);
   }
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;

@@ -111,6 +111,8 @@ struct vrh_s {
   // the dependency sets)
   hipEvent_t fork_ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t join_ev[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+  // fork and join of the pressure solve's FP64 low-mode chain, per substep
+  hipEvent_t poi_ev[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
   std::string err;
   // host copies of the radial vectors
   std::vector<double> hv[kNumVec];

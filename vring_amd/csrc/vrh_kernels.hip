@@ -2036,7 +2036,7 @@ template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const d
 static int ilog2(int n) { int l = 0; while ((1 << l) < n) ++l; return l; }
 
 template <typename T>
-static void radial_launch(vrh_t *h, T *S, const T *ib, long nm) {
+static void radial_launch(vrh_t *h, T *S, const T *ib, long nm, hipStream_t st) {
   const VGrid &g = h->g;
   // rows per wave: 16 keeps the segment's rhs and pivots in registers (a
   // 1024-lane block allows 128 VGPRs) up to nr = 256; 32 beyond (P <= 16)
@@ -2046,9 +2046,9 @@ static void radial_launch(vrh_t *h, T *S, const T *ib, long nm) {
   dim3 blk(64 * P), grd((unsigned)((nlines + 63) / 64));
   KTimer kt(h, KT_RADIAL);
   if (M == 16)
-    hipLaunchKernelGGL((k_radial<T, 16>), grd, blk, 0, h->stream, S, ib, nm, nlines, g.nr, h->d_aab, h->d_ccb);
+    hipLaunchKernelGGL((k_radial<T, 16>), grd, blk, 0, st, S, ib, nm, nlines, g.nr, h->d_aab, h->d_ccb);
   else
-    hipLaunchKernelGGL((k_radial<T, 32>), grd, blk, 0, h->stream, S, ib, nm, nlines, g.nr, h->d_aab, h->d_ccb);
+    hipLaunchKernelGGL((k_radial<T, 32>), grd, blk, 0, st, S, ib, nm, nlines, g.nr, h->d_aab, h->d_ccb);
 }
 
 template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_rhs, double *phi_out) {
@@ -2090,23 +2090,36 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   const int NTD = threads(TL), NTD64 = threads(TL64);
   const long nl = (long)N * g.nr;
   const size_t sh2 = sizeof(TW) * ((size_t)TL * (nz + 1) + nz), sh64 = sizeof(double) * ((size_t)TL64 * (nz + 1) + nz);
+  // The FP64 low-mode chain (mixed precision) reads only what k_divfft
+  // wrote to S64 and feeds only k_ifft: it runs on a side stream, forked
+  // after k_divfft and joined before k_ifft, beside the FP32 transforms.
+  // With the kernel timer on, or inside an ensemble graph, it stays in
+  // order on the handle's stream.
+  const bool par = S64 && !h->kt_on && !h->serial_sweeps;
+  hipStream_t s64 = par ? h->side[0] : h->stream;
+  const int ie = (irk >= 1 && irk <= 3) ? irk : 0;
+  if (par) {
+    (void)hipEventRecord(h->poi_ev[ie][0], h->stream);
+    (void)hipStreamWaitEvent(s64, h->poi_ev[ie][0], 0);
+  }
+  if (S64) {
+    const long nl64 = 5L * g.nr;
+    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(NTD64), sh64, s64, S64,
+                       nl64, nz, lz, TL64, ltl64, 1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
+    radial_launch<double>(h, S64, h->f.ib64, 5, s64);
+    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(NTD64), sh64, s64, S64,
+                       nl64, nz, lz, TL64, ltl64, -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
+  }
   { KTimer kt(h, KT_DCT_FWD);
   hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(NTD), sh2, h->stream, S, nl, nz, lz, TL,
                      ltl, 1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
-  if (S64) {
-    const long nl64 = 5L * g.nr;
-    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(NTD64), sh64, h->stream, S64,
-                       nl64, nz, lz, TL64, ltl64, 1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
-  }
-  radial_launch<TW>(h, S, (const TW *)h->f.ib, N);
-  if (S64) radial_launch<double>(h, S64, h->f.ib64, 5);
+  radial_launch<TW>(h, S, (const TW *)h->f.ib, N, h->stream);
   { KTimer kt(h, KT_DCT_INV);
   hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(NTD), sh2, h->stream, S, nl, nz, lz, TL,
                      ltl, -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
-  if (S64) {
-    const long nl64 = 5L * g.nr;
-    hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(NTD64), sh64, h->stream, S64,
-                       nl64, nz, lz, TL64, ltl64, -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
+  if (par) {
+    (void)hipEventRecord(h->poi_ev[ie][1], s64);
+    (void)hipStreamWaitEvent(h->stream, h->poi_ev[ie][1], 0);
   }
   size_t sh3 = al16(sizeof(TW) * ((size_t)2 * LB * N + N)) + sizeof(int) * 2 * LB;
   if (S64) sh3 = al16(sh3) + sizeof(double) * (5 * (size_t)LB + 2 * (size_t)N);
