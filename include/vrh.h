@@ -130,12 +130,14 @@ int vrh_synchronize(vrh_t *h);
 int vrh_ktimer_enable(vrh_t *h, int on);
 int vrh_ktimer_read(vrh_t *h, int idx, char *name, size_t len, double *total_ms, long *count);
 /* Autotuner: times every candidate segment length of the six partitioned
- * line sweeps, the FFT/DCT tile sizes and the three forms of the explicit
+ * line sweeps, the FFT/DCT tile sizes and the four forms of the explicit
  * kernel (one z plane per block, or z marching) with HIP events on the live
  * buffers and keeps the fastest.  Call between steps.  choice[9] (optional)
  * receives M for sweeps r1 r3 r2 z1 z3 z2, theta FFT lines/block, z DCT
  * lines/block and the explicit-kernel form (0 plane, 1 marching, 2 marching
- * with one wavefront per r row). */
+ * with one wavefront per r row within 64 KB of LDS per block, 3 the same
+ * with the rows per block that keep the most waves resident within the
+ * device's LDS per block and per CU). */
 int vrh_autotune(vrh_t *h, int reps, int *choice);
 /* Apply a tuning returned by vrh_autotune (9 entries; 0 keeps a segment
  * length or tile size, -1 selects the default explicit form). */

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 R, L = 4.0, 12.0
 
 
-@pytest.mark.parametrize("form,slip", [(0, 0.0), (1, 0.0), (2, 0.0), (-1, 0.5)])
+@pytest.mark.parametrize("form,slip", [(0, 0.0), (1, 0.0), (2, 0.0), (3, 0.0), (-1, 0.5)])
 def test_substep_phases_on_stretched_mesh_match_oracle(form, slip):
     """form -1: the default explicit form; slip 0.5: the wall-slip rows,
     whose q1 r-sweep coefficient carries dcr(nrc)."""

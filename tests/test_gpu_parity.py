@@ -10,11 +10,12 @@ import vring_amd
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("form", [0, 1, 2])
+@pytest.mark.parametrize("form", [0, 1, 2, 3])
 @pytest.mark.parametrize("nt,nr,nz", [(9, 16, 32), (32, 32, 64), (33, 24, 40), (16, 64, 128), (128, 20, 40)])
 def test_substep_phases_match_oracle(nt, nr, nz, form):
     """form: explicit kernel with one z plane per block (0), z marching (1),
-    or z marching with one wavefront per r row (2)."""
+    or z marching with one wavefront per r row within 64 KB of LDS (2) or
+    within the device's LDS limits (3)."""
     print(f"case phases nt={nt} nr={nr} nz={nz} form={form}", flush=True)
     o, s = make_pair(nt, nr, nz, slip=0.0)
     s.set_tuning(explicit_form=form)
@@ -175,17 +176,17 @@ def test_mixed_precision_at_other_ntheta_vs_fp64(nt, nr, nz, slip):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("nt,nr,nz", [(128, 40, 64), (48, 36, 64), (256, 16, 32)])
+@pytest.mark.parametrize("nt,nr,nz", [(128, 40, 64), (128, 22, 40), (48, 36, 64), (256, 16, 32)])
 def test_explicit_forms_agree_bitwise(precision, nt, nr, nz):
     """The marching forms of the explicit kernel (one block row per theta
-    line, or one wavefront per r row) do the same arithmetic per cell in the
-    same order: whole ring steps agree bit for bit in both precisions; the
-    one-plane form agrees to rounding."""
+    line, or one wavefront per r row with either row count per block) do the
+    same arithmetic per cell in the same order: whole ring steps agree bit
+    for bit in both precisions; the one-plane form agrees to rounding."""
     print(f"case explicit forms {precision} {nt}x{nr}x{nz}", flush=True)
     q = vring_amd.ring_initial_condition(nt, nr, nz, 4.0, 12.0, [(1.0, 3.0, 0.3, 1.0)], pert_amp=1e-2,
                                          pert_modes=(2, 4))
     out = []
-    for form in (1, 2, 0):
+    for form in (1, 2, 0, 3):
         s = vring_amd.Solver(nt, nr, nz, 4.0, 12.0, 1000.0, 0.01, precision=precision)
         s.set_tuning(explicit_form=form)
         for n, a in zip(("q1", "q2", "q3"), q):
@@ -193,8 +194,9 @@ def test_explicit_forms_agree_bitwise(precision, nt, nr, nz):
         s.step(3)
         out.append([s.download(n) for n in ("q1", "q2", "q3", "pres")])
         s.close()
-    for a, b in zip(out[0], out[1]):
+    for a, b, c in zip(out[0], out[1], out[3]):
         assert np.array_equal(a, b)
+        assert np.array_equal(a, c)
     tol = 1e-12 if precision == "fp64" else 1e-5
     for a, b in zip(out[0], out[2]):
         assert np.abs(a - b).max() <= tol * np.abs(a).max()

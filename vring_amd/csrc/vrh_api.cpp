@@ -266,6 +266,7 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
   const double rom[4] = {0, 0.0, -17.0 / 60.0, -5.0 / 12.0};
   for (int i = 1; i <= 3; i++) h->rk[i] = {gam[i], rom[i], gam[i] + rom[i]};
   HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  explicit_lds_optin(h);
   for (int q = 0; q < 2; q++) HIPCHK(h, hipStreamCreateWithFlags(&h->side[q], hipStreamNonBlocking));
   for (int irk = 0; irk < 4; irk++) {
     HIPCHK(h, hipEventCreateWithFlags(&h->fork_ev[irk], hipEventDisableTiming));
@@ -704,7 +705,7 @@ extern "C" int vrh_autotune(vrh_t *h, int reps, int *choice) {
   }
   {
     int best = 0; double bestt = 1e30;
-    for (int v : {0, 1, 2}) {
+    for (int v : {0, 1, 2, 3}) {
       h->explicit_variant = v;
       double t = h->prec == VRH_MIXED ? time_explicit<float>(h, reps) : time_explicit<double>(h, reps);
       if (t < bestt) { bestt = t; best = v; }
@@ -924,7 +925,7 @@ extern "C" int vrh_set_tuning(vrh_t *h, const int *choice) {
     return fail(h, "fft lines per block must be 2, 4, 8 or 16");
   if (choice[7] != 0 && choice[7] != 2 && choice[7] != 4 && choice[7] != 8 && choice[7] != 16)
     return fail(h, "dct lines per block must be 2, 4, 8 or 16");
-  if (choice[8] < -1 || choice[8] > 2) return fail(h, "explicit form must be -1, 0, 1 or 2");
+  if (choice[8] < -1 || choice[8] > 3) return fail(h, "explicit form must be -1, 0, 1, 2 or 3");
   for (int sw = 0; sw < 6; sw++) if (choice[sw]) h->Msw[sw] = choice[sw];
   if (choice[6]) h->fft_lines = choice[6];
   if (choice[7]) h->dct_lines = choice[7];

@@ -127,7 +127,9 @@ struct vrh_s {
   int Msw[6];        // segment length M of each sweep (partitioned Thomas)
   int fft_lines = 0;  // theta lines per block of the FFT kernels (0: default)
   int dct_lines = 0;  // z lines per block of the DCT kernels (0: default)
-  int explicit_variant = -1;  // 0: plane per block, 1: z marching, -1: default per precision
+  int explicit_variant = -1;  // 0: plane per block, 1: z marching, 2/3: one wave per r row, -1: default
+  // LDS a block may take beyond 64 KB (after the kernels' opt-in) and LDS per CU
+  int lds_block = 65536, lds_cu = 65536;
   int theta_levels[4];  // PCR levels of the theta solve per substep
   // transform tables (double): theta twiddles (nt/2 complex), z twiddles
   // (nz/2 complex), z post-twiddles (nz complex)
@@ -192,6 +194,7 @@ template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, const do
 void launch_scalars(vrh_t *h);
 void launch_analysis(vrh_t *h, double *part, int nblk, double *out);
 template <typename TW> void launch_explicit(vrh_t *h, int irk);
+void explicit_lds_optin(vrh_t *h);
 // hdnbc: also launch the outlet-row kernel of hdnbc at the head of the q3
 // sweep chain (whole steps), with the mass correction cor or *cor_dev
 template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const double *c3s_dev, bool hdnbc = false,

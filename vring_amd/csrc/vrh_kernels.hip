@@ -391,8 +391,8 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
   const int j0 = 1 + blockIdx.x * RJ;
   const int kb = 1 + blockIdx.y * kchunk, ke = min(kb + kchunk - 1, nz);
   const int LRW = RJ + 2, PL = LRW * NTP;
-  double *pcr = reinterpret_cast<double *>(smem_raw);             // [RJ][kMaxPcrLevels][2] + [RJ][2]
-  double *fin = pcr + RJ * kMaxPcrLevels * 2;
+  double *pcr = reinterpret_cast<double *>(smem_raw);             // [RJ][nlev][2] + [RJ][2]
+  double *fin = pcr + RJ * nlev * 2;
   TW *cfs = reinterpret_cast<TW *>(fin + RJ * 2);                 // [RJ][kRowCoefs]
   TW *ring = cfs + RJ * kRowCoefs;                                // [3 slots][3 arrays][LRW][NTP]
   TW *fb = ring + 9 * PL;                                         // [2][3][RJ][NTP]
@@ -404,8 +404,8 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
     double a2 = -(betadx * c.s2t11[jc]), b2 = 1.0 + 2.0 * (betadx * c.s2t11[jc]);
     for (int lvl = 0; lvl < nlev; ++lvl) {
       const double al1 = -a1 / b1, al2 = -a2 / b2;
-      pcr[(tj * kMaxPcrLevels + lvl) * 2] = al1;
-      pcr[(tj * kMaxPcrLevels + lvl) * 2 + 1] = al2;
+      pcr[(tj * nlev + lvl) * 2] = al1;
+      pcr[(tj * nlev + lvl) * 2 + 1] = al2;
       b1 = b1 + 2.0 * al1 * a1; a1 = al1 * a1;
       b2 = b2 + 2.0 * al2 * a2; a2 = al2 * a2;
     }
@@ -623,7 +623,7 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
     wave_sync();
     int sm = 1 % nt;
     for (int lvl = 0; lvl < nlev; ++lvl) {
-      const TW al1 = (TW)pcr[(tj * kMaxPcrLevels + lvl) * 2], al2 = (TW)pcr[(tj * kMaxPcrLevels + lvl) * 2 + 1];
+      const TW al1 = (TW)pcr[(tj * nlev + lvl) * 2], al2 = (TW)pcr[(tj * nlev + lvl) * 2 + 1];
       TW n1[CPL], n2[CPL], n3[CPL];
 #pragma unroll
       for (int cc = 0; cc < CPL; cc++) {
@@ -1935,17 +1935,39 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
   int rj = 512 / ntp;
   if (rj < 1) rj = 1;
   const int kchunk = 16;
-  if (var == 2) {
-    // one wavefront per r row: the most rows (<= 8) whose LDS fits 64 KB;
-    // the instantiations cover the shapes that fit, others fall back to the
-    // marching form.  WPE: waves per SIMD the LDS allows (register budget).
-    int rw = 8;
-    while (rw > 1 && lds(rw) > 65536) rw--;
+  // the wave form's PCR ratio table holds this substep's levels only
+  auto lds_w = [&](int rw) { return lds(rw) - sizeof(double) * (size_t)rw * 2 * (kMaxPcrLevels - nlev); };
+  if (var == 2 || var == 3) {
+    // one wavefront per r row.  Form 2: the most rows (<= 8) whose LDS fits
+    // 64 KB.  Form 3: among the instantiated row counts, the one that keeps
+    // the most waves resident on a CU within the device's LDS per block and
+    // per CU (ties: more rows, less r-halo re-read).  Shapes without an
+    // instantiation fall back to the marching form.  WPE: waves per SIMD the
+    // LDS allows (register budget).
     const int cpl = ntp / 64;
+    int rw = 8;
+    size_t lim = 65536;
+    if (var == 2) {
+      while (rw > 1 && lds(rw) > lim) rw--;
+    } else {
+      lim = (size_t)h->lds_block;
+      const int cand1[] = {8, 6}, cand2[] = {6, 4, 2}, cand4[] = {2};
+      const int *cd = cpl == 1 ? cand1 : cpl == 2 ? cand2 : cand4;
+      const int ncd = cpl == 1 ? 2 : cpl == 2 ? 3 : 1;
+      int best = 0;
+      rw = 0;
+      for (int i = 0; i < ncd; i++) {
+        const size_t L = lds_w(cd[i]);
+        if (L > lim) continue;
+        const int waves = cd[i] * (int)((size_t)h->lds_cu / L);
+        if (waves > best) { best = waves; rw = cd[i]; }
+      }
+      if (rw == 0) rw = 1;
+    }
     dim3 grd((g.nr + rw - 1) / rw, (g.nz + kchunk - 1) / kchunk);
-    bool done = lds(rw) <= 65536;
+    bool done = rw > 1 && (var == 2 ? lds(rw) : lds_w(rw)) <= lim;
 #define EXPL_WAVE(C, R, W)                                                                                          \
-  hipLaunchKernelGGL((k_explicit_wave<TW, C, R, W>), grd, dim3(64, R), lds(R), h->stream, g, h->coef, h->rk[irk],   \
+  hipLaunchKernelGGL((k_explicit_wave<TW, C, R, W>), grd, dim3(64, R), lds_w(R), h->stream, g, h->coef, h->rk[irk],   \
                      betadx, nlev, kchunk, h->f.q1, h->f.q2, h->f.q3, (const TW *)h->f.prg1, (const TW *)h->f.prg2, \
                      (const TW *)h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,          \
                      (TW *)h->f.r2, (TW *)h->f.r3)
@@ -1953,6 +1975,7 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
       if (cpl == 1 && rw == 8) EXPL_WAVE(1, 8, 4);
       else if (cpl == 1 && rw == 6) EXPL_WAVE(1, 6, 3);
       else if (cpl == 2 && rw == 6) EXPL_WAVE(2, 6, 3);
+      else if (cpl == 2 && rw == 4) EXPL_WAVE(2, 4, 2);
       else if (cpl == 2 && rw == 2) EXPL_WAVE(2, 2, 2);
       else if (cpl == 4 && rw == 2) EXPL_WAVE(4, 2, 2);
       else done = false;
@@ -1966,6 +1989,39 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
                      kchunk, h->f.q1, h->f.q2, h->f.q3, (const TW *)h->f.prg1, (const TW *)h->f.prg2,
                      (const TW *)h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,
                      (TW *)h->f.r2, (TW *)h->f.r3);
+}
+
+// Blocks of the explicit kernel's one-wave-per-row form may take more than
+// 64 KB of LDS where the device allows it (gfx950: 160 KB per CU).  The
+// instantiations that can exceed 64 KB opt in once per handle; if the
+// runtime refuses, the form keeps the 64 KB limit.
+void explicit_lds_optin(vrh_t *h) {
+  int dev = 0, lb = 0, lc = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lc, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
+    return;
+  if (lc < lb) lc = lb;
+  bool ok = true;
+  if (lb > 65536) {
+    const void *fns[] = {
+        reinterpret_cast<const void *>(&k_explicit_wave<double, 1, 8, 4>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, 1, 6, 3>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, 2, 6, 3>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, 2, 4, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, 2, 2, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, 4, 2, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, 1, 8, 4>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, 1, 6, 3>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, 2, 6, 3>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, 2, 4, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, 2, 2, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, 4, 2, 2>)};
+    for (const void *f : fns)
+      ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lb) == hipSuccess;
+  }
+  (void)hipGetLastError();
+  if (ok) { h->lds_block = lb; h->lds_cu = lc; }
 }
 
 template <typename TW, int MODE>
