@@ -38,20 +38,35 @@ PASSES_PER_STEP = 252
 
 
 def kernel_alg_bytes_per_cell(name, prec):
-    """Algorithmic bytes per cell of one launch of kernel `name`."""
+    """Algorithmic bytes per cell of one launch of kernel `name` (the phase
+    model's operands that the kernel's phases stream, each at its element
+    size in the path: the state q, predictor dq, phi, p and the pressure
+    gradient prg are FP64 in both paths; the work arrays ru, qrhs, the
+    spectral array and qcap take the work type)."""
     w = 8 if prec == "fp64" else 4   # work arrays
-    s = 8                            # state q, predictor dq, phi, p (always FP64)
-    if name == "explicit":           # A + B: in q(3), H(3); per comp q,H,ru,prg in, qrhs,ru out
-        return 3 * s + 3 * w + 3 * (s + 5 * w)
-    if name == "project":            # F + G: upsol in dq(3),phi out q(3) ... uppres 3x(phi in,qcap out,
-        # qcap in, prg in, prg out) + phi in, p in, p out
-        return (3 * s + s + 3 * s) + 3 * (s + w + w + w + w) + 3 * s
-    if name == "line_z":             # per launch: rhs in, q in, dq out (one component)
-        return w + s + s
-    if name == "line_r":             # rhs in, rhs out
-        return 2 * w
-    if name == "divfft":             # D: dq(3) in, rhs out; E: fft in/out
+    s = 8                            # FP64 operands
+    if name == "explicit":
+        # A: q1..3 in, H1..3 out; B per component: q, H, ru, prg in; qrhs, ru out
+        return 3 * s + 3 * w + 3 * (s + w + w + s + w + w)
+    if name == "project":
+        # F: dq1..3 and phi in, q1..3 out; G per component: phi in, qcap out,
+        # qcap in, prg in, prg out; then phi, p in, p out
+        return (3 * s + s + 3 * s) + 3 * (s + w + w + s + s) + 3 * s
+    if name == "line_r":
+        # per launch, averaged over the three: q1 and q3 r sweeps rhs in/out;
+        # q2's r sweep (after its z sweep) rhs and q2 in, dq2 out
+        return (2 * w + 2 * w + (w + 2 * s)) / 3.0
+    if name == "line_z":
+        # q1 and q3 z sweeps rhs and q in, dq out; q2's z sweep rhs in/out
+        return ((w + 2 * s) + (w + 2 * s) + 2 * w) / 3.0
+    if name == "divfft":             # D: dq(3) in, rhs out; E: FFT in/out
         return 3 * s + w + w + w
+    if name in ("dct_fwd", "dct_inv"):
+        return 2 * w                 # spectral array in/out
+    if name == "radial":
+        return 3 * w                 # spectral array in/out, pivots in
+    if name == "ifft":
+        return w + s                 # spectral array in, phi out
     return None
 
 
@@ -213,6 +228,31 @@ def run_learned(nr, nz, nslots, train_samples, seed=0, pool=8):
     return out
 
 
+def error_run(q, dt, steps=500, every=50):
+    """Mixed against FP64 from the same initial field over `steps` steps
+    (the workload's run length): per field, the largest error over the
+    checkpoints every `every` steps, relative to the velocity scale for q1,
+    q2, q3 and to max|p| for p, and the errors at the last step."""
+    import vring_amd
+    S = {}
+    for prec in ("fp64", "mixed"):
+        S[prec] = vring_amd.Solver(NT, NR, NZ, RMAX, ZMAX, RE, dt, precision=prec)
+        for n, a in zip(("q1", "q2", "q3"), q):
+            S[prec].upload(n, a)
+    worst = {n: 0.0 for n in ("q1", "q2", "q3", "pres")}
+    last = None
+    for _ in range(every, steps + 1, every):
+        for s in S.values():
+            s.step(every)
+        f = {p: {n: s.download(n) for n in worst} for p, s in S.items()}
+        last = rel_errors(f["mixed"], f["fp64"])
+        worst = {n: max(worst[n], last[n]) for n in worst}
+    for s in S.values():
+        s.close()
+    return {"steps": steps, "checkpoint_every": every, "worst": worst, "final": last,
+            "bar": 1e-6, "within_bar": max(worst.values()) < 1e-6}
+
+
 def rel_errors(a, b):
     vs = max(np.abs(b[n]).max() for n in ("q1", "q2", "q3"))
     out = {n: float(np.abs(a[n] - b[n]).max() / vs) for n in ("q1", "q2", "q3")}
@@ -221,6 +261,10 @@ def rel_errors(a, b):
 
 
 def roofline(kt, prec, ncell):
+    """The dominant kernel (most time per step) of a path: algorithmic bytes
+    per launch over its average launch time (HIP events on the solver's
+    stream), and the measured HBM traffic per launch (rocprofv3 counters,
+    profiles/traffic_256x128x512.json) over the same time."""
     best = max(kt.items(), key=lambda kv: kv[1][0])
     name, (ms, cnt) = best
     per_cell = kernel_alg_bytes_per_cell(name, prec)
@@ -234,9 +278,27 @@ def roofline(kt, prec, ncell):
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get(prec, {}).get(name)
-    return {"kernel": name, "avg_ms": round(avg_ms, 4), "bound": "hbm", "achieved": round(ach, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 3), "traffic": traffic,
-            "alg_bytes_per_cell": per_cell}
+    out = {"kernel": name, "avg_ms": round(avg_ms, 4), "bound": "hbm", "achieved": round(ach, 1),
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 3), "traffic": traffic,
+           "alg_bytes_per_cell": per_cell, "alg_bytes_per_launch": int(per_cell * ncell)}
+    if traffic:
+        out["counter_GBps"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+        out["frac_counter"] = round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)
+    return out
+
+
+def host_cpu():
+    """CPU model and logical core count of the host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
 
 
 def cpu_baseline(q, dt):
@@ -252,8 +314,10 @@ def cpu_baseline(q, dt):
     o.substep(1)
     t = time.perf_counter() - t0
     o.close()
+    model, ncpu = host_cpu()
     return {"value": round(NT * NR * NZ / (3.0 * t) / 1e6, 4), "unit": "Mcell-RK3-steps/s", "cores": 1,
-            "kind": "port", "sample": f"one RK3 substep at 256x128x512 took {t:.1f} s (x3 per step)"}
+            "kind": "port", "sample": f"one RK3 substep at 256x128x512 took {t:.1f} s (x3 per step)",
+            "cpu_model": model, "host_logical_cores": ncpu}
 
 
 def copy_calibration(mib, reps=3):
@@ -364,6 +428,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--particles", type=int, default=1000000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--error-steps", type=int, default=500,
+                    help="steps of the mixed-vs-FP64 error run (0: skip)")
     ap.add_argument("--autotune-reps", type=int, default=8)
     ap.add_argument("--only", choices=("fp64", "mixed", "particles"), default=None,
                     help="run one path only and print its timing (profiling runs)")
@@ -422,10 +488,16 @@ def main():
                  for prec in ("fp64", "mixed") for sch in ("lsq_euler", "trilinear_rk3")}
     v64 = ncell * args.steps / wall64 / 1e6
     vmx = ncell * args.steps / wallmx / 1e6
-    err = rel_errors(fmx, f64)
+    err = error_run(q, dt, args.error_steps) if args.error_steps else {"timed_run": rel_errors(fmx, f64)}
     rf = roofline(ktmx, "mixed", ncell)
     rf64 = roofline(kt64, "fp64", ncell)
     step_frac64 = PASSES_PER_STEP * 8 * ncell * (args.steps / wall64) / (HBM_PEAK_GBS * 1e9)
+    # the mixed path's phase-model bytes per step: the FP64 operands of the
+    # 84 passes at 8 B, the work arrays at 4 B
+    step_bytes_mx = 3 * ncell * sum(kernel_alg_bytes_per_cell(k, "mixed") * n for k, n in (
+        ("explicit", 1), ("line_r", 3), ("line_z", 3), ("divfft", 1), ("dct_fwd", 1), ("radial", 1),
+        ("dct_inv", 1), ("ifft", 1), ("project", 1)))
+    step_frac_mx = step_bytes_mx * (args.steps / wallmx) / (HBM_PEAK_GBS * 1e9)
     out = {
         "metric": "Mcell-RK3-steps/s at 256x128x512, mixed+learned vs plain FP64 @1e-6 err",
         "value": round(vmx, 2),
@@ -438,12 +510,17 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64 (mixed)",
+        "form": ("mixed precision: FP32 stencils, ADI sweeps, theta FFT, z DCT and radial solves of m >= 3; "
+                 "FP64 state, predictor, m = 0..2 pressure modes and the projection with the pressure "
+                 "gradient.  No learned solve in the step: on this uniform z grid the pressure solve is the "
+                 "direct FFT + DCT + tridiagonal one, exact and faster than any Krylov form"),
         "data": "synthetic: seeded Gaussian ring with m=2..8 core perturbations",
         "config": {"workload": "nr=256 ntheta=128 nz=512 Re=3000 ring, 1 GPU", "nr": NR, "ntheta": NT,
                    "nz": NZ, "dt": dt, "particles": args.particles},
         "fp64": {"value": round(v64, 2), "ms_per_step": round(wall64 / args.steps * 1e3, 4),
                  "step_hbm_frac": round(step_frac64, 3), "roofline": rf64},
         "mixed_over_fp64": round(vmx / v64, 3),
+        "step_hbm_frac": round(step_frac_mx, 3),
         "mixed_rel_err_vs_fp64": err,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ktmx.items()},
         "fp64_kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt64.items()},

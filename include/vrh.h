@@ -69,10 +69,22 @@ int vrh_last_error(vrh_t *h, char *buf, size_t len);
 int vrh_upload_fields(vrh_t *h, const double *q1, const double *q2, const double *q3,
                       const double *pres);
 int vrh_download_fields(vrh_t *h, double *q1, double *q2, double *q3, double *pres);
-/* Any named state array: q1 q2 q3 dq1 dq2 dq3 ru1 ru2 ru3 pres (haloed),
- * prg (ntheta,nr,nz,3), dph (haloed), csort dq3no corr3n (ntheta, nr+2). */
+/* Any named state array, as doubles (work arrays of the mixed path are
+ * converted):
+ *   q1 q2 q3 dq1 dq2 dq3 ru1 ru2 ru3 qrhs1 qrhs2 qrhs3 pres dph
+ *       haloed (ntheta, 0:nr+2, 0:nz+2)   ntheta*(nr+3)*(nz+3) values
+ *   prg                                   (ntheta, nr, nz, 3): 3*ntheta*nr*nz
+ *   prg1 prg2 prg3 spec                   (ntheta, nr, nz): ntheta*nr*nz
+ *   csort dq3no corr3n                    (ntheta, 0:nr+2): ntheta*(nr+3)
+ * (all three take the reference's dq3no shape (nthetac, 0:nrc+1) with
+ * nrc = nr + 1, solu_toolsII.F90:108; its csort and corr3n are
+ * (nthetac, nrc), solu_toolsII.F90:197-198, i.e. rows 1..nr+1 here).
+ * The host buffer must hold vrh_state_size(name) values. */
 int vrh_upload_state(vrh_t *h, const char *name, const double *host);
 int vrh_download_state(vrh_t *h, const char *name, double *host);
+/* Number of doubles vrh_upload_state/vrh_download_state move for `name` on
+ * `grid` (host only: no device call).  -1 for an unknown name. */
+int vrh_state_size(const vrh_grid *grid, const char *name, long *n);
 
 /* Simulation time (par%ntime).  vrh_step advances it as loop.F90:53 does. */
 int vrh_set_time(vrh_t *h, double ntime);

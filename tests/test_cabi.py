@@ -67,3 +67,26 @@ def test_create_rejects_grids_outside_the_limits_before_any_device_call():
         assert lib.vrh_create(ctypes.byref(VrhGrid(16, 32, 64, 4.0, 12.0)), ctypes.byref(p), 0, ctypes.byref(h)) == -1
     h = ctypes.c_void_p(1)
     assert lib.vrh_create(ctypes.byref(VrhGrid(16, 32, 64, 4.0, 12.0)), ctypes.byref(ok_p), 7, ctypes.byref(h)) == -1
+
+
+def test_state_sizes_match_the_header():
+    """vrh_state_size (host only) gives the element counts include/vrh.h
+    documents for every named array, and the Python shapes agree."""
+    nt, nr, nz = 5, 7, 11
+    H, C, PL = nt * (nr + 3) * (nz + 3), nt * nr * nz, nt * (nr + 3)
+    want = {n: H for n in ("q1", "q2", "q3", "dq1", "dq2", "dq3", "ru1", "ru2", "ru3",
+                           "qrhs1", "qrhs2", "qrhs3", "pres", "dph")}
+    want.update({n: C for n in ("prg1", "prg2", "prg3", "spec")})
+    want.update({"prg": 3 * C, "csort": PL, "dq3no": PL, "corr3n": PL})
+    for name, n in want.items():
+        assert vring_amd.lib.state_size(nt, nr, nz, name) == n, name
+    try:
+        vring_amd.lib.state_size(nt, nr, nz, "nope")
+        raise AssertionError("unknown name accepted")
+    except KeyError:
+        pass
+    s = object.__new__(vring_amd.Solver)  # shape_of only: no device handle
+    s.nt, s.nr, s.nz = nt, nr, nz
+    s.halo_shape, s.comp_shape, s.plane_shape = (nt, nr + 3, nz + 3), (nt, nr, nz), (nt, nr + 3)
+    for name, n in want.items():
+        assert int(np.prod(s.shape_of(name))) == n, name

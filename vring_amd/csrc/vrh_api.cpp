@@ -83,20 +83,47 @@ struct Named {
   size_t n;
   bool work;
 };
+// Element counts of the named state arrays (vrh_state_size): haloed fields
+// (nt, 0:nr+2, 0:nz+2), compact arrays (nt, nr, nz), prg (nt, nr, nz, 3)
+// and the outlet planes (nt, 0:nr+2).  0 for an unknown name.
+enum NameShape { NS_NONE = 0, NS_HALO, NS_COMPACT, NS_PRG, NS_PLANE };
+NameShape name_shape(const char *name) {
+  static const char *halo[] = {"q1", "q2", "q3", "dq1", "dq2", "dq3", "pres", "dph",
+                               "ru1", "ru2", "ru3", "qrhs1", "qrhs2", "qrhs3"};
+  static const char *comp[] = {"prg1", "prg2", "prg3", "spec"};
+  static const char *plane[] = {"csort", "dq3no", "corr3n"};
+  for (const char *n : halo) if (strcmp(n, name) == 0) return NS_HALO;
+  for (const char *n : comp) if (strcmp(n, name) == 0) return NS_COMPACT;
+  for (const char *n : plane) if (strcmp(n, name) == 0) return NS_PLANE;
+  if (strcmp(name, "prg") == 0) return NS_PRG;
+  return NS_NONE;
+}
+size_t shape_count(NameShape s, long nt, long nr, long nz) {
+  switch (s) {
+    case NS_HALO: return (size_t)(nt * (nr + 3) * (nz + 3));
+    case NS_COMPACT: return (size_t)(nt * nr * nz);
+    case NS_PRG: return (size_t)(3 * nt * nr * nz);
+    case NS_PLANE: return (size_t)(nt * (nr + 3));
+    default: return 0;
+  }
+}
+
 bool lookup(vrh_t *h, const char *name, Named &out) {
-  const size_t H = haloed(h), C = compact(h), PL = (size_t)h->nt * (h->nr + 3);
-  struct { const char *n; void *p; size_t c; bool w; } t[] = {
-      {"q1", h->f.q1, H, false}, {"q2", h->f.q2, H, false}, {"q3", h->f.q3, H, false},
-      {"dq1", h->f.dq1, H, false}, {"dq2", h->f.dq2, H, false}, {"dq3", h->f.dq3, H, false},
-      {"pres", h->f.pres, H, false}, {"dph", h->f.dph, H, false},
-      {"ru1", h->f.ru1, H, true}, {"ru2", h->f.ru2, H, true}, {"ru3", h->f.ru3, H, true},
-      {"qrhs1", h->f.r1, H, true}, {"qrhs2", h->f.r2, H, true}, {"qrhs3", h->f.r3, H, true},
-      {"prg1", h->f.prg1, C, true}, {"prg2", h->f.prg2, C, true}, {"prg3", h->f.prg3, C, true},
-      {"spec", h->f.S, C, true},
-      {"csort", h->f.csort, PL, false}, {"dq3no", h->f.dq3no, PL, false}, {"corr3n", h->f.corr3n, PL, false},
+  struct { const char *n; void *p; bool w; } t[] = {
+      {"q1", h->f.q1, false}, {"q2", h->f.q2, false}, {"q3", h->f.q3, false},
+      {"dq1", h->f.dq1, false}, {"dq2", h->f.dq2, false}, {"dq3", h->f.dq3, false},
+      {"pres", h->f.pres, false}, {"dph", h->f.dph, false},
+      {"ru1", h->f.ru1, true}, {"ru2", h->f.ru2, true}, {"ru3", h->f.ru3, true},
+      {"qrhs1", h->f.r1, true}, {"qrhs2", h->f.r2, true}, {"qrhs3", h->f.r3, true},
+      {"prg1", h->f.prg1, false}, {"prg2", h->f.prg2, false}, {"prg3", h->f.prg3, false},
+      {"spec", h->f.S, true},
+      {"csort", h->f.csort, false}, {"dq3no", h->f.dq3no, false}, {"corr3n", h->f.corr3n, false},
   };
   for (auto &e : t)
-    if (strcmp(e.n, name) == 0) { out = {e.p, e.c, e.w}; return true; }
+    if (strcmp(e.n, name) == 0) {
+      out = {e.p, shape_count(name_shape(name), h->nt, h->nr, h->nz), e.w};
+      return true;
+    }
   return false;
 }
 
@@ -321,8 +348,10 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
   for (auto p2 : dfs) if ((rc = dalloc(h, p2, H))) return rc;
   void **wfs[] = {&f.ru1, &f.ru2, &f.ru3, &f.r1, &f.r2, &f.r3};
   for (auto p2 : wfs) if ((rc = dalloc_bytes(h, p2, H * W))) return rc;
-  void **cfs[] = {&f.prg1, &f.prg2, &f.prg3, &f.S, &f.ib};
+  void **cfs[] = {&f.S, &f.ib};
   for (auto p2 : cfs) if ((rc = dalloc_bytes(h, p2, C * W))) return rc;
+  double **pgs[] = {&f.prg1, &f.prg2, &f.prg3};
+  for (auto p2 : pgs) if ((rc = dalloc(h, p2, C))) return rc;
   double **pls[] = {&f.csort, &f.dq3no, &f.corr3n};
   for (auto p2 : pls) if ((rc = dalloc(h, p2, PL))) return rc;
   if ((rc = dalloc(h, &f.flag, 1))) return rc;
@@ -404,7 +433,7 @@ int vrh_upload_state(vrh_t *h, const char *name, const double *host) {
   if (strcmp(name, "prg") == 0) {
     const size_t C = compact(h);
     void *d[3] = {h->f.prg1, h->f.prg2, h->f.prg3};
-    for (int c = 0; c < 3; c++) { int rc = upload_conv(h, d[c], host + c * C, C, true); if (rc) return rc; }
+    for (int c = 0; c < 3; c++) { int rc = upload_conv(h, d[c], host + c * C, C, false); if (rc) return rc; }
     return 0;
   }
   Named n;
@@ -417,12 +446,20 @@ int vrh_download_state(vrh_t *h, const char *name, double *host) {
   if (strcmp(name, "prg") == 0) {
     const size_t C = compact(h);
     void *d[3] = {h->f.prg1, h->f.prg2, h->f.prg3};
-    for (int c = 0; c < 3; c++) { int rc = download_conv(h, host + c * C, d[c], C, true); if (rc) return rc; }
+    for (int c = 0; c < 3; c++) { int rc = download_conv(h, host + c * C, d[c], C, false); if (rc) return rc; }
     return 0;
   }
   Named n;
   if (!lookup(h, name, n)) return fail(h, "unknown field name");
   return download_conv(h, host, n.p, n.n, n.work);
+}
+
+int vrh_state_size(const vrh_grid *grid, const char *name, long *n) {
+  if (!grid || !name || !n) return -1;
+  const NameShape s = name_shape(name);
+  if (s == NS_NONE || grid->ntheta < 1 || grid->nr < 1 || grid->nz < 1) return -1;
+  *n = (long)shape_count(s, grid->ntheta, grid->nr, grid->nz);
+  return 0;
 }
 
 int vrh_device_ptr(vrh_t *h, const char *name, void **ptr) {

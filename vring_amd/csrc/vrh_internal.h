@@ -54,7 +54,9 @@ enum SweepMode { SW_R1 = 0, SW_R3 = 1, SW_R2 = 2, SW_Z1 = 3, SW_Z3 = 4, SW_Z2 = 
 struct Fields {
   double *q1, *q2, *q3, *dq1, *dq2, *dq3, *pres, *dph;  // haloed, double
   void *ru1, *ru2, *ru3, *r1, *r2, *r3;                 // haloed, TW
-  void *prg1, *prg2, *prg3;                             // compact, TW
+  // compact, double in both paths: the accumulated pressure gradient of
+  // uppres is part of the projection correction, which stays FP64
+  double *prg1, *prg2, *prg3;
   void *S, *ib;                                         // compact spectral + radial pivots, TW
   double *S64, *ib64;                                   // low modes (5, nr, nz) in mixed mode
   double *csort, *dq3no, *corr3n;                       // planes (nt, nr+2)

@@ -84,11 +84,11 @@ __global__ void k_scalars(double *sc, double dt, double a1, double a2, double a3
 constexpr int kMaxPcrLevels = 40;
 constexpr int kRowCoefs = 24;
 
-template <typename TW>
-__global__ __launch_bounds__(512, sizeof(TW) == 4 ? 4 : 3) void k_explicit_march(VGrid g, VCoef c, VRK rk, double betadx, int nlev, int kchunk,
+template <typename TW, typename TC>
+__global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march(VGrid g, VCoef c, VRK rk, double betadx, int nlev, int kchunk,
                                                   const double *__restrict__ q1, const double *__restrict__ q2,
-                                                  const double *__restrict__ q3, const TW *__restrict__ prg1,
-                                                  const TW *__restrict__ prg2, const TW *__restrict__ prg3,
+                                                  const double *__restrict__ q3, const double *__restrict__ prg1,
+                                                  const double *__restrict__ prg2, const double *__restrict__ prg3,
                                                   TW *__restrict__ ru1, TW *__restrict__ ru2, TW *__restrict__ ru3,
                                                   TW *__restrict__ r1, TW *__restrict__ r2, TW *__restrict__ r3) {
   extern __shared__ unsigned char smem_raw[];
@@ -101,9 +101,9 @@ __global__ __launch_bounds__(512, sizeof(TW) == 4 ? 4 : 3) void k_explicit_march
   const int LRW = RJ + 2, PL = LRW * NTP;
   double *pcr = reinterpret_cast<double *>(smem_raw);             // [RJ][kMaxPcrLevels][2] + [RJ][2]
   double *fin = pcr + RJ * kMaxPcrLevels * 2;
-  TW *cfs = reinterpret_cast<TW *>(fin + RJ * 2);                 // [RJ][kRowCoefs]
-  TW *ring = cfs + RJ * kRowCoefs;                                // [3 slots][3 arrays][LRW][NTP]
-  TW *fb = ring + 9 * PL;                                         // [2][3][RJ][NTP]
+  TC *cfs = reinterpret_cast<TC *>(fin + RJ * 2);                 // [RJ][kRowCoefs]
+  TC *ring = cfs + RJ * kRowCoefs;                                // [3 slots][3 arrays][LRW][NTP]
+  TC *fb = ring + 9 * PL;                                         // [2][3][RJ][NTP]
   const int j = j0 + tj;
   const bool cell = (ti < nt) && (j <= nr);
   const int jc = (j <= nr) ? j : nr;
@@ -122,13 +122,13 @@ __global__ __launch_bounds__(512, sizeof(TW) == 4 ? 4 : 3) void k_explicit_march
     fin[tj * 2] = 1.0 / (pow2 ? b1 + 2.0 * a1 : b1);
     fin[tj * 2 + 1] = 1.0 / (pow2 ? b2 + 2.0 * a2 : b2);
     const int jm = (jc == 1) ? 1 : jc - 1;
-    TW *cf = cfs + tj * kRowCoefs;
-    cf[0] = (TW)c.h1d11[jc]; cf[1] = (TW)c.h1d22d[jc]; cf[2] = (TW)c.h1d22n[jc]; cf[3] = (TW)c.h1dare[jc];
-    cf[4] = (TW)c.h2d11[jc]; cf[5] = (TW)c.h2d22p[jc]; cf[6] = (TW)c.h2d22m[jc]; cf[7] = (TW)c.h2dare[jc];
-    cf[8] = (TW)c.s1t11[jc]; cf[9] = (TW)c.s1t2n[jc]; cf[10] = (TW)c.s1t2s[jc]; cf[11] = (TW)c.s1t2c[jc];
-    cf[12] = (TW)c.s2t11[jc]; cf[13] = (TW)c.s2t2p[jc]; cf[14] = (TW)c.s2t2c[jc]; cf[15] = (TW)c.s2t2m[jc];
-    cf[16] = (TW)c.s3t2n[jc]; cf[17] = (TW)c.s3t2s[jc];
-    cf[18] = (TW)(1.0 / c.rm[jc + 1]); cf[19] = (TW)(1.0 / c.rm[jc]); cf[20] = (TW)(1.0 / c.rm[jm]);
+    TC *cf = cfs + tj * kRowCoefs;
+    cf[0] = (TC)c.h1d11[jc]; cf[1] = (TC)c.h1d22d[jc]; cf[2] = (TC)c.h1d22n[jc]; cf[3] = (TC)c.h1dare[jc];
+    cf[4] = (TC)c.h2d11[jc]; cf[5] = (TC)c.h2d22p[jc]; cf[6] = (TC)c.h2d22m[jc]; cf[7] = (TC)c.h2dare[jc];
+    cf[8] = (TC)c.s1t11[jc]; cf[9] = (TC)c.s1t2n[jc]; cf[10] = (TC)c.s1t2s[jc]; cf[11] = (TC)c.s1t2c[jc];
+    cf[12] = (TC)c.s2t11[jc]; cf[13] = (TC)c.s2t2p[jc]; cf[14] = (TC)c.s2t2c[jc]; cf[15] = (TC)c.s2t2m[jc];
+    cf[16] = (TC)c.s3t2n[jc]; cf[17] = (TC)c.s3t2s[jc];
+    cf[18] = (TC)(1.0 / c.rm[jc + 1]); cf[19] = (TC)(1.0 / c.rm[jc]); cf[20] = (TC)(1.0 / c.rm[jm]);
   }
   // plane fetch: element e = tid + l*NTH of the (LRW x NTP) tile, three arrays
   constexpr int NL = 3;
@@ -144,16 +144,16 @@ __global__ __launch_bounds__(512, sizeof(TW) == 4 ? 4 : 3) void k_explicit_march
     lval[l] = (e < PL) && cc < nt && jj <= nr + 2;
     goff[l] = lval[l] ? (long)cc + (long)jj * sj : 0;
   }
-  TW pre[3][NL];
+  TC pre[3][NL];
   // unconditional loads (masked after) so that a plane's loads issue together
   auto fetch = [&](int kk) {
 #pragma unroll
     for (int l = 0; l < NL; l++) {
       const long o = goff[l] + (long)kk * sk;
       const double a1 = q1[o], a2 = q2[o], a3 = q3[o];
-      pre[0][l] = lval[l] ? (TW)a1 : (TW)0;
-      pre[1][l] = lval[l] ? (TW)a2 : (TW)0;
-      pre[2][l] = lval[l] ? (TW)a3 : (TW)0;
+      pre[0][l] = lval[l] ? (TC)a1 : (TC)0;
+      pre[1][l] = lval[l] ? (TC)a2 : (TC)0;
+      pre[2][l] = lval[l] ? (TC)a3 : (TC)0;
     }
   };
   auto stash = [&](int slot) {
@@ -170,9 +170,9 @@ __global__ __launch_bounds__(512, sizeof(TW) == 4 ? 4 : 3) void k_explicit_march
   fetch(kb); stash(s0);
   fetch(kb + 1);
   // per-row coefficients
-  const TW ga = (TW)rk.g, ro = (TW)rk.r, al = (TW)rk.a, dt = (TW)g.dt, aRe = (TW)(rk.a / g.Re);
-  const TW dx3 = (TW)g.dx3, dx3q = (TW)g.dx3q;
-  const TW *cf = cfs + tj * kRowCoefs;  // per-row metric coefficients (LDS, broadcast reads)
+  const TC ga = (TC)rk.g, ro = (TC)rk.r, al = (TC)rk.a, dt = (TC)g.dt, aRe = (TC)(rk.a / g.Re);
+  const TC dx3 = (TC)g.dx3, dx3q = (TC)g.dx3q;
+  const TC *cf = cfs + tj * kRowCoefs;  // per-row metric coefficients (LDS, broadcast reads)
 #define h1d11 cf[0]
 #define h1d22d cf[1]
 #define h1d22n cf[2]
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(512, sizeof(TW) == 4 ? 4 : 3) void k_explicit_march
   // ru and prg of the next plane stream in while this one is computed
   // (clamped in-bounds addresses for lanes and planes that do not use them)
   const int tic = min(ti, nt - 1);
-  TW n_ru[3], n_pg[3];
+  TC n_ru[3], n_pg[3];
   auto load_rp = [&](int kk) {
     const int kc = min(kk, ke);
     const long oo = (long)tic + (long)jc * sj + (long)kc * sk;
@@ -209,12 +209,12 @@ __global__ __launch_bounds__(512, sizeof(TW) == 4 ? 4 : 3) void k_explicit_march
   };
   // FP32: one plane ahead; FP64 (register budget): this plane, issued with
   // the ring fetch
-  constexpr bool kAhead = sizeof(TW) == 4;
+  constexpr bool kAhead = sizeof(TC) == 4;
   if (kAhead) load_rp(kb);
   for (int k = kb; k <= ke; k++) {
     stash(sP);
-    TW c_ru1 = n_ru[0], c_ru2 = n_ru[1], c_ru3 = n_ru[2];
-    TW c_pg1 = n_pg[0], c_pg2 = n_pg[1], c_pg3 = n_pg[2];
+    TC c_ru1 = n_ru[0], c_ru2 = n_ru[1], c_ru3 = n_ru[2];
+    TC c_pg1 = n_pg[0], c_pg2 = n_pg[1], c_pg3 = n_pg[2];
     __syncthreads();
     fetch(k + 2);
     if (kAhead) {
@@ -224,102 +224,102 @@ __global__ __launch_bounds__(512, sizeof(TW) == 4 ? 4 : 3) void k_explicit_march
       c_ru1 = n_ru[0]; c_ru2 = n_ru[1]; c_ru3 = n_ru[2];
       c_pg1 = n_pg[0]; c_pg2 = n_pg[1]; c_pg3 = n_pg[2];
     }
-    TW v1 = (TW)0, v2 = (TW)0, v3 = (TW)0;
+    TC v1 = (TC)0, v2 = (TC)0, v3 = (TC)0;
     const long o = (long)ti + (long)j * sj + (long)k * sk;
     if (cell) {
-      const TW *A1m = ring + (sM * 3 + 0) * PL, *A1 = ring + (s0 * 3 + 0) * PL, *A1p = ring + (sP * 3 + 0) * PL;
-      const TW *A2m = ring + (sM * 3 + 1) * PL, *A2 = ring + (s0 * 3 + 1) * PL, *A2p = ring + (sP * 3 + 1) * PL;
-      const TW *A3m = ring + (sM * 3 + 2) * PL, *A3 = ring + (s0 * 3 + 2) * PL, *A3p = ring + (sP * 3 + 2) * PL;
+      const TC *A1m = ring + (sM * 3 + 0) * PL, *A1 = ring + (s0 * 3 + 0) * PL, *A1p = ring + (sP * 3 + 0) * PL;
+      const TC *A2m = ring + (sM * 3 + 1) * PL, *A2 = ring + (s0 * 3 + 1) * PL, *A2p = ring + (sP * 3 + 1) * PL;
+      const TC *A3m = ring + (sM * 3 + 2) * PL, *A3 = ring + (s0 * 3 + 2) * PL, *A3p = ring + (sP * 3 + 2) * PL;
 #define L(A, row, col) A[(row) * NTP + (col)]
-      const TW q1c = L(A1, rw, ci), q1ip = L(A1, rw, cip), q1im = L(A1, rw, cim);
-      const TW q1n = L(A1, rw + 1, ci), q1jm = L(A1, rm1, ci), q1s = L(A1, rw - 1, ci), q1ips = L(A1, rw - 1, cip);
-      const TW q1kp = L(A1p, rw, ci), q1km = L(A1m, rw, ci), q1ipkm = L(A1m, rw, cip);
-      const TW q2c = L(A2, rw, ci), q2n = L(A2, rw + 1, ci), q2imn = L(A2, rw + 1, cim), q2im = L(A2, rw, cim);
-      const TW q2ip = L(A2, rw, cip), q2s = L(A2, rw - 1, ci), q2kp = L(A2p, rw, ci), q2km = L(A2m, rw, ci);
-      const TW q2nkm = L(A2m, rw + 1, ci);
-      const TW q3c = L(A3, rw, ci), q3kp = L(A3p, rw, ci), q3imkp = L(A3p, rw, cim), q3im = L(A3, rw, cim);
-      const TW q3ip = L(A3, rw, cip), q3s = L(A3, rw - 1, ci), q3skp = L(A3p, rw - 1, ci), q3n = L(A3, rw + 1, ci);
-      const TW q3jm = L(A3, rm1, ci), q3km = L(A3m, rw, ci);
+      const TC q1c = L(A1, rw, ci), q1ip = L(A1, rw, cip), q1im = L(A1, rw, cim);
+      const TC q1n = L(A1, rw + 1, ci), q1jm = L(A1, rm1, ci), q1s = L(A1, rw - 1, ci), q1ips = L(A1, rw - 1, cip);
+      const TC q1kp = L(A1p, rw, ci), q1km = L(A1m, rw, ci), q1ipkm = L(A1m, rw, cip);
+      const TC q2c = L(A2, rw, ci), q2n = L(A2, rw + 1, ci), q2imn = L(A2, rw + 1, cim), q2im = L(A2, rw, cim);
+      const TC q2ip = L(A2, rw, cip), q2s = L(A2, rw - 1, ci), q2kp = L(A2p, rw, ci), q2km = L(A2m, rw, ci);
+      const TC q2nkm = L(A2m, rw + 1, ci);
+      const TC q3c = L(A3, rw, ci), q3kp = L(A3p, rw, ci), q3imkp = L(A3p, rw, cim), q3im = L(A3, rw, cim);
+      const TC q3ip = L(A3, rw, cip), q3s = L(A3, rw - 1, ci), q3skp = L(A3p, rw - 1, ci), q3n = L(A3, rw + 1, ci);
+      const TC q3jm = L(A3, rm1, ci), q3km = L(A3m, rw, ci);
 #undef L
       // ---- q1 at (theta_c, r_m, z_m) ----
       {
-        TW qnn = q1ip + q1c, qss = q1im + q1c;
-        TW h11 = (qnn * qnn - qss * qss) * h1d11;
+        TC qnn = q1ip + q1c, qss = q1im + q1c;
+        TC h11 = (qnn * qnn - qss * qss) * h1d11;
         qnn = (q2n + q2imn) * (q1n + q1c);
         qss = (q1c + q1jm) * (q2c + q2im);
-        TW h12d = (qnn - qss) * h1d22d;
+        TC h12d = (qnn - qss) * h1d22d;
         qnn = (q1ip + q1c) * (q2n + q2c);
         qss = (q1c + q1im) * (q2imn + q2im);
-        TW h12n = (qnn + qss) * h1d22n;
-        qnn = (TW)0.25 * (q3kp + q3imkp) * (q1kp + q1c);
-        qss = (k == 1) ? (TW)0 : (TW)0.25 * (q3c + q3im) * (q1c + q1km);
-        TW h13 = (qnn - qss) * dx3;
+        TC h12n = (qnn + qss) * h1d22n;
+        qnn = (TC)0.25 * (q3kp + q3imkp) * (q1kp + q1c);
+        qss = (k == 1) ? (TC)0 : (TC)0.25 * (q3c + q3im) * (q1c + q1km);
+        TC h13 = (qnn - qss) * dx3;
         qnn = q2n + q2c;
         qss = q2imn + q2im;
-        TW d1q2 = (qnn - qss) * h1dare;
-        TW H1 = d1q2 - (h11 + (h12d + h12n) + h13);
-        TW d11 = (q1ip - (TW)2 * q1c + q1im) * s1t11;
-        TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (q1kp - (TW)3 * q1c) * dx3q : (q1kp - (TW)2 * q1c + q1km) * dx3q;
-        TW qn2 = (q1n * irmn - q1c * irm) * s1t2n;
-        TW qs2 = (q1c * irm - q1jm * irmm) * s1t2s;
-        TW d22 = (qn2 - qs2) * s1t2c;
+        TC d1q2 = (qnn - qss) * h1dare;
+        TC H1 = d1q2 - (h11 + (h12d + h12n) + h13);
+        TC d11 = (q1ip - (TC)2 * q1c + q1im) * s1t11;
+        TC d33 = (k == 1) ? (TC)(4.0 / 3.0) * (q1kp - (TC)3 * q1c) * dx3q : (q1kp - (TC)2 * q1c + q1km) * dx3q;
+        TC qn2 = (q1n * irmn - q1c * irm) * s1t2n;
+        TC qs2 = (q1c * irm - q1jm * irmm) * s1t2s;
+        TC d22 = (qn2 - qs2) * s1t2c;
         v1 = (ga * H1 + ro * c_ru1 - al * c_pg1 + aRe * (d11 + d33 + d22)) * dt;
         ru1[o] = H1;
       }
       // ---- q2 at (theta_m, r_c, z_m), j >= 2 ----
       if (j >= 2) {
-        TW qnn = (q1ip + q1ips) * (q2ip + q2c);
-        TW qss = (q1c + q1s) * (q2c + q2im);
-        TW h21 = (qnn - qss) * h2d11;
+        TC qnn = (q1ip + q1ips) * (q2ip + q2c);
+        TC qss = (q1c + q1s) * (q2c + q2im);
+        TC h21 = (qnn - qss) * h2d11;
         qnn = q2n + q2c;
         qss = q2c + q2s;
-        TW h22 = qnn * qnn * h2d22p - qss * qss * h2d22m;
-        qnn = (TW)0.5 * (q1c + q1ip);
-        qss = (TW)0.5 * (q1s + q1ips);
-        TW h11n = (TW)0.25 * (qnn + qss) * (qnn + qss);
-        qnn = (TW)0.25 * (q3kp + q3skp) * (q2kp + q2c);
-        qss = (k == 1) ? (TW)0 : (TW)0.25 * (q3c + q3s) * (q2c + q2km);
-        TW h23 = (qnn - qss) * dx3;
+        TC h22 = qnn * qnn * h2d22p - qss * qss * h2d22m;
+        qnn = (TC)0.5 * (q1c + q1ip);
+        qss = (TC)0.5 * (q1s + q1ips);
+        TC h11n = (TC)0.25 * (qnn + qss) * (qnn + qss);
+        qnn = (TC)0.25 * (q3kp + q3skp) * (q2kp + q2c);
+        qss = (k == 1) ? (TC)0 : (TC)0.25 * (q3c + q3s) * (q2c + q2km);
+        TC h23 = (qnn - qss) * dx3;
         qnn = q1ip + q1ips;
         qss = q1c + q1s;
-        TW d1q1 = -((qnn - qss) * h2dare);
-        TW H2 = d1q1 - (h21 + h22 + h23 - h11n);
-        TW d11 = (q2ip - (TW)2 * q2c + q2im) * s2t11;
-        TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (q2kp - (TW)3 * q2c) * dx3q : (q2kp - (TW)2 * q2c + q2km) * dx3q;
-        TW d22 = q2n * s2t2p - q2c * s2t2c + q2s * s2t2m;
+        TC d1q1 = -((qnn - qss) * h2dare);
+        TC H2 = d1q1 - (h21 + h22 + h23 - h11n);
+        TC d11 = (q2ip - (TC)2 * q2c + q2im) * s2t11;
+        TC d33 = (k == 1) ? (TC)(4.0 / 3.0) * (q2kp - (TC)3 * q2c) * dx3q : (q2kp - (TC)2 * q2c + q2km) * dx3q;
+        TC d22 = q2n * s2t2p - q2c * s2t2c + q2s * s2t2m;
         v2 = (ga * H2 + ro * c_ru2 - al * c_pg2 + aRe * (d11 + d33 + d22)) * dt;
         ru2[o] = H2;
       }
       // ---- q3 at (theta_m, r_m, z_c), k >= 2 ----
       if (k >= 2) {
-        TW qnn = (q3n + q3c) * (q2n + q2nkm);
-        TW qss = (q3jm + q3c) * (q2c + q2km);
-        TW h32 = (qnn - qss) * h1d22d;
+        TC qnn = (q3n + q3c) * (q2n + q2nkm);
+        TC qss = (q3jm + q3c) * (q2c + q2km);
+        TC h32 = (qnn - qss) * h1d22d;
         qnn = (q3c + q3ip) * (q1ip + q1ipkm);
         qss = (q3im + q3c) * (q1c + q1km);
-        TW h31 = (qnn - qss) * h1d11;
-        qnn = (TW)0.25 * (q3c + q3kp) * (q3c + q3kp);
-        qss = (TW)0.25 * (q3c + q3km) * (q3c + q3km);
-        TW h33 = (qnn - qss) * dx3;
-        TW H3 = -(h31 + h32 + h33);
-        TW d11 = (q3ip - (TW)2 * q3c + q3im) * s1t11;
-        TW d33 = (q3kp - (TW)2 * q3c + q3km) * dx3q;
-        TW qn2 = (q3n - q3c) * s3t2n;
-        TW qs2 = (q3c - q3jm) * s3t2s;
-        TW d22 = qn2 - qs2;
+        TC h31 = (qnn - qss) * h1d11;
+        qnn = (TC)0.25 * (q3c + q3kp) * (q3c + q3kp);
+        qss = (TC)0.25 * (q3c + q3km) * (q3c + q3km);
+        TC h33 = (qnn - qss) * dx3;
+        TC H3 = -(h31 + h32 + h33);
+        TC d11 = (q3ip - (TC)2 * q3c + q3im) * s1t11;
+        TC d33 = (q3kp - (TC)2 * q3c + q3km) * dx3q;
+        TC qn2 = (q3n - q3c) * s3t2n;
+        TC qs2 = (q3c - q3jm) * s3t2s;
+        TC d22 = qn2 - qs2;
         v3 = (ga * H3 + ro * c_ru3 - al * c_pg3 + aRe * (d11 + d33 + d22)) * dt;
         ru3[o] = H3;
       }
     }
     // ---- periodic theta solves by PCR (ping-pong buffers: one barrier per level) ----
     const long S3 = 3L * RJ * NTP;
-    TW *f1 = fb + tj * NTP, *f2 = f1 + RJ * NTP, *f3 = f2 + RJ * NTP;
+    TC *f1 = fb + tj * NTP, *f2 = f1 + RJ * NTP, *f3 = f2 + RJ * NTP;
     f1[ti] = v1; f2[ti] = v2; f3[ti] = v3;
     __syncthreads();
     int sm = 1 % nt;
     for (int lvl = 0; lvl < nlev; ++lvl) {
-      const TW al1 = (TW)pcr[(tj * kMaxPcrLevels + lvl) * 2], al2 = (TW)pcr[(tj * kMaxPcrLevels + lvl) * 2 + 1];
-      TW n1 = v1, n2 = v2, n3 = v3;
+      const TC al1 = (TC)pcr[(tj * kMaxPcrLevels + lvl) * 2], al2 = (TC)pcr[(tj * kMaxPcrLevels + lvl) * 2 + 1];
+      TC n1 = v1, n2 = v2, n3 = v3;
       if (ti < nt) {
         int imv = ti - sm; if (imv < 0) imv += nt;
         int ipv = ti + sm; if (ipv >= nt) ipv -= nt;
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(512, sizeof(TW) == 4 ? 4 : 3) void k_explicit_march
       __syncthreads();
     }
     if (cell) {
-      const TW d1 = (TW)fin[tj * 2], d2 = (TW)fin[tj * 2 + 1];
+      const TC d1 = (TC)fin[tj * 2], d2 = (TC)fin[tj * 2 + 1];
       r1[o] = v1 * d1;
       if (j >= 2) r2[o] = v2 * d2;
       if (k >= 2) r3[o] = v3 * d1;
@@ -374,11 +374,11 @@ __global__ __launch_bounds__(512, sizeof(TW) == 4 ? 4 : 3) void k_explicit_march
 // PCR inside the wavefront (LDS exchange ordered by wave-local fences, no
 // block barrier per level), which leaves two block barriers per z plane for
 // the plane ring.  Same arithmetic as k_explicit_march.
-template <typename TW, int CPL, int RJT, int WPE>
+template <typename TW, typename TC, int CPL, int RJT, int WPE>
 __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef c, VRK rk, double betadx, int nlev, int kchunk,
                                                   const double *__restrict__ q1, const double *__restrict__ q2,
-                                                  const double *__restrict__ q3, const TW *__restrict__ prg1,
-                                                  const TW *__restrict__ prg2, const TW *__restrict__ prg3,
+                                                  const double *__restrict__ q3, const double *__restrict__ prg1,
+                                                  const double *__restrict__ prg2, const double *__restrict__ prg3,
                                                   TW *__restrict__ ru1, TW *__restrict__ ru2, TW *__restrict__ ru3,
                                                   TW *__restrict__ r1, TW *__restrict__ r2, TW *__restrict__ r3) {
   extern __shared__ unsigned char smem_raw[];
@@ -393,9 +393,9 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
   const int LRW = RJ + 2, PL = LRW * NTP;
   double *pcr = reinterpret_cast<double *>(smem_raw);             // [RJ][nlev][2] + [RJ][2]
   double *fin = pcr + RJ * nlev * 2;
-  TW *cfs = reinterpret_cast<TW *>(fin + RJ * 2);                 // [RJ][kRowCoefs]
-  TW *ring = cfs + RJ * kRowCoefs;                                // [3 slots][3 arrays][LRW][NTP]
-  TW *fb = ring + 9 * PL;                                         // [2][3][RJ][NTP]
+  TC *cfs = reinterpret_cast<TC *>(fin + RJ * 2);                 // [RJ][kRowCoefs]
+  TC *ring = cfs + RJ * kRowCoefs;                                // [3 slots][3 arrays][LRW][NTP]
+  TC *fb = ring + 9 * PL;                                         // [2][3][RJ][NTP]
   const int j = j0 + tj;
   const int jc = (j <= nr) ? j : nr;
   // PCR ratios of this row (one lane per row)
@@ -413,13 +413,13 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
     fin[tj * 2] = 1.0 / (pow2 ? b1 + 2.0 * a1 : b1);
     fin[tj * 2 + 1] = 1.0 / (pow2 ? b2 + 2.0 * a2 : b2);
     const int jm = (jc == 1) ? 1 : jc - 1;
-    TW *cf = cfs + tj * kRowCoefs;
-    cf[0] = (TW)c.h1d11[jc]; cf[1] = (TW)c.h1d22d[jc]; cf[2] = (TW)c.h1d22n[jc]; cf[3] = (TW)c.h1dare[jc];
-    cf[4] = (TW)c.h2d11[jc]; cf[5] = (TW)c.h2d22p[jc]; cf[6] = (TW)c.h2d22m[jc]; cf[7] = (TW)c.h2dare[jc];
-    cf[8] = (TW)c.s1t11[jc]; cf[9] = (TW)c.s1t2n[jc]; cf[10] = (TW)c.s1t2s[jc]; cf[11] = (TW)c.s1t2c[jc];
-    cf[12] = (TW)c.s2t11[jc]; cf[13] = (TW)c.s2t2p[jc]; cf[14] = (TW)c.s2t2c[jc]; cf[15] = (TW)c.s2t2m[jc];
-    cf[16] = (TW)c.s3t2n[jc]; cf[17] = (TW)c.s3t2s[jc];
-    cf[18] = (TW)(1.0 / c.rm[jc + 1]); cf[19] = (TW)(1.0 / c.rm[jc]); cf[20] = (TW)(1.0 / c.rm[jm]);
+    TC *cf = cfs + tj * kRowCoefs;
+    cf[0] = (TC)c.h1d11[jc]; cf[1] = (TC)c.h1d22d[jc]; cf[2] = (TC)c.h1d22n[jc]; cf[3] = (TC)c.h1dare[jc];
+    cf[4] = (TC)c.h2d11[jc]; cf[5] = (TC)c.h2d22p[jc]; cf[6] = (TC)c.h2d22m[jc]; cf[7] = (TC)c.h2dare[jc];
+    cf[8] = (TC)c.s1t11[jc]; cf[9] = (TC)c.s1t2n[jc]; cf[10] = (TC)c.s1t2s[jc]; cf[11] = (TC)c.s1t2c[jc];
+    cf[12] = (TC)c.s2t11[jc]; cf[13] = (TC)c.s2t2p[jc]; cf[14] = (TC)c.s2t2c[jc]; cf[15] = (TC)c.s2t2m[jc];
+    cf[16] = (TC)c.s3t2n[jc]; cf[17] = (TC)c.s3t2s[jc];
+    cf[18] = (TC)(1.0 / c.rm[jc + 1]); cf[19] = (TC)(1.0 / c.rm[jc]); cf[20] = (TC)(1.0 / c.rm[jm]);
   }
   // plane fetch: element e = tid + l*NTH of the (LRW x NTP) tile, three arrays
   constexpr int NL = (CPL * (RJT + 2) + RJT - 1) / RJT;  // ceil((RJ + 2) NTP / (64 RJ))
@@ -435,16 +435,16 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
     lval[l] = (e < PL) && cc < nt && jj <= nr + 2;
     goff[l] = lval[l] ? (long)cc + (long)jj * sj : 0;
   }
-  TW pre[3][NL];
+  TC pre[3][NL];
   // unconditional loads (masked after) so that a plane's loads issue together
   auto fetch = [&](int kk) {
 #pragma unroll
     for (int l = 0; l < NL; l++) {
       const long o = goff[l] + (long)kk * sk;
       const double a1 = q1[o], a2 = q2[o], a3 = q3[o];
-      pre[0][l] = lval[l] ? (TW)a1 : (TW)0;
-      pre[1][l] = lval[l] ? (TW)a2 : (TW)0;
-      pre[2][l] = lval[l] ? (TW)a3 : (TW)0;
+      pre[0][l] = lval[l] ? (TC)a1 : (TC)0;
+      pre[1][l] = lval[l] ? (TC)a2 : (TC)0;
+      pre[2][l] = lval[l] ? (TC)a3 : (TC)0;
     }
   };
   auto stash = [&](int slot) {
@@ -461,9 +461,9 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
   fetch(kb); stash(s0);
   fetch(kb + 1);
   // per-row coefficients
-  const TW ga = (TW)rk.g, ro = (TW)rk.r, al = (TW)rk.a, dt = (TW)g.dt, aRe = (TW)(rk.a / g.Re);
-  const TW dx3 = (TW)g.dx3, dx3q = (TW)g.dx3q;
-  const TW *cf = cfs + tj * kRowCoefs;  // per-row metric coefficients (LDS, broadcast reads)
+  const TC ga = (TC)rk.g, ro = (TC)rk.r, al = (TC)rk.a, dt = (TC)g.dt, aRe = (TC)(rk.a / g.Re);
+  const TC dx3 = (TC)g.dx3, dx3q = (TC)g.dx3q;
+  const TC *cf = cfs + tj * kRowCoefs;  // per-row metric coefficients (LDS, broadcast reads)
 #define h1d11 cf[0]
 #define h1d22d cf[1]
 #define h1d22n cf[2]
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
   const int rw = tj + 1, rm1 = (jc == 1) ? rw : rw - 1;   // LDS rows of j and jm
   // ru and prg of the next plane stream in while this one is computed
   // (clamped in-bounds addresses for lanes and planes that do not use them)
-  TW n_ru[CPL][3], n_pg[CPL][3];
+  TC n_ru[CPL][3], n_pg[CPL][3];
   auto load_rp = [&](int kk) {
     const int kc = min(kk, ke);
 #pragma unroll
@@ -507,7 +507,7 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
   load_rp(kb);
   for (int k = kb; k <= ke; k++) {
     stash(sP);
-    TW cru[CPL][3], cpg[CPL][3];
+    TC cru[CPL][3], cpg[CPL][3];
 #pragma unroll
     for (int cc = 0; cc < CPL; cc++)
 #pragma unroll
@@ -515,97 +515,97 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
     __syncthreads();
     fetch(k + 2);
     load_rp(k + 1);
-    TW w1[CPL], w2[CPL], w3[CPL];
+    TC w1[CPL], w2[CPL], w3[CPL];
 #pragma unroll
     for (int cc = 0; cc < CPL; cc++) {
       const int ti = lane + 64 * cc;
       const int ci = ti, cip = (ti + 1 >= nt) ? 0 : ti + 1, cim = (ti == 0) ? nt - 1 : ti - 1;
       const bool cell = (ti < nt) && (j <= nr);
-      const TW c_ru1 = cru[cc][0], c_ru2 = cru[cc][1], c_ru3 = cru[cc][2];
-      const TW c_pg1 = cpg[cc][0], c_pg2 = cpg[cc][1], c_pg3 = cpg[cc][2];
-      TW v1 = (TW)0, v2 = (TW)0, v3 = (TW)0;
+      const TC c_ru1 = cru[cc][0], c_ru2 = cru[cc][1], c_ru3 = cru[cc][2];
+      const TC c_pg1 = cpg[cc][0], c_pg2 = cpg[cc][1], c_pg3 = cpg[cc][2];
+      TC v1 = (TC)0, v2 = (TC)0, v3 = (TC)0;
       const long o = (long)ti + (long)j * sj + (long)k * sk;
       if (cell) {
-      const TW *A1m = ring + (sM * 3 + 0) * PL, *A1 = ring + (s0 * 3 + 0) * PL, *A1p = ring + (sP * 3 + 0) * PL;
-      const TW *A2m = ring + (sM * 3 + 1) * PL, *A2 = ring + (s0 * 3 + 1) * PL, *A2p = ring + (sP * 3 + 1) * PL;
-      const TW *A3m = ring + (sM * 3 + 2) * PL, *A3 = ring + (s0 * 3 + 2) * PL, *A3p = ring + (sP * 3 + 2) * PL;
+      const TC *A1m = ring + (sM * 3 + 0) * PL, *A1 = ring + (s0 * 3 + 0) * PL, *A1p = ring + (sP * 3 + 0) * PL;
+      const TC *A2m = ring + (sM * 3 + 1) * PL, *A2 = ring + (s0 * 3 + 1) * PL, *A2p = ring + (sP * 3 + 1) * PL;
+      const TC *A3m = ring + (sM * 3 + 2) * PL, *A3 = ring + (s0 * 3 + 2) * PL, *A3p = ring + (sP * 3 + 2) * PL;
 #define L(A, row, col) A[(row) * NTP + (col)]
-      const TW q1c = L(A1, rw, ci), q1ip = L(A1, rw, cip), q1im = L(A1, rw, cim);
-      const TW q1n = L(A1, rw + 1, ci), q1jm = L(A1, rm1, ci), q1s = L(A1, rw - 1, ci), q1ips = L(A1, rw - 1, cip);
-      const TW q1kp = L(A1p, rw, ci), q1km = L(A1m, rw, ci), q1ipkm = L(A1m, rw, cip);
-      const TW q2c = L(A2, rw, ci), q2n = L(A2, rw + 1, ci), q2imn = L(A2, rw + 1, cim), q2im = L(A2, rw, cim);
-      const TW q2ip = L(A2, rw, cip), q2s = L(A2, rw - 1, ci), q2kp = L(A2p, rw, ci), q2km = L(A2m, rw, ci);
-      const TW q2nkm = L(A2m, rw + 1, ci);
-      const TW q3c = L(A3, rw, ci), q3kp = L(A3p, rw, ci), q3imkp = L(A3p, rw, cim), q3im = L(A3, rw, cim);
-      const TW q3ip = L(A3, rw, cip), q3s = L(A3, rw - 1, ci), q3skp = L(A3p, rw - 1, ci), q3n = L(A3, rw + 1, ci);
-      const TW q3jm = L(A3, rm1, ci), q3km = L(A3m, rw, ci);
+      const TC q1c = L(A1, rw, ci), q1ip = L(A1, rw, cip), q1im = L(A1, rw, cim);
+      const TC q1n = L(A1, rw + 1, ci), q1jm = L(A1, rm1, ci), q1s = L(A1, rw - 1, ci), q1ips = L(A1, rw - 1, cip);
+      const TC q1kp = L(A1p, rw, ci), q1km = L(A1m, rw, ci), q1ipkm = L(A1m, rw, cip);
+      const TC q2c = L(A2, rw, ci), q2n = L(A2, rw + 1, ci), q2imn = L(A2, rw + 1, cim), q2im = L(A2, rw, cim);
+      const TC q2ip = L(A2, rw, cip), q2s = L(A2, rw - 1, ci), q2kp = L(A2p, rw, ci), q2km = L(A2m, rw, ci);
+      const TC q2nkm = L(A2m, rw + 1, ci);
+      const TC q3c = L(A3, rw, ci), q3kp = L(A3p, rw, ci), q3imkp = L(A3p, rw, cim), q3im = L(A3, rw, cim);
+      const TC q3ip = L(A3, rw, cip), q3s = L(A3, rw - 1, ci), q3skp = L(A3p, rw - 1, ci), q3n = L(A3, rw + 1, ci);
+      const TC q3jm = L(A3, rm1, ci), q3km = L(A3m, rw, ci);
 #undef L
       // ---- q1 at (theta_c, r_m, z_m) ----
       {
-        TW qnn = q1ip + q1c, qss = q1im + q1c;
-        TW h11 = (qnn * qnn - qss * qss) * h1d11;
+        TC qnn = q1ip + q1c, qss = q1im + q1c;
+        TC h11 = (qnn * qnn - qss * qss) * h1d11;
         qnn = (q2n + q2imn) * (q1n + q1c);
         qss = (q1c + q1jm) * (q2c + q2im);
-        TW h12d = (qnn - qss) * h1d22d;
+        TC h12d = (qnn - qss) * h1d22d;
         qnn = (q1ip + q1c) * (q2n + q2c);
         qss = (q1c + q1im) * (q2imn + q2im);
-        TW h12n = (qnn + qss) * h1d22n;
-        qnn = (TW)0.25 * (q3kp + q3imkp) * (q1kp + q1c);
-        qss = (k == 1) ? (TW)0 : (TW)0.25 * (q3c + q3im) * (q1c + q1km);
-        TW h13 = (qnn - qss) * dx3;
+        TC h12n = (qnn + qss) * h1d22n;
+        qnn = (TC)0.25 * (q3kp + q3imkp) * (q1kp + q1c);
+        qss = (k == 1) ? (TC)0 : (TC)0.25 * (q3c + q3im) * (q1c + q1km);
+        TC h13 = (qnn - qss) * dx3;
         qnn = q2n + q2c;
         qss = q2imn + q2im;
-        TW d1q2 = (qnn - qss) * h1dare;
-        TW H1 = d1q2 - (h11 + (h12d + h12n) + h13);
-        TW d11 = (q1ip - (TW)2 * q1c + q1im) * s1t11;
-        TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (q1kp - (TW)3 * q1c) * dx3q : (q1kp - (TW)2 * q1c + q1km) * dx3q;
-        TW qn2 = (q1n * irmn - q1c * irm) * s1t2n;
-        TW qs2 = (q1c * irm - q1jm * irmm) * s1t2s;
-        TW d22 = (qn2 - qs2) * s1t2c;
+        TC d1q2 = (qnn - qss) * h1dare;
+        TC H1 = d1q2 - (h11 + (h12d + h12n) + h13);
+        TC d11 = (q1ip - (TC)2 * q1c + q1im) * s1t11;
+        TC d33 = (k == 1) ? (TC)(4.0 / 3.0) * (q1kp - (TC)3 * q1c) * dx3q : (q1kp - (TC)2 * q1c + q1km) * dx3q;
+        TC qn2 = (q1n * irmn - q1c * irm) * s1t2n;
+        TC qs2 = (q1c * irm - q1jm * irmm) * s1t2s;
+        TC d22 = (qn2 - qs2) * s1t2c;
         v1 = (ga * H1 + ro * c_ru1 - al * c_pg1 + aRe * (d11 + d33 + d22)) * dt;
         ru1[o] = H1;
       }
       // ---- q2 at (theta_m, r_c, z_m), j >= 2 ----
       if (j >= 2) {
-        TW qnn = (q1ip + q1ips) * (q2ip + q2c);
-        TW qss = (q1c + q1s) * (q2c + q2im);
-        TW h21 = (qnn - qss) * h2d11;
+        TC qnn = (q1ip + q1ips) * (q2ip + q2c);
+        TC qss = (q1c + q1s) * (q2c + q2im);
+        TC h21 = (qnn - qss) * h2d11;
         qnn = q2n + q2c;
         qss = q2c + q2s;
-        TW h22 = qnn * qnn * h2d22p - qss * qss * h2d22m;
-        qnn = (TW)0.5 * (q1c + q1ip);
-        qss = (TW)0.5 * (q1s + q1ips);
-        TW h11n = (TW)0.25 * (qnn + qss) * (qnn + qss);
-        qnn = (TW)0.25 * (q3kp + q3skp) * (q2kp + q2c);
-        qss = (k == 1) ? (TW)0 : (TW)0.25 * (q3c + q3s) * (q2c + q2km);
-        TW h23 = (qnn - qss) * dx3;
+        TC h22 = qnn * qnn * h2d22p - qss * qss * h2d22m;
+        qnn = (TC)0.5 * (q1c + q1ip);
+        qss = (TC)0.5 * (q1s + q1ips);
+        TC h11n = (TC)0.25 * (qnn + qss) * (qnn + qss);
+        qnn = (TC)0.25 * (q3kp + q3skp) * (q2kp + q2c);
+        qss = (k == 1) ? (TC)0 : (TC)0.25 * (q3c + q3s) * (q2c + q2km);
+        TC h23 = (qnn - qss) * dx3;
         qnn = q1ip + q1ips;
         qss = q1c + q1s;
-        TW d1q1 = -((qnn - qss) * h2dare);
-        TW H2 = d1q1 - (h21 + h22 + h23 - h11n);
-        TW d11 = (q2ip - (TW)2 * q2c + q2im) * s2t11;
-        TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (q2kp - (TW)3 * q2c) * dx3q : (q2kp - (TW)2 * q2c + q2km) * dx3q;
-        TW d22 = q2n * s2t2p - q2c * s2t2c + q2s * s2t2m;
+        TC d1q1 = -((qnn - qss) * h2dare);
+        TC H2 = d1q1 - (h21 + h22 + h23 - h11n);
+        TC d11 = (q2ip - (TC)2 * q2c + q2im) * s2t11;
+        TC d33 = (k == 1) ? (TC)(4.0 / 3.0) * (q2kp - (TC)3 * q2c) * dx3q : (q2kp - (TC)2 * q2c + q2km) * dx3q;
+        TC d22 = q2n * s2t2p - q2c * s2t2c + q2s * s2t2m;
         v2 = (ga * H2 + ro * c_ru2 - al * c_pg2 + aRe * (d11 + d33 + d22)) * dt;
         ru2[o] = H2;
       }
       // ---- q3 at (theta_m, r_m, z_c), k >= 2 ----
       if (k >= 2) {
-        TW qnn = (q3n + q3c) * (q2n + q2nkm);
-        TW qss = (q3jm + q3c) * (q2c + q2km);
-        TW h32 = (qnn - qss) * h1d22d;
+        TC qnn = (q3n + q3c) * (q2n + q2nkm);
+        TC qss = (q3jm + q3c) * (q2c + q2km);
+        TC h32 = (qnn - qss) * h1d22d;
         qnn = (q3c + q3ip) * (q1ip + q1ipkm);
         qss = (q3im + q3c) * (q1c + q1km);
-        TW h31 = (qnn - qss) * h1d11;
-        qnn = (TW)0.25 * (q3c + q3kp) * (q3c + q3kp);
-        qss = (TW)0.25 * (q3c + q3km) * (q3c + q3km);
-        TW h33 = (qnn - qss) * dx3;
-        TW H3 = -(h31 + h32 + h33);
-        TW d11 = (q3ip - (TW)2 * q3c + q3im) * s1t11;
-        TW d33 = (q3kp - (TW)2 * q3c + q3km) * dx3q;
-        TW qn2 = (q3n - q3c) * s3t2n;
-        TW qs2 = (q3c - q3jm) * s3t2s;
-        TW d22 = qn2 - qs2;
+        TC h31 = (qnn - qss) * h1d11;
+        qnn = (TC)0.25 * (q3c + q3kp) * (q3c + q3kp);
+        qss = (TC)0.25 * (q3c + q3km) * (q3c + q3km);
+        TC h33 = (qnn - qss) * dx3;
+        TC H3 = -(h31 + h32 + h33);
+        TC d11 = (q3ip - (TC)2 * q3c + q3im) * s1t11;
+        TC d33 = (q3kp - (TC)2 * q3c + q3km) * dx3q;
+        TC qn2 = (q3n - q3c) * s3t2n;
+        TC qs2 = (q3c - q3jm) * s3t2s;
+        TC d22 = qn2 - qs2;
         v3 = (ga * H3 + ro * c_ru3 - al * c_pg3 + aRe * (d11 + d33 + d22)) * dt;
         ru3[o] = H3;
       }
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
     }
     // ---- periodic theta solves by PCR within the wavefront ----
     const long S3 = 3L * RJ * NTP;
-    TW *f1 = fb + tj * NTP, *f2 = f1 + RJ * NTP, *f3 = f2 + RJ * NTP;
+    TC *f1 = fb + tj * NTP, *f2 = f1 + RJ * NTP, *f3 = f2 + RJ * NTP;
 #pragma unroll
     for (int cc = 0; cc < CPL; cc++) {
       const int ti = lane + 64 * cc;
@@ -623,8 +623,8 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
     wave_sync();
     int sm = 1 % nt;
     for (int lvl = 0; lvl < nlev; ++lvl) {
-      const TW al1 = (TW)pcr[(tj * nlev + lvl) * 2], al2 = (TW)pcr[(tj * nlev + lvl) * 2 + 1];
-      TW n1[CPL], n2[CPL], n3[CPL];
+      const TC al1 = (TC)pcr[(tj * nlev + lvl) * 2], al2 = (TC)pcr[(tj * nlev + lvl) * 2 + 1];
+      TC n1[CPL], n2[CPL], n3[CPL];
 #pragma unroll
       for (int cc = 0; cc < CPL; cc++) {
         const int ti = lane + 64 * cc;
@@ -648,7 +648,7 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
       }
       wave_sync();
     }
-    const TW d1 = (TW)fin[tj * 2], d2 = (TW)fin[tj * 2 + 1];
+    const TC d1 = (TC)fin[tj * 2], d2 = (TC)fin[tj * 2 + 1];
 #pragma unroll
     for (int cc = 0; cc < CPL; cc++) {
       const int ti = lane + 64 * cc;
@@ -685,11 +685,11 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
 #undef irmm
 }
 
-template <typename TW>
+template <typename TW, typename TC>
 __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk, double betadx, int nlev,
                                                    const double *__restrict__ q1, const double *__restrict__ q2,
-                                                   const double *__restrict__ q3, const TW *__restrict__ prg1,
-                                                   const TW *__restrict__ prg2, const TW *__restrict__ prg3,
+                                                   const double *__restrict__ q3, const double *__restrict__ prg1,
+                                                   const double *__restrict__ prg2, const double *__restrict__ prg3,
                                                    TW *__restrict__ ru1, TW *__restrict__ ru2, TW *__restrict__ ru3,
                                                    TW *__restrict__ r1, TW *__restrict__ r2, TW *__restrict__ r3) {
   extern __shared__ unsigned char smem_raw[];
@@ -699,86 +699,86 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
   const int k = blockIdx.y + 1;
   const bool cell = (i <= g.nt) && (j <= g.nr);
   const int jc = cell ? j : 1;  // coefficient row for idle threads
-  TW v1 = (TW)0, v2 = (TW)0, v3 = (TW)0;
+  TC v1 = (TC)0, v2 = (TC)0, v3 = (TC)0;
   if (cell) {
   const int ip = (i == g.nt) ? 1 : i + 1, im = (i == 1) ? g.nt : i - 1;
   const int jm = (j == 1) ? 1 : j - 1;
   const long o = hidx(g, i, j, k);
   const long dip = ip - i, dim = im - i, sj = g.sj, sk = g.sk, djm = (long)(jm - j) * sj;
-  const TW ga = (TW)rk.g, ro = (TW)rk.r, al = (TW)rk.a, dt = (TW)g.dt, aRe = (TW)(rk.a / g.Re);
-  const TW dx3 = (TW)g.dx3, dx3q = (TW)g.dx3q;
+  const TC ga = (TC)rk.g, ro = (TC)rk.r, al = (TC)rk.a, dt = (TC)g.dt, aRe = (TC)(rk.a / g.Re);
+  const TC dx3 = (TC)g.dx3, dx3q = (TC)g.dx3q;
   const long pc = cidx(g, i, j, k);
-#define Q1(d) ((TW)q1[o + (d)])
-#define Q2(d) ((TW)q2[o + (d)])
-#define Q3(d) ((TW)q3[o + (d)])
+#define Q1(d) ((TC)q1[o + (d)])
+#define Q2(d) ((TC)q2[o + (d)])
+#define Q3(d) ((TC)q3[o + (d)])
   // ---- q1 at (theta_c, r_m, z_m) ----
   {
-    TW qnn = Q1(dip) + Q1(0), qss = Q1(dim) + Q1(0);
-    TW h11 = (qnn * qnn - qss * qss) * (TW)c.h1d11[j];
+    TC qnn = Q1(dip) + Q1(0), qss = Q1(dim) + Q1(0);
+    TC h11 = (qnn * qnn - qss * qss) * (TC)c.h1d11[j];
     qnn = (Q2(sj) + Q2(dim + sj)) * (Q1(sj) + Q1(0));
     qss = (Q1(0) + Q1(djm)) * (Q2(0) + Q2(dim));
-    TW h12d = (qnn - qss) * (TW)c.h1d22d[j];
+    TC h12d = (qnn - qss) * (TC)c.h1d22d[j];
     qnn = (Q1(dip) + Q1(0)) * (Q2(sj) + Q2(0));
     qss = (Q1(0) + Q1(dim)) * (Q2(dim + sj) + Q2(dim));
-    TW h12n = (qnn + qss) * (TW)c.h1d22n[j];
-    qnn = (TW)0.25 * (Q3(sk) + Q3(dim + sk)) * (Q1(sk) + Q1(0));
-    qss = (k == 1) ? (TW)0 : (TW)0.25 * (Q3(0) + Q3(dim)) * (Q1(0) + Q1(-sk));
-    TW h13 = (qnn - qss) * dx3;
+    TC h12n = (qnn + qss) * (TC)c.h1d22n[j];
+    qnn = (TC)0.25 * (Q3(sk) + Q3(dim + sk)) * (Q1(sk) + Q1(0));
+    qss = (k == 1) ? (TC)0 : (TC)0.25 * (Q3(0) + Q3(dim)) * (Q1(0) + Q1(-sk));
+    TC h13 = (qnn - qss) * dx3;
     qnn = Q2(sj) + Q2(0);
     qss = Q2(dim + sj) + Q2(dim);
-    TW d1q2 = (qnn - qss) * (TW)c.h1dare[j];
-    TW H1 = d1q2 - (h11 + (h12d + h12n) + h13);
-    TW d11 = (Q1(dip) - (TW)2 * Q1(0) + Q1(dim)) * (TW)c.s1t11[j];
-    TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (Q1(sk) - (TW)3 * Q1(0)) * dx3q
-                      : (Q1(sk) - (TW)2 * Q1(0) + Q1(-sk)) * dx3q;
-    TW qn2 = (Q1(sj) / (TW)c.rm[j + 1] - Q1(0) / (TW)c.rm[j]) * (TW)c.s1t2n[j];
-    TW qs2 = (Q1(0) / (TW)c.rm[j] - Q1(djm) / (TW)c.rm[jm]) * (TW)c.s1t2s[j];
-    TW d22 = (qn2 - qs2) * (TW)c.s1t2c[j];
+    TC d1q2 = (qnn - qss) * (TC)c.h1dare[j];
+    TC H1 = d1q2 - (h11 + (h12d + h12n) + h13);
+    TC d11 = (Q1(dip) - (TC)2 * Q1(0) + Q1(dim)) * (TC)c.s1t11[j];
+    TC d33 = (k == 1) ? (TC)(4.0 / 3.0) * (Q1(sk) - (TC)3 * Q1(0)) * dx3q
+                      : (Q1(sk) - (TC)2 * Q1(0) + Q1(-sk)) * dx3q;
+    TC qn2 = (Q1(sj) / (TC)c.rm[j + 1] - Q1(0) / (TC)c.rm[j]) * (TC)c.s1t2n[j];
+    TC qs2 = (Q1(0) / (TC)c.rm[j] - Q1(djm) / (TC)c.rm[jm]) * (TC)c.s1t2s[j];
+    TC d22 = (qn2 - qs2) * (TC)c.s1t2c[j];
     v1 = (ga * H1 + ro * ru1[o] - al * prg1[pc] + aRe * (d11 + d33 + d22)) * dt;
     ru1[o] = H1;
   }
   // ---- q2 at (theta_m, r_c, z_m), j >= 2 ----
   if (j >= 2) {
-    TW qnn = (Q1(dip) + Q1(dip - sj)) * (Q2(dip) + Q2(0));
-    TW qss = (Q1(0) + Q1(-sj)) * (Q2(0) + Q2(dim));
-    TW h21 = (qnn - qss) * (TW)c.h2d11[j];
+    TC qnn = (Q1(dip) + Q1(dip - sj)) * (Q2(dip) + Q2(0));
+    TC qss = (Q1(0) + Q1(-sj)) * (Q2(0) + Q2(dim));
+    TC h21 = (qnn - qss) * (TC)c.h2d11[j];
     qnn = Q2(sj) + Q2(0);
     qss = Q2(0) + Q2(-sj);
-    TW h22 = qnn * qnn * (TW)c.h2d22p[j] - qss * qss * (TW)c.h2d22m[j];
-    qnn = (TW)0.5 * (Q1(0) + Q1(dip));
-    qss = (TW)0.5 * (Q1(-sj) + Q1(dip - sj));
-    TW h11n = (TW)0.25 * (qnn + qss) * (qnn + qss);
-    qnn = (TW)0.25 * (Q3(sk) + Q3(sk - sj)) * (Q2(sk) + Q2(0));
-    qss = (k == 1) ? (TW)0 : (TW)0.25 * (Q3(0) + Q3(-sj)) * (Q2(0) + Q2(-sk));
-    TW h23 = (qnn - qss) * dx3;
+    TC h22 = qnn * qnn * (TC)c.h2d22p[j] - qss * qss * (TC)c.h2d22m[j];
+    qnn = (TC)0.5 * (Q1(0) + Q1(dip));
+    qss = (TC)0.5 * (Q1(-sj) + Q1(dip - sj));
+    TC h11n = (TC)0.25 * (qnn + qss) * (qnn + qss);
+    qnn = (TC)0.25 * (Q3(sk) + Q3(sk - sj)) * (Q2(sk) + Q2(0));
+    qss = (k == 1) ? (TC)0 : (TC)0.25 * (Q3(0) + Q3(-sj)) * (Q2(0) + Q2(-sk));
+    TC h23 = (qnn - qss) * dx3;
     qnn = Q1(dip) + Q1(dip - sj);
     qss = Q1(0) + Q1(-sj);
-    TW d1q1 = -((qnn - qss) * (TW)c.h2dare[j]);
-    TW H2 = d1q1 - (h21 + h22 + h23 - h11n);
-    TW d11 = (Q2(dip) - (TW)2 * Q2(0) + Q2(dim)) * (TW)c.s2t11[j];
-    TW d33 = (k == 1) ? (TW)(4.0 / 3.0) * (Q2(sk) - (TW)3 * Q2(0)) * dx3q
-                      : (Q2(sk) - (TW)2 * Q2(0) + Q2(-sk)) * dx3q;
-    TW d22 = Q2(sj) * (TW)c.s2t2p[j] - Q2(0) * (TW)c.s2t2c[j] + Q2(-sj) * (TW)c.s2t2m[j];
+    TC d1q1 = -((qnn - qss) * (TC)c.h2dare[j]);
+    TC H2 = d1q1 - (h21 + h22 + h23 - h11n);
+    TC d11 = (Q2(dip) - (TC)2 * Q2(0) + Q2(dim)) * (TC)c.s2t11[j];
+    TC d33 = (k == 1) ? (TC)(4.0 / 3.0) * (Q2(sk) - (TC)3 * Q2(0)) * dx3q
+                      : (Q2(sk) - (TC)2 * Q2(0) + Q2(-sk)) * dx3q;
+    TC d22 = Q2(sj) * (TC)c.s2t2p[j] - Q2(0) * (TC)c.s2t2c[j] + Q2(-sj) * (TC)c.s2t2m[j];
     v2 = (ga * H2 + ro * ru2[o] - al * prg2[pc] + aRe * (d11 + d33 + d22)) * dt;
     ru2[o] = H2;
   }
   // ---- q3 at (theta_m, r_m, z_c), k >= 2 ----
   if (k >= 2) {
-    TW qnn = (Q3(sj) + Q3(0)) * (Q2(sj) + Q2(sj - sk));
-    TW qss = (Q3(djm) + Q3(0)) * (Q2(0) + Q2(-sk));
-    TW h32 = (qnn - qss) * (TW)c.h1d22d[j];
+    TC qnn = (Q3(sj) + Q3(0)) * (Q2(sj) + Q2(sj - sk));
+    TC qss = (Q3(djm) + Q3(0)) * (Q2(0) + Q2(-sk));
+    TC h32 = (qnn - qss) * (TC)c.h1d22d[j];
     qnn = (Q3(0) + Q3(dip)) * (Q1(dip) + Q1(dip - sk));
     qss = (Q3(dim) + Q3(0)) * (Q1(0) + Q1(-sk));
-    TW h31 = (qnn - qss) * (TW)c.h1d11[j];
-    qnn = (TW)0.25 * (Q3(0) + Q3(sk)) * (Q3(0) + Q3(sk));
-    qss = (TW)0.25 * (Q3(0) + Q3(-sk)) * (Q3(0) + Q3(-sk));
-    TW h33 = (qnn - qss) * dx3;
-    TW H3 = -(h31 + h32 + h33);
-    TW d11 = (Q3(dip) - (TW)2 * Q3(0) + Q3(dim)) * (TW)c.s1t11[j];
-    TW d33 = (Q3(sk) - (TW)2 * Q3(0) + Q3(-sk)) * dx3q;
-    TW qn2 = (Q3(sj) - Q3(0)) * (TW)c.s3t2n[j];
-    TW qs2 = (Q3(0) - Q3(djm)) * (TW)c.s3t2s[j];
-    TW d22 = qn2 - qs2;
+    TC h31 = (qnn - qss) * (TC)c.h1d11[j];
+    qnn = (TC)0.25 * (Q3(0) + Q3(sk)) * (Q3(0) + Q3(sk));
+    qss = (TC)0.25 * (Q3(0) + Q3(-sk)) * (Q3(0) + Q3(-sk));
+    TC h33 = (qnn - qss) * dx3;
+    TC H3 = -(h31 + h32 + h33);
+    TC d11 = (Q3(dip) - (TC)2 * Q3(0) + Q3(dim)) * (TC)c.s1t11[j];
+    TC d33 = (Q3(sk) - (TC)2 * Q3(0) + Q3(-sk)) * dx3q;
+    TC qn2 = (Q3(sj) - Q3(0)) * (TC)c.s3t2n[j];
+    TC qs2 = (Q3(0) - Q3(djm)) * (TC)c.s3t2s[j];
+    TC d22 = qn2 - qs2;
     v3 = (ga * H3 + ro * ru3[o] - al * prg3[pc] + aRe * (d11 + d33 + d22)) * dt;
     ru3[o] = H3;
   }
@@ -789,9 +789,9 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
   // ---- periodic theta solves of the three components (solqi/tripij,
   // step.F90:212-263, 487-534): constant coefficients (-acc, 1+2acc, -acc)
   // per line, parallel cyclic reduction on the LDS-resident lines ----
-  TW *f1 = reinterpret_cast<TW *>(smem_raw) + (size_t)tj * NTP;
-  TW *f2 = f1 + (size_t)RJ * NTP;
-  TW *f3 = f2 + (size_t)RJ * NTP;
+  TC *f1 = reinterpret_cast<TC *>(smem_raw) + (size_t)tj * NTP;
+  TC *f2 = f1 + (size_t)RJ * NTP;
+  TC *f3 = f2 + (size_t)RJ * NTP;
   f1[ti] = v1; f2[ti] = v2; f3[ti] = v3;
   double a1 = -(betadx * c.s1t11[jc]), b1 = 1.0 + 2.0 * (betadx * c.s1t11[jc]);
   double a2 = -(betadx * c.s2t11[jc]), b2 = 1.0 + 2.0 * (betadx * c.s2t11[jc]);
@@ -803,13 +803,13 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
   int sm = 1 % N;
   for (int lvl = 0; lvl < nlev; ++lvl) {
     const double al1 = -a1 / b1, al2 = -a2 / b2;
-    TW n1 = v1, n2 = v2, n3 = v3;
+    TC n1 = v1, n2 = v2, n3 = v3;
     if (ti < N) {
       int imv = ti - sm; if (imv < 0) imv += N;
       int ipv = ti + sm; if (ipv >= N) ipv -= N;
-      n1 = v1 + (TW)al1 * (f1[imv] + f1[ipv]);
-      n2 = v2 + (TW)al2 * (f2[imv] + f2[ipv]);
-      n3 = v3 + (TW)al1 * (f3[imv] + f3[ipv]);
+      n1 = v1 + (TC)al1 * (f1[imv] + f1[ipv]);
+      n2 = v2 + (TC)al2 * (f2[imv] + f2[ipv]);
+      n3 = v3 + (TC)al1 * (f3[imv] + f3[ipv]);
     }
     b1 = b1 + 2.0 * al1 * a1; a1 = al1 * a1;
     b2 = b2 + 2.0 * al2 * a2; a2 = al2 * a2;
@@ -823,9 +823,9 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
   if (!cell) return;
   const double d1 = pow2 ? b1 + 2.0 * a1 : b1, d2 = pow2 ? b2 + 2.0 * a2 : b2;
   const long o = hidx(g, i, j, k);
-  r1[o] = (TW)(v1 / (TW)d1);
-  if (j >= 2) r2[o] = (TW)(v2 / (TW)d2);
-  if (k >= 2) r3[o] = (TW)(v3 / (TW)d1);
+  r1[o] = (TC)(v1 / (TC)d1);
+  if (j >= 2) r2[o] = (TC)(v2 / (TC)d2);
+  if (k >= 2) r3[o] = (TC)(v3 / (TC)d1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1636,8 +1636,8 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
                                                     const double *__restrict__ dph, const double *__restrict__ dq1,
                                                     const double *__restrict__ dq2, const double *__restrict__ dq3,
                                                     double *__restrict__ q1, double *__restrict__ q2,
-                                                    double *__restrict__ q3, TW *__restrict__ prg1,
-                                                    TW *__restrict__ prg2, TW *__restrict__ prg3,
+                                                    double *__restrict__ q3, double *__restrict__ prg1,
+                                                    double *__restrict__ prg2, double *__restrict__ prg3,
                                                     double *__restrict__ pres, int *__restrict__ flag) {
   constexpr int W = 64, LC = W + 3, LR = R + 3, NE = LR * LC, NT = 64 * R;
   constexpr int NL = (NE + NT - 1) / NT;
@@ -1706,7 +1706,7 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
   // the streamed operands of the next plane are loaded while this one is computed
   const bool rowu = act && j >= 1 && j <= nr;
   double n_dq1 = 0.0, n_dq2 = 0.0, n_dq3 = 0.0, n_pres = 0.0;
-  TW n_p1 = (TW)0, n_p2 = (TW)0, n_p3 = (TW)0;
+  double n_p1 = 0.0, n_p2 = 0.0, n_p3 = 0.0;
   // operands of cell (i, j, kk); lanes, rows and planes where a value is not
   // used read a clamped in-bounds address instead of branching, so that the
   // seven loads issue together
@@ -1727,7 +1727,7 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
   for (int k = kb; k <= ke; k++) {
     stash(k + 1);
     const double c_dq1 = n_dq1, c_dq2 = n_dq2, c_dq3 = n_dq3, c_pres = n_pres;
-    const TW c_p1 = n_p1, c_p2 = n_p2, c_p3 = n_p3;
+    const double c_p1 = n_p1, c_p2 = n_p2, c_p3 = n_p3;
     __syncthreads();
     fetch(k + 2);
     load_cell(k + 1);
@@ -1770,7 +1770,7 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
           const double qnn = qjp * irmn - qc0 * irm;
           const double qss = qc0 * irm - qjm * irmm;
           const double d22 = (qnn * s1t2n - qss * s1t2s) * s1t2c;
-          prg1[pc] = (TW)((double)c_p1 + qc0 - be * (d11 + d33 + d22));
+          prg1[pc] = c_p1 + qc0 - be * (d11 + d33 + d22);
         }
         // r component: qcap = (dph(j) - dph(j-1)) dx2 rc, zero at j = 1 and nrc
         if (j >= 2) {
@@ -1784,7 +1784,7 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
           const double d11 = (qip - 2.0 * qc0 + qim) * s2t11;
           const double d33 = (qkp - 2.0 * qc0 + qkm) * dx3q;
           const double d22 = (qjp - qc0) * s2t2p - (qc0 - qjm) * s2t2m;
-          prg2[pc] = (TW)((double)c_p2 + qc0 - be * (d11 + d22 + d33));
+          prg2[pc] = c_p2 + qc0 - be * (d11 + d22 + d33);
         }
         // z component: qcap = (dph(k) - dph(k-1)) dx3, zero at k = 1
         {
@@ -1800,7 +1800,7 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
             const double d11 = (qip - 2.0 * qc0 + qim) * s1t11;
             const double d33 = (qkp - 2.0 * qc0 + qkm) * dx3q;
             const double d22 = (qjp - qc0) * s3t2n - (qc0 - qjm) * s3t2s;
-            prg3[pc] = (TW)((double)c_p3 + qc0 - be * (d11 + d22 + d33));
+            prg3[pc] = c_p3 + qc0 - be * (d11 + d22 + d33);
           }
         }
         {
@@ -1907,6 +1907,8 @@ void launch_scalars(vrh_t *h) {
 }
 
 template <typename TW> void launch_explicit(vrh_t *h, int irk) {
+  // compute type of the stencils: the work type (FP32 in the mixed path)
+  using TC = TW;
   const VGrid &g = h->g;
   int ntp = 1;
   while (ntp < g.nt) ntp <<= 1;
@@ -1920,17 +1922,17 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
     int rj = 256 / ntp;
     if (rj < 1) rj = 1;
     dim3 blk(ntp, rj), grd((g.nr + rj - 1) / rj, g.nz);
-    const size_t sh = sizeof(TW) * 6 * (size_t)rj * ntp;
-    hipLaunchKernelGGL(k_explicit_cell<TW>, grd, blk, sh, h->stream, g, h->coef, h->rk[irk], betadx, nlev,
-                       h->f.q1, h->f.q2, h->f.q3, (const TW *)h->f.prg1, (const TW *)h->f.prg2,
-                       (const TW *)h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,
+    const size_t sh = sizeof(TC) * 6 * (size_t)rj * ntp;
+    hipLaunchKernelGGL((k_explicit_cell<TW, TC>), grd, blk, sh, h->stream, g, h->coef, h->rk[irk], betadx, nlev,
+                       h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2,
+                       h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,
                        (TW *)h->f.r2, (TW *)h->f.r3);
     return;
   }
   // z marching: rows per block for 512 threads within 64 KB of LDS
   auto lds = [&](int rj) {
     return sizeof(double) * (size_t)rj * (kMaxPcrLevels * 2 + 2) +
-           sizeof(TW) * ((size_t)rj * kRowCoefs + (size_t)9 * (rj + 2) * ntp + (size_t)6 * rj * ntp);
+           sizeof(TC) * ((size_t)rj * kRowCoefs + (size_t)9 * (rj + 2) * ntp + (size_t)6 * rj * ntp);
   };
   int rj = 512 / ntp;
   if (rj < 1) rj = 1;
@@ -1967,9 +1969,9 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
     dim3 grd((g.nr + rw - 1) / rw, (g.nz + kchunk - 1) / kchunk);
     bool done = rw > 1 && (var == 2 ? lds(rw) : lds_w(rw)) <= lim;
 #define EXPL_WAVE(C, R, W)                                                                                          \
-  hipLaunchKernelGGL((k_explicit_wave<TW, C, R, W>), grd, dim3(64, R), lds_w(R), h->stream, g, h->coef, h->rk[irk],   \
-                     betadx, nlev, kchunk, h->f.q1, h->f.q2, h->f.q3, (const TW *)h->f.prg1, (const TW *)h->f.prg2, \
-                     (const TW *)h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,          \
+  hipLaunchKernelGGL((k_explicit_wave<TW, TC, C, R, W>), grd, dim3(64, R), lds_w(R), h->stream, g, h->coef, h->rk[irk],   \
+                     betadx, nlev, kchunk, h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2, \
+                     h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,          \
                      (TW *)h->f.r2, (TW *)h->f.r3)
     if (done) {
       if (cpl == 1 && rw == 8) EXPL_WAVE(1, 8, 4);
@@ -1985,9 +1987,9 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
   }
   while (rj > 1 && lds(rj) > 65536) rj--;
   dim3 blk(ntp, rj), grd((g.nr + rj - 1) / rj, (g.nz + kchunk - 1) / kchunk);
-  hipLaunchKernelGGL(k_explicit_march<TW>, grd, blk, lds(rj), h->stream, g, h->coef, h->rk[irk], betadx, nlev,
-                     kchunk, h->f.q1, h->f.q2, h->f.q3, (const TW *)h->f.prg1, (const TW *)h->f.prg2,
-                     (const TW *)h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,
+  hipLaunchKernelGGL((k_explicit_march<TW, TC>), grd, blk, lds(rj), h->stream, g, h->coef, h->rk[irk], betadx, nlev,
+                     kchunk, h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2,
+                     h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,
                      (TW *)h->f.r2, (TW *)h->f.r3);
 }
 
@@ -2005,18 +2007,18 @@ void explicit_lds_optin(vrh_t *h) {
   bool ok = true;
   if (lb > 65536) {
     const void *fns[] = {
-        reinterpret_cast<const void *>(&k_explicit_wave<double, 1, 8, 4>),
-        reinterpret_cast<const void *>(&k_explicit_wave<double, 1, 6, 3>),
-        reinterpret_cast<const void *>(&k_explicit_wave<double, 2, 6, 3>),
-        reinterpret_cast<const void *>(&k_explicit_wave<double, 2, 4, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<double, 2, 2, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<double, 4, 2, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, 1, 8, 4>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, 1, 6, 3>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, 2, 6, 3>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, 2, 4, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, 2, 2, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, 4, 2, 2>)};
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 1, 8, 4>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 1, 6, 3>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 6, 3>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 4, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 2, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 4, 2, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, double, 1, 8, 4>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, double, 1, 6, 3>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, double, 2, 6, 3>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, double, 2, 4, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, double, 2, 2, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, double, 4, 2, 2>)};
     for (const void *f : fns)
       ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lb) == hipSuccess;
   }
@@ -2193,8 +2195,8 @@ template <typename TW> void launch_project(vrh_t *h, int irk) {
   dim3 blk(64, R), grd((g.nt + 63) / 64, (g.nrc + 1 + R - 1) / R, (g.nzc + kchunk - 1) / kchunk);
   KTimer kt(h, KT_PROJECT);
   hipLaunchKernelGGL((k_project<TW, R>), grd, blk, 0, h->stream, g, h->coef, adt, be, kchunk, h->f.dph, h->f.dq1,
-                     h->f.dq2, h->f.dq3, h->f.q1, h->f.q2, h->f.q3, (TW *)h->f.prg1, (TW *)h->f.prg2,
-                     (TW *)h->f.prg3, h->f.pres, h->f.flag);
+                     h->f.dq2, h->f.dq3, h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2,
+                     h->f.prg3, h->f.pres, h->f.flag);
 }
 
 void launch_analysis(vrh_t *h, double *part, int nblk, double *out) {

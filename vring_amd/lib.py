@@ -30,7 +30,7 @@ class VrhParams(ctypes.Structure):
 
 
 EXPORTS = ["vrh_create", "vrh_destroy", "vrh_last_error", "vrh_upload_fields", "vrh_download_fields",
-           "vrh_upload_state", "vrh_download_state", "vrh_set_time", "vrh_get_time", "vrh_set_radial_mesh",
+           "vrh_upload_state", "vrh_download_state", "vrh_state_size", "vrh_set_time", "vrh_get_time", "vrh_set_radial_mesh",
            "vrh_substep",
            "vrh_step", "vrh_phase", "vrh_poisson_solve", "vrh_synchronize", "vrh_device_ptr",
            "vrh_ring_init", "vrh_cfl_dt", "vrh_particles_create", "vrh_particles_upload",
@@ -61,6 +61,7 @@ def load_library(path=LIBPATH):
     L.vrh_download_fields.argtypes = [P, P, P, P, P]
     L.vrh_upload_state.argtypes = [P, ctypes.c_char_p, dp]
     L.vrh_download_state.argtypes = [P, ctypes.c_char_p, dp]
+    L.vrh_state_size.argtypes = [ctypes.POINTER(VrhGrid), ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)]
     L.vrh_set_time.argtypes = [P, D]
     L.vrh_get_time.argtypes = [P, ctypes.POINTER(D)]
     L.vrh_set_radial_mesh.argtypes = [P, dp, dp, dp, dp]
@@ -95,6 +96,16 @@ def load_library(path=LIBPATH):
     L.vrh_ensemble_destroy.argtypes = [P]
     _lib = L
     return L
+
+
+def state_size(ntheta, nr, nz, name):
+    """Number of doubles vrh_upload_state/vrh_download_state move for `name`
+    (host query, no device call)."""
+    L = load_library()
+    n = ctypes.c_long(0)
+    if L.vrh_state_size(ctypes.byref(VrhGrid(ntheta, nr, nz, 1.0, 1.0)), name.encode(), ctypes.byref(n)) != 0:
+        raise KeyError(name)
+    return n.value
 
 
 def _f64(a):
@@ -196,12 +207,13 @@ class Solver:
 
     def upload(self, name, arr):
         a = _f64(arr)
-        assert a.size == int(np.prod(self.shape_of(name))), name
+        if a.size != state_size(self.nt, self.nr, self.nz, name):
+            raise ValueError(f"{name}: {a.size} values, the library moves {state_size(self.nt, self.nr, self.nz, name)}")
         self._check(self.L.vrh_upload_state(self.h, name.encode(), a), f"upload {name}")
 
     def download(self, name):
         shp = self.shape_of(name)
-        out = np.zeros(int(np.prod(shp)))
+        out = np.zeros(state_size(self.nt, self.nr, self.nz, name))
         self._check(self.L.vrh_download_state(self.h, name.encode(), out), f"download {name}")
         return out.reshape(shp, order="F")
 
