@@ -163,6 +163,18 @@ int vrh_device_ptr(vrh_t *h, const char *name, void **ptr);
  * j != 1), out[2..9] the sums stat(1:8) = xvort, yvort, vvort, uvort, xswrl,
  * yswrl, vswrl, uswrl of the positive vorticity and swirl. */
 int vrh_analysis(vrh_t *h, double *out);
+/* Allocation-free variant of vrh_analysis.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same ten numbers, bit for bit, are written to dev_out
+ * (device memory, 10 doubles) on the handle's stream, and the call returns
+ * without waiting (vrh_synchronize or the stream's order makes them
+ * visible).  It calls no allocating routine: the reduction's block partials
+ * live in a fixed device array of 1024 x 10 doubles (80 KB) per GPU, held
+ * until the process ends.  That size does not depend on the grid, so no
+ * grid is refused for size.  Every call on one GPU shares the array:
+ * calls on that GPU whose work overlaps (any handles, any streams) race. */
+int vrh_analysis_noalloc(vrh_t *h, double *dev_out);
 
 /* Host-side initial condition (no reference counterpart: the reference loads
  * RESTART, run.F90:469-539).  Gaussian-core vortex rings from a discrete

@@ -292,6 +292,9 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
   const double gam[4] = {0, 8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};   // para_tools.F90:103-113
   const double rom[4] = {0, 0.0, -17.0 / 60.0, -5.0 / 12.0};
   for (int i = 1; i <= 3; i++) h->rk[i] = {gam[i], rom[i], gam[i] + rom[i]};
+  HIPCHK(h, hipGetDevice(&h->device));
+  if (analysis_scratch_init(h->device) != 0)
+    return fail(h, "vrh_create: the analysis scratch array has no device address (hipGetSymbolAddress)", -3);
   HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   explicit_lds_optin(h);
   for (int q = 0; q < 2; q++) HIPCHK(h, hipStreamCreateWithFlags(&h->side[q], hipStreamNonBlocking));
@@ -977,7 +980,7 @@ extern "C" int vrh_set_tuning(vrh_t *h, const int *choice) {
 // vortmax, swrlmax, then stat(1:8).
 extern "C" int vrh_analysis(vrh_t *h, double *out) {
   if (!h || !out) return -1;
-  const int nblk = 1024;
+  const int nblk = kAnalysisBlocks;
   // every block writes its ten partials and the sum kernel writes the ten
   // results, so the buffer needs no clearing between calls
   if (!h->d_anl) HIPCHK(h, hipMalloc(&h->d_anl, sizeof(double) * (nblk * 10 + 16)));
@@ -986,5 +989,17 @@ extern "C" int vrh_analysis(vrh_t *h, double *out) {
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(out, part + nblk * 10, 10 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// The same reduction with the block partials in the per-device scratch
+// array (set up at vrh_create) and the results left in device memory:
+// nothing is allocated and nothing waits, so the call may be captured.
+extern "C" int vrh_analysis_noalloc(vrh_t *h, double *dev_out) {
+  if (!h || !dev_out) return -1;
+  double *part = analysis_scratch(h->device);
+  if (!part) return fail(h, "vrh_analysis_noalloc: no scratch address for the handle's device");
+  launch_analysis(h, part, kAnalysisBlocks, dev_out);
+  HIPCHK(h, hipGetLastError());
   return 0;
 }

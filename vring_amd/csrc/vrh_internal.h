@@ -100,6 +100,7 @@ struct KTimer {
 
 struct vrh_s {
   int nt, nr, nz, nrc, nzc, prec, theta_eigen;
+  int device = 0;  // HIP device current at creation (the handle's GPU)
   double rmax, zmax, Re, dt, slip, ntime;
   VGrid g;
   VRK rk[4];
@@ -195,6 +196,13 @@ void host_mint(vrh_t *h, double *mint);
 template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, const double *cor_dev);
 void launch_scalars(vrh_t *h);
 void launch_analysis(vrh_t *h, double *part, int nblk, double *out);
+// Partial blocks of the analysis reduction (both entry points), and the
+// allocation-free variant's per-device scratch (vrh_kernels.hip): set up
+// once per device with that device current, then looked up by device id.
+constexpr int kAnalysisBlocks = 1024;
+constexpr int kMaxDevices = 64;
+int analysis_scratch_init(int dev);
+double *analysis_scratch(int dev);
 template <typename TW> void launch_explicit(vrh_t *h, int irk);
 void explicit_lds_optin(vrh_t *h);
 // hdnbc: also launch the outlet-row kernel of hdnbc at the head of the q3

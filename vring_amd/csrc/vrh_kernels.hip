@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 
 #include "vrh_internal.h"
 
@@ -2203,6 +2204,35 @@ void launch_analysis(vrh_t *h, double *part, int nblk, double *out) {
   hipLaunchKernelGGL(k_analysis, dim3(nblk), dim3(256), 0, h->stream, h->g, h->rmax, h->zmax, h->coef.rm, h->f.q1,
                      h->f.q2, h->f.q3, part);
   hipLaunchKernelGGL(k_analysis_sum, dim3(1), dim3(64), 0, h->stream, part, nblk, out);
+}
+
+// Block partials of the allocation-free analysis (vrh_analysis_noalloc):
+// a file-scope device array, one copy per GPU, held until the process
+// ends.  Host code never passes the array's name to a kernel (on the host
+// it is only a stand-in for the symbol): each device's address is taken
+// once, with that device current, by analysis_scratch_init, and used from
+// then on.  Every block of k_analysis writes all ten of its partials, so
+// the array needs no clearing between calls.
+static __device__ __align__(256) double scratch[(size_t)kAnalysisBlocks * 10];
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int analysis_scratch_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return -1;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return 0;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return -1;
+  g_scratch_ptr[dev] = static_cast<double *>(p);
+  return 0;
+}
+
+double *analysis_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[dev];
 }
 
 #define INST(TW)                                                   \

@@ -38,7 +38,7 @@ EXPORTS = ["vrh_create", "vrh_destroy", "vrh_last_error", "vrh_upload_fields", "
            "vrh_set_graph", "vrh_set_tuning", "vrh_analysis", "vrh_step_async", "vrh_snapshot_begin",
            "vrh_snapshot_wait", "vrh_particles_compact", "vrh_particles_count",
            "vrh_particles_inject", "vrh_ensemble_create", "vrh_ensemble_step", "vrh_ensemble_last_error",
-           "vrh_ensemble_destroy"]
+           "vrh_ensemble_destroy", "vrh_analysis_noalloc"]
 
 _lib = None
 
@@ -83,6 +83,7 @@ def load_library(path=LIBPATH):
                                   ctypes.POINTER(ctypes.c_long)]
     L.vrh_set_graph.argtypes = [P, I]
     L.vrh_analysis.argtypes = [P, dp]
+    L.vrh_analysis_noalloc.argtypes = [P, P]
     L.vrh_step_async.argtypes = [P, I]
     L.vrh_snapshot_begin.argtypes = [P]
     L.vrh_snapshot_wait.argtypes = [P, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_double)]
