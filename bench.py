@@ -79,6 +79,76 @@ def ring_state(nt=NT, nr=NR, nz=NZ):
     return q, dt
 
 
+# the large single-GPU workload: (ntheta, nr, nz, rmax, zmax, Re)
+LARGE = (256, 512, 1024, 5.0, 20.0, 7500.0)
+
+
+def large_state():
+    """Two coaxial Gaussian rings (R = 1 at z = 3 and R = 1.2 at z = 4:
+    separation 1, sigma = 0.15, Gamma = 1) with 1e-3 core-radius
+    perturbations on m = 2..8 (phases from numpy PCG64(0)) at 512x256x1024,
+    r in [0,5], z in [0,20]; dt at CFL 0.3."""
+    import vring_amd
+    nt, nr, nz, rmax, zmax, _ = LARGE
+    phases = np.random.Generator(np.random.PCG64(0)).uniform(0.0, 2.0 * np.pi, 7)
+    q = vring_amd.ring_initial_condition(nt, nr, nz, rmax, zmax, [(1.0, 3.0, 0.15, 1.0), (1.2, 4.0, 0.15, 1.0)],
+                                         pert_amp=1e-3, pert_modes=(2, 8), phases=phases)
+    dt = vring_amd.cfl_dt(nt, nr, nz, rmax, zmax, *q, cfl=0.3, ujet=1.0)
+    return q, dt
+
+
+def run_large(steps, warmup, particles=16 * 1024 * 1024, tune_reps=3):
+    """The large workload on one GPU: FP64 and mixed paths, K timed steps
+    after W warm-up steps each; the mixed path carries `particles`
+    trilinear-RK3 particles in the two rings' torus (stepped inside each RK3
+    step) and is timed with and without them; the mixed-vs-FP64 error after
+    the warm-up and timed steps."""
+    import vring_amd
+    nt, nr, nz, rmax, zmax, Re = LARGE
+    q, dt = large_state()
+    ncell = nt * nr * nz
+    out = {"workload": f"nr={nr} ntheta={nt} nz={nz} r<={rmax} z<={zmax} Re={Re}, two rings", "dt": dt,
+           "steps": steps, "warmup": warmup}
+    fields = {}
+    for prec in ("fp64", "mixed"):
+        s = vring_amd.Solver(nt, nr, nz, rmax, zmax, Re, dt, precision=prec)
+        for n, a in zip(("q1", "q2", "q3"), q):
+            s.upload(n, a)
+        tune = s.autotune(reps=tune_reps)
+        s.step(warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        wall = time.perf_counter() - t0
+        row = {"ms_per_step": round(wall / steps * 1e3, 3),
+               "Mcell_RK3_steps_per_s": round(ncell * steps / wall / 1e6, 2), "autotune": tune}
+        if prec == "mixed" and particles:
+            s.seed_particles_torus(particles, R=1.1, z0=3.5, a=0.6, seed=0, scheme="trilinear_rk3")
+            s.step(1)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step(steps)
+            wallp = time.perf_counter() - t0
+            row["with_particles"] = {"particles": particles, "ms_per_step": round(wallp / steps * 1e3, 3),
+                                     "particle_ms_per_step": round((wallp - wall) / steps * 1e3, 3),
+                                     "particles_per_s": round(particles * steps / max(wallp - wall, 1e-9), 1)}
+        fields[prec] = s
+        out[prec] = row
+    # the mixed path ran `steps + 1` more steps with particles: compare after
+    # bringing the FP64 path to the same step
+    extra = steps + 1 if particles else 0
+    fields["fp64"].step(extra)
+    f = {p: {n: s.download(n) for n in ("q1", "q2", "q3", "pres")} for p, s in fields.items()}
+    vs = max(np.abs(f["fp64"][n]).max() for n in ("q1", "q2", "q3"))
+    err = {n: float(np.abs(f["mixed"][n] - f["fp64"][n]).max() / vs) for n in ("q1", "q2", "q3")}
+    err["pres"] = float(np.abs(f["mixed"]["pres"] - f["fp64"]["pres"]).max() / np.abs(f["fp64"]["pres"]).max())
+    out["mixed_rel_err_vs_fp64"] = {"after_steps": warmup + steps + extra, "errors": err}
+    out["mixed_over_fp64"] = round(out["mixed"]["Mcell_RK3_steps_per_s"] / out["fp64"]["Mcell_RK3_steps_per_s"], 3)
+    for s in fields.values():
+        s.close()
+    return out
+
+
 def run_path(prec, q, dt, steps, warmup, tune_reps=8):
     """Flow solver only: K timed RK3 steps after W warm-up steps."""
     import vring_amd
@@ -437,6 +507,8 @@ def main():
                     help="run only a plain device copy of MIB MiB (counter calibration)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="RINGS",
                     help="run only the batched-ensemble benchmark with RINGS rings")
+    ap.add_argument("--large", action="store_true",
+                    help="run only the large workload (512x256x1024, two rings, 16M particles)")
     ap.add_argument("--learned", type=int, default=0, metavar="SAMPLES",
                     help="run only the learned Helmholtz benchmark, training on SAMPLES pairs")
     ap.add_argument("--summarize-traffic", metavar="PMC_DIR", default=None,
@@ -462,6 +534,9 @@ def main():
         return
     if args.ensemble:
         print(json.dumps({"ensemble": run_ensemble(args.ensemble, args.steps, args.warmup)}))
+        return
+    if args.large:
+        print(json.dumps({"large": run_large(args.steps, args.warmup, args.particles * 16)}))
         return
     if args.learned:
         print(json.dumps({"learned": run_learned(256, 512, 64, args.learned)}))

@@ -57,8 +57,8 @@ enum {
 /* Lifecycle.  Replaces create_mesh_a/alct_solu/alct_solp/prep_poisson_solver
  * (main.F90:49-68): allocates and zero-initialises every field.  Grids:
  * 3 <= ntheta <= 256 (the explicit kernel holds whole theta rows in LDS),
- * 4 <= nr <= 512 and 4 <= nz <= 1023 (line solves of at most 16 segments of
- * at most 64 rows); power-of-two ntheta and nz take the FFT paths, other
+ * 4 <= nr <= 512 and 4 <= nz <= 1151 (line solves of at most 16 segments of
+ * at most 72 rows); power-of-two ntheta and nz take the FFT paths, other
  * sizes direct transforms.  Returns -1 for grids outside these limits. */
 int vrh_create(const vrh_grid *grid, const vrh_params *params, int precision, vrh_t **out);
 int vrh_destroy(vrh_t *h);

@@ -50,14 +50,14 @@ def test_ring_initial_condition_is_divergence_free_on_host():
 
 def test_create_rejects_grids_outside_the_limits_before_any_device_call():
     """vrh_create checks the grid limits (3 <= ntheta <= 256, 4 <= nr <= 512,
-    4 <= nz <= 1023) and the parameters before it touches a device, so the
+    4 <= nz <= 1151) and the parameters before it touches a device, so the
     rejections run here without a GPU."""
     from vring_amd.lib import VrhGrid, VrhParams
     lib = ctypes.CDLL(LIBPATH)
     lib.vrh_create.argtypes = [ctypes.POINTER(VrhGrid), ctypes.POINTER(VrhParams), ctypes.c_int,
                                ctypes.POINTER(ctypes.c_void_p)]
     ok_p = VrhParams(0.01, 1000.0, 0.0, 0)
-    for nt, nr, nz in [(2, 32, 64), (257, 32, 64), (16, 3, 64), (16, 513, 64), (16, 32, 3), (16, 32, 1024)]:
+    for nt, nr, nz in [(2, 32, 64), (257, 32, 64), (16, 3, 64), (16, 513, 64), (16, 32, 3), (16, 32, 1152)]:
         h = ctypes.c_void_p(1)
         assert lib.vrh_create(ctypes.byref(VrhGrid(nt, nr, nz, 4.0, 12.0)), ctypes.byref(ok_p), 0,
                               ctypes.byref(h)) == -1, (nt, nr, nz)

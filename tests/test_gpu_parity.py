@@ -35,12 +35,14 @@ def test_substep_phases_match_oracle(nt, nr, nz, form):
         compare(o, s, ["dph", "q1", "q2", "q3", "pres", "prg"], 1e-11)
 
 
-@pytest.mark.parametrize("nt,nr,nz", [(3, 4, 4), (8, 512, 8), (4, 8, 1023)])
+@pytest.mark.parametrize("nt,nr,nz", [(3, 4, 4), (8, 512, 8), (4, 8, 1023), (4, 8, 1024), (3, 4, 1151)])
 def test_substep_phases_at_size_limits_match_oracle(nt, nr, nz):
     """The smallest grid vrh_create accepts (ntheta = 3: direct theta DFT),
     the largest nr (512: radial solves of 16 segments of 32 rows, r sweeps
-    of 513 rows) and the longest z lines (nz = 1023: 1024-row z sweeps of 16
-    segments, direct z DCT for a length that is not a power of two)."""
+    of 513 rows), z lines of 1024 rows (nz = 1023: 16 segments of 64, direct
+    z DCT for a length that is not a power of two), of 1025 rows (nz = 1024,
+    the large workload's length: 15 segments of 72, FFT-based DCT) and the
+    longest (nz = 1151: 16 segments of 72)."""
     print(f"case phases at size limits nt={nt} nr={nr} nz={nz}", flush=True)
     o, s = make_pair(nt, nr, nz, slip=0.0)
     seed_random(o, s, seed=nt + nr + nz)

@@ -270,7 +270,7 @@ int vrh_last_error(vrh_t *h, char *buf, size_t len) {
 int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t **out) {
   if (!grid || !p || !out) return -1;
   *out = nullptr;
-  if (grid->ntheta < 3 || grid->nr < 4 || grid->nz < 4 || grid->nr > 16 * 32 || grid->nz > 16 * 64 - 1 ||
+  if (grid->ntheta < 3 || grid->nr < 4 || grid->nz < 4 || grid->nr > 16 * 32 || grid->nz > 16 * 72 - 1 ||
       grid->ntheta > 256 ||
       grid->rmax <= 0 || grid->zmax <= 0 || p->dt <= 0 || p->Re <= 0)
     return -1;
@@ -319,12 +319,12 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
     const int ns[6] = {h->nr, h->nr, h->nrc, h->nzc, h->nzc, h->nzc};
     for (int sw = 0; sw < 6; sw++) {
       int best = -1, bestw = 1 << 30;
-      for (int M : {16, 24, 32, 40, 48, 64}) {
+      for (int M : kLineSegs) {
         int P = (ns[sw] + M - 1) / M;
         if (P > 16) continue;
         if (P * M < bestw) { bestw = P * M; best = M; }
       }
-      if (best < 0) return fail(h, "grid too large for the line solver (nr+1, nz+1 <= 1024)");
+      if (best < 0) return fail(h, "grid too large for the line solver (nr+1, nz+1 <= 16 x 72)");
       h->Msw[sw] = best;
     }
   }
@@ -711,7 +711,7 @@ extern "C" int vrh_autotune(vrh_t *h, int reps, int *choice) {
   for (int sw = 0; sw < 6; sw++) {
     int best = h->Msw[sw];
     double bestt = 1e30;
-    for (int M : {16, 24, 32, 40, 48, 64}) {
+    for (int M : kLineSegs) {
       if ((ns[sw] + M - 1) / M > 16) continue;
       h->Msw[sw] = M;
       int rc = build_line_coefs(h);
@@ -958,7 +958,9 @@ extern "C" int vrh_set_tuning(vrh_t *h, const int *choice) {
   for (int sw = 0; sw < 6; sw++) {
     const int M = choice[sw];
     if (M == 0) continue;
-    if (M != 16 && M != 24 && M != 32 && M != 40 && M != 48 && M != 64) return fail(h, "segment length must be 16..64");
+    bool known = false;
+    for (int m : kLineSegs) known = known || m == M;
+    if (!known) return fail(h, "segment length must be 16, 24, 32, 40, 48, 64 or 72");
     if ((ns[sw] + M - 1) / M > 16) return fail(h, "segment length too short for the line length");
   }
   if (choice[6] != 0 && choice[6] != 2 && choice[6] != 4 && choice[6] != 8 && choice[6] != 16)

@@ -41,7 +41,10 @@ struct VRK {
 // segment totals.
 // A, B, C, Pk, Qk: n positions followed by kLinePad zeros (>= the longest
 // segment), so that a segment may read all its positions unclamped.
-constexpr int kLinePad = 64;
+constexpr int kLinePad = 72;
+// segment lengths of the partitioned line solves (at most 16 segments per
+// line: one wave each in a block of up to 1024 threads)
+constexpr int kLineSegs[] = {16, 24, 32, 40, 48, 64, 72};
 struct LineCoef {
   const double *A, *B, *C, *Pk, *Qk, *Pseg, *Qseg;
   int n, P;

@@ -2038,6 +2038,7 @@ static void line_launch(vrh_t *h, const LineArgs &a, int M, hipStream_t st) {
     case 32: hipLaunchKernelGGL((k_line<TW, 32, MODE>), grd, blk, 0, st, a); break;
     case 40: hipLaunchKernelGGL((k_line<TW, 40, MODE>), grd, blk, 0, st, a); break;
     case 48: hipLaunchKernelGGL((k_line<TW, 48, MODE>), grd, blk, 0, st, a); break;
+    case 72: hipLaunchKernelGGL((k_line<TW, 72, MODE>), grd, blk, 0, st, a); break;
     default: hipLaunchKernelGGL((k_line<TW, 64, MODE>), grd, blk, 0, st, a); break;
   }
 }
