@@ -1,4 +1,4 @@
-! vrh_f.F90 -- iso_c_binding interface to libvrh (include/vrh.h).
+! vrh_f.F90 -- iso_c_binding interface to libvrh: every function of include/vrh.h.
 module vrh_f
   use iso_c_binding
   implicit none
@@ -6,6 +6,8 @@ module vrh_f
   integer(c_int), parameter :: VRH_FP64 = 0, VRH_MIXED = 1
   integer(c_int), parameter :: VRH_THETA_EXACT_M2 = 0, VRH_THETA_CONSISTENT = 1
   integer(c_int), parameter :: VRH_PART_LSQ_EULER = 0, VRH_PART_LSQ_RK3 = 1, VRH_PART_TRILINEAR_RK3 = 2
+  integer(c_int), parameter :: VRH_PH_HDNBC = 2, VRH_PH_EXPLICIT = 3, VRH_PH_ADI = 6, VRH_PH_POISSON = 8, &
+                               VRH_PH_PROJECT = 9
 
   type, bind(C) :: vrh_grid
     integer(c_int) :: ntheta, nr, nz
@@ -149,6 +151,136 @@ module vrh_f
       type(c_ptr), value :: h
       real(c_double), intent(out) :: out(10)
     end function vrh_analysis
+
+    ! any named state array (include/vrh.h lists names and sizes)
+    integer(c_int) function vrh_upload_state(h, name, host) bind(C, name='vrh_upload_state')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h, host
+      character(kind=c_char), intent(in) :: name(*)
+    end function vrh_upload_state
+
+    integer(c_int) function vrh_download_state(h, name, host) bind(C, name='vrh_download_state')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h, host
+      character(kind=c_char), intent(in) :: name(*)
+    end function vrh_download_state
+
+    integer(c_int) function vrh_state_size(grid, name, n) bind(C, name='vrh_state_size')
+      import :: c_int, c_long, c_char, vrh_grid
+      type(vrh_grid), intent(in) :: grid
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_long), intent(out) :: n
+    end function vrh_state_size
+
+    ! one substep of the step that has reached the current time (loop.F90:71-127)
+    integer(c_int) function vrh_substep(h, irk) bind(C, name='vrh_substep')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: irk
+    end function vrh_substep
+
+    integer(c_int) function vrh_phase(h, phase, irk) bind(C, name='vrh_phase')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: phase, irk
+    end function vrh_phase
+
+    integer(c_int) function vrh_set_graph(h, on) bind(C, name='vrh_set_graph')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: on
+    end function vrh_set_graph
+
+    ! rhs, phi: (ntheta, nr, nz) host arrays (solve_lap_tripar)
+    integer(c_int) function vrh_poisson_solve(h, rhs, phi) bind(C, name='vrh_poisson_solve')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, rhs, phi
+    end function vrh_poisson_solve
+
+    integer(c_int) function vrh_ktimer_enable(h, on) bind(C, name='vrh_ktimer_enable')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: on
+    end function vrh_ktimer_enable
+
+    integer(c_int) function vrh_ktimer_read(h, idx, name, nlen, total_ms, cnt) bind(C, name='vrh_ktimer_read')
+      import :: c_int, c_ptr, c_size_t
+      type(c_ptr), value :: h, name, total_ms, cnt
+      integer(c_int), value :: idx
+      integer(c_size_t), value :: nlen
+    end function vrh_ktimer_read
+
+    integer(c_int) function vrh_set_tuning(h, choice) bind(C, name='vrh_set_tuning')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), intent(in) :: choice(9)
+    end function vrh_set_tuning
+
+    integer(c_int) function vrh_device_ptr(h, name, ptr) bind(C, name='vrh_device_ptr')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: name(*)
+      type(c_ptr), intent(out) :: ptr
+    end function vrh_device_ptr
+
+    ! dev_out: device memory of 10 doubles (e.g. from vrh_device_ptr)
+    integer(c_int) function vrh_analysis_noalloc(h, dev_out) bind(C, name='vrh_analysis_noalloc')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, dev_out
+    end function vrh_analysis_noalloc
+
+    integer(c_int) function vrh_particles_step(h, irk) bind(C, name='vrh_particles_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: irk
+    end function vrh_particles_step
+
+    integer(c_int) function vrh_particles_compact(h, batch) bind(C, name='vrh_particles_compact')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: batch
+    end function vrh_particles_compact
+
+    integer(c_int) function vrh_particles_count(h, batch, n) bind(C, name='vrh_particles_count')
+      import :: c_int, c_ptr, c_long
+      type(c_ptr), value :: h
+      integer(c_int), value :: batch
+      integer(c_long), intent(out) :: n
+    end function vrh_particles_count
+
+    ! ninj: c_loc of an integer(c_long) (the call then waits), or c_null_ptr
+    integer(c_int) function vrh_particles_inject(h, batch, r0, inj, seed, ninj) bind(C, name='vrh_particles_inject')
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: h, ninj
+      integer(c_int), value :: batch, inj
+      real(c_double), value :: r0
+      integer(c_long_long), value :: seed
+    end function vrh_particles_inject
+
+    integer(c_int) function vrh_ensemble_create(members, n, e) bind(C, name='vrh_ensemble_create')
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: members(*)
+      integer(c_int), value :: n
+      type(c_ptr), intent(out) :: e
+    end function vrh_ensemble_create
+
+    integer(c_int) function vrh_ensemble_step(e, nsteps) bind(C, name='vrh_ensemble_step')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: nsteps
+    end function vrh_ensemble_step
+
+    integer(c_int) function vrh_ensemble_last_error(e, buf, blen) bind(C, name='vrh_ensemble_last_error')
+      import :: c_int, c_ptr, c_char, c_size_t
+      type(c_ptr), value :: e
+      character(kind=c_char), intent(out) :: buf(*)
+      integer(c_size_t), value :: blen
+    end function vrh_ensemble_last_error
+
+    integer(c_int) function vrh_ensemble_destroy(e) bind(C, name='vrh_ensemble_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+    end function vrh_ensemble_destroy
   end interface
 
 contains

@@ -8,7 +8,8 @@
 !   partNNNNNsizeBB.dat (part_tools.F90:444-467)
 ! Optional sections of this driver (ignored by the reference): ===Ring===
 ! (Gaussian-ring initial condition, CFL time step), ===GPU=== (precision,
-! Poisson theta eigenvalue, autotuning), ===Part=== (seeded particles).
+! Poisson theta eigenvalue, autotuning), ===Part=== (particles seeded in a
+! torus, or the reference's inlet batches).
 module vring_io
   use iso_c_binding
   implicit none
@@ -32,6 +33,10 @@ module vring_io
     ! ===Part===
     integer :: npart = 0, nsub = 5, pscheme = 0, pseed = 0
     real(c_double) :: ptorus(3) = (/1.d0, 3.d0, 0.6d0/), pradius = 5.0d-6
+    ! inlet batches (loop.F90:39-59): the first pinlet of the reference's
+    ! eight droplet sizes, pcap particles each, injected every pinj steps
+    integer :: pinlet = 0, pinj = 10
+    integer(c_long) :: pcap = 50000
   end type deck
 
 contains
@@ -107,6 +112,8 @@ contains
       read(u, *) a, d%nsub
       read(u, *) a, d%pscheme
       read(u, *, iostat=ios) a, d%pseed
+      read(u, *, iostat=ios) a, d%pinlet, d%pcap
+      if (ios /= 0) d%pinlet = 0
     end if
     close(u)
   end subroutine read_deck

@@ -26,6 +26,17 @@ program vring_main
   integer, allocatable :: seedv(:)
   integer :: nseed
   character(len=32) :: arg
+  ! inlet batches: droplet radii in micrometres and Euler sub-steps of the
+  ! reference's eight batches (part_tools.F90:262-269, 493-502)
+  real, parameter :: rad_um(8) = (/0.5, 1.0, 2.0, 5.0, 10.0, 20.0, 50.0, 100.0/)
+  integer, parameter :: nsub_ref(8) = (/20, 10, 10, 5, 5, 5, 5, 5/)
+  integer :: nb, b, nvarp
+  integer(c_long), allocatable :: capb(:)
+  real(c_double), allocatable :: stb(:), radb(:)
+  integer(c_int), allocatable :: nsubb(:)
+  integer(c_long) :: nlive
+  real :: rs
+  real(c_double) :: r8
 
   call read_deck('inp.txt', d)
   nt = d%nthetag; nr = d%nrg; nz = d%nzg
@@ -99,6 +110,26 @@ program vring_main
     call vrh_check(h, vrh_particles_create(h, 1, capl, stv, nsubv, d%pscheme), 'particles_create')
     call vrh_check(h, vrh_particles_upload(h, 0, int(np, c_long), c_loc(pu), c_loc(palive)), 'particles_upload')
   end if
+  nb = 0
+  if (d%pinlet > 0) then
+    if (d%npart > 0) stop 'inlet batches and torus particles are exclusive'
+    nb = min(d%pinlet, 8)
+    nvarp = merge(6, 117, d%pscheme == VRH_PART_TRILINEAR_RK3)
+    allocate(capb(nb), stb(nb), radb(nb), nsubb(nb), pu(nvarp, d%pcap), palive(d%pcap))
+    do b = 1, nb
+      ! prt%r and prt%st as init_part evaluates them (part_tools.F90:262-273):
+      ! the radius in default real, the Stokes number's constants too
+      rs = rad_um(b) * (10.0**(-6.0)) / 0.02
+      r8 = rs
+      radb(b) = r8
+      stb(b) = (1.0 / 18.0) * (998.0 / 1.2) * (r8**2) * d%Re
+      capb(b) = d%pcap
+      nsubb(b) = nsub_ref(b)
+    end do
+    call vrh_check(h, vrh_particles_create(h, nb, capb, stb, nsubb, d%pscheme), 'particles_create')
+    write(*, '(a,i2,a,i8,a,i4,a)') 'inlet particles: ', nb, ' batches of ', d%pcap, &
+        ' slots, injection every ', d%pinj, ' steps'
+  end if
   if (d%autotune /= 0) then
     call vrh_check(h, vrh_autotune(h, 3, c_loc(choice)), 'autotune')
     write(*, '(a,9i5)') 'autotune choice: ', choice
@@ -109,6 +140,17 @@ program vring_main
   do istep = 0, d%totstp
     nstep = istep + nstepi
     call cpu_time(t1)
+    ! inlet injection of every batch (loop.F90:54-59: mod(nstep, inj) == 0,
+    ! from the inlet velocity before the step's substeps); positions from
+    ! the library's counter-based stream, seeded per batch
+    if (nb > 0) then
+      if (mod(nstep, d%pinj) == 0) then
+        do b = 1, nb
+          call vrh_check(h, vrh_particles_inject(h, b - 1, d%r0, d%pinj, int(8 * d%pseed + b, c_long_long), &
+                                                 c_null_ptr), 'inject')
+        end do
+      end if
+    end if
     ! the step runs on the GPU while the host writes the files of the last
     ! field export (taken as a device snapshot at that step)
     call vrh_check(h, vrh_step_async(h, 1), 'step')
@@ -131,6 +173,13 @@ program vring_main
         call vrh_check(h, vrh_particles_download(h, 0, int(np, c_long), c_loc(pu), c_loc(palive)), 'particles')
         call part_export(nstep, 1, d%pradius / 0.02d0, st, np, size(pu, 1), pu, palive)
       end if
+      do b = 1, nb   ! partNNNNNsizeBB.dat per batch (part_tools.F90:444-467)
+        call vrh_check(h, vrh_particles_compact(h, b - 1), 'compact')
+        call vrh_check(h, vrh_particles_count(h, b - 1, nlive), 'count')
+        if (nlive > 0) call vrh_check(h, vrh_particles_download(h, b - 1, nlive, c_loc(pu), c_loc(palive)), &
+                                      'particles')
+        call part_export(nstep, b, radb(b), stb(b), int(nlive), nvarp, pu, palive)
+      end do
     else if (anl) then
       ! statistics only: reduced on the device, ten numbers come back
       call vrh_check(h, vrh_analysis(h, astat), 'analysis')

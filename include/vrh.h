@@ -103,7 +103,14 @@ int vrh_get_time(vrh_t *h, double *ntime);
  * Particles need the uniform mesh; call before creating particles. */
 int vrh_set_radial_mesh(vrh_t *h, const double *rm, const double *rc, const double *dmr, const double *dcr);
 
-/* One RK substep irk=1..3 at the current time (loop.F90:71-127). */
+/* One RK substep irk=1..3 (loop.F90:71-127) of the step that has reached
+ * the current time: the reference advances ntime before its substeps
+ * (loop.F90:53), so a driver that keeps its own RK loop calls
+ * vrh_set_time(t + dt), then for irk = 1, 2, 3 vrh_particles_step(h, irk)
+ * (where loop.F90:67 calls iter_part) and vrh_substep(h, irk).  The step's
+ * scalars (inlet increment, mass correction) come from the same device
+ * computation as vrh_step's, so the result equals one vrh_step from t bit
+ * for bit. */
 int vrh_substep(vrh_t *h, int irk);
 /* nsteps full steps: ntime += dt, then substeps 1,2,3 (loop.F90:53-131). */
 int vrh_step(vrh_t *h, int nsteps);

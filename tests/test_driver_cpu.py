@@ -33,3 +33,16 @@ def test_driver_reads_reference_decks(deck, tmp_path):
         if i in skip:
             continue
         assert a == b, (i, a, b)
+
+
+def test_fortran_module_binds_every_c_entry_point():
+    """driver/vrh_f.F90 declares an interface for every function of
+    include/vrh.h (the Fortran side of the boundary is complete)."""
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with open(os.path.join(root, "include", "vrh.h")) as f:
+        declared = set(re.findall(r"^\s*int\s+(vrh_\w+)\s*\(", f.read(), re.M))
+    with open(os.path.join(root, "driver", "vrh_f.F90")) as f:
+        bound = set(re.findall(r"bind\(C,\s*name='(vrh_\w+)'\)", f.read()))
+    assert len(declared) > 35
+    assert declared == bound, (declared - bound, bound - declared)

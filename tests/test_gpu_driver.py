@@ -183,3 +183,56 @@ def test_driver_particle_run_exports_particles(tmp_path):
     assert np.abs(b - a).max() > 1e-4          # carried by the flow
     det0 = p0[:, 3 + 117]
     assert np.all(np.abs(det0 - 1.0) < 0.05), (det0.min(), det0.max())
+
+
+INLET_DECK = """
+===GPU===
+precision            0
+theta                0
+autotune             0
+
+===Part===
+npart                0
+ptorus             1.0 3.0 0.3
+pradius            0.0001
+nsub                 5
+pscheme              0
+pseed                3
+inlet                8   20000
+"""
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+def test_driver_inlet_batches_inject_every_ten_steps(tmp_path):
+    """The driver with the reference's particle set-up (loop.F90:39-59): the
+    eight inlet batches of init_part (radii 0.5..100 um, part_tools.F90:
+    262-269), inlet injection every 10 steps from the jet (r0 = 0.5 from the
+    deck's rw.r0.rh line), LSQ-Euler with each batch's sub-steps.  Export
+    files partNNNNNsizeBB.dat for every batch at nstep = 10 and 20: the
+    injections at nstep = 10 and 20 (after the inlet ramp, t > 0.2) put
+    particles in the inlet cells, and the count grows between the exports.
+    Re = 1e5 with dt = 0.021 keeps the smallest droplets' Euler relaxation
+    stable (partdt / st = 0.36; the reference's deck has Re = 1e4,
+    dt = 0.001)."""
+    nt, nr, nz = 32, 32, 64
+    print("case driver inlet batches 32x32x64", flush=True)
+    deck = DECK.format(nt=nt, nr=nr, nz=nz, totstp=20, restrt=0, dt=0.021, anlstp=0).replace(
+        "expstp               5", "expstp              10").replace("cfl 0.3 1.0", "cfl 0.0 1.0").replace(
+        "Re                1000.0", "Re              100000.0") + INLET_DECK
+    (tmp_path / "inp.txt").write_text(deck)
+    r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "inlet particles:  8 batches" in r.stdout, r.stdout[:2000]
+    counts = {}
+    for n in (10, 20):
+        for b in range(1, 9):
+            head, rows = read_part(tmp_path / f"part{n:05d}size{b:02d}.dat")
+            counts[(n, b)] = rows.shape[0] if rows.size else 0
+            if rows.size:
+                assert rows.shape[1] == 3 + 117 + 10
+                x, y, z = rows[:, 3], rows[:, 4], rows[:, 5]
+                assert np.all(np.hypot(x, y) < 0.5 + 4.0 / nr + 1e-12), (n, b)
+                assert np.all((z >= 0.0) & (z < 12.0)), (n, b)
+    print("particles per (step, batch)", counts, flush=True)
+    for b in range(1, 9):
+        assert counts[(10, b)] > 0 and counts[(20, b)] > counts[(10, b)], (b, counts)

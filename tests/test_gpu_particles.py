@@ -123,3 +123,80 @@ def test_inlet_injection_and_compaction_match_oracle():
     g, gal = s.particles_download(0, n)
     assert np.all(gal == 1)
     assert np.abs(g - expect).max() < 1e-12 * max(1.0, np.abs(expect).max())
+
+
+def test_inlet_batches_inside_whole_steps_match_oracle():
+    """The reference's particle loop (loop.F90:39-70): eight LSQ-Euler
+    batches with the droplet radii and sub-step counts of init_part
+    (part_tools.F90:262-273, 493-502), inlet injection of every batch at
+    mod(nstep, inj) == 0 before the step (inlt_part), iter_part at irk == 1
+    inside each graph-captured step, removal at the domain boundary; 21
+    steps (injections at nstep = 0, 10, 20) against the oracle doing the
+    same with the injection restatement.  Re = 1e5 and dt = 0.021: the
+    smallest droplets' Euler relaxation (partdt / st = 0.36) is stable, as
+    with the reference's own deck (Re = 1e4, dt = 0.001), and the inlet ramp
+    (t < 0.2) is over by the second injection."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import inject_oracle as io
+    print("case particles: eight inlet batches inside whole steps", flush=True)
+    nt, nr, nz, rmax, zmax, Re, dt = 16, 32, 48, 4.0, 12.0, 1.0e5, 0.021
+    q1, q2, q3 = vring_amd.ring_initial_condition(nt, nr, nz, rmax, zmax, [(1.0, 3.0, 0.3, 1.0)], pert_amp=0.05,
+                                                  pert_modes=(2, 3), phases=np.array([0.3, 1.1]))
+    o = Oracle(nt, nr, nz, rmax, zmax, Re, dt)
+    s = vring_amd.Solver(nt, nr, nz, rmax, zmax, Re, dt)
+    for name, a in (("q1", q1), ("q2", q2), ("q3", q3)):
+        o.field(name)[...] = a
+        s.upload(name, a)
+    f32 = np.float32
+    radii = [float(f32(f32(f32(r) * f32(10.0) ** f32(-6.0)) / f32(0.02)))
+             for r in (0.5, 1.0, 2.0, 5.0, 10.0, 20.0, 50.0, 100.0)]
+    sts = [stokes_number(r, Re) for r in radii]
+    nsubs = [20, 10, 10, 5, 5, 5, 5, 5]
+    cap, inj, r0 = 12000, 10, 0.5
+    s.particles_create([cap] * 8, sts, nsubs, "lsq_euler")
+    # oracle side: rows, alive flags, inflow accumulators, injection calls
+    U = [np.zeros((0, 117)) for _ in range(8)]
+    A = [np.zeros(0, np.int32) for _ in range(8)]
+    inflow = [np.zeros(nr + 2) for _ in range(8)]
+    ncall = [0] * 8
+    for nstep in range(21):
+        if nstep % inj == 0:
+            q3_inlet = s.download("q3")[:, :, 1]
+            for b in range(8):
+                s.particles_inject(b, r0=r0, inj=inj, seed=100 + b)
+                rows, _ = io.inject(q3_inlet, inflow[b], nt, nr, rmax, zmax, nz, dt, r0, inj, 100 + b, ncall[b])
+                ncall[b] += 1
+                U[b] = np.concatenate([U[b][A[b] == 1], rows])
+                A[b] = np.ones(U[b].shape[0], np.int32)
+        s.step(1)
+        o.set_time(dt + o.time())
+        for irk in (1, 2, 3):
+            if irk == 1:
+                for b in range(8):
+                    n = U[b].shape[0]
+                    if n == 0:
+                        continue
+                    u, d1, d2 = np.ascontiguousarray(U[b]), np.zeros((n, 117)), np.zeros((n, 117))
+                    for _ in range(nsubs[b]):
+                        assert o.L.orc_part_advance(o.h, 0, 0, 1, sts[b], dt / nsubs[b], 78, n, u.ravel(),
+                                                    d1.ravel(), d2.ravel(), A[b]) == 0
+                    U[b] = u
+            assert o.substep(irk) == 0
+    total = 0
+    for b in range(8):
+        s.particles_compact(b)
+        n = s.particles_count(b)
+        live = U[b][A[b] == 1]
+        assert n == live.shape[0], (b, n, live.shape[0])
+        g, gal = s.particles_download(b, n)
+        assert np.all(gal == 1)
+        if n:
+            err = np.abs(g - live).max() / np.abs(live).max()
+            print(f"batch {b + 1}: {n} particles, rel err {err:.2e}", flush=True)
+            assert err < 1e-9, (b, err)
+        total += n
+    assert total > 100
+    s.close()
+    o.close()

@@ -499,9 +499,20 @@ int vrh_set_radial_mesh(vrh_t *h, const double *rm, const double *rc, const doub
   return 0;
 }
 
+// One substep of the step that has reached the current time, with the
+// step's scalars computed on the device as vrh_step computes them, so that
+// three calls after vrh_set_time(t + dt) equal one vrh_step from t bit for
+// bit.  The device time follows the host's: a later vrh_step continues
+// from it.
 int vrh_substep(vrh_t *h, int irk) {
   if (!h) return -1;
-  return substep(h, irk, -1);
+  if (irk < 1 || irk > 3) return fail(h, "irk must be 1..3");
+  launch_scalars_at(h, h->ntime);
+  if (h->prec == VRH_MIXED) substep_dev<float>(h, irk);
+  else substep_dev<double>(h, irk);
+  HIPCHK(h, hipGetLastError());
+  h->time_dirty = false;
+  return 0;
 }
 
 int vrh_phase(vrh_t *h, int phase, int irk) {

@@ -198,6 +198,7 @@ void host_mint(vrh_t *h, double *mint);
 // kernel launchers (vrh_kernels.hip); TW = double or float
 template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, const double *cor_dev);
 void launch_scalars(vrh_t *h);
+void launch_scalars_at(vrh_t *h, double t);
 void launch_analysis(vrh_t *h, double *part, int nblk, double *out);
 // Partial blocks of the analysis reduction (both entry points), and the
 // allocation-free variant's per-device scratch (vrh_kernels.hip): set up

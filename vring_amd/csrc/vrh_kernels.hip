@@ -55,9 +55,9 @@ __device__ __host__ inline double profile_fn(double t) {
 // c3s = (profile(t+dt) - profile(t)) alpha and the mass-correction factor
 // cor = c3s * inlet_area / (sum qc^2 da + 1e-15).
 __global__ void k_scalars(double *sc, double dt, double a1, double a2, double a3, double inlet_area,
-                          double cor_red) {
+                          double cor_red, int advance, double t_set) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double t = dt + sc[0];
+  const double t = advance ? dt + sc[0] : t_set;
   sc[0] = t;
   const double u0 = profile_fn(t), u1 = profile_fn(t + dt);
   const double al[3] = {a1, a2, a3};
@@ -1904,7 +1904,12 @@ template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, const do
 
 void launch_scalars(vrh_t *h) {
   hipLaunchKernelGGL(k_scalars, dim3(1), dim3(64), 0, h->stream, h->d_sc, h->dt, h->rk[1].a, h->rk[2].a,
-                     h->rk[3].a, h->inlet_area, h->cor_red);
+                     h->rk[3].a, h->inlet_area, h->cor_red, 1, 0.0);
+}
+// the same scalars for the step that has reached time t (no advance)
+void launch_scalars_at(vrh_t *h, double t) {
+  hipLaunchKernelGGL(k_scalars, dim3(1), dim3(64), 0, h->stream, h->d_sc, h->dt, h->rk[1].a, h->rk[2].a,
+                     h->rk[3].a, h->inlet_area, h->cor_red, 0, t);
 }
 
 template <typename TW> void launch_explicit(vrh_t *h, int irk) {
