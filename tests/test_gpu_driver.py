@@ -236,3 +236,19 @@ def test_driver_inlet_batches_inject_every_ten_steps(tmp_path):
     print("particles per (step, batch)", counts, flush=True)
     for b in range(1, 9):
         assert counts[(10, b)] > 0 and counts[(20, b)] > counts[(10, b)], (b, counts)
+
+
+SUBSTEP_CHECK = os.path.join(ROOT, "driver", "substep_check")
+
+
+@pytest.mark.skipif(not os.path.exists(SUBSTEP_CHECK), reason="substep check not built")
+def test_reference_style_rk_loop_equals_vrh_step():
+    """driver/substep_check (flang): the reference's RK loop through the C
+    ABI (vrh_set_time(t + dt); vrh_particles_step and vrh_substep for irk =
+    1..3, loop.F90:53-131) against vrh_step: q1, q2, q3, pres, prg and two
+    LSQ-Euler particle batches bit for bit, FP64 and mixed."""
+    print("case substep loop vs vrh_step (flang)", flush=True)
+    r = subprocess.run([SUBSTEP_CHECK], capture_output=True, text=True, timeout=240)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "substep check: bit-identical" in r.stdout
