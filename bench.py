@@ -204,6 +204,34 @@ def run_particles(prec, q, dt, n, steps, scheme="lsq_euler"):
             "hbm_GBps": round(per * n * nsub * len(stages) * steps / t / 1e9, 1)}
 
 
+def ensemble_specs(nrings, seed=0):
+    """(nr, ntheta, nz, Re, sigma) of the batched-ensemble rings: shapes from
+    {64, 96, 128, 192} x {32, 64, 128} x {128, 256, 512}, Re ~ U[500, 5000],
+    core sigma ~ U[0.1, 0.3], drawn with numpy PCG64(seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(nrings):
+        nr, nt, nz = int(rng.choice([64, 96, 128, 192])), int(rng.choice([32, 64, 128])), int(rng.choice([128, 256, 512]))
+        Re, sigma = float(rng.uniform(500, 5000)), float(rng.uniform(0.1, 0.3))
+        out.append((nr, nt, nz, Re, sigma))
+    return out
+
+
+def ensemble_member(spec, particles, index):
+    """One ensemble ring (mixed precision, dt at CFL 0.3) with `particles`
+    trilinear-RK3 particles in its core's torus (seed = index)."""
+    import vring_amd
+    nr, nt, nz, Re, sigma = spec
+    q = vring_amd.ring_initial_condition(nt, nr, nz, RMAX, ZMAX, [(1.0, 3.0, sigma, 1.0)])
+    dt = vring_amd.cfl_dt(nt, nr, nz, RMAX, ZMAX, *q, cfl=0.3, ujet=1.0)
+    s = vring_amd.Solver(nt, nr, nz, RMAX, ZMAX, Re, dt, precision="mixed")
+    for n, a in zip(("q1", "q2", "q3"), q):
+        s.upload(n, a)
+    if particles:
+        s.seed_particles_torus(particles, R=1.0, z0=3.0, a=3 * sigma, seed=index, nsub=1, scheme="trilinear_rk3")
+    return s
+
+
 def run_ensemble(nrings, steps, warmup, seed=0, particles=50000):
     """Batched ensemble: nrings independent rings with (nr, ntheta, nz) drawn
     from {64, 96, 128, 192} x {32, 64, 128} x {128, 256, 512}, Re ~ U[500, 5000],
@@ -211,20 +239,10 @@ def run_ensemble(nrings, steps, warmup, seed=0, particles=50000):
     trilinear-RK3 particles each, advanced by one graph launch per RK3 step.
     Returns aggregate Mcell-RK3-steps/s and the same rings stepped one after
     another with their own graphs."""
-    import vring_amd
-    rng = np.random.Generator(np.random.PCG64(seed))
     members, shapes, cells = [], [], 0
-    for _ in range(nrings):
-        nr, nt, nz = int(rng.choice([64, 96, 128, 192])), int(rng.choice([32, 64, 128])), int(rng.choice([128, 256, 512]))
-        Re, sigma = float(rng.uniform(500, 5000)), float(rng.uniform(0.1, 0.3))
-        q = vring_amd.ring_initial_condition(nt, nr, nz, RMAX, ZMAX, [(1.0, 3.0, sigma, 1.0)])
-        dt = vring_amd.cfl_dt(nt, nr, nz, RMAX, ZMAX, *q, cfl=0.3, ujet=1.0)
-        s = vring_amd.Solver(nt, nr, nz, RMAX, ZMAX, Re, dt, precision="mixed")
-        for n, a in zip(("q1", "q2", "q3"), q):
-            s.upload(n, a)
-        if particles:
-            s.seed_particles_torus(particles, R=1.0, z0=3.0, a=3 * sigma, seed=len(members), nsub=1,
-                                   scheme="trilinear_rk3")
+    for i, spec in enumerate(ensemble_specs(nrings, seed)):
+        s = ensemble_member(spec, particles, i)
+        nr, nt, nz = spec[:3]
         members.append(s)
         shapes.append((nr, nt, nz))
         cells += nr * nt * nz

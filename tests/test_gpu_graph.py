@@ -96,3 +96,33 @@ def test_captured_step_after_set_time_equals_eager(precision):
         s.close()
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.timeout(600)
+def test_batched_ensemble_workload_matches_members_stepped_alone():
+    """The batched-ensemble workload's draw (bench.ensemble_specs: shapes
+    from {64..192} x {32..128} x {128..512}, Re ~ U[500, 5000], sigma ~
+    U[0.1, 0.3], PCG64(0)) for 8 rings, mixed precision, 50 000 trilinear-RK3
+    particles each: one graph launch per step for all members equals every
+    member stepped alone, fields and particles bit for bit."""
+    from bench import ensemble_member, ensemble_specs
+    specs = ensemble_specs(8, 0)
+    print("case ensemble workload, 8 rings:", [s[:3] for s in specs], flush=True)
+    npart = 50000
+    ens_members = [ensemble_member(sp, npart, i) for i, sp in enumerate(specs)]
+    alone = [ensemble_member(sp, npart, i) for i, sp in enumerate(specs)]
+    ens = vring_amd.Ensemble(ens_members)
+    for rep in range(2):
+        ens.step(2)
+        for s in alone:
+            s.step(2)
+        for m, s in zip(ens_members, alone):
+            assert m.time() == s.time()
+            for n in FIELDS:
+                assert np.array_equal(m.download(n), s.download(n)), (rep, m.nt, m.nr, m.nz, n)
+            um, am = m.particles_download(0, npart)
+            us, as_ = s.particles_download(0, npart)
+            assert np.array_equal(um, us) and np.array_equal(am, as_), (rep, m.nt, m.nr, m.nz)
+    ens.close()
+    for s in ens_members + alone:
+        s.close()

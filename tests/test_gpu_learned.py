@@ -169,3 +169,46 @@ def test_training_lowers_loss_and_network_improves_the_warm_start():
     assert np.abs(p1 - p0).max() < 1e-5 * np.abs(p0).max()
     assert np.abs(p2 - p0).max() < 1e-5 * np.abs(p0).max()
     hz.close()
+
+
+@pytest.mark.timeout(900)
+def test_learned_solver_at_full_mode_scale():
+    """The learned-solver workload's shape: nr=256, nz=512 on the tanh grid
+    (stretch 2), all 64 modes m = 0..63 in one handle, network on the 8x
+    pooled grid, trained on a reduced set of 2048 solver-generated pairs.
+    On held-out ring-core right-hand sides every solver (plain PCG, the
+    learned warm start + PCG, the learned flexible CG with its PCG fallback)
+    reaches a 1e-8 true residual and the direct solution; the trained
+    network's warm start is never worse than the best multiple of the
+    Jacobi estimate (Galerkin), and the energy errors and iteration counts
+    of both are printed."""
+    print("case hz 256x512, 64 modes, pool 8, 2048 training pairs", flush=True)
+    S, pool = 64, 8
+    hz, g = make(256, 512, list(range(S)), pool=pool)
+    hz.net_init(11)
+    loss = hz.train(2048, lr=1e-3, seed=3)
+    assert np.all(np.isfinite(loss))
+    print("loss first/last", float(loss[:4].mean()), float(loss[-4:].mean()), flush=True)
+    hz.set_modes(np.arange(S, dtype=np.int32))
+    f = hz.sample(777, 0)
+    pd = hz.direct(f)
+    out, scale = hz.net_forward(f)
+    ej, el = [], []
+    for m in range(1, S, 9):
+        rhs = lo.rhs(g, m, f[m])
+        corr = lo.prolong(out[m], pool) * scale[m]
+        zj, _, _ = lo.galerkin(g, m, rhs, np.zeros_like(rhs))
+        zl, _, _ = lo.galerkin(g, m, rhs, corr)
+        ej.append(_energy_error(g, m, zj, pd[m], rhs))
+        el.append(_energy_error(g, m, zl, pd[m], rhs))
+    print("energy error of the start (modes 1, 10, .., 55): jacobi", np.round(ej, 4).tolist(), "learned",
+          np.round(el, 4).tolist(), flush=True)
+    assert all(b <= a * (1 + 1e-9) for a, b in zip(ej, el))
+    res = {}
+    for method in ("pcg", "learned", "learned_fcg"):
+        p, it, rel, ms = hz.solve(f, method, tol=1e-8, maxit=50000)
+        res[method] = (int(it.sum()), round(ms, 1))
+        assert np.all(rel <= 1.05e-8), (method, rel.max())
+        assert np.abs(p - pd).max() < 1e-5 * np.abs(pd).max(), method
+    print("iterations, ms:", res, flush=True)
+    hz.close()

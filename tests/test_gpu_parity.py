@@ -95,11 +95,12 @@ def test_poisson_matches_oracle(nt, nr, nz, theta):
     assert np.abs(got - ref).max() <= 1e-11 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("precision,lines", [("fp64", 8), ("mixed", 16), ("mixed", 2)])
+@pytest.mark.parametrize("precision,lines", [("fp64", 8), ("mixed", 16), ("mixed", 2), ("mixed", 32), ("fp64", 32)])
 def test_poisson_long_z_lines_match_oracle(precision, lines):
-    """nz = 512 z transforms at the bench's tile widths (16 mixed lines per
-    block run with 512 threads) against the oracle; FP64 to 1e-11, mixed to
-    the FP32 transforms' rounding."""
+    """nz = 512 z transforms at the tile widths the tuner may pick (16 mixed
+    lines per block run with 512 threads, 32 lines with 1024 threads and
+    more than 64 KB of LDS) against the oracle; FP64 to 1e-11, mixed to the
+    FP32 transforms' rounding."""
     nt, nr, nz = 8, 12, 512
     print(f"case poisson long z {precision} lines={lines}", flush=True)
     o, s = make_pair(nt, nr, nz, precision=precision)

@@ -296,7 +296,7 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
   if (analysis_scratch_init(h->device) != 0)
     return fail(h, "vrh_create: the analysis scratch array has no device address (hipGetSymbolAddress)", -3);
   HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-  explicit_lds_optin(h);
+  lds_optin(h);
   for (int q = 0; q < 2; q++) HIPCHK(h, hipStreamCreateWithFlags(&h->side[q], hipStreamNonBlocking));
   for (int irk = 0; irk < 4; irk++) {
     HIPCHK(h, hipEventCreateWithFlags(&h->fork_ev[irk], hipEventDisableTiming));
@@ -747,7 +747,7 @@ extern "C" int vrh_autotune(vrh_t *h, int reps, int *choice) {
   }
   {
     int best = 0; double bestt = 1e30;
-    for (int tl : {2, 4, 8, 16}) {
+    for (int tl : {2, 4, 8, 16, 32}) {
       h->dct_lines = tl;
       double t = h->prec == VRH_MIXED ? time_poisson<float>(h, reps) : time_poisson<double>(h, reps);
       if (t < bestt) { bestt = t; best = tl; }
@@ -976,8 +976,8 @@ extern "C" int vrh_set_tuning(vrh_t *h, const int *choice) {
   }
   if (choice[6] != 0 && choice[6] != 2 && choice[6] != 4 && choice[6] != 8 && choice[6] != 16)
     return fail(h, "fft lines per block must be 2, 4, 8 or 16");
-  if (choice[7] != 0 && choice[7] != 2 && choice[7] != 4 && choice[7] != 8 && choice[7] != 16)
-    return fail(h, "dct lines per block must be 2, 4, 8 or 16");
+  if (choice[7] != 0 && choice[7] != 2 && choice[7] != 4 && choice[7] != 8 && choice[7] != 16 && choice[7] != 32)
+    return fail(h, "dct lines per block must be 2, 4, 8, 16 or 32");
   if (choice[8] < -1 || choice[8] > 3) return fail(h, "explicit form must be -1, 0, 1, 2 or 3");
   for (int sw = 0; sw < 6; sw++) if (choice[sw]) h->Msw[sw] = choice[sw];
   if (choice[6]) h->fft_lines = choice[6];

@@ -208,7 +208,7 @@ constexpr int kMaxDevices = 64;
 int analysis_scratch_init(int dev);
 double *analysis_scratch(int dev);
 template <typename TW> void launch_explicit(vrh_t *h, int irk);
-void explicit_lds_optin(vrh_t *h);
+void lds_optin(vrh_t *h);
 // hdnbc: also launch the outlet-row kernel of hdnbc at the head of the q3
 // sweep chain (whole steps), with the mass correction cor or *cor_dev
 template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const double *c3s_dev, bool hdnbc = false,

@@ -1280,7 +1280,7 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
 // stride nl).  Makhoul's reordering turns each pair of real lines into one
 // complex FFT of length nz.  dir=+1 forward, -1 inverse (REDFT01).
 template <typename TW>
-__global__ __launch_bounds__(512) void k_dctz(TW *__restrict__ S, long nl, int n, int logn, int TL, int logtl, int dir,
+__global__ __launch_bounds__(1024) void k_dctz(TW *__restrict__ S, long nl, int n, int logn, int TL, int logtl, int dir,
                                               const double *__restrict__ twr, const double *__restrict__ twi,
                                               const double *__restrict__ er, const double *__restrict__ ei) {
   extern __shared__ unsigned char smem_raw[];
@@ -1999,11 +1999,11 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
                      (TW *)h->f.r2, (TW *)h->f.r3);
 }
 
-// Blocks of the explicit kernel's one-wave-per-row form may take more than
-// 64 KB of LDS where the device allows it (gfx950: 160 KB per CU).  The
-// instantiations that can exceed 64 KB opt in once per handle; if the
-// runtime refuses, the form keeps the 64 KB limit.
-void explicit_lds_optin(vrh_t *h) {
+// Blocks of the explicit kernel's one-wave-per-row form and tuned z-DCT
+// tiles may take more than 64 KB of LDS where the device allows it (gfx950:
+// 160 KB per CU).  Those instantiations opt in once per handle; if the
+// runtime refuses, they keep the 64 KB limit.
+void lds_optin(vrh_t *h) {
   int dev = 0, lb = 0, lc = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&lb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
@@ -2019,12 +2019,14 @@ void explicit_lds_optin(vrh_t *h) {
         reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 4, 2>),
         reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 2, 2>),
         reinterpret_cast<const void *>(&k_explicit_wave<double, double, 4, 2, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, double, 1, 8, 4>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, double, 1, 6, 3>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, double, 2, 6, 3>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, double, 2, 4, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, double, 2, 2, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, double, 4, 2, 2>)};
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 1, 8, 4>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 1, 6, 3>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 2, 6, 3>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 2, 4, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 2, 2, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 4, 2, 2>),
+        reinterpret_cast<const void *>(&k_dctz<float>),
+        reinterpret_cast<const void *>(&k_dctz<double>)};
     for (const void *f : fns)
       ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lb) == hipSuccess;
   }
@@ -2139,17 +2141,24 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
                      1.0 / adt, h->f.dq1, h->f.dq2, h->f.dq3, ext_rhs, S, S64, h->d_twt, h->d_twt + N, LB, logn); }
   // z transforms
   const int nz = g.nz, lz = ilog2(nz);
-  // lines per block: a power of two, data + twiddles within 64 KB of LDS,
-  // one FFT butterfly per thread with up to 512 threads
+  // lines per block: a power of two, data + twiddles within 64 KB of LDS
+  // (a tuned width may take the device's larger per-block LDS: 32 FP32
+  // lines read whole 128-byte rows of the strided layout), one FFT
+  // butterfly per thread with up to 1024 threads
   auto lines = [&](size_t es, int want) {
+    const size_t lim = want > 0 ? (size_t)h->lds_block : 65536;
+    const int tmax = want > 0 ? 32 : 16;
     int tl = 2;
-    while (tl * 2 <= 16 && (size_t)(tl * 2) * (nz + 1) * es + (size_t)nz * es <= 65536 &&
-           (tl * 2 / 2) * nz / fft_min_radix(lz) <= 512)
+    while (tl * 2 <= tmax && (size_t)(tl * 2) * (nz + 1) * es + (size_t)nz * es <= lim &&
+           (tl * 2 / 2) * nz / fft_min_radix(lz) <= 1024)
       tl *= 2;
     if (want > 0) while (tl > 2 && tl > want) tl /= 2;
     return tl;
   };
-  auto threads = [&](int tl) { return (tl / 2) * nz / fft_min_radix(lz) > 256 ? 512 : 256; };
+  auto threads = [&](int tl) {
+    const int b = (tl / 2) * nz / fft_min_radix(lz);
+    return b > 512 ? 1024 : b > 256 ? 512 : 256;
+  };
   const int TL = lines(sizeof(TW), h->dct_lines), TL64 = lines(sizeof(double), 0);
   const int ltl = ilog2(TL), ltl64 = ilog2(TL64);
   const int NTD = threads(TL), NTD64 = threads(TL64);
