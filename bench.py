@@ -239,6 +239,7 @@ def run_ensemble(nrings, steps, warmup, seed=0, particles=50000):
     trilinear-RK3 particles each, advanced by one graph launch per RK3 step.
     Returns aggregate Mcell-RK3-steps/s and the same rings stepped one after
     another with their own graphs."""
+    import vring_amd
     members, shapes, cells = [], [], 0
     for i, spec in enumerate(ensemble_specs(nrings, seed)):
         s = ensemble_member(spec, particles, i)
