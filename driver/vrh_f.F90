@@ -63,6 +63,13 @@ module vrh_f
       real(c_double), intent(in) :: rm(*), rc(*), dmr(*), dcr(*)
     end function vrh_set_radial_mesh
 
+    ! zm, zc, dmz, dcz over k = 0..nz+1 (the axial counterpart)
+    integer(c_int) function vrh_set_axial_mesh(h, zm, zc, dmz, dcz) bind(C, name='vrh_set_axial_mesh')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: zm(*), zc(*), dmz(*), dcz(*)
+    end function vrh_set_axial_mesh
+
     integer(c_int) function vrh_get_time(h, t) bind(C, name='vrh_get_time')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h

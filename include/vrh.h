@@ -103,6 +103,21 @@ int vrh_get_time(vrh_t *h, double *ntime);
  * Particles need the uniform mesh; call before creating particles. */
 int vrh_set_radial_mesh(vrh_t *h, const double *rm, const double *rc, const double *dmr, const double *dcr);
 
+/* Non-uniform axial mesh: the z counterpart of vrh_set_radial_mesh (the
+ * reference's z mesh is uniform, mesh_tools.F90:321-342, and reads dx3
+ * wherever a z difference is taken: momentum_terms_swirl.F90, divergence.F90,
+ * momentum_coef.F90:314-465).  Four arrays over k = 0..nz+1 (nz + 2
+ * values): cell centres zm, faces zc (zc(1) = 0, the inlet), and dz/dzeta at
+ * the centres (dmz) and faces (dcz), zeta uniform with spacing zmax/nz.
+ * Every z difference then carries its metric, the z sweeps use the
+ * second-difference weights of the mesh, and the pressure solve replaces the
+ * z cosine transform by the generalised eigenvectors of its z operator
+ * (dense nz x nz transforms, GEMMs on the device), which on the uniform
+ * mesh are the same cosines.  No reference counterpart.  Particles need the
+ * uniform axial mesh; a handle with an axial mesh steps without a HIP graph
+ * and cannot join an ensemble. */
+int vrh_set_axial_mesh(vrh_t *h, const double *zm, const double *zc, const double *dmz, const double *dcz);
+
 /* One RK substep irk=1..3 (loop.F90:71-127) of the step that has reached
  * the current time: the reference advances ntime before its substeps
  * (loop.F90:53), so a driver that keeps its own RK loop calls
@@ -131,7 +146,8 @@ int vrh_snapshot_begin(vrh_t *h);
 int vrh_snapshot_wait(vrh_t *h, double *q1, double *q2, double *q3, double *pres, double *ntime);
 /* vrh_step replays one RK3 step captured as a HIP graph (captured on first
  * use, recaptured after vrh_autotune, vrh_set_tuning, vrh_set_radial_mesh or a
- * particle change; not used while the kernel timer is on).  on = 0 launches the kernels one by one instead.  No
+ * particle change; not used while the kernel timer is on or with an axial
+ * mesh).  on = 0 launches the kernels one by one instead.  No
  * reference counterpart. */
 int vrh_set_graph(vrh_t *h, int on);
 /* One phase of substep irk (parity tests and profiling). */

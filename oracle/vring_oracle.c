@@ -22,6 +22,13 @@ struct orc {
   int wvnex;
   double dtheta, dr, dz, dx1, dx2, dx3, dx1q, dx2q, dx3q, axsym;
   double *rm, *rc, *zm, *zc, *thm, *thc, *qc, *da, *dmr, *dcr;
+  /* axial mesh (no reference counterpart: its z mesh is uniform,
+   * mesh_tools.F90:219, 321-342): dz/dzeta at centres (dmz) and faces (dcz),
+   * k = 0..nz+1; zstr = 0 keeps every z expression of the reference, and
+   * the Poisson solve's z direction uses the generalised eigenvectors of
+   * the z operator (zvf: V, zvi: V^-1, zlam: eigenvalues) instead of the DCT */
+  int zstr;
+  double *dmz, *dcz, *zvf, *zvi, *zlam;
   double *b_rm, *b_rc, *b_zm, *b_zc, *b_thm, *b_thc, *b_qc, *b_da, *b_dmr, *b_dcr;
   int *imv, *ipv, *kmv, *kpv, *jmv, *jpv, *jmc, *jpc;
   double gam[4], rom[4], alm[4];
@@ -50,6 +57,17 @@ struct orc {
 #define RS(a, i, j, k) (a)[((i) - 1) + (size_t)o->nt * (((j) - 1) + (size_t)o->nr * ((k) - 1))]
 #define DP(i, j, k) o->dph[((i) - 1) + (size_t)o->nt * ((size_t)(j) + (size_t)(o->nr + 2) * (size_t)(k))]
 #define TS(a, j, i) (a)[(j) + (size_t)(o->nrc + 1) * (size_t)(i)]           /* (0:nrc,0:nt) */
+/* axial metric: inverse cell width at centre k, inverse centre spacing at
+ * face k; second-difference weights at centres (CM: k-1, CP: k+1) and at
+ * faces (FM, FP) */
+#define ZM3(k) (o->dx3 / o->dmz[k])
+#define ZC3(k) (o->dx3 / o->dcz[k])
+#define CM(k) (ZM3(k) * ZC3(k))
+#define CP(k) (ZM3(k) * ZC3((k) + 1))
+#define FM(k) (ZC3(k) * ZM3((k) - 1))
+#define FP(k) (ZC3(k) * ZM3(k))
+
+static void coesys_z_mesh(orc_t *o);
 
 static double *dalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
 
@@ -80,6 +98,8 @@ orc_t *orc_create(int nt, int nr, int nz, double rmax, double zmax, double Re,
   o->b_da = dalloc(nr + 2); o->da = o->b_da;
   o->b_dmr = dalloc(nr + 2); o->dmr = o->b_dmr;
   o->b_dcr = dalloc(nr + 2); o->dcr = o->b_dcr;
+  o->dmz = dalloc(nz + 3); o->dcz = dalloc(nz + 3);
+  for (int k = 0; k <= nz + 2; k++) { o->dmz[k] = 1.0; o->dcz[k] = 1.0; }
   for (int k = 0; k <= nz + 1; k++) o->zm[k] = 0.0 + (k - 0.5) * o->dz;   /* :321-323 */
   for (int j = 0; j <= nr + 1; j++) { o->dmr[j] = 1.0; o->dcr[j] = 1.0; } /* :325-330 */
   for (int j = 0; j <= nr + 1; j++) o->rm[j] = 0.0 + (j - 0.5) * o->dr;   /* :332-334 */
@@ -159,7 +179,7 @@ void orc_destroy(orc_t *o) {
                     o->corr2s, o->corr3s, o->corr1n, o->corr2n, o->corr3n, o->csort, o->tconv,
                     o->rhs, o->sol, o->dph, o->aa_base, o->bb_base, o->cc_base, o->ami, o->aci,
                     o->api, o->fi, o->gam2, o->atri, o->btri, o->ctri, o->rtri, o->stri, o->beta,
-                    o->gamm};
+                    o->gamm, o->dmz, o->dcz, o->zvf, o->zvi, o->zlam};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
   free(o->imv); free(o->ipv); free(o->kmv); free(o->kpv); free(o->jmv); free(o->jpv);
   free(o->jmc); free(o->jpc);
@@ -219,6 +239,93 @@ void orc_set_radial_mesh(orc_t *o, const double *rm, const double *rc, const dou
     o->bb_base[j] = -(o->aa_base[j] + o->cc_base[j]);
   }
 }
+/* Eigen-decomposition of a dense symmetric n x n matrix A (row-major, destroyed)
+ * by cyclic Jacobi rotations: on return A holds the eigenvalues on its
+ * diagonal and W (row-major, W[k*n+q] = component k of eigenvector q) the
+ * orthonormal eigenvectors. */
+static void jacobi_eig(int n, double *A, double *W) {
+  for (int i = 0; i < n * n; i++) W[i] = 0.0;
+  for (int i = 0; i < n; i++) W[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 60; sweep++) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        tot += A[i * n + j] * A[i * n + j];
+        if (i != j) off += A[i * n + j] * A[i * n + j];
+      }
+    if (off <= 1e-30 * tot) break;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        const double apq = A[p * n + q];
+        if (apq == 0.0) continue;
+        /* rotation angle zeroing A[p][q]: tan(2 phi) = 2 apq / (aqq - app) */
+        const double th = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+        const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < n; k++) {          /* columns p, q */
+          const double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - sn * akq;
+          A[k * n + q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {          /* rows p, q */
+          const double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - sn * aqk;
+          A[q * n + k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          const double wkp = W[k * n + p], wkq = W[k * n + q];
+          W[k * n + p] = c * wkp - sn * wkq;
+          W[k * n + q] = sn * wkp + c * wkq;
+        }
+      }
+  }
+}
+
+/* Non-uniform axial mesh (no reference counterpart; the radial hooks of
+ * mesh_tools.F90:325-346 carried over to z): cell centres zm, faces zc
+ * (zc(1) = 0, the inlet), and dz/dzeta at centres (dmz) and faces (dcz),
+ * k = 0..nz+1, zeta uniform with spacing zmax/nz (derivatives keep
+ * dx3 = nz/zmax).  Every z difference then carries the metric of where it
+ * sits; the pressure solve's z direction is diagonalised by the
+ * generalised eigenvectors of minus its z operator D_z G_z (Neumann ends),
+ * K v = lam M v with M the cell widths, which on the uniform mesh are the
+ * DCT's cosines with lam = (2/dz^2)(1 - cos(pi k/nz)). */
+void orc_set_axial_mesh(orc_t *o, const double *zm, const double *zc, const double *dmz, const double *dcz) {
+  const int nz = o->nz;
+  for (int k = 0; k <= nz + 1; k++) { o->zm[k] = zm[k]; o->zc[k] = zc[k]; o->dmz[k] = dmz[k]; o->dcz[k] = dcz[k]; }
+  o->dmz[nz + 2] = o->dmz[nz + 1];
+  o->dcz[nz + 2] = o->dcz[nz + 1];
+  o->zstr = 1;
+  free(o->zvf); free(o->zvi); free(o->zlam);
+  o->zvf = dalloc((size_t)nz * nz); o->zvi = dalloc((size_t)nz * nz); o->zlam = dalloc(nz);
+  double *A = dalloc((size_t)nz * nz), *W = dalloc((size_t)nz * nz);
+  for (int k = 1; k <= nz; k++) {          /* A = D K D, D = diag(sqrt(ZM3)) */
+    double lo = (k > 1) ? ZC3(k) : 0.0, hi = (k < nz) ? ZC3(k + 1) : 0.0;
+    A[(k - 1) * nz + (k - 1)] = ZM3(k) * (lo + hi);
+    if (k < nz) {
+      double e = -sqrt(ZM3(k) * ZM3(k + 1)) * ZC3(k + 1);
+      A[(k - 1) * nz + k] = e;
+      A[k * nz + (k - 1)] = e;
+    }
+  }
+  jacobi_eig(nz, A, W);
+  /* eigenvalues ascending (selection order); the smallest is the constant mode, exactly 0 */
+  int *ord = (int *)calloc(nz, sizeof(int));
+  for (int q = 0; q < nz; q++) ord[q] = q;
+  for (int a = 0; a < nz; a++)
+    for (int b = a + 1; b < nz; b++)
+      if (A[ord[b] * nz + ord[b]] < A[ord[a] * nz + ord[a]]) { int t = ord[a]; ord[a] = ord[b]; ord[b] = t; }
+  for (int q = 0; q < nz; q++) {
+    const int c = ord[q];
+    o->zlam[q] = (q == 0) ? 0.0 : A[c * nz + c];
+    for (int k = 1; k <= nz; k++) {
+      o->zvf[(k - 1) * nz + q] = sqrt(ZM3(k)) * W[(k - 1) * nz + c];   /* V = D W */
+      o->zvi[q * nz + (k - 1)] = W[(k - 1) * nz + c] / sqrt(ZM3(k));   /* V^-1 = W^T D^-1 */
+    }
+  }
+  free(ord); free(A); free(W);
+}
+
 double orc_get_time(const orc_t *o) { return o->ntime; }
 
 /* ---------------------------------------------------------------------------
@@ -294,6 +401,7 @@ static void coesys(orc_t *o) {
   o->amj3[nr] = -(betadx * o->s3t2s[nr]);
   o->acj3[nr] = 1.0 - (betadx * o->s3t2n[nr] * (o->slip - 1.0) + o->amj3[nr]);
   /* z direction, :314-465 (if_grad_null = 0) */
+  if (o->zstr) { coesys_z_mesh(o); return; }
   betadx = 0.5 * o->dx3q * o->alm[o->irk] * o->dt / o->Re;
   for (int kc = 1; kc <= nz; kc++) {
     o->amk1[kc] = -betadx; o->ack1[kc] = 1.0 + 2.0 * betadx; o->apk1[kc] = -betadx;
@@ -308,6 +416,32 @@ static void coesys(orc_t *o) {
   for (int kc = 1; kc <= nz; kc++) {
     o->amk3[kc] = -betadx; o->ack3[kc] = 1.0 + 2.0 * betadx; o->apk3[kc] = -betadx;
   }
+  o->amk3[nzc] = 0.0; o->ack3[nzc] = 1.0; o->apk3[nzc] = 0.0;
+  o->amk3[1] = 0.0; o->ack3[1] = 1.0; o->apk3[1] = 0.0;
+}
+
+/* z sweeps on the axial mesh: centre rows (q1, q2) and face rows (q3) from
+ * the second-difference weights; the inlet row of q1, q2 is the one-sided
+ * second difference through the wall value at z = 0 at distance h1 (half
+ * the first cell) and the next centre at h2,
+ * f'' = ws fs - (ws + w2) f1 + w2 f2, ws = 2/(h1 (h1+h2)), w2 = 2/(h2 (h1+h2)),
+ * which is momentum_coef.F90's (-8/3, 4, -4/3) beta dx3q row on the
+ * uniform mesh; the end rows keep the reference's forms. */
+static void coesys_z_mesh(orc_t *o) {
+  const int nz = o->nz, nzc = o->nzc;
+  const double bz = 0.5 * o->alm[o->irk] * o->dt / o->Re;
+  const double h1 = 0.5 / ZM3(1), h2 = 1.0 / ZC3(2);
+  const double ws = 2.0 / (h1 * (h1 + h2)), w2 = 2.0 / (h2 * (h1 + h2));
+  for (int kc = 1; kc <= nz; kc++) {
+    double cm = CM(kc), cp = CP(kc), fm = FM(kc), fp = FP(kc);
+    o->amk1[kc] = -(bz * cm); o->ack1[kc] = 1.0 + bz * (cm + cp); o->apk1[kc] = -(bz * cp);
+    o->amk2[kc] = o->amk1[kc]; o->ack2[kc] = o->ack1[kc]; o->apk2[kc] = o->apk1[kc];
+    o->amk3[kc] = -(bz * fm); o->ack3[kc] = 1.0 + bz * (fm + fp); o->apk3[kc] = -(bz * fp);
+  }
+  o->amk1[nzc] = 1.0; o->ack1[nzc] = 1.0; o->apk1[nzc] = 0.0;
+  o->amk1[1] = -(bz * ws); o->ack1[1] = 1.0 + bz * (ws + w2); o->apk1[1] = -(bz * w2);
+  o->amk2[nzc] = 1.0; o->ack2[nzc] = 1.0; o->apk2[nzc] = 0.0;
+  o->amk2[1] = o->amk1[1]; o->ack2[1] = o->ack1[1]; o->apk2[1] = o->apk1[1];
   o->amk3[nzc] = 0.0; o->ack3[nzc] = 1.0; o->apk3[nzc] = 0.0;
   o->amk3[1] = 0.0; o->ack3[1] = 1.0; o->apk3[1] = 0.0;
 }
@@ -357,7 +491,8 @@ static void hdnbc(orc_t *o) {
       if (F3(o->q3, ic, jc, nzc) > 0.0) CR(o->csort, ic, jc) = (double)0.6f * F3(o->q3, ic, jc, nzc);
   for (int ic = 1; ic <= nt; ic++)
     for (int jc = 1; jc <= nrc; jc++) {                               /* :234-241 */
-      CR(o->tconv, ic, jc) = -(CR(o->csort, ic, jc) * o->dx3) * (F3(o->q3, ic, jc, nzc) - F3(o->q3, ic, jc, nz));
+      CR(o->tconv, ic, jc) = -(CR(o->csort, ic, jc) * (o->zstr ? ZM3(nz) : o->dx3)) *
+                             (F3(o->q3, ic, jc, nzc) - F3(o->q3, ic, jc, nz));
       CR(o->corr3n, ic, jc) = (o->gam[irk] * CR(o->tconv, ic, jc) + o->rom[irk] * PL(o->dq3no, ic, jc)) * o->dt;
       PL(o->dq3no, ic, jc) = CR(o->tconv, ic, jc);
     }
@@ -401,7 +536,7 @@ static void hdnl1(orc_t *o) {
           qss = 0.5 * (PL(o->q3s, ic, jc) + PL(o->q3s, im, jc)) * PL(o->q1s, ic, jc);
         else
           qss = 0.25 * (F3(q3, ic, jc, kc) + F3(q3, im, jc, kc)) * (F3(q1, ic, jc, kc) + F3(q1, ic, jc, kc - 1));
-        h13 = (qnn - qss) * o->dx3;
+        h13 = (qnn - qss) * (o->zstr ? ZM3(kc) : o->dx3);
         qnn = F3(q2, ic, jc + 1, kc) + F3(q2, ic, jc, kc);
         qss = F3(q2, im, jc + 1, kc) + F3(q2, im, jc, kc);
         d1q2 = (qnn - qss) * o->h1dare[jc];
@@ -431,7 +566,7 @@ static void hdnl2(orc_t *o) {
           qss = 0.5 * (PL(o->q3s, ic, jc) + PL(o->q3s, ic, jc - 1)) * PL(o->q2s, ic, jc);
         else
           qss = 0.25 * (F3(q3, ic, jc, kc) + F3(q3, ic, jc - 1, kc)) * (F3(q2, ic, jc, kc) + F3(q2, ic, jc, kc - 1));
-        h23 = (qnn - qss) * o->dx3;
+        h23 = (qnn - qss) * (o->zstr ? ZM3(kc) : o->dx3);
         qnn = F3(q1, ip, jc, kc) + F3(q1, ip, jc - 1, kc);
         qss = F3(q1, ic, jc, kc) + F3(q1, ic, jc - 1, kc);
         d1q1 = -((qnn - qss) * o->h2dare[jc]);
@@ -455,7 +590,7 @@ static void hdnl3(orc_t *o) {
         h31 = (qnn - qss) * o->h1d11[jc];
         qnn = 0.25 * (F3(q3, ic, jc, kc) + F3(q3, ic, jc, kc + 1)) * (F3(q3, ic, jc, kc) + F3(q3, ic, jc, kc + 1));
         qss = 0.25 * (F3(q3, ic, jc, kc) + F3(q3, ic, jc, kc - 1)) * (F3(q3, ic, jc, kc) + F3(q3, ic, jc, kc - 1));
-        h33 = (qnn - qss) * o->dx3;
+        h33 = (qnn - qss) * (o->zstr ? ZC3(kc) : o->dx3);
         F3(o->dq3, ic, jc, kc) = -(h31 + h32 + h33);
       }
   }
@@ -590,6 +725,18 @@ static void solqk(orc_t *o, int ieq) {
     }
 }
 
+/* z second difference of a centre quantity on the axial mesh: the
+ * one-sided form through the wall value fs at the inlet cell, the
+ * centred weights elsewhere. */
+static double zdiff2c(const orc_t *o, double fm1, double f0, double fp1, double fs, int kc) {
+  if (kc == 1) {
+    const double h1 = 0.5 / ZM3(1), h2 = 1.0 / ZC3(2);
+    const double ws = 2.0 / (h1 * (h1 + h2)), w2 = 2.0 / (h2 * (h1 + h2));
+    return (fp1 - f0) * w2 - (f0 - fs) * ws;
+  }
+  return (fp1 - f0) * CP(kc) - (f0 - fm1) * CM(kc);
+}
+
 /* step, step.F90:17-210: explicit right-hand sides and the ADI sweeps. */
 static void step(orc_t *o) {
   const int irk = o->irk;
@@ -602,7 +749,10 @@ static void step(orc_t *o) {
       for (int kc = 1; kc <= o->nz; kc++) {
         double d11, d33, d22, qnn, qss;
         d11 = (F3(o->q1, ip, jc, kc) - 2.0 * F3(o->q1, ic, jc, kc) + F3(o->q1, im, jc, kc)) * o->s1t11[jc];
-        if (kc == 1)
+        if (o->zstr)
+          d33 = zdiff2c(o, F3(o->q1, ic, jc, kc - 1), F3(o->q1, ic, jc, kc), F3(o->q1, ic, jc, kc + 1),
+                        PL(o->q1s, ic, jc), kc);
+        else if (kc == 1)
           d33 = 4.0 / 3.0 * (F3(o->q1, ic, jc, kc + 1) - 3.0 * F3(o->q1, ic, jc, kc) + 2.0 * PL(o->q1s, ic, jc)) * o->dx3q;
         else
           d33 = (F3(o->q1, ic, jc, kc + 1) - 2.0 * F3(o->q1, ic, jc, kc) + F3(o->q1, ic, jc, kc - 1)) * o->dx3q;
@@ -627,7 +777,11 @@ static void step(orc_t *o) {
       for (int jc = 1; jc <= o->nr; jc++) {
         double d11, d33, d22, qnn, qss;
         d11 = (F3(o->q3, ip, jc, kc) - 2.0 * F3(o->q3, ic, jc, kc) + F3(o->q3, im, jc, kc)) * o->s1t11[jc];
-        d33 = (F3(o->q3, ic, jc, kc + 1) - 2.0 * F3(o->q3, ic, jc, kc) + F3(o->q3, ic, jc, kc - 1)) * o->dx3q;
+        if (o->zstr)
+          d33 = (F3(o->q3, ic, jc, kc + 1) - F3(o->q3, ic, jc, kc)) * FP(kc) -
+                (F3(o->q3, ic, jc, kc) - F3(o->q3, ic, jc, kc - 1)) * FM(kc);
+        else
+          d33 = (F3(o->q3, ic, jc, kc + 1) - 2.0 * F3(o->q3, ic, jc, kc) + F3(o->q3, ic, jc, kc - 1)) * o->dx3q;
         qnn = (F3(o->q3, ic, jc + 1, kc) - F3(o->q3, ic, jc, kc)) * o->s3t2n[jc];
         qss = (F3(o->q3, ic, jc, kc) - F3(o->q3, ic, o->jmv[jc], kc)) * o->s3t2s[jc];
         d22 = qnn - qss;
@@ -646,7 +800,10 @@ static void step(orc_t *o) {
       for (int jc = 2; jc <= o->nr; jc++) {
         double d11, d33, d22;
         d11 = (F3(o->q2, ip, jc, kc) - 2.0 * F3(o->q2, ic, jc, kc) + F3(o->q2, im, jc, kc)) * o->s2t11[jc];
-        if (kc == 1)
+        if (o->zstr)
+          d33 = zdiff2c(o, F3(o->q2, ic, jc, kc - 1), F3(o->q2, ic, jc, kc), F3(o->q2, ic, jc, kc + 1),
+                        PL(o->q2s, ic, jc), kc);
+        else if (kc == 1)
           d33 = 4.0 / 3.0 * (F3(o->q2, ic, jc, kc + 1) - 3.0 * F3(o->q2, ic, jc, kc) + 2.0 * PL(o->q2s, ic, jc)) * o->dx3q;
         else
           d33 = (F3(o->q2, ic, jc, kc + 1) - 2.0 * F3(o->q2, ic, jc, kc) + F3(o->q2, ic, jc, kc - 1)) * o->dx3q;
@@ -671,7 +828,7 @@ static void divq(orc_t *o) {
       for (int kc = 1; kc <= o->nz; kc++) {
         double t1 = (F3(o->dq1, ip, jc, kc) - F3(o->dq1, ic, jc, kc)) * o->dx1 * o->axsym / o->rm[jc];
         double t2 = (F3(o->dq2, ic, jc + 1, kc) - F3(o->dq2, ic, jc, kc)) * o->dx2 / (o->rm[jc] * o->dmr[jc]);
-        double t3 = (F3(o->dq3, ic, jc, kc + 1) - F3(o->dq3, ic, jc, kc)) * o->dx3;
+        double t3 = (F3(o->dq3, ic, jc, kc + 1) - F3(o->dq3, ic, jc, kc)) * (o->zstr ? ZM3(kc) : o->dx3);
         RS(o->rhs, ic, jc, kc) = -(t1 + t2 + t3) / den;
       }
   }
@@ -819,6 +976,7 @@ static void solve_radial(orc_t *o, int m, int mz, const double *rhs, double *sol
     wth = 2.0 / (o->dtheta * o->dtheta) * (1.0 - cos(2.0 * pi * m / o->nt));
     wz = 2.0 / (o->dz * o->dz) * (1.0 - cos(mz * pi / o->nz));
   }
+  if (o->zstr) wz = o->zlam[mz];
   for (int j = 1; j <= nr; j++) {
     aa[j] = o->aa_base[j];
     bb[j] = o->bb_base[j] + wth / (o->rm[j] * o->rm[j]) + wz;
@@ -844,6 +1002,14 @@ static void poisson_core(orc_t *o, const double *rhs_in, double *sol_out) {
   for (int m = 1; m <= nt; m++)
     for (int j = 1; j <= nr; j++) {
       for (int k = 1; k <= nz; k++) line[k - 1] = RS(act, m, j, k);
+      if (o->zstr) {                 /* generalised eigen-transform: V^-1 line */
+        for (int q = 0; q < nz; q++) {
+          double acc = 0.0;
+          for (int k = 0; k < nz; k++) acc += o->zvi[q * nz + k] * line[k];
+          RS(act, m, j, q + 1) = acc;
+        }
+        continue;
+      }
       redft10(nz, line, line2, 1.0, wr, wi);
       for (int k = 1; k <= nz; k++) RS(act, m, j, k) = line2[k - 1] / (2.0 * nz * 1);
     }
@@ -865,6 +1031,14 @@ static void poisson_core(orc_t *o, const double *rhs_in, double *sol_out) {
   for (int m = 1; m <= nt; m++)
     for (int j = 1; j <= nr; j++) {
       for (int k = 1; k <= nz; k++) line[k - 1] = RS(sol_out, m, j, k);
+      if (o->zstr) {                 /* V line_hat */
+        for (int k = 0; k < nz; k++) {
+          double acc = 0.0;
+          for (int q = 0; q < nz; q++) acc += o->zvf[k * nz + q] * line[q];
+          RS(sol_out, m, j, k + 1) = acc;
+        }
+        continue;
+      }
       redft01(nz, line, line2, wr, wi);
       for (int k = 1; k <= nz; k++) RS(sol_out, m, j, k) = line2[k - 1];
     }
@@ -908,7 +1082,8 @@ static void upsol(orc_t *o) {
   for (int kc = 2; kc <= nzc; kc++)                                               /* :167-174 */
     for (int jc = 1; jc <= nr; jc++)
       for (int ic = 1; ic <= nt; ic++)
-        F3(o->q3, ic, jc, kc) = F3(o->dq3, ic, jc, kc) - adt * o->dx3 * (DP(ic, jc, kc) - DP(ic, jc, kc - 1));
+        F3(o->q3, ic, jc, kc) = F3(o->dq3, ic, jc, kc) -
+                                adt * (o->zstr ? ZC3(kc) : o->dx3) * (DP(ic, jc, kc) - DP(ic, jc, kc - 1));
   for (int jc = 0; jc <= nrc; jc++)                                               /* :193-200 */
     for (int ic = 1; ic <= nt; ic++) F3(o->q3, ic, jc, 1) = F3(o->dq3, ic, jc, 1);
   for (int ic = 1; ic <= nt; ic++)                                                /* :203-208 */
@@ -935,7 +1110,9 @@ static int uppres(orc_t *o) {
           int km = o->kmv[kc], kp = kc + 1, jm = o->jmv[jc], jp = o->jpv[jc];
           int ip = o->ipv[ic], im = o->imv[ic];
           double d11 = (F3(qcap, ip, jc, kc) - 2.0 * F3(qcap, ic, jc, kc) + F3(qcap, im, jc, kc)) * o->s1t11[jc];
-          double d33 = (F3(qcap, ic, jc, kp) - 2.0 * F3(qcap, ic, jc, kc) + F3(qcap, ic, jc, km)) * o->dx3q;
+          double d33 = o->zstr ? (F3(qcap, ic, jc, kp) - F3(qcap, ic, jc, kc)) * CP(kc) -
+                                     (F3(qcap, ic, jc, kc) - F3(qcap, ic, jc, km)) * CM(kc)
+                               : (F3(qcap, ic, jc, kp) - 2.0 * F3(qcap, ic, jc, kc) + F3(qcap, ic, jc, km)) * o->dx3q;
           double qnn = (F3(qcap, ic, jp, kc) / o->rm[jc + 1] - F3(qcap, ic, jc, kc) / o->rm[jc]);
           double qss = (F3(qcap, ic, jc, kc) / o->rm[jc] - F3(qcap, ic, jm, kc) / o->rm[jm]);
           double d22 = (qnn * o->s1t2n[jc] - qss * o->s1t2s[jc]) * o->s1t2c[jc];
@@ -953,7 +1130,9 @@ static int uppres(orc_t *o) {
       for (int ic = 1; ic <= nt; ic++) {
         int km = o->kmv[kc], kp = kc + 1, ip = o->ipv[ic], im = o->imv[ic];
         double d11 = (F3(qcap, ip, jc, kc) - 2.0 * F3(qcap, ic, jc, kc) + F3(qcap, im, jc, kc)) * o->s2t11[jc];
-        double d33 = (F3(qcap, ic, jc, kp) - 2.0 * F3(qcap, ic, jc, kc) + F3(qcap, ic, jc, km)) * o->dx3q;
+        double d33 = o->zstr ? (F3(qcap, ic, jc, kp) - F3(qcap, ic, jc, kc)) * CP(kc) -
+                                   (F3(qcap, ic, jc, kc) - F3(qcap, ic, jc, km)) * CM(kc)
+                             : (F3(qcap, ic, jc, kp) - 2.0 * F3(qcap, ic, jc, kc) + F3(qcap, ic, jc, km)) * o->dx3q;
         double qnn = F3(qcap, ic, jc + 1, kc) - F3(qcap, ic, jc, kc);
         double qss = F3(qcap, ic, jc, kc) - F3(qcap, ic, jc - 1, kc);
         double d22 = qnn * o->s2t2p[jc] - qss * o->s2t2m[jc];
@@ -965,7 +1144,7 @@ static int uppres(orc_t *o) {
   for (int kc = 2; kc <= nzc; kc++)                                               /* :392-402 */
     for (int jc = 1; jc <= nr; jc++)
       for (int ic = 1; ic <= nt; ic++) {
-        F3(qcap, ic, jc, kc) = (DP(ic, jc, kc) - DP(ic, jc, kc - 1)) * o->dx3;
+        F3(qcap, ic, jc, kc) = (DP(ic, jc, kc) - DP(ic, jc, kc - 1)) * (o->zstr ? ZC3(kc) : o->dx3);
         if (isnan(F3(qcap, ic, jc, kc))) crash = 1;
       }
   if (crash) return -2;
@@ -974,7 +1153,9 @@ static int uppres(orc_t *o) {
       for (int ic = 1; ic <= nt; ic++) {
         int jm = o->jmv[jc], jp = o->jpv[jc], ip = o->ipv[ic], im = o->imv[ic];
         double d11 = (F3(qcap, ip, jc, kc) - 2.0 * F3(qcap, ic, jc, kc) + F3(qcap, im, jc, kc)) * o->s1t11[jc];
-        double d33 = (F3(qcap, ic, jc, kc + 1) - 2.0 * F3(qcap, ic, jc, kc) + F3(qcap, ic, jc, kc - 1)) * o->dx3q;
+        double d33 = o->zstr ? (F3(qcap, ic, jc, kc + 1) - F3(qcap, ic, jc, kc)) * FP(kc) -
+                                   (F3(qcap, ic, jc, kc) - F3(qcap, ic, jc, kc - 1)) * FM(kc)
+                             : (F3(qcap, ic, jc, kc + 1) - 2.0 * F3(qcap, ic, jc, kc) + F3(qcap, ic, jc, kc - 1)) * o->dx3q;
         double qnn = (F3(qcap, ic, jp, kc) - F3(qcap, ic, jc, kc));
         double qss = (F3(qcap, ic, jc, kc) - F3(qcap, ic, jm, kc));
         double d22 = qnn * o->s3t2n[jc] - qss * o->s3t2s[jc];
@@ -989,7 +1170,8 @@ static int uppres(orc_t *o) {
         double qnn = DP(ic, jp, kc) - DP(ic, jc, kc);
         double qss = DP(ic, jc, kc) - DP(ic, jm, kc);
         double d22 = qnn * o->s3t2n[jc] - qss * o->s3t2s[jc];
-        double d33 = (DP(ic, jc, kp) + DP(ic, jc, km) - 2.0 * DP(ic, jc, kc)) * o->dx3q;
+        double d33 = o->zstr ? (DP(ic, jc, kp) - DP(ic, jc, kc)) * CP(kc) - (DP(ic, jc, kc) - DP(ic, jc, km)) * CM(kc)
+                             : (DP(ic, jc, kp) + DP(ic, jc, km) - 2.0 * DP(ic, jc, kc)) * o->dx3q;
         F3(o->pres, ic, jc, kc) = F3(o->pres, ic, jc, kc) + DP(ic, jc, kc) - be * (d11 + d22 + d33);
       }
   return 0;
@@ -1038,7 +1220,7 @@ void orc_divergence(orc_t *o, const double *q1, const double *q2, const double *
       for (int kc = 1; kc <= o->nz; kc++) {
         double t1 = (F3(q1, ip, jc, kc) - F3(q1, ic, jc, kc)) * o->dx1 * o->axsym / o->rm[jc];
         double t2 = (F3(q2, ic, jc + 1, kc) - F3(q2, ic, jc, kc)) * o->dx2 / (o->rm[jc] * o->dmr[jc]);
-        double t3 = (F3(q3, ic, jc, kc + 1) - F3(q3, ic, jc, kc)) * o->dx3;
+        double t3 = (F3(q3, ic, jc, kc + 1) - F3(q3, ic, jc, kc)) * (o->zstr ? ZM3(kc) : o->dx3);
         RS(div, ic, jc, kc) = t1 + t2 + t3;
       }
   }

@@ -37,6 +37,9 @@ double *orc_field(orc_t *o, const char *name, long *count);
 void orc_set_time(orc_t *o, double ntime);
 /* non-uniform radial mesh (rm, rc, dmr, dcr over j = 0..nr+1) */
 void orc_set_radial_mesh(orc_t *o, const double *rm, const double *rc, const double *dmr, const double *dcr);
+/* Non-uniform axial mesh: zm, zc, dmz, dcz over k = 0..nz+1 (no reference
+ * counterpart; the radial hooks carried over to z). */
+void orc_set_axial_mesh(orc_t *o, const double *zm, const double *zc, const double *dmz, const double *dcz);
 double orc_get_time(const orc_t *o);
 int orc_phase(orc_t *o, int phase, int irk);
 int orc_substep(orc_t *o, int irk);

@@ -47,6 +47,7 @@ def lib():
         L.orc_poisson.argtypes = [P, dp, dp]
         L.orc_divergence.argtypes = [P, dp, dp, dp, dp]
         L.orc_set_radial_mesh.argtypes = [P, dp, dp, dp, dp]
+        L.orc_set_axial_mesh.argtypes = [P, dp, dp, dp, dp]
         L.orc_part_interp.argtypes = [P, I, dp, dp, dp, dp]
         L.orc_part_advance.argtypes = [P, I, I, I, D, D, I, ctypes.c_long, dp, dp, dp, ip]
         L.orc_part_trilinear.argtypes = [P, dp, dp]
@@ -105,6 +106,11 @@ class Oracle:
         assert all(x.size == self.nr + 2 for x in a)
         self.L.orc_set_radial_mesh(self.h, *a)
 
+    def set_axial_mesh(self, zm, zc, dmz, dcz):
+        a = [np.ascontiguousarray(np.asarray(x, dtype=np.float64)) for x in (zm, zc, dmz, dcz)]
+        assert all(x.size == self.nz + 2 for x in a)
+        self.L.orc_set_axial_mesh(self.h, *a)
+
     def time(self):
         return self.L.orc_get_time(self.h)
 
@@ -122,6 +128,32 @@ class Oracle:
         sol = np.zeros_like(rhs)
         self.L.orc_poisson(self.h, rhs, sol)
         return sol.reshape((self.nt, self.nr, self.nz), order="F")
+
+
+def tanh_axial_mesh(nz, zmax, s):
+    """Axial mesh with faces z(zeta) = zmax (1 + tanh(s (zeta/zmax - 1)) / tanh s)
+    (cells cluster at the inlet z = 0), zeta uniform with spacing zmax/nz:
+    over k = 0..nz+1 the centres zm (midpoints of the faces), faces zc
+    (zc(1) = 0), and dmz, dcz = the cell widths and centre spacings over
+    zmax/nz, with mirrored ghost cells at both ends."""
+    k = np.arange(nz + 1)
+    zf = zmax * (1.0 + np.tanh(s * (k / nz - 1.0)) / np.tanh(s)) if s > 0 else zmax * k / nz
+    zf[0], zf[-1] = 0.0, zmax
+    zc = np.zeros(nz + 2)
+    zc[1:] = zf
+    zc[0] = -zc[2]
+    zm = np.zeros(nz + 2)
+    zm[1 : nz + 1] = 0.5 * (zf[1:] + zf[:-1])
+    zm[0] = -zm[1]
+    zm[nz + 1] = 2.0 * zmax - zm[nz]
+    dz = zmax / nz
+    dmz = np.zeros(nz + 2)
+    dmz[1 : nz + 1] = (zf[1:] - zf[:-1]) / dz
+    dmz[0], dmz[nz + 1] = dmz[1], dmz[nz]
+    dcz = np.zeros(nz + 2)
+    dcz[1:] = (zm[1:] - zm[:-1]) / dz
+    dcz[0] = dcz[1]
+    return zm, zc, dmz, dcz
 
 
 def sinh_radial_mesh(nr, rmax, s):

@@ -1,6 +1,7 @@
 // vrh_api.cpp -- the C ABI of libvrh (declared in include/vrh.h): handle
 // lifecycle, device memory, transfers and the substep driver.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <cmath>
 #include <cstdio>
@@ -381,6 +382,9 @@ int vrh_destroy(vrh_t *h) {
                   f.q3, f.dq1, f.dq2, f.dq3, f.pres, f.dph, f.ru1, f.ru2, f.ru3, f.r1, f.r2, f.r3, f.prg1, f.prg2,
                   f.prg3, f.S, f.ib, f.S64, f.ib64, f.csort, f.dq3no, f.corr3n, f.flag};
   for (void *p : ptrs) if (p) hipFree(p);
+  void *zp[] = {h->d_zmet, h->d_zv, h->d_zvf, h->S2, h->S2_64};
+  for (void *p : zp) if (p) hipFree(p);
+  if (h->blas) rocblas_destroy_handle((rocblas_handle)h->blas);
   for (auto &e : h->kt_pool) hipEventDestroy(e);
   graph_invalidate(h);
   if (h->d_sc) hipFree(h->d_sc);
@@ -499,6 +503,79 @@ int vrh_set_radial_mesh(vrh_t *h, const double *rm, const double *rc, const doub
   return 0;
 }
 
+// Axial mesh: metric arrays, line coefficients, and the pressure solve's z
+// transforms (generalised eigenvectors of the z operator, host-computed).
+int vrh_set_axial_mesh(vrh_t *h, const double *zm, const double *zc, const double *dmz, const double *dcz) {
+  if (!h) return -1;
+  if (!zm || !zc || !dmz || !dcz) return fail(h, "axial mesh: null array");
+  if (!h->pb.empty()) return fail(h, "axial mesh: particles need the uniform mesh");
+  const int nz = h->nz;
+  bool ok = zc[1] == 0.0 && nz >= 2;
+  for (int k = 1; k <= nz; k++)
+    ok = ok && zc[k] < zm[k] && zm[k] < zc[k + 1] && dmz[k] > 0.0 && dcz[k] > 0.0;
+  ok = ok && dmz[0] > 0.0 && dcz[0] > 0.0 && dmz[nz + 1] > 0.0 && dcz[nz + 1] > 0.0 && zm[nz + 1] > zm[nz] &&
+       zm[0] < zm[1];
+  for (int k = 0; k <= nz + 1; k++) ok = ok && std::isfinite(zm[k]) && std::isfinite(zc[k]);
+  if (!ok)
+    return fail(h, "axial mesh: need nz >= 2, zc(1) = 0, zc(k) < zm(k) < zc(k+1), zm(0) < zm(1), zm(nz+1) > zm(nz), "
+                   "dmz, dcz > 0 (k = 0..nz+1)");
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  const double *src[4] = {zm, zc, dmz, dcz};
+  for (int q = 0; q < 4; q++) {
+    h->zmesh[q].assign(src[q], src[q] + nz + 2);
+    h->zmesh[q].push_back(h->zmesh[q][nz + 1]);  // k = nz+2 repeats nz+1
+  }
+  // metric arrays (k = 0..nz+2)
+  const int L = nz + 3;
+  std::vector<double> mb((size_t)7 * L, 0.0);
+  double *m3 = &mb[0], *c3 = &mb[L], *cm = &mb[2 * L], *cp = &mb[3 * L], *fm = &mb[4 * L], *fp = &mb[5 * L],
+         *zp = &mb[6 * L];
+  const double dx3 = h->g.dx3;
+  for (int k = 0; k < L; k++) { m3[k] = dx3 / h->zmesh[2][k]; c3[k] = dx3 / h->zmesh[3][k]; zp[k] = h->zmesh[0][k]; }
+  for (int k = 0; k < L; k++) {
+    cm[k] = m3[k] * c3[k];
+    cp[k] = m3[k] * c3[k + 1 < L ? k + 1 : k];
+    fm[k] = c3[k] * m3[k > 0 ? k - 1 : 0];
+    fp[k] = c3[k] * m3[k];
+  }
+  const double h1 = 0.5 / m3[1], h2 = 1.0 / c3[2];
+  std::vector<double> vi, vf;
+  std::string emsg;
+  if (host_axial_transforms(m3, c3, nz, vi, vf, h->zlam, emsg)) return fail(h, ("axial mesh: " + emsg).c_str());
+  int rc;
+  if (!h->d_zmet && (rc = dalloc(h, &h->d_zmet, mb.size()))) return rc;
+  if ((rc = h2d(h, h->d_zmet, mb.data(), mb.size() * sizeof(double)))) return rc;
+  h->zmet.m3 = h->d_zmet; h->zmet.c3 = h->d_zmet + L; h->zmet.cm = h->d_zmet + 2 * L; h->zmet.cp = h->d_zmet + 3 * L;
+  h->zmet.fm = h->d_zmet + 4 * L; h->zmet.fp = h->d_zmet + 5 * L; h->zmet.zpos = h->d_zmet + 6 * L;
+  h->zmet.ws = 2.0 / (h1 * (h1 + h2));
+  h->zmet.w2 = 2.0 / (h2 * (h1 + h2));
+  const size_t nn = (size_t)nz * nz;
+  std::vector<double> zv(2 * nn);
+  std::memcpy(zv.data(), vi.data(), nn * sizeof(double));
+  std::memcpy(zv.data() + nn, vf.data(), nn * sizeof(double));
+  if (!h->d_zv && (rc = dalloc(h, &h->d_zv, 2 * nn))) return rc;
+  if ((rc = h2d(h, h->d_zv, zv.data(), zv.size() * sizeof(double)))) return rc;
+  if (h->prec == VRH_MIXED) {
+    std::vector<float> zf(2 * nn);
+    for (size_t i = 0; i < 2 * nn; i++) zf[i] = (float)zv[i];
+    if (!h->d_zvf && (rc = dalloc(h, &h->d_zvf, 2 * nn))) return rc;
+    if ((rc = h2d(h, h->d_zvf, zf.data(), zf.size() * sizeof(float)))) return rc;
+    if (!h->S2_64 && (rc = dalloc(h, &h->S2_64, (size_t)5 * h->nr * nz))) return rc;
+  }
+  if (!h->S2 && (rc = dalloc_bytes(h, &h->S2, compact(h) * wsize(h)))) return rc;
+  if (!h->blas) {
+    rocblas_handle b = nullptr;
+    if (rocblas_create_handle(&b) != rocblas_status_success) return fail(h, "axial mesh: rocblas_create_handle failed", -3);
+    h->blas = b;
+    if (rocblas_set_stream(b, h->stream) != rocblas_status_success)
+      return fail(h, "axial mesh: rocblas_set_stream failed", -3);
+  }
+  if ((rc = refresh_mesh(h))) return rc;
+  graph_invalidate(h);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
 // One substep of the step that has reached the current time, with the
 // step's scalars computed on the device as vrh_step computes them, so that
 // three calls after vrh_set_time(t + dt) equal one vrh_step from t bit for
@@ -534,7 +611,8 @@ int step_enqueue(vrh_t *h, int nsteps) {
     HIPCHK(h, hipMemcpyAsync(h->d_sc, h->h_pin, sizeof(double), hipMemcpyHostToDevice, h->stream));
     h->time_dirty = false;
   }
-  const bool graph = h->use_graph && !h->kt_on;
+  // an axial mesh's z transforms are library GEMMs: launched eagerly
+  const bool graph = h->use_graph && !h->kt_on && h->zmesh[0].empty();
   if (graph && !h->graph_ok) { int rc = build_graph(h); if (rc) return rc; }
   for (int s = 0; s < nsteps; s++) {
     h->ntime = h->dt + h->ntime;  // loop.F90:53 (k_scalars does the same on the device)
@@ -921,6 +999,8 @@ int vrh_ensemble_step(vrh_ens_t *e, int nsteps) {
   for (size_t q = 0; q < e->m.size(); q++) {
     vrh_t *h = e->m[q];
     if (h->kt_on) return efail(e, "member " + std::to_string(q) + " has the kernel timer enabled");
+    if (!h->zmesh[0].empty())
+      return efail(e, "member " + std::to_string(q) + " has an axial mesh (stepped eagerly, not in a graph)");
     ECHK(e, hipStreamSynchronize(h->stream));
     if (h->epoch != e->epochs[q]) stale = true;
   }

@@ -1,9 +1,12 @@
 // vrh_host.cpp -- host-side setup math of libvrh: mesh vectors, tridiagonal
 // coefficients, Poisson radial pivots, inlet profile, and the initial-condition
 // and time-step helpers of the C ABI.
+#include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "vrh_internal.h"
@@ -144,6 +147,37 @@ void host_line_coefs(vrh_t *h, int irk, std::vector<double> out[6][7], const int
     line_coef(a2, b2, c2, M[2], out[SW_R2]);
   }
   // z sweeps (momentum_coef.F90:314-465, if_grad_null = 0)
+  if (!h->zmesh[0].empty()) {
+    // axial mesh: centre rows (q1, q2) and face rows (q3) from the second-
+    // difference weights; the inlet row of q1, q2 is the one-sided second
+    // difference through the wall value, f'' = ws fs - (ws + w2) f1 + w2 f2
+    // (momentum_coef.F90's (-8/3, 4, -4/3) row on the uniform mesh); the end
+    // rows keep the uniform forms
+    const double bz = 0.5 * alm * h->dt / h->Re, dx3 = g.dx3;
+    const std::vector<double> &dmz = h->zmesh[2], &dcz = h->zmesh[3];
+    auto zm3 = [&](int k) { return dx3 / dmz[k]; };
+    auto zc3 = [&](int k) { return dx3 / dcz[k]; };
+    const double h1 = 0.5 / zm3(1), h2 = 1.0 / zc3(2);
+    const double ws = 2.0 / (h1 * (h1 + h2)), w2 = 2.0 / (h2 * (h1 + h2));
+    for (int comp = 0; comp < 3; comp++) {
+      std::vector<double> a(nzc), b(nzc), c(nzc);
+      for (int k = 1; k <= nz; k++) {
+        const double wm = comp == 1 ? zc3(k) * zm3(k - 1) : zm3(k) * zc3(k);
+        const double wp = comp == 1 ? zc3(k) * zm3(k) : zm3(k) * zc3(k + 1);
+        a[k - 1] = -(bz * wm); b[k - 1] = 1.0 + bz * (wm + wp); c[k - 1] = -(bz * wp);
+      }
+      if (comp == 1) {
+        a[nzc - 1] = 0.0; b[nzc - 1] = 1.0; c[nzc - 1] = 0.0;
+        a[0] = 0.0; b[0] = 1.0; c[0] = 0.0;
+      } else {
+        a[nzc - 1] = 1.0; b[nzc - 1] = 1.0; c[nzc - 1] = 0.0;
+        a[0] = -(bz * ws); b[0] = 1.0 + bz * (ws + w2); c[0] = -(bz * w2);
+      }
+      int sw = comp == 0 ? SW_Z1 : comp == 1 ? SW_Z3 : SW_Z2;
+      line_coef(a, b, c, M[sw], out[sw]);
+    }
+    return;
+  }
   betadx = 0.5 * g.dx3q * alm * h->dt / h->Re;
   for (int comp = 0; comp < 3; comp++) {
     std::vector<double> a(nzc), b(nzc), c(nzc);
@@ -193,7 +227,8 @@ void host_radial_pivots(vrh_t *h, std::vector<double> &ib, std::vector<double> &
   ib64.assign((size_t)5 * nr * nz, 0.0);
   for (int kz = 1; kz <= nz; kz++) {
     int mz = kz - 1;
-    double wz = 2.0 / (dz * dz) * (1.0 - cos(mz * pi / nz));
+    // axial mesh: the eigenvalue of the z transform's mode mz
+    double wz = h->zlam.empty() || h->zmesh[0].empty() ? 2.0 / (dz * dz) * (1.0 - cos(mz * pi / nz)) : h->zlam[mz];
     for (int pos = 0; pos < nt; pos++) {
       int m = (pos + 1) / 2;
       double wth = (h->theta_eigen == VRH_THETA_EXACT_M2)
@@ -217,6 +252,157 @@ void host_radial_pivots(vrh_t *h, std::vector<double> &ib, std::vector<double> &
       }
     }
   }
+}
+
+// Pressure-solve z transforms on an axial mesh.  The z part of minus the
+// Laplacian, -(D_z G_z) with Neumann ends (the inlet and outlet faces carry
+// no pressure gradient), is M^-1 K with M = diag(1/m3) and K the symmetric
+// face-coupling matrix (K(k,k) = c3(k) + c3(k+1) without the end faces,
+// K(k,k+1) = -c3(k+1)).  With D = diag(sqrt(m3)) it is similar to the
+// symmetric tridiagonal A = D K D: A W = W diag(lam), V = D W, V^-1 = W^T D^-1.
+// Eigenvalues by Sturm-sequence bisection (Wilkinson's method); eigenvectors
+// by inverse iteration on A - lam I with a partially pivoted tridiagonal LU,
+// each orthogonalised against all earlier ones.  The constant mode is exact:
+// lam[0] = 0 with w0 ~ 1/sqrt(m3).  On the uniform mesh lam[q] =
+// (2/dz^2)(1 - cos(pi q/nz)) and V holds the DCT-II cosines.
+int host_axial_transforms(const double *m3, const double *c3, int nz, std::vector<double> &vi,
+                          std::vector<double> &vf, std::vector<double> &lam, std::string &err) {
+  const int n = nz;
+  std::vector<double> a(n), e(n > 1 ? n - 1 : 0), sq(n);
+  for (int i = 0; i < n; i++) {
+    const int k = i + 1;
+    a[i] = m3[k] * ((k > 1 ? c3[k] : 0.0) + (k < nz ? c3[k + 1] : 0.0));
+    sq[i] = std::sqrt(m3[k]);
+  }
+  for (int i = 0; i + 1 < n; i++) e[i] = -std::sqrt(m3[i + 1] * m3[i + 2]) * c3[i + 2];
+  double lo = 0.0, hi = 0.0, anorm = 0.0;
+  for (int i = 0; i < n; i++) {
+    const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(e[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+    anorm = std::max(anorm, std::fabs(a[i]) + r);
+  }
+  const double eps = 2.220446049250313e-16, tiny = eps * anorm;
+  // number of eigenvalues below x (Sturm count of the LDL^T pivots)
+  auto below = [&](double x) {
+    int cnt = 0;
+    double d = a[0] - x;
+    if (d == 0.0) d = -tiny;
+    if (d < 0.0) cnt++;
+    for (int i = 1; i < n; i++) {
+      d = (a[i] - x) - e[i - 1] * e[i - 1] / d;
+      if (d == 0.0) d = -tiny;
+      if (d < 0.0) cnt++;
+    }
+    return cnt;
+  };
+  lam.assign(n, 0.0);
+  for (int q = 1; q < n; q++) {
+    double l = lo, h = hi;
+    for (int it = 0; it < 200 && h - l > 2.0 * eps * std::max(std::fabs(l), std::fabs(h)) + tiny * 1e-3; it++) {
+      const double mid = 0.5 * (l + h);
+      if (below(mid) > q) h = mid; else l = mid;
+    }
+    lam[q] = 0.5 * (l + h);
+  }
+  std::vector<double> W((size_t)n * n), v(n), dl(n), dd(n), du(n), du2(n);
+  std::vector<char> piv(n);
+  {  // constant mode
+    double nrm = 0.0;
+    for (int i = 0; i < n; i++) { v[i] = 1.0 / sq[i]; nrm += v[i] * v[i]; }
+    nrm = std::sqrt(nrm);
+    for (int i = 0; i < n; i++) W[i] = v[i] / nrm;
+  }
+  auto orth = [&](int q) {  // v minus its projection on W rows 0..q-1 (twice), normalised
+    for (int pass = 0; pass < 2; pass++)
+      for (int p = 0; p < q; p++) {
+        const double *w = &W[(size_t)p * n];
+        double s2 = 0.0;
+        for (int i = 0; i < n; i++) s2 += w[i] * v[i];
+        for (int i = 0; i < n; i++) v[i] -= s2 * w[i];
+      }
+    double nrm = 0.0;
+    for (int i = 0; i < n; i++) nrm += v[i] * v[i];
+    nrm = std::sqrt(nrm);
+    for (int i = 0; i < n; i++) v[i] /= nrm;
+    return nrm;
+  };
+  for (int q = 1; q < n; q++) {
+    // LU with partial pivoting of A - lam I (rows i, i+1 swapped where the
+    // subdiagonal dominates; du2 the fill of the second superdiagonal)
+    for (int i = 0; i < n; i++) {
+      dd[i] = a[i] - lam[q];
+      du[i] = i + 1 < n ? e[i] : 0.0;
+      dl[i] = i + 1 < n ? e[i] : 0.0;
+      du2[i] = 0.0;
+      piv[i] = 0;
+    }
+    for (int i = 0; i + 1 < n; i++) {
+      if (std::fabs(dd[i]) >= std::fabs(dl[i])) {
+        if (dd[i] == 0.0) dd[i] = tiny;
+        const double f = dl[i] / dd[i];
+        dl[i] = f;
+        dd[i + 1] -= f * du[i];
+      } else {
+        const double f = dd[i] / dl[i];
+        dd[i] = dl[i];
+        dl[i] = f;
+        const double t = du[i];
+        du[i] = dd[i + 1];
+        dd[i + 1] = t - f * dd[i + 1];
+        if (i + 2 < n) { du2[i] = du[i + 1]; du[i + 1] = -f * du[i + 1]; }
+        piv[i] = 1;
+      }
+    }
+    if (dd[n - 1] == 0.0) dd[n - 1] = tiny;
+    for (int i = 0; i < n; i++) v[i] = 1.0 + 0.5 * std::sin(0.7 + 1.3 * (double)i * (q + 1));
+    orth(q);
+    for (int it = 0; it < 3; it++) {
+      for (int i = 0; i + 1 < n; i++) {
+        if (piv[i]) std::swap(v[i], v[i + 1]);
+        v[i + 1] -= dl[i] * v[i];
+      }
+      v[n - 1] /= dd[n - 1];
+      if (n > 1) v[n - 2] = (v[n - 2] - du[n - 2] * v[n - 1]) / dd[n - 2];
+      for (int i = n - 3; i >= 0; i--) v[i] = (v[i] - du[i] * v[i + 1] - du2[i] * v[i + 2]) / dd[i];
+      orth(q);
+    }
+    std::memcpy(&W[(size_t)q * n], v.data(), n * sizeof(double));
+  }
+  // residual max |A w - lam w| and orthogonality of neighbours and of the
+  // constant mode, relative to |A|
+  double res = 0.0, dot = 0.0;
+  for (int q = 0; q < n; q++) {
+    const double *w = &W[(size_t)q * n];
+    for (int i = 0; i < n; i++) {
+      double aw = a[i] * w[i] - lam[q] * w[i];
+      if (i > 0) aw += e[i - 1] * w[i - 1];
+      if (i + 1 < n) aw += e[i] * w[i + 1];
+      res = std::max(res, std::fabs(aw));
+    }
+    for (int p : {0, q - 1}) {
+      if (p < 0 || p == q) continue;
+      double s2 = 0.0;
+      for (int i = 0; i < n; i++) s2 += w[i] * W[(size_t)p * n + i];
+      dot = std::max(dot, std::fabs(s2));
+    }
+  }
+  if (!(res <= 1e-10 * anorm) || !(dot <= 1e-10)) {
+    char b[160];
+    snprintf(b, sizeof b, "z eigen-decomposition failed its check (residual %.3e of |A| %.3e, overlap %.3e)", res,
+             anorm, dot);
+    err = b;
+    return -1;
+  }
+  vi.assign((size_t)n * n, 0.0);
+  vf.assign((size_t)n * n, 0.0);
+  for (int q = 0; q < n; q++)
+    for (int i = 0; i < n; i++) {
+      const double w = W[(size_t)q * n + i];
+      vi[(size_t)q * n + i] = w / sq[i];  // V^-1 = W^T D^-1
+      vf[(size_t)i * n + q] = sq[i] * w;  // V = D W
+    }
+  return 0;
 }
 
 // Inlet time profile (gen_tools_swirl.F90:159-175); 0.1 is a default real.

@@ -34,6 +34,18 @@ constexpr int kNumVec = 23;  // vectors of VCoef, in that order
 struct VRK {
   double g, r, a;  // gamma, rho, alpha of the substep (para_tools.F90:103-113)
 };
+// Axial mesh metric (vrh_set_axial_mesh), each array nz+3 long and indexed by
+// the Fortran index k = 0..nz+2: m3 = dx3/dmz and c3 = dx3/dcz (the z
+// differences at centres and faces); the second-difference weights of a
+// centred variable cm = m3(k) c3(k), cp = m3(k) c3(k+1) and of a face
+// variable fm = c3(k) m3(k-1), fp = c3(k) m3(k); zpos the cell centres.
+// ws, w2: the one-sided inlet row of a centred variable (wall value at
+// distance h1 = dz(1)/2, next centre at h2).  m3 == nullptr: the uniform
+// mesh of mesh_tools.F90:325-346.
+struct ZMet {
+  const double *m3, *c3, *cm, *cp, *fm, *fp, *zpos;
+  double ws, w2;
+};
 
 // Line-independent coefficients of one tridiagonal sweep for the partitioned
 // solver: forward g_p = A_p r_p + B_p g_{p-1}; backward x_p = g_p - C_p x_{p+1};
@@ -126,6 +138,20 @@ struct vrh_s {
   // radial mesh set by vrh_set_radial_mesh (rm, rc, dmr, dcr over j = 0..nr+1);
   // empty: the uniform mesh with dmr = dcr = 1
   std::vector<double> mesh[4];
+  // axial mesh set by vrh_set_axial_mesh (zm, zc, dmz, dcz over k = 0..nz+2);
+  // empty: the uniform mesh.  zlam: eigenvalues of the pressure solve's z
+  // operator (ascending, zlam[0] = 0); d_zmet: the ZMet arrays; d_zv: the z
+  // transforms V^-1 then V (nz x nz each, double), d_zvf: their float copies
+  // (mixed precision); S2: the spectral work array's twin (out-of-place
+  // transforms), S2_64: the same for the FP64 low modes
+  std::vector<double> zmesh[4];
+  std::vector<double> zlam;
+  ZMet zmet{};
+  double *d_zmet = nullptr, *d_zv = nullptr;
+  float *d_zvf = nullptr;
+  void *S2 = nullptr;
+  double *S2_64 = nullptr;
+  void *blas = nullptr;  // rocBLAS handle of the z transforms (on `stream`)
   VCoef coef;
   // line coefficients: [irk 1..3][sweep 0..5]
   double *d_line;
@@ -192,6 +218,12 @@ void host_line_coefs(vrh_t *h, int irk, std::vector<double> out[6][7], const int
 int build_line_coefs(vrh_t *h);
 void host_radial_row(const vrh_t *h, int j, double &aa, double &cc, double &dd);
 void host_radial_pivots(vrh_t *h, std::vector<double> &ib, std::vector<double> &ib64);
+// Pressure-solve z transforms on an axial mesh with z differences m3 (centres)
+// and c3 (faces), k = 0..nz+2: vi = V^-1, vf = V (row-major nz x nz) and lam
+// (ascending, lam[0] = 0) with (-D_z G_z) V = V diag(lam).  Returns nonzero
+// (and err) if the decomposition fails its residual or orthogonality check.
+int host_axial_transforms(const double *m3, const double *c3, int nz, std::vector<double> &vi,
+                          std::vector<double> &vf, std::vector<double> &lam, std::string &err);
 double host_profile(double t);
 void host_mint(vrh_t *h, double *mint);
 

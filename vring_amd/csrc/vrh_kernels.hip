@@ -6,6 +6,7 @@
 // the FP64 path, float in the mixed path); the state q, the predictor dq,
 // phi and p are always double.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <cmath>
 #include <mutex>
@@ -18,9 +19,10 @@ template <typename T> __device__ __forceinline__ T ldg(const T *p) { return __bu
 
 // ---------------------------------------------------------------------------
 // hdnbc: outlet convective condition and mass correction, one thread per
-// (theta, r) outlet point (momentum_terms_swirl.F90:225-291).
+// (theta, r) outlet point (momentum_terms_swirl.F90:225-291).  zm3n: the
+// outlet cell's z difference dx3/dmz(nz) on an axial mesh, 0 on the uniform one.
 // ---------------------------------------------------------------------------
-__global__ void k_hdnbc(VGrid g, VCoef c, double gam, double rom, double cor, const double *__restrict__ cor_dev,
+__global__ void k_hdnbc(VGrid g, VCoef c, double zm3n, double gam, double rom, double cor, const double *__restrict__ cor_dev,
                         const double *__restrict__ q3, double *__restrict__ csort, double *__restrict__ dq3no,
                         double *__restrict__ corr3n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
@@ -32,7 +34,7 @@ __global__ void k_hdnbc(VGrid g, VCoef c, double gam, double rom, double cor, co
   double cs = csort[p];
   if (j <= g.nr && qn > 0.0) cs = (double)0.6f * qn;
   csort[p] = cs;
-  double tconv = -(cs * g.dx3) * (qn - qm);
+  double tconv = -(cs * (zm3n > 0.0 ? zm3n : g.dx3)) * (qn - qm);
   double c3n = (gam * tconv + rom * dq3no[p]) * g.dt;
   dq3no[p] = tconv;
   corr3n[p] = c3n + c.qc[j] * cor;
@@ -85,8 +87,20 @@ __global__ void k_scalars(double *sc, double dt, double a1, double a2, double a3
 constexpr int kMaxPcrLevels = 40;
 constexpr int kRowCoefs = 24;
 
+// z second differences on the axial mesh (ZMet, vrh_set_axial_mesh): at a
+// cell centre (q1, q2, p, the first two pressure-gradient components) with
+// the one-sided form through the wall value 0 in the inlet cell, and at a
+// face (q3, the third component)
+template <typename T> __device__ __forceinline__ T zd2c(const ZMet &z, int k, T fm1, T f0, T fp1) {
+  if (k == 1) return (fp1 - f0) * (T)z.w2 - (f0 - (T)0) * (T)z.ws;
+  return (fp1 - f0) * (T)z.cp[k] - (f0 - fm1) * (T)z.cm[k];
+}
+template <typename T> __device__ __forceinline__ T zd2f(const ZMet &z, int k, T fm1, T f0, T fp1) {
+  return (fp1 - f0) * (T)z.fp[k] - (f0 - fm1) * (T)z.fm[k];
+}
+
 template <typename TW, typename TC>
-__global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march(VGrid g, VCoef c, VRK rk, double betadx, int nlev, int kchunk,
+__global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march(VGrid g, VCoef c, VRK rk, ZMet zm, double betadx, int nlev, int kchunk,
                                                   const double *__restrict__ q1, const double *__restrict__ q2,
                                                   const double *__restrict__ q3, const double *__restrict__ prg1,
                                                   const double *__restrict__ prg2, const double *__restrict__ prg3,
@@ -227,6 +241,8 @@ __global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march
     }
     TC v1 = (TC)0, v2 = (TC)0, v3 = (TC)0;
     const long o = (long)ti + (long)j * sj + (long)k * sk;
+    const bool zs = zm.m3 != nullptr;  // axial mesh: per-plane metric
+    const TC zmk = zs ? (TC)zm.m3[k] : dx3, zck = zs ? (TC)zm.c3[k] : dx3;
     if (cell) {
       const TC *A1m = ring + (sM * 3 + 0) * PL, *A1 = ring + (s0 * 3 + 0) * PL, *A1p = ring + (sP * 3 + 0) * PL;
       const TC *A2m = ring + (sM * 3 + 1) * PL, *A2 = ring + (s0 * 3 + 1) * PL, *A2p = ring + (sP * 3 + 1) * PL;
@@ -254,13 +270,14 @@ __global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march
         TC h12n = (qnn + qss) * h1d22n;
         qnn = (TC)0.25 * (q3kp + q3imkp) * (q1kp + q1c);
         qss = (k == 1) ? (TC)0 : (TC)0.25 * (q3c + q3im) * (q1c + q1km);
-        TC h13 = (qnn - qss) * dx3;
+        TC h13 = (qnn - qss) * zmk;
         qnn = q2n + q2c;
         qss = q2imn + q2im;
         TC d1q2 = (qnn - qss) * h1dare;
         TC H1 = d1q2 - (h11 + (h12d + h12n) + h13);
         TC d11 = (q1ip - (TC)2 * q1c + q1im) * s1t11;
-        TC d33 = (k == 1) ? (TC)(4.0 / 3.0) * (q1kp - (TC)3 * q1c) * dx3q : (q1kp - (TC)2 * q1c + q1km) * dx3q;
+        TC d33 = zs ? zd2c<TC>(zm, k, q1km, q1c, q1kp)
+                    : (k == 1) ? (TC)(4.0 / 3.0) * (q1kp - (TC)3 * q1c) * dx3q : (q1kp - (TC)2 * q1c + q1km) * dx3q;
         TC qn2 = (q1n * irmn - q1c * irm) * s1t2n;
         TC qs2 = (q1c * irm - q1jm * irmm) * s1t2s;
         TC d22 = (qn2 - qs2) * s1t2c;
@@ -280,13 +297,14 @@ __global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march
         TC h11n = (TC)0.25 * (qnn + qss) * (qnn + qss);
         qnn = (TC)0.25 * (q3kp + q3skp) * (q2kp + q2c);
         qss = (k == 1) ? (TC)0 : (TC)0.25 * (q3c + q3s) * (q2c + q2km);
-        TC h23 = (qnn - qss) * dx3;
+        TC h23 = (qnn - qss) * zmk;
         qnn = q1ip + q1ips;
         qss = q1c + q1s;
         TC d1q1 = -((qnn - qss) * h2dare);
         TC H2 = d1q1 - (h21 + h22 + h23 - h11n);
         TC d11 = (q2ip - (TC)2 * q2c + q2im) * s2t11;
-        TC d33 = (k == 1) ? (TC)(4.0 / 3.0) * (q2kp - (TC)3 * q2c) * dx3q : (q2kp - (TC)2 * q2c + q2km) * dx3q;
+        TC d33 = zs ? zd2c<TC>(zm, k, q2km, q2c, q2kp)
+                    : (k == 1) ? (TC)(4.0 / 3.0) * (q2kp - (TC)3 * q2c) * dx3q : (q2kp - (TC)2 * q2c + q2km) * dx3q;
         TC d22 = q2n * s2t2p - q2c * s2t2c + q2s * s2t2m;
         v2 = (ga * H2 + ro * c_ru2 - al * c_pg2 + aRe * (d11 + d33 + d22)) * dt;
         ru2[o] = H2;
@@ -301,10 +319,10 @@ __global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march
         TC h31 = (qnn - qss) * h1d11;
         qnn = (TC)0.25 * (q3c + q3kp) * (q3c + q3kp);
         qss = (TC)0.25 * (q3c + q3km) * (q3c + q3km);
-        TC h33 = (qnn - qss) * dx3;
+        TC h33 = (qnn - qss) * zck;
         TC H3 = -(h31 + h32 + h33);
         TC d11 = (q3ip - (TC)2 * q3c + q3im) * s1t11;
-        TC d33 = (q3kp - (TC)2 * q3c + q3km) * dx3q;
+        TC d33 = zs ? zd2f<TC>(zm, k, q3km, q3c, q3kp) : (q3kp - (TC)2 * q3c + q3km) * dx3q;
         TC qn2 = (q3n - q3c) * s3t2n;
         TC qs2 = (q3c - q3jm) * s3t2s;
         TC d22 = qn2 - qs2;
@@ -376,7 +394,7 @@ __global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march
 // block barrier per level), which leaves two block barriers per z plane for
 // the plane ring.  Same arithmetic as k_explicit_march.
 template <typename TW, typename TC, int CPL, int RJT, int WPE>
-__global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef c, VRK rk, double betadx, int nlev, int kchunk,
+__global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef c, VRK rk, ZMet zm, double betadx, int nlev, int kchunk,
                                                   const double *__restrict__ q1, const double *__restrict__ q2,
                                                   const double *__restrict__ q3, const double *__restrict__ prg1,
                                                   const double *__restrict__ prg2, const double *__restrict__ prg3,
@@ -525,6 +543,8 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
       const TC c_ru1 = cru[cc][0], c_ru2 = cru[cc][1], c_ru3 = cru[cc][2];
       const TC c_pg1 = cpg[cc][0], c_pg2 = cpg[cc][1], c_pg3 = cpg[cc][2];
       TC v1 = (TC)0, v2 = (TC)0, v3 = (TC)0;
+      const bool zs = zm.m3 != nullptr;  // axial mesh: per-plane metric
+      const TC zmk = zs ? (TC)zm.m3[k] : dx3, zck = zs ? (TC)zm.c3[k] : dx3;
       const long o = (long)ti + (long)j * sj + (long)k * sk;
       if (cell) {
       const TC *A1m = ring + (sM * 3 + 0) * PL, *A1 = ring + (s0 * 3 + 0) * PL, *A1p = ring + (sP * 3 + 0) * PL;
@@ -553,13 +573,14 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
         TC h12n = (qnn + qss) * h1d22n;
         qnn = (TC)0.25 * (q3kp + q3imkp) * (q1kp + q1c);
         qss = (k == 1) ? (TC)0 : (TC)0.25 * (q3c + q3im) * (q1c + q1km);
-        TC h13 = (qnn - qss) * dx3;
+        TC h13 = (qnn - qss) * zmk;
         qnn = q2n + q2c;
         qss = q2imn + q2im;
         TC d1q2 = (qnn - qss) * h1dare;
         TC H1 = d1q2 - (h11 + (h12d + h12n) + h13);
         TC d11 = (q1ip - (TC)2 * q1c + q1im) * s1t11;
-        TC d33 = (k == 1) ? (TC)(4.0 / 3.0) * (q1kp - (TC)3 * q1c) * dx3q : (q1kp - (TC)2 * q1c + q1km) * dx3q;
+        TC d33 = zs ? zd2c<TC>(zm, k, q1km, q1c, q1kp)
+                    : (k == 1) ? (TC)(4.0 / 3.0) * (q1kp - (TC)3 * q1c) * dx3q : (q1kp - (TC)2 * q1c + q1km) * dx3q;
         TC qn2 = (q1n * irmn - q1c * irm) * s1t2n;
         TC qs2 = (q1c * irm - q1jm * irmm) * s1t2s;
         TC d22 = (qn2 - qs2) * s1t2c;
@@ -579,13 +600,14 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
         TC h11n = (TC)0.25 * (qnn + qss) * (qnn + qss);
         qnn = (TC)0.25 * (q3kp + q3skp) * (q2kp + q2c);
         qss = (k == 1) ? (TC)0 : (TC)0.25 * (q3c + q3s) * (q2c + q2km);
-        TC h23 = (qnn - qss) * dx3;
+        TC h23 = (qnn - qss) * zmk;
         qnn = q1ip + q1ips;
         qss = q1c + q1s;
         TC d1q1 = -((qnn - qss) * h2dare);
         TC H2 = d1q1 - (h21 + h22 + h23 - h11n);
         TC d11 = (q2ip - (TC)2 * q2c + q2im) * s2t11;
-        TC d33 = (k == 1) ? (TC)(4.0 / 3.0) * (q2kp - (TC)3 * q2c) * dx3q : (q2kp - (TC)2 * q2c + q2km) * dx3q;
+        TC d33 = zs ? zd2c<TC>(zm, k, q2km, q2c, q2kp)
+                    : (k == 1) ? (TC)(4.0 / 3.0) * (q2kp - (TC)3 * q2c) * dx3q : (q2kp - (TC)2 * q2c + q2km) * dx3q;
         TC d22 = q2n * s2t2p - q2c * s2t2c + q2s * s2t2m;
         v2 = (ga * H2 + ro * c_ru2 - al * c_pg2 + aRe * (d11 + d33 + d22)) * dt;
         ru2[o] = H2;
@@ -600,10 +622,10 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
         TC h31 = (qnn - qss) * h1d11;
         qnn = (TC)0.25 * (q3c + q3kp) * (q3c + q3kp);
         qss = (TC)0.25 * (q3c + q3km) * (q3c + q3km);
-        TC h33 = (qnn - qss) * dx3;
+        TC h33 = (qnn - qss) * zck;
         TC H3 = -(h31 + h32 + h33);
         TC d11 = (q3ip - (TC)2 * q3c + q3im) * s1t11;
-        TC d33 = (q3kp - (TC)2 * q3c + q3km) * dx3q;
+        TC d33 = zs ? zd2f<TC>(zm, k, q3km, q3c, q3kp) : (q3kp - (TC)2 * q3c + q3km) * dx3q;
         TC qn2 = (q3n - q3c) * s3t2n;
         TC qs2 = (q3c - q3jm) * s3t2s;
         TC d22 = qn2 - qs2;
@@ -687,7 +709,7 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
 }
 
 template <typename TW, typename TC>
-__global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk, double betadx, int nlev,
+__global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk, ZMet zm, double betadx, int nlev,
                                                    const double *__restrict__ q1, const double *__restrict__ q2,
                                                    const double *__restrict__ q3, const double *__restrict__ prg1,
                                                    const double *__restrict__ prg2, const double *__restrict__ prg3,
@@ -708,6 +730,8 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
   const long dip = ip - i, dim = im - i, sj = g.sj, sk = g.sk, djm = (long)(jm - j) * sj;
   const TC ga = (TC)rk.g, ro = (TC)rk.r, al = (TC)rk.a, dt = (TC)g.dt, aRe = (TC)(rk.a / g.Re);
   const TC dx3 = (TC)g.dx3, dx3q = (TC)g.dx3q;
+  const bool zs = zm.m3 != nullptr;  // axial mesh: per-plane metric
+  const TC zmk = zs ? (TC)zm.m3[k] : dx3, zck = zs ? (TC)zm.c3[k] : dx3;
   const long pc = cidx(g, i, j, k);
 #define Q1(d) ((TC)q1[o + (d)])
 #define Q2(d) ((TC)q2[o + (d)])
@@ -724,14 +748,15 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
     TC h12n = (qnn + qss) * (TC)c.h1d22n[j];
     qnn = (TC)0.25 * (Q3(sk) + Q3(dim + sk)) * (Q1(sk) + Q1(0));
     qss = (k == 1) ? (TC)0 : (TC)0.25 * (Q3(0) + Q3(dim)) * (Q1(0) + Q1(-sk));
-    TC h13 = (qnn - qss) * dx3;
+    TC h13 = (qnn - qss) * zmk;
     qnn = Q2(sj) + Q2(0);
     qss = Q2(dim + sj) + Q2(dim);
     TC d1q2 = (qnn - qss) * (TC)c.h1dare[j];
     TC H1 = d1q2 - (h11 + (h12d + h12n) + h13);
     TC d11 = (Q1(dip) - (TC)2 * Q1(0) + Q1(dim)) * (TC)c.s1t11[j];
-    TC d33 = (k == 1) ? (TC)(4.0 / 3.0) * (Q1(sk) - (TC)3 * Q1(0)) * dx3q
-                      : (Q1(sk) - (TC)2 * Q1(0) + Q1(-sk)) * dx3q;
+    TC d33 = zs ? zd2c<TC>(zm, k, Q1(-sk), Q1(0), Q1(sk))
+                : (k == 1) ? (TC)(4.0 / 3.0) * (Q1(sk) - (TC)3 * Q1(0)) * dx3q
+                           : (Q1(sk) - (TC)2 * Q1(0) + Q1(-sk)) * dx3q;
     TC qn2 = (Q1(sj) / (TC)c.rm[j + 1] - Q1(0) / (TC)c.rm[j]) * (TC)c.s1t2n[j];
     TC qs2 = (Q1(0) / (TC)c.rm[j] - Q1(djm) / (TC)c.rm[jm]) * (TC)c.s1t2s[j];
     TC d22 = (qn2 - qs2) * (TC)c.s1t2c[j];
@@ -751,14 +776,15 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
     TC h11n = (TC)0.25 * (qnn + qss) * (qnn + qss);
     qnn = (TC)0.25 * (Q3(sk) + Q3(sk - sj)) * (Q2(sk) + Q2(0));
     qss = (k == 1) ? (TC)0 : (TC)0.25 * (Q3(0) + Q3(-sj)) * (Q2(0) + Q2(-sk));
-    TC h23 = (qnn - qss) * dx3;
+    TC h23 = (qnn - qss) * zmk;
     qnn = Q1(dip) + Q1(dip - sj);
     qss = Q1(0) + Q1(-sj);
     TC d1q1 = -((qnn - qss) * (TC)c.h2dare[j]);
     TC H2 = d1q1 - (h21 + h22 + h23 - h11n);
     TC d11 = (Q2(dip) - (TC)2 * Q2(0) + Q2(dim)) * (TC)c.s2t11[j];
-    TC d33 = (k == 1) ? (TC)(4.0 / 3.0) * (Q2(sk) - (TC)3 * Q2(0)) * dx3q
-                      : (Q2(sk) - (TC)2 * Q2(0) + Q2(-sk)) * dx3q;
+    TC d33 = zs ? zd2c<TC>(zm, k, Q2(-sk), Q2(0), Q2(sk))
+                : (k == 1) ? (TC)(4.0 / 3.0) * (Q2(sk) - (TC)3 * Q2(0)) * dx3q
+                           : (Q2(sk) - (TC)2 * Q2(0) + Q2(-sk)) * dx3q;
     TC d22 = Q2(sj) * (TC)c.s2t2p[j] - Q2(0) * (TC)c.s2t2c[j] + Q2(-sj) * (TC)c.s2t2m[j];
     v2 = (ga * H2 + ro * ru2[o] - al * prg2[pc] + aRe * (d11 + d33 + d22)) * dt;
     ru2[o] = H2;
@@ -773,10 +799,10 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
     TC h31 = (qnn - qss) * (TC)c.h1d11[j];
     qnn = (TC)0.25 * (Q3(0) + Q3(sk)) * (Q3(0) + Q3(sk));
     qss = (TC)0.25 * (Q3(0) + Q3(-sk)) * (Q3(0) + Q3(-sk));
-    TC h33 = (qnn - qss) * dx3;
+    TC h33 = (qnn - qss) * zck;
     TC H3 = -(h31 + h32 + h33);
     TC d11 = (Q3(dip) - (TC)2 * Q3(0) + Q3(dim)) * (TC)c.s1t11[j];
-    TC d33 = (Q3(sk) - (TC)2 * Q3(0) + Q3(-sk)) * dx3q;
+    TC d33 = zs ? zd2f<TC>(zm, k, Q3(-sk), Q3(0), Q3(sk)) : (Q3(sk) - (TC)2 * Q3(0) + Q3(-sk)) * dx3q;
     TC qn2 = (Q3(sj) - Q3(0)) * (TC)c.s3t2n[j];
     TC qs2 = (Q3(0) - Q3(djm)) * (TC)c.s3t2s[j];
     TC d22 = qn2 - qs2;
@@ -1111,7 +1137,7 @@ __host__ __device__ constexpr size_t al16(size_t b) { return (b + 15) & ~(size_t
 // In mixed mode the packed positions 0..4 (m = 0,1,2) are also produced in
 // double from the double divergence into S64 (layout (5, nr, nz)).
 template <typename TW>
-__global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt, const double *__restrict__ dq1,
+__global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, ZMet zm, double inv_adt, const double *__restrict__ dq1,
                                                 const double *__restrict__ dq2, const double *__restrict__ dq3,
                                                 const double *__restrict__ ext, TW *__restrict__ S,
                                                 double *__restrict__ S64, const double *__restrict__ twr,
@@ -1188,6 +1214,7 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
     // indices are clamped to valid cells, so the loads are unconditional
     constexpr int U = 4;
     const double rmv = lirm[lpt], rdv = lcx2[lpt];
+    const double zkv = zm.m3 ? zm.m3[kl] : g.dx3;  // dx3, or the axial mesh's 1/width of plane kl
     const long ob = hidx(g, 1, jl, kl);
     for (int ub = 0; ub < NU; ub += U) {
       double a1[U], b1[U], a2[U], b2[U], a3[U], b3[U];
@@ -1208,7 +1235,7 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, double inv_adt
         const int ii = tl + TPL * (ub + u);
         if (ii < N) {
           const double t12 = ((b1[u] - a1[u]) * (g.dx1 * g.axsym) + (b2[u] - a2[u]) * rdv) * rmv;
-          const double t3 = (b3[u] - a3[u]) * g.dx3;
+          const double t3 = (b3[u] - a3[u]) * zkv;
           put(ii, lvalid ? -(t12 + t3) * inv_adt : 0.0);
         }
       }
@@ -1633,7 +1660,7 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
 // reciprocals of rm) is hoisted out of the z loop.
 // ---------------------------------------------------------------------------
 template <typename TW, int R>
-__global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt, double be, int kchunk,
+__global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, ZMet zm, double adt, double be, int kchunk,
                                                     const double *__restrict__ dph, const double *__restrict__ dq1,
                                                     const double *__restrict__ dq2, const double *__restrict__ dq3,
                                                     double *__restrict__ q1, double *__restrict__ q2,
@@ -1741,6 +1768,11 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
       const double(*Pkm)[LC + 1] = (k == 1 || k == nz) ? P0 : Pm;
       const long o = (long)(i - 1) + (long)j * sj + (long)k * sk;
       const double p0 = P0[cy][cx];
+      // axial mesh: the z gradient at faces k, k+1, k-1 and the z-gradient
+      // factor of upsol; the uniform mesh keeps dx3
+      const bool zs = zm.m3 != nullptr;
+      const double zc0 = zs ? zm.c3[k] : dx3, zcp = zs ? zm.c3[k + 1] : dx3, zcm = zs ? zm.c3[k - 1] : dx3;
+      const double a3k = zs ? adt * zc0 : a3;
       // ---- upsol ----
       if (j >= 1 && j <= nr) {
         const double v = c_dq1 - a1 * (p0 - P0[cy][cx - 1]);
@@ -1751,7 +1783,7 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
       if (j == 1 || j == nrc) q2[o] = 0.0;
       if (k == 1) q3[o] = c_dq3;
       if (k >= 2 && j >= 1 && j <= nr) {
-        const double v = c_dq3 - a3 * (p0 - Pm[cy][cx]);
+        const double v = c_dq3 - a3k * (p0 - Pm[cy][cx]);
         q3[o] = v;
         if (j == nr) q3[o + sj] = slip * v;
       }
@@ -1767,7 +1799,7 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
           const double qjp = (P0[cy + djp][cx] - P0[cy + djp][cx - 1]) * dx1 * irmp;
           const double qjm = (P0[cy + djm][cx] - P0[cy + djm][cx - 1]) * dx1 * irmm;
           const double d11 = (qip - 2.0 * qc0 + qim) * s1t11;
-          const double d33 = (qkp - 2.0 * qc0 + qkm) * dx3q;
+          const double d33 = zs ? (qkp - qc0) * zm.cp[k] - (qc0 - qkm) * zm.cm[k] : (qkp - 2.0 * qc0 + qkm) * dx3q;
           const double qnn = qjp * irmn - qc0 * irm;
           const double qss = qc0 * irm - qjm * irmm;
           const double d22 = (qnn * s1t2n - qss * s1t2s) * s1t2c;
@@ -1783,23 +1815,23 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
           const double qjp = (j + 1 < nrc) ? (P0[cy + 1][cx] - p0) * dx2 * rcp : 0.0;
           const double qjm = (j - 1 > 1) ? (P0[cy - 1][cx] - P0[cy - 2][cx]) * dx2 * rcm : 0.0;
           const double d11 = (qip - 2.0 * qc0 + qim) * s2t11;
-          const double d33 = (qkp - 2.0 * qc0 + qkm) * dx3q;
+          const double d33 = zs ? (qkp - qc0) * zm.cp[k] - (qc0 - qkm) * zm.cm[k] : (qkp - 2.0 * qc0 + qkm) * dx3q;
           const double d22 = (qjp - qc0) * s2t2p - (qc0 - qjm) * s2t2m;
           prg2[pc] = c_p2 + qc0 - be * (d11 + d22 + d33);
         }
         // z component: qcap = (dph(k) - dph(k-1)) dx3, zero at k = 1
         {
-          const double qkp = (Pp[cy][cx] - p0) * dx3;
+          const double qkp = (Pp[cy][cx] - p0) * zcp;
           if (qkp != qkp) atomicOr(flag, 1);  // "Flow field crash" (divergence.F90:396-399)
           if (k >= 2) {
-            const double qc0 = (p0 - Pm[cy][cx]) * dx3;
-            const double qip = (P0[cy][cx + 1] - Pm[cy][cx + 1]) * dx3;
-            const double qim = (P0[cy][cx - 1] - Pm[cy][cx - 1]) * dx3;
-            const double qkm = (k - 1 > 1) ? (Pm[cy][cx] - Pm2[cy][cx]) * dx3 : 0.0;
-            const double qjp = (P0[cy + djp][cx] - Pm[cy + djp][cx]) * dx3;
-            const double qjm = (P0[cy + djm][cx] - Pm[cy + djm][cx]) * dx3;
+            const double qc0 = (p0 - Pm[cy][cx]) * zc0;
+            const double qip = (P0[cy][cx + 1] - Pm[cy][cx + 1]) * zc0;
+            const double qim = (P0[cy][cx - 1] - Pm[cy][cx - 1]) * zc0;
+            const double qkm = (k - 1 > 1) ? (Pm[cy][cx] - Pm2[cy][cx]) * zcm : 0.0;
+            const double qjp = (P0[cy + djp][cx] - Pm[cy + djp][cx]) * zc0;
+            const double qjm = (P0[cy + djm][cx] - Pm[cy + djm][cx]) * zc0;
             const double d11 = (qip - 2.0 * qc0 + qim) * s1t11;
-            const double d33 = (qkp - 2.0 * qc0 + qkm) * dx3q;
+            const double d33 = zs ? (qkp - qc0) * zm.fp[k] - (qc0 - qkm) * zm.fm[k] : (qkp - 2.0 * qc0 + qkm) * dx3q;
             const double d22 = (qjp - qc0) * s3t2n - (qc0 - qjm) * s3t2s;
             prg3[pc] = c_p3 + qc0 - be * (d11 + d22 + d33);
           }
@@ -1807,7 +1839,8 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
         {
           const double d11 = (P0[cy][cx + 1] + P0[cy][cx - 1] - 2.0 * p0) * s1t11;
           const double d22 = (P0[cy + djp][cx] - p0) * s3t2n - (p0 - P0[cy + djm][cx]) * s3t2s;
-          const double d33 = (Pp[cy][cx] + Pkm[cy][cx] - 2.0 * p0) * dx3q;
+          const double d33 = zs ? (Pp[cy][cx] - p0) * zm.cp[k] - (p0 - Pkm[cy][cx]) * zm.cm[k]
+                                : (Pp[cy][cx] + Pkm[cy][cx] - 2.0 * p0) * dx3q;
           pres[o] = c_pres + p0 - be * (d11 + d22 + d33);
         }
       }
@@ -1825,6 +1858,7 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, double adt
 // dx3.  Per-block partials [block][10], summed in a fixed order.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_analysis(VGrid g, double rmax, double zmax, const double *__restrict__ rmv,
+                                                  const double *__restrict__ zpos,
                                                   const double *__restrict__ q1, const double *__restrict__ q2,
                                                   const double *__restrict__ q3, double *__restrict__ part) {
   const long ncell = (long)g.nt * g.nr * g.nz;
@@ -1834,14 +1868,15 @@ __global__ __launch_bounds__(256) void k_analysis(VGrid g, double rmax, double z
     const int ith = (int)(c % g.nt) + 1;
     const long rest = c / g.nt;
     const int ir = (int)(rest % g.nr) + 1, iz = (int)(rest / g.nr) + 1;
-    const double rm = rmv[ir], zm = (iz - 0.5) * dz;  // msh%rm: the radial mesh (uniform or set)
+    // msh%rm, msh%zm: the radial and axial meshes (uniform or set)
+    const double rm = rmv[ir], zm = zpos ? zpos[iz] : (iz - 0.5) * dz;
     if (!(zm < 0.90 * zmax && zm > 0.10 * zmax && rm < 0.80 * rmax && rm > rmax * 0.05 && ir != 1)) continue;
     const int ithp1 = ith == g.nt ? 1 : ith + 1;
     const double q1m = g.nt > 1 ? 0.5 * (q1[hidx(g, ith, ir, iz)] + q1[hidx(g, ithp1, ir, iz)]) : q1[hidx(g, ith, ir, iz)];
     const double rp = rmv[ir + 1], rmm = rmv[ir - 1];
     const double dwdr = 0.5 * ((q3[hidx(g, ith, ir + 1, iz)] + q3[hidx(g, ith, ir + 1, iz + 1)]) -
                                (q3[hidx(g, ith, ir - 1, iz)] + q3[hidx(g, ith, ir - 1, iz + 1)])) / (rp - rmm);
-    const double zp = (iz + 0.5) * dz, zmm = (iz - 1.5) * dz;
+    const double zp = zpos ? zpos[iz + 1] : (iz + 0.5) * dz, zmm = zpos ? zpos[iz - 1] : (iz - 1.5) * dz;
     const double dvdz = 0.5 * ((q2[hidx(g, ith, ir, iz + 1)] / rm + q2[hidx(g, ith, ir + 1, iz + 1)] / rp) -
                                (q2[hidx(g, ith, ir + 1, iz - 1)] / rp + q2[hidx(g, ith, ir, iz - 1)] / rm)) / (zp - zmm);
     const double vort = +(dvdz - dwdr), swrl = q1m;
@@ -1895,7 +1930,8 @@ static void hdnbc_on(vrh_t *h, int irk, double cor, const double *cor_dev, hipSt
   const VGrid &g = h->g;
   dim3 blk(64), grd((g.nt + 63) / 64, g.nrc);
   KTimer kt(h, KT_HDNBC);
-  hipLaunchKernelGGL(k_hdnbc, grd, blk, 0, st, g, h->coef, h->rk[irk].g, h->rk[irk].r, cor, cor_dev, h->f.q3,
+  const double zm3n = h->zmesh[2].empty() ? 0.0 : g.dx3 / h->zmesh[2][g.nz];
+  hipLaunchKernelGGL(k_hdnbc, grd, blk, 0, st, g, h->coef, zm3n, h->rk[irk].g, h->rk[irk].r, cor, cor_dev, h->f.q3,
                      h->f.csort, h->f.dq3no, h->f.corr3n);
 }
 template <typename TW> void launch_hdnbc(vrh_t *h, int irk, double cor, const double *cor_dev) {
@@ -1929,7 +1965,7 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
     if (rj < 1) rj = 1;
     dim3 blk(ntp, rj), grd((g.nr + rj - 1) / rj, g.nz);
     const size_t sh = sizeof(TC) * 6 * (size_t)rj * ntp;
-    hipLaunchKernelGGL((k_explicit_cell<TW, TC>), grd, blk, sh, h->stream, g, h->coef, h->rk[irk], betadx, nlev,
+    hipLaunchKernelGGL((k_explicit_cell<TW, TC>), grd, blk, sh, h->stream, g, h->coef, h->rk[irk], h->zmet, betadx, nlev,
                        h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2,
                        h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,
                        (TW *)h->f.r2, (TW *)h->f.r3);
@@ -1976,7 +2012,7 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
     bool done = rw > 1 && (var == 2 ? lds(rw) : lds_w(rw)) <= lim;
 #define EXPL_WAVE(C, R, W)                                                                                          \
   hipLaunchKernelGGL((k_explicit_wave<TW, TC, C, R, W>), grd, dim3(64, R), lds_w(R), h->stream, g, h->coef, h->rk[irk],   \
-                     betadx, nlev, kchunk, h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2, \
+                     h->zmet, betadx, nlev, kchunk, h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2, \
                      h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,          \
                      (TW *)h->f.r2, (TW *)h->f.r3)
     if (done) {
@@ -1993,7 +2029,7 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
   }
   while (rj > 1 && lds(rj) > 65536) rj--;
   dim3 blk(ntp, rj), grd((g.nr + rj - 1) / rj, (g.nz + kchunk - 1) / kchunk);
-  hipLaunchKernelGGL((k_explicit_march<TW, TC>), grd, blk, lds(rj), h->stream, g, h->coef, h->rk[irk], betadx, nlev,
+  hipLaunchKernelGGL((k_explicit_march<TW, TC>), grd, blk, lds(rj), h->stream, g, h->coef, h->rk[irk], h->zmet, betadx, nlev,
                      kchunk, h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2,
                      h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,
                      (TW *)h->f.r2, (TW *)h->f.r3);
@@ -2118,6 +2154,31 @@ static void radial_launch(vrh_t *h, T *S, const T *ib, long nm, hipStream_t st) 
     hipLaunchKernelGGL((k_radial<T, 32>), grd, blk, 0, st, S, ib, nm, nlines, g.nr, h->d_aab, h->d_ccb);
 }
 
+// z transform of the pressure solve on an axial mesh: every spectral line
+// (the nl lines of one z position are contiguous, so the array is a
+// column-major nl x nz matrix) times a dense nz x nz matrix, out of place.
+// Forward: out(:, q) = sum_k in(:, k) Vinv(q, k); inverse: out(:, k) =
+// sum_q in(:, q) V(k, q).  The row-major matrices read column-major are
+// their transposes, the right operand of both products.
+static void zgemm(vrh_t *h, hipStream_t st, const float *in, float *out, long nl, int dir) {
+  const int nz = h->g.nz;
+  const float one = 1.f, zero = 0.f;
+  rocblas_handle b = (rocblas_handle)h->blas;
+  rocblas_set_stream(b, st);
+  rocblas_sgemm(b, rocblas_operation_none, rocblas_operation_none, (rocblas_int)nl, nz, nz, &one, in,
+                (rocblas_int)nl, h->d_zvf + (dir > 0 ? 0 : (size_t)nz * nz), nz, &zero, out, (rocblas_int)nl);
+  rocblas_set_stream(b, h->stream);
+}
+static void zgemm(vrh_t *h, hipStream_t st, const double *in, double *out, long nl, int dir) {
+  const int nz = h->g.nz;
+  const double one = 1.0, zero = 0.0;
+  rocblas_handle b = (rocblas_handle)h->blas;
+  rocblas_set_stream(b, st);
+  rocblas_dgemm(b, rocblas_operation_none, rocblas_operation_none, (rocblas_int)nl, nz, nz, &one, in,
+                (rocblas_int)nl, h->d_zv + (dir > 0 ? 0 : (size_t)nz * nz), nz, &zero, out, (rocblas_int)nl);
+  rocblas_set_stream(b, h->stream);
+}
+
 template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_rhs, double *phi_out) {
   const VGrid &g = h->g;
   const int N = g.nt, logn = ilog2(N);
@@ -2138,7 +2199,7 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   sh1 = al16(al16(sh1 + sizeof(TW) * N) + sizeof(int) * 2 * LB) + sizeof(double) * 2 * LB;
   { KTimer kt(h, KT_DIVFFT);
   hipLaunchKernelGGL(k_divfft<TW>, dim3((unsigned)((nlines + LB - 1) / LB)), dim3(256), sh1, h->stream, g, h->coef,
-                     1.0 / adt, h->f.dq1, h->f.dq2, h->f.dq3, ext_rhs, S, S64, h->d_twt, h->d_twt + N, LB, logn); }
+                     h->zmet, 1.0 / adt, h->f.dq1, h->f.dq2, h->f.dq3, ext_rhs, S, S64, h->d_twt, h->d_twt + N, LB, logn); }
   // z transforms
   const int nz = g.nz, lz = ilog2(nz);
   // lines per block: a power of two, data + twiddles within 64 KB of LDS
@@ -2176,7 +2237,12 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
     (void)hipEventRecord(h->poi_ev[ie][0], h->stream);
     (void)hipStreamWaitEvent(s64, h->poi_ev[ie][0], 0);
   }
-  if (S64) {
+  if (S64 && h->zmet.m3) {
+    const long nl64 = 5L * g.nr;
+    zgemm(h, s64, S64, h->S2_64, nl64, 1);
+    radial_launch<double>(h, h->S2_64, h->f.ib64, 5, s64);
+    zgemm(h, s64, h->S2_64, S64, nl64, -1);
+  } else if (S64) {
     const long nl64 = 5L * g.nr;
     hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(NTD64), sh64, s64, S64,
                        nl64, nz, lz, TL64, ltl64, 1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
@@ -2184,6 +2250,12 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
     hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(NTD64), sh64, s64, S64,
                        nl64, nz, lz, TL64, ltl64, -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
   }
+  if (h->zmet.m3) {
+    TW *S2 = (TW *)h->S2;
+    { KTimer kt(h, KT_DCT_FWD); zgemm(h, h->stream, S, S2, nl, 1); }
+    radial_launch<TW>(h, S2, (const TW *)h->f.ib, N, h->stream);
+    { KTimer kt(h, KT_DCT_INV); zgemm(h, h->stream, S2, S, nl, -1); }
+  } else {
   { KTimer kt(h, KT_DCT_FWD);
   hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(NTD), sh2, h->stream, S, nl, nz, lz, TL,
                      ltl, 1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
@@ -2191,6 +2263,7 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   { KTimer kt(h, KT_DCT_INV);
   hipLaunchKernelGGL(k_dctz<TW>, dim3((unsigned)((nl + TL - 1) / TL)), dim3(NTD), sh2, h->stream, S, nl, nz, lz, TL,
                      ltl, -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz); }
+  }
   if (par) {
     (void)hipEventRecord(h->poi_ev[ie][1], s64);
     (void)hipStreamWaitEvent(h->stream, h->poi_ev[ie][1], 0);
@@ -2210,13 +2283,13 @@ template <typename TW> void launch_project(vrh_t *h, int irk) {
   const int kchunk = 32;
   dim3 blk(64, R), grd((g.nt + 63) / 64, (g.nrc + 1 + R - 1) / R, (g.nzc + kchunk - 1) / kchunk);
   KTimer kt(h, KT_PROJECT);
-  hipLaunchKernelGGL((k_project<TW, R>), grd, blk, 0, h->stream, g, h->coef, adt, be, kchunk, h->f.dph, h->f.dq1,
+  hipLaunchKernelGGL((k_project<TW, R>), grd, blk, 0, h->stream, g, h->coef, h->zmet, adt, be, kchunk, h->f.dph, h->f.dq1,
                      h->f.dq2, h->f.dq3, h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2,
                      h->f.prg3, h->f.pres, h->f.flag);
 }
 
 void launch_analysis(vrh_t *h, double *part, int nblk, double *out) {
-  hipLaunchKernelGGL(k_analysis, dim3(nblk), dim3(256), 0, h->stream, h->g, h->rmax, h->zmax, h->coef.rm, h->f.q1,
+  hipLaunchKernelGGL(k_analysis, dim3(nblk), dim3(256), 0, h->stream, h->g, h->rmax, h->zmax, h->coef.rm, h->zmet.zpos, h->f.q1,
                      h->f.q2, h->f.q3, part);
   hipLaunchKernelGGL(k_analysis_sum, dim3(1), dim3(64), 0, h->stream, part, nblk, out);
 }

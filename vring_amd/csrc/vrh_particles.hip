@@ -615,6 +615,7 @@ int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st
   if (scheme < VRH_PART_LSQ_EULER || scheme > VRH_PART_TRILINEAR_RK3) { h->err = "unknown particle scheme"; return -1; }
   if (!h->pb.empty()) { h->err = "particles already created"; return -1; }
   if (!h->mesh[0].empty()) { h->err = "particles need the uniform radial mesh"; return -1; }
+  if (!h->zmesh[0].empty()) { h->err = "particles need the uniform axial mesh"; return -1; }
   const int nvar = scheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
   h->pscheme = scheme;
   h->pb.resize(nbatch);

@@ -31,6 +31,7 @@ class VrhParams(ctypes.Structure):
 
 EXPORTS = ["vrh_create", "vrh_destroy", "vrh_last_error", "vrh_upload_fields", "vrh_download_fields",
            "vrh_upload_state", "vrh_download_state", "vrh_state_size", "vrh_set_time", "vrh_get_time", "vrh_set_radial_mesh",
+           "vrh_set_axial_mesh",
            "vrh_substep",
            "vrh_step", "vrh_phase", "vrh_poisson_solve", "vrh_synchronize", "vrh_device_ptr",
            "vrh_ring_init", "vrh_cfl_dt", "vrh_particles_create", "vrh_particles_upload",
@@ -65,6 +66,7 @@ def load_library(path=LIBPATH):
     L.vrh_set_time.argtypes = [P, D]
     L.vrh_get_time.argtypes = [P, ctypes.POINTER(D)]
     L.vrh_set_radial_mesh.argtypes = [P, dp, dp, dp, dp]
+    L.vrh_set_axial_mesh.argtypes = [P, dp, dp, dp, dp]
     L.vrh_substep.argtypes = [P, I]
     L.vrh_step.argtypes = [P, I]
     L.vrh_phase.argtypes = [P, I, I]
@@ -227,6 +229,13 @@ class Solver:
         arrs = [_f64(a) for a in (rm, rc, dmr, dcr)]
         assert all(a.size == self.nr + 2 for a in arrs)
         self._check(self.L.vrh_set_radial_mesh(self.h, *arrs), "set_radial_mesh")
+
+    def set_axial_mesh(self, zm, zc, dmz, dcz):
+        """Non-uniform axial mesh: zm, zc, dmz, dcz over k = 0..nz+1 (cell
+        centres, faces with zc[1] = 0, and dz/dzeta at centres and faces)."""
+        arrs = [_f64(a) for a in (zm, zc, dmz, dcz)]
+        assert all(a.size == self.nz + 2 for a in arrs)
+        self._check(self.L.vrh_set_axial_mesh(self.h, *arrs), "set_axial_mesh")
 
     def time(self):
         t = ctypes.c_double(0.0)
