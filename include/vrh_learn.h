@@ -23,6 +23,8 @@
 
 #include <stddef.h>
 
+#include "vrh.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -98,6 +100,22 @@ int vrh_hz_sample(vrh_hz_t *h, unsigned long long seed, int family, double *f);
  * divided by the cell volume, as the solver sees residuals).  loss
  * (optional) receives the mean loss of each batch. */
 int vrh_hz_train(vrh_hz_t *h, int nsamples, double lr, unsigned long long seed, float *loss);
+
+/* ---- in the time step.  The pressure solve of a stepper handle h with an
+ * axial mesh equal to hz's tanh mesh (vrh_set_axial_mesh), the uniform
+ * radial mesh and the exact theta eigenvalue (VRH_THETA_EXACT_M2) solves the
+ * theta positions of the modes m >= mmin (pos >= 2 mmin - 1; mmin >= 1, and
+ * >= 3 in the mixed path, whose m = 0..2 stay in the FP64 direct chain) by
+ * hz's iterative `method` to relative residual tol instead of the direct z
+ * transform and radial solve (poisson_solve.F90:140-179, 270-323 on the
+ * uniform grid).  hz needs one slot per such position (nt - 2 mmin + 1);
+ * this call sets their modes.  A slot that misses tol within maxit makes the
+ * step fail (code -4).  hz = NULL switches it off.  No reference
+ * counterpart (the reference has no stretched z grid). */
+int vrh_set_mode_solver(vrh_t *h, vrh_hz_t *hz, int method, int mmin, double tol, int maxit);
+/* Krylov iterations (summed over slots) and mode solves since the last
+ * vrh_set_mode_solver. */
+int vrh_mode_solver_stats(vrh_t *h, long long *iterations, long long *solves);
 
 #ifdef __cplusplus
 }

@@ -155,6 +155,7 @@ template <typename TW> int substep_t(vrh_t *h, int irk, int phase) {
   if (phase < 0 || phase == VRH_PH_POISSON) launch_poisson<TW>(h, irk, nullptr, nullptr);
   if (phase < 0 || phase == VRH_PH_PROJECT) launch_project<TW>(h, irk);
   HIPCHK(h, hipGetLastError());
+  if (h->mode_rc) { const int rc = h->mode_rc; h->mode_rc = 0; return rc; }
   return 0;
 }
 
@@ -184,6 +185,7 @@ int step_body(vrh_t *h) {
     if (h->prec == VRH_MIXED) substep_dev<float>(h, irk);
     else substep_dev<double>(h, irk);
   }
+  if (h->mode_rc) { const int rc = h->mode_rc; h->mode_rc = 0; return rc; }
   return hipGetLastError() == hipSuccess ? 0 : fail(h, "kernel launch failed", -3);
 }
 
@@ -576,6 +578,49 @@ int vrh_set_axial_mesh(vrh_t *h, const double *zm, const double *zc, const doubl
   return 0;
 }
 
+// Per-mode iterative pressure solve for the modes m >= mmin through a
+// Helmholtz handle (its operator is the pressure equation's (r, z) part of
+// one mode on the same tanh axial mesh); hz == NULL switches it off.
+int vrh_set_mode_solver(vrh_t *h, vrh_hz_t *hz, int method, int mmin, double tol, int maxit) {
+  if (!h) return -1;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->hz = nullptr;
+  h->hz_iters = h->hz_solves = 0;
+  h->mode_rc = 0;
+  if (!hz) return 0;
+  if (method != VRH_HZ_PCG && method != VRH_HZ_BICGSTAB && method != VRH_HZ_LEARNED && method != VRH_HZ_LEARNED_FCG)
+    return fail(h, "mode solver: unknown method");
+  if (!(tol > 0.0) || maxit < 1) return fail(h, "mode solver: need tol > 0 and maxit >= 1");
+  if (mmin < 1) return fail(h, "mode solver: mmin must be >= 1 (the (0,0) mode keeps the reference's pinned direct solve)");
+  if (h->prec == VRH_MIXED && mmin < 3)
+    return fail(h, "mode solver: the mixed path solves m = 0..2 in FP64 directly; mmin must be >= 3");
+  if (h->theta_eigen != VRH_THETA_EXACT_M2) return fail(h, "mode solver: needs the exact theta eigenvalue m^2");
+  if (!h->mesh[0].empty()) return fail(h, "mode solver: needs the uniform radial mesh");
+  if (h->zmesh[0].empty()) return fail(h, "mode solver: needs an axial mesh (vrh_set_axial_mesh)");
+  const int N = h->nt, pos0 = 2 * mmin - 1;
+  if (pos0 >= N) return fail(h, "mode solver: mmin beyond the grid's modes");
+  if (hz_slots(hz) != N - pos0) return fail(h, "mode solver: the Helmholtz handle needs one slot per position pos >= 2 mmin - 1");
+  std::string e;
+  if (hz_step_check(hz, h->nr, h->nz, h->rmax, h->zmax, h->zmesh[1].data(), h->zmesh[0].data(), e))
+    return fail(h, e.c_str());
+  std::vector<int> m(N - pos0);
+  for (int p = pos0; p < N; p++) m[p - pos0] = (p + 1) / 2;
+  if (vrh_hz_set_modes(hz, m.data())) return fail(h, (std::string("mode solver: ") + hz_error(hz)).c_str());
+  h->hz = hz;
+  h->hz_method = method;
+  h->hz_pos0 = pos0;
+  h->hz_tol = tol;
+  h->hz_maxit = maxit;
+  return 0;
+}
+
+int vrh_mode_solver_stats(vrh_t *h, long long *iterations, long long *solves) {
+  if (!h) return -1;
+  if (iterations) *iterations = h->hz_iters;
+  if (solves) *solves = h->hz_solves;
+  return 0;
+}
+
 // One substep of the step that has reached the current time, with the
 // step's scalars computed on the device as vrh_step computes them, so that
 // three calls after vrh_set_time(t + dt) equal one vrh_step from t bit for
@@ -589,6 +634,7 @@ int vrh_substep(vrh_t *h, int irk) {
   else substep_dev<double>(h, irk);
   HIPCHK(h, hipGetLastError());
   h->time_dirty = false;
+  if (h->mode_rc) { const int rc = h->mode_rc; h->mode_rc = 0; return rc; }
   return 0;
 }
 
@@ -697,6 +743,14 @@ int vrh_poisson_solve(vrh_t *h, const double *rhs, double *phi) {
   if (h->prec == VRH_MIXED) launch_poisson<float>(h, 1, d_rhs, d_phi);
   else launch_poisson<double>(h, 1, d_rhs, d_phi);
   HIPCHK(h, hipGetLastError());
+  if (h->mode_rc) {
+    const int rc = h->mode_rc;
+    h->mode_rc = 0;
+    (void)hipStreamSynchronize(h->stream);
+    hipFree(d_rhs);
+    hipFree(d_phi);
+    return rc;
+  }
   HIPCHK(h, hipMemcpyAsync(phi, d_phi, C * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   hipFree(d_rhs);

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "vrh.h"
+#include "vrh_learn.h"
 
 #define VRH_HD __host__ __device__ __forceinline__
 
@@ -152,6 +153,16 @@ struct vrh_s {
   void *S2 = nullptr;
   double *S2_64 = nullptr;
   void *blas = nullptr;  // rocBLAS handle of the z transforms (on `stream`)
+  // per-mode iterative pressure solve (vrh_set_mode_solver): theta
+  // positions pos >= hz_pos0 (modes m >= mmin) are solved by the Helmholtz
+  // handle hz instead of the direct z transform + radial solve; hz_f is the
+  // double staging copy of their right-hand sides taken before the direct
+  // path overwrites the spectral array
+  vrh_hz_t *hz = nullptr;
+  int hz_method = 0, hz_maxit = 0, hz_pos0 = 0;
+  double hz_tol = 0.0;
+  long long hz_iters = 0, hz_solves = 0;
+  int mode_rc = 0;  // error of the last in-step mode solve (err holds the message)
   VCoef coef;
   // line coefficients: [irk 1..3][sweep 0..5]
   double *d_line;
@@ -222,6 +233,14 @@ void host_radial_pivots(vrh_t *h, std::vector<double> &ib, std::vector<double> &
 // and c3 (faces), k = 0..nz+2: vi = V^-1, vf = V (row-major nz x nz) and lam
 // (ascending, lam[0] = 0) with (-D_z G_z) V = V diag(lam).  Returns nonzero
 // (and err) if the decomposition fails its residual or orthogonality check.
+// Helmholtz-handle hooks used by the in-step mode solve (vrh_learn.hip).
+int hz_slots(const vrh_hz_t *h);
+double *hz_step_input(vrh_hz_t *h);
+const double *hz_step_output(vrh_hz_t *h);
+const char *hz_error(const vrh_hz_t *h);
+int hz_step_check(const vrh_hz_t *h, int nr, int nz, double rmax, double zmax, const double *zc, const double *zm,
+                  std::string &err);
+int hz_step_solve(vrh_hz_t *h, hipStream_t caller, int method, double tol, int maxit, long long *iters);
 int host_axial_transforms(const double *m3, const double *c3, int nz, std::vector<double> &vi,
                           std::vector<double> &vf, std::vector<double> &lam, std::string &err);
 double host_profile(double t);

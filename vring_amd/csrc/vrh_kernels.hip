@@ -2179,6 +2179,46 @@ static void zgemm(vrh_t *h, hipStream_t st, const double *in, double *out, long 
   rocblas_set_stream(b, h->stream);
 }
 
+// Spectral array (position fastest, then r, then z) <-> the Helmholtz
+// handle's slots (slot, z, r), for positions pos0..N-1: 64 x 64 tiles of
+// (position, r) through LDS, one z plane per blockIdx.z.
+template <typename TW>
+__global__ __launch_bounds__(256) void k_hz_gather(const TW *__restrict__ S, double *__restrict__ f, int N, int nr,
+                                                   int nz, int pos0) {
+  __shared__ double t[64][65];
+  const int p0 = pos0 + blockIdx.x * 64, j0 = blockIdx.y * 64, k = blockIdx.z;
+  const int np = N - pos0;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int pp = e & 63, jj = e >> 6;
+    const int pos = p0 + pp, j = j0 + jj;
+    t[jj][pp] = (pos < N && j < nr) ? (double)S[(size_t)pos + (size_t)N * ((size_t)j + (size_t)nr * k)] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int jj = e & 63, pp = e >> 6;
+    const int b = p0 + pp - pos0, j = j0 + jj;
+    if (b < np && j < nr) f[((size_t)b * nz + k) * nr + j] = t[jj][pp];
+  }
+}
+template <typename TW>
+__global__ __launch_bounds__(256) void k_hz_scatter(const double *__restrict__ x, TW *__restrict__ S, int N, int nr,
+                                                    int nz, int pos0) {
+  __shared__ double t[64][65];
+  const int p0 = pos0 + blockIdx.x * 64, j0 = blockIdx.y * 64, k = blockIdx.z;
+  const int np = N - pos0;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int jj = e & 63, pp = e >> 6;
+    const int b = p0 + pp - pos0, j = j0 + jj;
+    t[pp][jj] = (b < np && j < nr) ? x[((size_t)b * nz + k) * nr + j] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int pp = e & 63, jj = e >> 6;
+    const int pos = p0 + pp, j = j0 + jj;
+    if (pos < N && j < nr) S[(size_t)pos + (size_t)N * ((size_t)j + (size_t)nr * k)] = (TW)t[pp][jj];
+  }
+}
+
 template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_rhs, double *phi_out) {
   const VGrid &g = h->g;
   const int N = g.nt, logn = ilog2(N);
@@ -2250,6 +2290,12 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
     hipLaunchKernelGGL(k_dctz<double>, dim3((unsigned)((nl64 + TL64 - 1) / TL64)), dim3(NTD64), sh64, s64, S64,
                        nl64, nz, lz, TL64, ltl64, -1, h->d_twz, h->d_twz + nz, h->d_ez, h->d_ez + nz);
   }
+  // per-mode iterative solve of the positions >= hz_pos0: stage their
+  // right-hand sides before the direct path overwrites them
+  const dim3 hzg((unsigned)((N - h->hz_pos0 + 63) / 64), (unsigned)((g.nr + 63) / 64), (unsigned)nz);
+  if (h->hz)
+    hipLaunchKernelGGL(k_hz_gather<TW>, hzg, dim3(256), 0, h->stream, S, hz_step_input(h->hz), N, g.nr, nz,
+                       h->hz_pos0);
   if (h->zmet.m3) {
     TW *S2 = (TW *)h->S2;
     { KTimer kt(h, KT_DCT_FWD); zgemm(h, h->stream, S, S2, nl, 1); }
@@ -2267,6 +2313,15 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   if (par) {
     (void)hipEventRecord(h->poi_ev[ie][1], s64);
     (void)hipStreamWaitEvent(h->stream, h->poi_ev[ie][1], 0);
+  }
+  if (h->hz) {
+    if (!h->mode_rc) {
+      const int rc = hz_step_solve(h->hz, h->stream, h->hz_method, h->hz_tol, h->hz_maxit, &h->hz_iters);
+      if (rc) { h->mode_rc = rc; h->err = hz_error(h->hz); }
+    }
+    h->hz_solves++;
+    hipLaunchKernelGGL(k_hz_scatter<TW>, hzg, dim3(256), 0, h->stream, hz_step_output(h->hz), S, N, g.nr, nz,
+                       h->hz_pos0);
   }
   size_t sh3 = al16(sizeof(TW) * ((size_t)2 * LB * N + N)) + sizeof(int) * 2 * LB;
   if (S64) sh3 = al16(sh3) + sizeof(double) * (5 * (size_t)LB + 2 * (size_t)N);

@@ -1307,6 +1307,73 @@ int true_residual(vrh_hz_t *h, const double *rhs, const double *x, std::vector<d
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// In-step use by the stepper's pressure solve (vrh_set_mode_solver): the
+// stepper writes each slot's right-hand side f (the mode's divergence, as the
+// direct path receives it) into the input buffer, solves in place ordered
+// after its own stream, and reads the solution back.
+// ---------------------------------------------------------------------------
+int hz_slots(const vrh_hz_t *h) { return h->S; }
+double *hz_step_input(vrh_hz_t *h) { return h->vec[0]; }
+const double *hz_step_output(vrh_hz_t *h) { return h->vec[10]; }
+const char *hz_error(const vrh_hz_t *h) { return h->err.c_str(); }
+
+// The grid must be the stepper's: same nr, nz, rmax, zmax; z faces zc(1..nz+1)
+// and centres zm(1..nz) equal to this handle's tanh mesh (1e-12 of zmax).
+int hz_step_check(const vrh_hz_t *h, int nr, int nz, double rmax, double zmax, const double *zc, const double *zm,
+                  std::string &err) {
+  if (h->nr != nr || h->nz != nz) { err = "mode solver: grid size differs from the handle's"; return -1; }
+  if (std::fabs(h->rmax - rmax) > 1e-14 * rmax || std::fabs(h->zmax - zmax) > 1e-14 * zmax) {
+    err = "mode solver: domain differs from the handle's"; return -1;
+  }
+  double d = 0.0;
+  for (int k = 0; k <= nz; k++) d = std::max(d, std::fabs(h->zf[k] - zc[k + 1]));
+  for (int k = 0; k < nz; k++) d = std::max(d, std::fabs(h->zm[k] - zm[k + 1]));
+  if (!(d <= 1e-12 * zmax)) { err = "mode solver: the handle's axial mesh is not the Helmholtz handle's tanh mesh"; return -1; }
+  return 0;
+}
+
+int hz_step_solve(vrh_hz_t *h, hipStream_t caller, int method, double tol, int maxit, long long *iters) {
+  hipEvent_t ev;
+  HZCHK(h, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  HZCHK(h, hipEventRecord(ev, caller));
+  HZCHK(h, hipStreamWaitEvent(h->stream, ev, 0));
+  double *rhs = h->vec[1], *x = h->vec[10];
+  launch_rhs(h, h->vec[0], rhs);
+  std::vector<int> its(h->S, 0), its2;
+  int rc = 0;
+  if (method == VRH_HZ_LEARNED || method == VRH_HZ_LEARNED_FCG) net_galerkin(h, rhs, x);
+  else HZCHK(h, hipMemsetAsync(x, 0, h->ncell() * sizeof(double), h->stream));
+  if (method == VRH_HZ_LEARNED_FCG) {
+    if ((rc = fcg_dev(h, rhs, x, tol, maxit, its))) return rc;
+    bool fallback = false;
+    for (int b = 0; b < h->S; b++) fallback = fallback || its[b] < 0;
+    if (fallback) {
+      if ((rc = pcg_dev(h, rhs, x, tol, maxit, its2))) return rc;
+      for (int b = 0; b < h->S; b++)
+        if (its[b] < 0) its[b] = its2[b] < 0 ? -1 : maxit + its2[b];
+    }
+  } else if (method == VRH_HZ_BICGSTAB) {
+    if ((rc = bicgstab_dev(h, rhs, x, tol, maxit, its))) return rc;
+  } else {
+    if ((rc = pcg_dev(h, rhs, x, tol, maxit, its))) return rc;
+  }
+  launch_gauge(h, x);
+  // the caller's stream reads x only after everything queued here
+  HZCHK(h, hipEventRecord(ev, h->stream));
+  HZCHK(h, hipStreamWaitEvent(caller, ev, 0));
+  HZCHK(h, hipEventDestroy(ev));
+  long long tot = 0;
+  bool ok = true;
+  for (int b = 0; b < h->S; b++) {
+    ok = ok && its[b] >= 0;
+    tot += its[b] > 0 ? its[b] : 0;
+  }
+  if (iters) *iters += tot;
+  if (!ok) return hfail(h, "mode solver: a slot did not converge within maxit", -4);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {

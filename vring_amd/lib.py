@@ -230,6 +230,23 @@ class Solver:
         assert all(a.size == self.nr + 2 for a in arrs)
         self._check(self.L.vrh_set_radial_mesh(self.h, *arrs), "set_radial_mesh")
 
+    def set_mode_solver(self, hz, method="pcg", mmin=1, tol=1e-8, maxit=20000):
+        """Solve the pressure modes m >= mmin with the Helmholtz handle `hz`
+        (one slot per theta position 2 mmin - 1 .. ntheta - 1, on this
+        solver's tanh axial mesh) by `method` instead of the direct z
+        transform; hz None switches it off."""
+        L = _hz_bind(self.L)
+        hh = hz.h if hz is not None else None
+        self._check(L.vrh_set_mode_solver(self.h, hh, HZ_METHOD[method], int(mmin), float(tol), int(maxit)),
+                    "set_mode_solver")
+
+    def mode_solver_stats(self):
+        """(Krylov iterations summed over slots, mode solves) since set_mode_solver."""
+        it, ns = ctypes.c_longlong(0), ctypes.c_longlong(0)
+        self._check(_hz_bind(self.L).vrh_mode_solver_stats(self.h, ctypes.byref(it), ctypes.byref(ns)),
+                    "mode_solver_stats")
+        return int(it.value), int(ns.value)
+
     def set_axial_mesh(self, zm, zc, dmz, dcz):
         """Non-uniform axial mesh: zm, zc, dmz, dcz over k = 0..nz+1 (cell
         centres, faces with zc[1] = 0, and dz/dzeta at centres and faces)."""
@@ -423,7 +440,7 @@ class VrhHzGrid(ctypes.Structure):
 HZ_EXPORTS = ["vrh_hz_create", "vrh_hz_destroy", "vrh_hz_last_error", "vrh_hz_set_modes", "vrh_hz_mesh",
               "vrh_hz_apply", "vrh_hz_direct", "vrh_hz_solve", "vrh_hz_net_size", "vrh_hz_net_init",
               "vrh_hz_net_get", "vrh_hz_net_set", "vrh_hz_net_forward", "vrh_hz_net_grad", "vrh_hz_sample",
-              "vrh_hz_train"]
+              "vrh_hz_train", "vrh_set_mode_solver", "vrh_mode_solver_stats"]
 HZ_METHOD = {"pcg": 0, "bicgstab": 1, "learned": 2, "learned_fcg": 3}
 
 
@@ -451,6 +468,8 @@ def _hz_bind(L):
     L.vrh_hz_net_grad.argtypes = [P, dp, fp, fp]
     L.vrh_hz_sample.argtypes = [P, U, I, dp]
     L.vrh_hz_train.argtypes = [P, I, D, U, fp]
+    L.vrh_set_mode_solver.argtypes = [P, P, I, I, D, I]
+    L.vrh_mode_solver_stats.argtypes = [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     L._hz_bound = True
     return L
 
