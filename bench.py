@@ -317,6 +317,88 @@ def run_learned(nr, nz, nslots, train_samples, seed=0, pool=8):
     return out
 
 
+def tanh_axial_mesh(nz, zmax, s):
+    """Axial mesh with faces z_k = zmax (1 + tanh(s (k/nz - 1)) / tanh s),
+    as vrh_set_axial_mesh takes it: centres, faces, dz/dzeta at centres and
+    faces over k = 0..nz+1, mirrored ghost cells."""
+    k = np.arange(nz + 1)
+    zf = zmax * (1.0 + np.tanh(s * (k / nz - 1.0)) / np.tanh(s))
+    zf[0], zf[-1] = 0.0, zmax
+    zc = np.concatenate([[-zf[1]], zf])
+    zm = np.zeros(nz + 2)
+    zm[1:nz + 1] = 0.5 * (zf[1:] + zf[:-1])
+    zm[0], zm[nz + 1] = -zm[1], 2.0 * zmax - zm[nz]
+    dz = zmax / nz
+    dmz = np.zeros(nz + 2)
+    dmz[1:nz + 1] = (zf[1:] - zf[:-1]) / dz
+    dmz[0], dmz[nz + 1] = dmz[1], dmz[nz]
+    dcz = np.zeros(nz + 2)
+    dcz[1:] = (zm[1:] - zm[:-1]) / dz
+    dcz[0] = dcz[1]
+    return zm, zc, dmz, dcz
+
+
+def run_learned_step(train_samples, steps=2, warmup=1, stretch=2.0, mmin=3, tol=1e-8, seed=0, pool=8,
+                     methods=("pcg", "learned", "learned_fcg")):
+    """The learned form inside the time step: the workload's grid with the
+    tanh-stretched axial mesh (stretch 2, the per-mode Helmholtz grid), mixed
+    precision, the pressure modes m >= mmin solved per mode to relative
+    residual `tol` by plain PCG, PCG from the learned warm start, or the
+    learned flexible CG (network trained on `train_samples` pairs), against
+    the direct eigen-transform solve of the same step.  Times are whole
+    steps (eager launches: the axial mesh's z transforms are rocBLAS GEMMs);
+    errors are against the direct mixed and FP64 runs after the same steps."""
+    import vring_amd
+    q, dt = ring_state()
+    dt = dt / 8.0  # the first cells are 7x thinner than the uniform grid's
+    mesh = tanh_axial_mesh(NZ, ZMAX, stretch)
+    ncell = NT * NR * NZ
+    nslots = NT - (2 * mmin - 1)
+    hz = vring_amd.Helmholtz(NR, NZ, nslots, RMAX, ZMAX, stretch, pool=pool)
+    hz.net_init(seed)
+    t0 = time.perf_counter()
+    loss = hz.train(train_samples, lr=1e-3, seed=seed)
+    out = {"workload": f"nr={NR} ntheta={NT} nz={NZ} Re={RE:g}, tanh axial mesh stretch {stretch:g}",
+           "modes_iterative": f"m >= {mmin} ({nslots} theta positions)", "tol": tol, "dt": dt,
+           "train_samples": train_samples, "train_s": round(time.perf_counter() - t0, 2),
+           "loss_first": float(loss[:4].mean()), "loss_last": float(loss[-4:].mean())}
+    fields = {}
+    runs = [("fp64_direct", "fp64", None), ("direct", "mixed", None)] + [(m, "mixed", m) for m in methods]
+    for name, prec, method in runs:
+        s = vring_amd.Solver(NT, NR, NZ, RMAX, ZMAX, RE, dt, precision=prec)
+        s.set_axial_mesh(*mesh)
+        if method:
+            s.set_mode_solver(hz, method, mmin, tol, 20000)
+        for n, a in zip(("q1", "q2", "q3"), q):
+            s.upload(n, a)
+        s.set_time(0.5)
+        s.step(warmup)
+        s.synchronize()
+        it0, ns0 = s.mode_solver_stats()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        it1, ns1 = s.mode_solver_stats()
+        fields[name] = {n: s.download(n) for n in ("q1", "q2", "q3", "pres")}
+        row = {"ms_per_step": round(wall / steps * 1e3, 3),
+               "Mcell_RK3_steps_per_s": round(ncell * steps / wall / 1e6, 2)}
+        if method:
+            row["krylov_iterations_per_solve"] = round((it1 - it0) / max(ns1 - ns0, 1), 1)
+            row["iterations_per_slot_solve"] = round((it1 - it0) / max(ns1 - ns0, 1) / nslots, 1)
+        if name != "fp64_direct":
+            row["err_vs_mixed_direct"] = rel_errors(fields[name], fields["direct"]) if name != "direct" else None
+            row["err_vs_fp64_direct"] = rel_errors(fields[name], fields["fp64_direct"])
+        out[name] = row
+        s.close()
+    hz.close()
+    if "learned" in out:
+        out["learned_over_direct"] = round(out["direct"]["ms_per_step"] / out["learned"]["ms_per_step"], 4)
+        if "pcg" in out:
+            out["learned_over_pcg"] = round(out["pcg"]["ms_per_step"] / out["learned"]["ms_per_step"], 4)
+    return out
+
+
 def error_run(q, dt, steps=500, every=50):
     """Mixed against FP64 from the same initial field over `steps` steps
     (the workload's run length): per field, the largest error over the
@@ -530,6 +612,10 @@ def main():
                     help="run only the large workload (512x256x1024, two rings, 16M particles)")
     ap.add_argument("--learned", type=int, default=0, metavar="SAMPLES",
                     help="run only the learned Helmholtz benchmark, training on SAMPLES pairs")
+    ap.add_argument("--learned-train", type=int, default=4096, metavar="SAMPLES",
+                    help="training pairs of the learned-in-the-step measurement of the default run (0: skip it)")
+    ap.add_argument("--learned-step", type=int, default=0, metavar="SAMPLES",
+                    help="run only the learned-in-the-step benchmark (tanh axial mesh), training on SAMPLES pairs")
     ap.add_argument("--summarize-traffic", metavar="PMC_DIR", default=None,
                     help="write profiles/traffic_256x128x512.json from rocprofv3 --pmc output under PMC_DIR")
     ap.add_argument("--stats-title", default="workload 256x128x512 ring (bench.py)",
@@ -560,6 +646,9 @@ def main():
     if args.learned:
         print(json.dumps({"learned": run_learned(256, 512, 64, args.learned)}))
         return
+    if args.learned_step:
+        print(json.dumps({"learned_step": run_learned_step(args.learned_step, args.steps, args.warmup)}))
+        return
     q, dt = ring_state()
     ncell = NT * NR * NZ
     if args.only == "particles":
@@ -583,6 +672,7 @@ def main():
     v64 = ncell * args.steps / wall64 / 1e6
     vmx = ncell * args.steps / wallmx / 1e6
     err = error_run(q, dt, args.error_steps) if args.error_steps else {"timed_run": rel_errors(fmx, f64)}
+    lstep = run_learned_step(args.learned_train, 1, 1, methods=("pcg", "learned")) if args.learned_train else None
     rf = roofline(ktmx, "mixed", ncell)
     rf64 = roofline(kt64, "fp64", ncell)
     step_frac64 = PASSES_PER_STEP * 8 * ncell * (args.steps / wall64) / (HBM_PEAK_GBS * 1e9)
@@ -606,8 +696,9 @@ def main():
         "dtype": "f32+f64 (mixed)",
         "form": ("mixed precision: FP32 stencils, ADI sweeps, theta FFT, z DCT and radial solves of m >= 3; "
                  "FP64 state, predictor, m = 0..2 pressure modes and the projection with the pressure "
-                 "gradient.  No learned solve in the step: on this uniform z grid the pressure solve is the "
-                 "direct FFT + DCT + tridiagonal one, exact and faster than any Krylov form"),
+                 "gradient.  No learned solve in the timed step: on this uniform z grid the pressure solve is "
+                 "the direct FFT + DCT + tridiagonal one; the learned form runs on the tanh-stretched axial "
+                 "mesh and is measured beside it as learned_in_step (a loss)"),
         "data": "synthetic: seeded Gaussian ring with m=2..8 core perturbations",
         "config": {"workload": "nr=256 ntheta=128 nz=512 Re=3000 ring, 1 GPU", "nr": NR, "ntheta": NT,
                    "nz": NZ, "dt": dt, "particles": args.particles},
@@ -620,6 +711,7 @@ def main():
         "fp64_kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt64.items()},
         "autotune": {"fp64": tune64, "mixed": tunemx},
         "particles": parts,
+        "learned_in_step": lstep,
         "roofline": rf,
     }
     if not args.no_cpu_baseline:
