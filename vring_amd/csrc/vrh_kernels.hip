@@ -1452,8 +1452,9 @@ __global__ __launch_bounds__(1024) void k_dctz(TW *__restrict__ S, long nl, int 
 // Radial tridiagonal solves of every (mode, kz) system (solve_radial,
 // poisson_solve.F90:270-323) with precomputed inverse pivots ib; lanes over
 // the packed mode index (contiguous), P waves of M rows each.
+// FP32: at most 64 VGPRs, so that two 1024-lane blocks share a CU.
 template <typename T, int M>
-__global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_per_eu(1, 4))) void k_radial(T *__restrict__ S, const T *__restrict__ ib, long nm, long nlines,
+__global__ __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 1) void k_radial(T *__restrict__ S, const T *__restrict__ ib, long nm, long nlines,
                                                  int nr, const double *__restrict__ aab, const double *__restrict__ ccb) {
   __shared__ T sh_v[16][64];
   __shared__ T sh_p[16][64];
