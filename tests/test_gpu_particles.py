@@ -200,3 +200,46 @@ def test_inlet_batches_inside_whole_steps_match_oracle():
     assert total > 100
     s.close()
     o.close()
+
+
+@pytest.mark.parametrize("grid,n,sample", [((32, 64, 128), 10_000, None), ((128, 256, 512), 1_000_000, 2000)])
+def test_lsq_particles_at_workload_sizes_match_oracle(grid, n, sample):
+    """The particle counts of the parity workload (10^4 on 32x64x128, all
+    compared) and of the headline workload (10^6 on 128x256x512): particles
+    are independent given the field, so the oracle steps a random sample of
+    them from the same initial states and the device's results for those
+    indices must agree (LSQ-27 + Euler, five substeps, FP64)."""
+    nt, nr, nz = grid
+    print(f"case particles at size {grid} n={n} sample={sample}", flush=True)
+    rmax, zmax, dt, nsub = 4.0, 12.0, 0.01, 5
+    q = vring_amd.ring_initial_condition(nt, nr, nz, rmax, zmax, [(1.0, 3.0, 0.2, 1.0)], pert_amp=1e-3,
+                                         pert_modes=(2, 8), phases=np.arange(7.0))
+    s = vring_amd.Solver(nt, nr, nz, rmax, zmax, 3000.0, dt)
+    for name, a in zip(("q1", "q2", "q3"), q):
+        s.upload(name, a)
+    st = stokes_number(20e-6 / 0.02, 1000.0)
+    u = np.zeros((n, 117))
+    u[:, :3] = torus_points(n, 1.0, 3.0, 0.6, 7)
+    u[:, [6, 10, 14]] = 1.0
+    s.particles_create([n], [st], [nsub], "lsq_euler")
+    s.particles_upload(0, u)
+    s.particles_step(1)
+    g, gal = s.particles_download(0, n)
+    s.close()
+    idx = np.arange(n) if sample is None else np.sort(np.random.default_rng(11).choice(n, sample, replace=False))
+    o = Oracle(nt, nr, nz, rmax, zmax, 3000.0, dt)
+    for name, a in zip(("q1", "q2", "q3"), q):
+        o.field(name)[...] = a
+    m = idx.size
+    uo = np.ascontiguousarray(u[idx])
+    d1, d2, alive = np.zeros((m, 117)), np.zeros((m, 117)), np.ones(m, np.int32)
+    for _ in range(nsub):
+        assert o.L.orc_part_advance(o.h, 0, 0, 1, st, dt / nsub, 78, m, uo.ravel(), d1.ravel(), d2.ravel(),
+                                    alive) == 0
+    o.close()
+    np.testing.assert_array_equal(gal[idx], alive)
+    err = np.abs(g[idx] - uo).max() / np.abs(uo).max()
+    moved = np.abs(g[idx, :3] - u[idx, :3]).max()
+    print(f"particle rel err {err:.3e} on {m} particles; max displacement {moved:.3e}", flush=True)
+    assert moved > 1e-6
+    assert err < 1e-10
