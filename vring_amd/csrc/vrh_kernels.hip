@@ -1318,11 +1318,22 @@ __global__ __launch_bounds__(1024) void k_dctz(TW *__restrict__ S, long nl, int 
   const int sn = n + 1;
   TW *im = re + (size_t)(TL / 2) * sn;
   TW *twl = im + (size_t)(TL / 2) * sn;  // n/2 cos, n/2 sin
+  // Makhoul post/pre-twiddles e^{i pi k/2n}: in FP32 an LDS copy (n cos,
+  // n sin) spares the per-element global loads; FP64 reads them from global
+  // memory (the LDS would cost the tile lines)
+  constexpr bool lds_tw = sizeof(TW) == 4;
+  TW *etw = twl + n;
+  auto ecos = [&](int k) { return lds_tw ? etw[k] : (TW)er[k]; };
+  auto esin = [&](int k) { return lds_tw ? etw[n + k] : (TW)ei[k]; };
   const int tid = threadIdx.x, nth = blockDim.x;
   const long l0 = xcd_tile(blockIdx.x, gridDim.x) * TL;
   const bool pow2 = n >= 2 && (n & (n - 1)) == 0;
   if (pow2) {
     for (int q = tid; q < n / 2; q += nth) { twl[q] = (TW)twr[q]; twl[n / 2 + q] = (TW)twi[q]; }
+    if (lds_tw) {
+      for (int q = tid; q < n; q += nth) { etw[q] = (TW)er[q]; etw[n + q] = (TW)ei[q]; }
+      if (dir < 0) __syncthreads();  // the inverse's pre-twiddle reads the table below
+    }
     // batches of elements: every load of a batch (line index clamped, then
     // masked; twiddles included) is issued before the first is used
     if (dir > 0) {
@@ -1353,8 +1364,7 @@ __global__ __launch_bounds__(1024) void k_dctz(TW *__restrict__ S, long nl, int 
       constexpr int U = 4;
       const int ltq = logtl - 1, TQ = TL >> 1, tot = TQ * n;
       for (int eb = tid; eb < tot; eb += U * nth) {
-        TW ya[U], yb[U], yan[U], ybn[U];
-        double cr[U], ci[U];
+        TW ya[U], yb[U], yan[U], ybn[U], cr[U], ci[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
           const int e = min(eb + u * nth, tot - 1);
@@ -1366,8 +1376,8 @@ __global__ __launch_bounds__(1024) void k_dctz(TW *__restrict__ S, long nl, int 
           yb[u] = S[cb + nl * k];
           yan[u] = S[ca + nl * kn];
           ybn[u] = S[cb + nl * kn];
-          cr[u] = er[k];  // e^{+i pi k / 2n}
-          ci[u] = ei[k];
+          cr[u] = ecos(k);  // e^{+i pi k / 2n}
+          ci[u] = esin(k);
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1379,7 +1389,7 @@ __global__ __launch_bounds__(1024) void k_dctz(TW *__restrict__ S, long nl, int 
             const TW a0 = (la < nl) ? ya[u] : (TW)0, b0 = (lb < nl) ? yb[u] : (TW)0;
             const TW an = (k > 0 && la < nl) ? yan[u] : (TW)0, bn = (k > 0 && lb < nl) ? ybn[u] : (TW)0;
             const TW uu = a0 + bn, w = b0 - an;
-            const TW c = (TW)cr[u], sv = (TW)ci[u];
+            const TW c = cr[u], sv = ci[u];
             const int slot = q * sn + k;
             re[slot] = uu * c - w * sv;
             im[slot] = uu * sv + w * c;
@@ -1392,13 +1402,13 @@ __global__ __launch_bounds__(1024) void k_dctz(TW *__restrict__ S, long nl, int 
     const TW inv_n = (TW)1 / (TW)n;  // exact: n is a power of two
     constexpr int US = 4;
     for (int eb = tid; eb < TL * n; eb += US * nth) {
-      double cr[US], ci[US];
+      TW cr[US], ci[US];
       if (dir > 0) {
 #pragma unroll
         for (int u = 0; u < US; ++u) {
           const int kz = min(eb + u * nth, TL * n - 1) >> logtl;
-          cr[u] = er[kz];
-          ci[u] = ei[kz];
+          cr[u] = ecos(kz);
+          ci[u] = esin(kz);
         }
       }
 #pragma unroll
@@ -1416,7 +1426,7 @@ __global__ __launch_bounds__(1024) void k_dctz(TW *__restrict__ S, long nl, int 
           TW vr, vi;
           if ((lp & 1) == 0) { vr = (TW)0.5 * (zr + wr); vi = (TW)0.5 * (zi - wi); }
           else { vr = (TW)0.5 * (zi + wi); vi = (TW)(-0.5) * (zr - wr); }
-          outv = (vr * (TW)cr[u] + vi * (TW)ci[u]) * inv_n;  // Re(e^{-i pi k/2n} V)/n
+          outv = (vr * cr[u] + vi * ci[u]) * inv_n;  // Re(e^{-i pi k/2n} V)/n
         } else {
           const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
           outv = (lp & 1) ? im[q * sn + p] : re[q * sn + p];
@@ -2251,7 +2261,7 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
     const size_t lim = want > 0 ? (size_t)h->lds_block : 65536;
     const int tmax = want > 0 ? 32 : 16;
     int tl = 2;
-    while (tl * 2 <= tmax && (size_t)(tl * 2) * (nz + 1) * es + (size_t)nz * es <= lim &&
+    while (tl * 2 <= tmax && (size_t)(tl * 2) * (nz + 1) * es + (size_t)(es == 4 ? 3 : 1) * nz * es <= lim &&
            (tl * 2 / 2) * nz / fft_min_radix(lz) <= 1024)
       tl *= 2;
     if (want > 0) while (tl > 2 && tl > want) tl /= 2;
@@ -2265,7 +2275,9 @@ template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_
   const int ltl = ilog2(TL), ltl64 = ilog2(TL64);
   const int NTD = threads(TL), NTD64 = threads(TL64);
   const long nl = (long)N * g.nr;
-  const size_t sh2 = sizeof(TW) * ((size_t)TL * (nz + 1) + nz), sh64 = sizeof(double) * ((size_t)TL64 * (nz + 1) + nz);
+  // tile lines, FFT twiddles, and in FP32 the Makhoul twiddles
+  const size_t sh2 = sizeof(TW) * ((size_t)TL * (nz + 1) + (size_t)(sizeof(TW) == 4 ? 3 : 1) * nz),
+               sh64 = sizeof(double) * ((size_t)TL64 * (nz + 1) + nz);
   // The FP64 low-mode chain (mixed precision) reads only what k_divfft
   // wrote to S64 and feeds only k_ifft: it runs on a side stream, forked
   // after k_divfft and joined before k_ifft, beside the FP32 transforms.
