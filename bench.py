@@ -300,6 +300,21 @@ def run_learned(nr, nz, nslots, train_samples, seed=0, pool=8):
         out[m] = {"iterations_total": int(it.sum()), "iterations_max": int(it.max()), "ms": round(ms, 3),
                   "max_true_relres": float(rel.max()),
                   "max_err_vs_direct": float(np.abs(p - pd).max() / np.abs(pd).max())}
+    # the same network trained on the residuals of up to 256 plain PCG
+    # iterations (the inputs a preconditioner meets), as the flexible CG's
+    # preconditioner
+    w_rhs = hz.net_get()
+    hz.net_init(seed)
+    t0 = time.perf_counter()
+    lr_ = hz.train_residuals(train_samples, lr=1e-3, seed=seed, max_cg=256)
+    hz.set_modes(np.arange(nslots, dtype=np.int32))
+    p, it, rel, ms = hz.solve(f, "learned_fcg", tol=1e-8, maxit=5000)
+    out["learned_fcg_residual_trained"] = {
+        "train_s": round(time.perf_counter() - t0 - ms / 1e3, 2), "loss_first": float(lr_[:4].mean()),
+        "loss_last": float(lr_[-4:].mean()), "iterations_total": int(it.sum()), "iterations_max": int(it.max()),
+        "ms": round(ms, 3), "max_true_relres": float(rel.max()),
+        "max_err_vs_direct": float(np.abs(p - pd).max() / np.abs(pd).max())}
+    hz.net_set(w_rhs)
     # loose tolerances (inner solves that need only a few digits)
     for tol in (1e-2, 1e-4):
         row = {}

@@ -100,6 +100,12 @@ int vrh_hz_sample(vrh_hz_t *h, unsigned long long seed, int family, double *f);
  * divided by the cell volume, as the solver sees residuals).  loss
  * (optional) receives the mean loss of each batch. */
 int vrh_hz_train(vrh_hz_t *h, int nsamples, double lr, unsigned long long seed, float *loss);
+/* The same training on the residuals a preconditioner meets: each batch's
+ * right-hand sides are replaced by their residuals after k plain PCG
+ * iterations from zero, k cycling through 0, 1, 4, 16, ... <= max_cg over the
+ * batches; the target is the exact solve of that residual. */
+int vrh_hz_train_residuals(vrh_hz_t *h, int nsamples, double lr, unsigned long long seed, int max_cg,
+                           float *loss);
 
 /* ---- in the time step.  The pressure solve of a stepper handle h with an
  * axial mesh equal to hz's tanh mesh (vrh_set_axial_mesh), the uniform

@@ -1306,6 +1306,58 @@ int true_residual(vrh_hz_t *h, const double *rhs, const double *x, std::vector<d
 
 }  // namespace
 
+namespace {
+// Training batches: each draws right-hand sides and random modes, and (with
+// max_cg > 0) replaces each right-hand side by the residual after k plain PCG
+// iterations from zero, k cycling through 0, 1, 4, 16, ... <= max_cg over the
+// batches, so that the network sees the residuals a preconditioner is
+// applied to; the target is the exact solve of whatever it sees.
+int train_batches(vrh_hz_t *h, int nsamples, double lr, unsigned long long seed, int max_cg, float *loss) {
+  const int nb = (nsamples + h->S - 1) / h->S;
+  if (nb > (1 << 16)) return hfail(h, "too many batches");
+  std::vector<int> saved(h->hm);
+  Geo g = h->geo();
+  double *f = h->vec[0], *rhs = h->vec[1], *pt = h->vec[11];
+  std::vector<int> levels(1, 0);
+  for (int k = 1; k <= max_cg; k *= 4) levels.push_back(k);
+  for (int it = 0; it < nb; it++) {
+    const unsigned long long sd = seed * 1000003ull + (unsigned long long)it;
+    hipLaunchKernelGGL(k_random_modes, dim3((h->S + 63) / 64), dim3(64), 0, h->stream, h->S, sd, 63, h->d_m);
+    hipLaunchKernelGGL(k_sample, red_grid(h), dim3(256), 0, h->stream, g, sd, -1, h->rmax, h->zmax, h->d_zm, f);
+    launch_rhs(h, f, rhs);
+    const double *src = rhs;
+    const int kcg = levels[(size_t)it % levels.size()];
+    if (kcg > 0) {
+      std::vector<int> its;
+      HZCHK(h, hipMemsetAsync(h->vec[10], 0, h->ncell() * sizeof(double), h->stream));
+      int rc = pcg_dev(h, rhs, h->vec[10], 1e-300, kcg, its);
+      if (rc) return rc;
+      src = h->vec[2];  // the residual after kcg iterations
+    }
+    int rc = direct_dev(h, src, pt);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_den, red_grid(h), dim3(256), 0, h->stream, g, pt, src, h->d_part);
+    hipLaunchKernelGGL(k_store_den, dim3((h->S + 63) / 64), dim3(64), 0, h->stream, h->S, h->d_part, h->d_sc);
+    net_galerkin(h, src, nullptr);
+    hipLaunchKernelGGL(k_loss, red_grid(h), dim3(256), 0, h->stream, g, h->pool, h->d_out, pt, src, h->d_sc,
+                       h->vec[9], h->d_part);
+    hipLaunchKernelGGL(k_pool_grad, red_grid(h), dim3(256), 0, h->stream, g, h->pool, h->vec[9], h->d_g4);
+    hipLaunchKernelGGL(k_batch_loss, dim3(1), dim3(64), 0, h->stream, h->S, h->d_part, h->d_sc, h->d_loss + it);
+    net_backward(h);
+    h->adam_t++;
+    const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)h->adam_t)));
+    const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)h->adam_t)));
+    hipLaunchKernelGGL(k_adam, dim3((h->nparam + 255) / 256), dim3(256), 0, h->stream, h->d_w, h->d_g, h->d_m1,
+                       h->d_m2, h->nparam, (float)lr, c1, c2);
+    pack_weights(h);
+    HZCHK(h, hipGetLastError());
+  }
+  if (loss) HZCHK(h, hipMemcpyAsync(loss, h->d_loss, nb * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  HZCHK(h, hipStreamSynchronize(h->stream));
+  return vrh_hz_set_modes(h, saved.data());
+}
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // In-step use by the stepper's pressure solve (vrh_set_mode_solver): the
 // stepper writes each slot's right-hand side f (the mode's divergence, as the
@@ -1653,37 +1705,12 @@ int vrh_hz_sample(vrh_hz_t *h, unsigned long long seed, int family, double *f) {
 
 int vrh_hz_train(vrh_hz_t *h, int nsamples, double lr, unsigned long long seed, float *loss) {
   if (!h || nsamples < 1 || lr <= 0) return -1;
-  const int nb = (nsamples + h->S - 1) / h->S;
-  if (nb > (1 << 16)) return hfail(h, "too many batches");
-  std::vector<int> saved(h->hm);
-  Geo g = h->geo();
-  double *f = h->vec[0], *rhs = h->vec[1], *pt = h->vec[11];
-  for (int it = 0; it < nb; it++) {
-    const unsigned long long sd = seed * 1000003ull + (unsigned long long)it;
-    hipLaunchKernelGGL(k_random_modes, dim3((h->S + 63) / 64), dim3(64), 0, h->stream, h->S, sd, 63, h->d_m);
-    hipLaunchKernelGGL(k_sample, red_grid(h), dim3(256), 0, h->stream, g, sd, -1, h->rmax, h->zmax, h->d_zm, f);
-    launch_rhs(h, f, rhs);
-    int rc = direct_dev(h, rhs, pt);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_den, red_grid(h), dim3(256), 0, h->stream, g, pt, rhs, h->d_part);
-    hipLaunchKernelGGL(k_store_den, dim3((h->S + 63) / 64), dim3(64), 0, h->stream, h->S, h->d_part, h->d_sc);
-    net_galerkin(h, rhs, nullptr);
-    hipLaunchKernelGGL(k_loss, red_grid(h), dim3(256), 0, h->stream, g, h->pool, h->d_out, pt, rhs, h->d_sc,
-                       h->vec[9], h->d_part);
-    hipLaunchKernelGGL(k_pool_grad, red_grid(h), dim3(256), 0, h->stream, g, h->pool, h->vec[9], h->d_g4);
-    hipLaunchKernelGGL(k_batch_loss, dim3(1), dim3(64), 0, h->stream, h->S, h->d_part, h->d_sc, h->d_loss + it);
-    net_backward(h);
-    h->adam_t++;
-    const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)h->adam_t)));
-    const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)h->adam_t)));
-    hipLaunchKernelGGL(k_adam, dim3((h->nparam + 255) / 256), dim3(256), 0, h->stream, h->d_w, h->d_g, h->d_m1,
-                       h->d_m2, h->nparam, (float)lr, c1, c2);
-    pack_weights(h);
-    HZCHK(h, hipGetLastError());
-  }
-  if (loss) HZCHK(h, hipMemcpyAsync(loss, h->d_loss, nb * sizeof(float), hipMemcpyDeviceToHost, h->stream));
-  HZCHK(h, hipStreamSynchronize(h->stream));
-  return vrh_hz_set_modes(h, saved.data());
+  return train_batches(h, nsamples, lr, seed, 0, loss);
+}
+
+int vrh_hz_train_residuals(vrh_hz_t *h, int nsamples, double lr, unsigned long long seed, int max_cg, float *loss) {
+  if (!h || nsamples < 1 || lr <= 0 || max_cg < 0) return -1;
+  return train_batches(h, nsamples, lr, seed, max_cg, loss);
 }
 
 }  // extern "C"

@@ -440,7 +440,7 @@ class VrhHzGrid(ctypes.Structure):
 HZ_EXPORTS = ["vrh_hz_create", "vrh_hz_destroy", "vrh_hz_last_error", "vrh_hz_set_modes", "vrh_hz_mesh",
               "vrh_hz_apply", "vrh_hz_direct", "vrh_hz_solve", "vrh_hz_net_size", "vrh_hz_net_init",
               "vrh_hz_net_get", "vrh_hz_net_set", "vrh_hz_net_forward", "vrh_hz_net_grad", "vrh_hz_sample",
-              "vrh_hz_train", "vrh_set_mode_solver", "vrh_mode_solver_stats"]
+              "vrh_hz_train", "vrh_hz_train_residuals", "vrh_set_mode_solver", "vrh_mode_solver_stats"]
 HZ_METHOD = {"pcg": 0, "bicgstab": 1, "learned": 2, "learned_fcg": 3}
 
 
@@ -468,6 +468,7 @@ def _hz_bind(L):
     L.vrh_hz_net_grad.argtypes = [P, dp, fp, fp]
     L.vrh_hz_sample.argtypes = [P, U, I, dp]
     L.vrh_hz_train.argtypes = [P, I, D, U, fp]
+    L.vrh_hz_train_residuals.argtypes = [P, I, D, U, I, fp]
     L.vrh_set_mode_solver.argtypes = [P, P, I, I, D, I]
     L.vrh_mode_solver_stats.argtypes = [P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]
     L._hz_bound = True
@@ -571,4 +572,13 @@ class Helmholtz:
         nb = (nsamples + self.S - 1) // self.S
         loss = np.zeros(nb, np.float32)
         self._check(self.L.vrh_hz_train(self.h, int(nsamples), float(lr), int(seed), loss), "train")
+        return loss
+
+    def train_residuals(self, nsamples, lr=1e-3, seed=0, max_cg=256):
+        """Training on the residuals of 0, 1, 4, 16, ... <= max_cg plain PCG
+        iterations (the inputs a preconditioner is applied to)."""
+        nb = (nsamples + self.S - 1) // self.S
+        loss = np.zeros(nb, np.float32)
+        self._check(self.L.vrh_hz_train_residuals(self.h, int(nsamples), float(lr), int(seed), int(max_cg), loss),
+                    "train_residuals")
         return loss
