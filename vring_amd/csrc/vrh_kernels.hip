@@ -99,7 +99,7 @@ template <typename T> __device__ __forceinline__ T zd2f(const ZMet &z, int k, T 
   return (fp1 - f0) * (T)z.fp[k] - (f0 - fm1) * (T)z.fm[k];
 }
 
-template <typename TW, typename TC>
+template <typename TW, typename TC, bool ZS>
 __global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march(VGrid g, VCoef c, VRK rk, ZMet zm, double betadx, int nlev, int kchunk,
                                                   const double *__restrict__ q1, const double *__restrict__ q2,
                                                   const double *__restrict__ q3, const double *__restrict__ prg1,
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march
     }
     TC v1 = (TC)0, v2 = (TC)0, v3 = (TC)0;
     const long o = (long)ti + (long)j * sj + (long)k * sk;
-    const bool zs = zm.m3 != nullptr;  // axial mesh: per-plane metric
+    constexpr bool zs = ZS;  // axial mesh: per-plane metric (a separate instantiation)
     const TC zmk = zs ? (TC)zm.m3[k] : dx3, zck = zs ? (TC)zm.c3[k] : dx3;
     if (cell) {
       const TC *A1m = ring + (sM * 3 + 0) * PL, *A1 = ring + (s0 * 3 + 0) * PL, *A1p = ring + (sP * 3 + 0) * PL;
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march
 // PCR inside the wavefront (LDS exchange ordered by wave-local fences, no
 // block barrier per level), which leaves two block barriers per z plane for
 // the plane ring.  Same arithmetic as k_explicit_march.
-template <typename TW, typename TC, int CPL, int RJT, int WPE>
+template <typename TW, typename TC, int CPL, int RJT, int WPE, bool ZS>
 __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef c, VRK rk, ZMet zm, double betadx, int nlev, int kchunk,
                                                   const double *__restrict__ q1, const double *__restrict__ q2,
                                                   const double *__restrict__ q3, const double *__restrict__ prg1,
@@ -543,7 +543,7 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
       const TC c_ru1 = cru[cc][0], c_ru2 = cru[cc][1], c_ru3 = cru[cc][2];
       const TC c_pg1 = cpg[cc][0], c_pg2 = cpg[cc][1], c_pg3 = cpg[cc][2];
       TC v1 = (TC)0, v2 = (TC)0, v3 = (TC)0;
-      const bool zs = zm.m3 != nullptr;  // axial mesh: per-plane metric
+      constexpr bool zs = ZS;  // axial mesh: per-plane metric (a separate instantiation)
       const TC zmk = zs ? (TC)zm.m3[k] : dx3, zck = zs ? (TC)zm.c3[k] : dx3;
       const long o = (long)ti + (long)j * sj + (long)k * sk;
       if (cell) {
@@ -708,7 +708,7 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
 #undef irmm
 }
 
-template <typename TW, typename TC>
+template <typename TW, typename TC, bool ZS>
 __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk, ZMet zm, double betadx, int nlev,
                                                    const double *__restrict__ q1, const double *__restrict__ q2,
                                                    const double *__restrict__ q3, const double *__restrict__ prg1,
@@ -730,7 +730,7 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
   const long dip = ip - i, dim = im - i, sj = g.sj, sk = g.sk, djm = (long)(jm - j) * sj;
   const TC ga = (TC)rk.g, ro = (TC)rk.r, al = (TC)rk.a, dt = (TC)g.dt, aRe = (TC)(rk.a / g.Re);
   const TC dx3 = (TC)g.dx3, dx3q = (TC)g.dx3q;
-  const bool zs = zm.m3 != nullptr;  // axial mesh: per-plane metric
+  constexpr bool zs = ZS;  // axial mesh: per-plane metric (a separate instantiation)
   const TC zmk = zs ? (TC)zm.m3[k] : dx3, zck = zs ? (TC)zm.c3[k] : dx3;
   const long pc = cidx(g, i, j, k);
 #define Q1(d) ((TC)q1[o + (d)])
@@ -1670,7 +1670,7 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
 // them.  Everything that depends on the thread's r row only (metric ratios,
 // reciprocals of rm) is hoisted out of the z loop.
 // ---------------------------------------------------------------------------
-template <typename TW, int R>
+template <typename TW, int R, bool ZS>
 __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, ZMet zm, double adt, double be, int kchunk,
                                                     const double *__restrict__ dph, const double *__restrict__ dq1,
                                                     const double *__restrict__ dq2, const double *__restrict__ dq3,
@@ -1781,7 +1781,7 @@ __global__ __launch_bounds__(64 * R) void k_project(VGrid g, VCoef c, ZMet zm, d
       const double p0 = P0[cy][cx];
       // axial mesh: the z gradient at faces k, k+1, k-1 and the z-gradient
       // factor of upsol; the uniform mesh keeps dx3
-      const bool zs = zm.m3 != nullptr;
+      constexpr bool zs = ZS;  // axial mesh (a separate instantiation)
       const double zc0 = zs ? zm.c3[k] : dx3, zcp = zs ? zm.c3[k + 1] : dx3, zcm = zs ? zm.c3[k - 1] : dx3;
       const double a3k = zs ? adt * zc0 : a3;
       // ---- upsol ----
@@ -1976,7 +1976,8 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
     if (rj < 1) rj = 1;
     dim3 blk(ntp, rj), grd((g.nr + rj - 1) / rj, g.nz);
     const size_t sh = sizeof(TC) * 6 * (size_t)rj * ntp;
-    hipLaunchKernelGGL((k_explicit_cell<TW, TC>), grd, blk, sh, h->stream, g, h->coef, h->rk[irk], h->zmet, betadx, nlev,
+    auto kern = h->zmet.m3 ? k_explicit_cell<TW, TC, true> : k_explicit_cell<TW, TC, false>;
+    hipLaunchKernelGGL(kern, grd, blk, sh, h->stream, g, h->coef, h->rk[irk], h->zmet, betadx, nlev,
                        h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2,
                        h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,
                        (TW *)h->f.r2, (TW *)h->f.r3);
@@ -2022,7 +2023,8 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
     dim3 grd((g.nr + rw - 1) / rw, (g.nz + kchunk - 1) / kchunk);
     bool done = rw > 1 && (var == 2 ? lds(rw) : lds_w(rw)) <= lim;
 #define EXPL_WAVE(C, R, W)                                                                                          \
-  hipLaunchKernelGGL((k_explicit_wave<TW, TC, C, R, W>), grd, dim3(64, R), lds_w(R), h->stream, g, h->coef, h->rk[irk],   \
+  hipLaunchKernelGGL((h->zmet.m3 ? k_explicit_wave<TW, TC, C, R, W, true> : k_explicit_wave<TW, TC, C, R, W, false>), \
+                     grd, dim3(64, R), lds_w(R), h->stream, g, h->coef, h->rk[irk],                                     \
                      h->zmet, betadx, nlev, kchunk, h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2, \
                      h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,          \
                      (TW *)h->f.r2, (TW *)h->f.r3)
@@ -2040,7 +2042,8 @@ template <typename TW> void launch_explicit(vrh_t *h, int irk) {
   }
   while (rj > 1 && lds(rj) > 65536) rj--;
   dim3 blk(ntp, rj), grd((g.nr + rj - 1) / rj, (g.nz + kchunk - 1) / kchunk);
-  hipLaunchKernelGGL((k_explicit_march<TW, TC>), grd, blk, lds(rj), h->stream, g, h->coef, h->rk[irk], h->zmet, betadx, nlev,
+  auto kern = h->zmet.m3 ? k_explicit_march<TW, TC, true> : k_explicit_march<TW, TC, false>;
+  hipLaunchKernelGGL(kern, grd, blk, lds(rj), h->stream, g, h->coef, h->rk[irk], h->zmet, betadx, nlev,
                      kchunk, h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2,
                      h->f.prg3, (TW *)h->f.ru1, (TW *)h->f.ru2, (TW *)h->f.ru3, (TW *)h->f.r1,
                      (TW *)h->f.r2, (TW *)h->f.r3);
@@ -2060,18 +2063,30 @@ void lds_optin(vrh_t *h) {
   bool ok = true;
   if (lb > 65536) {
     const void *fns[] = {
-        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 1, 8, 4>),
-        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 1, 6, 3>),
-        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 6, 3>),
-        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 4, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 2, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 4, 2, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 1, 8, 4>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 1, 6, 3>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 2, 6, 3>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 2, 4, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 2, 2, 2>),
-        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 4, 2, 2>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 1, 8, 4, false>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 1, 6, 3, false>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 6, 3, false>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 4, 2, false>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 2, 2, false>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 4, 2, 2, false>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 1, 8, 4, false>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 1, 6, 3, false>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 2, 6, 3, false>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 2, 4, 2, false>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 2, 2, 2, false>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 4, 2, 2, false>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 1, 8, 4, true>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 1, 6, 3, true>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 6, 3, true>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 4, 2, true>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 2, 2, 2, true>),
+        reinterpret_cast<const void *>(&k_explicit_wave<double, double, 4, 2, 2, true>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 1, 8, 4, true>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 1, 6, 3, true>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 2, 6, 3, true>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 2, 4, 2, true>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 2, 2, 2, true>),
+        reinterpret_cast<const void *>(&k_explicit_wave<float, float, 4, 2, 2, true>),
         reinterpret_cast<const void *>(&k_dctz<float>),
         reinterpret_cast<const void *>(&k_dctz<double>)};
     for (const void *f : fns)
@@ -2351,7 +2366,8 @@ template <typename TW> void launch_project(vrh_t *h, int irk) {
   const int kchunk = 32;
   dim3 blk(64, R), grd((g.nt + 63) / 64, (g.nrc + 1 + R - 1) / R, (g.nzc + kchunk - 1) / kchunk);
   KTimer kt(h, KT_PROJECT);
-  hipLaunchKernelGGL((k_project<TW, R>), grd, blk, 0, h->stream, g, h->coef, h->zmet, adt, be, kchunk, h->f.dph, h->f.dq1,
+  auto kern = h->zmet.m3 ? k_project<TW, R, true> : k_project<TW, R, false>;
+  hipLaunchKernelGGL(kern, grd, blk, 0, h->stream, g, h->coef, h->zmet, adt, be, kchunk, h->f.dph, h->f.dq1,
                      h->f.dq2, h->f.dq3, h->f.q1, h->f.q2, h->f.q3, h->f.prg1, h->f.prg2,
                      h->f.prg3, h->f.pres, h->f.flag);
 }
