@@ -627,6 +627,8 @@ def main():
                     help="run only the large workload (512x256x1024, two rings, 16M particles)")
     ap.add_argument("--learned", type=int, default=0, metavar="SAMPLES",
                     help="run only the learned Helmholtz benchmark, training on SAMPLES pairs")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the 64-ring ensemble and the large workload in the default run")
     ap.add_argument("--learned-train", type=int, default=4096, metavar="SAMPLES",
                     help="training pairs of the learned-in-the-step measurement of the default run (0: skip it)")
     ap.add_argument("--learned-step", type=int, default=0, metavar="SAMPLES",
@@ -688,6 +690,8 @@ def main():
     vmx = ncell * args.steps / wallmx / 1e6
     err = error_run(q, dt, args.error_steps) if args.error_steps else {"timed_run": rel_errors(fmx, f64)}
     lstep = run_learned_step(args.learned_train, 1, 1, methods=("pcg", "learned")) if args.learned_train else None
+    # the other workloads' single-GPU measurements, beside the headline
+    extras = {} if args.no_extras else {"ensemble_64": run_ensemble(64, 10, 2), "large": run_large(10, 2)}
     rf = roofline(ktmx, "mixed", ncell)
     rf64 = roofline(kt64, "fp64", ncell)
     step_frac64 = PASSES_PER_STEP * 8 * ncell * (args.steps / wall64) / (HBM_PEAK_GBS * 1e9)
@@ -727,6 +731,7 @@ def main():
         "autotune": {"fp64": tune64, "mixed": tunemx},
         "particles": parts,
         "learned_in_step": lstep,
+        **extras,
         "roofline": rf,
     }
     if not args.no_cpu_baseline:
