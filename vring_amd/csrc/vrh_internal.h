@@ -162,7 +162,9 @@ struct vrh_s {
   int hz_method = 0, hz_maxit = 0, hz_pos0 = 0;
   double hz_tol = 0.0;
   long long hz_iters = 0, hz_solves = 0;
-  int mode_rc = 0;  // error of the last in-step mode solve (err holds the message)
+  // error of the last in-step mode solve or rocBLAS z transform (err holds
+  // the message), returned by the substep or step that met it
+  int mode_rc = 0;
   VCoef coef;
   // line coefficients: [irk 1..3][sweep 0..5]
   double *d_line;

@@ -2186,23 +2186,32 @@ static void radial_launch(vrh_t *h, T *S, const T *ib, long nm, hipStream_t st) 
 // Forward: out(:, q) = sum_k in(:, k) Vinv(q, k); inverse: out(:, k) =
 // sum_q in(:, q) V(k, q).  The row-major matrices read column-major are
 // their transposes, the right operand of both products.
+// A failed call is reported through the step's error (mode_rc).
+static void zgemm_status(vrh_t *h, rocblas_status st) {
+  if (st != rocblas_status_success && !h->mode_rc) {
+    h->mode_rc = -3;
+    h->err = "rocBLAS z transform failed (status " + std::to_string((int)st) + ")";
+  }
+}
 static void zgemm(vrh_t *h, hipStream_t st, const float *in, float *out, long nl, int dir) {
   const int nz = h->g.nz;
   const float one = 1.f, zero = 0.f;
   rocblas_handle b = (rocblas_handle)h->blas;
-  rocblas_set_stream(b, st);
-  rocblas_sgemm(b, rocblas_operation_none, rocblas_operation_none, (rocblas_int)nl, nz, nz, &one, in,
-                (rocblas_int)nl, h->d_zvf + (dir > 0 ? 0 : (size_t)nz * nz), nz, &zero, out, (rocblas_int)nl);
-  rocblas_set_stream(b, h->stream);
+  zgemm_status(h, rocblas_set_stream(b, st));
+  zgemm_status(h, rocblas_sgemm(b, rocblas_operation_none, rocblas_operation_none, (rocblas_int)nl, nz, nz, &one,
+                                in, (rocblas_int)nl, h->d_zvf + (dir > 0 ? 0 : (size_t)nz * nz), nz, &zero, out,
+                                (rocblas_int)nl));
+  zgemm_status(h, rocblas_set_stream(b, h->stream));
 }
 static void zgemm(vrh_t *h, hipStream_t st, const double *in, double *out, long nl, int dir) {
   const int nz = h->g.nz;
   const double one = 1.0, zero = 0.0;
   rocblas_handle b = (rocblas_handle)h->blas;
-  rocblas_set_stream(b, st);
-  rocblas_dgemm(b, rocblas_operation_none, rocblas_operation_none, (rocblas_int)nl, nz, nz, &one, in,
-                (rocblas_int)nl, h->d_zv + (dir > 0 ? 0 : (size_t)nz * nz), nz, &zero, out, (rocblas_int)nl);
-  rocblas_set_stream(b, h->stream);
+  zgemm_status(h, rocblas_set_stream(b, st));
+  zgemm_status(h, rocblas_dgemm(b, rocblas_operation_none, rocblas_operation_none, (rocblas_int)nl, nz, nz, &one,
+                                in, (rocblas_int)nl, h->d_zv + (dir > 0 ? 0 : (size_t)nz * nz), nz, &zero, out,
+                                (rocblas_int)nl));
+  zgemm_status(h, rocblas_set_stream(b, h->stream));
 }
 
 // Spectral array (position fastest, then r, then z) <-> the Helmholtz
