@@ -130,7 +130,7 @@ def test_host_eigen_transform_uniform_is_the_dct(eig_harness):
     assert np.abs(vi @ vf - np.eye(nz)).max() <= 1e-12
 
 
-@pytest.mark.parametrize("nz,s", [(64, 2.0), (200, 3.0)])
+@pytest.mark.parametrize("nz,s", [(64, 2.0), (200, 3.0), (1151, 3.0)])
 def test_host_eigen_transform_stretched_against_lapack(eig_harness, nz, s):
     m3, c3 = metric(nz, L, s)
     lam, vi, vf = run_eig(eig_harness, m3, c3)
