@@ -17,6 +17,28 @@ namespace {
 
 template <typename T> __device__ __forceinline__ T ldg(const T *p) { return __builtin_nontemporal_load(p); }
 
+// Work-group order across the eight XCDs: the dispatcher hands work group p
+// (linear id) to XCD p mod 8, and each XCD has its own L2.  xcd_tile maps p
+// to a logical tile so that XCD x runs one contiguous range of tiles, in
+// order, one dispatch round apart: neighbouring tiles meet in one L2.  A
+// bijection on [0, nblk) for any nblk.
+__device__ __forceinline__ long xcd_tile(long b, long nblk) {
+  constexpr long kX = 8;
+  const long q = nblk / kX, rem = nblk % kX, x = b % kX, r = b / kX;
+  return x < rem ? x * (q + 1) + r : rem * (q + 1) + (x - rem) * q + r;
+}
+// The block's logical (x, y, z) coordinates of a 2-D or 3-D grid in the
+// order of xcd_tile, x fastest: tiles adjacent in x (whose halo rows
+// overlap) share an L2.
+__device__ __forceinline__ void xcd_block(int &bx, int &by, int &bz) {
+  const long nx = gridDim.x, ny = gridDim.y, nb = nx * ny * (long)gridDim.z;
+  const long l = xcd_tile((long)blockIdx.x + nx * ((long)blockIdx.y + ny * (long)blockIdx.z), nb);
+  bx = (int)(l % nx);
+  const long r = l / nx;
+  by = (int)(r % ny);
+  bz = (int)(r / ny);
+}
+
 // ---------------------------------------------------------------------------
 // hdnbc: outlet convective condition and mass correction, one thread per
 // (theta, r) outlet point (momentum_terms_swirl.F90:225-291).  zm3n: the
@@ -111,8 +133,10 @@ __global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march
   const int NTH = NTP * RJ, tid = tj * NTP + ti;
   const int nt = g.nt, nr = g.nr, nz = g.nz;
   const long sj = g.sj, sk = g.sk;
-  const int j0 = 1 + blockIdx.x * RJ;
-  const int kb = 1 + blockIdx.y * kchunk, ke = min(kb + kchunk - 1, nz);
+  int bx, by, bz;
+  xcd_block(bx, by, bz);  // r blocks of one z chunk (sharing halo rows) on one XCD
+  const int j0 = 1 + bx * RJ;
+  const int kb = 1 + by * kchunk, ke = min(kb + kchunk - 1, nz);
   const int LRW = RJ + 2, PL = LRW * NTP;
   double *pcr = reinterpret_cast<double *>(smem_raw);             // [RJ][kMaxPcrLevels][2] + [RJ][2]
   double *fin = pcr + RJ * kMaxPcrLevels * 2;
@@ -407,8 +431,10 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
   const int NTH = 64 * RJ, tid = tj * 64 + lane;
   const int nt = g.nt, nr = g.nr, nz = g.nz;
   const long sj = g.sj, sk = g.sk;
-  const int j0 = 1 + blockIdx.x * RJ;
-  const int kb = 1 + blockIdx.y * kchunk, ke = min(kb + kchunk - 1, nz);
+  int bx, by, bz;
+  xcd_block(bx, by, bz);  // r blocks of one z chunk (sharing halo rows) on one XCD
+  const int j0 = 1 + bx * RJ;
+  const int kb = 1 + by * kchunk, ke = min(kb + kchunk - 1, nz);
   const int LRW = RJ + 2, PL = LRW * NTP;
   double *pcr = reinterpret_cast<double *>(smem_raw);             // [RJ][nlev][2] + [RJ][2]
   double *fin = pcr + RJ * nlev * 2;
@@ -718,8 +744,10 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
   extern __shared__ unsigned char smem_raw[];
   const int NTP = blockDim.x, RJ = blockDim.y, ti = threadIdx.x, tj = threadIdx.y;
   const int i = ti + 1;
-  const int j = blockIdx.x * RJ + tj + 1;
-  const int k = blockIdx.y + 1;
+  int bx, by, bz;
+  xcd_block(bx, by, bz);  // r blocks of one plane (sharing halo rows) on one XCD
+  const int j = bx * RJ + tj + 1;
+  const int k = by + 1;
   const bool cell = (i <= g.nt) && (j <= g.nr);
   const int jc = cell ? j : 1;  // coefficient row for idle threads
   TC v1 = (TC)0, v2 = (TC)0, v3 = (TC)0;
@@ -1122,11 +1150,6 @@ __device__ __forceinline__ void lds_fft(T *re, T *im, int nfft, int logn, int ss
 // share 128-byte lines (narrow tiles of a strided layout) meet in one L2
 // instead of each fetching the whole line from HBM.  A bijection on
 // [0, nblk) for any nblk.
-__device__ __forceinline__ long xcd_tile(long b, long nblk) {
-  constexpr long kX = 8;
-  const long q = nblk / kX, rem = nblk % kX, x = b % kX, r = b / kX;
-  return x < rem ? x * (q + 1) + r : rem * (q + 1) + (x - rem) * q + r;
-}
 
 // byte offset rounded up to 16
 __host__ __device__ constexpr size_t al16(size_t b) { return (b + 15) & ~(size_t)15; }
