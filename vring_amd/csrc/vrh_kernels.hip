@@ -239,11 +239,14 @@ __global__ __launch_bounds__(512, sizeof(TC) == 4 ? 4 : 3) void k_explicit_march
   // (clamped in-bounds addresses for lanes and planes that do not use them)
   const int tic = min(ti, nt - 1);
   TC n_ru[3], n_pg[3];
+  // rho = 0 (the first substep): the previous RHS is not needed
+  const bool use_ru = rk.r != 0.0;
   auto load_rp = [&](int kk) {
     const int kc = min(kk, ke);
     const long oo = (long)tic + (long)jc * sj + (long)kc * sk;
     const long pp = (long)tic + (long)nt * ((long)(jc - 1) + (long)nr * (long)(kc - 1));
-    n_ru[0] = ru1[oo]; n_ru[1] = ru2[oo]; n_ru[2] = ru3[oo];
+    if (use_ru) { n_ru[0] = ru1[oo]; n_ru[1] = ru2[oo]; n_ru[2] = ru3[oo]; }
+    else { n_ru[0] = (TC)0; n_ru[1] = (TC)0; n_ru[2] = (TC)0; }
     n_pg[0] = prg1[pp]; n_pg[1] = prg2[pp]; n_pg[2] = prg3[pp];
   };
   // FP32: one plane ahead; FP64 (register budget): this plane, issued with
@@ -534,6 +537,8 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
   // ru and prg of the next plane stream in while this one is computed
   // (clamped in-bounds addresses for lanes and planes that do not use them)
   TC n_ru[CPL][3], n_pg[CPL][3];
+  // rho = 0 (the first substep): the previous RHS is not needed
+  const bool use_ru = rk.r != 0.0;
   auto load_rp = [&](int kk) {
     const int kc = min(kk, ke);
 #pragma unroll
@@ -541,7 +546,8 @@ __global__ __launch_bounds__(64 * RJT, WPE) void k_explicit_wave(VGrid g, VCoef 
       const int tic = min(lane + 64 * cc, nt - 1);
       const long oo = (long)tic + (long)jc * sj + (long)kc * sk;
       const long pp = (long)tic + (long)nt * ((long)(jc - 1) + (long)nr * (long)(kc - 1));
-      n_ru[cc][0] = ru1[oo]; n_ru[cc][1] = ru2[oo]; n_ru[cc][2] = ru3[oo];
+      if (use_ru) { n_ru[cc][0] = ru1[oo]; n_ru[cc][1] = ru2[oo]; n_ru[cc][2] = ru3[oo]; }
+      else { n_ru[cc][0] = (TC)0; n_ru[cc][1] = (TC)0; n_ru[cc][2] = (TC)0; }
       n_pg[cc][0] = prg1[pp]; n_pg[cc][1] = prg2[pp]; n_pg[cc][2] = prg3[pp];
     }
   };
@@ -757,6 +763,7 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
   const long o = hidx(g, i, j, k);
   const long dip = ip - i, dim = im - i, sj = g.sj, sk = g.sk, djm = (long)(jm - j) * sj;
   const TC ga = (TC)rk.g, ro = (TC)rk.r, al = (TC)rk.a, dt = (TC)g.dt, aRe = (TC)(rk.a / g.Re);
+  const bool use_ru = rk.r != 0.0;  // rho = 0 (the first substep): the previous RHS is not needed
   const TC dx3 = (TC)g.dx3, dx3q = (TC)g.dx3q;
   constexpr bool zs = ZS;  // axial mesh: per-plane metric (a separate instantiation)
   const TC zmk = zs ? (TC)zm.m3[k] : dx3, zck = zs ? (TC)zm.c3[k] : dx3;
@@ -788,7 +795,7 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
     TC qn2 = (Q1(sj) / (TC)c.rm[j + 1] - Q1(0) / (TC)c.rm[j]) * (TC)c.s1t2n[j];
     TC qs2 = (Q1(0) / (TC)c.rm[j] - Q1(djm) / (TC)c.rm[jm]) * (TC)c.s1t2s[j];
     TC d22 = (qn2 - qs2) * (TC)c.s1t2c[j];
-    v1 = (ga * H1 + ro * ru1[o] - al * prg1[pc] + aRe * (d11 + d33 + d22)) * dt;
+    v1 = (ga * H1 + ro * (use_ru ? ru1[o] : (TC)0) - al * prg1[pc] + aRe * (d11 + d33 + d22)) * dt;
     ru1[o] = H1;
   }
   // ---- q2 at (theta_m, r_c, z_m), j >= 2 ----
@@ -814,7 +821,7 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
                 : (k == 1) ? (TC)(4.0 / 3.0) * (Q2(sk) - (TC)3 * Q2(0)) * dx3q
                            : (Q2(sk) - (TC)2 * Q2(0) + Q2(-sk)) * dx3q;
     TC d22 = Q2(sj) * (TC)c.s2t2p[j] - Q2(0) * (TC)c.s2t2c[j] + Q2(-sj) * (TC)c.s2t2m[j];
-    v2 = (ga * H2 + ro * ru2[o] - al * prg2[pc] + aRe * (d11 + d33 + d22)) * dt;
+    v2 = (ga * H2 + ro * (use_ru ? ru2[o] : (TC)0) - al * prg2[pc] + aRe * (d11 + d33 + d22)) * dt;
     ru2[o] = H2;
   }
   // ---- q3 at (theta_m, r_m, z_c), k >= 2 ----
@@ -834,7 +841,7 @@ __global__ __launch_bounds__(1024) void k_explicit_cell(VGrid g, VCoef c, VRK rk
     TC qn2 = (Q3(sj) - Q3(0)) * (TC)c.s3t2n[j];
     TC qs2 = (Q3(0) - Q3(djm)) * (TC)c.s3t2s[j];
     TC d22 = qn2 - qs2;
-    v3 = (ga * H3 + ro * ru3[o] - al * prg3[pc] + aRe * (d11 + d33 + d22)) * dt;
+    v3 = (ga * H3 + ro * (use_ru ? ru3[o] : (TC)0) - al * prg3[pc] + aRe * (d11 + d33 + d22)) * dt;
     ru3[o] = H3;
   }
 #undef Q1
