@@ -116,9 +116,10 @@ int vrh_hz_train_residuals(vrh_hz_t *h, int nsamples, double lr, unsigned long l
  * transform and radial solve (poisson_solve.F90:140-179, 270-323 on the
  * uniform grid).  hz needs one slot per such position (nt - 2 mmin + 1);
  * this call sets their modes.  A slot that misses tol within maxit makes the
- * step fail (code -4).  hz = NULL switches it off.  hz must stay alive
- * (and its modes unchanged) while h uses it; vrh_autotune leaves the mode
- * solve out of its timing.  No reference counterpart (the reference has no
+ * step fail (code -4).  hz = NULL switches it off, and so does a later
+ * vrh_set_radial_mesh or vrh_set_axial_mesh.  hz must stay alive (and its
+ * modes unchanged) while h uses it; vrh_autotune leaves the mode solve out
+ * of its timing.  No reference counterpart (the reference has no
  * stretched z grid). */
 int vrh_set_mode_solver(vrh_t *h, vrh_hz_t *hz, int method, int mmin, double tol, int maxit);
 /* Krylov iterations (summed over slots) and mode solves since the last

@@ -93,6 +93,11 @@ def test_mode_solver_guards():
         s.step(1)
     s.set_mode_solver(None)
     s.step(1)
+    # a new mesh switches the mode solve off (no iterations counted)
+    s.set_mode_solver(hz, "pcg", 1, 1e-10, 20000)
+    s.set_axial_mesh(*tanh_axial_mesh(NZ, L, 2.0))
+    s.step(1)
+    assert s.mode_solver_stats() == (0, 0)
     s.close()
     m = vring_amd.Solver(NT, NR, NZ, R, L, 1000.0, 0.01, precision="mixed")
     m.set_axial_mesh(*tanh_axial_mesh(NZ, L, 2.0))

@@ -500,6 +500,7 @@ int vrh_set_radial_mesh(vrh_t *h, const double *rm, const double *rc, const doub
     return fail(h, "radial mesh: need rc(1) = 0, rc(j) < rm(j) < rc(j+1), rm(nr+1) > rm(nr), dmr, dcr > 0");
   const double *src[4] = {rm, rc, dmr, dcr};
   for (int q = 0; q < 4; q++) h->mesh[q].assign(src[q], src[q] + nr + 2);
+  h->hz = nullptr;  // the per-mode solve needs the uniform radial mesh: switched off
   int rc2 = refresh_mesh(h);
   if (rc2) return rc2;
   return 0;
@@ -522,6 +523,7 @@ int vrh_set_axial_mesh(vrh_t *h, const double *zm, const double *zc, const doubl
     return fail(h, "axial mesh: need nz >= 2, zc(1) = 0, zc(k) < zm(k) < zc(k+1), zm(0) < zm(1), zm(nz+1) > zm(nz), "
                    "dmz, dcz > 0 (k = 0..nz+1)");
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->hz = nullptr;  // a per-mode solve set for another mesh: switched off
   const double *src[4] = {zm, zc, dmz, dcz};
   for (int q = 0; q < 4; q++) {
     h->zmesh[q].assign(src[q], src[q] + nz + 2);
