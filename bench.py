@@ -252,6 +252,12 @@ def run_ensemble(nrings, steps, warmup, seed=0, particles=50000):
     t0 = time.perf_counter()
     ens.step(steps)
     t_ens = time.perf_counter() - t0
+    # all `steps` steps of every ring in one graph launch
+    ens.set_batch(steps)
+    ens.step(steps)
+    t0 = time.perf_counter()
+    ens.step(steps)
+    t_batch = time.perf_counter() - t0
     t0 = time.perf_counter()
     for s in members:
         s.step(steps)
@@ -262,6 +268,7 @@ def run_ensemble(nrings, steps, warmup, seed=0, particles=50000):
     return {"rings": nrings, "cells": cells, "steps": steps, "particles_per_ring": particles,
             "ms_per_step": round(t_ens / steps * 1e3, 3),
             "Mcell_RK3_steps_per_s": round(cells * steps / t_ens / 1e6, 2),
+            "one_launch_per_run_Mcell_RK3_steps_per_s": round(cells * steps / t_batch / 1e6, 2),
             "one_by_one_Mcell_RK3_steps_per_s": round(cells * steps / t_seq / 1e6, 2),
             "shapes_nr_nt_nz": shapes}
 

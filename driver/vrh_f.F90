@@ -277,6 +277,12 @@ module vrh_f
       integer(c_int), value :: nsteps
     end function vrh_ensemble_step
 
+    integer(c_int) function vrh_ensemble_set_batch(e, steps_per_launch) bind(C, name='vrh_ensemble_set_batch')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: e
+      integer(c_int), value :: steps_per_launch
+    end function vrh_ensemble_set_batch
+
     integer(c_int) function vrh_ensemble_last_error(e, buf, blen) bind(C, name='vrh_ensemble_last_error')
       import :: c_int, c_ptr, c_char, c_size_t
       type(c_ptr), value :: e

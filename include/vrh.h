@@ -215,13 +215,19 @@ int vrh_cfl_dt(const vrh_grid *grid, const double *q1, const double *q2, const d
 /* ---- Ensembles of independent rings ----
  * No reference counterpart (the reference runs one ring per MPI job).  The
  * members (handles of any grid size, Re, precision and particle load) are
- * advanced together: one RK3 step of every member is one HIP graph launch,
- * the members' kernels running concurrently.  Members must not be used
+ * advanced together: one RK3 step of every member (or K steps, see
+ * vrh_ensemble_set_batch) is one HIP graph launch, the members' kernels
+ * running concurrently.  Members must not be used
  * elsewhere during vrh_ensemble_step; their times advance as with vrh_step.
  * The ensemble keeps the handle pointers; destroy it before its members. */
 typedef struct vrh_ens_s vrh_ens_t;
 int vrh_ensemble_create(vrh_t *const *members, int n, vrh_ens_t **out);
 int vrh_ensemble_step(vrh_ens_t *e, int nsteps);
+/* Steps per graph launch (default 1): with K > 1 vrh_ensemble_step advances
+ * every member K steps per launch of one graph (K whole steps of every ring
+ * captured in sequence), then single steps for a remainder.  Results are
+ * those of stepping each member alone. */
+int vrh_ensemble_set_batch(vrh_ens_t *e, int steps_per_launch);
 int vrh_ensemble_last_error(vrh_ens_t *e, char *buf, size_t len);
 int vrh_ensemble_destroy(vrh_ens_t *e);
 

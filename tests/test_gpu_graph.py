@@ -51,9 +51,11 @@ def test_graph_replays_match_eager_and_oracle():
     se.close()
 
 
-@pytest.mark.parametrize("stretch", [0.0, 1.5])
-def test_ensemble_matches_members_stepped_alone(stretch):
-    """stretch > 0: every other member on a sinh-stretched radial mesh."""
+@pytest.mark.parametrize("stretch,batch", [(0.0, 1), (1.5, 1), (0.0, 3)])
+def test_ensemble_matches_members_stepped_alone(stretch, batch):
+    """stretch > 0: every other member on a sinh-stretched radial mesh.
+    batch 3: three steps of every member per graph launch (each call of 2,
+    then 7 steps: single-step launches, then batches and a remainder)."""
     cases = [(16, 24, 48, 500.0, "fp64"), (33, 32, 64, 2000.0, "fp64"), (32, 48, 96, 1000.0, "mixed"),
              (64, 32, 128, 4000.0, "mixed")]
     print("case ensemble", cases, "stretch", stretch, flush=True)
@@ -63,10 +65,11 @@ def test_ensemble_matches_members_stepped_alone(stretch):
         ens_members.append(ring_solver(nt, nr, nz, Re, prec, graph=True, amp=1e-2, stretch=st)[0])
         alone.append(ring_solver(nt, nr, nz, Re, prec, graph=False, amp=1e-2, stretch=st)[0])
     ens = vring_amd.Ensemble(ens_members)
-    for rep in range(3):
-        ens.step(2)
+    ens.set_batch(batch)
+    for rep, n in enumerate((2, 7, 2)):
+        ens.step(n)
         for s in alone:
-            s.step(2)
+            s.step(n)
         for m, s in zip(ens_members, alone):
             assert m.time() == s.time()
             for n in FIELDS:

@@ -985,8 +985,11 @@ struct vrh_ens_s {
   hipStream_t origin = nullptr;
   hipEvent_t fork = nullptr;
   std::vector<hipEvent_t> join;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  // graph[0]: one step of every member; graph[1]: `batch` steps of every
+  // member (built when batch > 1)
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  int batch = 1;
   std::string err;
 };
 
@@ -1002,14 +1005,18 @@ int efail(vrh_ens_t *e, const std::string &msg, int code = -1) {
   } while (0)
 
 void ens_release_graph(vrh_ens_t *e) {
-  if (e->gexec) hipGraphExecDestroy(e->gexec);
-  if (e->graph) hipGraphDestroy(e->graph);
-  e->gexec = nullptr;
-  e->graph = nullptr;
+  for (int g = 0; g < 2; g++) {
+    if (e->gexec[g]) hipGraphExecDestroy(e->gexec[g]);
+    if (e->graph[g]) hipGraphDestroy(e->graph[g]);
+    e->gexec[g] = nullptr;
+    e->graph[g] = nullptr;
+  }
 }
 
-int ens_build(vrh_ens_t *e) {
-  ens_release_graph(e);
+// Capture `nsteps` consecutive steps of every member into graph slot g:
+// forked from the origin stream into each member's stream, the member's
+// steps in order (each advances its device time), joined back.
+int ens_capture(vrh_ens_t *e, int g, int nsteps) {
   ECHK(e, hipStreamBeginCapture(e->origin, hipStreamCaptureModeThreadLocal));
   ECHK(e, hipEventRecord(e->fork, e->origin));
   int rc = 0;
@@ -1017,7 +1024,7 @@ int ens_build(vrh_ens_t *e) {
     vrh_t *h = e->m[q];
     if (hipStreamWaitEvent(h->stream, e->fork, 0) != hipSuccess) { rc = -3; break; }
     h->serial_sweeps = true;
-    rc = step_body(h);
+    for (int s = 0; s < nsteps && rc == 0; s++) rc = step_body(h);
     h->serial_sweeps = false;
     if (rc) { e->err = "member " + std::to_string(q) + ": " + h->err; break; }
     if (hipEventRecord(e->join[q], h->stream) != hipSuccess ||
@@ -1027,8 +1034,16 @@ int ens_build(vrh_ens_t *e) {
   hipError_t ee = hipStreamEndCapture(e->origin, &gr);
   if (rc) { if (gr) hipGraphDestroy(gr); return rc; }
   ECHK(e, ee);
-  e->graph = gr;
-  ECHK(e, hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+  e->graph[g] = gr;
+  ECHK(e, hipGraphInstantiate(&e->gexec[g], e->graph[g], nullptr, nullptr, 0));
+  return 0;
+}
+
+int ens_build(vrh_ens_t *e) {
+  ens_release_graph(e);
+  int rc = ens_capture(e, 0, 1);
+  if (rc == 0 && e->batch > 1) rc = ens_capture(e, 1, e->batch);
+  if (rc) return rc;
   for (size_t q = 0; q < e->m.size(); q++) e->epochs[q] = e->m[q]->epoch;
   return 0;
 }
@@ -1059,7 +1074,7 @@ int vrh_ensemble_create(vrh_t *const *members, int n, vrh_ens_t **out) {
 int vrh_ensemble_step(vrh_ens_t *e, int nsteps) {
   if (!e || nsteps < 0) return -1;
   if (nsteps == 0) return 0;
-  bool stale = !e->gexec;
+  bool stale = !e->gexec[0] || (e->batch > 1 && !e->gexec[1]);
   for (size_t q = 0; q < e->m.size(); q++) {
     vrh_t *h = e->m[q];
     if (h->kt_on) return efail(e, "member " + std::to_string(q) + " has the kernel timer enabled");
@@ -1074,14 +1089,32 @@ int vrh_ensemble_step(vrh_ens_t *e, int nsteps) {
     ECHK(e, hipMemcpyAsync(h->d_sc, h->h_pin, sizeof(double), hipMemcpyHostToDevice, e->origin));
     h->time_dirty = false;
   }
-  for (int s = 0; s < nsteps; s++) {
-    for (vrh_t *h : e->m) h->ntime = h->dt + h->ntime;  // loop.F90:53
-    ECHK(e, hipGraphLaunch(e->gexec, e->origin));
+  // whole batches first (one launch advances every member `batch` steps),
+  // then single steps; the host times advance step by step as the
+  // device's do
+  int left = nsteps;
+  while (left > 0) {
+    const int g = (e->batch > 1 && left >= e->batch) ? 1 : 0;
+    const int n = g ? e->batch : 1;
+    for (int s = 0; s < n; s++)
+      for (vrh_t *h : e->m) h->ntime = h->dt + h->ntime;  // loop.F90:53
+    ECHK(e, hipGraphLaunch(e->gexec[g], e->origin));
+    left -= n;
   }
   ECHK(e, hipStreamSynchronize(e->origin));
   for (size_t q = 0; q < e->m.size(); q++) {
     int rc = check_flag(e->m[q]);
     if (rc) return efail(e, "member " + std::to_string(q) + ": " + e->m[q]->err, rc);
+  }
+  return 0;
+}
+
+int vrh_ensemble_set_batch(vrh_ens_t *e, int steps_per_launch) {
+  if (!e || steps_per_launch < 1) return -1;
+  if (steps_per_launch != e->batch) {
+    ECHK(e, hipStreamSynchronize(e->origin));
+    ens_release_graph(e);
+    e->batch = steps_per_launch;
   }
   return 0;
 }

@@ -38,7 +38,8 @@ EXPORTS = ["vrh_create", "vrh_destroy", "vrh_last_error", "vrh_upload_fields", "
            "vrh_particles_download", "vrh_particles_step", "vrh_ktimer_enable", "vrh_ktimer_read", "vrh_autotune",
            "vrh_set_graph", "vrh_set_tuning", "vrh_analysis", "vrh_step_async", "vrh_snapshot_begin",
            "vrh_snapshot_wait", "vrh_particles_compact", "vrh_particles_count",
-           "vrh_particles_inject", "vrh_ensemble_create", "vrh_ensemble_step", "vrh_ensemble_last_error",
+           "vrh_particles_inject", "vrh_ensemble_create", "vrh_ensemble_step", "vrh_ensemble_set_batch",
+           "vrh_ensemble_last_error",
            "vrh_ensemble_destroy", "vrh_analysis_noalloc"]
 
 _lib = None
@@ -95,6 +96,7 @@ def load_library(path=LIBPATH):
     L.vrh_set_tuning.argtypes = [P, ip]
     L.vrh_ensemble_create.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(P)]
     L.vrh_ensemble_step.argtypes = [P, I]
+    L.vrh_ensemble_set_batch.argtypes = [P, I]
     L.vrh_ensemble_last_error.argtypes = [P, ctypes.c_char_p, ctypes.c_size_t]
     L.vrh_ensemble_destroy.argtypes = [P]
     _lib = L
@@ -415,6 +417,10 @@ class Ensemble:
 
     def step(self, n=1):
         self._check(self.L.vrh_ensemble_step(self.e, int(n)), "ensemble_step")
+
+    def set_batch(self, steps_per_launch):
+        """Steps of every member per graph launch."""
+        self._check(self.L.vrh_ensemble_set_batch(self.e, int(steps_per_launch)), "ensemble_set_batch")
 
     def close(self):
         if self.e:
