@@ -8,8 +8,10 @@
 !   partNNNNNsizeBB.dat (part_tools.F90:444-467)
 ! Optional sections of this driver (ignored by the reference): ===Ring===
 ! (Gaussian-ring initial condition, CFL time step), ===GPU=== (precision,
-! Poisson theta eigenvalue, autotuning), ===Part=== (particles seeded in a
-! torus, or the reference's inlet batches).
+! Poisson theta eigenvalue, autotuning, and optional stretched meshes:
+! `mesh rs zs`, a sinh radial mesh of stretch rs and a tanh axial mesh of
+! stretch zs, 0 for uniform), ===Part=== (particles seeded in a torus, or
+! the reference's inlet batches).
 module vring_io
   use iso_c_binding
   implicit none
@@ -30,6 +32,7 @@ module vring_io
     real(c_double) :: cfl = 0.d0, ujet = 1.d0
     ! ===GPU===
     integer :: precision = 0, theta_eigen = 0, autotune = 0
+    real(c_double) :: rstretch = 0.d0, zstretch = 0.d0
     ! ===Part===
     integer :: npart = 0, nsub = 5, pscheme = 0, pseed = 0
     real(c_double) :: ptorus(3) = (/1.d0, 3.d0, 0.6d0/), pradius = 5.0d-6
@@ -39,7 +42,64 @@ module vring_io
     integer(c_long) :: pcap = 50000
   end type deck
 
+  ! cell centres of a stretched mesh (set by the driver after
+  ! vrh_set_radial_mesh / vrh_set_axial_mesh); unallocated: uniform
+  real(c_double), allocatable :: mesh_rm(:), mesh_zm(:)
+
 contains
+
+  ! Stretched meshes as the library takes them (over index 0..n+1): radial
+  ! r(xi) = R sinh(s xi/R)/sinh(s) with centres at xi = (j - 1/2) R/nr and
+  ! faces at (j - 1) R/nr (rc(1) = 0), dr/dxi at both; axial faces
+  ! z_k = Z (1 + tanh(s (k/nz - 1))/tanh(s)), centres midway, mirrored
+  ! ghost cells, dz/dzeta as widths and centre spacings over Z/nz.
+  subroutine sinh_radial_mesh(nr, rmax, s, rm, rc, dmr, dcr)
+    integer, intent(in) :: nr
+    real(c_double), intent(in) :: rmax, s
+    real(c_double), intent(out) :: rm(0:nr+1), rc(0:nr+1), dmr(0:nr+1), dcr(0:nr+1)
+    integer :: j
+    real(c_double) :: dxi, xm, xc
+    dxi = rmax / nr
+    do j = 0, nr + 1
+      xm = (j - 0.5d0) * dxi
+      xc = (j - 1.d0) * dxi
+      rm(j) = rmax * sinh(s * xm / rmax) / sinh(s)
+      rc(j) = rmax * sinh(s * xc / rmax) / sinh(s)
+      dmr(j) = s * cosh(s * xm / rmax) / sinh(s)
+      dcr(j) = s * cosh(s * xc / rmax) / sinh(s)
+    end do
+    rc(1) = 0.d0
+  end subroutine sinh_radial_mesh
+
+  subroutine tanh_axial_mesh(nz, zmax, s, zm, zc, dmz, dcz)
+    integer, intent(in) :: nz
+    real(c_double), intent(in) :: zmax, s
+    real(c_double), intent(out) :: zm(0:nz+1), zc(0:nz+1), dmz(0:nz+1), dcz(0:nz+1)
+    real(c_double) :: zf(0:nz), dz
+    integer :: k
+    do k = 0, nz
+      zf(k) = zmax * (1.d0 + tanh(s * (dble(k) / nz - 1.d0)) / tanh(s))
+    end do
+    zf(0) = 0.d0
+    zf(nz) = zmax
+    zc(1:nz+1) = zf(0:nz)
+    zc(0) = -zc(2)
+    do k = 1, nz
+      zm(k) = 0.5d0 * (zf(k) + zf(k - 1))
+    end do
+    zm(0) = -zm(1)
+    zm(nz + 1) = 2.d0 * zmax - zm(nz)
+    dz = zmax / nz
+    do k = 1, nz
+      dmz(k) = (zf(k) - zf(k - 1)) / dz
+    end do
+    dmz(0) = dmz(1)
+    dmz(nz + 1) = dmz(nz)
+    do k = 1, nz + 1
+      dcz(k) = (zm(k) - zm(k - 1)) / dz
+    end do
+    dcz(0) = dcz(1)
+  end subroutine tanh_axial_mesh
 
   logical function find_section(u, name)
     integer, intent(in) :: u
@@ -104,6 +164,10 @@ contains
       read(u, *, iostat=ios) a, d%precision
       read(u, *, iostat=ios) a, d%theta_eigen
       read(u, *, iostat=ios) a, d%autotune
+      read(u, *, iostat=ios) a, d%rstretch, d%zstretch
+      if (ios /= 0) then
+        d%rstretch = 0.d0; d%zstretch = 0.d0
+      end if
     end if
     if (find_section(u, '===Part===')) then
       read(u, *) a, d%npart
@@ -224,6 +288,8 @@ contains
     do iz = 0, nz + 1
       zm(iz) = (iz - 0.5d0) * dz
     end do
+    if (allocated(mesh_rm)) rm = mesh_rm   ! msh%rm of a stretched mesh
+    if (allocated(mesh_zm)) zm = mesh_zm
     do ith = 1, nt
       thm(ith) = (ith - 0.5d0) * dth
     end do

@@ -37,6 +37,7 @@ program vring_main
   integer(c_long) :: nlive
   real :: rs
   real(c_double) :: r8
+  real(c_double), allocatable, target :: m1(:), m2(:), m3(:), m4(:)
 
   call read_deck('inp.txt', d)
   nt = d%nthetag; nr = d%nrg; nz = d%nzg
@@ -48,7 +49,7 @@ program vring_main
     write(*, '(a,es16.8,4i8)') 'para ', d%dt, d%totstp, d%expstp, d%anlstp, d%restrt
     write(*, '(a,3es16.8,i4)') 'flow ', d%Re, d%Pe, d%slip, d%nrkstep
     write(*, '(a,i4,4es16.8)') 'ring ', d%nring, d%rings(:, 1)
-    write(*, '(a,3i4)') 'gpu  ', d%precision, d%theta_eigen, d%autotune
+    write(*, '(a,3i4,2f10.4)') 'gpu  ', d%precision, d%theta_eigen, d%autotune, d%rstretch, d%zstretch
     stop
   end if
   if (d%npth(1) * d%npth(2) * d%npth(3) /= 1) &
@@ -81,6 +82,26 @@ program vring_main
   call vrh_check(h, vrh_create(grid, params, d%precision, h), 'vrh_create')
   call vrh_check(h, vrh_upload_fields(h, c_loc(q1), c_loc(q2), c_loc(q3), c_loc(pres)), 'upload')
   call vrh_check(h, vrh_set_time(h, ntime), 'set_time')
+  ! stretched meshes (the library's hooks; particles need uniform meshes)
+  if (d%rstretch > 0.d0 .or. d%zstretch > 0.d0) then
+    if (d%npart > 0 .or. d%pinlet > 0) stop 'particles need the uniform meshes (mesh 0 0)'
+  end if
+  if (d%rstretch > 0.d0) then
+    allocate(m1(0:nr+1), m2(0:nr+1), m3(0:nr+1), m4(0:nr+1), mesh_rm(0:nr+1))
+    call sinh_radial_mesh(nr, d%rmax, d%rstretch, m1, m2, m3, m4)
+    call vrh_check(h, vrh_set_radial_mesh(h, m1, m2, m3, m4), 'radial mesh')
+    mesh_rm = m1
+    deallocate(m1, m2, m3, m4)
+    write(*, '(a,f8.4)') 'radial mesh: sinh stretch ', d%rstretch
+  end if
+  if (d%zstretch > 0.d0) then
+    allocate(m1(0:nz+1), m2(0:nz+1), m3(0:nz+1), m4(0:nz+1), mesh_zm(0:nz+1))
+    call tanh_axial_mesh(nz, d%zmax, d%zstretch, m1, m2, m3, m4)
+    call vrh_check(h, vrh_set_axial_mesh(h, m1, m2, m3, m4), 'axial mesh')
+    mesh_zm = m1
+    deallocate(m1, m2, m3, m4)
+    write(*, '(a,f8.4)') 'axial mesh: tanh stretch ', d%zstretch
+  end if
   if (d%npart > 0) then
     np = d%npart
     allocate(pu(merge(6, 117, d%pscheme == VRH_PART_TRILINEAR_RK3), np), palive(np))
