@@ -361,12 +361,13 @@ def tanh_axial_mesh(nz, zmax, s):
 
 
 def run_learned_step(train_samples, steps=2, warmup=1, stretch=2.0, mmin=3, tol=1e-8, seed=0, pool=8,
-                     methods=("pcg", "learned", "learned_fcg")):
+                     methods=("pcg", "learned", "learned_fcg"), max_cg=64):
     """The learned form inside the time step: the workload's grid with the
     tanh-stretched axial mesh (stretch 2, the per-mode Helmholtz grid), mixed
     precision, the pressure modes m >= mmin solved per mode to relative
     residual `tol` by plain PCG, PCG from the learned warm start, or the
-    learned flexible CG (network trained on `train_samples` pairs), against
+    learned flexible CG (network trained on `train_samples` pairs, as the
+    residuals of up to `max_cg` plain PCG iterations), against
     the direct eigen-transform solve of the same step.  Times are whole
     steps (eager launches: the axial mesh's z transforms are rocBLAS GEMMs);
     errors are against the direct mixed and FP64 runs after the same steps."""
@@ -379,10 +380,11 @@ def run_learned_step(train_samples, steps=2, warmup=1, stretch=2.0, mmin=3, tol=
     hz = vring_amd.Helmholtz(NR, NZ, nslots, RMAX, ZMAX, stretch, pool=pool)
     hz.net_init(seed)
     t0 = time.perf_counter()
-    loss = hz.train(train_samples, lr=1e-3, seed=seed)
+    loss = hz.train_residuals(train_samples, lr=1e-3, seed=seed, max_cg=max_cg)
     out = {"workload": f"nr={NR} ntheta={NT} nz={NZ} Re={RE:g}, tanh axial mesh stretch {stretch:g}",
            "modes_iterative": f"m >= {mmin} ({nslots} theta positions)", "tol": tol, "dt": dt,
-           "train_samples": train_samples, "train_s": round(time.perf_counter() - t0, 2),
+           "train_samples": train_samples, "train_on": f"CG residuals (0..{max_cg} iterations)",
+           "train_s": round(time.perf_counter() - t0, 2),
            "loss_first": float(loss[:4].mean()), "loss_last": float(loss[-4:].mean())}
     fields = {}
     runs = [("fp64_direct", "fp64", None), ("direct", "mixed", None)] + [(m, "mixed", m) for m in methods]
@@ -636,7 +638,7 @@ def main():
                     help="run only the learned Helmholtz benchmark, training on SAMPLES pairs")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 64-ring ensemble and the large workload in the default run")
-    ap.add_argument("--learned-train", type=int, default=4096, metavar="SAMPLES",
+    ap.add_argument("--learned-train", type=int, default=8192, metavar="SAMPLES",
                     help="training pairs of the learned-in-the-step measurement of the default run (0: skip it)")
     ap.add_argument("--learned-step", type=int, default=0, metavar="SAMPLES",
                     help="run only the learned-in-the-step benchmark (tanh axial mesh), training on SAMPLES pairs")
@@ -696,7 +698,7 @@ def main():
     v64 = ncell * args.steps / wall64 / 1e6
     vmx = ncell * args.steps / wallmx / 1e6
     err = error_run(q, dt, args.error_steps) if args.error_steps else {"timed_run": rel_errors(fmx, f64)}
-    lstep = run_learned_step(args.learned_train, 1, 1, methods=("pcg", "learned")) if args.learned_train else None
+    lstep = run_learned_step(args.learned_train, 1, 1) if args.learned_train else None
     # the other workloads' single-GPU measurements, beside the headline
     extras = {} if args.no_extras else {"ensemble_64": run_ensemble(64, 10, 2), "large": run_large(10, 2)}
     rf = roofline(ktmx, "mixed", ncell)
