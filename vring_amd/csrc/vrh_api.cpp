@@ -523,11 +523,13 @@ int vrh_set_axial_mesh(vrh_t *h, const double *zm, const double *zc, const doubl
     return fail(h, "axial mesh: need nz >= 2, zc(1) = 0, zc(k) < zm(k) < zc(k+1), zm(0) < zm(1), zm(nz+1) > zm(nz), "
                    "dmz, dcz > 0 (k = 0..nz+1)");
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  h->hz = nullptr;  // a per-mode solve set for another mesh: switched off
+  // everything is computed and the device buffers allocated before the
+  // handle's mesh state changes: a failure leaves the previous mesh in place
+  std::vector<double> zmn[4];
   const double *src[4] = {zm, zc, dmz, dcz};
   for (int q = 0; q < 4; q++) {
-    h->zmesh[q].assign(src[q], src[q] + nz + 2);
-    h->zmesh[q].push_back(h->zmesh[q][nz + 1]);  // k = nz+2 repeats nz+1
+    zmn[q].assign(src[q], src[q] + nz + 2);
+    zmn[q].push_back(zmn[q][nz + 1]);  // k = nz+2 repeats nz+1
   }
   // metric arrays (k = 0..nz+2)
   const int L = nz + 3;
@@ -535,7 +537,7 @@ int vrh_set_axial_mesh(vrh_t *h, const double *zm, const double *zc, const doubl
   double *m3 = &mb[0], *c3 = &mb[L], *cm = &mb[2 * L], *cp = &mb[3 * L], *fm = &mb[4 * L], *fp = &mb[5 * L],
          *zp = &mb[6 * L];
   const double dx3 = h->g.dx3;
-  for (int k = 0; k < L; k++) { m3[k] = dx3 / h->zmesh[2][k]; c3[k] = dx3 / h->zmesh[3][k]; zp[k] = h->zmesh[0][k]; }
+  for (int k = 0; k < L; k++) { m3[k] = dx3 / zmn[2][k]; c3[k] = dx3 / zmn[3][k]; zp[k] = zmn[0][k]; }
   for (int k = 0; k < L; k++) {
     cm[k] = m3[k] * c3[k];
     cp[k] = m3[k] * c3[k + 1 < L ? k + 1 : k];
@@ -543,27 +545,15 @@ int vrh_set_axial_mesh(vrh_t *h, const double *zm, const double *zc, const doubl
     fp[k] = c3[k] * m3[k];
   }
   const double h1 = 0.5 / m3[1], h2 = 1.0 / c3[2];
-  std::vector<double> vi, vf;
+  std::vector<double> vi, vf, lam;
   std::string emsg;
-  if (host_axial_transforms(m3, c3, nz, vi, vf, h->zlam, emsg)) return fail(h, ("axial mesh: " + emsg).c_str());
+  if (host_axial_transforms(m3, c3, nz, vi, vf, lam, emsg)) return fail(h, ("axial mesh: " + emsg).c_str());
+  const size_t nn = (size_t)nz * nz;
   int rc;
   if (!h->d_zmet && (rc = dalloc(h, &h->d_zmet, mb.size()))) return rc;
-  if ((rc = h2d(h, h->d_zmet, mb.data(), mb.size() * sizeof(double)))) return rc;
-  h->zmet.m3 = h->d_zmet; h->zmet.c3 = h->d_zmet + L; h->zmet.cm = h->d_zmet + 2 * L; h->zmet.cp = h->d_zmet + 3 * L;
-  h->zmet.fm = h->d_zmet + 4 * L; h->zmet.fp = h->d_zmet + 5 * L; h->zmet.zpos = h->d_zmet + 6 * L;
-  h->zmet.ws = 2.0 / (h1 * (h1 + h2));
-  h->zmet.w2 = 2.0 / (h2 * (h1 + h2));
-  const size_t nn = (size_t)nz * nz;
-  std::vector<double> zv(2 * nn);
-  std::memcpy(zv.data(), vi.data(), nn * sizeof(double));
-  std::memcpy(zv.data() + nn, vf.data(), nn * sizeof(double));
   if (!h->d_zv && (rc = dalloc(h, &h->d_zv, 2 * nn))) return rc;
-  if ((rc = h2d(h, h->d_zv, zv.data(), zv.size() * sizeof(double)))) return rc;
   if (h->prec == VRH_MIXED) {
-    std::vector<float> zf(2 * nn);
-    for (size_t i = 0; i < 2 * nn; i++) zf[i] = (float)zv[i];
     if (!h->d_zvf && (rc = dalloc(h, &h->d_zvf, 2 * nn))) return rc;
-    if ((rc = h2d(h, h->d_zvf, zf.data(), zf.size() * sizeof(float)))) return rc;
     if (!h->S2_64 && (rc = dalloc(h, &h->S2_64, (size_t)5 * h->nr * nz))) return rc;
   }
   if (!h->S2 && (rc = dalloc_bytes(h, &h->S2, compact(h) * wsize(h)))) return rc;
@@ -573,6 +563,23 @@ int vrh_set_axial_mesh(vrh_t *h, const double *zm, const double *zc, const doubl
     h->blas = b;
     if (rocblas_set_stream(b, h->stream) != rocblas_status_success)
       return fail(h, "axial mesh: rocblas_set_stream failed", -3);
+  }
+  h->hz = nullptr;  // a per-mode solve set for another mesh: switched off
+  for (int q = 0; q < 4; q++) h->zmesh[q] = zmn[q];
+  h->zlam = lam;
+  if ((rc = h2d(h, h->d_zmet, mb.data(), mb.size() * sizeof(double)))) return rc;
+  h->zmet.m3 = h->d_zmet; h->zmet.c3 = h->d_zmet + L; h->zmet.cm = h->d_zmet + 2 * L; h->zmet.cp = h->d_zmet + 3 * L;
+  h->zmet.fm = h->d_zmet + 4 * L; h->zmet.fp = h->d_zmet + 5 * L; h->zmet.zpos = h->d_zmet + 6 * L;
+  h->zmet.ws = 2.0 / (h1 * (h1 + h2));
+  h->zmet.w2 = 2.0 / (h2 * (h1 + h2));
+  std::vector<double> zv(2 * nn);
+  std::memcpy(zv.data(), vi.data(), nn * sizeof(double));
+  std::memcpy(zv.data() + nn, vf.data(), nn * sizeof(double));
+  if ((rc = h2d(h, h->d_zv, zv.data(), zv.size() * sizeof(double)))) return rc;
+  if (h->prec == VRH_MIXED) {
+    std::vector<float> zf(2 * nn);
+    for (size_t i = 0; i < 2 * nn; i++) zf[i] = (float)zv[i];
+    if ((rc = h2d(h, h->d_zvf, zf.data(), zf.size() * sizeof(float)))) return rc;
   }
   if ((rc = refresh_mesh(h))) return rc;
   graph_invalidate(h);
