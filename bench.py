@@ -195,13 +195,21 @@ def run_particles(prec, q, dt, n, steps, scheme="lsq_euler"):
             s.particles_step(irk)
     s.synchronize()
     t = time.perf_counter() - t0
+    # the cell sort that upload and compaction run (vrh_particles_sort)
+    ts = time.perf_counter()
+    for _ in range(3):
+        s.particles_sort(0)
+    s.synchronize()
+    sort_ms = (time.perf_counter() - ts) / 3 * 1e3
     s.close()
     nvar = 117 if scheme == "lsq_euler" else 6
     per = (2 * nvar + (81 if scheme == "lsq_euler" else 2 * nvar)) * 8
     return {"scheme": scheme, "particles": n, "substeps_per_step": nsub * len(stages),
             "ms_per_step": round(t / steps * 1e3, 3), "particles_per_s": round(n * steps / t, 1),
             "hbm_bytes_per_particle_substep": per,
-            "hbm_GBps": round(per * n * nsub * len(stages) * steps / t / 1e9, 1)}
+            "hbm_GBps": round(per * n * nsub * len(stages) * steps / t / 1e9, 1),
+            "order": "sorted by cell in device memory (upload and compaction sort)",
+            "sort_ms": round(sort_ms, 3)}
 
 
 def ensemble_specs(nrings, seed=0):

@@ -248,6 +248,12 @@ module vrh_f
       integer(c_int), value :: batch
     end function vrh_particles_compact
 
+    integer(c_int) function vrh_particles_sort(h, batch) bind(C, name='vrh_particles_sort')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: batch
+    end function vrh_particles_sort
+
     integer(c_int) function vrh_particles_count(h, batch, n) bind(C, name='vrh_particles_count')
       import :: c_int, c_ptr, c_long
       type(c_ptr), value :: h

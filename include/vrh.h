@@ -248,6 +248,13 @@ int vrh_particles_step(vrh_t *h, int irk);
 /* Pack the live particles of a batch to the front, order preserved (the
  * reference's free list, remv_part/injc_part, part_tools.F90:141-225). */
 int vrh_particles_compact(vrh_t *h, int batch);
+/* Sort a batch's particles in device memory by their cells (no reference
+ * counterpart: part_tools.F90 keeps its slots in injection order), so that
+ * the interpolation gathers of neighbouring particles meet the same cache
+ * lines.  Upload and compaction sort as well; this call refreshes the order
+ * on long runs between them.  The order the API shows (upload, download,
+ * compaction, injection) does not change. */
+int vrh_particles_sort(vrh_t *h, int batch);
 /* Live particles of a batch (synchronous). */
 int vrh_particles_count(vrh_t *h, int batch, long *n);
 /* Inlet injection (inlt_part, part_tools.F90:95-138), after a compaction:

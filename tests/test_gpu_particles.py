@@ -72,6 +72,43 @@ def test_trilinear_rk3_particles_match_oracle():
     assert np.abs(g - xv).max() / np.abs(xv).max() < 1e-11
 
 
+def test_sorting_keeps_the_shown_order():
+    """The particles sit in device memory sorted by cell (upload, compaction
+    and vrh_particles_sort reorder them); downloads, compaction and the
+    particle step still follow the order the caller uploaded: the rows come
+    back bit for bit, dead rows included, trilinear RK3 stages with sorts
+    between them equal the oracle's, and compaction keeps the live rows in
+    their uploaded order."""
+    print("case particles sort", flush=True)
+    o, s, u0, st, dt = setup(scheme="trilinear_rk3", nsub=1, n=2000, cap=2600)
+    n = u0.shape[0]
+    g, gal = s.particles_download(0, n)
+    np.testing.assert_array_equal(g, u0)
+    np.testing.assert_array_equal(gal, np.ones(n, np.int32))
+    xv = u0.copy()
+    r2, alive = np.zeros((n, 6)), np.ones(n, np.int32)
+    for irk in (1, 2, 3):
+        s.particles_step(irk)
+        s.particles_sort(0)
+        o.L.orc_part_rk3_trilinear(o.h, n, st, irk, dt, xv.ravel(), r2.ravel(), alive)
+    g, gal = s.particles_download(0, n)
+    np.testing.assert_array_equal(gal, alive)
+    assert np.abs(g - xv).max() / np.abs(xv).max() < 1e-11
+    alive0 = np.ones(n, np.int32)
+    alive0[::7] = 0
+    s.particles_upload(0, u0, alive0)
+    s.particles_sort(0)
+    g, gal = s.particles_download(0, n)
+    np.testing.assert_array_equal(gal, alive0)
+    np.testing.assert_array_equal(g, u0)
+    s.particles_compact(0)
+    m = s.particles_count(0)
+    assert m == int(alive0.sum())
+    g, gal = s.particles_download(0, m)
+    np.testing.assert_array_equal(gal, np.ones(m, np.int32))
+    np.testing.assert_array_equal(g, u0[alive0 == 1])
+
+
 def test_particles_advance_inside_whole_steps():
     """vrh_step (graph-captured) with trilinear RK3 particles: one particle
     stage before each fluid substep, against the oracle doing the same."""

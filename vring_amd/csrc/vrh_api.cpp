@@ -402,7 +402,8 @@ int vrh_destroy(vrh_t *h) {
     if (b.fit) hipFree(b.fit);
     if (b.ddt2) hipFree(b.ddt2);
     if (b.alive) hipFree(b.alive);
-    void *extra[] = {b.d_n, b.inflow, b.u_tmp, b.ddt2_tmp, b.alive_tmp, b.work};
+    void *extra[] = {b.d_n, b.inflow, b.u_tmp, b.ddt2_tmp, b.alive_tmp, b.work, b.slot, b.slot_tmp,
+                     b.lflag, b.lrank, b.skeys, b.stmp};
     for (void *q : extra) if (q) hipFree(q);
   }
   for (int q = 0; q < 2; q++)

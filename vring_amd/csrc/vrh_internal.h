@@ -94,6 +94,15 @@ struct PartBatch {
   int *alive_tmp = nullptr;
   long *work = nullptr;      // scan scratch
   long ncall = 0;            // injection calls so far (random stream index)
+  // Physical slots are kept sorted by the particles' cells (upload,
+  // compaction, vrh_particles_sort), so that the interpolation gathers of a
+  // wavefront meet few cache lines; slot[p] is the slot that physical slot p
+  // holds in the order the API shows (upload, download, compaction order).
+  int *slot = nullptr;
+  int *slot_tmp = nullptr, *lflag = nullptr, *lrank = nullptr;  // compaction renumbering
+  unsigned *skeys = nullptr;  // sort keys and values, in and out (4 x cap)
+  void *stmp = nullptr;       // radix sort scratch
+  size_t stmp_bytes = 0;
 };
 
 // Kernel ids for the per-kernel event timer.

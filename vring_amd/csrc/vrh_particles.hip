@@ -7,6 +7,7 @@
 // interpolation and right-hand side (double in FP64 mode, float in mixed
 // mode); the state update is accumulated in double.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <cmath>
 #include <cstring>
@@ -527,8 +528,8 @@ __global__ void k_inflow(VGrid g, VCoef c, const double *__restrict__ q3, double
 // (0, 0, q3), identity Jacobian (part_tools.F90:120-134, 164-181)
 __global__ void k_inject(VGrid g, VCoef c, const double *__restrict__ q3, const long *__restrict__ counts,
                          const long *__restrict__ offs, const long *n_dev, long cap, int nvar, double *__restrict__ u,
-                         double *__restrict__ d2, int *__restrict__ alive, int jr0, double dth, double dr, double dz,
-                         unsigned long long seed, long call) {
+                         double *__restrict__ d2, int *__restrict__ alive, int *__restrict__ slot, int jr0, double dth,
+                         double dr, double dz, unsigned long long seed, long call) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)jr0 * g.nt) return;
   const int ir = (int)(t / g.nt) + 1, it = (int)(t - (long)(ir - 1) * g.nt) + 1;
@@ -552,12 +553,98 @@ __global__ void k_inject(VGrid g, VCoef c, const double *__restrict__ q3, const 
     u[5 * cap + idx] = uz;
     if (nvar == 117) { u[6 * cap + idx] = 1.0; u[10 * cap + idx] = 1.0; u[14 * cap + idx] = 1.0; }
     alive[idx] = 1;
+    slot[idx] = (int)idx;  // appended after a compaction: physical and shown order agree
   }
 }
 
 __global__ void k_add_count(long *n_dev, const long *total, long cap) {
   const long n = *n_dev + *total;
   *n_dev = n < cap ? n : cap;
+}
+
+// ---------------------------------------------------------------------------
+// Spatial order.  The particle kernels run over physical slots; sorting the
+// slots by cell (theta fastest, then r, then z) makes the 8- and 27-point
+// gathers of one wavefront fall on a few neighbouring rows of the fields.
+// The order the API shows (and the reference keeps) travels in slot[]:
+// physical slot p holds the particle of shown slot slot[p].
+// ---------------------------------------------------------------------------
+__global__ void k_iota(int *__restrict__ v, long cap) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < cap) v[p] = (int)p;
+}
+
+// cell key of each live particle; dead and unused slots take kmax, so the
+// stable sort keeps them behind the live ones in their present order
+__global__ void k_sort_keys(const double *__restrict__ u, const int *__restrict__ alive, const long *n_dev, long cap,
+                            int nt, int nr, int nz, double dth, double dr, double dz, unsigned kmax,
+                            unsigned *__restrict__ key, unsigned *__restrict__ val) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= cap) return;
+  unsigned k = kmax;
+  if (p < *n_dev && alive[p]) {
+    const double x = u[p], y = u[cap + p], z = u[2 * cap + p];
+    double th = atan2(y, x);
+    if (th < 0.0) th += 2.0 * M_PI;
+    // fmin/fmax clamp, and map a NaN coordinate to 0
+    const int i = (int)fmin(fmax(th / dth, 0.0), nt - 1.0);
+    const int j = (int)fmin(fmax(sqrt(x * x + y * y) / dr, 0.0), nr + 1.0);
+    const int kz = (int)fmin(fmax(z / dz, 0.0), nz + 1.0);
+    k = ((unsigned)kz * (unsigned)(nr + 2) + (unsigned)j) * (unsigned)nt + (unsigned)i;
+  }
+  key[p] = k;
+  val[p] = (unsigned)p;
+}
+
+// physical slot i <- slot order[i]: state rows, previous rates, alive flag
+// and shown slot.  Compacting (rank != nullptr): the first nlive slots are
+// the live particles, their shown slots renumbered to their ranks; the rest
+// are unused, numbered as their position
+__global__ void k_permute(const unsigned *__restrict__ order, const double *__restrict__ u,
+                          const double *__restrict__ d2, const int *__restrict__ alive, const int *__restrict__ slot,
+                          const int *__restrict__ rank, const long *nlive, long cap, int nvar,
+                          double *__restrict__ ut, double *__restrict__ d2t, int *__restrict__ at,
+                          int *__restrict__ st) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cap) return;
+  const long src = order[i];
+  for (int v = 0; v < nvar; v++) ut[(long)v * cap + i] = u[(long)v * cap + src];
+  if (d2)
+    for (int v = 0; v < nvar; v++) d2t[(long)v * cap + i] = d2[(long)v * cap + src];
+  if (rank) {
+    const bool live = i < *nlive;
+    at[i] = live ? 1 : 0;
+    st[i] = live ? rank[slot[src]] : (int)i;
+  } else {
+    at[i] = alive[src];
+    st[i] = slot[src];
+  }
+}
+
+// flag[s] = 1 for the shown slots s < n whose particle is alive (flag zeroed
+// before)
+__global__ void k_slot_flags(const int *__restrict__ alive, const int *__restrict__ slot, const long *n_dev, long cap,
+                             int *__restrict__ flag) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < cap && p < *n_dev && alive[p]) flag[slot[p]] = 1;
+}
+
+// exclusive rank of each flagged shown slot: block offset boff (scan of the
+// per-block counts) plus the rank within the block of 256
+__global__ void k_slot_rank(const int *__restrict__ flag, const long *n_dev, long cap, const long *__restrict__ boff,
+                            int *__restrict__ rank) {
+  const long s = (long)blockIdx.x * 256 + threadIdx.x;
+  const int f = (s < cap && s < *n_dev && flag[s]) ? 1 : 0;
+  __shared__ int cnt[256];
+  cnt[threadIdx.x] = f;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {  // inclusive scan by doubling
+    const int x = ((int)threadIdx.x >= o) ? cnt[threadIdx.x - o] : 0;
+    __syncthreads();
+    cnt[threadIdx.x] += x;
+    __syncthreads();
+  }
+  if (s < cap) rank[s] = (int)(boff[blockIdx.x] + cnt[threadIdx.x] - f);
 }
 
 }  // namespace
@@ -609,6 +696,8 @@ void launch_particles(vrh_t *h, int irk) {
 
 extern "C" {
 
+static int sort_batch(vrh_t *h, PartBatch &B, bool compact);
+
 int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st, const int *nsub, int scheme) {
   if (h) graph_invalidate(h);
   if (!h || nbatch < 1 || !cap || !st || !nsub) return -1;
@@ -622,13 +711,18 @@ int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st
   for (int b = 0; b < nbatch; b++) {
     PartBatch &B = h->pb[b];
     B.cap = cap[b]; B.n = 0; B.st = st[b]; B.nsub = nsub[b] < 1 ? 1 : nsub[b];
-    if (B.cap < 1 || (scheme != VRH_PART_TRILINEAR_RK3 && !(B.st > 0.0))) { h->err = "bad particle batch"; return -1; }
+    if (B.cap < 1 || B.cap > 0x7fffffffL || (scheme != VRH_PART_TRILINEAR_RK3 && !(B.st > 0.0))) {
+      h->err = "bad particle batch";
+      return -1;
+    }
     if (hipMalloc(&B.u, sizeof(double) * nvar * B.cap) != hipSuccess ||
         hipMalloc(&B.alive, sizeof(int) * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
     hipMemsetAsync(B.u, 0, sizeof(double) * nvar * B.cap, h->stream);
     hipMemsetAsync(B.alive, 0, sizeof(int) * B.cap, h->stream);
-    if (hipMalloc(&B.d_n, sizeof(long)) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
+    if (hipMalloc(&B.d_n, sizeof(long)) != hipSuccess ||
+        hipMalloc(&B.slot, sizeof(int) * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
     hipMemsetAsync(B.d_n, 0, sizeof(long), h->stream);
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((B.cap + 255) / 256)), dim3(256), 0, h->stream, B.slot, B.cap);
     B.ddt2 = nullptr;
     B.fit = nullptr;
     if (scheme != VRH_PART_TRILINEAR_RK3 &&
@@ -661,12 +755,13 @@ int vrh_particles_upload(vrh_t *h, int batch, long n, const double *u, const int
                 hipSuccess &&
             hipMemcpyAsync(B.d_n, &n, sizeof(long), hipMemcpyHostToDevice, h->stream) == hipSuccess;
   if (ok && B.ddt2) ok = hipMemsetAsync(B.ddt2, 0, sizeof(double) * nvar * B.cap, h->stream) == hipSuccess;
+  if (ok) hipLaunchKernelGGL(k_iota, dim3((unsigned)((B.cap + 255) / 256)), dim3(256), 0, h->stream, B.slot, B.cap);
   if (hipStreamSynchronize(h->stream) != hipSuccess || !ok) {
     h->err = "particle upload failed";
     return -3;
   }
   B.n = n;
-  return 0;
+  return sort_batch(h, B, false);
 }
 
 int vrh_particles_download(vrh_t *h, int batch, long n, double *u, int *alive) {
@@ -675,54 +770,111 @@ int vrh_particles_download(vrh_t *h, int batch, long n, double *u, int *alive) {
   if (n > B.cap) n = B.cap;
   const int nvar = h->pscheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
   std::vector<double> soa((size_t)nvar * B.cap);
-  std::vector<int> al(B.cap);
+  std::vector<int> al(B.cap), sl(B.cap);
   if (hipMemcpyAsync(soa.data(), B.u, soa.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
       hipMemcpyAsync(al.data(), B.alive, al.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipMemcpyAsync(sl.data(), B.slot, sl.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
       hipStreamSynchronize(h->stream) != hipSuccess) {
     h->err = "particle download failed";
     return -3;
   }
-  for (long p = 0; p < n; p++)
-    for (int v = 0; v < nvar; v++) u[(size_t)p * nvar + v] = soa[(size_t)v * B.cap + p];
-  if (alive)
-    for (long p = 0; p < n; p++) alive[p] = al[p];
+  // physical slot p holds shown slot sl[p] (a permutation of [0, cap))
+  for (long p = 0; p < B.cap; p++) {
+    const long q = sl[p];
+    if (q < 0 || q >= B.cap) { h->err = "particle slot map corrupt"; return -3; }
+    if (q >= n) continue;
+    for (int v = 0; v < nvar; v++) u[(size_t)q * nvar + v] = soa[(size_t)v * B.cap + p];
+    if (alive) alive[q] = al[p];
+  }
+  return 0;
+}
+
+// compaction and sort buffers of a batch (allocated on first use)
+static int particle_buffers(vrh_t *h, PartBatch &B) {
+  if (B.u_tmp) return 0;
+  const int nvar = h->pscheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
+  const long nblk = (B.cap + 255) / 256;
+  if (hipMalloc(&B.u_tmp, sizeof(double) * nvar * B.cap) != hipSuccess ||
+      hipMalloc(&B.alive_tmp, sizeof(int) * B.cap) != hipSuccess ||
+      hipMalloc(&B.work, sizeof(long) * (nblk + 8 + (long)(h->nr + 1) * h->nt * 2)) != hipSuccess ||
+      hipMalloc(&B.inflow, sizeof(double) * (h->nr + 2)) != hipSuccess ||
+      hipMalloc(&B.slot_tmp, sizeof(int) * B.cap) != hipSuccess ||
+      hipMalloc(&B.lflag, sizeof(int) * B.cap) != hipSuccess ||
+      hipMalloc(&B.lrank, sizeof(int) * B.cap) != hipSuccess ||
+      hipMalloc(&B.skeys, sizeof(unsigned) * 4 * B.cap) != hipSuccess ||
+      (B.ddt2 && hipMalloc(&B.ddt2_tmp, sizeof(double) * nvar * B.cap) != hipSuccess)) {
+    h->err = "particle allocation failed";
+    return -3;
+  }
+  hipMemsetAsync(B.inflow, 0, sizeof(double) * (h->nr + 2), h->stream);
+  return 0;
+}
+
+// Sort the physical slots of batch b by cell (stable: particles of one cell,
+// and the dead and unused slots, keep their order).  compact: drop the dead
+// particles as well, renumbering the shown slots of the live ones to their
+// ranks (the reference's order, remv_part/injc_part, part_tools.F90:141-225)
+// and setting the count to the live particles.  Device only.
+static int sort_batch(vrh_t *h, PartBatch &B, bool compact) {
+  int rc = particle_buffers(h, B);
+  if (rc) return rc;
+  const int nvar = h->pscheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
+  const long cap = B.cap, nblk = (cap + 255) / 256;
+  long *bsum = B.work, *total = B.work + nblk;
+  hipStream_t st = h->stream;
+  if (compact) {
+    hipMemsetAsync(B.lflag, 0, sizeof(int) * cap, st);
+    hipLaunchKernelGGL(k_slot_flags, dim3((unsigned)nblk), dim3(256), 0, st, B.alive, B.slot, B.d_n, cap, B.lflag);
+    hipLaunchKernelGGL(k_alive_count, dim3((unsigned)nblk), dim3(256), 0, st, B.lflag, B.d_n, cap, bsum);
+    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, st, bsum, nblk, total);
+    hipLaunchKernelGGL(k_slot_rank, dim3((unsigned)nblk), dim3(256), 0, st, B.lflag, B.d_n, cap, bsum, B.lrank);
+  }
+  // keys: cells (nt x (nr + 2) x (nz + 2)), kmax for dead and unused slots
+  const unsigned kmax = (unsigned)((long)h->nt * (h->nr + 2) * (h->nz + 2));
+  int bits = 0;
+  while (bits < 32 && (kmax >> bits) != 0u) bits++;
+  unsigned *kin = B.skeys, *kout = kin + cap, *vin = kout + cap, *vout = vin + cap;
+  const double dth = 2.0 * acos(-1.0) / h->nt, dr = h->rmax / h->nr, dz = h->zmax / h->nz;
+  hipLaunchKernelGGL(k_sort_keys, dim3((unsigned)nblk), dim3(256), 0, st, B.u, B.alive, B.d_n, cap, h->nt, h->nr,
+                     h->nz, dth, dr, dz, kmax, kin, vin);
+  if (!B.stmp) {
+    size_t bytes = 0;
+    if (rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (unsigned)cap, 0u, (unsigned)bits, st) !=
+            hipSuccess ||
+        hipMalloc(&B.stmp, bytes ? bytes : 1) != hipSuccess) {
+      h->err = "particle sort allocation failed";
+      return -3;
+    }
+    B.stmp_bytes = bytes;
+  }
+  size_t bytes = B.stmp_bytes;
+  if (rocprim::radix_sort_pairs(B.stmp, bytes, kin, kout, vin, vout, (unsigned)cap, 0u, (unsigned)bits, st) !=
+      hipSuccess) {
+    h->err = "particle sort failed";
+    return -3;
+  }
+  hipLaunchKernelGGL(k_permute, dim3((unsigned)nblk), dim3(256), 0, st, vout, B.u, B.ddt2, B.alive, B.slot,
+                     compact ? B.lrank : nullptr, total, cap, nvar, B.u_tmp, B.ddt2_tmp, B.alive_tmp, B.slot_tmp);
+  hipMemcpyAsync(B.u, B.u_tmp, sizeof(double) * nvar * cap, hipMemcpyDeviceToDevice, st);
+  if (B.ddt2) hipMemcpyAsync(B.ddt2, B.ddt2_tmp, sizeof(double) * nvar * cap, hipMemcpyDeviceToDevice, st);
+  hipMemcpyAsync(B.alive, B.alive_tmp, sizeof(int) * cap, hipMemcpyDeviceToDevice, st);
+  hipMemcpyAsync(B.slot, B.slot_tmp, sizeof(int) * cap, hipMemcpyDeviceToDevice, st);
+  if (compact) hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, st, B.d_n, total);
+  if (hipGetLastError() != hipSuccess) { h->err = "particle sort failed"; return -3; }
   return 0;
 }
 
 // pack the live particles of batch b to the front (device only)
-static int compact_batch(vrh_t *h, PartBatch &B) {
-  const int nvar = h->pscheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
-  const long nblk = (B.cap + 255) / 256;
-  if (!B.u_tmp) {
-    if (hipMalloc(&B.u_tmp, sizeof(double) * nvar * B.cap) != hipSuccess ||
-        hipMalloc(&B.alive_tmp, sizeof(int) * B.cap) != hipSuccess ||
-        hipMalloc(&B.work, sizeof(long) * (nblk + 8 + (long)(h->nr + 1) * h->nt * 2)) != hipSuccess ||
-        hipMalloc(&B.inflow, sizeof(double) * (h->nr + 2)) != hipSuccess) {
-      h->err = "particle allocation failed";
-      return -3;
-    }
-    if (B.ddt2 && hipMalloc(&B.ddt2_tmp, sizeof(double) * nvar * B.cap) != hipSuccess) {
-      h->err = "particle allocation failed";
-      return -3;
-    }
-    hipMemsetAsync(B.inflow, 0, sizeof(double) * (h->nr + 2), h->stream);
-  }
-  long *bsum = B.work, *total = B.work + nblk;
-  hipLaunchKernelGGL(k_alive_count, dim3((unsigned)nblk), dim3(256), 0, h->stream, B.alive, B.d_n, B.cap, bsum);
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, h->stream, bsum, nblk, total);
-  hipMemsetAsync(B.alive_tmp, 0, sizeof(int) * B.cap, h->stream);
-  hipLaunchKernelGGL(k_compact, dim3((unsigned)nblk), dim3(256), 0, h->stream, B.u, B.ddt2, B.alive, B.d_n, B.cap,
-                     nvar, bsum, B.u_tmp, B.ddt2_tmp, B.alive_tmp);
-  hipMemcpyAsync(B.u, B.u_tmp, sizeof(double) * nvar * B.cap, hipMemcpyDeviceToDevice, h->stream);
-  if (B.ddt2) hipMemcpyAsync(B.ddt2, B.ddt2_tmp, sizeof(double) * nvar * B.cap, hipMemcpyDeviceToDevice, h->stream);
-  hipMemcpyAsync(B.alive, B.alive_tmp, sizeof(int) * B.cap, hipMemcpyDeviceToDevice, h->stream);
-  hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, h->stream, B.d_n, total);
-  return hipGetLastError() == hipSuccess ? 0 : -3;
-}
+static int compact_batch(vrh_t *h, PartBatch &B) { return sort_batch(h, B, true); }
 
 int vrh_particles_compact(vrh_t *h, int batch) {
   if (!h || batch < 0 || batch >= h->nbatch) return -1;
   return compact_batch(h, h->pb[batch]);
+}
+
+int vrh_particles_sort(vrh_t *h, int batch) {
+  if (!h || batch < 0 || batch >= h->nbatch) return -1;
+  return sort_batch(h, h->pb[batch], false);
 }
 
 int vrh_particles_count(vrh_t *h, int batch, long *n) {
@@ -759,8 +911,8 @@ int vrh_particles_inject(vrh_t *h, int batch, double r0, int inj, unsigned long 
   hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, h->stream, offs, (long)jr0 * h->nt, total);
   const long ncell = (long)jr0 * h->nt;
   hipLaunchKernelGGL(k_inject, dim3((unsigned)((ncell + 127) / 128)), dim3(128), 0, h->stream, h->g, h->coef,
-                     h->f.q3, counts, offs, B.d_n, B.cap, nvar, B.u, B.ddt2, B.alive, jr0, dth, dr, dz, seed,
-                     B.ncall);
+                     h->f.q3, counts, offs, B.d_n, B.cap, nvar, B.u, B.ddt2, B.alive, B.slot, jr0, dth, dr, dz,
+                     seed, B.ncall);
   hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, h->stream, B.d_n, total, B.cap);
   B.ncall++;
   if (hipGetLastError() != hipSuccess) { h->err = "particle injection failed"; return -3; }

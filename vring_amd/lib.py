@@ -37,7 +37,7 @@ EXPORTS = ["vrh_create", "vrh_destroy", "vrh_last_error", "vrh_upload_fields", "
            "vrh_ring_init", "vrh_cfl_dt", "vrh_particles_create", "vrh_particles_upload",
            "vrh_particles_download", "vrh_particles_step", "vrh_ktimer_enable", "vrh_ktimer_read", "vrh_autotune",
            "vrh_set_graph", "vrh_set_tuning", "vrh_analysis", "vrh_step_async", "vrh_snapshot_begin",
-           "vrh_snapshot_wait", "vrh_particles_compact", "vrh_particles_count",
+           "vrh_snapshot_wait", "vrh_particles_compact", "vrh_particles_sort", "vrh_particles_count",
            "vrh_particles_inject", "vrh_ensemble_create", "vrh_ensemble_step", "vrh_ensemble_set_batch",
            "vrh_ensemble_last_error",
            "vrh_ensemble_destroy", "vrh_analysis_noalloc"]
@@ -91,6 +91,7 @@ def load_library(path=LIBPATH):
     L.vrh_snapshot_begin.argtypes = [P]
     L.vrh_snapshot_wait.argtypes = [P, dp, dp, dp, dp, ctypes.POINTER(ctypes.c_double)]
     L.vrh_particles_compact.argtypes = [P, I]
+    L.vrh_particles_sort.argtypes = [P, I]
     L.vrh_particles_count.argtypes = [P, I, ctypes.POINTER(ctypes.c_long)]
     L.vrh_particles_inject.argtypes = [P, I, D, I, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_long)]
     L.vrh_set_tuning.argtypes = [P, ip]
@@ -377,6 +378,11 @@ class Solver:
 
     def particles_compact(self, batch=0):
         self._check(self.L.vrh_particles_compact(self.h, batch), "particles_compact")
+
+    def particles_sort(self, batch=0):
+        """Sort the batch's particles in device memory by cell (the order
+        downloads show is unchanged)."""
+        self._check(self.L.vrh_particles_sort(self.h, batch), "particles_sort")
 
     def particles_count(self, batch=0):
         n = ctypes.c_long(0)
