@@ -39,9 +39,8 @@ struct PartArgs {
 // live registers small enough for several waves per SIMD.
 constexpr int kFitRows = 39;
 
-// the pseudo-inverse by value: kernel arguments sit in the constant segment,
-// so its entries are uniform scalar loads the compiler re-reads at will
-// instead of 243 vector registers
+// the pseudo-inverse by value (a kernel argument); the fit kernel copies it
+// to LDS once per block and reads it back in every fit pass
 template <typename T> struct MintArg { T m[243]; };
 
 // cell of a particle and its azimuth (part_tools.F90:519-538); false if
@@ -61,14 +60,18 @@ __device__ inline bool lsq_locate(const PartArgs &a, double xp, double yp, doubl
   return (r == r) && ir >= 1 && ir <= g.nr + 1 && iz >= 1 && iz <= g.nz + 1;
 }
 
+// sin and cos of the particle's azimuth and of twice it, in the work type
+// (shared by the three components' chain rules)
+template <typename T> struct Trig { T sn, cs, sn2, cs2; };
+
 // fit of component ivar (1..3) and its value, Cartesian gradient and
 // Hessian at the particle (part_tools.F90:540-698)
 template <typename T>
-__device__ inline void lsq_component(const PartArgs &a, const MintArg<T> &mt, int ivar, double theta, double r,
-                                     double zp, int it, int ir, int iz, T &fv, T df[3], T ddf[3][3]) {
+__device__ inline void lsq_component(const PartArgs &a, const T *mtl, int ivar, double theta, double r,
+                                     double zp, int it, int ir, int iz, const Trig<T> &tg, T &fv, T df[3],
+                                     T ddf[3][3]) {
   const VGrid &g = a.g;
-  const T sn = (T)sin(theta), cs = (T)cos(theta);
-  const T sn2 = (T)sin(2.0 * theta), cs2 = (T)cos(2.0 * theta);
+  const T sn = tg.sn, cs = tg.cs, sn2 = tg.sn2, cs2 = tg.cs2;
   const T rr = (T)r, r2 = rr * rr;
   const double dx[3] = {a.dtheta, a.dr, a.dz};
   T y[27];
@@ -113,6 +116,11 @@ __device__ inline void lsq_component(const PartArgs &a, const MintArg<T> &mt, in
   for (int q = 0; q < 9; q++) acoef[q] = (T)0;
 #pragma unroll 1
   for (int iint = 1; iint <= 100; iint++) {
+    // the pseudo-inverse entries are read from LDS in every pass: an offset
+    // the compiler cannot see through keeps it from hoisting all 243 of them
+    // out of the loops (they would not fit the scalar registers)
+    int mo = 0;
+    __asm__ volatile("" : "+s"(mo));
     T mp[9];
     T nrm = (T)0;
 #pragma unroll
@@ -125,7 +133,7 @@ __device__ inline void lsq_component(const PartArgs &a, const MintArg<T> &mt, in
                      x1 * x1 * acoef[5] + x1 * x2 * acoef[6] + x2 * x2 * acoef[7] + x2 * x0 * acoef[8];
       const T res = rint - y[kk];
 #pragma unroll
-      for (int q = 0; q < 9; q++) mp[q] = mp[q] - mt.m[q * 27 + kk] * res;
+      for (int q = 0; q < 9; q++) mp[q] = mp[q] - mtl[mo + q * 27 + kk] * res;
     }
 #pragma unroll
     for (int q = 0; q < 9; q++) { acoef[q] = acoef[q] + mp[q]; nrm += mp[q] * mp[q]; }
@@ -242,6 +250,9 @@ __device__ inline void lsq_rotate(double theta, double r, const T *fit, long cap
 // is removed, part_tools.F90:519-538)
 template <typename T>
 __global__ __launch_bounds__(128, sizeof(T) == 4 ? 3 : 2) void k_lsq_fit(PartArgs a, MintArg<T> mt) {
+  __shared__ T mtl[243];
+  for (int q = threadIdx.x; q < 243; q += blockDim.x) mtl[q] = mt.m[q];
+  __syncthreads();
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.cap || p >= *a.n_dev || !a.alive[p]) return;
   const long cap = a.cap;
@@ -256,10 +267,11 @@ __global__ __launch_bounds__(128, sizeof(T) == 4 ? 3 : 2) void k_lsq_fit(PartArg
     return;
   }
   T *fit = reinterpret_cast<T *>(a.fit) + p;
+  const Trig<T> tg{(T)sin(theta), (T)cos(theta), (T)sin(2.0 * theta), (T)cos(2.0 * theta)};
 #pragma unroll 1
   for (int kd = 0; kd < 3; kd++) {
     T f, df[3], ddf[3][3];
-    lsq_component<T>(a, mt, kd + 1, theta, r, zp, it, ir, iz, f, df, ddf);
+    lsq_component<T>(a, mtl, kd + 1, theta, r, zp, it, ir, iz, tg, f, df, ddf);
     T *fr = fit + (long)(kd * 13) * cap;
     fr[0] = f;
 #pragma unroll
