@@ -1360,108 +1360,85 @@ __global__ __launch_bounds__(1024) void k_dctz(TW *__restrict__ S, long nl, int 
   const bool pow2 = n >= 2 && (n & (n - 1)) == 0;
   if (pow2) {
     for (int q = tid; q < n / 2; q += nth) { twl[q] = (TW)twr[q]; twl[n / 2 + q] = (TW)twi[q]; }
-    if (lds_tw) {
+    if (lds_tw)
       for (int q = tid; q < n; q += nth) { etw[q] = (TW)er[q]; etw[n + q] = (TW)ei[q]; }
-      if (dir < 0) __syncthreads();  // the inverse's pre-twiddle reads the table below
+    // Element u of a thread is line lp = tid mod TL (consecutive lanes,
+    // consecutive lines) at position kz0 + u dk, dk = nth / TL (a multiple
+    // of 8, so every position of a thread has the parity of kz0).  The
+    // launch keeps TL n <= 16 nth; lanes with kz0 >= n (TL n < nth) only
+    // help with the LDS phases.  Lines past nl load the last line and are
+    // zeroed (forward) or not stored.
+    constexpr int PF = 16;
+    const int tot = TL * n, lp = tid & (TL - 1), kz0 = tid >> logtl, dk = nth >> logtl;
+    const int nu = min(PF, (tot + nth - 1) / nth);
+    const bool lin = l0 + lp < nl;
+    TW v[PF];
+    {
+      const int kc = min(kz0, n - 1);
+      const TW *src = S + min(l0 + lp, nl - 1) + nl * kc;
+      const long step = nl * dk, last = (long)((n - 1 - kc) / dk) * step;
+#pragma unroll
+      for (int u = 0; u < PF; ++u)
+        if (u < nu) v[u] = src[min(u * step, last)];
     }
-    // batches of elements: every load of a batch (line index clamped, then
-    // masked; twiddles included) is issued before the first is used
-    if (dir > 0) {
-      constexpr int U = 8;
-      const int tot = TL * n;
-      for (int eb = tid; eb < tot; eb += U * nth) {
-        TW v[U];
+    __syncthreads();  // twiddle tables
+    // Makhoul order of position kz: kz/2 for even kz, n-1-kz/2 for odd kz,
+    // for the thread's positions pm0 + u dpm.  Forward: the lines go into
+    // the pair's complex sequence in Makhoul order (and come out in natural
+    // order); inverse: in natural order (and out in Makhoul order).
+    TW *const dpair = ((lp & 1) ? im : re) + (lp >> 1) * sn;
+    const int pm0 = (kz0 & 1) ? n - 1 - (kz0 >> 1) : (kz0 >> 1);
+    const int dpm = (kz0 & 1) ? -(dk >> 1) : (dk >> 1);
+    const int pi0 = dir > 0 ? pm0 : kz0, dpi = dir > 0 ? dpm : dk;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int e = min(eb + u * nth, tot - 1);
-          const int lp = e & (TL - 1), kz = e >> logtl;  // consecutive threads -> consecutive lines
-          v[u] = S[min(l0 + lp, nl - 1) + nl * kz];
-        }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int e = eb + u * nth;
-          if (e < tot) {
-            const int lp = e & (TL - 1), kz = e >> logtl;
-            const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
-            TW *dst = (lp & 1) ? im : re;
-            dst[(lp >> 1) * sn + p] = (l0 + lp < nl) ? v[u] : (TW)0;
-          }
-        }
-      }
-    } else {
-      // W[k] = e^{i pi k/2n} ((Ya[k] + Yb[n-k]) + i (Yb[k] - Ya[n-k])), Y[n]=0
-      constexpr int U = 4;
-      const int ltq = logtl - 1, TQ = TL >> 1, tot = TQ * n;
-      for (int eb = tid; eb < tot; eb += U * nth) {
-        TW ya[U], yb[U], yan[U], ybn[U], cr[U], ci[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int e = min(eb + u * nth, tot - 1);
-          const int q = e & (TQ - 1), k = e >> ltq;
-          const long la = l0 + 2 * q;
-          const long ca = min(la, nl - 1), cb = min(la + 1, nl - 1);
-          const int kn = (n - k) & (n - 1);
-          ya[u] = S[ca + nl * k];
-          yb[u] = S[cb + nl * k];
-          yan[u] = S[ca + nl * kn];
-          ybn[u] = S[cb + nl * kn];
-          cr[u] = ecos(k);  // e^{+i pi k / 2n}
-          ci[u] = esin(k);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int e = eb + u * nth;
-          if (e < tot) {
-            const int q = e & (TQ - 1), k = e >> ltq;
-            const long la = l0 + 2 * q, lb = la + 1;
-            const TW a0 = (la < nl) ? ya[u] : (TW)0, b0 = (lb < nl) ? yb[u] : (TW)0;
-            const TW an = (k > 0 && la < nl) ? yan[u] : (TW)0, bn = (k > 0 && lb < nl) ? ybn[u] : (TW)0;
-            const TW uu = a0 + bn, w = b0 - an;
-            const TW c = cr[u], sv = ci[u];
-            const int slot = q * sn + k;
-            re[slot] = uu * c - w * sv;
-            im[slot] = uu * sv + w * c;
-          }
-        }
-      }
-    }
+    for (int u = 0; u < PF; ++u)
+      if (u < nu && kz0 + u * dk < n) dpair[pi0 + u * dpi] = lin ? v[u] : (TW)0;
     __syncthreads();
+    if (dir < 0) {
+      // in place, one (k, n-k) pair per task (k = 0..n/2, Y[n] = 0):
+      // W[k] = e^{i pi k/2n} ((Ya[k] + Yb[n-k]) + i (Yb[k] - Ya[n-k]))
+      const int ltq = logtl - 1, TQ = TL >> 1, nh = n >> 1, tq = TQ * (nh + 1);
+      for (int e = tid; e < tq; e += nth) {
+        const int q = e & (TQ - 1), k = e >> ltq, kn = (n - k) & (n - 1);
+        TW *rq = re + q * sn, *iq = im + q * sn;
+        const TW a0 = rq[k], b0 = iq[k];
+        const TW an = k > 0 ? rq[kn] : (TW)0, bn = k > 0 ? iq[kn] : (TW)0;
+        {
+          const TW uu = a0 + bn, w = b0 - an, c = ecos(k), sv = esin(k);
+          rq[k] = uu * c - w * sv;
+          iq[k] = uu * sv + w * c;
+        }
+        if (k > 0 && k < nh) {
+          const TW uu = an + b0, w = bn - a0, c = ecos(kn), sv = esin(kn);
+          rq[kn] = uu * c - w * sv;
+          iq[kn] = uu * sv + w * c;
+        }
+      }
+      __syncthreads();
+    }
     lds_fft(re, im, TL / 2, logn, sn, twl, twl + n / 2, (TW)(dir > 0 ? -1 : 1), tid, nth);
     const TW inv_n = (TW)1 / (TW)n;  // exact: n is a power of two
-    constexpr int US = 4;
-    for (int eb = tid; eb < TL * n; eb += US * nth) {
-      TW cr[US], ci[US];
-      if (dir > 0) {
+    if (kz0 < n && lin) {
+      TW *dst = S + (l0 + lp) + nl * kz0;
+      const long step = nl * dk;
+      const TW *zq = re + (lp >> 1) * sn, *zi = im + (lp >> 1) * sn;
 #pragma unroll
-        for (int u = 0; u < US; ++u) {
-          const int kz = min(eb + u * nth, TL * n - 1) >> logtl;
-          cr[u] = ecos(kz);
-          ci[u] = esin(kz);
+      for (int u = 0; u < PF; ++u) {
+        const int kz = kz0 + u * dk;
+        if (u < nu && kz < n) {
+          TW outv;
+          if (dir > 0) {
+            const int kr = (n - kz) & (n - 1);
+            const TW zr = zq[kz], zim = zi[kz], wr = zq[kr], wi = zi[kr];
+            TW vr, vi;
+            if ((lp & 1) == 0) { vr = (TW)0.5 * (zr + wr); vi = (TW)0.5 * (zim - wi); }
+            else { vr = (TW)0.5 * (zim + wi); vi = (TW)(-0.5) * (zr - wr); }
+            outv = (vr * ecos(kz) + vi * esin(kz)) * inv_n;  // Re(e^{-i pi k/2n} V)/n
+          } else {
+            outv = dpair[pm0 + u * dpm];
+          }
+          dst[u * step] = outv;
         }
-      }
-#pragma unroll
-      for (int u = 0; u < US; ++u) {
-        const int e = eb + u * nth;
-        if (e >= TL * n) break;
-        const int lp = e & (TL - 1), kz = e >> logtl;
-        const long l = l0 + lp;
-        if (l >= nl) continue;
-        const int q = lp >> 1;
-        TW outv;
-        if (dir > 0) {
-          const int k = kz, kr = (n - k) & (n - 1);
-          const TW zr = re[q * sn + k], zi = im[q * sn + k], wr = re[q * sn + kr], wi = im[q * sn + kr];
-          TW vr, vi;
-          if ((lp & 1) == 0) { vr = (TW)0.5 * (zr + wr); vi = (TW)0.5 * (zi - wi); }
-          else { vr = (TW)0.5 * (zi + wi); vi = (TW)(-0.5) * (zr - wr); }
-          outv = (vr * cr[u] + vi * ci[u]) * inv_n;  // Re(e^{-i pi k/2n} V)/n
-        } else {
-          const int p = (kz & 1) ? n - 1 - (kz >> 1) : (kz >> 1);
-          outv = (lp & 1) ? im[q * sn + p] : re[q * sn + p];
-        }
-        S[l + nl * kz] = outv;
       }
     }
   } else {
