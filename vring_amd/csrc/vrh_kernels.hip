@@ -1629,6 +1629,46 @@ __global__ __launch_bounds__(256) void k_ifft(VGrid g, const TW *__restrict__ S,
     __syncthreads();
     lds_fft(re, im, LB / 2, logn, N, twl, twl + N / 2, (TW)1, tid, nth);
   }
+  if (pow2 && (nth & (N - 1)) == 0) {
+    // N divides the block: every element of a thread has the same theta
+    // index ii, so the low-mode twiddles of the merge are loaded once
+    const int ii = tid & (N - 1), lstep = nth >> logn;
+    double cw1 = 0.0, sw1 = 0.0, cw2 = 0.0, sw2 = 0.0;
+    if (S64) {
+      int ph2 = 2 * ii;
+      if (ph2 >= N) ph2 -= N;
+      cw1 = tw64[ii]; sw1 = tw64[N + ii];
+      if (3 < N) { cw2 = tw64[ph2]; sw2 = tw64[N + ph2]; }
+    }
+    for (int lp = tid >> logn; lp < LB; lp += lstep) {
+      const long l = l0 + lp;
+      if (l >= nlines) break;
+      const int q = lp >> 1;
+      double v = (lp & 1) ? (double)im[q * N + ii] : (double)re[q * N + ii];
+      if (S64) {
+        // the terms of the generic loop below, in its order
+        const double *t = s64l + 5 * lp;
+        v += t[0];
+        {
+          const double a = t[1], b = (2 < N) ? t[2] : 0.0;
+          if (2 == N) v += 0.5 * a * cw1;
+          else v += a * cw1 + b * sw1;
+        }
+        if (3 < N) {
+          const double a = t[3], b = (4 < N) ? t[4] : 0.0;
+          if (4 == N) v += 0.5 * a * cw2;
+          else v += a * cw2 + b * sw2;
+        }
+      }
+      const int j = jk[2 * lp], k = jk[2 * lp + 1];
+      if (phi_out) phi_out[cidx(g, ii + 1, j, k)] = v;
+      if (dph) {
+        dph[hidx(g, ii + 1, j, k)] = v;
+        if (k == g.nz) dph[hidx(g, ii + 1, j, g.nzc)] = v;
+      }
+    }
+    return;
+  }
   for (int e = tid; e < LB * N; e += nth) {
     const int lp = pow2 ? e >> logn : e / N, ii = e - lp * N;
     const long l = l0 + lp;
