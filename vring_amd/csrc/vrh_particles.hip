@@ -834,8 +834,9 @@ static int sort_batch(vrh_t *h, PartBatch &B, bool compact) {
   const long cap = B.cap, nblk = (cap + 255) / 256;
   long *bsum = B.work, *total = B.work + nblk;
   hipStream_t st = h->stream;
+  bool ok = true;
   if (compact) {
-    hipMemsetAsync(B.lflag, 0, sizeof(int) * cap, st);
+    ok = hipMemsetAsync(B.lflag, 0, sizeof(int) * cap, st) == hipSuccess;
     hipLaunchKernelGGL(k_slot_flags, dim3((unsigned)nblk), dim3(256), 0, st, B.alive, B.slot, B.d_n, cap, B.lflag);
     hipLaunchKernelGGL(k_alive_count, dim3((unsigned)nblk), dim3(256), 0, st, B.lflag, B.d_n, cap, bsum);
     hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, st, bsum, nblk, total);
@@ -867,12 +868,14 @@ static int sort_batch(vrh_t *h, PartBatch &B, bool compact) {
   }
   hipLaunchKernelGGL(k_permute, dim3((unsigned)nblk), dim3(256), 0, st, vout, B.u, B.ddt2, B.alive, B.slot,
                      compact ? B.lrank : nullptr, total, cap, nvar, B.u_tmp, B.ddt2_tmp, B.alive_tmp, B.slot_tmp);
-  hipMemcpyAsync(B.u, B.u_tmp, sizeof(double) * nvar * cap, hipMemcpyDeviceToDevice, st);
-  if (B.ddt2) hipMemcpyAsync(B.ddt2, B.ddt2_tmp, sizeof(double) * nvar * cap, hipMemcpyDeviceToDevice, st);
-  hipMemcpyAsync(B.alive, B.alive_tmp, sizeof(int) * cap, hipMemcpyDeviceToDevice, st);
-  hipMemcpyAsync(B.slot, B.slot_tmp, sizeof(int) * cap, hipMemcpyDeviceToDevice, st);
+  ok = ok && hipMemcpyAsync(B.u, B.u_tmp, sizeof(double) * nvar * cap, hipMemcpyDeviceToDevice, st) == hipSuccess;
+  if (B.ddt2)
+    ok = ok && hipMemcpyAsync(B.ddt2, B.ddt2_tmp, sizeof(double) * nvar * cap, hipMemcpyDeviceToDevice, st) ==
+                   hipSuccess;
+  ok = ok && hipMemcpyAsync(B.alive, B.alive_tmp, sizeof(int) * cap, hipMemcpyDeviceToDevice, st) == hipSuccess;
+  ok = ok && hipMemcpyAsync(B.slot, B.slot_tmp, sizeof(int) * cap, hipMemcpyDeviceToDevice, st) == hipSuccess;
   if (compact) hipLaunchKernelGGL(k_set_count, dim3(1), dim3(1), 0, st, B.d_n, total);
-  if (hipGetLastError() != hipSuccess) { h->err = "particle sort failed"; return -3; }
+  if (!ok || hipGetLastError() != hipSuccess) { h->err = "particle sort failed"; return -3; }
   return 0;
 }
 
