@@ -280,3 +280,47 @@ def test_lsq_particles_at_workload_sizes_match_oracle(grid, n, sample):
     print(f"particle rel err {err:.3e} on {m} particles; max displacement {moved:.3e}", flush=True)
     assert moved > 1e-6
     assert err < 1e-10
+
+
+def test_trilinear_particles_at_workload_size_match_oracle():
+    """10^6 trilinear-RK3 particles on the headline grid (128x256x512), in
+    random upload order, so that the cell sort of upload (and one more sort
+    between stages) reorders them in device memory: three RK3 stages, and a
+    random sample of 20 000 particles stepped by the oracle from the same
+    initial states must agree by their uploaded indices (FP64)."""
+    nt, nr, nz = 128, 256, 512
+    n, sample = 1_000_000, 20_000
+    print(f"case trilinear particles at size {(nt, nr, nz)} n={n} sample={sample}", flush=True)
+    rmax, zmax, dt = 4.0, 12.0, 0.01
+    q = vring_amd.ring_initial_condition(nt, nr, nz, rmax, zmax, [(1.0, 3.0, 0.2, 1.0)], pert_amp=1e-3,
+                                         pert_modes=(2, 8), phases=np.arange(7.0))
+    s = vring_amd.Solver(nt, nr, nz, rmax, zmax, 3000.0, dt)
+    for name, a in zip(("q1", "q2", "q3"), q):
+        s.upload(name, a)
+    st = stokes_number(20e-6 / 0.02, 1000.0)
+    u = np.zeros((n, 6))
+    u[:, :3] = torus_points(n, 1.0, 3.0, 0.6, 9)
+    u[:, 3:6] = 0.05 * np.random.default_rng(3).standard_normal((n, 3))
+    s.particles_create([n], [st], [1], "trilinear_rk3")
+    s.particles_upload(0, u)
+    for irk in (1, 2, 3):
+        s.particles_step(irk)
+        if irk == 2:
+            s.particles_sort(0)
+    g, gal = s.particles_download(0, n)
+    s.close()
+    idx = np.sort(np.random.default_rng(13).choice(n, sample, replace=False))
+    o = Oracle(nt, nr, nz, rmax, zmax, 3000.0, dt)
+    for name, a in zip(("q1", "q2", "q3"), q):
+        o.field(name)[...] = a
+    xv = np.ascontiguousarray(u[idx])
+    r2, alive = np.zeros((sample, 6)), np.ones(sample, np.int32)
+    for irk in (1, 2, 3):
+        o.L.orc_part_rk3_trilinear(o.h, sample, st, irk, dt, xv.ravel(), r2.ravel(), alive)
+    o.close()
+    np.testing.assert_array_equal(gal[idx], alive)
+    err = np.abs(g[idx] - xv).max() / np.abs(xv).max()
+    moved = np.abs(g[idx, :3] - u[idx, :3]).max()
+    print(f"particle rel err {err:.3e} on {sample} particles; max displacement {moved:.3e}", flush=True)
+    assert moved > 1e-6
+    assert err < 1e-11
