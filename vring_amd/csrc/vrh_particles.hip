@@ -246,15 +246,11 @@ __device__ inline void lsq_rotate(double theta, double r, const T *fit, long cap
   }
 }
 
-// one substep, part 1: the fits (a particle whose stencil leaves the arrays
-// is removed, part_tools.F90:519-538)
+// one substep, part 1: the fits of live particle p into 39 rows fit[v fs]
+// (a particle whose stencil leaves the arrays is removed, part_tools.F90:
+// 519-538, and false returned)
 template <typename T>
-__global__ __launch_bounds__(128, sizeof(T) == 4 ? 3 : 2) void k_lsq_fit(PartArgs a, MintArg<T> mt) {
-  __shared__ T mtl[243];
-  for (int q = threadIdx.x; q < 243; q += blockDim.x) mtl[q] = mt.m[q];
-  __syncthreads();
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.cap || p >= *a.n_dev || !a.alive[p]) return;
+__device__ inline bool lsq_fit_particle(const PartArgs &a, const T *mtl, long p, T *fit, long fs) {
   const long cap = a.cap;
   double *u = a.u + p;
 #define U(v) u[(long)(v) * cap]
@@ -264,30 +260,29 @@ __global__ __launch_bounds__(128, sizeof(T) == 4 ? 3 : 2) void k_lsq_fit(PartArg
   if (!lsq_locate(a, U(0), U(1), zp, theta, r, it, ir, iz)) {
     a.alive[p] = 0;
     for (int v = 0; v < a.nvar; v++) U(v) = 0.0;
-    return;
+    return false;
   }
-  T *fit = reinterpret_cast<T *>(a.fit) + p;
   const Trig<T> tg{(T)sin(theta), (T)cos(theta), (T)sin(2.0 * theta), (T)cos(2.0 * theta)};
 #pragma unroll 1
   for (int kd = 0; kd < 3; kd++) {
     T f, df[3], ddf[3][3];
     lsq_component<T>(a, mtl, kd + 1, theta, r, zp, it, ir, iz, tg, f, df, ddf);
-    T *fr = fit + (long)(kd * 13) * cap;
+    T *fr = fit + (long)(kd * 13) * fs;
     fr[0] = f;
 #pragma unroll
-    for (int q = 0; q < 3; q++) fr[(long)(1 + q) * cap] = df[q];
+    for (int q = 0; q < 3; q++) fr[(long)(1 + q) * fs] = df[q];
 #pragma unroll
-    for (int q = 0; q < 9; q++) fr[(long)(4 + q) * cap] = ddf[q % 3][q / 3];
+    for (int q = 0; q < 9; q++) fr[(long)(4 + q) * fs] = ddf[q % 3][q / 3];
   }
 #undef U
+  return true;
 }
 
-// one substep, part 2: rotation, rates and state update (part_tools.F90:
-// 700-930), then removal of particles that left the domain (:962-1025)
+// one substep, part 2: rotation of the fit rows fit[v fs], rates and state
+// update of live particle p (part_tools.F90:700-930), then removal of
+// particles that left the domain (:962-1025)
 template <typename T>
-__global__ __launch_bounds__(128, 2) void k_lsq_step(PartArgs a) {
-  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.cap || p >= *a.n_dev || !a.alive[p]) return;
+__device__ inline void lsq_step_particle(const PartArgs &a, long p, const T *fit, long fs) {
   const long cap = a.cap;
   double *u = a.u + p;      // u[v*cap]
   double *d2 = a.ddt2 ? a.ddt2 + p : nullptr;
@@ -302,7 +297,7 @@ __global__ __launch_bounds__(128, 2) void k_lsq_step(PartArgs a) {
       if (xp >= 0.0 && yp >= 0.0) theta = acos(xp / r);
       else if (xp < 0.0 && yp >= 0.0) theta = acos(xp / r);
       else theta = acos(-xp / r) + pi;
-      lsq_rotate<T>(theta, r, reinterpret_cast<const T *>(a.fit) + p, cap, vel, dvel, ddvel);
+      lsq_rotate<T>(theta, r, fit, fs, vel, dvel, ddvel);
     }
     // rates (part_tools.F90:836-870) and the update of states 1..78
     // (part_tools.F90:926-930 Euler; dertest RK3 form), state by state: every
@@ -373,6 +368,41 @@ __global__ __launch_bounds__(128, 2) void k_lsq_step(PartArgs a) {
     }
     }
 #undef U
+}
+
+// the pseudo-inverse of a block in LDS, then the fits into global rows
+template <typename T>
+__global__ __launch_bounds__(128, sizeof(T) == 4 ? 3 : 2) void k_lsq_fit(PartArgs a, MintArg<T> mt) {
+  __shared__ T mtl[243];
+  for (int q = threadIdx.x; q < 243; q += blockDim.x) mtl[q] = mt.m[q];
+  __syncthreads();
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.cap || p >= *a.n_dev || !a.alive[p]) return;
+  lsq_fit_particle<T>(a, mtl, p, reinterpret_cast<T *>(a.fit) + p, a.cap);
+}
+
+template <typename T>
+__global__ __launch_bounds__(128, 2) void k_lsq_step(PartArgs a) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.cap || p >= *a.n_dev || !a.alive[p]) return;
+  lsq_step_particle<T>(a, p, reinterpret_cast<const T *>(a.fit) + p, a.cap);
+}
+
+// both parts in one kernel: the 39 fit rows of a particle stay in LDS
+// (each thread reads back only its own rows), which spares their round trip
+// through HBM and lets one wave's fit arithmetic overlap other waves' state
+// traffic
+template <typename T>
+__global__ __launch_bounds__(128, 2) void k_lsq_substep(PartArgs a, MintArg<T> mt) {
+  __shared__ T mtl[243];
+  __shared__ T fl[kFitRows * 128];
+  for (int q = threadIdx.x; q < 243; q += blockDim.x) mtl[q] = mt.m[q];
+  __syncthreads();
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.cap || p >= *a.n_dev || !a.alive[p]) return;
+  T *fit = fl + threadIdx.x;
+  if (!lsq_fit_particle<T>(a, mtl, p, fit, 128)) return;
+  lsq_step_particle<T>(a, p, fit, 128);
 }
 
 // ---- trilinear option: (u_theta, u_r, u_z) from the staggered grids ----
@@ -693,10 +723,11 @@ void launch_particles(vrh_t *h, int irk) {
       MintArg<float> mf;
       MintArg<double> md;
       for (int q = 0; q < 243; q++) { mf.m[q] = (float)h->h_mint[q]; md.m[q] = h->h_mint[q]; }
+      // mixed: fit and update in one kernel (fit rows in LDS, 2 % faster);
+      // FP64: two kernels (the fused one spills at 256 VGPRs, 3 % slower)
       for (int isub = 0; isub < b.nsub; isub++) {
         if (h->prec == VRH_MIXED) {
-          hipLaunchKernelGGL(k_lsq_fit<float>, grd, blk, 0, h->stream, a, mf);
-          hipLaunchKernelGGL(k_lsq_step<float>, grd, blk, 0, h->stream, a);
+          hipLaunchKernelGGL(k_lsq_substep<float>, grd, blk, 0, h->stream, a, mf);
         } else {
           hipLaunchKernelGGL(k_lsq_fit<double>, grd, blk, 0, h->stream, a, md);
           hipLaunchKernelGGL(k_lsq_step<double>, grd, blk, 0, h->stream, a);
@@ -737,7 +768,8 @@ int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((B.cap + 255) / 256)), dim3(256), 0, h->stream, B.slot, B.cap);
     B.ddt2 = nullptr;
     B.fit = nullptr;
-    if (scheme != VRH_PART_TRILINEAR_RK3 &&
+    // global fit rows: the FP64 LSQ path only (the mixed path keeps them in LDS)
+    if (scheme != VRH_PART_TRILINEAR_RK3 && h->prec != VRH_MIXED &&
         hipMalloc(&B.fit, sizeof(double) * kFitRows * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
     if (scheme != VRH_PART_LSQ_EULER) {
       if (hipMalloc(&B.ddt2, sizeof(double) * nvar * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
