@@ -1068,7 +1068,7 @@ template <typename T> __device__ __forceinline__ void dft8(T *vr, T *vi, T sign)
 
 // radix of pass with `bits` bits still to do: 8 while the rest stays a
 // multiple of 3, else 4 (2 for a lone bit)
-__host__ __device__ inline int fft_pass_bits(int bits) {
+__host__ __device__ constexpr int fft_pass_bits(int bits) {
   if (bits % 3 == 0) return 3;
   return bits == 1 ? 1 : 2;
 }
@@ -1085,10 +1085,17 @@ template <typename T> constexpr int fft_qt() { return 1; }
 // One radix-2^LR pass of the Stockham autosort FFT (ls bits already done):
 // every butterfly reads its R inputs at stride n/R, applies the twiddles,
 // and after the barrier writes the in-register DFT at stride 2^ls.
-template <typename T, int LR, int QT>
-__device__ __forceinline__ void fft_pass(T *re, T *im, int nfft, int logn, int sstr, int ls, const T *twr,
+// One radix-2^LR pass of the Stockham autosort FFT (ls bits already done):
+// every butterfly reads its R inputs at stride n/R, applies the twiddles,
+// and after the barrier writes the in-register DFT at stride 2^ls.  The
+// length and pass (logn, ls) come either from template arguments (CL > 0:
+// logn = CL, ls = CS; the LDS offsets r << lg then fold into immediates) or
+// from the run-time arguments (CL = 0).
+template <typename T, int LR, int QT, int CL = 0, int CS = 0>
+__device__ __forceinline__ void fft_pass(T *re, T *im, int nfft, int logn_rt, int sstr, int ls_rt, const T *twr,
                                          const T *twi, T sign, int tid, int nth) {
   constexpr int R = 1 << LR;
+  const int logn = CL > 0 ? CL : logn_rt, ls = CL > 0 ? CS : ls_rt;
   const int nh = 1 << (logn - 1), lg = logn - LR;
   const int tasks = nfft << lg;
   T vr[QT][R], vi[QT][R];
@@ -1130,6 +1137,17 @@ __device__ __forceinline__ void fft_pass(T *re, T *im, int nfft, int logn, int s
   __syncthreads();
 }
 
+// the passes of a transform of compile-time length 2^CL from pass CS on
+template <typename T, int QT, int CL, int CS>
+__device__ __forceinline__ void lds_fft_c(T *re, T *im, int nfft, int sstr, const T *twr, const T *twi, T sign,
+                                          int tid, int nth) {
+  if constexpr (CS < CL) {
+    constexpr int LR = fft_pass_bits(CL - CS);
+    fft_pass<T, LR, QT, CL, CS>(re, im, nfft, CL, sstr, CS, twr, twi, sign, tid, nth);
+    lds_fft_c<T, QT, CL, CS + LR>(re, im, nfft, sstr, twr, twi, sign, tid, nth);
+  }
+}
+
 // In-place Stockham autosort FFT of nfft sequences of length n = 2^logn in
 // LDS (re, im; sequence f starts at f * sstr), natural order in and out,
 // radix-8/4/2 passes with the DFT in
@@ -1141,6 +1159,17 @@ template <typename T>
 __device__ __forceinline__ void lds_fft(T *re, T *im, int nfft, int logn, int sstr, const T *twr, const T *twi, T sign,
                                         int tid, int nth) {
   constexpr int QT = fft_qt<T>();
+  // the lengths of the workloads' theta and z transforms: passes with
+  // compile-time geometry
+  switch (logn) {
+    case 5: lds_fft_c<T, QT, 5, 0>(re, im, nfft, sstr, twr, twi, sign, tid, nth); return;
+    case 6: lds_fft_c<T, QT, 6, 0>(re, im, nfft, sstr, twr, twi, sign, tid, nth); return;
+    case 7: lds_fft_c<T, QT, 7, 0>(re, im, nfft, sstr, twr, twi, sign, tid, nth); return;
+    case 8: lds_fft_c<T, QT, 8, 0>(re, im, nfft, sstr, twr, twi, sign, tid, nth); return;
+    case 9: lds_fft_c<T, QT, 9, 0>(re, im, nfft, sstr, twr, twi, sign, tid, nth); return;
+    case 10: lds_fft_c<T, QT, 10, 0>(re, im, nfft, sstr, twr, twi, sign, tid, nth); return;
+    default: break;
+  }
   int ls = 0;
   while (ls < logn) {
     const int lr = fft_pass_bits(logn - ls);
