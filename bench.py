@@ -666,7 +666,7 @@ def main():
         summarize_traffic(args.summarize_traffic, os.path.join(ROOT, "profiles", "traffic_256x128x512.json"))
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        print("bench.py: vring_amd runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
