@@ -212,6 +212,32 @@ def run_particles(prec, q, dt, n, steps, scheme="lsq_euler"):
             "sort_ms": round(sort_ms, 3)}
 
 
+def particle_errors(q, dt, n, scheme):
+    """The mixed path's particle step against the FP64 one from the same
+    field and particles: one RK3 step's particle work (LSQ-Euler: five
+    sub-steps at irk = 1; trilinear RK3: its three stages), then the largest
+    difference of the live particles' positions and of their velocities,
+    each relative to the largest FP64 magnitude."""
+    import vring_amd
+    res = {}
+    for prec in ("fp64", "mixed"):
+        s = vring_amd.Solver(NT, NR, NZ, RMAX, ZMAX, RE, dt, precision=prec)
+        for name, a in zip(("q1", "q2", "q3"), q):
+            s.upload(name, a)
+        nsub = 5 if scheme == "lsq_euler" else 1
+        s.seed_particles_torus(n, R=1.0, z0=3.0, a=0.6, seed=0, nsub=nsub, scheme=scheme)
+        for irk in ((1,) if scheme == "lsq_euler" else (1, 2, 3)):
+            s.particles_step(irk)
+        u, alive = s.particles_download(0, n)
+        res[prec] = (u[:, :6].copy(), alive)
+        s.close()
+    (u64, a64), (umx, amx) = res["fp64"], res["mixed"]
+    live = (a64 == 1) & (amx == 1)
+    rel = lambda c: float(np.abs(umx[live][:, c] - u64[live][:, c]).max() / np.abs(u64[live][:, c]).max())
+    return {"particles": n, "positions": rel(slice(0, 3)), "velocities": rel(slice(3, 6)),
+            "alive_differs": int((a64 != amx).sum())}
+
+
 def ensemble_specs(nrings, seed=0):
     """(nr, ntheta, nz, Re, sigma) of the batched-ensemble rings: shapes from
     {64, 96, 128, 192} x {32, 64, 128} x {128, 256, 512}, Re ~ U[500, 5000],
@@ -540,7 +566,7 @@ def copy_calibration(mib, reps=3):
 
 KERNEL_KEYS = {  # rocprof kernel name prefix -> kernel timer name
     "k_explicit_cell": "explicit", "k_explicit_march": "explicit", "k_explicit_wave": "explicit",
-    "k_explicit": "explicit", "k_lsq_fit": "particles", "k_lsq_step": "particles",
+    "k_explicit": "explicit", "k_lsq_fit": "particles", "k_lsq_step": "particles", "k_lsq_substep": "particles",
     "k_project": "project", "k_divfft": "divfft", "k_dctz": "dct", "k_radial": "radial", "k_ifft": "ifft",
     "k_hdnbc": "hdnbc", "k_part_lsq": "particles", "k_part_tri": "particles", "k_scalars": "scalars",
 }
@@ -686,8 +712,11 @@ def main():
     q, dt = ring_state()
     ncell = NT * NR * NZ
     if args.only == "particles":
-        print(json.dumps({f"{prec}_{sch}": run_particles(prec, q, dt, args.particles, 5, sch)
-                          for prec in ("fp64", "mixed") for sch in ("lsq_euler", "trilinear_rk3")}))
+        rows = {f"{prec}_{sch}": run_particles(prec, q, dt, args.particles, 5, sch)
+                for prec in ("fp64", "mixed") for sch in ("lsq_euler", "trilinear_rk3")}
+        rows["mixed_rel_err_vs_fp64"] = {sch: particle_errors(q, dt, args.particles, sch)
+                                         for sch in ("lsq_euler", "trilinear_rk3")}
+        print(json.dumps(rows))
         return
     if args.only:
         wall, _, kt, tune = run_path(args.only, q, dt, args.steps, args.warmup, args.autotune_reps)
@@ -703,6 +732,8 @@ def main():
         ps = max(2, args.steps // 4)
         parts = {f"{prec}_{sch}": run_particles(prec, q, dt, args.particles, ps, sch)
                  for prec in ("fp64", "mixed") for sch in ("lsq_euler", "trilinear_rk3")}
+        parts["mixed_rel_err_vs_fp64"] = {sch: particle_errors(q, dt, args.particles, sch)
+                                          for sch in ("lsq_euler", "trilinear_rk3")}
     v64 = ncell * args.steps / wall64 / 1e6
     vmx = ncell * args.steps / wallmx / 1e6
     err = error_run(q, dt, args.error_steps) if args.error_steps else {"timed_run": rel_errors(fmx, f64)}

@@ -324,3 +324,36 @@ def test_trilinear_particles_at_workload_size_match_oracle():
     print(f"particle rel err {err:.3e} on {sample} particles; max displacement {moved:.3e}", flush=True)
     assert moved > 1e-6
     assert err < 1e-11
+
+
+@pytest.mark.parametrize("scheme", ["lsq_euler", "trilinear_rk3"])
+def test_mixed_particles_within_1e6_of_fp64(scheme):
+    """The mixed path's particle step (FP32 interpolation and rates, FP64
+    state) against the FP64 path from the same ring field and 10^5
+    particles on the headline grid: one step's particle work, positions and
+    velocities within 1e-6 of the FP64 ones (relative to their largest
+    magnitude), the same particles alive."""
+    nt, nr, nz = 128, 256, 512
+    n = 100_000
+    print(f"case mixed vs fp64 particles {scheme} n={n}", flush=True)
+    rmax, zmax, dt = 4.0, 12.0, 0.006
+    q = vring_amd.ring_initial_condition(nt, nr, nz, rmax, zmax, [(1.0, 3.0, 0.2, 1.0)], pert_amp=1e-3,
+                                         pert_modes=(2, 8), phases=np.arange(7.0))
+    out = {}
+    for prec in ("fp64", "mixed"):
+        s = vring_amd.Solver(nt, nr, nz, rmax, zmax, 3000.0, dt, precision=prec)
+        for name, a in zip(("q1", "q2", "q3"), q):
+            s.upload(name, a)
+        s.seed_particles_torus(n, R=1.0, z0=3.0, a=0.6, seed=4, nsub=5 if scheme == "lsq_euler" else 1,
+                               scheme=scheme)
+        for irk in ((1,) if scheme == "lsq_euler" else (1, 2, 3)):
+            s.particles_step(irk)
+        out[prec] = s.particles_download(0, n)
+        s.close()
+    (u64, a64), (umx, amx) = out["fp64"], out["mixed"]
+    np.testing.assert_array_equal(amx, a64)
+    live = a64 == 1
+    pos = np.abs(umx[live, :3] - u64[live, :3]).max() / np.abs(u64[live, :3]).max()
+    vel = np.abs(umx[live, 3:6] - u64[live, 3:6]).max() / np.abs(u64[live, 3:6]).max()
+    print(f"mixed vs fp64: positions {pos:.2e}, velocities {vel:.2e}", flush=True)
+    assert pos < 1e-6 and vel < 1e-6
