@@ -491,7 +491,7 @@ __device__ __forceinline__ double pu01(unsigned long long seed, unsigned long lo
   return ((psmix(seed ^ psmix(i)) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
 
-// live slots per block of 256
+// nonzero flags among the first n entries, per block of 256
 __global__ void k_alive_count(const int *__restrict__ alive, const long *n_dev, long cap, long *__restrict__ bsum) {
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
   const long n = *n_dev;
@@ -521,30 +521,6 @@ __global__ void k_scan1(long *v, long m, long *total) {
   __syncthreads();
   long acc = part[t];
   for (long q = b0; q < b1; q++) { const long x = v[q]; v[q] = acc; acc += x; }
-}
-
-// scatter the live particles of each block to boff[block] + rank
-__global__ void k_compact(const double *__restrict__ u, const double *__restrict__ d2, const int *__restrict__ alive,
-                          const long *n_dev, long cap, int nvar, const long *__restrict__ boff,
-                          double *__restrict__ ut, double *__restrict__ d2t, int *__restrict__ at) {
-  const long p = (long)blockIdx.x * 256 + threadIdx.x;
-  const long n = *n_dev;
-  const int a = (p < cap && p < n && alive[p]) ? 1 : 0;
-  __shared__ int cnt[256];
-  cnt[threadIdx.x] = a;
-  __syncthreads();
-  for (int o = 1; o < 256; o <<= 1) {  // inclusive scan by doubling
-    const int x = ((int)threadIdx.x >= o) ? cnt[threadIdx.x - o] : 0;
-    __syncthreads();
-    cnt[threadIdx.x] += x;
-    __syncthreads();
-  }
-  if (!a) return;
-  const long dst = boff[blockIdx.x] + cnt[threadIdx.x] - 1;
-  for (int v = 0; v < nvar; v++) ut[(long)v * cap + dst] = u[(long)v * cap + p];
-  if (d2)
-    for (int v = 0; v < nvar; v++) d2t[(long)v * cap + dst] = d2[(long)v * cap + p];
-  at[dst] = 1;
 }
 
 __global__ void k_set_count(long *n_dev, const long *total) { *n_dev = *total; }
