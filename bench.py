@@ -271,16 +271,26 @@ def run_ensemble(nrings, steps, warmup, seed=0, particles=50000):
     from {64, 96, 128, 192} x {32, 64, 128} x {128, 256, 512}, Re ~ U[500, 5000],
     core sigma ~ U[0.1, 0.3] (numpy PCG64(seed)), mixed precision, `particles`
     trilinear-RK3 particles each, advanced by one graph launch per RK3 step.
-    Returns aggregate Mcell-RK3-steps/s and the same rings stepped one after
-    another with their own graphs."""
+    Each distinct grid shape is autotuned once (its first ring) and the
+    others of that shape take the same launch shapes.  Returns aggregate
+    Mcell-RK3-steps/s and the same rings stepped one after another with
+    their own graphs."""
     import vring_amd
-    members, shapes, cells = [], [], 0
+    members, shapes, cells, tuned = [], [], 0, {}
+    t_tune = time.perf_counter()
     for i, spec in enumerate(ensemble_specs(nrings, seed)):
         s = ensemble_member(spec, particles, i)
         nr, nt, nz = spec[:3]
+        if (nr, nt, nz) not in tuned:
+            tuned[(nr, nt, nz)] = s.autotune(reps=2)
+        else:
+            c = tuned[(nr, nt, nz)]
+            s.set_tuning(M=[c[k] for k in ("M_r1", "M_r3", "M_r2", "M_z1", "M_z3", "M_z2")],
+                         fft_lines=c["fft_lines"], dct_lines=c["dct_lines"], explicit_form=c["explicit_form"])
         members.append(s)
         shapes.append((nr, nt, nz))
         cells += nr * nt * nz
+    t_tune = time.perf_counter() - t_tune
     ens = vring_amd.Ensemble(members)
     ens.step(warmup)
     t0 = time.perf_counter()
@@ -304,6 +314,7 @@ def run_ensemble(nrings, steps, warmup, seed=0, particles=50000):
             "Mcell_RK3_steps_per_s": round(cells * steps / t_ens / 1e6, 2),
             "one_launch_per_run_Mcell_RK3_steps_per_s": round(cells * steps / t_batch / 1e6, 2),
             "one_by_one_Mcell_RK3_steps_per_s": round(cells * steps / t_seq / 1e6, 2),
+            "autotuned_shapes": len(tuned), "setup_and_tuning_s": round(t_tune, 1),
             "shapes_nr_nt_nz": shapes}
 
 
