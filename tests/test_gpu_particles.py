@@ -357,3 +357,38 @@ def test_mixed_particles_within_1e6_of_fp64(scheme):
     vel = np.abs(umx[live, 3:6] - u64[live, 3:6]).max() / np.abs(u64[live, 3:6]).max()
     print(f"mixed vs fp64: positions {pos:.2e}, velocities {vel:.2e}", flush=True)
     assert pos < 1e-6 and vel < 1e-6
+
+
+def test_sort_and_compaction_with_lost_and_nan_particles():
+    """Particles the step removes (outside the domain after the step, or
+    with a NaN coordinate) go through the cell sort like any other: the
+    step marks exactly the oracle's removals, downloads keep the uploaded
+    order, and compaction packs the survivors in that order."""
+    print("case particles sort with removals", flush=True)
+    o, s, u0, st, dt = setup(scheme="trilinear_rk3", nsub=1, n=600, cap=700)
+    n = u0.shape[0]
+    u = u0.copy()
+    u[5, :3] = [np.nan, 0.5, 3.0]          # NaN coordinate
+    u[17, :3] = [0.3, 0.2, 11.9999]        # leaves through the outlet
+    u[17, 5] = 50.0
+    u[33, :3] = [3.99, 0.0, 3.0]           # leaves through the outer wall
+    u[33, 3] = 40.0
+    s.particles_upload(0, u)
+    xv = u.copy()
+    r2, alive = np.zeros((n, 6)), np.ones(n, np.int32)
+    for irk in (1, 2, 3):
+        s.particles_step(irk)
+        o.L.orc_part_rk3_trilinear(o.h, n, st, irk, dt, xv.ravel(), r2.ravel(), alive)
+    g, gal = s.particles_download(0, n)
+    np.testing.assert_array_equal(gal, alive)
+    assert alive[17] == 0 and alive[33] == 0
+    keep = (alive == 1) & np.all(np.isfinite(xv), axis=1)
+    assert np.abs(g[keep] - xv[keep]).max() / np.abs(xv[keep]).max() < 1e-11
+    s.particles_compact(0)
+    m = s.particles_count(0)
+    assert m == int(alive.sum())
+    gc, gcal = s.particles_download(0, m)
+    np.testing.assert_array_equal(gcal, np.ones(m, np.int32))
+    live = np.flatnonzero(alive == 1)
+    fin = np.all(np.isfinite(xv[live]), axis=1)
+    assert np.abs(gc[fin] - xv[live][fin]).max() / np.abs(xv[live][fin]).max() < 1e-11
