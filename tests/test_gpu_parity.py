@@ -11,7 +11,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("form", [0, 1, 2, 3])
-@pytest.mark.parametrize("nt,nr,nz", [(9, 16, 32), (32, 32, 64), (33, 24, 40), (16, 64, 128), (128, 20, 40)])
+@pytest.mark.parametrize("nt,nr,nz", [(9, 16, 32), (32, 32, 64), (33, 24, 40), (16, 64, 128), (128, 20, 40),
+                                      (64, 96, 128)])
 def test_substep_phases_match_oracle(nt, nr, nz, form):
     """form: explicit kernel with one z plane per block (0), z marching (1),
     or z marching with one wavefront per r row within 64 KB of LDS (2) or
@@ -85,7 +86,8 @@ def test_substep_phases_with_wall_slip_match_oracle(slip):
 
 
 @pytest.mark.parametrize("nt,nr,nz,theta", [(9, 16, 32, "exact_m2"), (16, 32, 64, "consistent"),
-                                            (33, 20, 24, "exact_m2"), (128, 64, 128, "exact_m2")])
+                                            (33, 20, 24, "exact_m2"), (128, 64, 128, "exact_m2"),
+                                            (64, 96, 256, "consistent"), (32, 192, 512, "exact_m2")])
 def test_poisson_matches_oracle(nt, nr, nz, theta):
     print(f"case poisson nt={nt} nr={nr} nz={nz} {theta}", flush=True)
     o, s = make_pair(nt, nr, nz, theta=theta)
