@@ -130,11 +130,13 @@ bool lookup(vrh_t *h, const char *name, Named &out) {
 
 // One substep whose time-dependent scalars (inlet increment, mass
 // correction) are read from d_sc, as written by k_scalars at the start of
-// the step: the form that is captured in a graph.
-template <typename TW> void substep_dev(vrh_t *h, int irk) {
+// the step: the form that is captured in a graph.  join (optional): work on
+// another stream that must end before the projection writes q.
+template <typename TW> void substep_dev(vrh_t *h, int irk, hipEvent_t join) {
   launch_explicit<TW>(h, irk);
   launch_sweeps<TW>(h, irk, 0.0, h->d_sc + irk, true, 0.0, h->d_sc + 3 + irk);
   launch_poisson<TW>(h, irk, nullptr, nullptr);
+  if (join) (void)hipStreamWaitEvent(h->stream, join, 0);
   launch_project<TW>(h, irk);
 }
 
@@ -178,12 +180,32 @@ int check_flag(vrh_t *h) {
 // substeps.  Nothing here synchronises or copies, so it can be captured.
 int step_body(vrh_t *h) {
   launch_scalars(h);
+  // The particle stage reads q as the previous projection left it and
+  // writes only particle state; of the substep's kernels only the projection
+  // writes q.  So the LSQ stage runs on a side stream beside the explicit,
+  // sweep and pressure kernels and is joined before the projection (10^6
+  // particles: 2.67 ms per step instead of 2.81).  The short trilinear stage
+  // stays in order (beside the fluid kernels its gathers slowed it by 9 %),
+  // as does everything with the kernel timer on or inside an ensemble graph.
+  const bool ppar = !h->kt_on && !h->serial_sweeps && h->side[2] && h->pscheme != VRH_PART_TRILINEAR_RK3;
   for (int irk = 1; irk <= 3; irk++) {
     // loop.F90:66-70 calls iter_part at irk == 1 (the Euler scheme); the RK3
     // particle schemes take one stage per substep (launch_particles decides)
-    if (h->nbatch > 0) launch_particles(h, irk);
-    if (h->prec == VRH_MIXED) substep_dev<float>(h, irk);
-    else substep_dev<double>(h, irk);
+    hipEvent_t join = nullptr;
+    if (h->nbatch > 0) {
+      if (ppar) {
+        (void)hipEventRecord(h->part_ev[irk][0], h->stream);
+        (void)hipStreamWaitEvent(h->side[2], h->part_ev[irk][0], 0);
+        if (launch_particles(h, irk, h->side[2])) {
+          (void)hipEventRecord(h->part_ev[irk][1], h->side[2]);
+          join = h->part_ev[irk][1];
+        }
+      } else {
+        launch_particles(h, irk, h->stream);
+      }
+    }
+    if (h->prec == VRH_MIXED) substep_dev<float>(h, irk, join);
+    else substep_dev<double>(h, irk, join);
   }
   if (h->mode_rc) { const int rc = h->mode_rc; h->mode_rc = 0; return rc; }
   return hipGetLastError() == hipSuccess ? 0 : fail(h, "kernel launch failed", -3);
@@ -300,9 +322,10 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
     return fail(h, "vrh_create: the analysis scratch array has no device address (hipGetSymbolAddress)", -3);
   HIPCHK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   lds_optin(h);
-  for (int q = 0; q < 2; q++) HIPCHK(h, hipStreamCreateWithFlags(&h->side[q], hipStreamNonBlocking));
+  for (int q = 0; q < 3; q++) HIPCHK(h, hipStreamCreateWithFlags(&h->side[q], hipStreamNonBlocking));
   for (int irk = 0; irk < 4; irk++) {
     HIPCHK(h, hipEventCreateWithFlags(&h->fork_ev[irk], hipEventDisableTiming));
+    for (int q = 0; q < 2; q++) HIPCHK(h, hipEventCreateWithFlags(&h->part_ev[irk][q], hipEventDisableTiming));
     for (int q = 0; q < 2; q++) HIPCHK(h, hipEventCreateWithFlags(&h->join_ev[irk][q], hipEventDisableTiming));
     for (int q = 0; q < 2; q++) HIPCHK(h, hipEventCreateWithFlags(&h->poi_ev[irk][q], hipEventDisableTiming));
   }
@@ -406,10 +429,12 @@ int vrh_destroy(vrh_t *h) {
                      b.lflag, b.lrank, b.skeys, b.stmp};
     for (void *q : extra) if (q) hipFree(q);
   }
-  for (int q = 0; q < 2; q++)
+  for (int q = 0; q < 3; q++)
     if (h->side[q]) hipStreamDestroy(h->side[q]);
   for (int irk = 0; irk < 4; irk++) {
     if (h->fork_ev[irk]) hipEventDestroy(h->fork_ev[irk]);
+    for (int q = 0; q < 2; q++)
+      if (h->part_ev[irk][q]) hipEventDestroy(h->part_ev[irk][q]);
     for (int q = 0; q < 2; q++)
       if (h->join_ev[irk][q]) hipEventDestroy(h->join_ev[irk][q]);
     for (int q = 0; q < 2; q++)
@@ -640,8 +665,8 @@ int vrh_substep(vrh_t *h, int irk) {
   if (!h) return -1;
   if (irk < 1 || irk > 3) return fail(h, "irk must be 1..3");
   launch_scalars_at(h, h->ntime);
-  if (h->prec == VRH_MIXED) substep_dev<float>(h, irk);
-  else substep_dev<double>(h, irk);
+  if (h->prec == VRH_MIXED) substep_dev<float>(h, irk, nullptr);
+  else substep_dev<double>(h, irk, nullptr);
   HIPCHK(h, hipGetLastError());
   h->time_dirty = false;
   if (h->mode_rc) { const int rc = h->mode_rc; h->mode_rc = 0; return rc; }

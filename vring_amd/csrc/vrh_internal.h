@@ -132,13 +132,17 @@ struct vrh_s {
   hipStream_t stream;
   // side streams for the independent component chains of the line sweeps
   // (forked from and joined back into `stream`; captured with the step)
-  hipStream_t side[2] = {nullptr, nullptr};
+  // side streams: [0], [1] the sweep chains and the pressure solve's FP64
+  // low-mode chain, [2] the particle stage of a substep (beside the fluid
+  // kernels up to the projection, which is the only kernel that writes q)
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};
   bool serial_sweeps = false;  // set while captured into an ensemble graph
   // one fork and two join events per substep: a captured step records each
   // event once (an event recorded twice in one capture keeps only one of
   // the dependency sets)
   hipEvent_t fork_ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t join_ev[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+  hipEvent_t part_ev[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
   // fork and join of the pressure solve's FP64 low-mode chain, per substep
   hipEvent_t poi_ev[4][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
   std::string err;
@@ -277,4 +281,6 @@ template <typename TW> void launch_sweeps(vrh_t *h, int irk, double c3s, const d
                                           double cor = 0.0, const double *cor_dev = nullptr);
 template <typename TW> void launch_poisson(vrh_t *h, int irk, const double *ext_rhs, double *phi_out);
 template <typename TW> void launch_project(vrh_t *h, int irk);
-void launch_particles(vrh_t *h, int irk);
+// the particle stage of substep irk on stream st (nothing launched where
+// the scheme has no stage at irk); returns whether anything was launched
+bool launch_particles(vrh_t *h, int irk, hipStream_t st);

@@ -667,10 +667,10 @@ __global__ void k_slot_rank(const int *__restrict__ flag, const long *n_dev, lon
 
 }  // namespace
 
-void launch_particles(vrh_t *h, int irk) {
-  if (h->nbatch <= 0) return;
+bool launch_particles(vrh_t *h, int irk, hipStream_t st) {
+  if (h->nbatch <= 0) return false;
   const bool every = h->pscheme != VRH_PART_LSQ_EULER;
-  if (!every && irk != 1) return;  // loop.F90:67 calls iter_part at irk==1 only
+  if (!every && irk != 1) return false;  // loop.F90:67 calls iter_part at irk==1 only
   KTimer kt(h, KT_PARTICLES);
   for (auto &b : h->pb) {
     PartArgs a{};
@@ -690,8 +690,8 @@ void launch_particles(vrh_t *h, int irk) {
       a.partdt = h->dt;
       a.nsub = 1;
       dim3 blk(256), grd((unsigned)((b.cap + 255) / 256));
-      if (h->prec == VRH_MIXED) hipLaunchKernelGGL(k_part_tri<float>, grd, blk, 0, h->stream, a);
-      else hipLaunchKernelGGL(k_part_tri<double>, grd, blk, 0, h->stream, a);
+      if (h->prec == VRH_MIXED) hipLaunchKernelGGL(k_part_tri<float>, grd, blk, 0, st, a);
+      else hipLaunchKernelGGL(k_part_tri<double>, grd, blk, 0, st, a);
     } else {
       a.fit = b.fit;
       a.nsub = 1;
@@ -703,14 +703,15 @@ void launch_particles(vrh_t *h, int irk) {
       // FP64: two kernels (the fused one spills at 256 VGPRs, 3 % slower)
       for (int isub = 0; isub < b.nsub; isub++) {
         if (h->prec == VRH_MIXED) {
-          hipLaunchKernelGGL(k_lsq_substep<float>, grd, blk, 0, h->stream, a, mf);
+          hipLaunchKernelGGL(k_lsq_substep<float>, grd, blk, 0, st, a, mf);
         } else {
-          hipLaunchKernelGGL(k_lsq_fit<double>, grd, blk, 0, h->stream, a, md);
-          hipLaunchKernelGGL(k_lsq_step<double>, grd, blk, 0, h->stream, a);
+          hipLaunchKernelGGL(k_lsq_fit<double>, grd, blk, 0, st, a, md);
+          hipLaunchKernelGGL(k_lsq_step<double>, grd, blk, 0, st, a);
         }
       }
     }
   }
+  return true;
 }
 
 extern "C" {
@@ -952,7 +953,7 @@ int vrh_particles_inject(vrh_t *h, int batch, double r0, int inj, unsigned long 
 int vrh_particles_step(vrh_t *h, int irk) {
   if (!h || irk < 1 || irk > 3) return -1;
   if (h->nbatch <= 0) { h->err = "no particles"; return -1; }
-  launch_particles(h, irk);
+  launch_particles(h, irk, h->stream);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
