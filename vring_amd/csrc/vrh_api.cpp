@@ -403,7 +403,7 @@ int vrh_destroy(vrh_t *h) {
   if (!h) return -1;
   if (h->stream) hipStreamSynchronize(h->stream);
   Fields &f = h->f;
-  void *ptrs[] = {h->d_vec, h->d_line, h->d_twt, h->d_twz, h->d_ez, h->d_aab, h->d_ccb, h->d_mint, f.q1, f.q2,
+  void *ptrs[] = {h->d_vec, h->d_line, h->d_linef, h->d_twt, h->d_twz, h->d_ez, h->d_aab, h->d_ccb, h->d_mint, f.q1, f.q2,
                   f.q3, f.dq1, f.dq2, f.dq3, f.pres, f.dph, f.ru1, f.ru2, f.ru3, f.r1, f.r2, f.r3, f.prg1, f.prg2,
                   f.prg3, f.S, f.ib, f.S64, f.ib64, f.csort, f.dq3no, f.corr3n, f.flag};
   for (void *p : ptrs) if (p) hipFree(p);
@@ -808,8 +808,11 @@ int build_line_coefs(vrh_t *h) {
       for (int q = 0; q < 7; q++) tot += lcv[irk][s][q].size() + (q < 5 ? kLinePad : 0);
   }
   if (h->d_line) { hipStreamSynchronize(h->stream); hipFree(h->d_line); h->d_line = nullptr; }
+  if (h->d_linef) { hipFree(h->d_linef); h->d_linef = nullptr; }
   int rc = dalloc(h, &h->d_line, tot);
   if (rc) return rc;
+  const bool fcopy = h->prec == VRH_MIXED;
+  if (fcopy && (rc = dalloc_bytes(h, reinterpret_cast<void **>(&h->d_linef), tot * sizeof(float)))) return rc;
   std::vector<double> lb(tot, 0.0);
   size_t off = 0;
   for (int irk = 1; irk <= 3; irk++)
@@ -824,10 +827,18 @@ int build_line_coefs(vrh_t *h) {
       }
       LineCoef &L = h->lc[irk][s];
       L.A = ptr[0]; L.B = ptr[1]; L.C = ptr[2]; L.Pk = ptr[3]; L.Qk = ptr[4]; L.Pseg = ptr[5]; L.Qseg = ptr[6];
+      const float *pf[7];
+      for (int q = 0; q < 7; q++) pf[q] = fcopy ? h->d_linef + (ptr[q] - h->d_line) : nullptr;
+      L.Af = pf[0]; L.Bf = pf[1]; L.Cf = pf[2]; L.Pkf = pf[3]; L.Qkf = pf[4]; L.Psegf = pf[5]; L.Qsegf = pf[6];
       L.n = (int)lcv[irk][s][0].size();
       L.P = (int)lcv[irk][s][5].size();
     }
   if ((rc = h2d(h, h->d_line, lb.data(), tot * sizeof(double)))) return rc;
+  if (fcopy) {
+    std::vector<float> lf(tot);
+    for (size_t i = 0; i < tot; i++) lf[i] = (float)lb[i];  // round to nearest, as the device conversion
+    if ((rc = h2d(h, h->d_linef, lf.data(), tot * sizeof(float)))) return rc;
+  }
   return 0;
 }
 

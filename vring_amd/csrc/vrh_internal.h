@@ -60,6 +60,9 @@ constexpr int kLinePad = 72;
 constexpr int kLineSegs[] = {16, 24, 32, 40, 48, 64, 72};
 struct LineCoef {
   const double *A, *B, *C, *Pk, *Qk, *Pseg, *Qseg;
+  // the same arrays rounded to float (mixed path): the FP32 sweeps read
+  // half the scalar registers' worth and convert nothing
+  const float *Af, *Bf, *Cf, *Pkf, *Qkf, *Psegf, *Qsegf;
   int n, P;
 };
 
@@ -181,6 +184,7 @@ struct vrh_s {
   VCoef coef;
   // line coefficients: [irk 1..3][sweep 0..5]
   double *d_line;
+  float *d_linef = nullptr;  // float copy of d_line (mixed path)
   LineCoef lc[4][6];
   int Msw[6];        // segment length M of each sweep (partitioned Thomas)
   int fft_lines = 0;  // theta lines per block of the FFT kernels (0: default)

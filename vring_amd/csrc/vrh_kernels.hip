@@ -926,6 +926,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_pe
   const int p0 = s * M;
   const int len = min(M, n - p0);
   TW *rr = reinterpret_cast<TW *>(a.r);
+  // the coefficients in the work type (the mixed path's float copies)
+  auto cw = [](const double *d, const float *f) {
+    if constexpr (sizeof(TW) == 4) { (void)d; return f; } else { (void)f; return d; }
+  };
+  const TW *cA = cw(a.lc.A, a.lc.Af), *cB = cw(a.lc.B, a.lc.Bf), *cC = cw(a.lc.C, a.lc.Cf);
+  const TW *cPk = cw(a.lc.Pk, a.lc.Pkf), *cQk = cw(a.lc.Qk, a.lc.Qkf);
+  const TW *cPs = cw(a.lc.Pseg, a.lc.Psegf), *cQs = cw(a.lc.Qseg, a.lc.Qsegf);
   // line -> (i, second index)
   const int i = (int)(Lc % g.nt) + 1;
   const int l2 = (int)(Lc / g.nt);
@@ -953,7 +960,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_pe
       if (p == 0) rv = (jz <= a.jlim) ? (TW)(a.c3s_dev ? *a.c3s_dev : a.c3s) : (TW)0;
       else if (p == n - 1) rv = rlast;
     }
-    const TW v = (TW)a.lc.A[p] * rv + (TW)a.lc.B[p] * gp;
+    const TW v = cA[p] * rv + cB[p] * gp;
     const bool in = t < len;
     x[t] = in ? v : (TW)0;
     gp = in ? v : gp;
@@ -961,13 +968,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_pe
   sh_a[s][lane] = gp;
   __syncthreads();
   TW G = (TW)0;
-  for (int s2 = 0; s2 < s; ++s2) G = sh_a[s2][lane] + (TW)a.lc.Pseg[s2] * G;
+  for (int s2 = 0; s2 < s; ++s2) G = sh_a[s2][lane] + cPs[s2] * G;
   __syncthreads();
   TW xp = (TW)0;
 #pragma unroll
   for (int t = M - 1; t >= 0; --t) {
     const int p = p0 + t;
-    const TW v = x[t] + (TW)a.lc.Pk[p] * G - (TW)a.lc.C[p] * xp;
+    const TW v = x[t] + cPk[p] * G - cC[p] * xp;
     const bool in = t < len;
     x[t] = in ? v : x[t];
     xp = in ? v : xp;
@@ -975,9 +982,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 1024), amdgpu_waves_pe
   sh_a[s][lane] = xp;
   __syncthreads();
   TW X = (TW)0;
-  for (int s2 = P - 1; s2 > s; --s2) X = sh_a[s2][lane] + (TW)a.lc.Qseg[s2] * X;
+  for (int s2 = P - 1; s2 > s; --s2) X = sh_a[s2][lane] + cQs[s2] * X;
 #pragma unroll
-  for (int t = 0; t < M; ++t) x[t] = x[t] + (TW)a.lc.Qk[p0 + t] * X;
+  for (int t = 0; t < M; ++t) x[t] = x[t] + cQk[p0 + t] * X;
   if (MODE == SW_R1 || MODE == SW_R3 || MODE == SW_Z2) {
     if (!act) return;
 #pragma unroll
