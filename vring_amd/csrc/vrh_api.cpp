@@ -256,6 +256,11 @@ int refresh_mesh(vrh_t *h) {
   for (int j = 1; j <= h->nr; j++) { double dd; host_radial_row(h, j, aab[j], ccb[j], dd); }
   if ((rc = h2d(h, h->d_aab, aab.data(), aab.size() * sizeof(double)))) return rc;
   if ((rc = h2d(h, h->d_ccb, ccb.data(), ccb.size() * sizeof(double)))) return rc;
+  if (h->d_aabf) {  // round to nearest, as the device conversion
+    std::vector<float> af(aab.begin(), aab.end()), cf(ccb.begin(), ccb.end());
+    if ((rc = h2d(h, h->d_aabf, af.data(), af.size() * sizeof(float)))) return rc;
+    if ((rc = h2d(h, h->d_ccbf, cf.data(), cf.size() * sizeof(float)))) return rc;
+  }
   std::vector<double> ib, ib64;
   host_radial_pivots(h, ib, ib64);
   if ((rc = upload_conv(h, h->f.ib, ib.data(), ib.size(), true))) return rc;
@@ -369,6 +374,10 @@ int vrh_create(const vrh_grid *grid, const vrh_params *p, int precision, vrh_t *
     if ((rc = h2d(h, h->d_twz, tz.data(), tz.size() * sizeof(double)))) return rc;
     if ((rc = h2d(h, h->d_ez, ez.data(), ez.size() * sizeof(double)))) return rc;
     if ((rc = dalloc(h, &h->d_aab, (size_t)h->nr + 1)) || (rc = dalloc(h, &h->d_ccb, (size_t)h->nr + 1))) return rc;
+    if (h->prec == VRH_MIXED &&
+        ((rc = dalloc_bytes(h, reinterpret_cast<void **>(&h->d_aabf), sizeof(float) * ((size_t)h->nr + 1))) ||
+         (rc = dalloc_bytes(h, reinterpret_cast<void **>(&h->d_ccbf), sizeof(float) * ((size_t)h->nr + 1)))))
+      return rc;
   }
   // fields
   const size_t H = haloed(h), C = compact(h), W = wsize(h), PL = (size_t)h->nt * (h->nr + 3);
@@ -403,7 +412,7 @@ int vrh_destroy(vrh_t *h) {
   if (!h) return -1;
   if (h->stream) hipStreamSynchronize(h->stream);
   Fields &f = h->f;
-  void *ptrs[] = {h->d_vec, h->d_line, h->d_linef, h->d_twt, h->d_twz, h->d_ez, h->d_aab, h->d_ccb, h->d_mint, f.q1, f.q2,
+  void *ptrs[] = {h->d_vec, h->d_line, h->d_linef, h->d_twt, h->d_twz, h->d_ez, h->d_aab, h->d_ccb, h->d_aabf, h->d_ccbf, h->d_mint, f.q1, f.q2,
                   f.q3, f.dq1, f.dq2, f.dq3, f.pres, f.dph, f.ru1, f.ru2, f.ru3, f.r1, f.r2, f.r3, f.prg1, f.prg2,
                   f.prg3, f.S, f.ib, f.S64, f.ib64, f.csort, f.dq3no, f.corr3n, f.flag};
   for (void *p : ptrs) if (p) hipFree(p);

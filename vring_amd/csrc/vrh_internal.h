@@ -197,6 +197,7 @@ struct vrh_s {
   // (nz/2 complex), z post-twiddles (nz complex)
   double *d_twt, *d_twz, *d_ez;
   double *d_aab, *d_ccb;  // Poisson radial base coefficients (nr+1)
+  float *d_aabf = nullptr, *d_ccbf = nullptr;  // the same rounded to float (FP32 radial solves)
   Fields f;
   double cor_red;  // sum qc^2 da (hdnbc mass correction denominator)
   int jlim;        // rc(j) <= 0.4999 for j <= jlim (inlet rows)

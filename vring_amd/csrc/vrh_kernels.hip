@@ -1508,7 +1508,7 @@ __global__ __launch_bounds__(1024) void k_dctz(TW *__restrict__ S, long nl, int 
 // FP32: at most 64 VGPRs, so that two 1024-lane blocks share a CU.
 template <typename T, int M>
 __global__ __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 1) void k_radial(T *__restrict__ S, const T *__restrict__ ib, long nm, long nlines,
-                                                 int nr, const double *__restrict__ aab, const double *__restrict__ ccb) {
+                                                 int nr, const T *__restrict__ aab, const T *__restrict__ ccb) {
   __shared__ T sh_v[16][64];
   __shared__ T sh_p[16][64];
   const int lane = threadIdx.x & 63;
@@ -1536,7 +1536,7 @@ __global__ __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 1) void k_radial(T *__re
     const int p = p0 + t, pc = min(p, nr - 1);
     const bool in = t < len;
     const T ibv = w[t];
-    const T B = -(T)aab[pc + 1] * ibv * (T)(p > 0);
+    const T B = -aab[pc + 1] * ibv * (T)(p > 0);
     const T v = ibv * x[t] + B * gp;
     x[t] = in ? v : (T)0;
     gp = in ? v : gp;
@@ -1553,7 +1553,7 @@ __global__ __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 1) void k_radial(T *__re
   for (int t = 0; t < M; ++t) {
     const int p = p0 + t, pc = min(p, nr - 1);
     const bool in = t < len;
-    Pr = in ? Pr * (-(T)aab[pc + 1] * w[t] * (T)(p > 0)) : Pr;
+    Pr = in ? Pr * (-aab[pc + 1] * w[t] * (T)(p > 0)) : Pr;
     x[t] = in ? x[t] + Pr * G : x[t];
   }
   T xp = (T)0, Qacc = (T)1;
@@ -1561,7 +1561,7 @@ __global__ __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 1) void k_radial(T *__re
   for (int t = M - 1; t >= 0; --t) {
     const int p = p0 + t, pc = min(p, nr - 1);
     const bool in = t < len;
-    const T C = (T)ccb[pc + 1] * w[t] * (T)(p < nr - 1);
+    const T C = ccb[pc + 1] * w[t] * (T)(p < nr - 1);
     const T v = x[t] - C * xp;
     x[t] = in ? v : x[t];
     xp = in ? v : xp;
@@ -1578,7 +1578,7 @@ __global__ __launch_bounds__(1024, sizeof(T) == 4 ? 8 : 1) void k_radial(T *__re
   for (int t = M - 1; t >= 0; --t) {
     if (t < len) {
       const int p = p0 + t;
-      Qr *= -(T)ccb[p + 1] * w[t] * (T)(p < nr - 1);
+      Qr *= -ccb[p + 1] * w[t] * (T)(p < nr - 1);
       S[base + nm * p] = x[t] + Qr * X;
     }
   }
@@ -2256,11 +2256,16 @@ static void radial_launch(vrh_t *h, T *S, const T *ib, long nm, hipStream_t st) 
   const int P = (g.nr + M - 1) / M;
   const long nlines = nm * g.nz;
   dim3 blk(64 * P), grd((unsigned)((nlines + 63) / 64));
+  // the radial base coefficients in the solve's type (float copies rounded
+  // as the device conversion)
+  const T *ra, *rc;
+  if constexpr (sizeof(T) == 4) { ra = h->d_aabf; rc = h->d_ccbf; }
+  else { ra = h->d_aab; rc = h->d_ccb; }
   KTimer kt(h, KT_RADIAL);
   if (M == 16)
-    hipLaunchKernelGGL((k_radial<T, 16>), grd, blk, 0, st, S, ib, nm, nlines, g.nr, h->d_aab, h->d_ccb);
+    hipLaunchKernelGGL((k_radial<T, 16>), grd, blk, 0, st, S, ib, nm, nlines, g.nr, ra, rc);
   else
-    hipLaunchKernelGGL((k_radial<T, 32>), grd, blk, 0, st, S, ib, nm, nlines, g.nr, h->d_aab, h->d_ccb);
+    hipLaunchKernelGGL((k_radial<T, 32>), grd, blk, 0, st, S, ib, nm, nlines, g.nr, ra, rc);
 }
 
 // z transform of the pressure solve on an axial mesh: every spectral line
