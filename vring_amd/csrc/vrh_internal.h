@@ -88,6 +88,10 @@ struct PartBatch {
   int nsub;
   double *u;     // nvar rows of cap (structure of arrays)
   double *ddt2;  // nvar rows of cap, previous rate (RK3 schemes)
+  // bytes per element of the ddt2 rows: 8, or 4 for the trilinear scheme in
+  // mixed precision (its rates are formed from the FP32 interpolation; the
+  // state update stays FP64); ddt2 then holds float rows
+  int d2w = 8;
   double *fit = nullptr;  // LSQ schemes: 39 rows of cap, per-component fits of a substep
   int *alive;
   long *d_n;     // used slots [0, n) on the device (kernels read it)

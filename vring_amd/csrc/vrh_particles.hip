@@ -447,7 +447,9 @@ __global__ __launch_bounds__(256) void k_part_tri(PartArgs a) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= a.cap || p >= *a.n_dev || !a.alive[p]) return;
   const long cap = a.cap;
-  double *s = a.u + p, *r2 = a.ddt2 + p;
+  // previous rates in the work type (float rows in mixed precision)
+  double *s = a.u + p;
+  T *r2 = reinterpret_cast<T *>(a.ddt2) + p;
 #define S(v) s[(long)(v) * cap]
 #define R(v) r2[(long)(v) * cap]
   const double x = S(0), y = S(1), z = S(2), r = sqrt(x * x + y * y);
@@ -463,8 +465,8 @@ __global__ __launch_bounds__(256) void k_part_tri(PartArgs a) {
     else { f[i] = u[i]; f[3 + i] = 0.0; }
   }
   for (int i = 0; i < 6; i++) {
-    S(i) = S(i) + a.gam * a.partdt * f[i] + a.rom * a.partdt * R(i);
-    R(i) = f[i];
+    S(i) = S(i) + a.gam * a.partdt * f[i] + a.rom * a.partdt * (double)R(i);
+    R(i) = (T)f[i];
   }
   if (a.st <= 0.0) for (int i = 0; i < 3; i++) S(3 + i) = u[i];
   const double rn = sqrt(S(0) * S(0) + S(1) * S(1));
@@ -546,7 +548,7 @@ __global__ void k_inflow(VGrid g, VCoef c, const double *__restrict__ q3, double
 // (0, 0, q3), identity Jacobian (part_tools.F90:120-134, 164-181)
 __global__ void k_inject(VGrid g, VCoef c, const double *__restrict__ q3, const long *__restrict__ counts,
                          const long *__restrict__ offs, const long *n_dev, long cap, int nvar, double *__restrict__ u,
-                         double *__restrict__ d2, int *__restrict__ alive, int *__restrict__ slot, int jr0, double dth,
+                         double *__restrict__ d2, int d2w, int *__restrict__ alive, int *__restrict__ slot, int jr0, double dth,
                          double dr, double dz, unsigned long long seed, long call) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)jr0 * g.nt) return;
@@ -563,8 +565,10 @@ __global__ void k_inject(VGrid g, VCoef c, const double *__restrict__ q3, const 
     const double r = c.rc[ir] + dr * pu01(seed, key + 1);
     const double z = 0.0 + dz * pu01(seed, key + 2);
     for (int v = 0; v < nvar; v++) u[(long)v * cap + idx] = 0.0;
-    if (d2)
+    if (d2 && d2w == 8)
       for (int v = 0; v < nvar; v++) d2[(long)v * cap + idx] = 0.0;
+    if (d2 && d2w == 4)
+      for (int v = 0; v < nvar; v++) reinterpret_cast<float *>(d2)[(long)v * cap + idx] = 0.0f;
     u[idx] = r * cos(th);
     u[cap + idx] = r * sin(th);
     u[2 * cap + idx] = z;
@@ -620,15 +624,20 @@ __global__ void k_sort_keys(const double *__restrict__ u, const int *__restrict_
 // are unused, numbered as their position
 __global__ void k_permute(const unsigned *__restrict__ order, const double *__restrict__ u,
                           const double *__restrict__ d2, const int *__restrict__ alive, const int *__restrict__ slot,
-                          const int *__restrict__ rank, const long *nlive, long cap, int nvar,
+                          const int *__restrict__ rank, const long *nlive, long cap, int nvar, int d2w,
                           double *__restrict__ ut, double *__restrict__ d2t, int *__restrict__ at,
                           int *__restrict__ st) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= cap) return;
   const long src = order[i];
   for (int v = 0; v < nvar; v++) ut[(long)v * cap + i] = u[(long)v * cap + src];
-  if (d2)
+  if (d2 && d2w == 8)
     for (int v = 0; v < nvar; v++) d2t[(long)v * cap + i] = d2[(long)v * cap + src];
+  if (d2 && d2w == 4) {
+    const float *f2 = reinterpret_cast<const float *>(d2);
+    float *f2t = reinterpret_cast<float *>(d2t);
+    for (int v = 0; v < nvar; v++) f2t[(long)v * cap + i] = f2[(long)v * cap + src];
+  }
   if (rank) {
     const bool live = i < *nlive;
     at[i] = live ? 1 : 0;
@@ -749,8 +758,9 @@ int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st
     if (scheme != VRH_PART_TRILINEAR_RK3 && h->prec != VRH_MIXED &&
         hipMalloc(&B.fit, sizeof(double) * kFitRows * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
     if (scheme != VRH_PART_LSQ_EULER) {
-      if (hipMalloc(&B.ddt2, sizeof(double) * nvar * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
-      hipMemsetAsync(B.ddt2, 0, sizeof(double) * nvar * B.cap, h->stream);
+      B.d2w = (scheme == VRH_PART_TRILINEAR_RK3 && h->prec == VRH_MIXED) ? 4 : 8;
+      if (hipMalloc(&B.ddt2, (size_t)B.d2w * nvar * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
+      hipMemsetAsync(B.ddt2, 0, (size_t)B.d2w * nvar * B.cap, h->stream);
     }
   }
   h->nbatch = nbatch;
@@ -775,7 +785,7 @@ int vrh_particles_upload(vrh_t *h, int batch, long n, const double *u, const int
             hipMemcpyAsync(B.alive, al.data(), al.size() * sizeof(int), hipMemcpyHostToDevice, h->stream) ==
                 hipSuccess &&
             hipMemcpyAsync(B.d_n, &n, sizeof(long), hipMemcpyHostToDevice, h->stream) == hipSuccess;
-  if (ok && B.ddt2) ok = hipMemsetAsync(B.ddt2, 0, sizeof(double) * nvar * B.cap, h->stream) == hipSuccess;
+  if (ok && B.ddt2) ok = hipMemsetAsync(B.ddt2, 0, (size_t)B.d2w * nvar * B.cap, h->stream) == hipSuccess;
   if (ok) hipLaunchKernelGGL(k_iota, dim3((unsigned)((B.cap + 255) / 256)), dim3(256), 0, h->stream, B.slot, B.cap);
   if (hipStreamSynchronize(h->stream) != hipSuccess || !ok) {
     h->err = "particle upload failed";
@@ -823,7 +833,7 @@ static int particle_buffers(vrh_t *h, PartBatch &B) {
       hipMalloc(&B.lflag, sizeof(int) * B.cap) != hipSuccess ||
       hipMalloc(&B.lrank, sizeof(int) * B.cap) != hipSuccess ||
       hipMalloc(&B.skeys, sizeof(unsigned) * 4 * B.cap) != hipSuccess ||
-      (B.ddt2 && hipMalloc(&B.ddt2_tmp, sizeof(double) * nvar * B.cap) != hipSuccess)) {
+      (B.ddt2 && hipMalloc(&B.ddt2_tmp, (size_t)B.d2w * nvar * B.cap) != hipSuccess)) {
     h->err = "particle allocation failed";
     return -3;
   }
@@ -876,10 +886,10 @@ static int sort_batch(vrh_t *h, PartBatch &B, bool compact) {
     return -3;
   }
   hipLaunchKernelGGL(k_permute, dim3((unsigned)nblk), dim3(256), 0, st, vout, B.u, B.ddt2, B.alive, B.slot,
-                     compact ? B.lrank : nullptr, total, cap, nvar, B.u_tmp, B.ddt2_tmp, B.alive_tmp, B.slot_tmp);
+                     compact ? B.lrank : nullptr, total, cap, nvar, B.d2w, B.u_tmp, B.ddt2_tmp, B.alive_tmp, B.slot_tmp);
   ok = ok && hipMemcpyAsync(B.u, B.u_tmp, sizeof(double) * nvar * cap, hipMemcpyDeviceToDevice, st) == hipSuccess;
   if (B.ddt2)
-    ok = ok && hipMemcpyAsync(B.ddt2, B.ddt2_tmp, sizeof(double) * nvar * cap, hipMemcpyDeviceToDevice, st) ==
+    ok = ok && hipMemcpyAsync(B.ddt2, B.ddt2_tmp, (size_t)B.d2w * nvar * cap, hipMemcpyDeviceToDevice, st) ==
                    hipSuccess;
   ok = ok && hipMemcpyAsync(B.alive, B.alive_tmp, sizeof(int) * cap, hipMemcpyDeviceToDevice, st) == hipSuccess;
   ok = ok && hipMemcpyAsync(B.slot, B.slot_tmp, sizeof(int) * cap, hipMemcpyDeviceToDevice, st) == hipSuccess;
@@ -935,7 +945,7 @@ int vrh_particles_inject(vrh_t *h, int batch, double r0, int inj, unsigned long 
   hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, h->stream, offs, (long)jr0 * h->nt, total);
   const long ncell = (long)jr0 * h->nt;
   hipLaunchKernelGGL(k_inject, dim3((unsigned)((ncell + 127) / 128)), dim3(128), 0, h->stream, h->g, h->coef,
-                     h->f.q3, counts, offs, B.d_n, B.cap, nvar, B.u, B.ddt2, B.alive, B.slot, jr0, dth, dr, dz,
+                     h->f.q3, counts, offs, B.d_n, B.cap, nvar, B.u, B.ddt2, B.d2w, B.alive, B.slot, jr0, dth, dr, dz,
                      seed, B.ncall);
   hipLaunchKernelGGL(k_add_count, dim3(1), dim3(1), 0, h->stream, B.d_n, total, B.cap);
   B.ncall++;
