@@ -203,7 +203,12 @@ def run_particles(prec, q, dt, n, steps, scheme="lsq_euler"):
     sort_ms = (time.perf_counter() - ts) / 3 * 1e3
     s.close()
     nvar = 117 if scheme == "lsq_euler" else 6
-    per = (2 * nvar + (81 if scheme == "lsq_euler" else 2 * nvar)) * 8
+    # state rows in and out (FP64); LSQ: the 81-value stencil per component
+    # set; trilinear: the previous rates in and out (float rows in mixed)
+    if scheme == "lsq_euler":
+        per = (2 * nvar + 81) * 8
+    else:
+        per = 2 * nvar * 8 + 2 * nvar * (4 if prec == "mixed" else 8)
     return {"scheme": scheme, "particles": n, "substeps_per_step": nsub * len(stages),
             "ms_per_step": round(t / steps * 1e3, 3), "particles_per_s": round(n * steps / t, 1),
             "hbm_bytes_per_particle_substep": per,
