@@ -392,3 +392,32 @@ def test_sort_and_compaction_with_lost_and_nan_particles():
     live = np.flatnonzero(alive == 1)
     fin = np.all(np.isfinite(xv[live]), axis=1)
     assert np.abs(gc[fin] - xv[live][fin]).max() / np.abs(xv[live][fin]).max() < 1e-11
+
+
+def test_failed_create_can_be_retried():
+    """A rejected batch (checked before anything is allocated) or a device
+    allocation that fails (a 2 TB batch) leaves no batches behind: the same
+    handle then creates particles and steps them as the oracle does."""
+    print("case particles create after failures", flush=True)
+    o, s, u0, st, dt = setup(scheme="trilinear_rk3", nsub=1, n=200)
+    s.close()
+    nt, nr, nz = 16, 64, 96
+    s = vring_amd.Solver(nt, nr, nz, 4.0, 12.0, 1000.0, dt)
+    for name in ("q1", "q2", "q3"):
+        s.upload(name, o.field(name).copy())
+    with pytest.raises(RuntimeError, match="bad particle batch"):
+        s.particles_create([100, 0], [st, st], [1, 1], "trilinear_rk3")
+    with pytest.raises(RuntimeError, match="allocation failed"):
+        s.particles_create([100, 2**31 - 1], [st, st], [1, 1], "lsq_euler")
+    s.particles_create([200], [st], [1], "trilinear_rk3")
+    s.particles_upload(0, u0)
+    n = u0.shape[0]
+    xv, r2, alive = u0.copy(), np.zeros((n, 6)), np.ones(n, np.int32)
+    for irk in (1, 2, 3):
+        s.particles_step(irk)
+        o.L.orc_part_rk3_trilinear(o.h, n, st, irk, dt, xv.ravel(), r2.ravel(), alive)
+    g, gal = s.particles_download(0, n)
+    np.testing.assert_array_equal(gal, alive)
+    keep = alive == 1
+    assert np.abs(g[keep] - xv[keep]).max() / np.abs(xv[keep]).max() < 1e-11
+    s.close()

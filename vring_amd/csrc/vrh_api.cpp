@@ -429,15 +429,7 @@ int vrh_destroy(vrh_t *h) {
   if (h->h_snap) hipHostFree(h->h_snap);
   if (h->snap_ev) hipEventDestroy(h->snap_ev);
   if (h->copy_stream) hipStreamDestroy(h->copy_stream);
-  for (auto &b : h->pb) {
-    if (b.u) hipFree(b.u);
-    if (b.fit) hipFree(b.fit);
-    if (b.ddt2) hipFree(b.ddt2);
-    if (b.alive) hipFree(b.alive);
-    void *extra[] = {b.d_n, b.inflow, b.u_tmp, b.ddt2_tmp, b.alive_tmp, b.work, b.slot, b.slot_tmp,
-                     b.lflag, b.lrank, b.skeys, b.stmp};
-    for (void *q : extra) if (q) hipFree(q);
-  }
+  particles_release(h);
   for (int q = 0; q < 3; q++)
     if (h->side[q]) hipStreamDestroy(h->side[q]);
   for (int irk = 0; irk < 4; irk++) {

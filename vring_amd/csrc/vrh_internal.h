@@ -83,18 +83,18 @@ struct Fields {
 };
 
 struct PartBatch {
-  long cap, n;
-  double st;
-  int nsub;
-  double *u;     // nvar rows of cap (structure of arrays)
-  double *ddt2;  // nvar rows of cap, previous rate (RK3 schemes)
+  long cap = 0, n = 0;
+  double st = 0.0;
+  int nsub = 1;
+  double *u = nullptr;  // nvar rows of cap (structure of arrays)
+  double *ddt2 = nullptr;  // nvar rows of cap, previous rate (RK3 schemes)
   // bytes per element of the ddt2 rows: 8, or 4 for the trilinear scheme in
   // mixed precision (its rates are formed from the FP32 interpolation; the
   // state update stays FP64); ddt2 then holds float rows
   int d2w = 8;
   double *fit = nullptr;  // LSQ schemes: 39 rows of cap, per-component fits of a substep
-  int *alive;
-  long *d_n;     // used slots [0, n) on the device (kernels read it)
+  int *alive = nullptr;
+  long *d_n = nullptr;  // used slots [0, n) on the device (kernels read it)
   // inlet injection and compaction (allocated on first use)
   double *inflow = nullptr;  // accumulated inlet flux per r row (nr + 1)
   double *u_tmp = nullptr, *ddt2_tmp = nullptr;
@@ -293,3 +293,5 @@ template <typename TW> void launch_project(vrh_t *h, int irk);
 // the particle stage of substep irk on stream st (nothing launched where
 // the scheme has no stage at irk); returns whether anything was launched
 bool launch_particles(vrh_t *h, int irk, hipStream_t st);
+// frees every particle batch (after the handle's stream drains); no batches remain
+void particles_release(vrh_t *h);

@@ -723,6 +723,17 @@ bool launch_particles(vrh_t *h, int irk, hipStream_t st) {
   return true;
 }
 
+void particles_release(vrh_t *h) {
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto &b : h->pb) {
+    void *bufs[] = {b.u, b.fit, b.ddt2, b.alive, b.d_n, b.inflow, b.u_tmp, b.ddt2_tmp, b.alive_tmp, b.work,
+                    b.slot, b.slot_tmp, b.lflag, b.lrank, b.skeys, b.stmp};
+    for (void *q : bufs) if (q) (void)hipFree(q);
+  }
+  h->pb.clear();
+  h->nbatch = 0;
+}
+
 extern "C" {
 
 static int sort_batch(vrh_t *h, PartBatch &B, bool compact);
@@ -735,31 +746,41 @@ int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st
   if (!h->mesh[0].empty()) { h->err = "particles need the uniform radial mesh"; return -1; }
   if (!h->zmesh[0].empty()) { h->err = "particles need the uniform axial mesh"; return -1; }
   const int nvar = scheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
-  h->pscheme = scheme;
-  h->pb.resize(nbatch);
-  for (int b = 0; b < nbatch; b++) {
-    PartBatch &B = h->pb[b];
-    B.cap = cap[b]; B.n = 0; B.st = st[b]; B.nsub = nsub[b] < 1 ? 1 : nsub[b];
-    if (B.cap < 1 || B.cap > 0x7fffffffL || (scheme != VRH_PART_TRILINEAR_RK3 && !(B.st > 0.0))) {
+  for (int b = 0; b < nbatch; b++)  // every batch is checked before anything is allocated
+    if (cap[b] < 1 || cap[b] > 0x7fffffffL || (scheme != VRH_PART_TRILINEAR_RK3 && !(st[b] > 0.0))) {
       h->err = "bad particle batch";
       return -1;
     }
+  h->pscheme = scheme;
+  h->pb.resize(nbatch);
+  // an allocation failure releases what was allocated, so the call can be
+  // retried; the failed hipMalloc's error is cleared so that the next launch
+  // check does not report it
+  auto oom = [&]() {
+    (void)hipGetLastError();
+    particles_release(h);
+    h->err = "particle allocation failed";
+    return -3;
+  };
+  for (int b = 0; b < nbatch; b++) {
+    PartBatch &B = h->pb[b];
+    B.cap = cap[b]; B.n = 0; B.st = st[b]; B.nsub = nsub[b] < 1 ? 1 : nsub[b];
     if (hipMalloc(&B.u, sizeof(double) * nvar * B.cap) != hipSuccess ||
-        hipMalloc(&B.alive, sizeof(int) * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
+        hipMalloc(&B.alive, sizeof(int) * B.cap) != hipSuccess) return oom();
     hipMemsetAsync(B.u, 0, sizeof(double) * nvar * B.cap, h->stream);
     hipMemsetAsync(B.alive, 0, sizeof(int) * B.cap, h->stream);
     if (hipMalloc(&B.d_n, sizeof(long)) != hipSuccess ||
-        hipMalloc(&B.slot, sizeof(int) * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
+        hipMalloc(&B.slot, sizeof(int) * B.cap) != hipSuccess) return oom();
     hipMemsetAsync(B.d_n, 0, sizeof(long), h->stream);
     hipLaunchKernelGGL(k_iota, dim3((unsigned)((B.cap + 255) / 256)), dim3(256), 0, h->stream, B.slot, B.cap);
     B.ddt2 = nullptr;
     B.fit = nullptr;
     // global fit rows: the FP64 LSQ path only (the mixed path keeps them in LDS)
     if (scheme != VRH_PART_TRILINEAR_RK3 && h->prec != VRH_MIXED &&
-        hipMalloc(&B.fit, sizeof(double) * kFitRows * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
+        hipMalloc(&B.fit, sizeof(double) * kFitRows * B.cap) != hipSuccess) return oom();
     if (scheme != VRH_PART_LSQ_EULER) {
       B.d2w = (scheme == VRH_PART_TRILINEAR_RK3 && h->prec == VRH_MIXED) ? 4 : 8;
-      if (hipMalloc(&B.ddt2, (size_t)B.d2w * nvar * B.cap) != hipSuccess) { h->err = "particle allocation failed"; return -3; }
+      if (hipMalloc(&B.ddt2, (size_t)B.d2w * nvar * B.cap) != hipSuccess) return oom();
       hipMemsetAsync(B.ddt2, 0, (size_t)B.d2w * nvar * B.cap, h->stream);
     }
   }
