@@ -236,7 +236,10 @@ enum { VRH_PART_LSQ_EULER = 0, VRH_PART_LSQ_RK3 = 1, VRH_PART_TRILINEAR_RK3 = 2 
 /* Create nbatch particle batches (init_part, part_tools.F90:228-359): batch b
  * holds up to cap[b] particles with Stokes number st[b] and nsub[b] sub-steps
  * per call (part_tools.F90:493-502).  State rows hold 117 values
- * (part_tools.F90:241-252) for the LSQ schemes, 6 for TRILINEAR_RK3. */
+ * (part_tools.F90:241-252) for the LSQ schemes, 6 for TRILINEAR_RK3.
+ * Returns -1 for a bad argument or batch (nothing allocated) and -3 when a
+ * device allocation fails (what was allocated is freed); either way the
+ * handle keeps no batches and the call may be made again. */
 int vrh_particles_create(vrh_t *h, int nbatch, const long *cap, const double *st,
                          const int *nsub, int scheme);
 /* Host rows (n x nvar, row-major by particle) and alive flags. */
