@@ -1325,6 +1325,9 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, ZMet zm, doubl
   }
   __syncthreads();
   if (pow2) lds_fft(re, im, LB / 2, logn, N, twl, twl + N / 2, (TW)-1, tid, nth);
+  // a power-of-two N scales exactly, so x * (1/N) is x / N bit for bit
+  // without a double division per element
+  const double inv_n = 1.0 / N;
   for (int e = tid; e < LB * N; e += nth) {
     const int lp = pow2 ? e >> logn : e / N, pos = e - lp * N;
     const long l = l0 + lp;
@@ -1347,10 +1350,8 @@ __global__ __launch_bounds__(256) void k_divfft(VGrid g, VCoef c, ZMet zm, doubl
         if (ph >= N) ph -= N;
       }
     }
-    double outv;
-    if (pos == 0) outv = Fr / N;
-    else if (pos & 1) outv = 2.0 * Fr / N;
-    else outv = -2.0 * Fi / N;
+    const double num = (pos == 0) ? Fr : (pos & 1) ? 2.0 * Fr : -2.0 * Fi;
+    const double outv = pow2 ? num * inv_n : num / N;
     S[(size_t)pos + (size_t)N * l] = (TW)outv;
   }
   if (S64) {
