@@ -846,15 +846,24 @@ static int particle_buffers(vrh_t *h, PartBatch &B) {
   if (B.u_tmp) return 0;
   const int nvar = h->pscheme == VRH_PART_TRILINEAR_RK3 ? 6 : 117;
   const long nblk = (B.cap + 255) / 256;
-  if (hipMalloc(&B.u_tmp, sizeof(double) * nvar * B.cap) != hipSuccess ||
-      hipMalloc(&B.alive_tmp, sizeof(int) * B.cap) != hipSuccess ||
+  // u_tmp is allocated last: it marks the whole set as present.  A failure
+  // frees the part that was allocated (and clears the failed hipMalloc's
+  // error), so a later call starts again from nothing.
+  if (hipMalloc(&B.alive_tmp, sizeof(int) * B.cap) != hipSuccess ||
       hipMalloc(&B.work, sizeof(long) * (nblk + 8 + (long)(h->nr + 1) * h->nt * 2)) != hipSuccess ||
       hipMalloc(&B.inflow, sizeof(double) * (h->nr + 2)) != hipSuccess ||
       hipMalloc(&B.slot_tmp, sizeof(int) * B.cap) != hipSuccess ||
       hipMalloc(&B.lflag, sizeof(int) * B.cap) != hipSuccess ||
       hipMalloc(&B.lrank, sizeof(int) * B.cap) != hipSuccess ||
       hipMalloc(&B.skeys, sizeof(unsigned) * 4 * B.cap) != hipSuccess ||
-      (B.ddt2 && hipMalloc(&B.ddt2_tmp, (size_t)B.d2w * nvar * B.cap) != hipSuccess)) {
+      (B.ddt2 && hipMalloc(&B.ddt2_tmp, (size_t)B.d2w * nvar * B.cap) != hipSuccess) ||
+      hipMalloc(&B.u_tmp, sizeof(double) * nvar * B.cap) != hipSuccess) {
+    (void)hipGetLastError();
+    void **bufs[] = {(void **)&B.alive_tmp, (void **)&B.work, (void **)&B.inflow, (void **)&B.slot_tmp,
+                     (void **)&B.lflag, (void **)&B.lrank, (void **)&B.skeys, (void **)&B.ddt2_tmp,
+                     (void **)&B.u_tmp};
+    for (void **q : bufs)
+      if (*q) { (void)hipFree(*q); *q = nullptr; }
     h->err = "particle allocation failed";
     return -3;
   }
@@ -895,6 +904,8 @@ static int sort_batch(vrh_t *h, PartBatch &B, bool compact) {
     if (rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (unsigned)cap, 0u, (unsigned)bits, st) !=
             hipSuccess ||
         hipMalloc(&B.stmp, bytes ? bytes : 1) != hipSuccess) {
+      (void)hipGetLastError();
+      B.stmp = nullptr;
       h->err = "particle sort allocation failed";
       return -3;
     }
