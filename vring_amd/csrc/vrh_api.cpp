@@ -25,6 +25,7 @@ int fail(vrh_t *h, const char *msg, int code = -1) {
     if (e_ != hipSuccess) {                                                         \
       char b_[256];                                                                 \
       snprintf(b_, sizeof b_, "%s: %s", #call, hipGetErrorString(e_));              \
+      (void)hipGetLastError(); /* reported here, not by the next launch check */    \
       return fail(h, b_, -3);                                                       \
     }                                                                               \
   } while (0)
@@ -726,10 +727,12 @@ int vrh_snapshot_begin(vrh_t *h) {
   if (!h) return -1;
   if (h->snap_pending) return fail(h, "a snapshot is pending: vrh_snapshot_wait first");
   const size_t n = haloed(h);
-  if (!h->d_snap) {
-    HIPCHK(h, hipMalloc(&h->d_snap, 4 * n * sizeof(double)));
-    HIPCHK(h, hipHostMalloc(&h->h_snap, 4 * n * sizeof(double), hipHostMallocDefault));
-    HIPCHK(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+  if (!h->snap_ev) {
+    // created on first use, the event last: a failure part way leaves what
+    // was made for the next call to complete, never a set taken as whole
+    if (!h->d_snap) HIPCHK(h, hipMalloc(&h->d_snap, 4 * n * sizeof(double)));
+    if (!h->h_snap) HIPCHK(h, hipHostMalloc(&h->h_snap, 4 * n * sizeof(double), hipHostMallocDefault));
+    if (!h->copy_stream) HIPCHK(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
     HIPCHK(h, hipEventCreateWithFlags(&h->snap_ev, hipEventDisableTiming));
   }
   const double *src[4] = {h->f.q1, h->f.q2, h->f.q3, h->f.pres};
@@ -772,9 +775,14 @@ int vrh_synchronize(vrh_t *h) {
 int vrh_poisson_solve(vrh_t *h, const double *rhs, double *phi) {
   if (!h || !rhs || !phi) return -1;
   const size_t C = compact(h);
-  double *d_rhs = nullptr, *d_phi = nullptr;
-  HIPCHK(h, hipMalloc(&d_rhs, C * sizeof(double)));
-  HIPCHK(h, hipMalloc(&d_phi, C * sizeof(double)));
+  // freed on every return (hipFree waits for the device)
+  struct Buf {
+    double *p = nullptr;
+    ~Buf() { if (p) (void)hipFree(p); }
+  } rb, pb;
+  HIPCHK(h, hipMalloc(&rb.p, C * sizeof(double)));
+  HIPCHK(h, hipMalloc(&pb.p, C * sizeof(double)));
+  double *d_rhs = rb.p, *d_phi = pb.p;
   HIPCHK(h, hipMemcpyAsync(d_rhs, rhs, C * sizeof(double), hipMemcpyHostToDevice, h->stream));
   if (h->prec == VRH_MIXED) launch_poisson<float>(h, 1, d_rhs, d_phi);
   else launch_poisson<double>(h, 1, d_rhs, d_phi);
@@ -783,14 +791,10 @@ int vrh_poisson_solve(vrh_t *h, const double *rhs, double *phi) {
     const int rc = h->mode_rc;
     h->mode_rc = 0;
     (void)hipStreamSynchronize(h->stream);
-    hipFree(d_rhs);
-    hipFree(d_phi);
     return rc;
   }
   HIPCHK(h, hipMemcpyAsync(phi, d_phi, C * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  hipFree(d_rhs);
-  hipFree(d_phi);
   return 0;
 }
 

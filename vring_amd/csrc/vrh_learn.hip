@@ -938,7 +938,10 @@ int hfail(vrh_hz_t *h, const std::string &m, int code = -1) {
 #define HZCHK(h, call)                                                                     \
   do {                                                                                     \
     hipError_t e_ = (call);                                                                \
-    if (e_ != hipSuccess) return hfail(h, std::string(#call) + ": " + hipGetErrorString(e_), -3); \
+    if (e_ != hipSuccess) {                                                                \
+      (void)hipGetLastError(); /* reported here, not by the next launch check */           \
+      return hfail(h, std::string(#call) + ": " + hipGetErrorString(e_), -3);              \
+    }                                                                                      \
   } while (0)
 #define BLCHK(h, call)                                                                     \
   do {                                                                                     \
